@@ -1,0 +1,203 @@
+#!/usr/bin/env python3
+"""Benchmark of the MDACP LJ step on one MI355X.
+
+Workload (the configuration the metric is quoted on): 4,000,000 particles on
+an FCC lattice at density 0.5 in a periodic cube of side 200, smoothed LJ cut
+at rc = 2.5, Verlet margin 0.5, dt = 0.001, NVE, |v0| = sqrt(3) (T = 1),
+observables (T, P, E) every 100 steps as benchmark.cc:32-40 does.  A step is
+one MDManager::Calculate: expiry test, pair-list rebuild when expired, half
+drift, halo refresh, force kick, half drift.
+
+Reported: particle-updates/s of the FP64 path (`value`) and of the mixed FP32
+path, the energy drift of both over --drift-steps, the pair-force kernel's
+roofline position (HIP events on the domain's stream) and the oracle timed on
+the host as the CPU baseline.
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "particle-updates/s, 4M LJ ρ=0.5: FP64 vs mixed FP32 at equal energy drift"
+HBM_PEAK_GBS = 8000.0
+FP64_VECTOR_PEAK_TFLOPS = 78.6
+FP32_VECTOR_PEAK_TFLOPS = 157.3
+
+
+def algorithmic_per_update(rho, rc, margin):
+    """Half-list model: bytes and flops per particle-update."""
+    n_L = 2.0 * math.pi / 3.0 * rho * (rc + margin) ** 3
+    n_C = 2.0 * math.pi / 3.0 * rho * rc ** 3
+    return 96.0 + 4.0 + 4.0 * n_L, 26.0 * n_C + 8.0 * (n_L - n_C) + 15.0, n_L, n_C
+
+
+def run_gpu(precision, x, p, L, args, observe_every=100):
+    from mdacp_amd import mdgpu
+
+    n = len(x)
+    d = mdgpu.Domain(L, dt=args.dt, rc=args.rc, margin=args.margin, precision=precision, capacity=n)
+    d.add_particles(x, p)
+    d.make_pair_list()
+    d.calculate(args.warmup)
+    d.observables()
+    d.enable_timing(True)
+    force_ms = 0.0
+    force_launches = 0
+    rebuilds = 0
+    rebuild_ms = 0.0
+    trace = []
+    t0 = time.perf_counter()
+    done = 0
+    while done < args.steps:
+        k = min(observe_every, args.steps - done)
+        d.calculate(k)
+        tm = d.timing()
+        force_ms += tm["force_ms"]
+        force_launches += tm["force_launches"]
+        rebuilds += tm["rebuilds"]
+        rebuild_ms += tm["rebuild_ms"]
+        done += k
+        trace.append(d.observables())
+    t1 = time.perf_counter()
+    d.enable_timing(False)
+    elapsed = t1 - t0
+    res = dict(pu_s=args.steps * n / elapsed, ms_per_step=1e3 * elapsed / args.steps,
+               force_avg_ms=force_ms / max(1, force_launches), force_launches=force_launches,
+               rebuilds=rebuilds, rebuild_avg_ms=rebuild_ms / max(1, rebuilds), list=d.list_stats())
+    return d, res
+
+
+def energy_drift(d, steps, every=100):
+    """Slope of E/N (linear fit) scaled to 10^4 steps, and the RMS residual."""
+    e, t = [], []
+    for k in range(0, steps + 1, every):
+        if k:
+            d.calculate(every)
+        ob = d.observables()
+        e.append(ob["E"])
+        t.append(k)
+    e, t = np.array(e), np.array(t, dtype=float)
+    a, b = np.polyfit(t, e, 1)
+    resid = e - (a * t + b)
+    return dict(slope_per_1e4_steps=float(a * 1e4), rms_fluct=float(np.sqrt(np.mean(resid ** 2))),
+                e_start=float(e[0]), e_end=float(e[-1]), steps=steps)
+
+
+def cpu_baseline(x, p, L, args, gpu_rebuilds_per_step):
+    from oracle import oracle as O
+
+    s = O.OracleSystem(L, args.dt, args.rc, args.margin)
+    s.add(x, p)
+    t0 = time.perf_counter()
+    s.make_pair_list()
+    t_build = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    for _ in range(args.cpu_steps):
+        s.step()
+    t_steps = (time.perf_counter() - t0) / args.cpu_steps
+    per_step = t_steps + t_build * gpu_rebuilds_per_step
+    n = len(x)
+    return dict(value=n / per_step, unit="particle-updates/s", cores=1, kind="port",
+                sample=(f"oracle (C restatement of the reference step, 1 thread) on the same "
+                        f"{n}-particle workload: {args.cpu_steps} steps timed ({t_steps:.2f} s/step) "
+                        f"plus one pair-list build ({t_build:.2f} s) amortised at the GPU run's "
+                        f"rebuild rate ({gpu_rebuilds_per_step:.4f}/step)"))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2000)
+    ap.add_argument("--warmup", type=int, default=200)
+    ap.add_argument("--drift-steps", type=int, default=10000)
+    ap.add_argument("--cpu-steps", type=int, default=5)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--fcc-cells", type=int, default=100, help="N = 4 k^3")
+    ap.add_argument("--density", type=float, default=0.5)
+    ap.add_argument("--dt", type=float, default=0.001)
+    ap.add_argument("--rc", type=float, default=2.5)
+    ap.add_argument("--margin", type=float, default=0.5)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs one domain on one GPU; --gpus {args.gpus} is not supported "
+              f"(multi-GPU runs would shard by spatial domain decomposition, see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    from mdacp_amd import mdgpu
+    from mdacp_amd.lattice import fcc, initial_velocity
+
+    k = args.fcc_cells
+    L = k * (4.0 / args.density) ** (1.0 / 3.0)
+    x = fcc(L, args.density)
+    n = len(x)
+    p = initial_velocity(n, math.sqrt(3.0), 0)
+    print(f"# workload N={n} L={L} rho={args.density}", file=sys.stderr, flush=True)
+
+    d64, r64 = run_gpu(mdgpu.FP64, x, p, L, args)
+    print(f"# fp64 {r64}", file=sys.stderr, flush=True)
+    drift64 = energy_drift(d64, args.drift_steps) if args.drift_steps else None
+    print(f"# fp64 drift {drift64}", file=sys.stderr, flush=True)
+    # force error of the mixed kick on the FP64 trajectory's state
+    k64 = d64.kick()
+    d64.set_precision(mdgpu.MIXED)
+    k32 = d64.kick()
+    force_rms = float(math.sqrt(np.sum((k32 - k64) ** 2) / np.sum(k64 ** 2)))
+    d64.close()
+    del k64, k32
+
+    d32, r32 = run_gpu(mdgpu.MIXED, x, p, L, args)
+    print(f"# mixed {r32}", file=sys.stderr, flush=True)
+    drift32 = energy_drift(d32, args.drift_steps) if args.drift_steps else None
+    print(f"# mixed drift {drift32}", file=sys.stderr, flush=True)
+    d32.close()
+
+    bytes_pu, flops_pu, n_L, n_C = algorithmic_per_update(args.density, args.rc, args.margin)
+    avg_s = r64["force_avg_ms"] * 1e-3
+    achieved = bytes_pu * n / avg_s / 1e9
+    roofline = dict(bound="hbm", achieved=achieved, peak=HBM_PEAK_GBS, unit="GB/s", frac=achieved / HBM_PEAK_GBS,
+                    traffic=None, kernel="k_force_fp64",
+                    kernel_avg_ms=r64["force_avg_ms"], bytes_per_particle_update=bytes_pu,
+                    vector_flop_frac=flops_pu * n / avg_s / 1e12 / FP64_VECTOR_PEAK_TFLOPS)
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu = cpu_baseline(x, p, L, args, r64["rebuilds"] / args.steps)
+
+    avg32 = r32["force_avg_ms"] * 1e-3
+    line = {
+        "metric": METRIC,
+        "value": r64["pu_s"],
+        "unit": "particle-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": r64["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic: FCC lattice (confmaker.cc:8-44), |v0|=sqrt(3) random directions (variables.cc:195-222)",
+        "config": {"workload": (f"LJ NVE N={n} FCC rho={args.density} L={L:.6g} rc={args.rc} margin={args.margin} "
+                                f"dt={args.dt}, T/P/E observed every 100 steps, rebuilds by the Verlet budget"),
+                   "particles": n, "rebuilds_in_timed_steps": r64["rebuilds"]},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "mixed": {"value": r32["pu_s"], "ms_per_step": r32["ms_per_step"], "speedup_vs_fp64": r32["pu_s"] / r64["pu_s"],
+                  "force_kernel_avg_ms": r32["force_avg_ms"],
+                  "force_kernel_frac_hbm": bytes_pu * n / avg32 / 1e9 / HBM_PEAK_GBS,
+                  "force_rms_rel_err_vs_fp64": force_rms, "rebuilds_in_timed_steps": r32["rebuilds"]},
+        "energy_drift": {"fp64": drift64, "mixed": drift32},
+        "pair_list": r64["list"],
+        "rebuild_avg_ms": r64["rebuild_avg_ms"],
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,139 @@
+/*
+ * mdgpu.h -- C ABI of the MI355X (gfx950) implementation of MDACP's
+ * short-range Lennard-Jones step: cell binning, Verlet pair list with a search
+ * margin, smoothed-LJ force kick, position-Verlet / Langevin / Nose-Hoover
+ * updates with periodic images, and the observers.
+ *
+ * One `mdgpu_domain` is one MDACP domain (one MDUnit: its Variables, MeshList,
+ * PairList and the ghost images of its box) resident in HBM, with its own HIP
+ * stream.  Plain pointers and sizes only; host arrays are xyz triplets
+ * (n*3 doubles).  Every function returns MDGPU_OK or an error code; the text
+ * of the last error of a domain is available from mdgpu_last_error().
+ *
+ * The entry points replace these reference interfaces (kaityo256/mdacp):
+ *   mdgpu_create            Variables::Variables (variables.cc:130-142),
+ *                           SimulationInfo (simulationinfo.cc:5-51),
+ *                           MeshList::MeshList/ChangeScale (meshlist.cc:73-120)
+ *   mdgpu_add_particles     MDUnit::AddParticle (mdunit.cc:113-127)
+ *   mdgpu_set_momenta       writes to Variables::p (variables.h:18)
+ *   mdgpu_get_particles     reads Variables::q/p/type (variables.h:16-18)
+ *   mdgpu_make_pair_list    MDManager::MakePairList (mdmanager.cc:261-280) on one
+ *                           domain: SendParticles wrap, FindBorderParticles,
+ *                           MDUnit::MakePairList -> PairList::Init + MeshList::MakeList
+ *   mdgpu_is_pair_list_expired  PairList::IsPairListExpired (pairlist.cc:58-154)
+ *   mdgpu_update_position_half  ForceCalculator::UpdatePositionHalf (fcalculator.cc:21-32)
+ *   mdgpu_send_border_particles MDManager::SendBorderParticles (mdmanager.cc:282-348)
+ *   mdgpu_calculate_force   ForceCalculator::CalculateForce (fcalculator.cc:34-50,
+ *                           semantics of CalculateForceNext :228-290)
+ *   mdgpu_langevin          ForceCalculator::Langevin (fcalculator.cc:764-778)
+ *   mdgpu_heatbath_zeta     ForceCalculator::HeatbathZeta (fcalculator.cc:743-749)
+ *   mdgpu_heatbath_momenta  ForceCalculator::HeatbathMomenta (fcalculator.cc:751-762)
+ *   mdgpu_calculate         MDManager::Calculate (mdmanager.cc:115-192), n steps
+ *   mdgpu_observe           KineticEnergyObserver, PotentialEnergyObserver,
+ *                           VirialObserver (observer.cc:4-79)
+ *   mdgpu_change_scale      MDManager::ChangeScale (mdmanager.cc:431-443)
+ *   mdgpu_get_pairs         MeshList::GetKeyParticles/GetPartnerParticles
+ *                           (meshlist.h:46-48), as canonical pairs
+ */
+#ifndef MDGPU_H
+#define MDGPU_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct mdgpu_domain mdgpu_domain;
+
+/* status codes */
+#define MDGPU_OK 0
+#define MDGPU_ERR_TOO_FAST 1         /* "Too fast particles exists" (pairlist.cc:92-96) */
+#define MDGPU_ERR_INVALID_POSITION 2 /* "Invalid position" (variables.cc:184-192) */
+#define MDGPU_ERR_INVALID_INDEX 3    /* "Invalid index" (meshlist.cc:244-249) */
+#define MDGPU_ERR_CAPACITY 4         /* particle or pair capacity exceeded */
+#define MDGPU_ERR_HIP 5              /* a HIP runtime call failed */
+#define MDGPU_ERR_ARGUMENT 6         /* invalid argument */
+
+/* force precision */
+#define MDGPU_FP64 0   /* FP64 throughout */
+#define MDGPU_MIXED 1  /* FP32 cell-relative pair forces, compensated FP32 sums,
+                          FP64 positions, momenta, integration and reductions */
+
+/* thermostat (SimulationInfo::ControlTemperature / HeatbathType) */
+#define MDGPU_NVE 0
+#define MDGPU_NOSEHOOVER 1
+#define MDGPU_LANGEVIN 2
+
+typedef struct {
+  double L[3];            /* box edges (SimulationInfo::L) */
+  double dt;              /* TimeStep */
+  double rc;              /* cutoff (CUTOFF_LENGTH, mdconfig.h:15) */
+  double margin;          /* BufferLength (simulationinfo.cc:6) */
+  int check_list_length;  /* CheckListLength (simulationinfo.cc:343), 500 */
+  int precision;          /* MDGPU_FP64 or MDGPU_MIXED */
+  int capacity;           /* initial owned-particle capacity (grows on demand) */
+  int device;             /* HIP device ordinal */
+} mdgpu_params;
+
+/* lifetime */
+int mdgpu_create(const mdgpu_params *prm, mdgpu_domain **out);
+int mdgpu_destroy(mdgpu_domain *d);
+const char *mdgpu_last_error(const mdgpu_domain *d);
+int mdgpu_set_precision(mdgpu_domain *d, int precision);
+
+/* state (host arrays, n*3 doubles; type = canonical particle id) */
+int mdgpu_add_particles(mdgpu_domain *d, int n, const double *q, const double *p, const int *type);
+int mdgpu_particle_number(const mdgpu_domain *d, int *pn, int *tn);
+int mdgpu_get_particles(mdgpu_domain *d, double *q, double *p, int *type);
+int mdgpu_set_momenta(mdgpu_domain *d, const double *p); /* in mdgpu_get_particles order */
+int mdgpu_get_ghosts(mdgpu_domain *d, int *src_type, int *shift_code, double *q);
+
+/* thermostat and run parameters */
+int mdgpu_set_thermostat(mdgpu_domain *d, int kind, double T, double gamma, double tau);
+int mdgpu_set_seed(mdgpu_domain *d, uint64_t seed);
+int mdgpu_get_zeta(const mdgpu_domain *d, double *zeta);
+int mdgpu_get_time(const mdgpu_domain *d, double *sim_time, uint64_t *step);
+
+/* the step, piece by piece (same order as MDManager::Calculate) */
+int mdgpu_make_pair_list(mdgpu_domain *d);
+int mdgpu_is_pair_list_expired(mdgpu_domain *d, int *expired);
+int mdgpu_update_position_half(mdgpu_domain *d);
+int mdgpu_send_border_particles(mdgpu_domain *d);
+int mdgpu_calculate_force(mdgpu_domain *d);
+int mdgpu_langevin(mdgpu_domain *d);
+int mdgpu_heatbath_zeta(mdgpu_domain *d, double current_temperature);
+int mdgpu_heatbath_momenta(mdgpu_domain *d);
+
+/* n fused steps of MDManager::Calculate; *rebuilds (nullable) receives the
+   number of pair-list constructions done inside */
+int mdgpu_calculate(mdgpu_domain *d, int nsteps, int *rebuilds);
+
+/* observers: raw sums as returned by the reference observers */
+int mdgpu_observe(mdgpu_domain *d, double *ke, double *pe, double *virial);
+
+/* MDManager::ChangeScale: rebuild, scale box and owned positions by alpha, rebuild */
+int mdgpu_change_scale(mdgpu_domain *d, double alpha);
+
+/* parity export: canonical pairs (owned id, partner owner id, partner image
+   shift code (sx+1)+3(sy+1)+9(sz+1)); owned-owned pairs as (min id, max id, 13).
+   Call with out3 == NULL to get the count. */
+int mdgpu_get_pairs(mdgpu_domain *d, int64_t *npairs, int32_t *out3);
+/* kick of CalculateForce from zero momenta, per owned particle in
+   mdgpu_get_particles order (dp = F*dt) */
+int mdgpu_compute_kick(mdgpu_domain *d, double *dp);
+
+/* timing of the last mdgpu_calculate call, from HIP events recorded on the
+   domain's stream: count and total milliseconds of the pair-force launches and
+   of the whole step sequence.  Enabled by mdgpu_enable_timing(d, 1). */
+int mdgpu_enable_timing(mdgpu_domain *d, int on);
+int mdgpu_get_timing(const mdgpu_domain *d, int *force_launches, double *force_ms, double *total_ms,
+                     int *rebuilds, double *rebuild_ms);
+
+/* sizes of the device-resident structures (for roofline accounting) */
+int mdgpu_get_list_stats(const mdgpu_domain *d, int64_t *list_entries, int64_t *list_slots,
+                         int *ncells, int *ghosts);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,285 @@
+// build_kernels.hpp -- pair-list construction on gfx950: periodic wrap,
+// counting-sort cell binning (atomics + prefix scan, then a per-cell order by
+// canonical key so the result does not depend on atomic ordering), ghost
+// images of the single-domain halo, and the full neighbour list in sliced-ELL
+// layout (64 rows per slice, one wavefront).
+//
+// Reference semantics (kaityo256/mdacp): AdjustPeriodicBoundary
+// (variables.cc:172-193), MakeMesh (meshlist.cc:217-270), FindBorderParticles
+// and MakeBufferForBorderParticles (mdunit.cc:202-254, mdrect.cc:356-380),
+// SearchMesh (meshlist.cc:284-336).  The distance test is evaluated without
+// fused multiply-adds (the library is compiled with -ffp-contract=off), so the
+// pair set is bit-exact with the reference's default build.
+#pragma once
+#include "device_common.hpp"
+
+namespace mdgpu {
+
+struct Grid {
+  double s[3];      // domain start (MDRect::s)
+  double im[3];     // 1 / mesh size
+  int m[3];         // cells per axis incl. one ghost layer per side
+};
+
+__device__ __forceinline__ int cell_of(double x, double y, double z, const Grid &g) {
+  int ix = (int)((x - g.s[0]) * g.im[0]) + 1;
+  int iy = (int)((y - g.s[1]) * g.im[1]) + 1;
+  int iz = (int)((z - g.s[2]) * g.im[2]) + 1;
+  if (ix < 0) ix = g.m[0] - 1; else if (ix >= g.m[0]) ix = 0;
+  if (iy < 0) iy = g.m[1] - 1; else if (iy >= g.m[1]) iy = 0;
+  if (iz < 0) iz = g.m[2] - 1; else if (iz >= g.m[2]) iz = 0;
+  return g.m[0] * g.m[1] * iz + g.m[0] * iy + ix;
+}
+
+// Wrap owned particles into the box (one +-L per component), flag positions
+// still outside, bin, and take a slot in the cell by atomic increment.
+__global__ void k_wrap_bin(double4 *__restrict__ pos, int n, double Lx, double Ly, double Lz,
+                           Grid g, int *__restrict__ cell, int *__restrict__ slot,
+                           int *__restrict__ ccount, int *__restrict__ err, int wrap) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double4 q = pos[i];
+  if (wrap) {
+    if (q.x < 0.0) q.x += Lx; else if (q.x > Lx) q.x -= Lx;
+    if (q.y < 0.0) q.y += Ly; else if (q.y > Ly) q.y -= Ly;
+    if (q.z < 0.0) q.z += Lz; else if (q.z > Lz) q.z -= Lz;
+    pos[i] = q;
+    // the reference lets NaN through this test; a NaN is refused here
+    if (!(q.x >= 0.0 && q.x <= Lx && q.y >= 0.0 && q.y <= Ly && q.z >= 0.0 && q.z <= Lz))
+      atomicOr(err, kErrInvalidPosition);
+  }
+  const int c = cell_of(q.x, q.y, q.z, g);
+  cell[i] = c;
+  slot[i] = atomicAdd(&ccount[c], 1);
+}
+
+__global__ void k_scatter_perm(const int *__restrict__ cell, const int *__restrict__ slot,
+                               const int *__restrict__ cstart, int n, int *__restrict__ perm) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  perm[cstart[cell[i]] + slot[i]] = i;
+}
+
+// Order each cell's members by a canonical 64-bit key (insertion sort; cells
+// hold tens of particles).
+__global__ void k_sort_cells(int *__restrict__ perm, const int *__restrict__ cstart,
+                             const int *__restrict__ ccount, const long long *__restrict__ key,
+                             int ncell) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= ncell) return;
+  int *a = perm + cstart[c];
+  const int n = ccount[c];
+  for (int t = 1; t < n; t++) {
+    const int v = a[t];
+    const long long kv = key[v];
+    int u = t - 1;
+    while (u >= 0 && key[a[u]] > kv) {
+      a[u + 1] = a[u];
+      u--;
+    }
+    a[u + 1] = v;
+  }
+}
+
+__global__ void k_owned_key(const int *__restrict__ id, int n, long long *__restrict__ key) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) key[i] = (long long)id[i];
+}
+
+__global__ void k_gather_owned(const int *__restrict__ perm, int n, const double4 *__restrict__ pos_in,
+                               const double4 *__restrict__ mom_in, const int *__restrict__ id_in,
+                               const int *__restrict__ cell_in, double4 *__restrict__ pos_out,
+                               double4 *__restrict__ mom_out, int *__restrict__ id_out,
+                               int *__restrict__ cell_out) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  const int i = perm[k];
+  pos_out[k] = pos_in[i];
+  mom_out[k] = mom_in[i];
+  id_out[k] = id_in[i];
+  cell_out[k] = cell_in[i];
+}
+
+// Halo of one domain (MDManager::MakePairList's six sequential
+// FindBorderParticles/SendBorderParticlesSub passes, mdmanager.cc:269-275):
+// along each axis a particle strictly within SL of the low face gets an image
+// at +L, strictly within SL of the high face an image at -L; images of images
+// make the edges and corners.  Equivalently, one image per non-zero shift
+// vector whose every non-zero component passes its face test.
+struct Edges {
+  double lo_hi[3];   // s + SL
+  double hi_lo[3];   // e - SL
+  double s[3], e[3];
+};
+
+__device__ __forceinline__ void face_options(const double4 q, const Edges &E, int lo[3], int hi[3]) {
+  const double v[3] = {q.x, q.y, q.z};
+  for (int d = 0; d < 3; d++) {
+    lo[d] = (v[d] < E.lo_hi[d] && v[d] > E.s[d]);
+    hi[d] = (v[d] > E.hi_lo[d] && v[d] < E.e[d]);
+  }
+}
+
+__global__ void k_count_images(const double4 *__restrict__ pos, int n, Edges E, int *__restrict__ nimg) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  int lo[3], hi[3];
+  face_options(pos[i], E, lo, hi);
+  nimg[i] = (1 + lo[0] + hi[0]) * (1 + lo[1] + hi[1]) * (1 + lo[2] + hi[2]) - 1;
+}
+
+__global__ void k_emit_images(const double4 *__restrict__ pos, int n, Edges E, double Lx, double Ly,
+                              double Lz, Grid g, const int *__restrict__ goff, int ng,
+                              int *__restrict__ gsrc, int *__restrict__ gcode,
+                              double4 *__restrict__ gpos, int *__restrict__ gcell,
+                              int *__restrict__ gslot, int *__restrict__ gcount,
+                              long long *__restrict__ gkey, const int *__restrict__ id) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double4 q = pos[i];
+  int lo[3], hi[3];
+  face_options(q, E, lo, hi);
+  int k = goff[i];
+  const int sxs[3] = {0, 1, -1};
+  for (int a = 0; a < 3; a++) {
+    const int sx = sxs[a];
+    if ((sx == 1 && !lo[0]) || (sx == -1 && !hi[0])) continue;
+    for (int b = 0; b < 3; b++) {
+      const int sy = sxs[b];
+      if ((sy == 1 && !lo[1]) || (sy == -1 && !hi[1])) continue;
+      for (int c = 0; c < 3; c++) {
+        const int sz = sxs[c];
+        if ((sz == 1 && !lo[2]) || (sz == -1 && !hi[2])) continue;
+        if (!sx && !sy && !sz) continue;
+        if (k >= ng) return;  // cannot happen: goff is the scan of k_count_images
+        const int code = (sx + 1) + 3 * (sy + 1) + 9 * (sz + 1);
+        const double3 y = image_of(q, code, Lx, Ly, Lz);
+        gsrc[k] = i;
+        gcode[k] = code;
+        gpos[k] = make_double4(y.x, y.y, y.z, 0.0);
+        const int cl = cell_of(y.x, y.y, y.z, g);
+        gcell[k] = cl;
+        gslot[k] = atomicAdd(&gcount[cl], 1);
+        gkey[k] = (long long)id[i] * 32 + code;
+        k++;
+      }
+    }
+  }
+}
+
+__global__ void k_gather_ghosts(const int *__restrict__ perm, int ng, int pn,
+                                const int *__restrict__ src_in, const int *__restrict__ code_in,
+                                const double4 *__restrict__ gpos_in, const int *__restrict__ gcell_in,
+                                int *__restrict__ gsrc, int *__restrict__ gcode,
+                                double4 *__restrict__ pos, int *__restrict__ cell) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= ng) return;
+  const int g = perm[k];
+  gsrc[k] = src_in[g];
+  gcode[k] = code_in[g];
+  pos[pn + k] = gpos_in[g];
+  cell[pn + k] = gcell_in[g];
+}
+
+__global__ void k_set_sentinel(double4 *a, double4 *b, float4 *ra, float4 *rb, int idx) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    const double far = 1.0e200;
+    a[idx] = make_double4(far, far, far, 0.0);
+    b[idx] = make_double4(far, far, far, 0.0);
+    if (ra) ra[idx] = make_float4(1.0e30f, 1.0e30f, 1.0e30f, 0.0f);
+    if (rb) rb[idx] = make_float4(1.0e30f, 1.0e30f, 1.0e30f, 0.0f);
+  }
+}
+
+// ---- neighbour search: one lane per owned row, 27 cells of the extended
+// grid (no wrap), owned members then ghost members of each cell in order.
+struct CellTables {
+  const int *ostart, *ocount, *gstart, *gcount;
+};
+
+template <bool kFill>
+__device__ __forceinline__ int search_row(int i, const double4 *__restrict__ pos, const int *__restrict__ cell,
+                                          int pn, Grid g, CellTables T, double S2,
+                                          uint32_t *__restrict__ out, long long stride, int cap) {
+  const double4 qi = pos[i];
+  const int c = cell[i];
+  const int mx = g.m[0], my = g.m[1], mz = g.m[2];
+  const int ix = c % mx, iy = (c / mx) % my, iz = c / (mx * my);
+  int cnt = 0;
+  for (int dz = -1; dz <= 1; dz++) {
+    const int jz = iz + dz;
+    if (jz < 0 || jz >= mz) continue;
+    for (int dy = -1; dy <= 1; dy++) {
+      const int jy = iy + dy;
+      if (jy < 0 || jy >= my) continue;
+      for (int dx = -1; dx <= 1; dx++) {
+        const int jx = ix + dx;
+        if (jx < 0 || jx >= mx) continue;
+        const int c2 = mx * my * jz + mx * jy + jx;
+        const uint32_t off = (uint32_t)((dx + 1) | ((dy + 1) << 2) | ((dz + 1) << 4)) << kIndexBits;
+        for (int pass = 0; pass < 2; pass++) {
+          const int b = pass ? pn + T.gstart[c2] : T.ostart[c2];
+          const int e = b + (pass ? T.gcount[c2] : T.ocount[c2]);
+          for (int j = b; j < e; j++) {
+            if (j == i) continue;
+            const double4 qj = pos[j];
+            const double ex = qi.x - qj.x, ey = qi.y - qj.y, ez = qi.z - qj.z;
+            const double r2 = (ex * ex + ey * ey + ez * ez);
+            if (r2 > S2) continue;
+            if (kFill) {
+              if (cnt < cap) out[(long long)cnt * stride] = (uint32_t)j | off;
+            }
+            cnt++;
+          }
+        }
+      }
+    }
+  }
+  return cnt;
+}
+
+__global__ void k_count_rows(const double4 *__restrict__ pos, const int *__restrict__ cell, int pn, Grid g,
+                             CellTables T, double S2, int *__restrict__ row_len) {
+  const int i = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+  if (i >= pn) return;
+  row_len[i] = search_row<false>(i, pos, cell, pn, g, T, S2, nullptr, 0, 0);
+}
+
+// slice length = longest row of its 64 rows; slots = 64 * length
+__global__ void k_slice_len(const int *__restrict__ row_len, int pn, int nslice, int *__restrict__ slen,
+                            long long *__restrict__ slots) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= nslice) return;
+  int m = 0;
+  const int b = s * kWave, e = min(pn, b + kWave);
+  for (int i = b; i < e; i++) m = max(m, row_len[i]);
+  slen[s] = m;
+  slots[s] = (long long)m * kWave;
+}
+
+__global__ void k_fill_rows(const double4 *__restrict__ pos, const int *__restrict__ cell, int pn, Grid g,
+                            CellTables T, double S2, const long long *__restrict__ sbase,
+                            const int *__restrict__ slen, uint32_t *__restrict__ nbr, uint32_t pad,
+                            int *__restrict__ err) {
+  const int i = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+  if (i >= pn) return;
+  const int s = i / kWave, lane = i % kWave;
+  const int len = slen[s];
+  uint32_t *row = nbr + sbase[s] + lane;
+  const int cnt = search_row<true>(i, pos, cell, pn, g, T, S2, row, kWave, len);
+  if (cnt > len) atomicOr(err, kErrCapacity);
+  for (int k = cnt; k < len; k++) row[(long long)k * kWave] = pad;
+}
+
+// rows past pn inside the last slice are all padding
+__global__ void k_fill_tail(int pn, int nslice, const long long *__restrict__ sbase,
+                            const int *__restrict__ slen, uint32_t *__restrict__ nbr, uint32_t pad) {
+  const int t = threadIdx.x;
+  const int s = nslice - 1;
+  const int i = s * kWave + t;
+  if (t >= kWave || i < pn) return;
+  const int len = slen[s];
+  for (int k = 0; k < len; k++) nbr[sbase[s] + (long long)k * kWave + t] = pad;
+}
+
+}  // namespace mdgpu

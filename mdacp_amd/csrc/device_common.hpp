@@ -1,0 +1,55 @@
+// device_common.hpp -- shared device helpers for the gfx950 LJ kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace mdgpu {
+
+constexpr int kWave = 64;      // CDNA wavefront width
+constexpr int kBlock = 256;    // 4 waves per workgroup
+constexpr int kXcds = 8;       // MI355X: 8 XCDs, workgroups dispatched round-robin
+constexpr uint32_t kIndexBits = 26;                 // neighbour index field
+constexpr uint32_t kIndexMask = (1u << kIndexBits) - 1u;
+constexpr int kMaxParticles = (1 << kIndexBits) - 2;  // owned + ghosts + sentinel
+constexpr uint32_t kZeroOffset = 0x15u;             // cell offset (0,0,0): 01 01 01
+
+// Error bits raised by kernels; read back at the existing sync points.
+constexpr int kErrInvalidPosition = 1;
+constexpr int kErrInvalidIndex = 2;
+constexpr int kErrCapacity = 4;
+
+// Workgroups are dispatched round-robin over the XCDs, each with its own L2.
+// Remap so that every XCD walks one contiguous run of logical blocks: rows in
+// cell order then share neighbour positions inside one L2.
+__device__ __forceinline__ int xcd_block(int b, int nb) {
+  const int q = nb / kXcds, r = nb % kXcds;
+  const int x = b % kXcds, k = b / kXcds;
+  return x * q + (x < r ? x : r) + k;
+}
+
+__device__ __forceinline__ double wave_max(double v) {
+  for (int o = kWave / 2; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, kWave));
+  return v;
+}
+
+__device__ __forceinline__ double wave_sum(double v) {
+  for (int o = kWave / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
+  return v;
+}
+
+// Bits of a non-negative double order like the value.
+__device__ __forceinline__ void atomic_max_nonneg(unsigned long long *addr, double v) {
+  atomicMax(addr, (unsigned long long)__double_as_longlong(v));
+}
+
+// Image displacement of a shift code (sx+1)+3(sy+1)+9(sz+1): one +-L or 0.0
+// per component, added as the reference's halo adds it (mdunit.cc:215-245).
+__device__ __forceinline__ double3 image_of(double4 q, int code, double Lx, double Ly, double Lz) {
+  const int sx = code % 3 - 1, sy = (code / 3) % 3 - 1, sz = code / 9 - 1;
+  const double dx = sx > 0 ? Lx : (sx < 0 ? -Lx : 0.0);
+  const double dy = sy > 0 ? Ly : (sy < 0 ? -Ly : 0.0);
+  const double dz = sz > 0 ? Lz : (sz < 0 ? -Lz : 0.0);
+  return make_double3(q.x + dx, q.y + dy, q.z + dz);
+}
+
+}  // namespace mdgpu

@@ -1,0 +1,1130 @@
+// mdgpu.hip -- host side of the gfx950 LJ domain: device state, the pair-list
+// rebuild pipeline, the step sequence of MDManager::Calculate and the C ABI
+// declared in include/mdgpu.h.
+//
+// Build (see mdacp_amd/build.py):
+//   hipcc --offload-arch=gfx950 -O3 -ffp-contract=off -fPIC -shared
+//         -Xclang -target-feature -Xclang -packed-fp32-ops
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "mdgpu.h"
+#include "build_kernels.hpp"
+#include "scan.hpp"
+#include "step_kernels.hpp"
+
+using namespace mdgpu;
+
+namespace {
+
+struct Buf {
+  void *p = nullptr;
+  size_t bytes = 0;
+};
+
+}  // namespace
+
+struct mdgpu_domain {
+  mdgpu_params prm{};
+  hipStream_t stream = nullptr;
+  std::string err;
+  int precision = MDGPU_FP64;
+  int kahan = 1;
+
+  // SimulationInfo / MDRect / MeshList geometry
+  double L[3], rs[3], re[3];
+  double dt, rc, margin, SL, S2, CL2, C0, C2;
+  double msize[3];
+  int m[3], ncell = 0;
+  int check_list_length = 500;
+
+  // thermostat
+  int thermostat = MDGPU_NVE;
+  double T_aim = 1.0, gamma = 0.1, tau = 0.1, zeta = 0.0;
+  uint64_t seed = 0, step = 0;
+  double sim_time = 0.0;
+
+  // PairList state (pairlist.h:10-31)
+  bool initialized = false;
+  double buffer_length = 0.0;
+  int lifetime = 0;
+  bool vmax_valid = false;
+
+  // counts and capacities
+  int pn = 0, ng = 0, tn = 0;
+  int cap = 0, gcap = 0;
+  int nslice = 0;
+  long long nbr_slots = 0, nbr_entries = 0, nbr_cap = 0;
+
+  // device state
+  double4 *pos = nullptr, *pos_tmp = nullptr, *pos_old = nullptr, *mom = nullptr, *mom_alt = nullptr;
+  float4 *rel = nullptr, *rel_tmp = nullptr;
+  int *id = nullptr, *id_alt = nullptr, *cell = nullptr, *cell_alt = nullptr;
+  int *gsrc = nullptr, *gcode = nullptr;
+  int *c_ocount = nullptr, *c_ostart = nullptr, *c_gcount = nullptr, *c_gstart = nullptr;
+  double *org = nullptr;  // origins of the cell columns, x then y then z
+  uint32_t *nbr = nullptr;
+  long long *sbase = nullptr;
+  int *slen = nullptr, *row_len = nullptr;
+  // build scratch
+  int *slot = nullptr, *perm = nullptr, *nimg = nullptr, *goff = nullptr;
+  int *t_gsrc = nullptr, *t_gcode = nullptr, *t_gcell = nullptr;
+  long long *key = nullptr, *scan_scratch = nullptr, *slots64 = nullptr;
+  long long scan_scratch_n = 0;
+  double *disp = nullptr, *partial = nullptr;
+  int partial_n = 0;
+  int *cand = nullptr;
+  // small device scalars: [0] vmax2 bits, [1] dmax2 bits (u64); ints: err, count, flag
+  unsigned long long *u64 = nullptr;
+  long long *i64 = nullptr;
+  int *i32 = nullptr;
+  double *d3 = nullptr;
+  // pinned host mirror of the small scalars
+  unsigned long long *h_u64 = nullptr;
+  long long *h_i64 = nullptr;
+  int *h_i32 = nullptr;
+  double *h_d3 = nullptr;
+
+  // timing
+  bool timing = false;
+  std::vector<hipEvent_t> ev;
+  int force_launches = 0, rebuild_count = 0;
+  double force_ms = 0.0, total_ms = 0.0, rebuild_ms = 0.0;
+};
+
+namespace {
+
+#define HIPCHK(d, call)                                                                   \
+  do {                                                                                    \
+    hipError_t e_ = (call);                                                               \
+    if (e_ != hipSuccess) {                                                               \
+      (d)->err = std::string(#call) + ": " + hipGetErrorString(e_);                        \
+      return MDGPU_ERR_HIP;                                                               \
+    }                                                                                     \
+  } while (0)
+
+#define CHK(call)               \
+  do {                          \
+    int r_ = (call);            \
+    if (r_ != MDGPU_OK) return r_; \
+  } while (0)
+
+inline unsigned nblk(long long n, int b = kBlock) { return (unsigned)std::max<long long>(1, (n + b - 1) / b); }
+
+template <class T>
+int dalloc(mdgpu_domain *d, T **p, size_t n) {
+  if (*p) {
+    hipFree(*p);
+    *p = nullptr;
+  }
+  if (n == 0) n = 1;
+  HIPCHK(d, hipMalloc((void **)p, n * sizeof(T)));
+  return MDGPU_OK;
+}
+
+template <class T>
+int dgrow_copy(mdgpu_domain *d, T **p, size_t old_n, size_t keep, size_t new_n) {
+  T *np = nullptr;
+  HIPCHK(d, hipMalloc((void **)&np, std::max<size_t>(1, new_n) * sizeof(T)));
+  if (*p && keep) HIPCHK(d, hipMemcpyAsync(np, *p, keep * sizeof(T), hipMemcpyDeviceToDevice, d->stream));
+  if (*p) {
+    HIPCHK(d, hipStreamSynchronize(d->stream));
+    hipFree(*p);
+  }
+  (void)old_n;
+  *p = np;
+  return MDGPU_OK;
+}
+
+int sync(mdgpu_domain *d) {
+  HIPCHK(d, hipStreamSynchronize(d->stream));
+  return MDGPU_OK;
+}
+
+int check_launch(mdgpu_domain *d, const char *what) {
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) {
+    d->err = std::string(what) + ": " + hipGetErrorString(e);
+    return MDGPU_ERR_HIP;
+  }
+  return MDGPU_OK;
+}
+
+// MeshList::ChangeScale (meshlist.cc:95-120) and the column origins used by
+// the cell-relative FP32 coordinates.
+int set_geometry(mdgpu_domain *d) {
+  for (int k = 0; k < 3; k++) {
+    const double w = d->re[k] - d->rs[k];
+    const int ms = (int)(w / d->SL);
+    if (ms < 1) {
+      d->err = "box edge shorter than the search length";
+      return MDGPU_ERR_ARGUMENT;
+    }
+    d->msize[k] = w / (double)ms;
+    d->m[k] = (int)(w / d->msize[k]) + 2;
+  }
+  const long long nc = (long long)d->m[0] * d->m[1] * d->m[2];
+  if (nc > (1ll << 30)) {
+    d->err = "too many cells";
+    return MDGPU_ERR_CAPACITY;
+  }
+  d->ncell = (int)nc;
+  CHK(dalloc(d, &d->c_ocount, d->ncell));
+  CHK(dalloc(d, &d->c_ostart, d->ncell));
+  CHK(dalloc(d, &d->c_gcount, d->ncell));
+  CHK(dalloc(d, &d->c_gstart, d->ncell));
+  std::vector<double> org(d->m[0] + d->m[1] + d->m[2]);
+  int o = 0;
+  for (int k = 0; k < 3; k++)
+    for (int c = 0; c < d->m[k]; c++) org[o++] = d->rs[k] + (double)(c - 1) * d->msize[k];
+  CHK(dalloc(d, &d->org, org.size()));
+  HIPCHK(d, hipMemcpy(d->org, org.data(), org.size() * sizeof(double), hipMemcpyHostToDevice));
+  return MDGPU_OK;
+}
+
+Grid grid_of(const mdgpu_domain *d) {
+  Grid g;
+  for (int k = 0; k < 3; k++) {
+    g.s[k] = d->rs[k];
+    g.im[k] = 1.0 / d->msize[k];
+    g.m[k] = d->m[k];
+  }
+  return g;
+}
+
+Origins origins_of(const mdgpu_domain *d) {
+  Origins O;
+  O.x = d->org;
+  O.y = d->org + d->m[0];
+  O.z = d->org + d->m[0] + d->m[1];
+  O.mx = d->m[0];
+  O.my = d->m[1];
+  return O;
+}
+
+// capacity for owned particles (arrays indexed by owned index)
+int ensure_owned(mdgpu_domain *d, int need) {
+  if (need <= d->cap) return MDGPU_OK;
+  if (need > kMaxParticles) {
+    d->err = "particle capacity exceeded";
+    return MDGPU_ERR_CAPACITY;
+  }
+  int nc = std::max(need, std::max(1024, d->cap + d->cap / 4));
+  nc = std::min(nc, kMaxParticles);
+  const size_t tc_old = (size_t)d->cap + d->gcap + 1, tc_new = (size_t)nc + d->gcap + 1;
+  CHK(dgrow_copy(d, &d->pos, tc_old, d->pn, tc_new));
+  CHK(dgrow_copy(d, &d->mom, d->cap, d->pn, nc));
+  CHK(dgrow_copy(d, &d->id, d->cap, d->pn, nc));
+  CHK(dalloc(d, &d->pos_tmp, tc_new));
+  CHK(dalloc(d, &d->pos_old, tc_new));
+  CHK(dalloc(d, &d->rel, tc_new));
+  CHK(dalloc(d, &d->rel_tmp, tc_new));
+  CHK(dalloc(d, &d->cell, tc_new));
+  CHK(dalloc(d, &d->cell_alt, tc_new));
+  CHK(dalloc(d, &d->disp, tc_new));
+  CHK(dalloc(d, &d->mom_alt, nc));
+  CHK(dalloc(d, &d->id_alt, nc));
+  CHK(dalloc(d, &d->slot, tc_new));
+  CHK(dalloc(d, &d->perm, tc_new));
+  CHK(dalloc(d, &d->key, tc_new));
+  CHK(dalloc(d, &d->nimg, nc));
+  CHK(dalloc(d, &d->goff, nc));
+  CHK(dalloc(d, &d->row_len, nc));
+  const int ns = (nc + kWave - 1) / kWave;
+  CHK(dalloc(d, &d->slen, ns));
+  CHK(dalloc(d, &d->sbase, ns));
+  CHK(dalloc(d, &d->slots64, ns));
+  d->cap = nc;
+  return MDGPU_OK;
+}
+
+// capacity for ghost images (arrays indexed by owned+ghost index)
+int ensure_ghosts(mdgpu_domain *d, int need) {
+  if (need <= d->gcap) return MDGPU_OK;
+  if ((long long)d->cap + need + 1 > kMaxParticles) {
+    d->err = "particle capacity exceeded (owned + ghosts)";
+    return MDGPU_ERR_CAPACITY;
+  }
+  const int ngc = std::max(need, d->gcap + d->gcap / 4);
+  const size_t tc_old = (size_t)d->cap + d->gcap + 1, tc_new = (size_t)d->cap + ngc + 1;
+  CHK(dgrow_copy(d, &d->pos, tc_old, d->pn, tc_new));
+  CHK(dalloc(d, &d->pos_tmp, tc_new));
+  CHK(dalloc(d, &d->pos_old, tc_new));
+  CHK(dalloc(d, &d->rel, tc_new));
+  CHK(dalloc(d, &d->rel_tmp, tc_new));
+  CHK(dgrow_copy(d, &d->cell, tc_old, d->pn, tc_new));
+  CHK(dalloc(d, &d->cell_alt, tc_new));
+  CHK(dalloc(d, &d->disp, tc_new));
+  CHK(dalloc(d, &d->slot, tc_new));
+  CHK(dalloc(d, &d->perm, tc_new));
+  CHK(dalloc(d, &d->key, tc_new));
+  CHK(dalloc(d, &d->gsrc, ngc));
+  CHK(dalloc(d, &d->gcode, ngc));
+  CHK(dalloc(d, &d->t_gsrc, ngc));
+  CHK(dalloc(d, &d->t_gcode, ngc));
+  CHK(dalloc(d, &d->t_gcell, ngc));
+  d->gcap = ngc;
+  return MDGPU_OK;
+}
+
+int ensure_scan(mdgpu_domain *d, long long n) {
+  const long long need = scan_scratch_size(n);
+  if (need <= d->scan_scratch_n) return MDGPU_OK;
+  CHK(dalloc(d, &d->scan_scratch, (size_t)need));
+  d->scan_scratch_n = need;
+  return MDGPU_OK;
+}
+
+int ensure_partial(mdgpu_domain *d, int nb) {
+  if (nb * 3 <= d->partial_n) return MDGPU_OK;
+  CHK(dalloc(d, &d->partial, (size_t)nb * 3));
+  d->partial_n = nb * 3;
+  return MDGPU_OK;
+}
+
+int read_err_flags(mdgpu_domain *d) {
+  HIPCHK(d, hipMemcpyAsync(d->h_i32, d->i32, sizeof(int) * 4, hipMemcpyDeviceToHost, d->stream));
+  CHK(sync(d));
+  const int e = d->h_i32[0];
+  if (e & kErrInvalidPosition) {
+    d->err = "Invalid position";
+    return MDGPU_ERR_INVALID_POSITION;
+  }
+  if (e & kErrInvalidIndex) {
+    d->err = "Invalid index";
+    return MDGPU_ERR_INVALID_INDEX;
+  }
+  if (e & kErrCapacity) {
+    d->err = "pair list capacity";
+    return MDGPU_ERR_CAPACITY;
+  }
+  return MDGPU_OK;
+}
+
+int read_total(mdgpu_domain *d, long long *v) {
+  HIPCHK(d, hipMemcpyAsync(d->h_i64, d->i64, sizeof(long long), hipMemcpyDeviceToHost, d->stream));
+  CHK(sync(d));
+  *v = d->h_i64[0];
+  return MDGPU_OK;
+}
+
+// Stable-by-key counting sort of n items already binned (cell[], slot[] from
+// atomics, counts[] per cell): starts[] = scan(counts), perm[] = new order.
+int cell_sort(mdgpu_domain *d, const int *cellv, int n, int *counts, int *starts, const long long *keys) {
+  hipStream_t st = d->stream;
+  CHK(ensure_scan(d, d->ncell));
+  exclusive_scan<int, int>(counts, starts, d->ncell, d->scan_scratch, nullptr, st);
+  hipLaunchKernelGGL(k_scatter_perm, dim3(nblk(n)), dim3(kBlock), 0, st, cellv, d->slot, starts, n, d->perm);
+  hipLaunchKernelGGL(k_sort_cells, dim3(nblk(d->ncell)), dim3(kBlock), 0, st, d->perm, starts, counts, keys,
+                     d->ncell);
+  return check_launch(d, "cell sort");
+}
+
+// MDManager::MakePairList for the single domain (mdmanager.cc:261-280).
+// The reference also permutes particles that cross a face
+// (MakeBufferForSendingParticle, mdunit.cc:155-179); that only renames
+// indices, and particles are put in cell order here anyway.
+int make_pair_list(mdgpu_domain *d) {
+  hipStream_t st = d->stream;
+  const Grid g = grid_of(d);
+  const int pn = d->pn;
+  d->ng = 0;
+  d->tn = pn;
+  HIPCHK(d, hipMemsetAsync(d->i32, 0, sizeof(int) * 4, st));
+  // 1. wrap (AdjustPeriodicBoundary) and bin the owned particles
+  HIPCHK(d, hipMemsetAsync(d->c_ocount, 0, sizeof(int) * d->ncell, st));
+  if (pn > 0) {
+    hipLaunchKernelGGL(k_wrap_bin, dim3(nblk(pn)), dim3(kBlock), 0, st, d->pos, pn, d->L[0], d->L[1], d->L[2],
+                       g, d->cell, d->slot, d->c_ocount, d->i32, 1);
+    hipLaunchKernelGGL(k_owned_key, dim3(nblk(pn)), dim3(kBlock), 0, st, d->id, pn, d->key);
+  }
+  CHK(read_err_flags(d));
+  CHK(cell_sort(d, d->cell, pn, d->c_ocount, d->c_ostart, d->key));
+  if (pn > 0) {
+    hipLaunchKernelGGL(k_gather_owned, dim3(nblk(pn)), dim3(kBlock), 0, st, d->perm, pn, d->pos, d->mom, d->id,
+                       d->cell, d->pos_tmp, d->mom_alt, d->id_alt, d->cell_alt);
+    std::swap(d->pos, d->pos_tmp);
+    std::swap(d->mom, d->mom_alt);
+    std::swap(d->id, d->id_alt);
+    std::swap(d->cell, d->cell_alt);
+  }
+  // 2. ghost images (FindBorderParticles / SendBorderParticlesSub x 6)
+  Edges E;
+  for (int k = 0; k < 3; k++) {
+    E.s[k] = d->rs[k];
+    E.e[k] = d->re[k];
+    E.lo_hi[k] = d->rs[k] + d->SL;
+    E.hi_lo[k] = d->re[k] - d->SL;
+  }
+  CHK(ensure_scan(d, std::max(pn, d->ncell)));
+  long long ngl = 0;
+  if (pn > 0) {
+    hipLaunchKernelGGL(k_count_images, dim3(nblk(pn)), dim3(kBlock), 0, st, d->pos, pn, E, d->nimg);
+    exclusive_scan<int, int>(d->nimg, d->goff, pn, d->scan_scratch, d->i64, st);
+    CHK(read_total(d, &ngl));
+  }
+  if (ngl + pn + 1 > kMaxParticles) {
+    d->err = "particle capacity exceeded (owned + ghosts)";
+    return MDGPU_ERR_CAPACITY;
+  }
+  const int ng = (int)ngl;
+  CHK(ensure_ghosts(d, ng));
+  HIPCHK(d, hipMemsetAsync(d->c_gcount, 0, sizeof(int) * d->ncell, st));
+  if (ng > 0) {
+    // pos_tmp[0..ng) and cell_alt[] hold the unsorted images
+    hipLaunchKernelGGL(k_emit_images, dim3(nblk(pn)), dim3(kBlock), 0, st, d->pos, pn, E, d->L[0], d->L[1],
+                       d->L[2], g, d->goff, ng, d->t_gsrc, d->t_gcode, d->pos_tmp, d->t_gcell, d->slot,
+                       d->c_gcount, d->key, d->id);
+    CHK(check_launch(d, "emit images"));
+    CHK(cell_sort(d, d->t_gcell, ng, d->c_gcount, d->c_gstart, d->key));
+    hipLaunchKernelGGL(k_gather_ghosts, dim3(nblk(ng)), dim3(kBlock), 0, st, d->perm, ng, pn, d->t_gsrc,
+                       d->t_gcode, d->pos_tmp, d->t_gcell, d->gsrc, d->gcode, d->pos, d->cell);
+  } else {
+    CHK(ensure_scan(d, d->ncell));
+    exclusive_scan<int, int>(d->c_gcount, d->c_gstart, d->ncell, d->scan_scratch, nullptr, st);
+  }
+  d->ng = ng;
+  d->tn = pn + ng;
+  hipLaunchKernelGGL(k_set_sentinel, dim3(1), dim3(1), 0, st, d->pos, d->pos_tmp, d->rel, d->rel_tmp, d->tn);
+  // 3. PairList::Init: snapshot of owned + ghost positions
+  if (d->tn > 0)
+    hipLaunchKernelGGL(k_copy_pos, dim3(nblk(d->tn)), dim3(kBlock), 0, st, d->pos, d->pos_old, d->tn);
+  // 4. MeshList::MakeList: full neighbour list in sliced-ELL layout
+  const CellTables T{d->c_ostart, d->c_ocount, d->c_gstart, d->c_gcount};
+  d->nslice = (pn + kWave - 1) / kWave;
+  long long slots = 0;
+  if (pn > 0) {
+    hipLaunchKernelGGL(k_count_rows, dim3(nblk(pn)), dim3(kBlock), 0, st, d->pos, d->cell, pn, g, T, d->S2,
+                       d->row_len);
+    hipLaunchKernelGGL(k_slice_len, dim3(nblk(d->nslice)), dim3(kBlock), 0, st, d->row_len, pn, d->nslice,
+                       d->slen, d->slots64);
+    // number of listed (row, partner) entries, for accounting
+    exclusive_scan<int, long long>(d->row_len, d->key, pn, d->scan_scratch, d->i64, st);
+    CHK(read_total(d, &d->nbr_entries));
+    // slice offsets in entries: scan of 64 * slice length
+    exclusive_scan<long long, long long>(d->slots64, d->sbase, d->nslice, d->scan_scratch, d->i64, st);
+    CHK(read_total(d, &slots));
+  } else {
+    d->nbr_entries = 0;
+  }
+  if (slots > d->nbr_cap) {
+    const long long nc = slots + slots / 8 + 1024;
+    CHK(dalloc(d, &d->nbr, (size_t)nc));
+    d->nbr_cap = nc;
+  }
+  d->nbr_slots = slots;
+  if (pn > 0) {
+    const uint32_t pad = (uint32_t)d->tn | (kZeroOffset << kIndexBits);
+    hipLaunchKernelGGL(k_fill_rows, dim3(nblk(pn)), dim3(kBlock), 0, st, d->pos, d->cell, pn, g, T, d->S2,
+                       d->sbase, d->slen, d->nbr, pad, d->i32);
+    hipLaunchKernelGGL(k_fill_tail, dim3(1), dim3(kWave), 0, st, pn, d->nslice, d->sbase, d->slen, d->nbr, pad);
+  }
+  CHK(check_launch(d, "neighbour list"));
+  CHK(read_err_flags(d));
+  d->initialized = true;
+  d->buffer_length = d->margin;
+  d->lifetime = 0;
+  d->vmax_valid = false;
+  return MDGPU_OK;
+}
+
+int max_p2(mdgpu_domain *d, double *v2) {
+  if (!d->vmax_valid) {
+    HIPCHK(d, hipMemsetAsync(d->u64, 0, sizeof(unsigned long long), d->stream));
+    if (d->pn > 0)
+      hipLaunchKernelGGL(k_max_p2, dim3(nblk(d->pn)), dim3(kBlock), 0, d->stream, d->mom, d->pn, d->u64);
+  }
+  HIPCHK(d, hipMemcpyAsync(d->h_u64, d->u64, sizeof(unsigned long long), hipMemcpyDeviceToHost, d->stream));
+  CHK(sync(d));
+  unsigned long long b = d->h_u64[0];
+  double v;
+  std::memcpy(&v, &b, sizeof(v));
+  *v2 = v;
+  return MDGPU_OK;
+}
+
+// PairList::CheckByDisplacement (pairlist.cc:100-154)
+int check_by_displacement(mdgpu_domain *d, int *expired) {
+  hipStream_t st = d->stream;
+  const int tn = d->tn;
+  *expired = 0;
+  HIPCHK(d, hipMemsetAsync(d->u64 + 1, 0, sizeof(unsigned long long), st));
+  if (tn > 0)
+    hipLaunchKernelGGL(k_displacement, dim3(nblk(tn)), dim3(kBlock), 0, st, d->pos, d->pos_old, tn, d->disp,
+                       d->u64 + 1);
+  HIPCHK(d, hipMemcpyAsync(d->h_u64 + 1, d->u64 + 1, sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
+  CHK(sync(d));
+  double dr2_max;
+  std::memcpy(&dr2_max, d->h_u64 + 1, sizeof(double));
+  const int Nl = (tn > d->check_list_length) ? d->check_list_length : tn;
+  const double c_distance = d->margin - std::sqrt(dr2_max);
+  const double c_d2 = c_distance * c_distance;
+  HIPCHK(d, hipMemsetAsync(d->i32 + 1, 0, sizeof(int) * 2, st));
+  if (tn > 0)
+    hipLaunchKernelGGL(k_candidates, dim3(nblk(tn)), dim3(kBlock), 0, st, d->disp, tn, c_d2, d->i32 + 1,
+                       d->cand, d->check_list_length);
+  HIPCHK(d, hipMemcpyAsync(d->h_i32, d->i32, sizeof(int) * 4, hipMemcpyDeviceToHost, st));
+  CHK(sync(d));
+  const int count = d->h_i32[1];
+  if (count > 0 && count >= Nl) {
+    *expired = 1;
+    return MDGPU_OK;
+  }
+  const int l = count;
+  if (l > 1) {
+    const long long total = (long long)l * l;
+    hipLaunchKernelGGL(k_candidate_pairs, dim3(nblk(total)), dim3(kBlock), 0, st, d->pos, d->pos_old, d->cand, l,
+                       d->SL * d->SL, d->CL2, d->i32 + 2);
+    HIPCHK(d, hipMemcpyAsync(d->h_i32, d->i32, sizeof(int) * 4, hipMemcpyDeviceToHost, st));
+    CHK(sync(d));
+    *expired = d->h_i32[2] != 0;
+  }
+  return MDGPU_OK;
+}
+
+// PairList::IsPairListExpired (pairlist.cc:58-78) with
+// UpdatePairListValidity (pairlist.cc:80-98)
+int is_expired(mdgpu_domain *d, int *expired) {
+  if (!d->initialized) {
+    *expired = 1;
+    return MDGPU_OK;
+  }
+  d->lifetime++;
+  if (d->buffer_length > 0.0) {
+    double v2 = 0.0;
+    CHK(max_p2(d, &v2));
+    const double vmax = std::sqrt(v2);
+    if (vmax * 2.0 * d->dt > d->margin) {
+      char b[160];
+      snprintf(b, sizeof b, "Too fast particles exists. Try smaller time step (max_velocity = %f)", vmax);
+      d->err = b;
+      return MDGPU_ERR_TOO_FAST;
+    }
+    d->buffer_length = d->buffer_length - vmax * 2.0 * d->dt;
+  }
+  *expired = 0;
+  if (d->buffer_length < 0.0) CHK(check_by_displacement(d, expired));
+  return MDGPU_OK;
+}
+
+ForceConsts fconsts(const mdgpu_domain *d) {
+  ForceConsts fc;
+  fc.CL2 = d->CL2;
+  fc.A = 24.0 * d->dt;
+  fc.B = 48.0 * d->dt;
+  fc.C = 8.0 * d->C2 * d->dt;
+  fc.dt2 = d->dt * 0.5;
+  return fc;
+}
+
+ForceConstsF fconsts32(const mdgpu_domain *d) {
+  ForceConstsF f;
+  f.CL2 = (float)d->CL2;
+  f.A = (float)(24.0 * d->dt);
+  f.B = (float)(48.0 * d->dt);
+  f.C = (float)(8.0 * d->C2 * d->dt);
+  for (int k = 0; k < 3; k++) f.w[k] = (float)d->msize[k];
+  return f;
+}
+
+LangevinConsts lconsts(const mdgpu_domain *d) {
+  LangevinConsts lc;
+  lc.gamma = d->gamma;
+  lc.hb_D = std::sqrt(2.0 * d->gamma * d->T_aim / d->dt);
+  lc.dt = d->dt;
+  lc.seed = d->seed;
+  lc.step = d->step;
+  return lc;
+}
+
+// One pair-force launch.  `src`/`rel_src` are the positions the forces are
+// evaluated at; `pos_out` receives the drifted (or copied) owned positions.
+int launch_force(mdgpu_domain *d, const double4 *src, const float4 *rel_src, double4 *pos_out, int epilogue,
+                 bool want_vmax) {
+  if (d->pn == 0) return MDGPU_OK;
+  hipStream_t st = d->stream;
+  const ForceConsts fc = fconsts(d);
+  const LangevinConsts lc = lconsts(d);
+  unsigned long long *vb = want_vmax ? d->u64 : nullptr;
+  const int ev0 = d->force_launches * 2;
+  if (d->timing && ev0 + 1 < (int)d->ev.size()) HIPCHK(d, hipEventRecord(d->ev[ev0], st));
+  if (d->precision == MDGPU_FP64) {
+    hipLaunchKernelGGL(k_force_fp64, dim3(nblk(d->pn)), dim3(kBlock), 0, st, src, d->mom, pos_out, d->nbr,
+                       d->sbase, d->slen, d->pn, fc, epilogue, lc, d->id, vb);
+  } else if (d->kahan) {
+    hipLaunchKernelGGL((k_force_mixed<true>), dim3(nblk(d->pn)), dim3(kBlock), 0, st, rel_src, src, d->mom,
+                       pos_out, d->nbr, d->sbase, d->slen, d->pn, fconsts32(d), fc.dt2, epilogue, lc, d->id, vb);
+  } else {
+    hipLaunchKernelGGL((k_force_mixed<false>), dim3(nblk(d->pn)), dim3(kBlock), 0, st, rel_src, src, d->mom,
+                       pos_out, d->nbr, d->sbase, d->slen, d->pn, fconsts32(d), fc.dt2, epilogue, lc, d->id, vb);
+  }
+  if (d->timing && ev0 + 1 < (int)d->ev.size()) {
+    HIPCHK(d, hipEventRecord(d->ev[ev0 + 1], st));
+    d->force_launches++;
+  }
+  return check_launch(d, "force");
+}
+
+int kinetic(mdgpu_domain *d, double *ke) {
+  const unsigned nb = nblk(d->pn);
+  CHK(ensure_partial(d, (int)nb));
+  if (d->pn > 0) {
+    hipLaunchKernelGGL(k_kinetic, dim3(nb), dim3(kBlock), 0, d->stream, d->mom, d->pn, d->partial);
+    hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(kBlock), 0, d->stream, d->partial, (int)nb, 1, d->d3);
+    HIPCHK(d, hipMemcpyAsync(d->h_d3, d->d3, sizeof(double), hipMemcpyDeviceToHost, d->stream));
+    CHK(sync(d));
+    *ke = 0.0 + d->h_d3[0];
+  } else {
+    *ke = 0.0;
+  }
+  return MDGPU_OK;
+}
+
+int temperature(mdgpu_domain *d, double *t) {
+  double ke;
+  CHK(kinetic(d, &ke));
+  *t = (ke / (double)d->pn) / 1.5;
+  return MDGPU_OK;
+}
+
+int heatbath_zeta(mdgpu_domain *d, double t) {
+  const double dt2 = d->dt * 0.5;
+  const double t1 = (t - d->T_aim) / (d->tau * d->tau);
+  d->zeta += t1 * dt2;
+  return MDGPU_OK;
+}
+
+int heatbath_momenta(mdgpu_domain *d) {
+  const double e1 = std::exp(-(d->dt * 0.5) * d->zeta);
+  if (d->pn > 0)
+    hipLaunchKernelGGL(k_scale_momenta, dim3(nblk(d->pn)), dim3(kBlock), 0, d->stream, d->mom, d->pn, e1);
+  d->vmax_valid = false;
+  return check_launch(d, "heatbath momenta");
+}
+
+int drift(mdgpu_domain *d) {
+  if (d->pn > 0)
+    hipLaunchKernelGGL(k_drift, dim3(nblk(d->pn)), dim3(kBlock), 0, d->stream, d->pos, d->mom, d->pn,
+                       d->dt * 0.5);
+  return check_launch(d, "drift");
+}
+
+int refresh(mdgpu_domain *d) {
+  if (d->ng > 0)
+    hipLaunchKernelGGL(k_refresh, dim3(nblk(d->ng)), dim3(kBlock), 0, d->stream, d->pos, d->gsrc, d->gcode, d->pn,
+                       d->ng, d->L[0], d->L[1], d->L[2]);
+  return check_launch(d, "refresh");
+}
+
+// CalculateForce on the current positions (owned + ghosts in `pos`)
+int force_in_place(mdgpu_domain *d) {
+  if (d->precision == MDGPU_MIXED && d->tn > 0)
+    hipLaunchKernelGGL(k_to_rel, dim3(nblk(d->tn)), dim3(kBlock), 0, d->stream, d->pos, d->cell, d->tn,
+                       origins_of(d), d->rel);
+  CHK(launch_force(d, d->pos, d->rel, d->pos, kKickOnly, false));
+  d->vmax_valid = false;
+  return MDGPU_OK;
+}
+
+// MDManager::Calculate (mdmanager.cc:115-148) for one domain
+int calculate_step(mdgpu_domain *d) {
+  hipStream_t st = d->stream;
+  int expired = 0;
+  CHK(is_expired(d, &expired));
+  if (expired) {
+    if (d->timing) {
+      hipEvent_t a, b;
+      HIPCHK(d, hipEventCreate(&a));
+      HIPCHK(d, hipEventCreate(&b));
+      HIPCHK(d, hipEventRecord(a, st));
+      CHK(make_pair_list(d));
+      HIPCHK(d, hipEventRecord(b, st));
+      HIPCHK(d, hipEventSynchronize(b));
+      float ms = 0.f;
+      HIPCHK(d, hipEventElapsedTime(&ms, a, b));
+      d->rebuild_ms += ms;
+      hipEventDestroy(a);
+      hipEventDestroy(b);
+    } else {
+      CHK(make_pair_list(d));
+    }
+    d->rebuild_count++;
+  }
+  if (d->thermostat == MDGPU_NOSEHOOVER) {
+    // CalculateNoseHoover (mdmanager.cc:159-175)
+    CHK(drift(d));
+    CHK(refresh(d));
+    double t;
+    CHK(temperature(d, &t));
+    CHK(heatbath_zeta(d, t));
+    CHK(heatbath_momenta(d));
+    CHK(force_in_place(d));
+    CHK(heatbath_momenta(d));
+    CHK(temperature(d, &t));
+    CHK(heatbath_zeta(d, t));
+    CHK(drift(d));
+  } else {
+    // fused: drift + halo refresh into pos_tmp, then kick (+ drift or Langevin)
+    const int tot = d->pn + d->ng;
+    const bool mixed = d->precision == MDGPU_MIXED;
+    if (tot > 0) {
+      if (mixed)
+        hipLaunchKernelGGL((k_drift_refresh<true>), dim3(nblk(tot)), dim3(kBlock), 0, st, d->pos, d->mom,
+                           d->pos_tmp, d->gsrc, d->gcode, d->pn, d->ng, d->dt * 0.5, d->L[0], d->L[1], d->L[2],
+                           d->cell, origins_of(d), d->rel_tmp, d->u64);
+      else
+        hipLaunchKernelGGL((k_drift_refresh<false>), dim3(nblk(tot)), dim3(kBlock), 0, st, d->pos, d->mom,
+                           d->pos_tmp, d->gsrc, d->gcode, d->pn, d->ng, d->dt * 0.5, d->L[0], d->L[1], d->L[2],
+                           d->cell, origins_of(d), d->rel_tmp, d->u64);
+      CHK(check_launch(d, "drift+refresh"));
+    }
+    const int epi = d->thermostat == MDGPU_LANGEVIN ? kKickLangevin : kKickDrift;
+    CHK(launch_force(d, d->pos_tmp, d->rel_tmp, d->pos, epi, true));
+    d->vmax_valid = true;
+  }
+  d->sim_time += d->dt;
+  d->step++;
+  return MDGPU_OK;
+}
+
+}  // namespace
+
+// ======================================================================= C ABI
+extern "C" {
+
+int mdgpu_create(const mdgpu_params *prm, mdgpu_domain **out) {
+  if (!prm || !out) return MDGPU_ERR_ARGUMENT;
+  *out = nullptr;
+  mdgpu_domain *d = new mdgpu_domain();
+  d->prm = *prm;
+  if (hipSetDevice(prm->device) != hipSuccess || hipStreamCreateWithFlags(&d->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete d;
+    return MDGPU_ERR_HIP;
+  }
+  for (int k = 0; k < 3; k++) {
+    d->L[k] = prm->L[k];
+    // single domain of a 1x1x1 grid: s = L*0, e = s + L (mdunit.cc:96-101)
+    d->rs[k] = prm->L[k] * 0.0;
+    d->re[k] = d->rs[k] + prm->L[k];
+  }
+  d->dt = prm->dt;
+  d->rc = prm->rc;
+  d->margin = prm->margin;
+  d->SL = prm->rc + prm->margin;
+  d->S2 = d->SL * d->SL;
+  d->CL2 = prm->rc * prm->rc;
+  d->check_list_length = prm->check_list_length > 0 ? prm->check_list_length : 500;
+  d->precision = prm->precision;
+  // Variables::Variables (variables.cc:130-142)
+  const double s2 = 1.0 / (prm->rc * prm->rc);
+  const double s6 = s2 * s2 * s2;
+  const double s8 = s6 * s2;
+  const double s12 = s6 * s6;
+  const double s14 = s12 * s2;
+  d->C2 = 6.0 * s14 - 3.0 * s8;
+  d->C0 = -s12 + s6 - d->C2 / s2;
+  int r;
+  if ((r = set_geometry(d)) != MDGPU_OK || (r = ensure_owned(d, std::max(prm->capacity, 1024))) != MDGPU_OK ||
+      (r = ensure_ghosts(d, std::max(1024, prm->capacity / 4))) != MDGPU_OK) {
+    mdgpu_destroy(d);
+    return r;
+  }
+  if ((r = dalloc(d, &d->u64, 4)) || (r = dalloc(d, &d->i64, 4)) || (r = dalloc(d, &d->i32, 4)) ||
+      (r = dalloc(d, &d->d3, 4)) || (r = dalloc(d, &d->cand, (size_t)d->check_list_length + 1))) {
+    mdgpu_destroy(d);
+    return r;
+  }
+  if (hipHostMalloc((void **)&d->h_u64, 4 * sizeof(unsigned long long)) != hipSuccess ||
+      hipHostMalloc((void **)&d->h_i64, 4 * sizeof(long long)) != hipSuccess ||
+      hipHostMalloc((void **)&d->h_i32, 4 * sizeof(int)) != hipSuccess ||
+      hipHostMalloc((void **)&d->h_d3, 4 * sizeof(double)) != hipSuccess) {
+    mdgpu_destroy(d);
+    return MDGPU_ERR_HIP;
+  }
+  hipMemset(d->i32, 0, 4 * sizeof(int));
+  hipMemset(d->u64, 0, 4 * sizeof(unsigned long long));
+  *out = d;
+  return MDGPU_OK;
+}
+
+int mdgpu_destroy(mdgpu_domain *d) {
+  if (!d) return MDGPU_OK;
+  if (d->stream) hipStreamSynchronize(d->stream);
+  void *bufs[] = {d->pos, d->pos_tmp, d->pos_old, d->mom, d->mom_alt, d->rel, d->rel_tmp, d->id, d->id_alt,
+                  d->cell, d->cell_alt, d->gsrc, d->gcode, d->c_ocount, d->c_ostart, d->c_gcount, d->c_gstart,
+                  d->org, d->nbr, d->sbase, d->slen, d->row_len, d->slot, d->perm, d->nimg, d->goff, d->t_gsrc,
+                  d->t_gcode, d->t_gcell, d->key, d->scan_scratch, d->slots64, d->disp, d->partial, d->cand,
+                  d->u64, d->i64, d->i32, d->d3};
+  for (void *b : bufs)
+    if (b) hipFree(b);
+  if (d->h_u64) hipHostFree(d->h_u64);
+  if (d->h_i64) hipHostFree(d->h_i64);
+  if (d->h_i32) hipHostFree(d->h_i32);
+  if (d->h_d3) hipHostFree(d->h_d3);
+  for (hipEvent_t e : d->ev) hipEventDestroy(e);
+  if (d->stream) hipStreamDestroy(d->stream);
+  delete d;
+  return MDGPU_OK;
+}
+
+const char *mdgpu_last_error(const mdgpu_domain *d) { return d ? d->err.c_str() : "null domain"; }
+
+int mdgpu_set_precision(mdgpu_domain *d, int precision) {
+  if (!d || (precision != MDGPU_FP64 && precision != MDGPU_MIXED)) return MDGPU_ERR_ARGUMENT;
+  d->precision = precision;
+  return MDGPU_OK;
+}
+
+int mdgpu_add_particles(mdgpu_domain *d, int n, const double *q, const double *p, const int *type) {
+  if (!d || n < 0 || (n > 0 && !q)) return MDGPU_ERR_ARGUMENT;
+  std::vector<double4> hq, hp;
+  std::vector<int> hid;
+  hq.reserve(n);
+  for (int i = 0; i < n; i++) {
+    const double *x = q + 3 * (size_t)i;
+    // MDRect::IsInside (mdrect.h:319-322)
+    if (!(d->rs[0] <= x[0] && x[0] < d->re[0] && d->rs[1] <= x[1] && x[1] < d->re[1] && d->rs[2] <= x[2] &&
+          x[2] < d->re[2]))
+      continue;
+    hq.push_back(make_double4(x[0], x[1], x[2], 0.0));
+    hp.push_back(p ? make_double4(p[3 * (size_t)i], p[3 * (size_t)i + 1], p[3 * (size_t)i + 2], 0.0)
+                   : make_double4(0.0, 0.0, 0.0, 0.0));
+    hid.push_back(type ? type[i] : 0);
+  }
+  const int add = (int)hq.size();
+  if (add == 0) return MDGPU_OK;
+  CHK(ensure_owned(d, d->pn + add));
+  // ghosts of an earlier build are dropped (Variables::AddParticle sets total = owned)
+  HIPCHK(d, hipMemcpyAsync(d->pos + d->pn, hq.data(), add * sizeof(double4), hipMemcpyHostToDevice, d->stream));
+  HIPCHK(d, hipMemcpyAsync(d->mom + d->pn, hp.data(), add * sizeof(double4), hipMemcpyHostToDevice, d->stream));
+  HIPCHK(d, hipMemcpyAsync(d->id + d->pn, hid.data(), add * sizeof(int), hipMemcpyHostToDevice, d->stream));
+  CHK(sync(d));
+  d->pn += add;
+  d->tn = d->pn;
+  d->ng = 0;
+  d->vmax_valid = false;
+  // indices of the old list and its sentinel are stale: the next step rebuilds
+  d->initialized = false;
+  return MDGPU_OK;
+}
+
+int mdgpu_particle_number(const mdgpu_domain *d, int *pn, int *tn) {
+  if (!d) return MDGPU_ERR_ARGUMENT;
+  if (pn) *pn = d->pn;
+  if (tn) *tn = d->tn;
+  return MDGPU_OK;
+}
+
+int mdgpu_get_particles(mdgpu_domain *d, double *q, double *p, int *type) {
+  if (!d) return MDGPU_ERR_ARGUMENT;
+  const int n = d->pn;
+  std::vector<double4> hq(n), hp(n);
+  if (n > 0) {
+    HIPCHK(d, hipMemcpyAsync(hq.data(), d->pos, n * sizeof(double4), hipMemcpyDeviceToHost, d->stream));
+    HIPCHK(d, hipMemcpyAsync(hp.data(), d->mom, n * sizeof(double4), hipMemcpyDeviceToHost, d->stream));
+    if (type) HIPCHK(d, hipMemcpyAsync(type, d->id, n * sizeof(int), hipMemcpyDeviceToHost, d->stream));
+  }
+  CHK(sync(d));
+  for (int i = 0; i < n; i++) {
+    if (q) {
+      q[3 * (size_t)i] = hq[i].x;
+      q[3 * (size_t)i + 1] = hq[i].y;
+      q[3 * (size_t)i + 2] = hq[i].z;
+    }
+    if (p) {
+      p[3 * (size_t)i] = hp[i].x;
+      p[3 * (size_t)i + 1] = hp[i].y;
+      p[3 * (size_t)i + 2] = hp[i].z;
+    }
+  }
+  return MDGPU_OK;
+}
+
+int mdgpu_set_momenta(mdgpu_domain *d, const double *p) {
+  if (!d || !p) return MDGPU_ERR_ARGUMENT;
+  const int n = d->pn;
+  std::vector<double4> hp(n);
+  for (int i = 0; i < n; i++) hp[i] = make_double4(p[3 * (size_t)i], p[3 * (size_t)i + 1], p[3 * (size_t)i + 2], 0.0);
+  if (n > 0) HIPCHK(d, hipMemcpyAsync(d->mom, hp.data(), n * sizeof(double4), hipMemcpyHostToDevice, d->stream));
+  CHK(sync(d));
+  d->vmax_valid = false;
+  return MDGPU_OK;
+}
+
+int mdgpu_get_ghosts(mdgpu_domain *d, int *src_type, int *shift_code, double *q) {
+  if (!d) return MDGPU_ERR_ARGUMENT;
+  const int ng = d->ng, pn = d->pn;
+  std::vector<int> gs(ng), gc(ng), ids(pn);
+  std::vector<double4> gq(ng);
+  if (ng > 0) {
+    HIPCHK(d, hipMemcpyAsync(gs.data(), d->gsrc, ng * sizeof(int), hipMemcpyDeviceToHost, d->stream));
+    HIPCHK(d, hipMemcpyAsync(gc.data(), d->gcode, ng * sizeof(int), hipMemcpyDeviceToHost, d->stream));
+    HIPCHK(d, hipMemcpyAsync(gq.data(), d->pos + pn, ng * sizeof(double4), hipMemcpyDeviceToHost, d->stream));
+    HIPCHK(d, hipMemcpyAsync(ids.data(), d->id, pn * sizeof(int), hipMemcpyDeviceToHost, d->stream));
+  }
+  CHK(sync(d));
+  for (int k = 0; k < ng; k++) {
+    if (src_type) src_type[k] = ids[gs[k]];
+    if (shift_code) shift_code[k] = gc[k];
+    if (q) {
+      q[3 * (size_t)k] = gq[k].x;
+      q[3 * (size_t)k + 1] = gq[k].y;
+      q[3 * (size_t)k + 2] = gq[k].z;
+    }
+  }
+  return MDGPU_OK;
+}
+
+int mdgpu_set_thermostat(mdgpu_domain *d, int kind, double T, double gamma, double tau) {
+  if (!d || kind < MDGPU_NVE || kind > MDGPU_LANGEVIN) return MDGPU_ERR_ARGUMENT;
+  d->thermostat = kind;
+  d->T_aim = T;
+  d->gamma = gamma;
+  d->tau = tau;
+  return MDGPU_OK;
+}
+
+int mdgpu_set_seed(mdgpu_domain *d, uint64_t seed) {
+  if (!d) return MDGPU_ERR_ARGUMENT;
+  d->seed = seed;
+  return MDGPU_OK;
+}
+
+int mdgpu_get_zeta(const mdgpu_domain *d, double *zeta) {
+  if (!d || !zeta) return MDGPU_ERR_ARGUMENT;
+  *zeta = d->zeta;
+  return MDGPU_OK;
+}
+
+int mdgpu_get_time(const mdgpu_domain *d, double *sim_time, uint64_t *step) {
+  if (!d) return MDGPU_ERR_ARGUMENT;
+  if (sim_time) *sim_time = d->sim_time;
+  if (step) *step = d->step;
+  return MDGPU_OK;
+}
+
+int mdgpu_make_pair_list(mdgpu_domain *d) {
+  if (!d) return MDGPU_ERR_ARGUMENT;
+  return make_pair_list(d);
+}
+
+int mdgpu_is_pair_list_expired(mdgpu_domain *d, int *expired) {
+  if (!d || !expired) return MDGPU_ERR_ARGUMENT;
+  return is_expired(d, expired);
+}
+
+int mdgpu_update_position_half(mdgpu_domain *d) {
+  if (!d) return MDGPU_ERR_ARGUMENT;
+  return drift(d);
+}
+
+int mdgpu_send_border_particles(mdgpu_domain *d) {
+  if (!d) return MDGPU_ERR_ARGUMENT;
+  return refresh(d);
+}
+
+int mdgpu_calculate_force(mdgpu_domain *d) {
+  if (!d) return MDGPU_ERR_ARGUMENT;
+  if (!d->initialized) {
+    d->err = "pair list not built";
+    return MDGPU_ERR_ARGUMENT;
+  }
+  return force_in_place(d);
+}
+
+int mdgpu_langevin(mdgpu_domain *d) {
+  if (!d) return MDGPU_ERR_ARGUMENT;
+  if (d->pn > 0)
+    hipLaunchKernelGGL(k_langevin, dim3(nblk(d->pn)), dim3(kBlock), 0, d->stream, d->mom, d->id, d->pn,
+                       lconsts(d));
+  d->vmax_valid = false;
+  return check_launch(d, "langevin");
+}
+
+int mdgpu_heatbath_zeta(mdgpu_domain *d, double t) {
+  if (!d) return MDGPU_ERR_ARGUMENT;
+  return heatbath_zeta(d, t);
+}
+
+int mdgpu_heatbath_momenta(mdgpu_domain *d) {
+  if (!d) return MDGPU_ERR_ARGUMENT;
+  return heatbath_momenta(d);
+}
+
+int mdgpu_calculate(mdgpu_domain *d, int nsteps, int *rebuilds) {
+  if (!d || nsteps < 0) return MDGPU_ERR_ARGUMENT;
+  hipEvent_t t0 = nullptr, t1 = nullptr;
+  d->force_launches = 0;
+  d->force_ms = d->total_ms = d->rebuild_ms = 0.0;
+  d->rebuild_count = 0;
+  if (d->timing) {
+    const size_t need = (size_t)nsteps * 2 + 2;
+    while (d->ev.size() < need) {
+      hipEvent_t e;
+      HIPCHK(d, hipEventCreate(&e));
+      d->ev.push_back(e);
+    }
+    HIPCHK(d, hipEventCreate(&t0));
+    HIPCHK(d, hipEventCreate(&t1));
+    HIPCHK(d, hipEventRecord(t0, d->stream));
+  }
+  int r = MDGPU_OK;
+  for (int s = 0; s < nsteps && r == MDGPU_OK; s++) r = calculate_step(d);
+  if (r == MDGPU_OK) r = sync(d);
+  if (d->timing && t0) {
+    HIPCHK(d, hipEventRecord(t1, d->stream));
+    HIPCHK(d, hipEventSynchronize(t1));
+    float ms = 0.f;
+    hipEventElapsedTime(&ms, t0, t1);
+    d->total_ms = ms;
+    for (int k = 0; k < d->force_launches; k++) {
+      hipEventElapsedTime(&ms, d->ev[2 * k], d->ev[2 * k + 1]);
+      d->force_ms += ms;
+    }
+    hipEventDestroy(t0);
+    hipEventDestroy(t1);
+  }
+  if (rebuilds) *rebuilds = d->rebuild_count;
+  return r;
+}
+
+int mdgpu_observe(mdgpu_domain *d, double *ke, double *pe, double *virial) {
+  if (!d) return MDGPU_ERR_ARGUMENT;
+  if (!d->initialized) {
+    d->err = "pair list not built";
+    return MDGPU_ERR_ARGUMENT;
+  }
+  const unsigned nb = nblk(d->pn);
+  CHK(ensure_partial(d, (int)nb));
+  double out[3] = {0.0, 0.0, 0.0};
+  if (d->pn > 0) {
+    hipLaunchKernelGGL(k_observe, dim3(nb), dim3(kBlock), 0, d->stream, d->pos, d->mom, d->nbr, d->sbase, d->slen,
+                       d->pn, d->CL2, d->C2, d->C0, d->partial);
+    hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(kBlock), 0, d->stream, d->partial, (int)nb, 3, d->d3);
+    CHK(check_launch(d, "observe"));
+    HIPCHK(d, hipMemcpyAsync(d->h_d3, d->d3, 3 * sizeof(double), hipMemcpyDeviceToHost, d->stream));
+    CHK(sync(d));
+    std::memcpy(out, d->h_d3, sizeof(out));
+  }
+  if (ke) *ke = out[0];
+  if (pe) *pe = out[1];
+  if (virial) *virial = out[2] / 3.0;
+  return MDGPU_OK;
+}
+
+int mdgpu_change_scale(mdgpu_domain *d, double alpha) {
+  if (!d || !(alpha > 0.0)) return MDGPU_ERR_ARGUMENT;
+  CHK(make_pair_list(d));
+  for (int k = 0; k < 3; k++) {
+    d->L[k] = d->L[k] * alpha;
+    d->rs[k] *= alpha;
+    d->re[k] *= alpha;
+  }
+  if (d->pn > 0)
+    hipLaunchKernelGGL(k_scale_pos, dim3(nblk(d->pn)), dim3(kBlock), 0, d->stream, d->pos, d->pn, alpha);
+  CHK(check_launch(d, "scale"));
+  CHK(set_geometry(d));
+  return make_pair_list(d);
+}
+
+int mdgpu_get_pairs(mdgpu_domain *d, int64_t *npairs, int32_t *out3) {
+  if (!d || !npairs) return MDGPU_ERR_ARGUMENT;
+  const int pn = d->pn, ng = d->ng;
+  std::vector<uint32_t> nbr((size_t)d->nbr_slots);
+  std::vector<long long> sb(d->nslice);
+  std::vector<int> sl(d->nslice), ids(pn), gs(ng), gc(ng);
+  if (d->nbr_slots)
+    HIPCHK(d, hipMemcpyAsync(nbr.data(), d->nbr, nbr.size() * 4, hipMemcpyDeviceToHost, d->stream));
+  if (d->nslice) {
+    HIPCHK(d, hipMemcpyAsync(sb.data(), d->sbase, d->nslice * 8, hipMemcpyDeviceToHost, d->stream));
+    HIPCHK(d, hipMemcpyAsync(sl.data(), d->slen, d->nslice * 4, hipMemcpyDeviceToHost, d->stream));
+  }
+  if (pn) HIPCHK(d, hipMemcpyAsync(ids.data(), d->id, pn * 4, hipMemcpyDeviceToHost, d->stream));
+  if (ng) {
+    HIPCHK(d, hipMemcpyAsync(gs.data(), d->gsrc, ng * 4, hipMemcpyDeviceToHost, d->stream));
+    HIPCHK(d, hipMemcpyAsync(gc.data(), d->gcode, ng * 4, hipMemcpyDeviceToHost, d->stream));
+  }
+  CHK(sync(d));
+  int64_t cnt = 0;
+  for (int i = 0; i < pn; i++) {
+    const int s = i / kWave, lane = i % kWave;
+    for (int k = 0; k < sl[s]; k++) {
+      const int j = (int)(nbr[sb[s] + (long long)k * kWave + lane] & kIndexMask);
+      if (j >= d->tn) continue;
+      int a = ids[i], b, code = 13;
+      if (j < pn) {
+        b = ids[j];
+        if (b < a) continue;  // owned-owned pairs appear in both rows
+      } else {
+        b = ids[gs[j - pn]];
+        code = gc[j - pn];
+      }
+      if (out3) {
+        out3[3 * cnt] = a;
+        out3[3 * cnt + 1] = b;
+        out3[3 * cnt + 2] = code;
+      }
+      cnt++;
+    }
+  }
+  *npairs = cnt;
+  return MDGPU_OK;
+}
+
+int mdgpu_compute_kick(mdgpu_domain *d, double *dp) {
+  if (!d || !dp) return MDGPU_ERR_ARGUMENT;
+  if (!d->initialized) {
+    d->err = "pair list not built";
+    return MDGPU_ERR_ARGUMENT;
+  }
+  const int n = d->pn;
+  if (n == 0) return MDGPU_OK;
+  HIPCHK(d, hipMemcpyAsync(d->mom_alt, d->mom, n * sizeof(double4), hipMemcpyDeviceToDevice, d->stream));
+  HIPCHK(d, hipMemsetAsync(d->mom, 0, n * sizeof(double4), d->stream));
+  CHK(force_in_place(d));
+  std::vector<double4> hp(n);
+  HIPCHK(d, hipMemcpyAsync(hp.data(), d->mom, n * sizeof(double4), hipMemcpyDeviceToHost, d->stream));
+  HIPCHK(d, hipMemcpyAsync(d->mom, d->mom_alt, n * sizeof(double4), hipMemcpyDeviceToDevice, d->stream));
+  CHK(sync(d));
+  for (int i = 0; i < n; i++) {
+    dp[3 * (size_t)i] = hp[i].x;
+    dp[3 * (size_t)i + 1] = hp[i].y;
+    dp[3 * (size_t)i + 2] = hp[i].z;
+  }
+  return MDGPU_OK;
+}
+
+int mdgpu_enable_timing(mdgpu_domain *d, int on) {
+  if (!d) return MDGPU_ERR_ARGUMENT;
+  d->timing = on != 0;
+  return MDGPU_OK;
+}
+
+int mdgpu_get_timing(const mdgpu_domain *d, int *force_launches, double *force_ms, double *total_ms,
+                     int *rebuilds, double *rebuild_ms) {
+  if (!d) return MDGPU_ERR_ARGUMENT;
+  if (force_launches) *force_launches = d->force_launches;
+  if (force_ms) *force_ms = d->force_ms;
+  if (total_ms) *total_ms = d->total_ms;
+  if (rebuilds) *rebuilds = d->rebuild_count;
+  if (rebuild_ms) *rebuild_ms = d->rebuild_ms;
+  return MDGPU_OK;
+}
+
+int mdgpu_get_list_stats(const mdgpu_domain *d, int64_t *list_entries, int64_t *list_slots, int *ncells,
+                         int *ghosts) {
+  if (!d) return MDGPU_ERR_ARGUMENT;
+  if (list_entries) *list_entries = d->nbr_entries;
+  if (list_slots) *list_slots = d->nbr_slots;
+  if (ncells) *ncells = d->ncell;
+  if (ghosts) *ghosts = d->ng;
+  return MDGPU_OK;
+}
+
+}  // extern "C"

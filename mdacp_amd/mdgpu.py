@@ -1,0 +1,254 @@
+"""ctypes binding of include/mdgpu.h (libmdgpu.so, the gfx950 LJ domain).
+
+There is no CPU fallback: if the HIP library is not built, or no GPU is
+present, constructing a Domain raises.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libmdgpu.so")
+
+OK = 0
+ERR_TOO_FAST, ERR_INVALID_POSITION, ERR_INVALID_INDEX, ERR_CAPACITY, ERR_HIP, ERR_ARGUMENT = 1, 2, 3, 4, 5, 6
+FP64, MIXED = 0, 1
+NVE, NOSEHOOVER, LANGEVIN = 0, 1, 2
+
+# the symbols include/mdgpu.h declares
+EXPORTS = [
+    "mdgpu_create", "mdgpu_destroy", "mdgpu_last_error", "mdgpu_set_precision", "mdgpu_add_particles",
+    "mdgpu_particle_number", "mdgpu_get_particles", "mdgpu_set_momenta", "mdgpu_get_ghosts",
+    "mdgpu_set_thermostat", "mdgpu_set_seed", "mdgpu_get_zeta", "mdgpu_get_time", "mdgpu_make_pair_list",
+    "mdgpu_is_pair_list_expired", "mdgpu_update_position_half", "mdgpu_send_border_particles",
+    "mdgpu_calculate_force", "mdgpu_langevin", "mdgpu_heatbath_zeta", "mdgpu_heatbath_momenta",
+    "mdgpu_calculate", "mdgpu_observe", "mdgpu_change_scale", "mdgpu_get_pairs", "mdgpu_compute_kick",
+    "mdgpu_enable_timing", "mdgpu_get_timing", "mdgpu_get_list_stats",
+]
+
+
+class Params(ctypes.Structure):
+    _fields_ = [("L", ctypes.c_double * 3), ("dt", ctypes.c_double), ("rc", ctypes.c_double),
+                ("margin", ctypes.c_double), ("check_list_length", ctypes.c_int),
+                ("precision", ctypes.c_int), ("capacity", ctypes.c_int), ("device", ctypes.c_int)]
+
+
+class MDGPUError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"mdgpu error {code}: {msg}")
+        self.code = code
+
+
+_lib = None
+
+
+def load():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is not built; run `python -m mdacp_amd.build`")
+        L = ctypes.CDLL(LIB_PATH)
+        P, I, D = ctypes.c_void_p, ctypes.c_int, ctypes.c_double
+        PI, PD, PL = ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int64)
+        sig = {
+            "mdgpu_create": [ctypes.POINTER(Params), ctypes.POINTER(ctypes.c_void_p)],
+            "mdgpu_destroy": [P],
+            "mdgpu_set_precision": [P, I],
+            "mdgpu_add_particles": [P, I, P, P, P],
+            "mdgpu_particle_number": [P, PI, PI],
+            "mdgpu_get_particles": [P, P, P, P],
+            "mdgpu_set_momenta": [P, P],
+            "mdgpu_get_ghosts": [P, P, P, P],
+            "mdgpu_set_thermostat": [P, I, D, D, D],
+            "mdgpu_set_seed": [P, ctypes.c_uint64],
+            "mdgpu_get_zeta": [P, PD],
+            "mdgpu_get_time": [P, PD, ctypes.POINTER(ctypes.c_uint64)],
+            "mdgpu_make_pair_list": [P],
+            "mdgpu_is_pair_list_expired": [P, PI],
+            "mdgpu_update_position_half": [P],
+            "mdgpu_send_border_particles": [P],
+            "mdgpu_calculate_force": [P],
+            "mdgpu_langevin": [P],
+            "mdgpu_heatbath_zeta": [P, D],
+            "mdgpu_heatbath_momenta": [P],
+            "mdgpu_calculate": [P, I, PI],
+            "mdgpu_observe": [P, PD, PD, PD],
+            "mdgpu_change_scale": [P, D],
+            "mdgpu_get_pairs": [P, PL, P],
+            "mdgpu_compute_kick": [P, P],
+            "mdgpu_enable_timing": [P, I],
+            "mdgpu_get_timing": [P, PI, PD, PD, PI, PD],
+            "mdgpu_get_list_stats": [P, PL, PL, PI, PI],
+        }
+        for name, args in sig.items():
+            f = getattr(L, name)
+            f.argtypes = args
+            f.restype = ctypes.c_int
+        L.mdgpu_last_error.argtypes = [P]
+        L.mdgpu_last_error.restype = ctypes.c_char_p
+        _lib = L
+    return _lib
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data_as(ctypes.c_void_p)
+
+
+class Domain:
+    """One MDACP domain resident on the GPU (libmdgpu.so)."""
+
+    def __init__(self, L, dt=0.001, rc=2.5, margin=0.5, precision=FP64, capacity=0,
+                 check_list_length=500, device=0):
+        lib = load()
+        Lx, Ly, Lz = (float(L),) * 3 if np.isscalar(L) else (float(v) for v in L)
+        prm = Params((ctypes.c_double * 3)(Lx, Ly, Lz), dt, rc, margin, check_list_length, precision,
+                     int(capacity), device)
+        h = ctypes.c_void_p()
+        rc_ = lib.mdgpu_create(ctypes.byref(prm), ctypes.byref(h))
+        if rc_ != OK:
+            raise MDGPUError(rc_, "mdgpu_create failed")
+        self.h = h
+        self.L = np.array([Lx, Ly, Lz])
+        self.dt = dt
+
+    def close(self):
+        if getattr(self, "h", None):
+            load().mdgpu_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def _chk(self, code):
+        if code != OK:
+            raise MDGPUError(code, load().mdgpu_last_error(self.h).decode())
+
+    # ---------------------------------------------------------------- state
+    def add_particles(self, q, p=None, ids=None):
+        q = np.ascontiguousarray(q, dtype=np.float64)
+        n = len(q)
+        p = None if p is None else np.ascontiguousarray(p, dtype=np.float64)
+        ids = np.arange(n, dtype=np.int32) if ids is None else np.ascontiguousarray(ids, dtype=np.int32)
+        self._chk(load().mdgpu_add_particles(self.h, n, _ptr(q), _ptr(p), _ptr(ids)))
+
+    def counts(self):
+        pn, tn = ctypes.c_int(), ctypes.c_int()
+        self._chk(load().mdgpu_particle_number(self.h, ctypes.byref(pn), ctypes.byref(tn)))
+        return pn.value, tn.value
+
+    def particles(self):
+        pn, _ = self.counts()
+        q = np.zeros((pn, 3)); p = np.zeros((pn, 3)); t = np.zeros(pn, dtype=np.int32)
+        self._chk(load().mdgpu_get_particles(self.h, _ptr(q), _ptr(p), _ptr(t)))
+        return q, p, t
+
+    def particles_by_id(self):
+        q, p, t = self.particles()
+        o = np.argsort(t, kind="stable")
+        return q[o], p[o], t[o]
+
+    def set_momenta(self, p):
+        p = np.ascontiguousarray(p, dtype=np.float64)
+        self._chk(load().mdgpu_set_momenta(self.h, _ptr(p)))
+
+    def ghosts(self):
+        pn, tn = self.counts()
+        ng = tn - pn
+        src = np.zeros(ng, dtype=np.int32); code = np.zeros(ng, dtype=np.int32); q = np.zeros((ng, 3))
+        self._chk(load().mdgpu_get_ghosts(self.h, _ptr(src), _ptr(code), _ptr(q)))
+        return src, code, q
+
+    def set_precision(self, precision):
+        self._chk(load().mdgpu_set_precision(self.h, precision))
+
+    def set_thermostat(self, kind, T=1.0, gamma=0.1, tau=0.1):
+        self._chk(load().mdgpu_set_thermostat(self.h, kind, T, gamma, tau))
+
+    def set_seed(self, seed):
+        self._chk(load().mdgpu_set_seed(self.h, seed))
+
+    def zeta(self):
+        z = ctypes.c_double()
+        self._chk(load().mdgpu_get_zeta(self.h, ctypes.byref(z)))
+        return z.value
+
+    def time(self):
+        t, s = ctypes.c_double(), ctypes.c_uint64()
+        self._chk(load().mdgpu_get_time(self.h, ctypes.byref(t), ctypes.byref(s)))
+        return t.value, s.value
+
+    # ----------------------------------------------------------------- step
+    def make_pair_list(self):
+        self._chk(load().mdgpu_make_pair_list(self.h))
+
+    def is_pair_list_expired(self):
+        e = ctypes.c_int()
+        self._chk(load().mdgpu_is_pair_list_expired(self.h, ctypes.byref(e)))
+        return bool(e.value)
+
+    def update_position_half(self):
+        self._chk(load().mdgpu_update_position_half(self.h))
+
+    def send_border_particles(self):
+        self._chk(load().mdgpu_send_border_particles(self.h))
+
+    def calculate_force(self):
+        self._chk(load().mdgpu_calculate_force(self.h))
+
+    def langevin(self):
+        self._chk(load().mdgpu_langevin(self.h))
+
+    def calculate(self, nsteps=1):
+        r = ctypes.c_int()
+        self._chk(load().mdgpu_calculate(self.h, int(nsteps), ctypes.byref(r)))
+        return r.value
+
+    def change_scale(self, alpha):
+        self._chk(load().mdgpu_change_scale(self.h, alpha))
+        self.L = self.L * alpha
+
+    # ------------------------------------------------------------- observe
+    def observe_raw(self):
+        ke, pe, vir = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+        self._chk(load().mdgpu_observe(self.h, ctypes.byref(ke), ctypes.byref(pe), ctypes.byref(vir)))
+        return ke.value, pe.value, vir.value
+
+    def observables(self):
+        """T, P, E per MDManager::Temperature/Pressure/TotalEnergy (mdmanager.cc:389-429)."""
+        ke, pe, vir = self.observe_raw()
+        pn = float(self.counts()[0])
+        T = (ke / pn) / 1.5
+        V = self.L[0] * self.L[1] * self.L[2]
+        return dict(ke=ke, pe=pe, vir=vir, T=T, P=(T - vir / pn) * pn / V, E=ke / pn + pe / pn)
+
+    # -------------------------------------------------------------- export
+    def pairs(self):
+        n = ctypes.c_int64()
+        self._chk(load().mdgpu_get_pairs(self.h, ctypes.byref(n), None))
+        out = np.zeros((n.value, 3), dtype=np.int32)
+        self._chk(load().mdgpu_get_pairs(self.h, ctypes.byref(n), _ptr(out)))
+        return out
+
+    def kick(self):
+        """dp = F*dt from zero momenta, in particles() order."""
+        pn, _ = self.counts()
+        dp = np.zeros((pn, 3))
+        self._chk(load().mdgpu_compute_kick(self.h, _ptr(dp)))
+        return dp
+
+    def enable_timing(self, on=True):
+        self._chk(load().mdgpu_enable_timing(self.h, int(on)))
+
+    def timing(self):
+        fl, fm, tm, rb, rm = ctypes.c_int(), ctypes.c_double(), ctypes.c_double(), ctypes.c_int(), ctypes.c_double()
+        self._chk(load().mdgpu_get_timing(self.h, ctypes.byref(fl), ctypes.byref(fm), ctypes.byref(tm),
+                                          ctypes.byref(rb), ctypes.byref(rm)))
+        return dict(force_launches=fl.value, force_ms=fm.value, total_ms=tm.value, rebuilds=rb.value,
+                    rebuild_ms=rm.value)
+
+    def list_stats(self):
+        e, s, c, g = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int(), ctypes.c_int()
+        self._chk(load().mdgpu_get_list_stats(self.h, ctypes.byref(e), ctypes.byref(s), ctypes.byref(c),
+                                              ctypes.byref(g)))
+        return dict(entries=e.value, slots=s.value, ncell=c.value, ghosts=g.value)

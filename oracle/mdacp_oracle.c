@@ -93,6 +93,12 @@ static double mt_normal(mt19937 *g, normal_state *ns, double mean, double sd) {
   return ret * sd + mean;
 }
 
+void ora_mt_raw(uint32_t seed, int n, uint32_t *out) {
+  mt19937 g;
+  mt_seed(&g, seed);
+  for (int i = 0; i < n; i++) out[i] = mt_next(&g);
+}
+
 void ora_mt_uniform(uint32_t seed, int n, double *out) {
   mt19937 g;
   mt_seed(&g, seed);

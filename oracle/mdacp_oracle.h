@@ -73,6 +73,7 @@ double ora_buffer_length(const ora_sys *s);
 int ora_lifetime(const ora_sys *s);
 
 /* standalone pieces of the reference's third-party arithmetic */
+void ora_mt_raw(uint32_t seed, int n, uint32_t *out);                /* std::mt19937 outputs */
 void ora_mt_uniform(uint32_t seed, int n, double *out);            /* mt19937 + uniform_real(0,1) */
 void ora_mt_normal(uint32_t seed, double sd, int n, double *out);   /* normal_distribution(0,sd) */
 void ora_philox4x32(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);

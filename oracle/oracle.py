@@ -63,6 +63,7 @@ def lib():
             "ora_get_cells": (None, [P, P]),
             "ora_buffer_length": (D, [P]),
             "ora_lifetime": (I, [P]),
+            "ora_mt_raw": (None, [ctypes.c_uint32, I, P]),
             "ora_mt_uniform": (None, [ctypes.c_uint32, I, P]),
             "ora_mt_normal": (None, [ctypes.c_uint32, D, I, P]),
             "ora_philox4x32": (None, [P, P, P]),

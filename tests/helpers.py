@@ -1,0 +1,74 @@
+"""Shared helpers for the parity tests (numpy restatements used as checks)."""
+import hashlib
+import json
+import os
+
+import numpy as np
+
+from oracle import oracle as O
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def load_golden(name):
+    with open(os.path.join(GOLDEN, name + ".json")) as f:
+        return json.load(f)
+
+
+def pair_hash(pairs3):
+    keys = O.canonical_pairs(pairs3)
+    return hashlib.sha256(keys.tobytes()).hexdigest(), int(len(keys))
+
+
+def state_hash_by_id(q, p, ids):
+    o = np.argsort(ids, kind="stable")
+    h = hashlib.sha256()
+    h.update(np.ascontiguousarray(q[o]).tobytes())
+    h.update(np.ascontiguousarray(p[o]).tobytes())
+    return h.hexdigest()
+
+
+def shift_vec(code, L):
+    code = np.asarray(code)
+    s = np.stack([code % 3 - 1, (code // 3) % 3 - 1, code // 9 - 1], axis=-1).astype(np.float64)
+    return s * np.asarray(L)
+
+
+def image_positions(q_by_id, b, code, L):
+    """fl(q_b + shift*L) per component, as the reference's halo computes it."""
+    s = shift_vec(code, L)
+    return q_by_id[b] + s  # elementwise double add: q + (+-L or 0.0)
+
+
+def pair_force_sums(q_by_id, pairs3, L, rc, dt, C2):
+    """Per-particle sum of |f_ij| * dt over listed pairs within rc (tolerance scale)."""
+    a, b, code = pairs3[:, 0], pairs3[:, 1], pairs3[:, 2]
+    xb = image_positions(q_by_id, b, code, L)
+    d = xb - q_by_id[a]
+    r2 = np.einsum("ij,ij->i", d, d)
+    inside = r2 <= rc * rc
+    r6 = r2 ** 3
+    df = ((24.0 * r6 - 48.0) / (r6 * r6 * r2) + 8.0 * C2) * dt
+    mag = np.abs(df) * np.sqrt(r2) * inside
+    n = len(q_by_id)
+    S = np.bincount(a, weights=mag, minlength=n)
+    own = code == 13
+    S += np.bincount(b[own], weights=mag[own], minlength=n)
+    return S
+
+
+def lj_C(rc):
+    s2 = 1.0 / (rc * rc)
+    s6 = s2 * s2 * s2
+    s8 = s6 * s2
+    s12 = s6 * s6
+    s14 = s12 * s2
+    C2 = 6.0 * s14 - 3.0 * s8
+    C0 = -s12 + s6 - C2 / s2
+    return C0, C2
+
+
+def oracle_system(L, x, p, ids, dt, rc, margin):
+    s = O.OracleSystem(L, dt, rc, margin)
+    s.add(x, p, ids)
+    return s
