@@ -68,6 +68,12 @@ def lj_C(rc):
     return C0, C2
 
 
+def wrap_like_reference(q, L):
+    """Variables::AdjustPeriodicBoundary (variables.cc:172-182): one +-L per component."""
+    L = np.broadcast_to(np.asarray(L, dtype=np.float64), q.shape)
+    return np.where(q < 0.0, q + L, np.where(q > L, q - L, q))
+
+
 def oracle_system(L, x, p, ids, dt, rc, margin):
     s = O.OracleSystem(L, dt, rc, margin)
     s.add(x, p, ids)
