@@ -129,6 +129,15 @@ int mdgpu_enable_timing(mdgpu_domain *d, int on);
 int mdgpu_get_timing(const mdgpu_domain *d, int *force_launches, double *force_ms, double *total_ms,
                      int *rebuilds, double *rebuild_ms);
 
+/* tuning knobs (no counterpart in the reference; results do not depend on
+   them beyond floating-point summation order) */
+#define MDGPU_OPT_UNROLL 1   /* partner entries per batch in the force kernels: 4 or 8 */
+#define MDGPU_OPT_KAHAN 2    /* mixed precision: 1 = compensated FP32 sums, 0 = plain */
+#define MDGPU_OPT_TILE_WAVES 3 /* 64-row wavefronts per force tile: 1, 2 or 4 (rebuilds the list) */
+#define MDGPU_OPT_LDS_KB 4   /* LDS budget for one tile's staged partners, KiB (8..device limit) */
+int mdgpu_set_option(mdgpu_domain *d, int option, int value);
+int mdgpu_get_option(const mdgpu_domain *d, int option, int *value);
+
 /* sizes of the device-resident structures (for roofline accounting) */
 int mdgpu_get_list_stats(const mdgpu_domain *d, int64_t *list_entries, int64_t *list_slots,
                          int *ncells, int *ghosts);

@@ -8,10 +8,9 @@ namespace mdgpu {
 constexpr int kWave = 64;      // CDNA wavefront width
 constexpr int kBlock = 256;    // 4 waves per workgroup
 constexpr int kXcds = 8;       // MI355X: 8 XCDs, workgroups dispatched round-robin
-constexpr uint32_t kIndexBits = 26;                 // neighbour index field
-constexpr uint32_t kIndexMask = (1u << kIndexBits) - 1u;
-constexpr int kMaxParticles = (1 << kIndexBits) - 2;  // owned + ghosts + sentinel
+constexpr int kMaxParticles = (1 << 26) - 2;        // owned + ghosts + sentinel per domain
 constexpr uint32_t kZeroOffset = 0x15u;             // cell offset (0,0,0): 01 01 01
+constexpr int kSliceRound = 8;                      // slice lengths are multiples of this
 
 // Error bits raised by kernels; read back at the existing sync points.
 constexpr int kErrInvalidPosition = 1;
@@ -37,9 +36,33 @@ __device__ __forceinline__ double wave_sum(double v) {
   return v;
 }
 
-// Bits of a non-negative double order like the value.
-__device__ __forceinline__ void atomic_max_nonneg(unsigned long long *addr, double v) {
-  atomicMax(addr, (unsigned long long)__double_as_longlong(v));
+// Maximum over the workgroup, written to partial[blockIdx.x] by thread 0
+// (no same-address atomics: those serialise in L2 across ~10^4 workgroups).
+// Every thread of the workgroup must call it.
+__device__ __forceinline__ void block_max_to_partial(double v, double *__restrict__ partial) {
+  __shared__ double wm[kBlock / kWave];
+  v = wave_max(v);
+  if ((threadIdx.x % kWave) == 0) wm[threadIdx.x / kWave] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double m = wm[0];
+    for (int k = 1; k < (int)(blockDim.x / kWave); k++) m = fmax(m, wm[k]);
+    partial[blockIdx.x] = m;
+  }
+}
+
+// One workgroup: max of n partials, stored as the bits of a double.
+__global__ void k_reduce_max(const double *__restrict__ partial, int n, unsigned long long *__restrict__ out) {
+  __shared__ double red[kBlock];
+  double m = 0.0;
+  for (int k = threadIdx.x; k < n; k += blockDim.x) m = fmax(m, partial[k]);
+  red[threadIdx.x] = m;
+  __syncthreads();
+  for (int o = blockDim.x / 2; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + o]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *out = (unsigned long long)__double_as_longlong(red[0]);
 }
 
 // Image displacement of a shift code (sx+1)+3(sy+1)+9(sz+1): one +-L or 0.0

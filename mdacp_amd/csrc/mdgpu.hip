@@ -18,6 +18,7 @@
 #include "build_kernels.hpp"
 #include "scan.hpp"
 #include "step_kernels.hpp"
+#include "force_kernels.hpp"
 
 using namespace mdgpu;
 
@@ -35,7 +36,12 @@ struct mdgpu_domain {
   hipStream_t stream = nullptr;
   std::string err;
   int precision = MDGPU_FP64;
-  int kahan = 1;
+  int kahan = 1;   // compensated FP32 sums in the mixed kernel
+  int unroll = 8;      // partner entries per batch in the force kernels (4 or 8)
+  int tile_waves = 2;  // wavefronts (64 rows each) per tile / workgroup: 1, 2 or 4
+  int lds_kb = 60;     // LDS budget of one tile's staged partners, KiB
+  int lds_max_kb = 60; // device limit per workgroup less the kernels' static LDS
+  int ntile = 0, max_staged = 0;
 
   // SimulationInfo / MDRect / MeshList geometry
   double L[3], rs[3], re[3];
@@ -69,7 +75,10 @@ struct mdgpu_domain {
   int *gsrc = nullptr, *gcode = nullptr;
   int *c_ocount = nullptr, *c_ostart = nullptr, *c_gcount = nullptr, *c_gstart = nullptr;
   double *org = nullptr;  // origins of the cell columns, x then y then z
-  uint32_t *nbr = nullptr;
+  uint16_t *nbr = nullptr;   // staged partner index per list slot
+  uint8_t *ncode = nullptr;  // partner cell offset per list slot
+  int *tstart = nullptr, *tbase = nullptr;
+  int tile_cap = 0;
   long long *sbase = nullptr;
   int *slen = nullptr, *row_len = nullptr;
   // build scratch
@@ -77,7 +86,7 @@ struct mdgpu_domain {
   int *t_gsrc = nullptr, *t_gcode = nullptr, *t_gcell = nullptr;
   long long *key = nullptr, *scan_scratch = nullptr, *slots64 = nullptr;
   long long scan_scratch_n = 0;
-  double *disp = nullptr, *partial = nullptr;
+  double *disp = nullptr, *partial = nullptr, *maxpart = nullptr;
   int partial_n = 0;
   int *cand = nullptr;
   // small device scalars: [0] vmax2 bits, [1] dmax2 bits (u64); ints: err, count, flag
@@ -228,6 +237,7 @@ int ensure_owned(mdgpu_domain *d, int need) {
   CHK(dalloc(d, &d->cell, tc_new));
   CHK(dalloc(d, &d->cell_alt, tc_new));
   CHK(dalloc(d, &d->disp, tc_new));
+  CHK(dalloc(d, &d->maxpart, nblk((long long)tc_new, kWave)));  // one per workgroup of >= 64 threads
   CHK(dalloc(d, &d->mom_alt, nc));
   CHK(dalloc(d, &d->id_alt, nc));
   CHK(dalloc(d, &d->slot, tc_new));
@@ -261,6 +271,7 @@ int ensure_ghosts(mdgpu_domain *d, int need) {
   CHK(dgrow_copy(d, &d->cell, tc_old, d->pn, tc_new));
   CHK(dalloc(d, &d->cell_alt, tc_new));
   CHK(dalloc(d, &d->disp, tc_new));
+  CHK(dalloc(d, &d->maxpart, nblk((long long)tc_new, kWave)));  // one per workgroup of >= 64 threads
   CHK(dalloc(d, &d->slot, tc_new));
   CHK(dalloc(d, &d->perm, tc_new));
   CHK(dalloc(d, &d->key, tc_new));
@@ -395,9 +406,34 @@ int make_pair_list(mdgpu_domain *d) {
   // 3. PairList::Init: snapshot of owned + ghost positions
   if (d->tn > 0)
     hipLaunchKernelGGL(k_copy_pos, dim3(nblk(d->tn)), dim3(kBlock), 0, st, d->pos, d->pos_old, d->tn);
-  // 4. MeshList::MakeList: full neighbour list in sliced-ELL layout
+  // 4. MeshList::MakeList: full neighbour list in sliced-ELL layout, partners
+  //    as indices into their tile's staged ranges
   const CellTables T{d->c_ostart, d->c_ocount, d->c_gstart, d->c_gcount};
   d->nslice = (pn + kWave - 1) / kWave;
+  const int rows_per_tile = kWave * d->tile_waves;
+  d->ntile = (pn + rows_per_tile - 1) / rows_per_tile;
+  if (d->ntile > d->tile_cap) {
+    const int tc = d->ntile + d->ntile / 4 + 16;
+    CHK(dalloc(d, &d->tstart, (size_t)tc * kTileRanges));
+    CHK(dalloc(d, &d->tbase, (size_t)tc * (kTileRanges + 1)));
+    d->tile_cap = tc;
+  }
+  d->max_staged = 0;
+  if (pn > 0) {
+    const unsigned nb = nblk(d->ntile);
+    hipLaunchKernelGGL(k_tile_ranges, dim3(nb), dim3(kBlock), 0, st, d->cell, pn, rows_per_tile, d->ntile, g, T,
+                       d->tstart, d->tbase, d->maxpart);
+    hipLaunchKernelGGL(k_reduce_max, dim3(1), dim3(kBlock), 0, st, d->maxpart, (int)nb, d->u64 + 2);
+    HIPCHK(d, hipMemcpyAsync(d->h_u64 + 2, d->u64 + 2, sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
+    CHK(sync(d));
+    double m;
+    std::memcpy(&m, d->h_u64 + 2, sizeof(m));
+    d->max_staged = (int)m;
+    if (d->max_staged > 65534) {
+      d->err = "a tile's partner set exceeds 16-bit indexing (extreme local density)";
+      return MDGPU_ERR_CAPACITY;
+    }
+  }
   long long slots = 0;
   if (pn > 0) {
     hipLaunchKernelGGL(k_count_rows, dim3(nblk(pn)), dim3(kBlock), 0, st, d->pos, d->cell, pn, g, T, d->S2,
@@ -416,14 +452,15 @@ int make_pair_list(mdgpu_domain *d) {
   if (slots > d->nbr_cap) {
     const long long nc = slots + slots / 8 + 1024;
     CHK(dalloc(d, &d->nbr, (size_t)nc));
+    CHK(dalloc(d, &d->ncode, (size_t)nc));
     d->nbr_cap = nc;
   }
   d->nbr_slots = slots;
   if (pn > 0) {
-    const uint32_t pad = (uint32_t)d->tn | (kZeroOffset << kIndexBits);
     hipLaunchKernelGGL(k_fill_rows, dim3(nblk(pn)), dim3(kBlock), 0, st, d->pos, d->cell, pn, g, T, d->S2,
-                       d->sbase, d->slen, d->nbr, pad, d->i32);
-    hipLaunchKernelGGL(k_fill_tail, dim3(1), dim3(kWave), 0, st, pn, d->nslice, d->sbase, d->slen, d->nbr, pad);
+                       d->sbase, d->slen, rows_per_tile, d->tstart, d->tbase, d->nbr, d->ncode, d->i32);
+    hipLaunchKernelGGL(k_fill_tail, dim3(1), dim3(kWave), 0, st, pn, d->nslice, rows_per_tile, d->sbase, d->slen,
+                       d->tbase, d->nbr, d->ncode);
   }
   CHK(check_launch(d, "neighbour list"));
   CHK(read_err_flags(d));
@@ -436,9 +473,9 @@ int make_pair_list(mdgpu_domain *d) {
 
 int max_p2(mdgpu_domain *d, double *v2) {
   if (!d->vmax_valid) {
-    HIPCHK(d, hipMemsetAsync(d->u64, 0, sizeof(unsigned long long), d->stream));
-    if (d->pn > 0)
-      hipLaunchKernelGGL(k_max_p2, dim3(nblk(d->pn)), dim3(kBlock), 0, d->stream, d->mom, d->pn, d->u64);
+    const unsigned nb = nblk(d->pn);
+    hipLaunchKernelGGL(k_max_p2, dim3(nb), dim3(kBlock), 0, d->stream, d->mom, d->pn, d->maxpart);
+    hipLaunchKernelGGL(k_reduce_max, dim3(1), dim3(kBlock), 0, d->stream, d->maxpart, (int)nb, d->u64);
   }
   HIPCHK(d, hipMemcpyAsync(d->h_u64, d->u64, sizeof(unsigned long long), hipMemcpyDeviceToHost, d->stream));
   CHK(sync(d));
@@ -454,10 +491,9 @@ int check_by_displacement(mdgpu_domain *d, int *expired) {
   hipStream_t st = d->stream;
   const int tn = d->tn;
   *expired = 0;
-  HIPCHK(d, hipMemsetAsync(d->u64 + 1, 0, sizeof(unsigned long long), st));
-  if (tn > 0)
-    hipLaunchKernelGGL(k_displacement, dim3(nblk(tn)), dim3(kBlock), 0, st, d->pos, d->pos_old, tn, d->disp,
-                       d->u64 + 1);
+  hipLaunchKernelGGL(k_displacement, dim3(nblk(tn)), dim3(kBlock), 0, st, d->pos, d->pos_old, tn, d->disp,
+                     d->maxpart);
+  hipLaunchKernelGGL(k_reduce_max, dim3(1), dim3(kBlock), 0, st, d->maxpart, (int)nblk(tn), d->u64 + 1);
   HIPCHK(d, hipMemcpyAsync(d->h_u64 + 1, d->u64 + 1, sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
   CHK(sync(d));
   double dr2_max;
@@ -545,29 +581,82 @@ LangevinConsts lconsts(const mdgpu_domain *d) {
 
 // One pair-force launch.  `src`/`rel_src` are the positions the forces are
 // evaluated at; `pos_out` receives the drifted (or copied) owned positions.
+// The list as the tile kernels read it; lds_cap = staged entries per tile
+// that fit the LDS budget (tiles with more read their partners from global
+// memory).
+TileList tile_list(const mdgpu_domain *d, int bytes_per_entry) {
+  TileList tl;
+  tl.nbr = d->nbr;
+  tl.code = d->ncode;
+  tl.sbase = d->sbase;
+  tl.slen = d->slen;
+  tl.tstart = d->tstart;
+  tl.tbase = d->tbase;
+  tl.nslice = d->nslice;
+  const int budget = d->lds_kb * 1024 / bytes_per_entry;
+  tl.lds_cap = std::max(1, std::min(d->max_staged + 1, budget));
+  return tl;
+}
+
+template <int W, int U>
+void launch_fp64(mdgpu_domain *d, const double4 *src, double4 *pos_out, const TileList &tl, int epi,
+                 const ForceConsts &fc, const LangevinConsts &lc, double *vb) {
+  const size_t shm = (size_t)tl.lds_cap * 3 * sizeof(double);
+  hipLaunchKernelGGL((k_force_tile_fp64<W, U>), dim3(d->ntile), dim3(kWave * W), shm, d->stream, src, d->mom,
+                     pos_out, tl, d->pn, d->tn, fc, epi, lc, d->id, vb);
+}
+
+template <int W, int U, bool K>
+void launch_mixed(mdgpu_domain *d, const double4 *src, const float4 *rel_src, double4 *pos_out,
+                  const TileList &tl, int epi, const ForceConstsF &f32, double dt2, const LangevinConsts &lc,
+                  double *vb) {
+  const size_t shm = (size_t)tl.lds_cap * sizeof(float4);
+  hipLaunchKernelGGL((k_force_tile_mixed<W, U, K>), dim3(d->ntile), dim3(kWave * W), shm, d->stream, rel_src, src,
+                     d->mom, pos_out, tl, d->pn, d->tn, f32, dt2, epi, lc, d->id, vb);
+}
+
+template <int W>
+void launch_force_w(mdgpu_domain *d, const double4 *src, const float4 *rel_src, double4 *pos_out, int epi,
+                    const ForceConsts &fc, const ForceConstsF &f32, const LangevinConsts &lc, double *vb) {
+  if (d->precision == MDGPU_FP64) {
+    const TileList tl = tile_list(d, 3 * sizeof(double));
+    if (d->unroll == 4) launch_fp64<W, 4>(d, src, pos_out, tl, epi, fc, lc, vb);
+    else launch_fp64<W, 8>(d, src, pos_out, tl, epi, fc, lc, vb);
+  } else {
+    const TileList tl = tile_list(d, sizeof(float4));
+    if (d->kahan) {
+      if (d->unroll == 4) launch_mixed<W, 4, true>(d, src, rel_src, pos_out, tl, epi, f32, fc.dt2, lc, vb);
+      else launch_mixed<W, 8, true>(d, src, rel_src, pos_out, tl, epi, f32, fc.dt2, lc, vb);
+    } else {
+      if (d->unroll == 4) launch_mixed<W, 4, false>(d, src, rel_src, pos_out, tl, epi, f32, fc.dt2, lc, vb);
+      else launch_mixed<W, 8, false>(d, src, rel_src, pos_out, tl, epi, f32, fc.dt2, lc, vb);
+    }
+  }
+}
+
+// One pair-force launch.  `src`/`rel_src` are the positions the forces are
+// evaluated at; `pos_out` receives the drifted (or copied) owned positions.
 int launch_force(mdgpu_domain *d, const double4 *src, const float4 *rel_src, double4 *pos_out, int epilogue,
                  bool want_vmax) {
   if (d->pn == 0) return MDGPU_OK;
   hipStream_t st = d->stream;
   const ForceConsts fc = fconsts(d);
+  const ForceConstsF f32 = fconsts32(d);
   const LangevinConsts lc = lconsts(d);
-  unsigned long long *vb = want_vmax ? d->u64 : nullptr;
+  double *vb = want_vmax ? d->maxpart : nullptr;
   const int ev0 = d->force_launches * 2;
   if (d->timing && ev0 + 1 < (int)d->ev.size()) HIPCHK(d, hipEventRecord(d->ev[ev0], st));
-  if (d->precision == MDGPU_FP64) {
-    hipLaunchKernelGGL(k_force_fp64, dim3(nblk(d->pn)), dim3(kBlock), 0, st, src, d->mom, pos_out, d->nbr,
-                       d->sbase, d->slen, d->pn, fc, epilogue, lc, d->id, vb);
-  } else if (d->kahan) {
-    hipLaunchKernelGGL((k_force_mixed<true>), dim3(nblk(d->pn)), dim3(kBlock), 0, st, rel_src, src, d->mom,
-                       pos_out, d->nbr, d->sbase, d->slen, d->pn, fconsts32(d), fc.dt2, epilogue, lc, d->id, vb);
-  } else {
-    hipLaunchKernelGGL((k_force_mixed<false>), dim3(nblk(d->pn)), dim3(kBlock), 0, st, rel_src, src, d->mom,
-                       pos_out, d->nbr, d->sbase, d->slen, d->pn, fconsts32(d), fc.dt2, epilogue, lc, d->id, vb);
+  switch (d->tile_waves) {
+    case 1: launch_force_w<1>(d, src, rel_src, pos_out, epilogue, fc, f32, lc, vb); break;
+    case 4: launch_force_w<4>(d, src, rel_src, pos_out, epilogue, fc, f32, lc, vb); break;
+    default: launch_force_w<2>(d, src, rel_src, pos_out, epilogue, fc, f32, lc, vb); break;
   }
   if (d->timing && ev0 + 1 < (int)d->ev.size()) {
     HIPCHK(d, hipEventRecord(d->ev[ev0 + 1], st));
     d->force_launches++;
   }
+  if (want_vmax)
+    hipLaunchKernelGGL(k_reduce_max, dim3(1), dim3(kBlock), 0, st, d->maxpart, d->ntile, d->u64);
   return check_launch(d, "force");
 }
 
@@ -677,11 +766,11 @@ int calculate_step(mdgpu_domain *d) {
       if (mixed)
         hipLaunchKernelGGL((k_drift_refresh<true>), dim3(nblk(tot)), dim3(kBlock), 0, st, d->pos, d->mom,
                            d->pos_tmp, d->gsrc, d->gcode, d->pn, d->ng, d->dt * 0.5, d->L[0], d->L[1], d->L[2],
-                           d->cell, origins_of(d), d->rel_tmp, d->u64);
+                           d->cell, origins_of(d), d->rel_tmp);
       else
         hipLaunchKernelGGL((k_drift_refresh<false>), dim3(nblk(tot)), dim3(kBlock), 0, st, d->pos, d->mom,
                            d->pos_tmp, d->gsrc, d->gcode, d->pn, d->ng, d->dt * 0.5, d->L[0], d->L[1], d->L[2],
-                           d->cell, origins_of(d), d->rel_tmp, d->u64);
+                           d->cell, origins_of(d), d->rel_tmp);
       CHK(check_launch(d, "drift+refresh"));
     }
     const int epi = d->thermostat == MDGPU_LANGEVIN ? kKickLangevin : kKickDrift;
@@ -712,6 +801,14 @@ int mdgpu_create(const mdgpu_params *prm, mdgpu_domain **out) {
     // single domain of a 1x1x1 grid: s = L*0, e = s + L (mdunit.cc:96-101)
     d->rs[k] = prm->L[k] * 0.0;
     d->re[k] = d->rs[k] + prm->L[k];
+  }
+  {
+    int shm = 0;
+    if (hipDeviceGetAttribute(&shm, hipDeviceAttributeMaxSharedMemoryPerBlock, prm->device) == hipSuccess &&
+        shm > 8 * 1024) {
+      d->lds_max_kb = shm / 1024 - 4;
+      d->lds_kb = std::min(d->lds_kb, d->lds_max_kb);
+    }
   }
   d->dt = prm->dt;
   d->rc = prm->rc;
@@ -758,8 +855,8 @@ int mdgpu_destroy(mdgpu_domain *d) {
   if (d->stream) hipStreamSynchronize(d->stream);
   void *bufs[] = {d->pos, d->pos_tmp, d->pos_old, d->mom, d->mom_alt, d->rel, d->rel_tmp, d->id, d->id_alt,
                   d->cell, d->cell_alt, d->gsrc, d->gcode, d->c_ocount, d->c_ostart, d->c_gcount, d->c_gstart,
-                  d->org, d->nbr, d->sbase, d->slen, d->row_len, d->slot, d->perm, d->nimg, d->goff, d->t_gsrc,
-                  d->t_gcode, d->t_gcell, d->key, d->scan_scratch, d->slots64, d->disp, d->partial, d->cand,
+                  d->org, d->nbr, d->ncode, d->tstart, d->tbase, d->sbase, d->slen, d->row_len, d->slot, d->perm, d->nimg, d->goff, d->t_gsrc,
+                  d->t_gcode, d->t_gcell, d->key, d->scan_scratch, d->slots64, d->disp, d->partial, d->maxpart, d->cand,
                   d->u64, d->i64, d->i32, d->d3};
   for (void *b : bufs)
     if (b) hipFree(b);
@@ -1000,13 +1097,27 @@ int mdgpu_observe(mdgpu_domain *d, double *ke, double *pe, double *virial) {
     d->err = "pair list not built";
     return MDGPU_ERR_ARGUMENT;
   }
-  const unsigned nb = nblk(d->pn);
-  CHK(ensure_partial(d, (int)nb));
+  const int nb = d->ntile;
+  CHK(ensure_partial(d, std::max(1, nb)));
   double out[3] = {0.0, 0.0, 0.0};
   if (d->pn > 0) {
-    hipLaunchKernelGGL(k_observe, dim3(nb), dim3(kBlock), 0, d->stream, d->pos, d->mom, d->nbr, d->sbase, d->slen,
-                       d->pn, d->CL2, d->C2, d->C0, d->partial);
-    hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(kBlock), 0, d->stream, d->partial, (int)nb, 3, d->d3);
+    const TileList tl = tile_list(d, 3 * sizeof(double));
+    const size_t shm = (size_t)tl.lds_cap * 3 * sizeof(double);
+    switch (d->tile_waves) {
+      case 1:
+        hipLaunchKernelGGL(k_observe_tile<1>, dim3(nb), dim3(kWave), shm, d->stream, d->pos, d->mom, tl, d->pn, d->tn,
+                           d->CL2, d->C2, d->C0, d->partial);
+        break;
+      case 4:
+        hipLaunchKernelGGL(k_observe_tile<4>, dim3(nb), dim3(kWave * 4), shm, d->stream, d->pos, d->mom, tl, d->pn,
+                           d->tn, d->CL2, d->C2, d->C0, d->partial);
+        break;
+      default:
+        hipLaunchKernelGGL(k_observe_tile<2>, dim3(nb), dim3(kWave * 2), shm, d->stream, d->pos, d->mom, tl, d->pn,
+                           d->tn, d->CL2, d->C2, d->C0, d->partial);
+        break;
+    }
+    hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(kBlock), 0, d->stream, d->partial, nb, 3, d->d3);
     CHK(check_launch(d, "observe"));
     HIPCHK(d, hipMemcpyAsync(d->h_d3, d->d3, 3 * sizeof(double), hipMemcpyDeviceToHost, d->stream));
     CHK(sync(d));
@@ -1036,11 +1147,16 @@ int mdgpu_change_scale(mdgpu_domain *d, double alpha) {
 int mdgpu_get_pairs(mdgpu_domain *d, int64_t *npairs, int32_t *out3) {
   if (!d || !npairs) return MDGPU_ERR_ARGUMENT;
   const int pn = d->pn, ng = d->ng;
-  std::vector<uint32_t> nbr((size_t)d->nbr_slots);
+  std::vector<uint16_t> nbr((size_t)d->nbr_slots);
   std::vector<long long> sb(d->nslice);
   std::vector<int> sl(d->nslice), ids(pn), gs(ng), gc(ng);
+  std::vector<int> ts((size_t)d->ntile * kTileRanges), tb((size_t)d->ntile * (kTileRanges + 1));
   if (d->nbr_slots)
-    HIPCHK(d, hipMemcpyAsync(nbr.data(), d->nbr, nbr.size() * 4, hipMemcpyDeviceToHost, d->stream));
+    HIPCHK(d, hipMemcpyAsync(nbr.data(), d->nbr, nbr.size() * 2, hipMemcpyDeviceToHost, d->stream));
+  if (d->ntile) {
+    HIPCHK(d, hipMemcpyAsync(ts.data(), d->tstart, ts.size() * 4, hipMemcpyDeviceToHost, d->stream));
+    HIPCHK(d, hipMemcpyAsync(tb.data(), d->tbase, tb.size() * 4, hipMemcpyDeviceToHost, d->stream));
+  }
   if (d->nslice) {
     HIPCHK(d, hipMemcpyAsync(sb.data(), d->sbase, d->nslice * 8, hipMemcpyDeviceToHost, d->stream));
     HIPCHK(d, hipMemcpyAsync(sl.data(), d->slen, d->nslice * 4, hipMemcpyDeviceToHost, d->stream));
@@ -1052,11 +1168,17 @@ int mdgpu_get_pairs(mdgpu_domain *d, int64_t *npairs, int32_t *out3) {
   }
   CHK(sync(d));
   int64_t cnt = 0;
+  const int rows_per_tile = kWave * d->tile_waves;
   for (int i = 0; i < pn; i++) {
-    const int s = i / kWave, lane = i % kWave;
+    const int s = i / kWave, lane = i % kWave, t = i / rows_per_tile;
+    const int *tst = ts.data() + (size_t)t * kTileRanges;
+    const int *tbs = tb.data() + (size_t)t * (kTileRanges + 1);
     for (int k = 0; k < sl[s]; k++) {
-      const int j = (int)(nbr[sb[s] + (long long)k * kWave + lane] & kIndexMask);
-      if (j >= d->tn) continue;
+      const int e = nbr[sb[s] + (long long)k * kWave + lane];
+      if (e >= tbs[kTileRanges]) continue;  // padding
+      int r = 0;
+      while (r + 1 < kTileRanges && tbs[r + 1] <= e) r++;
+      const int j = tst[r] + (e - tbs[r]);
       int a = ids[i], b, code = 13;
       if (j < pn) {
         b = ids[j];
@@ -1115,6 +1237,42 @@ int mdgpu_get_timing(const mdgpu_domain *d, int *force_launches, double *force_m
   if (rebuilds) *rebuilds = d->rebuild_count;
   if (rebuild_ms) *rebuild_ms = d->rebuild_ms;
   return MDGPU_OK;
+}
+
+int mdgpu_set_option(mdgpu_domain *d, int option, int value) {
+  if (!d) return MDGPU_ERR_ARGUMENT;
+  switch (option) {
+    case MDGPU_OPT_UNROLL:
+      if (value != 4 && value != 8) return MDGPU_ERR_ARGUMENT;
+      d->unroll = value;
+      return MDGPU_OK;
+    case MDGPU_OPT_KAHAN:
+      d->kahan = value != 0;
+      return MDGPU_OK;
+    case MDGPU_OPT_TILE_WAVES:
+      if (value != 1 && value != 2 && value != 4) return MDGPU_ERR_ARGUMENT;
+      if (value != d->tile_waves) {
+        d->tile_waves = value;
+        d->initialized = false;  // tile tables and staged indices depend on it
+      }
+      return MDGPU_OK;
+    case MDGPU_OPT_LDS_KB:
+      if (value < 8 || value > d->lds_max_kb) return MDGPU_ERR_ARGUMENT;
+      d->lds_kb = value;
+      return MDGPU_OK;
+  }
+  return MDGPU_ERR_ARGUMENT;
+}
+
+int mdgpu_get_option(const mdgpu_domain *d, int option, int *value) {
+  if (!d || !value) return MDGPU_ERR_ARGUMENT;
+  switch (option) {
+    case MDGPU_OPT_UNROLL: *value = d->unroll; return MDGPU_OK;
+    case MDGPU_OPT_KAHAN: *value = d->kahan; return MDGPU_OK;
+    case MDGPU_OPT_TILE_WAVES: *value = d->tile_waves; return MDGPU_OK;
+    case MDGPU_OPT_LDS_KB: *value = d->lds_kb; return MDGPU_OK;
+  }
+  return MDGPU_ERR_ARGUMENT;
 }
 
 int mdgpu_get_list_stats(const mdgpu_domain *d, int64_t *list_entries, int64_t *list_slots, int *ncells,

@@ -1,21 +1,14 @@
-// step_kernels.hpp -- per-step kernels on gfx950: half drift, halo refresh,
-// the smoothed-LJ pair-force kick (FP64 and mixed FP32), Langevin noise,
-// Nose-Hoover scaling, observers and the Verlet-list expiry test.
+// step_kernels.hpp -- per-step kernels on gfx950 other than the pair loops:
+// half drift, halo refresh, cell-relative FP32 coordinates, Langevin noise,
+// Nose-Hoover scaling, kinetic energy and the Verlet-list expiry test.
 //
 // Reference semantics (kaityo256/mdacp):
 //   UpdatePositionHalf   fcalculator.cc:21-32   q += p*(dt*0.5)
 //   SendBorderParticles  mdmanager.cc:282-348, mdunit.cc:214-269
-//   CalculateForceNext   fcalculator.cc:228-290 (kick p += F*dt, F from the
-//                        potential 4(r^-12 - r^-6 + C2 r^2 + C0), variables.cc:130-142)
 //   Langevin             fcalculator.cc:764-778
 //   HeatbathMomenta      fcalculator.cc:751-762
-//   observers            observer.cc:4-79
+//   KineticEnergyObserver observer.cc:4-15
 //   expiry               pairlist.cc:58-154
-//
-// The force kernels read a full neighbour list (each owned row lists every
-// owned or ghost partner within SL), so no momentum scatter and no atomics:
-// results are bit-reproducible run to run.  Where the reference applies
-// Newton's third law inside a half list, this computes both sides.
 #pragma once
 #include "device_common.hpp"
 
@@ -86,9 +79,8 @@ __global__ void k_drift_refresh(double4 *__restrict__ pos, const double4 *__rest
                                 double4 *__restrict__ tmp, const int *__restrict__ gsrc,
                                 const int *__restrict__ gcode, int pn, int ng, double dt2, double Lx,
                                 double Ly, double Lz, const int *__restrict__ cell, Origins O,
-                                float4 *__restrict__ rel, unsigned long long *__restrict__ vmax_bits) {
+                                float4 *__restrict__ rel) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i == 0) *vmax_bits = 0ull;
   if (i < pn) {
     double4 q = pos[i];
     const double4 p = mom[i];
@@ -113,13 +105,7 @@ __global__ void k_drift_refresh(double4 *__restrict__ pos, const double4 *__rest
   }
 }
 
-// --------------------------------------------------------------- force
-enum Epilogue {
-  kKickOnly = 0,      // p += kick                       (CalculateForce)
-  kKickDrift = 1,     // p += kick; q_out = q + p*dt2    (CalculateForce + UpdatePositionHalf)
-  kKickLangevin = 2,  // p += kick; Langevin update      (CalculateLangevin)
-};
-
+// ----------------------------------------------------------- Langevin RNG
 __device__ __forceinline__ void philox4x32_10(uint32_t c[4], uint32_t k0, uint32_t k1) {
 #pragma unroll
   for (int r = 0; r < 10; r++) {
@@ -159,132 +145,6 @@ __device__ __forceinline__ void langevin_update(double4 &p, const LangevinConsts
   p.z += (-g * p.z + (n[2] * lc.hb_D + 0.0)) * lc.dt;
 }
 
-// 1/x to double precision: hardware reciprocal estimate + two Newton steps.
-__device__ __forceinline__ double rcp64(double x) {
-  double r = __builtin_amdgcn_rcp(x);
-  double e = fma(-x, r, 1.0);
-  r = fma(r, e, r);
-  e = fma(-x, r, 1.0);
-  return fma(r, e, r);
-}
-
-__device__ __forceinline__ void finish_particle(int i, int lane_ok, double4 p, const double4 qi,
-                                                double4 *__restrict__ mom, double4 *__restrict__ pos_out,
-                                                int epilogue, double dt2, const LangevinConsts &lc,
-                                                const int *__restrict__ id,
-                                                unsigned long long *__restrict__ vmax_bits) {
-  double v2 = 0.0;
-  if (lane_ok) {
-    if (epilogue == kKickLangevin) langevin_update(p, lc, (uint32_t)id[i]);
-    mom[i] = p;
-    if (epilogue == kKickDrift) {
-      double4 q = qi;
-      q.x += p.x * dt2;
-      q.y += p.y * dt2;
-      q.z += p.z * dt2;
-      pos_out[i] = q;
-    } else if (epilogue == kKickLangevin) {
-      pos_out[i] = qi;  // one half drift per Langevin step (mdmanager.cc:185-192)
-    }
-    v2 = p.x * p.x + p.y * p.y + p.z * p.z;
-  }
-  if (vmax_bits) {
-    v2 = wave_max(v2);
-    if ((threadIdx.x % kWave) == 0) atomic_max_nonneg(vmax_bits, v2);
-  }
-}
-
-// FP64 kick.  One lane per owned row, rows in cell order, one wavefront per
-// 64-row slice of the neighbour list (coalesced 256-byte loads per entry).
-__global__ __launch_bounds__(kBlock) void k_force_fp64(
-    const double4 *__restrict__ pos, double4 *__restrict__ mom, double4 *__restrict__ pos_out,
-    const uint32_t *__restrict__ nbr, const long long *__restrict__ sbase, const int *__restrict__ slen,
-    int pn, ForceConsts fc, int epilogue, LangevinConsts lc, const int *__restrict__ id,
-    unsigned long long *__restrict__ vmax_bits) {
-  const int i = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
-  const int s = i / kWave, lane = i % kWave;
-  const int ok = i < pn;
-  const int ii = ok ? i : pn - 1;
-  const double4 qi = pos[ii];
-  double ax = 0.0, ay = 0.0, az = 0.0;
-  const int len = ok || (s * kWave < pn) ? slen[s] : 0;
-  const uint32_t *row = nbr + sbase[s < (pn + kWave - 1) / kWave ? s : 0] + lane;
-  for (int k = 0; k < len; k++) {
-    const uint32_t e = row[(long long)k * kWave];
-    const double4 qj = pos[e & kIndexMask];
-    const double dx = qj.x - qi.x, dy = qj.y - qi.y, dz = qj.z - qi.z;
-    const double r2 = fma(dx, dx, fma(dy, dy, dz * dz));
-    if (r2 <= fc.CL2) {
-      const double ir2 = rcp64(r2);
-      const double ir6 = ir2 * ir2 * ir2;
-      const double df = fma(ir2 * ir6, fma(-fc.B, ir6, fc.A), fc.C);
-      ax = fma(df, dx, ax);
-      ay = fma(df, dy, ay);
-      az = fma(df, dz, az);
-    }
-  }
-  double4 p = mom[ii];
-  p.x += ax;
-  p.y += ay;
-  p.z += az;
-  finish_particle(i, ok, p, qi, mom, pos_out, epilogue, fc.dt2, lc, id, vmax_bits);
-}
-
-// Mixed precision kick: FP32 cell-relative coordinates, FP32 pair forces,
-// compensated (Kahan) FP32 sums; the sum is added to the FP64 momentum and
-// the drift is FP64.  The neighbour entry carries the cell offset of the
-// partner, so dx = (rel_j - rel_i) + off*w stays small.
-template <bool kKahan>
-__global__ __launch_bounds__(kBlock) void k_force_mixed(
-    const float4 *__restrict__ rel, const double4 *__restrict__ pos, double4 *__restrict__ mom,
-    double4 *__restrict__ pos_out, const uint32_t *__restrict__ nbr, const long long *__restrict__ sbase,
-    const int *__restrict__ slen, int pn, ForceConstsF fc, double dt2, int epilogue, LangevinConsts lc,
-    const int *__restrict__ id, unsigned long long *__restrict__ vmax_bits) {
-  __shared__ float4 offs[64];
-  if (threadIdx.x < 64) {
-    const int t = threadIdx.x;
-    const int ox = (t & 3) - 1, oy = ((t >> 2) & 3) - 1, oz = ((t >> 4) & 3) - 1;
-    offs[t] = make_float4(ox * fc.w[0], oy * fc.w[1], oz * fc.w[2], 0.0f);
-  }
-  __syncthreads();
-  const int i = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
-  const int s = i / kWave, lane = i % kWave;
-  const int ok = i < pn;
-  const int ii = ok ? i : pn - 1;
-  const float4 ri = rel[ii];
-  float ax = 0.f, ay = 0.f, az = 0.f, cx = 0.f, cy = 0.f, cz = 0.f;
-  const int len = ok || (s * kWave < pn) ? slen[s] : 0;
-  const uint32_t *row = nbr + sbase[s < (pn + kWave - 1) / kWave ? s : 0] + lane;
-  for (int k = 0; k < len; k++) {
-    const uint32_t e = row[(long long)k * kWave];
-    const float4 rj = rel[e & kIndexMask];
-    const float4 o = offs[e >> kIndexBits];
-    const float dx = (rj.x - ri.x) + o.x, dy = (rj.y - ri.y) + o.y, dz = (rj.z - ri.z) + o.z;
-    const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
-    if (r2 <= fc.CL2) {
-      const float ir2 = __builtin_amdgcn_rcpf(r2);
-      const float ir6 = ir2 * ir2 * ir2;
-      const float df = fmaf(ir2 * ir6, fmaf(-fc.B, ir6, fc.A), fc.C);
-      if (kKahan) {
-        float y, t;
-        y = fmaf(df, dx, -cx); t = ax + y; cx = (t - ax) - y; ax = t;
-        y = fmaf(df, dy, -cy); t = ay + y; cy = (t - ay) - y; ay = t;
-        y = fmaf(df, dz, -cz); t = az + y; cz = (t - az) - y; az = t;
-      } else {
-        ax = fmaf(df, dx, ax);
-        ay = fmaf(df, dy, ay);
-        az = fmaf(df, dz, az);
-      }
-    }
-  }
-  double4 p = mom[ii];
-  p.x += (double)ax - (double)cx;
-  p.y += (double)ay - (double)cy;
-  p.z += (double)az - (double)cz;
-  const double4 qi = pos[ii];
-  finish_particle(i, ok, p, qi, mom, pos_out, epilogue, dt2, lc, id, vmax_bits);
-}
-
 // ----------------------------------------------------------- thermostats
 __global__ void k_langevin(double4 *__restrict__ mom, const int *__restrict__ id, int pn, LangevinConsts lc) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -302,55 +162,8 @@ __global__ void k_scale_momenta(double4 *__restrict__ mom, int pn, double f) {
   mom[i] = p;
 }
 
-// ------------------------------------------------------------- observers
-// Per-block partial sums, reduced in fixed order by k_reduce_partials, so the
-// results are deterministic.  Each list entry carries weight 1/2: owned-owned
-// pairs appear in two rows, owned-ghost pairs in one row and are halved by
-// the reference (observer.cc:39-41,70-72).
-__global__ __launch_bounds__(kBlock) void k_observe(const double4 *__restrict__ pos,
-                                                    const double4 *__restrict__ mom,
-                                                    const uint32_t *__restrict__ nbr,
-                                                    const long long *__restrict__ sbase,
-                                                    const int *__restrict__ slen, int pn, double CL2,
-                                                    double C2, double C0, double *__restrict__ partial) {
-  __shared__ double red[3][kBlock / kWave];
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  const int s = i / kWave, lane = i % kWave;
-  double ke = 0.0, pe = 0.0, vir = 0.0;
-  if (i < pn) {
-    const double4 p = mom[i];
-    ke = 0.5 * p.x * p.x + 0.5 * p.y * p.y + 0.5 * p.z * p.z + 0.5 * p.w * p.w;
-    const double4 qi = pos[i];
-    const int len = slen[s];
-    const uint32_t *row = nbr + sbase[s] + lane;
-    for (int k = 0; k < len; k++) {
-      const double4 qj = pos[row[(long long)k * kWave] & kIndexMask];
-      const double dx = qi.x - qj.x, dy = qi.y - qj.y, dz = qi.z - qj.z;
-      const double r2 = (dx * dx + dy * dy + dz * dz);
-      if (r2 > CL2) continue;
-      const double r6 = r2 * r2 * r2;
-      pe += 0.5 * (4.0 * (1.0 / (r2 * r2 * r2 * r2 * r2 * r2) - 1.0 / r6 + C2 * r2 + C0));
-      vir += 0.5 * (((24.0 * r6 - 48.0) / (r6 * r6 * r2) + C2 * 8.0) * r2);
-    }
-  }
-  ke = wave_sum(ke);
-  pe = wave_sum(pe);
-  vir = wave_sum(vir);
-  const int w = threadIdx.x / kWave;
-  if ((threadIdx.x % kWave) == 0) {
-    red[0][w] = ke;
-    red[1][w] = pe;
-    red[2][w] = vir;
-  }
-  __syncthreads();
-  if (threadIdx.x < 3) {
-    double t = 0.0;
-    for (int k = 0; k < kBlock / kWave; k++) t += red[threadIdx.x][k];
-    partial[(long long)blockIdx.x * 3 + threadIdx.x] = t;
-  }
-}
-
-// One workgroup reduces nblk triples of partial sums in a fixed order.
+// ------------------------------------------------------------- reductions
+// One workgroup reduces nblk tuples of partial sums in a fixed order.
 __global__ void k_reduce_partials(const double *__restrict__ partial, int nblk, int ncomp,
                                   double *__restrict__ out) {
   __shared__ double red[kBlock];
@@ -368,6 +181,7 @@ __global__ void k_reduce_partials(const double *__restrict__ partial, int nblk, 
   }
 }
 
+// KineticEnergyObserver: sum of p^2/2 over the four lanes (padding lane 0)
 __global__ void k_kinetic(const double4 *__restrict__ mom, int pn, double *__restrict__ partial) {
   __shared__ double red[kBlock / kWave];
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -387,21 +201,20 @@ __global__ void k_kinetic(const double4 *__restrict__ mom, int pn, double *__res
 }
 
 // ---------------------------------------------------------------- expiry
-__global__ void k_max_p2(const double4 *__restrict__ mom, int pn, unsigned long long *__restrict__ out) {
+__global__ void k_max_p2(const double4 *__restrict__ mom, int pn, double *__restrict__ partial) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   double v = 0.0;
   if (i < pn) {
     const double4 p = mom[i];
     v = p.x * p.x + p.y * p.y + p.z * p.z;
   }
-  v = wave_max(v);
-  if ((threadIdx.x % kWave) == 0) atomic_max_nonneg(out, v);
+  block_max_to_partial(v, partial);
 }
 
 // PairList::CheckByDisplacement, first pass: squared displacement since the
 // build for owned and ghost entries, and its maximum.
 __global__ void k_displacement(const double4 *__restrict__ pos, const double4 *__restrict__ old, int tn,
-                               double *__restrict__ disp, unsigned long long *__restrict__ dmax) {
+                               double *__restrict__ disp, double *__restrict__ partial) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   double v = 0.0;
   if (i < tn) {
@@ -410,8 +223,7 @@ __global__ void k_displacement(const double4 *__restrict__ pos, const double4 *_
     v = dx * dx + dy * dy + dz * dz;
     disp[i] = v;
   }
-  v = wave_max(v);
-  if ((threadIdx.x % kWave) == 0) atomic_max_nonneg(dmax, v);
+  block_max_to_partial(v, partial);
 }
 
 // second pass: particles displaced by more than (margin - max displacement);
@@ -462,11 +274,6 @@ __global__ void k_scale_pos(double4 *__restrict__ pos, int pn, double alpha) {
   q.y *= alpha;
   q.z *= alpha;
   pos[i] = q;
-}
-
-__global__ void k_fill_int(int *__restrict__ a, int n, int v) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) a[i] = v;
 }
 
 }  // namespace mdgpu

@@ -24,8 +24,9 @@ EXPORTS = [
     "mdgpu_is_pair_list_expired", "mdgpu_update_position_half", "mdgpu_send_border_particles",
     "mdgpu_calculate_force", "mdgpu_langevin", "mdgpu_heatbath_zeta", "mdgpu_heatbath_momenta",
     "mdgpu_calculate", "mdgpu_observe", "mdgpu_change_scale", "mdgpu_get_pairs", "mdgpu_compute_kick",
-    "mdgpu_enable_timing", "mdgpu_get_timing", "mdgpu_get_list_stats",
+    "mdgpu_enable_timing", "mdgpu_get_timing", "mdgpu_get_list_stats", "mdgpu_set_option", "mdgpu_get_option",
 ]
+OPT_UNROLL, OPT_KAHAN, OPT_TILE_WAVES, OPT_LDS_KB = 1, 2, 3, 4
 
 
 class Params(ctypes.Structure):
@@ -80,6 +81,8 @@ def load():
             "mdgpu_enable_timing": [P, I],
             "mdgpu_get_timing": [P, PI, PD, PD, PI, PD],
             "mdgpu_get_list_stats": [P, PL, PL, PI, PI],
+            "mdgpu_set_option": [P, I, I],
+            "mdgpu_get_option": [P, I, PI],
         }
         for name, args in sig.items():
             f = getattr(L, name)
@@ -246,6 +249,14 @@ class Domain:
                                           ctypes.byref(rb), ctypes.byref(rm)))
         return dict(force_launches=fl.value, force_ms=fm.value, total_ms=tm.value, rebuilds=rb.value,
                     rebuild_ms=rm.value)
+
+    def set_option(self, option, value):
+        self._chk(load().mdgpu_set_option(self.h, option, int(value)))
+
+    def get_option(self, option):
+        v = ctypes.c_int()
+        self._chk(load().mdgpu_get_option(self.h, option, ctypes.byref(v)))
+        return v.value
 
     def list_stats(self):
         e, s, c, g = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int(), ctypes.c_int()
