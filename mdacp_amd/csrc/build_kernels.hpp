@@ -191,174 +191,4 @@ __global__ void k_set_sentinel(double4 *a, double4 *b, float4 *ra, float4 *rb, i
   }
 }
 
-// ---- neighbour search: one lane per owned row, 27 cells of the extended
-// grid (no wrap), owned members then ghost members of each cell in order.
-struct CellTables {
-  const int *ostart, *ocount, *gstart, *gcount;
-};
-
-// Tiles: kWave * W consecutive owned rows (cell order) per workgroup.  The
-// partners of a tile's rows lie in 9 contiguous cell ranges (one per (dy,dz)
-// neighbour row of the cell grid), i.e. in 9 owned and 9 ghost index ranges;
-// the force kernels stage those 18 ranges into LDS and the list stores each
-// partner as a 16-bit index into the staged array.
-constexpr int kTileRanges = 18;
-
-__device__ __forceinline__ void range_of(int k, int c0, int c1, Grid g, CellTables T, int pn, int &start,
-                                         int &count) {
-  const int kk = k % 9;
-  const int dz = kk / 3 - 1, dy = kk % 3 - 1;
-  const int ncell = g.m[0] * g.m[1] * g.m[2];
-  const int o = dy * g.m[0] + dz * g.m[0] * g.m[1];
-  const int lo = max(0, c0 - 1 + o), hi = min(ncell - 1, c1 + 1 + o);
-  if (lo > hi) {
-    start = 0;
-    count = 0;
-    return;
-  }
-  if (k < 9) {
-    start = T.ostart[lo];
-    count = T.ostart[hi] + T.ocount[hi] - start;
-  } else {
-    start = pn + T.gstart[lo];
-    count = T.gstart[hi] + T.gcount[hi] - T.gstart[lo];
-  }
-}
-
-// tstart[t][18] = first particle index of each range, tbase[t][19] = offset of
-// each range in the staged array (tbase[t][18] = staged count = the pad slot)
-__global__ void k_tile_ranges(const int *__restrict__ cell, int pn, int rows_per_tile, int nt, Grid g,
-                              CellTables T, int *__restrict__ tstart, int *__restrict__ tbase,
-                              double *__restrict__ maxpart) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  double total = 0.0;
-  if (t < nt) {
-    const int r0 = t * rows_per_tile, r1 = min(pn, r0 + rows_per_tile) - 1;
-    const int c0 = cell[r0], c1 = cell[r1];
-    int base = 0;
-    for (int k = 0; k < kTileRanges; k++) {
-      int st, n;
-      range_of(k, c0, c1, g, T, pn, st, n);
-      tstart[t * kTileRanges + k] = st;
-      tbase[t * (kTileRanges + 1) + k] = base;
-      base += n;
-    }
-    tbase[t * (kTileRanges + 1) + kTileRanges] = base;
-    total = (double)base;
-  }
-  block_max_to_partial(total, maxpart);
-}
-
-template <bool kFill>
-__device__ __forceinline__ int search_row(int i, const double4 *__restrict__ pos, const int *__restrict__ cell,
-                                          int pn, Grid g, CellTables T, double S2, uint16_t *__restrict__ out,
-                                          uint8_t *__restrict__ code_out, long long stride, int cap,
-                                          const int *__restrict__ ts, const int *__restrict__ tb,
-                                          int *__restrict__ err) {
-  const double4 qi = pos[i];
-  const int c = cell[i];
-  const int mx = g.m[0], my = g.m[1], mz = g.m[2];
-  const int ix = c % mx, iy = (c / mx) % my, iz = c / (mx * my);
-  int cnt = 0;
-  for (int dz = -1; dz <= 1; dz++) {
-    const int jz = iz + dz;
-    if (jz < 0 || jz >= mz) continue;
-    for (int dy = -1; dy <= 1; dy++) {
-      const int jy = iy + dy;
-      if (jy < 0 || jy >= my) continue;
-      const int k = (dz + 1) * 3 + (dy + 1);
-      for (int dx = -1; dx <= 1; dx++) {
-        const int jx = ix + dx;
-        if (jx < 0 || jx >= mx) continue;
-        const int c2 = mx * my * jz + mx * jy + jx;
-        const uint8_t code = (uint8_t)((dx + 1) | ((dy + 1) << 2) | ((dz + 1) << 4));
-        for (int pass = 0; pass < 2; pass++) {
-          const int b = pass ? pn + T.gstart[c2] : T.ostart[c2];
-          const int e = b + (pass ? T.gcount[c2] : T.ocount[c2]);
-          const int kk = pass ? 9 + k : k;
-          for (int j = b; j < e; j++) {
-            if (j == i) continue;
-            const double4 qj = pos[j];
-            const double ex = qi.x - qj.x, ey = qi.y - qj.y, ez = qi.z - qj.z;
-            const double r2 = (ex * ex + ey * ey + ez * ez);
-            if (r2 > S2) continue;
-            if (kFill) {
-              const int off = j - ts[kk];
-              const int n = tb[kk + 1] - tb[kk];
-              if (off < 0 || off >= n) {
-                atomicOr(err, kErrCapacity);  // partner outside its tile range: cannot happen
-              } else if (cnt < cap) {
-                out[(long long)cnt * stride] = (uint16_t)(tb[kk] + off);
-                code_out[(long long)cnt * stride] = code;
-              }
-            }
-            cnt++;
-          }
-        }
-      }
-    }
-  }
-  return cnt;
-}
-
-__global__ void k_count_rows(const double4 *__restrict__ pos, const int *__restrict__ cell, int pn, Grid g,
-                             CellTables T, double S2, int *__restrict__ row_len) {
-  const int i = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
-  if (i >= pn) return;
-  row_len[i] = search_row<false>(i, pos, cell, pn, g, T, S2, nullptr, nullptr, 0, 0, nullptr, nullptr, nullptr);
-}
-
-// slice length = longest row of its 64 rows, rounded up to a multiple of
-// kSliceRound so the force loop runs whole batches; slots = 64 * length
-__global__ void k_slice_len(const int *__restrict__ row_len, int pn, int nslice, int *__restrict__ slen,
-                            long long *__restrict__ slots) {
-  const int s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= nslice) return;
-  int m = 0;
-  const int b = s * kWave, e = min(pn, b + kWave);
-  for (int i = b; i < e; i++) m = max(m, row_len[i]);
-  m = (m + kSliceRound - 1) / kSliceRound * kSliceRound;
-  slen[s] = m;
-  slots[s] = (long long)m * kWave;
-}
-
-__global__ void k_fill_rows(const double4 *__restrict__ pos, const int *__restrict__ cell, int pn, Grid g,
-                            CellTables T, double S2, const long long *__restrict__ sbase,
-                            const int *__restrict__ slen, int rows_per_tile, const int *__restrict__ tstart,
-                            const int *__restrict__ tbase, uint16_t *__restrict__ nbr, uint8_t *__restrict__ ncode,
-                            int *__restrict__ err) {
-  const int i = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
-  if (i >= pn) return;
-  const int s = i / kWave, lane = i % kWave, t = i / rows_per_tile;
-  const int *ts = tstart + (long long)t * kTileRanges;
-  const int *tb = tbase + (long long)t * (kTileRanges + 1);
-  const int len = slen[s];
-  uint16_t *row = nbr + sbase[s] + lane;
-  uint8_t *crow = ncode + sbase[s] + lane;
-  const int cnt = search_row<true>(i, pos, cell, pn, g, T, S2, row, crow, kWave, len, ts, tb, err);
-  if (cnt > len) atomicOr(err, kErrCapacity);
-  const uint16_t pad = (uint16_t)tb[kTileRanges];
-  for (int k = cnt; k < len; k++) {
-    row[(long long)k * kWave] = pad;
-    crow[(long long)k * kWave] = (uint8_t)kZeroOffset;
-  }
-}
-
-// rows past pn inside the last slice are all padding
-__global__ void k_fill_tail(int pn, int nslice, int rows_per_tile, const long long *__restrict__ sbase,
-                            const int *__restrict__ slen, const int *__restrict__ tbase, uint16_t *__restrict__ nbr,
-                            uint8_t *__restrict__ ncode) {
-  const int lane = threadIdx.x;
-  const int s = nslice - 1;
-  const int i = s * kWave + lane;
-  if (lane >= kWave || i < pn) return;
-  const int t = (s * kWave) / rows_per_tile;
-  const uint16_t pad = (uint16_t)tbase[(long long)t * (kTileRanges + 1) + kTileRanges];
-  const int len = slen[s];
-  for (int k = 0; k < len; k++) {
-    nbr[sbase[s] + (long long)k * kWave + lane] = pad;
-    ncode[sbase[s] + (long long)k * kWave + lane] = (uint8_t)kZeroOffset;
-  }
-}
-
 }  // namespace mdgpu

@@ -16,6 +16,7 @@ constexpr int kSliceRound = 8;                      // slice lengths are multipl
 constexpr int kErrInvalidPosition = 1;
 constexpr int kErrInvalidIndex = 2;
 constexpr int kErrCapacity = 4;
+constexpr int kErrLayout = 8;  // a listed partner outside its brick's staged ranges
 
 // Workgroups are dispatched round-robin over the XCDs, each with its own L2.
 // Remap so that every XCD walks one contiguous run of logical blocks: rows in
@@ -40,7 +41,7 @@ __device__ __forceinline__ double wave_sum(double v) {
 // (no same-address atomics: those serialise in L2 across ~10^4 workgroups).
 // Every thread of the workgroup must call it.
 __device__ __forceinline__ void block_max_to_partial(double v, double *__restrict__ partial) {
-  __shared__ double wm[kBlock / kWave];
+  __shared__ double wm[16];  // up to 1024 threads
   v = wave_max(v);
   if ((threadIdx.x % kWave) == 0) wm[threadIdx.x / kWave] = v;
   __syncthreads();

@@ -1,5 +1,5 @@
 // force_kernels.hpp -- the pair loops on gfx950: the smoothed-LJ force kick
-// (FP64 and mixed FP32) and the pair observers, over LDS-staged cell tiles.
+// (FP64 and mixed FP32) and the pair observers, one workgroup per cell brick.
 //
 // Reference semantics (kaityo256/mdacp):
 //   CalculateForceNext  fcalculator.cc:228-290: for each listed pair within rc,
@@ -7,22 +7,24 @@
 //   potential           4(r^-12 - r^-6 + C2 r^2 + C0), C0/C2 from variables.cc:130-142
 //   observers           observer.cc:17-79 (PE and virial, ghost pairs halved)
 //
-// A workgroup takes one tile of kWave*W consecutive owned rows (cell order).
-// It first copies the 18 contiguous particle ranges its rows' partners can
-// lie in (9 owned + 9 ghost, see k_tile_ranges) into LDS with coalesced
-// loads, then each lane walks its row of the sliced-ELL list (16-bit indices
-// into the staged array) and gathers partners from LDS instead of through the
-// texture path.  Tiles whose staged set does not fit the LDS budget read the
-// same partners from global memory instead (same arithmetic, slower).
+// A workgroup first copies its brick's neighbourhood (list_kernels.hpp) into
+// LDS with coalesced loads: owned particles and ghost images of each
+// neighbourhood cell, the ghosts as placed by the last halo refresh.  Its wavefronts
+// then walk the brick's 64-row slices of the sliced-ELL list (16-bit indices
+// into the staged array) and gather partners from LDS.  A brick whose staged
+// set does not fit the LDS budget reads the same partners from global memory
+// (same arithmetic, slower).
 //
 // Full neighbour list: each owned row lists every partner within SL, so no
 // momentum scatter and no atomics: results are bit-reproducible run to run.
 #pragma once
-#include "build_kernels.hpp"
 #include "device_common.hpp"
+#include "list_kernels.hpp"
 #include "step_kernels.hpp"
 
 namespace mdgpu {
+
+constexpr int kMaxBrickRanges = kRangesPerCell * 6 * 6 * 6;  // bricks of at most 4x4x4 cells
 
 enum Epilogue {
   kKickOnly = 0,      // p += kick                       (CalculateForce)
@@ -30,26 +32,18 @@ enum Epilogue {
   kKickLangevin = 2,  // p += kick; Langevin update      (CalculateLangevin)
 };
 
-struct TileList {
+struct BrickList {
   const uint16_t *nbr;      // staged index per (slice, slot, lane)
   const uint8_t *code;      // cell offset of the partner: (dx+1) | (dy+1)<<2 | (dz+1)<<4
   const long long *sbase;   // first slot of each 64-row slice
   const int *slen;          // slots per slice (multiple of kSliceRound)
-  const int *tstart;        // [tile][18] first particle of each staged range
-  const int *tbase;         // [tile][19] offset of each range in the staged array
-  int nslice;
+  const int *tstart;        // [brick][2 rmax] first particle of each staged range
+  const int *tbase;         // [brick][2 rmax + 1] offset of each range in the staged array
+  const int *nrange;        // ranges of each brick (2 per neighbourhood cell)
+  const int *row0, *nrow, *vslice0;
+  int rstride;              // 2 rmax
   int lds_cap;              // staged entries that fit the LDS budget
 };
-
-// staged index -> particle index, for tiles read from global memory
-__device__ __forceinline__ int tile_global(int e, const int *__restrict__ ts, const int *__restrict__ tb, int total,
-                                           int sentinel) {
-  if (e >= total) return sentinel;
-  int k = 0;
-#pragma unroll
-  for (int r = 1; r < kTileRanges; r++) k += (e >= tb[r]) ? 1 : 0;
-  return ts[k] + (e - tb[k]);
-}
 
 // 1/x to double precision: hardware reciprocal estimate + two Newton steps.
 __device__ __forceinline__ double rcp64(double x) {
@@ -60,133 +54,169 @@ __device__ __forceinline__ double rcp64(double x) {
   return fma(r, e, r);
 }
 
-__device__ __forceinline__ void finish_particle(int i, int lane_ok, double4 p, const double4 qi,
-                                                double4 *__restrict__ mom, double4 *__restrict__ pos_out,
-                                                int epilogue, double dt2, const LangevinConsts &lc,
-                                                const int *__restrict__ id, double *__restrict__ vpart) {
-  double v2 = 0.0;
-  if (lane_ok) {
-    if (epilogue == kKickLangevin) langevin_update(p, lc, (uint32_t)id[i]);
-    mom[i] = p;
-    if (epilogue == kKickDrift) {
-      double4 q = qi;
-      q.x += p.x * dt2;
-      q.y += p.y * dt2;
-      q.z += p.z * dt2;
-      pos_out[i] = q;
-    } else if (epilogue == kKickLangevin) {
-      pos_out[i] = qi;  // one half drift per Langevin step (mdmanager.cc:185-192)
-    }
-    v2 = p.x * p.x + p.y * p.y + p.z * p.z;
+// range r of a staged index u: largest r with base[r] <= u (base[0] = 0 <= u < base[R])
+__device__ __forceinline__ int range_of_staged(const int *base, int R, int u) {
+  int lo = 0, hi = R;
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (base[mid] <= u) lo = mid;
+    else hi = mid;
   }
-  if (vpart) block_max_to_partial(v2, vpart);
+  return lo;
 }
 
-// Stage the tile's partner ranges as SoA doubles (24 bytes per particle).
+// Position of staged index u: owned ranges (even) read `cur`, ghost ranges
+// (odd) read `gpos`, whose entries [pn, pn + ng) hold the ghost images.
+template <class T>
+__device__ __forceinline__ T staged_load(int u, int R, const int *base, const int *__restrict__ ts,
+                                         const T *__restrict__ cur, const T *__restrict__ gpos) {
+  const int r = range_of_staged(base, R, u);
+  const int j = ts[r] + (u - base[r]);
+  return (r & 1) ? gpos[j] : cur[j];
+}
+
+// LDS layout of the brick kernels: staged-range bases, then the partners.
+struct BrickShared {
+  int base[kMaxBrickRanges + 1];
+};
+
 template <int T>
-__device__ __forceinline__ void stage_fp64(const double4 *__restrict__ src, const int *__restrict__ ts,
-                                           const int *__restrict__ tb, int total, double *lx, double *ly,
-                                           double *lz) {
-  for (int k = 0; k < kTileRanges; k++) {
-    const int b = tb[k], n = tb[k + 1] - b, st = ts[k];
-    for (int u = threadIdx.x; u < n; u += T) {
-      const double4 q = src[st + u];
-      lx[b + u] = q.x;
-      ly[b + u] = q.y;
-      lz[b + u] = q.z;
-    }
+__device__ __forceinline__ int load_bases(const BrickList &bl, int b, int *base) {
+  const int R = bl.nrange[b];
+  const int *tb = bl.tbase + (long long)b * (bl.rstride + 1);
+  for (int r = threadIdx.x; r <= R; r += T) base[r] = tb[r];
+  return R;
+}
+
+__device__ __forceinline__ double finish_row(int i, double4 p, const double4 qi, double4 *__restrict__ mom,
+                                             double4 *__restrict__ pos_out, int epilogue, double dt2,
+                                             const LangevinConsts &lc, const int *__restrict__ id) {
+  if (epilogue == kKickLangevin) langevin_update(p, lc, (uint32_t)id[i]);
+  mom[i] = p;
+  if (epilogue == kKickDrift) {
+    double4 q = qi;
+    q.x += p.x * dt2;
+    q.y += p.y * dt2;
+    q.z += p.z * dt2;
+    pos_out[i] = q;
+  } else if (epilogue == kKickLangevin) {
+    pos_out[i] = qi;  // one half drift per Langevin step (mdmanager.cc:185-192)
   }
-  if (threadIdx.x == 0) {
-    lx[total] = 1.0e200;
-    ly[total] = 1.0e200;
-    lz[total] = 1.0e200;
-  }
+  return p.x * p.x + p.y * p.y + p.z * p.z;
 }
 
 // FP64 kick.  Entries are consumed in batches of U: the next batch's indices
 // are loaded before the current batch is computed.  The cutoff is a select,
 // not a branch (padding entries point at a far sentinel: r2 = inf).
 template <int W, int U>
-__global__ __launch_bounds__(kWave *W) void k_force_tile_fp64(
-    const double4 *__restrict__ src, double4 *__restrict__ mom, double4 *__restrict__ pos_out, TileList tl,
-    int pn, int tn, ForceConsts fc, int epilogue, LangevinConsts lc, const int *__restrict__ id,
-    double *__restrict__ vpart) {
+__global__ __launch_bounds__(kWave *W) void k_force_brick_fp64(
+    const double4 *__restrict__ src, const double4 *__restrict__ gpos, double4 *__restrict__ mom,
+    double4 *__restrict__ pos_out, BrickList bl, ForceConsts fc, int epilogue, LangevinConsts lc,
+    const int *__restrict__ id, double *__restrict__ vpart) {
   extern __shared__ double lds64[];
+  __shared__ BrickShared sh;
   constexpr int T = kWave * W;
-  const int t = xcd_block(blockIdx.x, gridDim.x);
-  const int i = t * T + threadIdx.x;
-  const int s = i / kWave, lane = threadIdx.x % kWave;
-  const int ok = i < pn;
-  const int ii = ok ? i : pn - 1;
-  const int *ts = tl.tstart + (long long)t * kTileRanges;
-  const int *tb = tl.tbase + (long long)t * (kTileRanges + 1);
-  const int total = tb[kTileRanges];
-  const bool staged = total < tl.lds_cap;  // uniform across the workgroup
-  double *lx = lds64, *ly = lds64 + tl.lds_cap, *lz = lds64 + 2 * tl.lds_cap;
-  if (staged) stage_fp64<T>(src, ts, tb, total, lx, ly, lz);
+  const int b = xcd_block(blockIdx.x, gridDim.x);
+  const int R = load_bases<T>(bl, b, sh.base);
   __syncthreads();
-  const double4 qi = src[ii];
-  double ax = 0.0, ay = 0.0, az = 0.0;
-  const bool live = s < tl.nslice;  // uniform across the wavefront
-  const int len = live ? tl.slen[s] : 0;
-  const uint16_t *row = tl.nbr + (live ? tl.sbase[s] : 0) + lane;
-  int e[U];
-  if (len > 0) {
-#pragma unroll
-    for (int u = 0; u < U; u++) e[u] = row[(long long)u * kWave];
-  }
-  for (int k = 0; k < len; k += U) {
-    double qx[U], qy[U], qz[U];
-    if (staged) {
-#pragma unroll
-      for (int u = 0; u < U; u++) {
-        qx[u] = lx[e[u]];
-        qy[u] = ly[e[u]];
-        qz[u] = lz[e[u]];
-      }
-    } else {
-#pragma unroll
-      for (int u = 0; u < U; u++) {
-        const double4 q = src[tile_global(e[u], ts, tb, total, tn)];
-        qx[u] = q.x;
-        qy[u] = q.y;
-        qz[u] = q.z;
-      }
+  const int *ts = bl.tstart + (long long)b * bl.rstride;
+  const int total = sh.base[R];
+  const bool staged = total < bl.lds_cap;  // uniform across the workgroup
+  double *lx = lds64, *ly = lds64 + bl.lds_cap, *lz = lds64 + 2 * bl.lds_cap;
+  if (staged) {
+    for (int u = threadIdx.x; u < total; u += T) {
+      const double4 q = staged_load(u, R, sh.base, ts, src, gpos);
+      lx[u] = q.x;
+      ly[u] = q.y;
+      lz[u] = q.z;
     }
-    if (k + U < len) {
-#pragma unroll
-      for (int u = 0; u < U; u++) e[u] = row[(long long)(k + U + u) * kWave];
-    }
-#pragma unroll
-    for (int u = 0; u < U; u++) {
-      const double dx = qx[u] - qi.x, dy = qy[u] - qi.y, dz = qz[u] - qi.z;
-      const double r2 = fma(dx, dx, fma(dy, dy, dz * dz));
-      const double ir2 = rcp64(r2);
-      const double ir6 = ir2 * ir2 * ir2;
-      double df = fma(ir2 * ir6, fma(-fc.B, ir6, fc.A), fc.C);
-      df = (r2 <= fc.CL2) ? df : 0.0;
-      ax = fma(df, dx, ax);
-      ay = fma(df, dy, ay);
-      az = fma(df, dz, az);
+    if (threadIdx.x == 0) {
+      lx[total] = 1.0e200;
+      ly[total] = 1.0e200;
+      lz[total] = 1.0e200;
     }
   }
-  double4 p = mom[ii];
-  p.x += ax;
-  p.y += ay;
-  p.z += az;
-  finish_particle(i, ok, p, qi, mom, pos_out, epilogue, fc.dt2, lc, id, vpart);
+  __syncthreads();
+  const int nrow = bl.nrow[b], row0 = bl.row0[b], vs0 = bl.vslice0[b];
+  const int nsl = (nrow + kWave - 1) / kWave;
+  const int wv = threadIdx.x / kWave, lane = threadIdx.x % kWave;
+  double vmax = 0.0;
+  for (int sl = wv; sl < nsl; sl += W) {
+    const int s = vs0 + sl;
+    const int v = sl * kWave + lane;
+    const bool ok = v < nrow;
+    const int i = row0 + (ok ? v : nrow - 1);
+    const double4 qi = src[i];
+    double ax = 0.0, ay = 0.0, az = 0.0;
+    const int len = bl.slen[s];
+    const uint16_t *row = bl.nbr + bl.sbase[s] + lane;
+    int e[U];
+    // len is 0 or a multiple of kSliceRound >= U: an empty slice reads nothing
+#pragma unroll
+    for (int u = 0; u < U; u++) e[u] = len > 0 ? row[(long long)u * kWave] : 0;
+    for (int k = 0; k < len; k += U) {
+      double qx[U], qy[U], qz[U];
+      if (staged) {
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+          qx[u] = lx[e[u]];
+          qy[u] = ly[e[u]];
+          qz[u] = lz[e[u]];
+        }
+      } else {
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+          if (e[u] < total) {
+            const double4 q = staged_load(e[u], R, sh.base, ts, src, gpos);
+            qx[u] = q.x;
+            qy[u] = q.y;
+            qz[u] = q.z;
+          } else {
+            qx[u] = qy[u] = qz[u] = 1.0e200;
+          }
+        }
+      }
+      if (k + U < len) {
+#pragma unroll
+        for (int u = 0; u < U; u++) e[u] = row[(long long)(k + U + u) * kWave];
+      }
+#pragma unroll
+      for (int u = 0; u < U; u++) {
+        const double dx = qx[u] - qi.x, dy = qy[u] - qi.y, dz = qz[u] - qi.z;
+        const double r2 = fma(dx, dx, fma(dy, dy, dz * dz));
+        const double ir2 = rcp64(r2);
+        const double ir6 = ir2 * ir2 * ir2;
+        double df = fma(ir2 * ir6, fma(-fc.B, ir6, fc.A), fc.C);
+        df = (r2 <= fc.CL2) ? df : 0.0;
+        ax = fma(df, dx, ax);
+        ay = fma(df, dy, ay);
+        az = fma(df, dz, az);
+      }
+    }
+    if (ok) {
+      double4 p = mom[i];
+      p.x += ax;
+      p.y += ay;
+      p.z += az;
+      vmax = fmax(vmax, finish_row(i, p, qi, mom, pos_out, epilogue, fc.dt2, lc, id));
+    }
+  }
+  if (vpart) block_max_to_partial(vmax, vpart);
 }
 
 // Mixed precision kick: FP32 cell-relative coordinates (staged as float4),
 // FP32 pair forces, compensated (Kahan) FP32 sums; the sum is added to the
 // FP64 momentum and the drift is FP64.  The partner's cell offset comes from
-// the code array, so dx = (rel_j - rel_i) + off*w stays small.
+// the code array, so dx = (rel_j - rel_i) + off*w stays small.  Ghost images
+// carry coordinates relative to the extended-grid cell they are binned in,
+// like owned particles.
 template <int W, int U, bool kKahan>
-__global__ __launch_bounds__(kWave *W) void k_force_tile_mixed(
-    const float4 *__restrict__ rel, const double4 *__restrict__ pos, double4 *__restrict__ mom,
-    double4 *__restrict__ pos_out, TileList tl, int pn, int tn, ForceConstsF fc, double dt2, int epilogue,
-    LangevinConsts lc, const int *__restrict__ id, double *__restrict__ vpart) {
+__global__ __launch_bounds__(kWave *W) void k_force_brick_mixed(
+    const float4 *__restrict__ rel, const float4 *__restrict__ grel, const double4 *__restrict__ pos,
+    double4 *__restrict__ mom, double4 *__restrict__ pos_out, BrickList bl, ForceConstsF fc, double dt2,
+    int epilogue, LangevinConsts lc, const int *__restrict__ id, double *__restrict__ vpart) {
   extern __shared__ float4 lds32[];
+  __shared__ BrickShared sh;
   __shared__ float4 offs[64];
   constexpr int T = kWave * W;
   if (threadIdx.x < 64) {
@@ -194,131 +224,151 @@ __global__ __launch_bounds__(kWave *W) void k_force_tile_mixed(
     const int ox = (c & 3) - 1, oy = ((c >> 2) & 3) - 1, oz = ((c >> 4) & 3) - 1;
     offs[c] = make_float4(ox * fc.w[0], oy * fc.w[1], oz * fc.w[2], 0.0f);
   }
-  const int t = xcd_block(blockIdx.x, gridDim.x);
-  const int i = t * T + threadIdx.x;
-  const int s = i / kWave, lane = threadIdx.x % kWave;
-  const int ok = i < pn;
-  const int ii = ok ? i : pn - 1;
-  const int *ts = tl.tstart + (long long)t * kTileRanges;
-  const int *tb = tl.tbase + (long long)t * (kTileRanges + 1);
-  const int total = tb[kTileRanges];
-  const bool staged = total < tl.lds_cap;
+  const int b = xcd_block(blockIdx.x, gridDim.x);
+  const int R = load_bases<T>(bl, b, sh.base);
+  __syncthreads();
+  const int *ts = bl.tstart + (long long)b * bl.rstride;
+  const int total = sh.base[R];
+  const bool staged = total < bl.lds_cap;
   if (staged) {
-    for (int k = 0; k < kTileRanges; k++) {
-      const int b = tb[k], n = tb[k + 1] - b, st = ts[k];
-      for (int u = threadIdx.x; u < n; u += T) lds32[b + u] = rel[st + u];
+    for (int u = threadIdx.x; u < total; u += T) {
+      lds32[u] = staged_load(u, R, sh.base, ts, rel, grel);
     }
     if (threadIdx.x == 0) lds32[total] = make_float4(1.0e30f, 1.0e30f, 1.0e30f, 0.0f);
   }
   __syncthreads();
-  const float4 ri = rel[ii];
-  float ax = 0.f, ay = 0.f, az = 0.f, cx = 0.f, cy = 0.f, cz = 0.f;
-  const bool live = s < tl.nslice;
-  const int len = live ? tl.slen[s] : 0;
-  const long long base = (live ? tl.sbase[s] : 0) + lane;
-  const uint16_t *row = tl.nbr + base;
-  const uint8_t *crow = tl.code + base;
-  int e[U], c[U];
-  if (len > 0) {
+  const int nrow = bl.nrow[b], row0 = bl.row0[b], vs0 = bl.vslice0[b];
+  const int nsl = (nrow + kWave - 1) / kWave;
+  const int wv = threadIdx.x / kWave, lane = threadIdx.x % kWave;
+  double vmax = 0.0;
+  for (int sl = wv; sl < nsl; sl += W) {
+    const int s = vs0 + sl;
+    const int v = sl * kWave + lane;
+    const bool ok = v < nrow;
+    const int i = row0 + (ok ? v : nrow - 1);
+    const float4 ri = rel[i];
+    float ax = 0.f, ay = 0.f, az = 0.f, cx = 0.f, cy = 0.f, cz = 0.f;
+    const int len = bl.slen[s];
+    const long long base = bl.sbase[s] + lane;
+    const uint16_t *row = bl.nbr + base;
+    const uint8_t *crow = bl.code + base;
+    int e[U], c[U];
 #pragma unroll
     for (int u = 0; u < U; u++) {
-      e[u] = row[(long long)u * kWave];
-      c[u] = crow[(long long)u * kWave];
+      e[u] = len > 0 ? row[(long long)u * kWave] : 0;
+      c[u] = len > 0 ? crow[(long long)u * kWave] : 0;
     }
-  }
-  for (int k = 0; k < len; k += U) {
-    float4 rj[U];
-    int cc[U];
-    if (staged) {
+    for (int k = 0; k < len; k += U) {
+      float4 rj[U];
+      int cc[U];
+      if (staged) {
 #pragma unroll
-      for (int u = 0; u < U; u++) rj[u] = lds32[e[u]];
-    } else {
+        for (int u = 0; u < U; u++) rj[u] = lds32[e[u]];
+      } else {
 #pragma unroll
-      for (int u = 0; u < U; u++) rj[u] = rel[tile_global(e[u], ts, tb, total, tn)];
-    }
+        for (int u = 0; u < U; u++) {
+          if (e[u] < total) {
+            rj[u] = staged_load(e[u], R, sh.base, ts, rel, grel);
+          } else {
+            rj[u] = make_float4(1.0e30f, 1.0e30f, 1.0e30f, 0.0f);
+          }
+        }
+      }
 #pragma unroll
-    for (int u = 0; u < U; u++) cc[u] = c[u];
-    if (k + U < len) {
+      for (int u = 0; u < U; u++) cc[u] = c[u];
+      if (k + U < len) {
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+          e[u] = row[(long long)(k + U + u) * kWave];
+          c[u] = crow[(long long)(k + U + u) * kWave];
+        }
+      }
 #pragma unroll
       for (int u = 0; u < U; u++) {
-        e[u] = row[(long long)(k + U + u) * kWave];
-        c[u] = crow[(long long)(k + U + u) * kWave];
+        const float4 o = offs[cc[u]];
+        const float dx = (rj[u].x - ri.x) + o.x, dy = (rj[u].y - ri.y) + o.y, dz = (rj[u].z - ri.z) + o.z;
+        const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+        const float ir2 = __builtin_amdgcn_rcpf(r2);
+        const float ir6 = ir2 * ir2 * ir2;
+        float df = fmaf(ir2 * ir6, fmaf(-fc.B, ir6, fc.A), fc.C);
+        df = (r2 <= fc.CL2) ? df : 0.0f;
+        if (kKahan) {
+          float y, q;
+          y = fmaf(df, dx, -cx); q = ax + y; cx = (q - ax) - y; ax = q;
+          y = fmaf(df, dy, -cy); q = ay + y; cy = (q - ay) - y; ay = q;
+          y = fmaf(df, dz, -cz); q = az + y; cz = (q - az) - y; az = q;
+        } else {
+          ax = fmaf(df, dx, ax);
+          ay = fmaf(df, dy, ay);
+          az = fmaf(df, dz, az);
+        }
       }
     }
-#pragma unroll
-    for (int u = 0; u < U; u++) {
-      const float4 o = offs[cc[u]];
-      const float dx = (rj[u].x - ri.x) + o.x, dy = (rj[u].y - ri.y) + o.y, dz = (rj[u].z - ri.z) + o.z;
-      const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
-      const float ir2 = __builtin_amdgcn_rcpf(r2);
-      const float ir6 = ir2 * ir2 * ir2;
-      float df = fmaf(ir2 * ir6, fmaf(-fc.B, ir6, fc.A), fc.C);
-      df = (r2 <= fc.CL2) ? df : 0.0f;
-      if (kKahan) {
-        float y, q;
-        y = fmaf(df, dx, -cx); q = ax + y; cx = (q - ax) - y; ax = q;
-        y = fmaf(df, dy, -cy); q = ay + y; cy = (q - ay) - y; ay = q;
-        y = fmaf(df, dz, -cz); q = az + y; cz = (q - az) - y; az = q;
-      } else {
-        ax = fmaf(df, dx, ax);
-        ay = fmaf(df, dy, ay);
-        az = fmaf(df, dz, az);
-      }
+    if (ok) {
+      double4 p = mom[i];
+      p.x += (double)ax - (double)cx;
+      p.y += (double)ay - (double)cy;
+      p.z += (double)az - (double)cz;
+      vmax = fmax(vmax, finish_row(i, p, pos[i], mom, pos_out, epilogue, dt2, lc, id));
     }
   }
-  double4 p = mom[ii];
-  p.x += (double)ax - (double)cx;
-  p.y += (double)ay - (double)cy;
-  p.z += (double)az - (double)cz;
-  const double4 qi = pos[ii];
-  finish_particle(i, ok, p, qi, mom, pos_out, epilogue, dt2, lc, id, vpart);
+  if (vpart) block_max_to_partial(vmax, vpart);
 }
 
 // Observers over the list: kinetic energy, PE and virial sums per workgroup
 // (partial[block*3 + {0,1,2}]), reduced in fixed order by k_reduce_partials,
 // so the results are deterministic.  Each list entry carries weight 1/2:
 // owned-owned pairs appear in two rows, owned-ghost pairs in one row and are
-// halved by the reference (observer.cc:39-41,70-72).  Ghost positions are the
-// ones of the last halo refresh, as in the reference.
+// halved by the reference (observer.cc:39-41,70-72).  Owned partners are at
+// their current positions, image partners at the last halo refresh, as the
+// reference's ghosts are.
 template <int W>
-__global__ __launch_bounds__(kWave *W) void k_observe_tile(const double4 *__restrict__ pos,
-                                                         const double4 *__restrict__ mom, TileList tl, int pn,
-                                                         int tn, double CL2, double C2, double C0,
-                                                         double *__restrict__ partial) {
+__global__ __launch_bounds__(kWave *W) void k_observe_brick(const double4 *__restrict__ pos,
+                                                          const double4 *__restrict__ gpos,
+                                                          const double4 *__restrict__ mom, BrickList bl,
+                                                          double CL2, double C2, double C0,
+                                                          double *__restrict__ partial) {
   extern __shared__ double lds64[];
+  __shared__ BrickShared sh;
   __shared__ double red[3][W];
   constexpr int T = kWave * W;
-  const int t = xcd_block(blockIdx.x, gridDim.x);
-  const int i = t * T + threadIdx.x;
-  const int s = i / kWave, lane = threadIdx.x % kWave;
-  const int *ts = tl.tstart + (long long)t * kTileRanges;
-  const int *tb = tl.tbase + (long long)t * (kTileRanges + 1);
-  const int total = tb[kTileRanges];
-  const bool staged = total < tl.lds_cap;
-  double *lx = lds64, *ly = lds64 + tl.lds_cap, *lz = lds64 + 2 * tl.lds_cap;
-  if (staged) stage_fp64<T>(pos, ts, tb, total, lx, ly, lz);
+  const int b = xcd_block(blockIdx.x, gridDim.x);
+  const int R = load_bases<T>(bl, b, sh.base);
   __syncthreads();
+  const int *ts = bl.tstart + (long long)b * bl.rstride;
+  const int total = sh.base[R];
+  const bool staged = total < bl.lds_cap;
+  double *lx = lds64, *ly = lds64 + bl.lds_cap, *lz = lds64 + 2 * bl.lds_cap;
+  if (staged) {
+    for (int u = threadIdx.x; u < total; u += T) {
+      const double4 q = staged_load(u, R, sh.base, ts, pos, gpos);
+      lx[u] = q.x;
+      ly[u] = q.y;
+      lz[u] = q.z;
+    }
+  }
+  __syncthreads();
+  const int nrow = bl.nrow[b], row0 = bl.row0[b], vs0 = bl.vslice0[b];
+  const int nsl = (nrow + kWave - 1) / kWave;
+  const int wv = threadIdx.x / kWave, lane = threadIdx.x % kWave;
   double ke = 0.0, pe = 0.0, vir = 0.0;
-  if (i < pn) {
+  for (int sl = wv; sl < nsl; sl += W) {
+    const int v = sl * kWave + lane;
+    if (v >= nrow) continue;
+    const int s = vs0 + sl;
+    const int i = row0 + v;
     const double4 p = mom[i];
-    ke = 0.5 * p.x * p.x + 0.5 * p.y * p.y + 0.5 * p.z * p.z + 0.5 * p.w * p.w;
+    ke += 0.5 * p.x * p.x + 0.5 * p.y * p.y + 0.5 * p.z * p.z + 0.5 * p.w * p.w;
     const double4 qi = pos[i];
-    const int len = tl.slen[s];
-    const uint16_t *row = tl.nbr + tl.sbase[s] + lane;
+    const int len = bl.slen[s];
+    const uint16_t *row = bl.nbr + bl.sbase[s] + lane;
     for (int k = 0; k < len; k++) {
       const int e = row[(long long)k * kWave];
-      double xj, yj, zj;
-      if (staged) {
-        xj = lx[e];
-        yj = ly[e];
-        zj = lz[e];
-      } else {
-        const double4 q = pos[tile_global(e, ts, tb, total, tn)];
-        xj = q.x;
-        yj = q.y;
-        zj = q.z;
-      }
-      const double dx = qi.x - xj, dy = qi.y - yj, dz = qi.z - zj;
+      if (e >= total) continue;  // padding
+      double4 qj;
+      if (staged) qj = make_double4(lx[e], ly[e], lz[e], 0.0);
+      else qj = staged_load(e, R, sh.base, ts, pos, gpos);
+      const double dx = qi.x - qj.x, dy = qi.y - qj.y, dz = qi.z - qj.z;
       const double r2 = (dx * dx + dy * dy + dz * dz);
       if (r2 > CL2) continue;
       const double r6 = r2 * r2 * r2;
@@ -329,11 +379,10 @@ __global__ __launch_bounds__(kWave *W) void k_observe_tile(const double4 *__rest
   ke = wave_sum(ke);
   pe = wave_sum(pe);
   vir = wave_sum(vir);
-  const int w = threadIdx.x / kWave;
   if (lane == 0) {
-    red[0][w] = ke;
-    red[1][w] = pe;
-    red[2][w] = vir;
+    red[0][wv] = ke;
+    red[1][wv] = pe;
+    red[2][wv] = vir;
   }
   __syncthreads();
   if (threadIdx.x < 3) {

@@ -16,32 +16,27 @@
 
 #include "mdgpu.h"
 #include "build_kernels.hpp"
+#include "force_kernels.hpp"
+#include "list_kernels.hpp"
 #include "scan.hpp"
 #include "step_kernels.hpp"
-#include "force_kernels.hpp"
 
 using namespace mdgpu;
-
-namespace {
-
-struct Buf {
-  void *p = nullptr;
-  size_t bytes = 0;
-};
-
-}  // namespace
 
 struct mdgpu_domain {
   mdgpu_params prm{};
   hipStream_t stream = nullptr;
+  int device = 0;
   std::string err;
   int precision = MDGPU_FP64;
-  int kahan = 1;   // compensated FP32 sums in the mixed kernel
-  int unroll = 8;      // partner entries per batch in the force kernels (4 or 8)
-  int tile_waves = 2;  // wavefronts (64 rows each) per tile / workgroup: 1, 2 or 4
-  int lds_kb = 60;     // LDS budget of one tile's staged partners, KiB
-  int lds_max_kb = 60; // device limit per workgroup less the kernels' static LDS
-  int ntile = 0, max_staged = 0;
+
+  // tuning
+  int kahan = 1;          // compensated FP32 sums in the mixed kernel
+  int unroll = 8;         // partner entries per batch in the force kernels (4 or 8)
+  int waves = 8;          // wavefronts per brick workgroup: 2, 4 or 8
+  int bd[3] = {4, 4, 4};  // cells per brick
+  int lds_kb = 96;        // LDS budget for one brick's staged partners, KiB
+  int lds_max_kb = 60;    // device limit per workgroup less the kernels' static LDS
 
   // SimulationInfo / MDRect / MeshList geometry
   double L[3], rs[3], re[3];
@@ -49,6 +44,9 @@ struct mdgpu_domain {
   double msize[3];
   int m[3], ncell = 0;
   int check_list_length = 500;
+  // bricks
+  int nb[3], nbrick = 0, rmax = 0;
+  std::vector<int> h_brick_of_cell;
 
   // thermostat
   int thermostat = MDGPU_NVE;
@@ -65,36 +63,44 @@ struct mdgpu_domain {
   // counts and capacities
   int pn = 0, ng = 0, tn = 0;
   int cap = 0, gcap = 0;
-  int nslice = 0;
+  int nslice = 0, slice_cap = 0, max_staged = 0;
   long long nbr_slots = 0, nbr_entries = 0, nbr_cap = 0;
 
-  // device state
-  double4 *pos = nullptr, *pos_tmp = nullptr, *pos_old = nullptr, *mom = nullptr, *mom_alt = nullptr;
-  float4 *rel = nullptr, *rel_tmp = nullptr;
+  // device state: positions as double4 (x, y, z, 0); owned [0, pn), ghosts [pn, tn)
+  double4 *pos = nullptr;      // current owned positions; ghosts of the last halo refresh
+  double4 *pos_tmp = nullptr;  // half-drifted owned positions inside the fused step; build scratch
+  double4 *pos_old = nullptr;  // PairList::Init snapshot
+  double4 *mom = nullptr, *mom_alt = nullptr;
+  float4 *rel = nullptr, *rel_tmp = nullptr;  // cell-relative FP32 copies (owned + ghosts)
   int *id = nullptr, *id_alt = nullptr, *cell = nullptr, *cell_alt = nullptr;
   int *gsrc = nullptr, *gcode = nullptr;
   int *c_ocount = nullptr, *c_ostart = nullptr, *c_gcount = nullptr, *c_gstart = nullptr;
+  int *count_r = nullptr, *start_r = nullptr;
   double *org = nullptr;  // origins of the cell columns, x then y then z
+  // bricks
+  int *rank = nullptr, *brick_of_cell = nullptr, *brick_rank0 = nullptr;
+  int *row0 = nullptr, *nrow = nullptr, *nsl = nullptr, *vslice0 = nullptr, *slice_brick = nullptr;
+  int *tstart = nullptr, *tbase = nullptr, *nrange = nullptr;
+  // list
   uint16_t *nbr = nullptr;   // staged partner index per list slot
   uint8_t *ncode = nullptr;  // partner cell offset per list slot
-  int *tstart = nullptr, *tbase = nullptr;
-  int tile_cap = 0;
-  long long *sbase = nullptr;
+  long long *sbase = nullptr, *slots64 = nullptr;
   int *slen = nullptr, *row_len = nullptr;
   // build scratch
   int *slot = nullptr, *perm = nullptr, *nimg = nullptr, *goff = nullptr;
   int *t_gsrc = nullptr, *t_gcode = nullptr, *t_gcell = nullptr;
-  long long *key = nullptr, *scan_scratch = nullptr, *slots64 = nullptr;
+  long long *key = nullptr, *scan_scratch = nullptr;
   long long scan_scratch_n = 0;
   double *disp = nullptr, *partial = nullptr, *maxpart = nullptr;
+  double *bpart = nullptr;  // one maximum per brick workgroup
   int partial_n = 0;
   int *cand = nullptr;
-  // small device scalars: [0] vmax2 bits, [1] dmax2 bits (u64); ints: err, count, flag
+  // small device scalars: u64 [0] vmax2 [1] dmax2 [2] max staged; i32 [0] err [1] count [2] flag
   unsigned long long *u64 = nullptr;
   long long *i64 = nullptr;
   int *i32 = nullptr;
   double *d3 = nullptr;
-  // pinned host mirror of the small scalars
+  // pinned host mirrors
   unsigned long long *h_u64 = nullptr;
   long long *h_i64 = nullptr;
   int *h_i32 = nullptr;
@@ -118,9 +124,9 @@ namespace {
     }                                                                                     \
   } while (0)
 
-#define CHK(call)               \
-  do {                          \
-    int r_ = (call);            \
+#define CHK(call)                  \
+  do {                             \
+    int r_ = (call);               \
     if (r_ != MDGPU_OK) return r_; \
   } while (0)
 
@@ -129,7 +135,7 @@ inline unsigned nblk(long long n, int b = kBlock) { return (unsigned)std::max<lo
 template <class T>
 int dalloc(mdgpu_domain *d, T **p, size_t n) {
   if (*p) {
-    hipFree(*p);
+    (void)hipFree(*p);
     *p = nullptr;
   }
   if (n == 0) n = 1;
@@ -138,15 +144,14 @@ int dalloc(mdgpu_domain *d, T **p, size_t n) {
 }
 
 template <class T>
-int dgrow_copy(mdgpu_domain *d, T **p, size_t old_n, size_t keep, size_t new_n) {
+int dgrow_copy(mdgpu_domain *d, T **p, size_t keep, size_t new_n) {
   T *np = nullptr;
   HIPCHK(d, hipMalloc((void **)&np, std::max<size_t>(1, new_n) * sizeof(T)));
   if (*p && keep) HIPCHK(d, hipMemcpyAsync(np, *p, keep * sizeof(T), hipMemcpyDeviceToDevice, d->stream));
   if (*p) {
     HIPCHK(d, hipStreamSynchronize(d->stream));
-    hipFree(*p);
+    (void)hipFree(*p);
   }
-  (void)old_n;
   *p = np;
   return MDGPU_OK;
 }
@@ -165,14 +170,14 @@ int check_launch(mdgpu_domain *d, const char *what) {
   return MDGPU_OK;
 }
 
-// MeshList::ChangeScale (meshlist.cc:95-120) and the column origins used by
-// the cell-relative FP32 coordinates.
+// MeshList::ChangeScale (meshlist.cc:95-120), the column origins of the
+// cell-relative FP32 coordinates, and the brick order of the cells.
 int set_geometry(mdgpu_domain *d) {
   for (int k = 0; k < 3; k++) {
     const double w = d->re[k] - d->rs[k];
     const int ms = (int)(w / d->SL);
-    if (ms < 1) {
-      d->err = "box edge shorter than the search length";
+    if (ms < 2) {
+      d->err = "box edge shorter than two search lengths (rc + margin)";
       return MDGPU_ERR_ARGUMENT;
     }
     d->msize[k] = w / (double)ms;
@@ -188,12 +193,50 @@ int set_geometry(mdgpu_domain *d) {
   CHK(dalloc(d, &d->c_ostart, d->ncell));
   CHK(dalloc(d, &d->c_gcount, d->ncell));
   CHK(dalloc(d, &d->c_gstart, d->ncell));
+  CHK(dalloc(d, &d->count_r, d->ncell + 1));
+  CHK(dalloc(d, &d->start_r, d->ncell + 1));
   std::vector<double> org(d->m[0] + d->m[1] + d->m[2]);
   int o = 0;
   for (int k = 0; k < 3; k++)
     for (int c = 0; c < d->m[k]; c++) org[o++] = d->rs[k] + (double)(c - 1) * d->msize[k];
   CHK(dalloc(d, &d->org, org.size()));
   HIPCHK(d, hipMemcpy(d->org, org.data(), org.size() * sizeof(double), hipMemcpyHostToDevice));
+  // bricks: bricks z-major, cells z, y, x inside each brick
+  for (int k = 0; k < 3; k++) d->nb[k] = (d->m[k] + d->bd[k] - 1) / d->bd[k];
+  d->nbrick = d->nb[0] * d->nb[1] * d->nb[2];
+  d->rmax = (d->bd[0] + 2) * (d->bd[1] + 2) * (d->bd[2] + 2);
+  std::vector<int> rank(d->ncell), boc(d->ncell), rank0(d->nbrick + 1);
+  int r = 0;
+  for (int bz = 0; bz < d->nb[2]; bz++)
+    for (int by = 0; by < d->nb[1]; by++)
+      for (int bx = 0; bx < d->nb[0]; bx++) {
+        const int b = (bz * d->nb[1] + by) * d->nb[0] + bx;
+        rank0[b] = r;
+        for (int z = bz * d->bd[2]; z < std::min(d->m[2], (bz + 1) * d->bd[2]); z++)
+          for (int y = by * d->bd[1]; y < std::min(d->m[1], (by + 1) * d->bd[1]); y++)
+            for (int x = bx * d->bd[0]; x < std::min(d->m[0], (bx + 1) * d->bd[0]); x++) {
+              const int c = (z * d->m[1] + y) * d->m[0] + x;
+              rank[c] = r++;
+              boc[c] = b;
+            }
+      }
+  rank0[d->nbrick] = r;
+  d->h_brick_of_cell = boc;
+  CHK(dalloc(d, &d->rank, d->ncell));
+  CHK(dalloc(d, &d->brick_of_cell, d->ncell));
+  CHK(dalloc(d, &d->brick_rank0, d->nbrick + 1));
+  HIPCHK(d, hipMemcpy(d->rank, rank.data(), rank.size() * sizeof(int), hipMemcpyHostToDevice));
+  HIPCHK(d, hipMemcpy(d->brick_of_cell, boc.data(), boc.size() * sizeof(int), hipMemcpyHostToDevice));
+  HIPCHK(d, hipMemcpy(d->brick_rank0, rank0.data(), rank0.size() * sizeof(int), hipMemcpyHostToDevice));
+  CHK(dalloc(d, &d->row0, d->nbrick));
+  CHK(dalloc(d, &d->nrow, d->nbrick));
+  CHK(dalloc(d, &d->nsl, d->nbrick));
+  CHK(dalloc(d, &d->vslice0, d->nbrick));
+  CHK(dalloc(d, &d->nrange, d->nbrick));
+  CHK(dalloc(d, &d->bpart, d->nbrick));
+  CHK(dalloc(d, &d->tstart, (size_t)d->nbrick * kRangesPerCell * d->rmax));
+  CHK(dalloc(d, &d->tbase, (size_t)d->nbrick * (kRangesPerCell * d->rmax + 1)));
+  d->initialized = false;
   return MDGPU_OK;
 }
 
@@ -217,6 +260,17 @@ Origins origins_of(const mdgpu_domain *d) {
   return O;
 }
 
+Bricks bricks_of(const mdgpu_domain *d) {
+  Bricks B;
+  for (int k = 0; k < 3; k++) {
+    B.bd[k] = d->bd[k];
+    B.nb[k] = d->nb[k];
+    B.m[k] = d->m[k];
+  }
+  B.rmax = d->rmax;
+  return B;
+}
+
 // capacity for owned particles (arrays indexed by owned index)
 int ensure_owned(mdgpu_domain *d, int need) {
   if (need <= d->cap) return MDGPU_OK;
@@ -226,10 +280,10 @@ int ensure_owned(mdgpu_domain *d, int need) {
   }
   int nc = std::max(need, std::max(1024, d->cap + d->cap / 4));
   nc = std::min(nc, kMaxParticles);
-  const size_t tc_old = (size_t)d->cap + d->gcap + 1, tc_new = (size_t)nc + d->gcap + 1;
-  CHK(dgrow_copy(d, &d->pos, tc_old, d->pn, tc_new));
-  CHK(dgrow_copy(d, &d->mom, d->cap, d->pn, nc));
-  CHK(dgrow_copy(d, &d->id, d->cap, d->pn, nc));
+  const size_t tc_new = (size_t)nc + d->gcap + 1;
+  CHK(dgrow_copy(d, &d->pos, d->pn, tc_new));
+  CHK(dgrow_copy(d, &d->mom, d->pn, nc));
+  CHK(dgrow_copy(d, &d->id, d->pn, nc));
   CHK(dalloc(d, &d->pos_tmp, tc_new));
   CHK(dalloc(d, &d->pos_old, tc_new));
   CHK(dalloc(d, &d->rel, tc_new));
@@ -246,10 +300,6 @@ int ensure_owned(mdgpu_domain *d, int need) {
   CHK(dalloc(d, &d->nimg, nc));
   CHK(dalloc(d, &d->goff, nc));
   CHK(dalloc(d, &d->row_len, nc));
-  const int ns = (nc + kWave - 1) / kWave;
-  CHK(dalloc(d, &d->slen, ns));
-  CHK(dalloc(d, &d->sbase, ns));
-  CHK(dalloc(d, &d->slots64, ns));
   d->cap = nc;
   return MDGPU_OK;
 }
@@ -262,16 +312,16 @@ int ensure_ghosts(mdgpu_domain *d, int need) {
     return MDGPU_ERR_CAPACITY;
   }
   const int ngc = std::max(need, d->gcap + d->gcap / 4);
-  const size_t tc_old = (size_t)d->cap + d->gcap + 1, tc_new = (size_t)d->cap + ngc + 1;
-  CHK(dgrow_copy(d, &d->pos, tc_old, d->pn, tc_new));
+  const size_t tc_new = (size_t)d->cap + ngc + 1;
+  CHK(dgrow_copy(d, &d->pos, d->pn, tc_new));
+  CHK(dgrow_copy(d, &d->cell, d->pn, tc_new));
   CHK(dalloc(d, &d->pos_tmp, tc_new));
   CHK(dalloc(d, &d->pos_old, tc_new));
   CHK(dalloc(d, &d->rel, tc_new));
   CHK(dalloc(d, &d->rel_tmp, tc_new));
-  CHK(dgrow_copy(d, &d->cell, tc_old, d->pn, tc_new));
   CHK(dalloc(d, &d->cell_alt, tc_new));
   CHK(dalloc(d, &d->disp, tc_new));
-  CHK(dalloc(d, &d->maxpart, nblk((long long)tc_new, kWave)));  // one per workgroup of >= 64 threads
+  CHK(dalloc(d, &d->maxpart, nblk((long long)tc_new, kWave)));
   CHK(dalloc(d, &d->slot, tc_new));
   CHK(dalloc(d, &d->perm, tc_new));
   CHK(dalloc(d, &d->key, tc_new));
@@ -311,6 +361,10 @@ int read_err_flags(mdgpu_domain *d) {
     d->err = "Invalid index";
     return MDGPU_ERR_INVALID_INDEX;
   }
+  if (e & kErrLayout) {
+    d->err = "internal: a listed partner lies outside its brick's staged ranges";
+    return MDGPU_ERR_HIP;
+  }
   if (e & kErrCapacity) {
     d->err = "pair list capacity";
     return MDGPU_ERR_CAPACITY;
@@ -325,12 +379,31 @@ int read_total(mdgpu_domain *d, long long *v) {
   return MDGPU_OK;
 }
 
-// Stable-by-key counting sort of n items already binned (cell[], slot[] from
-// atomics, counts[] per cell): starts[] = scan(counts), perm[] = new order.
-int cell_sort(mdgpu_domain *d, const int *cellv, int n, int *counts, int *starts, const long long *keys) {
+// a double stored as bits in u64[k]
+int read_max(mdgpu_domain *d, int k, double *v) {
+  HIPCHK(d, hipMemcpyAsync(d->h_u64 + k, d->u64 + k, sizeof(unsigned long long), hipMemcpyDeviceToHost,
+                           d->stream));
+  CHK(sync(d));
+  std::memcpy(v, d->h_u64 + k, sizeof(double));
+  return MDGPU_OK;
+}
+
+// Order n binned items (cellv[], slot[] from atomics, counts[] per cell):
+// starts[] over cells (in brick rank order when `ranked`), perm[] = new order,
+// members of a cell ordered by keys[].
+int cell_sort(mdgpu_domain *d, const int *cellv, int n, int *counts, int *starts, const long long *keys,
+              bool ranked) {
   hipStream_t st = d->stream;
   CHK(ensure_scan(d, d->ncell));
-  exclusive_scan<int, int>(counts, starts, d->ncell, d->scan_scratch, nullptr, st);
+  if (ranked) {
+    hipLaunchKernelGGL(k_rank_counts, dim3(nblk(d->ncell)), dim3(kBlock), 0, st, counts, d->rank, d->ncell,
+                       d->count_r);
+    exclusive_scan<int, int>(d->count_r, d->start_r, d->ncell, d->scan_scratch, nullptr, st);
+    hipLaunchKernelGGL(k_unrank_starts, dim3(nblk(d->ncell)), dim3(kBlock), 0, st, d->start_r, d->rank, d->ncell,
+                       starts);
+  } else {
+    exclusive_scan<int, int>(counts, starts, d->ncell, d->scan_scratch, nullptr, st);
+  }
   hipLaunchKernelGGL(k_scatter_perm, dim3(nblk(n)), dim3(kBlock), 0, st, cellv, d->slot, starts, n, d->perm);
   hipLaunchKernelGGL(k_sort_cells, dim3(nblk(d->ncell)), dim3(kBlock), 0, st, d->perm, starts, counts, keys,
                      d->ncell);
@@ -340,7 +413,7 @@ int cell_sort(mdgpu_domain *d, const int *cellv, int n, int *counts, int *starts
 // MDManager::MakePairList for the single domain (mdmanager.cc:261-280).
 // The reference also permutes particles that cross a face
 // (MakeBufferForSendingParticle, mdunit.cc:155-179); that only renames
-// indices, and particles are put in cell order here anyway.
+// indices, and particles are put in brick order here anyway.
 int make_pair_list(mdgpu_domain *d) {
   hipStream_t st = d->stream;
   const Grid g = grid_of(d);
@@ -348,7 +421,7 @@ int make_pair_list(mdgpu_domain *d) {
   d->ng = 0;
   d->tn = pn;
   HIPCHK(d, hipMemsetAsync(d->i32, 0, sizeof(int) * 4, st));
-  // 1. wrap (AdjustPeriodicBoundary) and bin the owned particles
+  // 1. wrap (AdjustPeriodicBoundary) and bin the owned particles, brick order
   HIPCHK(d, hipMemsetAsync(d->c_ocount, 0, sizeof(int) * d->ncell, st));
   if (pn > 0) {
     hipLaunchKernelGGL(k_wrap_bin, dim3(nblk(pn)), dim3(kBlock), 0, st, d->pos, pn, d->L[0], d->L[1], d->L[2],
@@ -356,7 +429,7 @@ int make_pair_list(mdgpu_domain *d) {
     hipLaunchKernelGGL(k_owned_key, dim3(nblk(pn)), dim3(kBlock), 0, st, d->id, pn, d->key);
   }
   CHK(read_err_flags(d));
-  CHK(cell_sort(d, d->cell, pn, d->c_ocount, d->c_ostart, d->key));
+  CHK(cell_sort(d, d->cell, pn, d->c_ocount, d->c_ostart, d->key, true));
   if (pn > 0) {
     hipLaunchKernelGGL(k_gather_owned, dim3(nblk(pn)), dim3(kBlock), 0, st, d->perm, pn, d->pos, d->mom, d->id,
                        d->cell, d->pos_tmp, d->mom_alt, d->id_alt, d->cell_alt);
@@ -388,62 +461,63 @@ int make_pair_list(mdgpu_domain *d) {
   CHK(ensure_ghosts(d, ng));
   HIPCHK(d, hipMemsetAsync(d->c_gcount, 0, sizeof(int) * d->ncell, st));
   if (ng > 0) {
-    // pos_tmp[0..ng) and cell_alt[] hold the unsorted images
+    // pos_tmp[0..ng) holds the unsorted images
     hipLaunchKernelGGL(k_emit_images, dim3(nblk(pn)), dim3(kBlock), 0, st, d->pos, pn, E, d->L[0], d->L[1],
                        d->L[2], g, d->goff, ng, d->t_gsrc, d->t_gcode, d->pos_tmp, d->t_gcell, d->slot,
                        d->c_gcount, d->key, d->id);
     CHK(check_launch(d, "emit images"));
-    CHK(cell_sort(d, d->t_gcell, ng, d->c_gcount, d->c_gstart, d->key));
+    CHK(cell_sort(d, d->t_gcell, ng, d->c_gcount, d->c_gstart, d->key, false));
     hipLaunchKernelGGL(k_gather_ghosts, dim3(nblk(ng)), dim3(kBlock), 0, st, d->perm, ng, pn, d->t_gsrc,
                        d->t_gcode, d->pos_tmp, d->t_gcell, d->gsrc, d->gcode, d->pos, d->cell);
   } else {
-    CHK(ensure_scan(d, d->ncell));
     exclusive_scan<int, int>(d->c_gcount, d->c_gstart, d->ncell, d->scan_scratch, nullptr, st);
   }
   d->ng = ng;
   d->tn = pn + ng;
-  hipLaunchKernelGGL(k_set_sentinel, dim3(1), dim3(1), 0, st, d->pos, d->pos_tmp, d->rel, d->rel_tmp, d->tn);
   // 3. PairList::Init: snapshot of owned + ghost positions
   if (d->tn > 0)
     hipLaunchKernelGGL(k_copy_pos, dim3(nblk(d->tn)), dim3(kBlock), 0, st, d->pos, d->pos_old, d->tn);
-  // 4. MeshList::MakeList: full neighbour list in sliced-ELL layout, partners
-  //    as indices into their tile's staged ranges
+  // 4. bricks: rows, slices, staged ranges
+  const Bricks B = bricks_of(d);
   const CellTables T{d->c_ostart, d->c_ocount, d->c_gstart, d->c_gcount};
-  d->nslice = (pn + kWave - 1) / kWave;
-  const int rows_per_tile = kWave * d->tile_waves;
-  d->ntile = (pn + rows_per_tile - 1) / rows_per_tile;
-  if (d->ntile > d->tile_cap) {
-    const int tc = d->ntile + d->ntile / 4 + 16;
-    CHK(dalloc(d, &d->tstart, (size_t)tc * kTileRanges));
-    CHK(dalloc(d, &d->tbase, (size_t)tc * (kTileRanges + 1)));
-    d->tile_cap = tc;
+  hipLaunchKernelGGL(k_brick_rows, dim3(nblk(d->nbrick)), dim3(kBlock), 0, st, d->brick_rank0, d->start_r,
+                     d->nbrick, d->ncell, pn, d->row0, d->nrow, d->nsl);
+  CHK(ensure_scan(d, d->nbrick));
+  exclusive_scan<int, int>(d->nsl, d->vslice0, d->nbrick, d->scan_scratch, d->i64, st);
+  long long nsl_total = 0;
+  CHK(read_total(d, &nsl_total));
+  d->nslice = (int)nsl_total;
+  if (d->nslice + 1 > d->slice_cap) {
+    const int sc = d->nslice + d->nslice / 4 + 64;
+    CHK(dalloc(d, &d->slen, sc));
+    CHK(dalloc(d, &d->sbase, sc));
+    CHK(dalloc(d, &d->slots64, sc));
+    CHK(dalloc(d, &d->slice_brick, sc));
+    d->slice_cap = sc;
   }
-  d->max_staged = 0;
-  if (pn > 0) {
-    const unsigned nb = nblk(d->ntile);
-    hipLaunchKernelGGL(k_tile_ranges, dim3(nb), dim3(kBlock), 0, st, d->cell, pn, rows_per_tile, d->ntile, g, T,
-                       d->tstart, d->tbase, d->maxpart);
-    hipLaunchKernelGGL(k_reduce_max, dim3(1), dim3(kBlock), 0, st, d->maxpart, (int)nb, d->u64 + 2);
-    HIPCHK(d, hipMemcpyAsync(d->h_u64 + 2, d->u64 + 2, sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
-    CHK(sync(d));
-    double m;
-    std::memcpy(&m, d->h_u64 + 2, sizeof(m));
-    d->max_staged = (int)m;
-    if (d->max_staged > 65534) {
-      d->err = "a tile's partner set exceeds 16-bit indexing (extreme local density)";
-      return MDGPU_ERR_CAPACITY;
-    }
+  hipLaunchKernelGGL(k_slice_brick, dim3(nblk(d->nbrick)), dim3(kBlock), 0, st, d->vslice0, d->nsl, d->nbrick,
+                     d->slice_brick);
+  hipLaunchKernelGGL(k_brick_ranges, dim3(nblk(d->nbrick)), dim3(kBlock), 0, st, B, T, d->nbrick, pn, d->tstart,
+                     d->tbase, d->nrange, d->bpart);
+  hipLaunchKernelGGL(k_reduce_max, dim3(1), dim3(kBlock), 0, st, d->bpart, (int)nblk(d->nbrick), d->u64 + 2);
+  double mstaged = 0.0;
+  CHK(read_max(d, 2, &mstaged));
+  d->max_staged = (int)mstaged;
+  if (d->max_staged > 65534) {
+    d->err = "a brick's neighbourhood exceeds 16-bit indexing (extreme local density): use smaller bricks";
+    return MDGPU_ERR_CAPACITY;
   }
+  // 5. MeshList::MakeList: full neighbour list in sliced-ELL layout
   long long slots = 0;
   if (pn > 0) {
     hipLaunchKernelGGL(k_count_rows, dim3(nblk(pn)), dim3(kBlock), 0, st, d->pos, d->cell, pn, g, T, d->S2,
                        d->row_len);
-    hipLaunchKernelGGL(k_slice_len, dim3(nblk(d->nslice)), dim3(kBlock), 0, st, d->row_len, pn, d->nslice,
-                       d->slen, d->slots64);
     // number of listed (row, partner) entries, for accounting
     exclusive_scan<int, long long>(d->row_len, d->key, pn, d->scan_scratch, d->i64, st);
     CHK(read_total(d, &d->nbr_entries));
-    // slice offsets in entries: scan of 64 * slice length
+    hipLaunchKernelGGL(k_slice_len, dim3(nblk(d->nslice)), dim3(kBlock), 0, st, d->row_len, d->slice_brick,
+                       d->vslice0, d->row0, d->nrow, d->nslice, d->slen, d->slots64);
+    CHK(ensure_scan(d, d->nslice));
     exclusive_scan<long long, long long>(d->slots64, d->sbase, d->nslice, d->scan_scratch, d->i64, st);
     CHK(read_total(d, &slots));
   } else {
@@ -457,10 +531,12 @@ int make_pair_list(mdgpu_domain *d) {
   }
   d->nbr_slots = slots;
   if (pn > 0) {
-    hipLaunchKernelGGL(k_fill_rows, dim3(nblk(pn)), dim3(kBlock), 0, st, d->pos, d->cell, pn, g, T, d->S2,
-                       d->sbase, d->slen, rows_per_tile, d->tstart, d->tbase, d->nbr, d->ncode, d->i32);
-    hipLaunchKernelGGL(k_fill_tail, dim3(1), dim3(kWave), 0, st, pn, d->nslice, rows_per_tile, d->sbase, d->slen,
-                       d->tbase, d->nbr, d->ncode);
+    const BrickRows BR{d->brick_of_cell, d->row0, d->vslice0, d->tstart, d->tbase};
+    hipLaunchKernelGGL(k_fill_rows, dim3(nblk(pn)), dim3(kBlock), 0, st, d->pos, d->cell, pn, g, T, d->S2, B, BR,
+                       d->sbase, d->slen, d->nbr, d->ncode, d->i32);
+    hipLaunchKernelGGL(k_fill_tail, dim3(nblk((long long)d->nslice * kWave)), dim3(kBlock), 0, st, d->nslice, B,
+                       d->slice_brick, d->vslice0, d->nrow, d->nrange, d->tbase, d->sbase, d->slen, d->nbr,
+                       d->ncode);
   }
   CHK(check_launch(d, "neighbour list"));
   CHK(read_err_flags(d));
@@ -477,13 +553,7 @@ int max_p2(mdgpu_domain *d, double *v2) {
     hipLaunchKernelGGL(k_max_p2, dim3(nb), dim3(kBlock), 0, d->stream, d->mom, d->pn, d->maxpart);
     hipLaunchKernelGGL(k_reduce_max, dim3(1), dim3(kBlock), 0, d->stream, d->maxpart, (int)nb, d->u64);
   }
-  HIPCHK(d, hipMemcpyAsync(d->h_u64, d->u64, sizeof(unsigned long long), hipMemcpyDeviceToHost, d->stream));
-  CHK(sync(d));
-  unsigned long long b = d->h_u64[0];
-  double v;
-  std::memcpy(&v, &b, sizeof(v));
-  *v2 = v;
-  return MDGPU_OK;
+  return read_max(d, 0, v2);
 }
 
 // PairList::CheckByDisplacement (pairlist.cc:100-154)
@@ -494,10 +564,8 @@ int check_by_displacement(mdgpu_domain *d, int *expired) {
   hipLaunchKernelGGL(k_displacement, dim3(nblk(tn)), dim3(kBlock), 0, st, d->pos, d->pos_old, tn, d->disp,
                      d->maxpart);
   hipLaunchKernelGGL(k_reduce_max, dim3(1), dim3(kBlock), 0, st, d->maxpart, (int)nblk(tn), d->u64 + 1);
-  HIPCHK(d, hipMemcpyAsync(d->h_u64 + 1, d->u64 + 1, sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
-  CHK(sync(d));
   double dr2_max;
-  std::memcpy(&dr2_max, d->h_u64 + 1, sizeof(double));
+  CHK(read_max(d, 1, &dr2_max));
   const int Nl = (tn > d->check_list_length) ? d->check_list_length : tn;
   const double c_distance = d->margin - std::sqrt(dr2_max);
   const double c_d2 = c_distance * c_distance;
@@ -579,84 +647,92 @@ LangevinConsts lconsts(const mdgpu_domain *d) {
   return lc;
 }
 
-// One pair-force launch.  `src`/`rel_src` are the positions the forces are
-// evaluated at; `pos_out` receives the drifted (or copied) owned positions.
-// The list as the tile kernels read it; lds_cap = staged entries per tile
-// that fit the LDS budget (tiles with more read their partners from global
-// memory).
-TileList tile_list(const mdgpu_domain *d, int bytes_per_entry) {
-  TileList tl;
-  tl.nbr = d->nbr;
-  tl.code = d->ncode;
-  tl.sbase = d->sbase;
-  tl.slen = d->slen;
-  tl.tstart = d->tstart;
-  tl.tbase = d->tbase;
-  tl.nslice = d->nslice;
-  const int budget = d->lds_kb * 1024 / bytes_per_entry;
-  tl.lds_cap = std::max(1, std::min(d->max_staged + 1, budget));
-  return tl;
+// The list as the brick kernels read it; lds_cap = staged entries per brick
+// that fit the LDS budget (bricks with more read partners from global memory).
+BrickList brick_list(const mdgpu_domain *d, int bytes_per_entry) {
+  BrickList bl;
+  bl.nbr = d->nbr;
+  bl.code = d->ncode;
+  bl.sbase = d->sbase;
+  bl.slen = d->slen;
+  bl.tstart = d->tstart;
+  bl.tbase = d->tbase;
+  bl.nrange = d->nrange;
+  bl.row0 = d->row0;
+  bl.nrow = d->nrow;
+  bl.vslice0 = d->vslice0;
+  bl.rstride = kRangesPerCell * d->rmax;
+  const int budget = std::min(d->lds_kb, d->lds_max_kb) * 1024 / bytes_per_entry;
+  bl.lds_cap = std::max(1, std::min(d->max_staged + 1, budget));
+  return bl;
+}
+
+template <class K>
+void allow_lds(K kernel, size_t bytes) {
+  if (bytes > 64 * 1024)
+    (void)hipFuncSetAttribute((const void *)kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
 }
 
 template <int W, int U>
-void launch_fp64(mdgpu_domain *d, const double4 *src, double4 *pos_out, const TileList &tl, int epi,
+void launch_fp64(mdgpu_domain *d, const double4 *src, const double4 *gsrc, double4 *pos_out, int epi,
                  const ForceConsts &fc, const LangevinConsts &lc, double *vb) {
-  const size_t shm = (size_t)tl.lds_cap * 3 * sizeof(double);
-  hipLaunchKernelGGL((k_force_tile_fp64<W, U>), dim3(d->ntile), dim3(kWave * W), shm, d->stream, src, d->mom,
-                     pos_out, tl, d->pn, d->tn, fc, epi, lc, d->id, vb);
+  const BrickList bl = brick_list(d, 3 * sizeof(double));
+  const size_t shm = (size_t)bl.lds_cap * 3 * sizeof(double);
+  allow_lds(k_force_brick_fp64<W, U>, shm);
+  hipLaunchKernelGGL((k_force_brick_fp64<W, U>), dim3(d->nbrick), dim3(kWave * W), shm, d->stream, src, gsrc,
+                     d->mom, pos_out, bl, fc, epi, lc, d->id, vb);
 }
 
 template <int W, int U, bool K>
-void launch_mixed(mdgpu_domain *d, const double4 *src, const float4 *rel_src, double4 *pos_out,
-                  const TileList &tl, int epi, const ForceConstsF &f32, double dt2, const LangevinConsts &lc,
-                  double *vb) {
-  const size_t shm = (size_t)tl.lds_cap * sizeof(float4);
-  hipLaunchKernelGGL((k_force_tile_mixed<W, U, K>), dim3(d->ntile), dim3(kWave * W), shm, d->stream, rel_src, src,
-                     d->mom, pos_out, tl, d->pn, d->tn, f32, dt2, epi, lc, d->id, vb);
+void launch_mixed(mdgpu_domain *d, const double4 *src, const float4 *rel, const float4 *grel, double4 *pos_out,
+                  int epi, const ForceConstsF &f32, double dt2, const LangevinConsts &lc, double *vb) {
+  const BrickList bl = brick_list(d, sizeof(float4));
+  const size_t shm = (size_t)bl.lds_cap * sizeof(float4);
+  allow_lds(k_force_brick_mixed<W, U, K>, shm);
+  hipLaunchKernelGGL((k_force_brick_mixed<W, U, K>), dim3(d->nbrick), dim3(kWave * W), shm, d->stream, rel, grel,
+                     src, d->mom, pos_out, bl, f32, dt2, epi, lc, d->id, vb);
 }
 
 template <int W>
-void launch_force_w(mdgpu_domain *d, const double4 *src, const float4 *rel_src, double4 *pos_out, int epi,
-                    const ForceConsts &fc, const ForceConstsF &f32, const LangevinConsts &lc, double *vb) {
+void launch_force_w(mdgpu_domain *d, const double4 *src, const double4 *gsrc, const float4 *rel,
+                    const float4 *grel, double4 *pos_out, int epi, const ForceConsts &fc, const ForceConstsF &f32,
+                    const LangevinConsts &lc, double *vb) {
   if (d->precision == MDGPU_FP64) {
-    const TileList tl = tile_list(d, 3 * sizeof(double));
-    if (d->unroll == 4) launch_fp64<W, 4>(d, src, pos_out, tl, epi, fc, lc, vb);
-    else launch_fp64<W, 8>(d, src, pos_out, tl, epi, fc, lc, vb);
+    if (d->unroll == 4) launch_fp64<W, 4>(d, src, gsrc, pos_out, epi, fc, lc, vb);
+    else launch_fp64<W, 8>(d, src, gsrc, pos_out, epi, fc, lc, vb);
+  } else if (d->kahan) {
+    if (d->unroll == 4) launch_mixed<W, 4, true>(d, src, rel, grel, pos_out, epi, f32, fc.dt2, lc, vb);
+    else launch_mixed<W, 8, true>(d, src, rel, grel, pos_out, epi, f32, fc.dt2, lc, vb);
   } else {
-    const TileList tl = tile_list(d, sizeof(float4));
-    if (d->kahan) {
-      if (d->unroll == 4) launch_mixed<W, 4, true>(d, src, rel_src, pos_out, tl, epi, f32, fc.dt2, lc, vb);
-      else launch_mixed<W, 8, true>(d, src, rel_src, pos_out, tl, epi, f32, fc.dt2, lc, vb);
-    } else {
-      if (d->unroll == 4) launch_mixed<W, 4, false>(d, src, rel_src, pos_out, tl, epi, f32, fc.dt2, lc, vb);
-      else launch_mixed<W, 8, false>(d, src, rel_src, pos_out, tl, epi, f32, fc.dt2, lc, vb);
-    }
+    if (d->unroll == 4) launch_mixed<W, 4, false>(d, src, rel, grel, pos_out, epi, f32, fc.dt2, lc, vb);
+    else launch_mixed<W, 8, false>(d, src, rel, grel, pos_out, epi, f32, fc.dt2, lc, vb);
   }
 }
 
-// One pair-force launch.  `src`/`rel_src` are the positions the forces are
-// evaluated at; `pos_out` receives the drifted (or copied) owned positions.
-int launch_force(mdgpu_domain *d, const double4 *src, const float4 *rel_src, double4 *pos_out, int epilogue,
-                 bool want_vmax) {
+// One pair-force launch.  Rows and owned partners are read from `src`
+// (`rel` in FP32), ghost partners from `gsrc` (`grel`) at [pn, pn + ng);
+// `pos_out` receives the drifted (or copied) owned positions.
+int launch_force(mdgpu_domain *d, const double4 *src, const double4 *gsrc, const float4 *rel, const float4 *grel,
+                 double4 *pos_out, int epilogue, bool want_vmax) {
   if (d->pn == 0) return MDGPU_OK;
   hipStream_t st = d->stream;
   const ForceConsts fc = fconsts(d);
   const ForceConstsF f32 = fconsts32(d);
   const LangevinConsts lc = lconsts(d);
-  double *vb = want_vmax ? d->maxpart : nullptr;
+  double *vb = want_vmax ? d->bpart : nullptr;
   const int ev0 = d->force_launches * 2;
   if (d->timing && ev0 + 1 < (int)d->ev.size()) HIPCHK(d, hipEventRecord(d->ev[ev0], st));
-  switch (d->tile_waves) {
-    case 1: launch_force_w<1>(d, src, rel_src, pos_out, epilogue, fc, f32, lc, vb); break;
-    case 4: launch_force_w<4>(d, src, rel_src, pos_out, epilogue, fc, f32, lc, vb); break;
-    default: launch_force_w<2>(d, src, rel_src, pos_out, epilogue, fc, f32, lc, vb); break;
+  switch (d->waves) {
+    case 2: launch_force_w<2>(d, src, gsrc, rel, grel, pos_out, epilogue, fc, f32, lc, vb); break;
+    case 8: launch_force_w<8>(d, src, gsrc, rel, grel, pos_out, epilogue, fc, f32, lc, vb); break;
+    default: launch_force_w<4>(d, src, gsrc, rel, grel, pos_out, epilogue, fc, f32, lc, vb); break;
   }
   if (d->timing && ev0 + 1 < (int)d->ev.size()) {
     HIPCHK(d, hipEventRecord(d->ev[ev0 + 1], st));
     d->force_launches++;
   }
   if (want_vmax)
-    hipLaunchKernelGGL(k_reduce_max, dim3(1), dim3(kBlock), 0, st, d->maxpart, d->ntile, d->u64);
+    hipLaunchKernelGGL(k_reduce_max, dim3(1), dim3(kBlock), 0, st, d->bpart, d->nbrick, d->u64);
   return check_launch(d, "force");
 }
 
@@ -711,12 +787,13 @@ int refresh(mdgpu_domain *d) {
   return check_launch(d, "refresh");
 }
 
-// CalculateForce on the current positions (owned + ghosts in `pos`)
+// CalculateForce on the current positions (owned and the ghosts of the last
+// halo refresh, all in `pos`)
 int force_in_place(mdgpu_domain *d) {
   if (d->precision == MDGPU_MIXED && d->tn > 0)
     hipLaunchKernelGGL(k_to_rel, dim3(nblk(d->tn)), dim3(kBlock), 0, d->stream, d->pos, d->cell, d->tn,
                        origins_of(d), d->rel);
-  CHK(launch_force(d, d->pos, d->rel, d->pos, kKickOnly, false));
+  CHK(launch_force(d, d->pos, d->pos, d->rel, d->rel, d->pos, kKickOnly, false));
   d->vmax_valid = false;
   return MDGPU_OK;
 }
@@ -738,8 +815,8 @@ int calculate_step(mdgpu_domain *d) {
       float ms = 0.f;
       HIPCHK(d, hipEventElapsedTime(&ms, a, b));
       d->rebuild_ms += ms;
-      hipEventDestroy(a);
-      hipEventDestroy(b);
+      (void)hipEventDestroy(a);
+      (void)hipEventDestroy(b);
     } else {
       CHK(make_pair_list(d));
     }
@@ -759,11 +836,10 @@ int calculate_step(mdgpu_domain *d) {
     CHK(heatbath_zeta(d, t));
     CHK(drift(d));
   } else {
-    // fused: drift + halo refresh into pos_tmp, then kick (+ drift or Langevin)
+    // fused: drift into pos_tmp + halo refresh into pos, then kick (+ drift or Langevin)
     const int tot = d->pn + d->ng;
-    const bool mixed = d->precision == MDGPU_MIXED;
     if (tot > 0) {
-      if (mixed)
+      if (d->precision == MDGPU_MIXED)
         hipLaunchKernelGGL((k_drift_refresh<true>), dim3(nblk(tot)), dim3(kBlock), 0, st, d->pos, d->mom,
                            d->pos_tmp, d->gsrc, d->gcode, d->pn, d->ng, d->dt * 0.5, d->L[0], d->L[1], d->L[2],
                            d->cell, origins_of(d), d->rel_tmp);
@@ -774,12 +850,19 @@ int calculate_step(mdgpu_domain *d) {
       CHK(check_launch(d, "drift+refresh"));
     }
     const int epi = d->thermostat == MDGPU_LANGEVIN ? kKickLangevin : kKickDrift;
-    CHK(launch_force(d, d->pos_tmp, d->rel_tmp, d->pos, epi, true));
+    CHK(launch_force(d, d->pos_tmp, d->pos, d->rel_tmp, d->rel_tmp, d->pos, epi, true));
     d->vmax_valid = true;
   }
   d->sim_time += d->dt;
   d->step++;
   return MDGPU_OK;
+}
+
+template <int W>
+void launch_observe(mdgpu_domain *d, const BrickList &bl, size_t shm) {
+  allow_lds(k_observe_brick<W>, shm);
+  hipLaunchKernelGGL(k_observe_brick<W>, dim3(d->nbrick), dim3(kWave * W), shm, d->stream, d->pos, d->pos,
+                     d->mom, bl, d->CL2, d->C2, d->C0, d->partial);
 }
 
 }  // namespace
@@ -792,23 +875,23 @@ int mdgpu_create(const mdgpu_params *prm, mdgpu_domain **out) {
   *out = nullptr;
   mdgpu_domain *d = new mdgpu_domain();
   d->prm = *prm;
-  if (hipSetDevice(prm->device) != hipSuccess || hipStreamCreateWithFlags(&d->stream, hipStreamNonBlocking) != hipSuccess) {
+  d->device = prm->device;
+  if (hipSetDevice(prm->device) != hipSuccess ||
+      hipStreamCreateWithFlags(&d->stream, hipStreamNonBlocking) != hipSuccess) {
     delete d;
     return MDGPU_ERR_HIP;
+  }
+  {
+    int shm = 0;
+    if (hipDeviceGetAttribute(&shm, hipDeviceAttributeMaxSharedMemoryPerBlock, prm->device) == hipSuccess &&
+        shm > 8 * 1024)
+      d->lds_max_kb = shm / 1024 - 4;
   }
   for (int k = 0; k < 3; k++) {
     d->L[k] = prm->L[k];
     // single domain of a 1x1x1 grid: s = L*0, e = s + L (mdunit.cc:96-101)
     d->rs[k] = prm->L[k] * 0.0;
     d->re[k] = d->rs[k] + prm->L[k];
-  }
-  {
-    int shm = 0;
-    if (hipDeviceGetAttribute(&shm, hipDeviceAttributeMaxSharedMemoryPerBlock, prm->device) == hipSuccess &&
-        shm > 8 * 1024) {
-      d->lds_max_kb = shm / 1024 - 4;
-      d->lds_kb = std::min(d->lds_kb, d->lds_max_kb);
-    }
   }
   d->dt = prm->dt;
   d->rc = prm->rc;
@@ -844,28 +927,30 @@ int mdgpu_create(const mdgpu_params *prm, mdgpu_domain **out) {
     mdgpu_destroy(d);
     return MDGPU_ERR_HIP;
   }
-  hipMemset(d->i32, 0, 4 * sizeof(int));
-  hipMemset(d->u64, 0, 4 * sizeof(unsigned long long));
+  (void)hipMemset(d->i32, 0, 4 * sizeof(int));
+  (void)hipMemset(d->u64, 0, 4 * sizeof(unsigned long long));
   *out = d;
   return MDGPU_OK;
 }
 
 int mdgpu_destroy(mdgpu_domain *d) {
   if (!d) return MDGPU_OK;
-  if (d->stream) hipStreamSynchronize(d->stream);
+  if (d->stream) (void)hipStreamSynchronize(d->stream);
   void *bufs[] = {d->pos, d->pos_tmp, d->pos_old, d->mom, d->mom_alt, d->rel, d->rel_tmp, d->id, d->id_alt,
                   d->cell, d->cell_alt, d->gsrc, d->gcode, d->c_ocount, d->c_ostart, d->c_gcount, d->c_gstart,
-                  d->org, d->nbr, d->ncode, d->tstart, d->tbase, d->sbase, d->slen, d->row_len, d->slot, d->perm, d->nimg, d->goff, d->t_gsrc,
-                  d->t_gcode, d->t_gcell, d->key, d->scan_scratch, d->slots64, d->disp, d->partial, d->maxpart, d->cand,
+                  d->count_r, d->start_r, d->org, d->rank, d->brick_of_cell, d->brick_rank0, d->row0, d->nrow,
+                  d->nsl, d->vslice0, d->slice_brick, d->tstart, d->tbase, d->nrange, d->nbr, d->ncode,
+                  d->sbase, d->slots64, d->slen, d->row_len, d->slot, d->perm, d->nimg, d->goff, d->t_gsrc,
+                  d->t_gcode, d->t_gcell, d->key, d->scan_scratch, d->disp, d->partial, d->maxpart, d->bpart, d->cand,
                   d->u64, d->i64, d->i32, d->d3};
   for (void *b : bufs)
-    if (b) hipFree(b);
-  if (d->h_u64) hipHostFree(d->h_u64);
-  if (d->h_i64) hipHostFree(d->h_i64);
-  if (d->h_i32) hipHostFree(d->h_i32);
-  if (d->h_d3) hipHostFree(d->h_d3);
-  for (hipEvent_t e : d->ev) hipEventDestroy(e);
-  if (d->stream) hipStreamDestroy(d->stream);
+    if (b) (void)hipFree(b);
+  if (d->h_u64) (void)hipHostFree(d->h_u64);
+  if (d->h_i64) (void)hipHostFree(d->h_i64);
+  if (d->h_i32) (void)hipHostFree(d->h_i32);
+  if (d->h_d3) (void)hipHostFree(d->h_d3);
+  for (hipEvent_t e : d->ev) (void)hipEventDestroy(e);
+  if (d->stream) (void)hipStreamDestroy(d->stream);
   delete d;
   return MDGPU_OK;
 }
@@ -906,7 +991,7 @@ int mdgpu_add_particles(mdgpu_domain *d, int n, const double *q, const double *p
   d->tn = d->pn;
   d->ng = 0;
   d->vmax_valid = false;
-  // indices of the old list and its sentinel are stale: the next step rebuilds
+  // indices of the old list are stale: the next step rebuilds
   d->initialized = false;
   return MDGPU_OK;
 }
@@ -1078,14 +1163,14 @@ int mdgpu_calculate(mdgpu_domain *d, int nsteps, int *rebuilds) {
     HIPCHK(d, hipEventRecord(t1, d->stream));
     HIPCHK(d, hipEventSynchronize(t1));
     float ms = 0.f;
-    hipEventElapsedTime(&ms, t0, t1);
+    (void)hipEventElapsedTime(&ms, t0, t1);
     d->total_ms = ms;
     for (int k = 0; k < d->force_launches; k++) {
-      hipEventElapsedTime(&ms, d->ev[2 * k], d->ev[2 * k + 1]);
+      (void)hipEventElapsedTime(&ms, d->ev[2 * k], d->ev[2 * k + 1]);
       d->force_ms += ms;
     }
-    hipEventDestroy(t0);
-    hipEventDestroy(t1);
+    (void)hipEventDestroy(t0);
+    (void)hipEventDestroy(t1);
   }
   if (rebuilds) *rebuilds = d->rebuild_count;
   return r;
@@ -1097,25 +1182,16 @@ int mdgpu_observe(mdgpu_domain *d, double *ke, double *pe, double *virial) {
     d->err = "pair list not built";
     return MDGPU_ERR_ARGUMENT;
   }
-  const int nb = d->ntile;
+  const int nb = d->nbrick;
   CHK(ensure_partial(d, std::max(1, nb)));
   double out[3] = {0.0, 0.0, 0.0};
   if (d->pn > 0) {
-    const TileList tl = tile_list(d, 3 * sizeof(double));
-    const size_t shm = (size_t)tl.lds_cap * 3 * sizeof(double);
-    switch (d->tile_waves) {
-      case 1:
-        hipLaunchKernelGGL(k_observe_tile<1>, dim3(nb), dim3(kWave), shm, d->stream, d->pos, d->mom, tl, d->pn, d->tn,
-                           d->CL2, d->C2, d->C0, d->partial);
-        break;
-      case 4:
-        hipLaunchKernelGGL(k_observe_tile<4>, dim3(nb), dim3(kWave * 4), shm, d->stream, d->pos, d->mom, tl, d->pn,
-                           d->tn, d->CL2, d->C2, d->C0, d->partial);
-        break;
-      default:
-        hipLaunchKernelGGL(k_observe_tile<2>, dim3(nb), dim3(kWave * 2), shm, d->stream, d->pos, d->mom, tl, d->pn,
-                           d->tn, d->CL2, d->C2, d->C0, d->partial);
-        break;
+    const BrickList bl = brick_list(d, 3 * sizeof(double));
+    const size_t shm = (size_t)bl.lds_cap * 3 * sizeof(double);
+    switch (d->waves) {
+      case 2: launch_observe<2>(d, bl, shm); break;
+      case 8: launch_observe<8>(d, bl, shm); break;
+      default: launch_observe<4>(d, bl, shm); break;
     }
     hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(kBlock), 0, d->stream, d->partial, nb, 3, d->d3);
     CHK(check_launch(d, "observe"));
@@ -1146,50 +1222,60 @@ int mdgpu_change_scale(mdgpu_domain *d, double alpha) {
 
 int mdgpu_get_pairs(mdgpu_domain *d, int64_t *npairs, int32_t *out3) {
   if (!d || !npairs) return MDGPU_ERR_ARGUMENT;
-  const int pn = d->pn, ng = d->ng;
+  if (!d->initialized) {
+    d->err = "pair list not built";
+    return MDGPU_ERR_ARGUMENT;
+  }
+  const int pn = d->pn, ng = d->ng, nbk = d->nbrick, rs = kRangesPerCell * d->rmax;
   std::vector<uint16_t> nbr((size_t)d->nbr_slots);
   std::vector<long long> sb(d->nslice);
-  std::vector<int> sl(d->nslice), ids(pn), gs(ng), gc(ng);
-  std::vector<int> ts((size_t)d->ntile * kTileRanges), tb((size_t)d->ntile * (kTileRanges + 1));
-  if (d->nbr_slots)
-    HIPCHK(d, hipMemcpyAsync(nbr.data(), d->nbr, nbr.size() * 2, hipMemcpyDeviceToHost, d->stream));
-  if (d->ntile) {
-    HIPCHK(d, hipMemcpyAsync(ts.data(), d->tstart, ts.size() * 4, hipMemcpyDeviceToHost, d->stream));
-    HIPCHK(d, hipMemcpyAsync(tb.data(), d->tbase, tb.size() * 4, hipMemcpyDeviceToHost, d->stream));
-  }
+  std::vector<int> sl(d->nslice), ids(pn), cl(pn), r0(nbk), vs0(nbk), nr(nbk), gs(ng), gc(ng);
+  std::vector<int> ts((size_t)nbk * rs), tb((size_t)nbk * (rs + 1));
+  if (d->nbr_slots) HIPCHK(d, hipMemcpyAsync(nbr.data(), d->nbr, nbr.size() * 2, hipMemcpyDeviceToHost, d->stream));
   if (d->nslice) {
     HIPCHK(d, hipMemcpyAsync(sb.data(), d->sbase, d->nslice * 8, hipMemcpyDeviceToHost, d->stream));
     HIPCHK(d, hipMemcpyAsync(sl.data(), d->slen, d->nslice * 4, hipMemcpyDeviceToHost, d->stream));
   }
-  if (pn) HIPCHK(d, hipMemcpyAsync(ids.data(), d->id, pn * 4, hipMemcpyDeviceToHost, d->stream));
+  if (pn) {
+    HIPCHK(d, hipMemcpyAsync(ids.data(), d->id, pn * 4, hipMemcpyDeviceToHost, d->stream));
+    HIPCHK(d, hipMemcpyAsync(cl.data(), d->cell, pn * 4, hipMemcpyDeviceToHost, d->stream));
+  }
   if (ng) {
     HIPCHK(d, hipMemcpyAsync(gs.data(), d->gsrc, ng * 4, hipMemcpyDeviceToHost, d->stream));
     HIPCHK(d, hipMemcpyAsync(gc.data(), d->gcode, ng * 4, hipMemcpyDeviceToHost, d->stream));
   }
+  HIPCHK(d, hipMemcpyAsync(r0.data(), d->row0, nbk * 4, hipMemcpyDeviceToHost, d->stream));
+  HIPCHK(d, hipMemcpyAsync(vs0.data(), d->vslice0, nbk * 4, hipMemcpyDeviceToHost, d->stream));
+  HIPCHK(d, hipMemcpyAsync(nr.data(), d->nrange, nbk * 4, hipMemcpyDeviceToHost, d->stream));
+  HIPCHK(d, hipMemcpyAsync(ts.data(), d->tstart, ts.size() * 4, hipMemcpyDeviceToHost, d->stream));
+  HIPCHK(d, hipMemcpyAsync(tb.data(), d->tbase, tb.size() * 4, hipMemcpyDeviceToHost, d->stream));
   CHK(sync(d));
   int64_t cnt = 0;
-  const int rows_per_tile = kWave * d->tile_waves;
   for (int i = 0; i < pn; i++) {
-    const int s = i / kWave, lane = i % kWave, t = i / rows_per_tile;
-    const int *tst = ts.data() + (size_t)t * kTileRanges;
-    const int *tbs = tb.data() + (size_t)t * (kTileRanges + 1);
+    const int b = d->h_brick_of_cell[cl[i]];
+    const int v = i - r0[b];
+    const int s = vs0[b] + v / kWave, lane = v % kWave;
+    const int *tbb = tb.data() + (size_t)b * (rs + 1);
+    const int *tsb = ts.data() + (size_t)b * rs;
+    const int R = nr[b];
     for (int k = 0; k < sl[s]; k++) {
       const int e = nbr[sb[s] + (long long)k * kWave + lane];
-      if (e >= tbs[kTileRanges]) continue;  // padding
-      int r = 0;
-      while (r + 1 < kTileRanges && tbs[r + 1] <= e) r++;
-      const int j = tst[r] + (e - tbs[r]);
-      int a = ids[i], b, code = 13;
+      if (e >= tbb[R]) continue;  // padding
+      const int r = (int)(std::upper_bound(tbb, tbb + R + 1, e) - tbb) - 1;
+      const int j = tsb[r] + (e - tbb[r]);
+      const int a = ids[i];
+      int bb, code;
       if (j < pn) {
-        b = ids[j];
-        if (b < a) continue;  // owned-owned pairs appear in both rows
+        bb = ids[j];
+        code = 13;
+        if (bb < a) continue;  // owned-owned pairs appear in both rows
       } else {
-        b = ids[gs[j - pn]];
+        bb = ids[gs[j - pn]];
         code = gc[j - pn];
       }
       if (out3) {
         out3[3 * cnt] = a;
-        out3[3 * cnt + 1] = b;
+        out3[3 * cnt + 1] = bb;
         out3[3 * cnt + 2] = code;
       }
       cnt++;
@@ -1249,17 +1335,23 @@ int mdgpu_set_option(mdgpu_domain *d, int option, int value) {
     case MDGPU_OPT_KAHAN:
       d->kahan = value != 0;
       return MDGPU_OK;
-    case MDGPU_OPT_TILE_WAVES:
-      if (value != 1 && value != 2 && value != 4) return MDGPU_ERR_ARGUMENT;
-      if (value != d->tile_waves) {
-        d->tile_waves = value;
-        d->initialized = false;  // tile tables and staged indices depend on it
-      }
+    case MDGPU_OPT_WAVES:
+      if (value != 2 && value != 4 && value != 8) return MDGPU_ERR_ARGUMENT;
+      d->waves = value;
       return MDGPU_OK;
     case MDGPU_OPT_LDS_KB:
-      if (value < 8 || value > d->lds_max_kb) return MDGPU_ERR_ARGUMENT;
+      if (value < 8) return MDGPU_ERR_ARGUMENT;
       d->lds_kb = value;
       return MDGPU_OK;
+    case MDGPU_OPT_BRICK: {
+      const int bx = value & 0xff, by = (value >> 8) & 0xff, bz = (value >> 16) & 0xff;
+      if (bx < 1 || bx > 4 || by < 1 || by > 4 || bz < 1 || bz > 4) return MDGPU_ERR_ARGUMENT;
+      if (bx == d->bd[0] && by == d->bd[1] && bz == d->bd[2]) return MDGPU_OK;
+      d->bd[0] = bx;
+      d->bd[1] = by;
+      d->bd[2] = bz;
+      return set_geometry(d);  // the next step rebuilds the list
+    }
   }
   return MDGPU_ERR_ARGUMENT;
 }
@@ -1269,8 +1361,9 @@ int mdgpu_get_option(const mdgpu_domain *d, int option, int *value) {
   switch (option) {
     case MDGPU_OPT_UNROLL: *value = d->unroll; return MDGPU_OK;
     case MDGPU_OPT_KAHAN: *value = d->kahan; return MDGPU_OK;
-    case MDGPU_OPT_TILE_WAVES: *value = d->tile_waves; return MDGPU_OK;
-    case MDGPU_OPT_LDS_KB: *value = d->lds_kb; return MDGPU_OK;
+    case MDGPU_OPT_WAVES: *value = d->waves; return MDGPU_OK;
+    case MDGPU_OPT_LDS_KB: *value = std::min(d->lds_kb, d->lds_max_kb); return MDGPU_OK;
+    case MDGPU_OPT_BRICK: *value = d->bd[0] | (d->bd[1] << 8) | (d->bd[2] << 16); return MDGPU_OK;
   }
   return MDGPU_ERR_ARGUMENT;
 }

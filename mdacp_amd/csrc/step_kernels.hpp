@@ -42,8 +42,9 @@ __global__ void k_drift(double4 *__restrict__ pos, const double4 *__restrict__ m
   pos[i] = q;
 }
 
-__global__ void k_refresh(double4 *__restrict__ pos, const int *__restrict__ gsrc,
-                          const int *__restrict__ gcode, int pn, int ng, double Lx, double Ly, double Lz) {
+// SendBorderParticles: ghosts [pn, pn + ng) from the current owned positions.
+__global__ void k_refresh(double4 *__restrict__ pos, const int *__restrict__ gsrc, const int *__restrict__ gcode,
+                          int pn, int ng, double Lx, double Ly, double Lz) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= ng) return;
   const double3 y = image_of(pos[gsrc[k]], gcode[k], Lx, Ly, Lz);
@@ -69,11 +70,12 @@ __global__ void k_to_rel(const double4 *__restrict__ pos, const int *__restrict_
   rel[i] = to_rel(pos[i], cell[i], O.x, O.y, O.z, O.mx, O.my);
 }
 
-// Fused first half drift + halo refresh into a second buffer:
-//   owned:  tmp[i] = q_i + p_i*dt2
-//   ghost:  tmp[g] = pos[g] = (q_src + p_src*dt2) + shift*L
+// Fused first half drift + halo refresh:
+//   owned:  tmp[i] = q_i + p_i*dt2         (and its cell-relative FP32 copy)
+//   ghost:  pos[g] = (q_src + p_src*dt2) + shift*L   (and its FP32 copy)
 // (the same two roundings as UpdatePositionHalf then SendBorderParticles).
-// `pos` keeps the owned positions until the force kernel writes them.
+// `pos` keeps the owned positions until the force kernel writes the drifted
+// ones from `tmp`; the ghosts stay in `pos` until the next refresh.
 template <bool kRel>
 __global__ void k_drift_refresh(double4 *__restrict__ pos, const double4 *__restrict__ mom,
                                 double4 *__restrict__ tmp, const int *__restrict__ gsrc,
@@ -99,7 +101,6 @@ __global__ void k_drift_refresh(double4 *__restrict__ pos, const double4 *__rest
     q.z += p.z * dt2;
     const double3 y = image_of(q, gcode[k], Lx, Ly, Lz);
     const double4 g = make_double4(y.x, y.y, y.z, 0.0);
-    tmp[i] = g;
     pos[i] = g;
     if (kRel) rel[i] = to_rel(g, cell[i], O.x, O.y, O.z, O.mx, O.my);
   }

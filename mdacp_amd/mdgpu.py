@@ -26,7 +26,7 @@ EXPORTS = [
     "mdgpu_calculate", "mdgpu_observe", "mdgpu_change_scale", "mdgpu_get_pairs", "mdgpu_compute_kick",
     "mdgpu_enable_timing", "mdgpu_get_timing", "mdgpu_get_list_stats", "mdgpu_set_option", "mdgpu_get_option",
 ]
-OPT_UNROLL, OPT_KAHAN, OPT_TILE_WAVES, OPT_LDS_KB = 1, 2, 3, 4
+OPT_UNROLL, OPT_KAHAN, OPT_WAVES, OPT_LDS_KB, OPT_BRICK = 1, 2, 3, 4, 5
 
 
 class Params(ctypes.Structure):
