@@ -27,6 +27,10 @@ sys.path.insert(0, ROOT)
 
 METRIC = "particle-updates/s, 4M LJ ρ=0.5: FP64 vs mixed FP32 at equal energy drift"
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per launch of the FP64 force kernel on this workload, from the
+# FETCH_SIZE and WRITE_SIZE counters (rocprofv3 --pmc) scaled by a copy of known
+# size; written from the profile summaries.
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_force_brick_fp64_4M_traffic.json")
 FP64_VECTOR_PEAK_TFLOPS = 78.6
 FP32_VECTOR_PEAK_TFLOPS = 157.3
 
@@ -161,8 +165,12 @@ def main():
     bytes_pu, flops_pu, n_L, n_C = algorithmic_per_update(args.density, args.rc, args.margin)
     avg_s = r64["force_avg_ms"] * 1e-3
     achieved = bytes_pu * n / avg_s / 1e9
+    traffic = None
+    if os.path.exists(TRAFFIC_FILE) and n == 4_000_000:
+        with open(TRAFFIC_FILE) as f:
+            traffic = json.load(f).get("hbm_bytes_per_launch")
     roofline = dict(bound="hbm", achieved=achieved, peak=HBM_PEAK_GBS, unit="GB/s", frac=achieved / HBM_PEAK_GBS,
-                    traffic=None, kernel="k_force_fp64",
+                    traffic=traffic, kernel="k_force_brick_fp64",
                     kernel_avg_ms=r64["force_avg_ms"], bytes_per_particle_update=bytes_pu,
                     vector_flop_frac=flops_pu * n / avg_s / 1e12 / FP64_VECTOR_PEAK_TFLOPS)
     cpu = None

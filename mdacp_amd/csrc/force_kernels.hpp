@@ -8,12 +8,15 @@
 //   observers           observer.cc:17-79 (PE and virial, ghost pairs halved)
 //
 // A workgroup first copies its brick's neighbourhood (list_kernels.hpp) into
-// LDS with coalesced loads: owned particles and ghost images of each
-// neighbourhood cell, the ghosts as placed by the last halo refresh.  Its wavefronts
-// then walk the brick's 64-row slices of the sliced-ELL list (16-bit indices
-// into the staged array) and gather partners from LDS.  A brick whose staged
-// set does not fit the LDS budget reads the same partners from global memory
-// (same arithmetic, slower).
+// LDS: the owned particles and ghost images of the cells around the brick,
+// gathered through the brick's gather list with several loads in flight per
+// thread.  Its wavefronts then walk the brick's 64-row slices of the
+// sliced-ELL list (16-bit indices into the staged array) and read partners
+// from LDS.  A brick whose staged set does not fit the LDS budget reads the
+// same partners from global memory (same arithmetic, slower).
+//
+// Positions of owned particles and ghosts live in one array (`src`, ghosts at
+// [pn, pn + ng)), as the halo refresh left them.
 //
 // Full neighbour list: each owned row lists every partner within SL, so no
 // momentum scatter and no atomics: results are bit-reproducible run to run.
@@ -23,8 +26,6 @@
 #include "step_kernels.hpp"
 
 namespace mdgpu {
-
-constexpr int kMaxBrickRanges = kRangesPerCell * 6 * 6 * 6;  // bricks of at most 4x4x4 cells
 
 enum Epilogue {
   kKickOnly = 0,      // p += kick                       (CalculateForce)
@@ -37,12 +38,10 @@ struct BrickList {
   const uint8_t *code;      // cell offset of the partner: (dx+1) | (dy+1)<<2 | (dz+1)<<4
   const long long *sbase;   // first slot of each 64-row slice
   const int *slen;          // slots per slice (multiple of kSliceRound)
-  const int *tstart;        // [brick][2 rmax] first particle of each staged range
-  const int *tbase;         // [brick][2 rmax + 1] offset of each range in the staged array
-  const int *nrange;        // ranges of each brick (2 per neighbourhood cell)
+  const int *glist;         // staged entry -> particle index (owned, or pn + ghost)
+  const long long *gbase;   // [brick + 1] first gather-list entry of each brick
   const int *row0, *nrow, *vslice0;
-  int rstride;              // 2 rmax
-  int lds_cap;              // staged entries that fit the LDS budget
+  int lds_cap;              // staged entries that fit the LDS budget (incl. the pad entry)
 };
 
 // 1/x to double precision: hardware reciprocal estimate + two Newton steps.
@@ -52,40 +51,6 @@ __device__ __forceinline__ double rcp64(double x) {
   r = fma(r, e, r);
   e = fma(-x, r, 1.0);
   return fma(r, e, r);
-}
-
-// range r of a staged index u: largest r with base[r] <= u (base[0] = 0 <= u < base[R])
-__device__ __forceinline__ int range_of_staged(const int *base, int R, int u) {
-  int lo = 0, hi = R;
-  while (hi - lo > 1) {
-    const int mid = (lo + hi) >> 1;
-    if (base[mid] <= u) lo = mid;
-    else hi = mid;
-  }
-  return lo;
-}
-
-// Position of staged index u: owned ranges (even) read `cur`, ghost ranges
-// (odd) read `gpos`, whose entries [pn, pn + ng) hold the ghost images.
-template <class T>
-__device__ __forceinline__ T staged_load(int u, int R, const int *base, const int *__restrict__ ts,
-                                         const T *__restrict__ cur, const T *__restrict__ gpos) {
-  const int r = range_of_staged(base, R, u);
-  const int j = ts[r] + (u - base[r]);
-  return (r & 1) ? gpos[j] : cur[j];
-}
-
-// LDS layout of the brick kernels: staged-range bases, then the partners.
-struct BrickShared {
-  int base[kMaxBrickRanges + 1];
-};
-
-template <int T>
-__device__ __forceinline__ int load_bases(const BrickList &bl, int b, int *base) {
-  const int R = bl.nrange[b];
-  const int *tb = bl.tbase + (long long)b * (bl.rstride + 1);
-  for (int r = threadIdx.x; r <= R; r += T) base[r] = tb[r];
-  return R;
 }
 
 __device__ __forceinline__ double finish_row(int i, double4 p, const double4 qi, double4 *__restrict__ mom,
@@ -109,33 +74,20 @@ __device__ __forceinline__ double finish_row(int i, double4 p, const double4 qi,
 // are loaded before the current batch is computed.  The cutoff is a select,
 // not a branch (padding entries point at a far sentinel: r2 = inf).
 template <int W, int U>
-__global__ __launch_bounds__(kWave *W) void k_force_brick_fp64(
-    const double4 *__restrict__ src, const double4 *__restrict__ gpos, double4 *__restrict__ mom,
-    double4 *__restrict__ pos_out, BrickList bl, ForceConsts fc, int epilogue, LangevinConsts lc,
-    const int *__restrict__ id, double *__restrict__ vpart) {
+__global__ __launch_bounds__(kWave *W) void k_force_brick_fp64(const double4 *__restrict__ src,
+                                                              double4 *__restrict__ mom,
+                                                              double4 *__restrict__ pos_out, BrickList bl,
+                                                              ForceConsts fc, int epilogue, LangevinConsts lc,
+                                                              const int *__restrict__ id,
+                                                              double *__restrict__ vpart) {
   extern __shared__ double lds64[];
-  __shared__ BrickShared sh;
   constexpr int T = kWave * W;
   const int b = xcd_block(blockIdx.x, gridDim.x);
-  const int R = load_bases<T>(bl, b, sh.base);
-  __syncthreads();
-  const int *ts = bl.tstart + (long long)b * bl.rstride;
-  const int total = sh.base[R];
+  const long long g0 = bl.gbase[b];
+  const int total = (int)(bl.gbase[b + 1] - g0);
+  const int *gl = bl.glist + g0;
   const bool staged = total < bl.lds_cap;  // uniform across the workgroup
-  double *lx = lds64, *ly = lds64 + bl.lds_cap, *lz = lds64 + 2 * bl.lds_cap;
-  if (staged) {
-    for (int u = threadIdx.x; u < total; u += T) {
-      const double4 q = staged_load(u, R, sh.base, ts, src, gpos);
-      lx[u] = q.x;
-      ly[u] = q.y;
-      lz[u] = q.z;
-    }
-    if (threadIdx.x == 0) {
-      lx[total] = 1.0e200;
-      ly[total] = 1.0e200;
-      lz[total] = 1.0e200;
-    }
-  }
+  if (staged) stage_fp64<T>(gl, total, src, lds64);
   __syncthreads();
   const int nrow = bl.nrow[b], row0 = bl.row0[b], vs0 = bl.vslice0[b];
   const int nsl = (nrow + kWave - 1) / kWave;
@@ -159,15 +111,16 @@ __global__ __launch_bounds__(kWave *W) void k_force_brick_fp64(
       if (staged) {
 #pragma unroll
         for (int u = 0; u < U; u++) {
-          qx[u] = lx[e[u]];
-          qy[u] = ly[e[u]];
-          qz[u] = lz[e[u]];
+          const double *q = lds64 + 3 * e[u];
+          qx[u] = q[0];
+          qy[u] = q[1];
+          qz[u] = q[2];
         }
       } else {
 #pragma unroll
         for (int u = 0; u < U; u++) {
           if (e[u] < total) {
-            const double4 q = staged_load(e[u], R, sh.base, ts, src, gpos);
+            const double4 q = src[gl[e[u]]];
             qx[u] = q.x;
             qy[u] = q.y;
             qz[u] = q.z;
@@ -212,11 +165,10 @@ __global__ __launch_bounds__(kWave *W) void k_force_brick_fp64(
 // like owned particles.
 template <int W, int U, bool kKahan>
 __global__ __launch_bounds__(kWave *W) void k_force_brick_mixed(
-    const float4 *__restrict__ rel, const float4 *__restrict__ grel, const double4 *__restrict__ pos,
-    double4 *__restrict__ mom, double4 *__restrict__ pos_out, BrickList bl, ForceConstsF fc, double dt2,
-    int epilogue, LangevinConsts lc, const int *__restrict__ id, double *__restrict__ vpart) {
+    const float4 *__restrict__ rel, const double4 *__restrict__ pos, double4 *__restrict__ mom,
+    double4 *__restrict__ pos_out, BrickList bl, ForceConstsF fc, double dt2, int epilogue, LangevinConsts lc,
+    const int *__restrict__ id, double *__restrict__ vpart) {
   extern __shared__ float4 lds32[];
-  __shared__ BrickShared sh;
   __shared__ float4 offs[64];
   constexpr int T = kWave * W;
   if (threadIdx.x < 64) {
@@ -225,14 +177,24 @@ __global__ __launch_bounds__(kWave *W) void k_force_brick_mixed(
     offs[c] = make_float4(ox * fc.w[0], oy * fc.w[1], oz * fc.w[2], 0.0f);
   }
   const int b = xcd_block(blockIdx.x, gridDim.x);
-  const int R = load_bases<T>(bl, b, sh.base);
-  __syncthreads();
-  const int *ts = bl.tstart + (long long)b * bl.rstride;
-  const int total = sh.base[R];
+  const long long g0 = bl.gbase[b];
+  const int total = (int)(bl.gbase[b + 1] - g0);
+  const int *gl = bl.glist + g0;
   const bool staged = total < bl.lds_cap;
   if (staged) {
-    for (int u = threadIdx.x; u < total; u += T) {
-      lds32[u] = staged_load(u, R, sh.base, ts, rel, grel);
+    for (int u0 = threadIdx.x; u0 < total; u0 += T * kStageBatch) {
+      int j[kStageBatch];
+#pragma unroll
+      for (int k = 0; k < kStageBatch; k++) {
+        const int u = u0 + k * T;
+        j[k] = u < total ? gl[u] : 0;
+      }
+      float4 q[kStageBatch];
+#pragma unroll
+      for (int k = 0; k < kStageBatch; k++) q[k] = rel[j[k]];
+#pragma unroll
+      for (int k = 0; k < kStageBatch; k++)
+        if (u0 + k * T < total) lds32[u0 + k * T] = q[k];
     }
     if (threadIdx.x == 0) lds32[total] = make_float4(1.0e30f, 1.0e30f, 1.0e30f, 0.0f);
   }
@@ -266,13 +228,8 @@ __global__ __launch_bounds__(kWave *W) void k_force_brick_mixed(
         for (int u = 0; u < U; u++) rj[u] = lds32[e[u]];
       } else {
 #pragma unroll
-        for (int u = 0; u < U; u++) {
-          if (e[u] < total) {
-            rj[u] = staged_load(e[u], R, sh.base, ts, rel, grel);
-          } else {
-            rj[u] = make_float4(1.0e30f, 1.0e30f, 1.0e30f, 0.0f);
-          }
-        }
+        for (int u = 0; u < U; u++)
+          rj[u] = e[u] < total ? rel[gl[e[u]]] : make_float4(1.0e30f, 1.0e30f, 1.0e30f, 0.0f);
       }
 #pragma unroll
       for (int u = 0; u < U; u++) cc[u] = c[u];
@@ -320,33 +277,22 @@ __global__ __launch_bounds__(kWave *W) void k_force_brick_mixed(
 // so the results are deterministic.  Each list entry carries weight 1/2:
 // owned-owned pairs appear in two rows, owned-ghost pairs in one row and are
 // halved by the reference (observer.cc:39-41,70-72).  Owned partners are at
-// their current positions, image partners at the last halo refresh, as the
-// reference's ghosts are.
+// their current positions, ghosts where the last halo refresh put them, as
+// the reference's are.
 template <int W>
 __global__ __launch_bounds__(kWave *W) void k_observe_brick(const double4 *__restrict__ pos,
-                                                          const double4 *__restrict__ gpos,
                                                           const double4 *__restrict__ mom, BrickList bl,
                                                           double CL2, double C2, double C0,
                                                           double *__restrict__ partial) {
   extern __shared__ double lds64[];
-  __shared__ BrickShared sh;
   __shared__ double red[3][W];
   constexpr int T = kWave * W;
   const int b = xcd_block(blockIdx.x, gridDim.x);
-  const int R = load_bases<T>(bl, b, sh.base);
-  __syncthreads();
-  const int *ts = bl.tstart + (long long)b * bl.rstride;
-  const int total = sh.base[R];
+  const long long g0 = bl.gbase[b];
+  const int total = (int)(bl.gbase[b + 1] - g0);
+  const int *gl = bl.glist + g0;
   const bool staged = total < bl.lds_cap;
-  double *lx = lds64, *ly = lds64 + bl.lds_cap, *lz = lds64 + 2 * bl.lds_cap;
-  if (staged) {
-    for (int u = threadIdx.x; u < total; u += T) {
-      const double4 q = staged_load(u, R, sh.base, ts, pos, gpos);
-      lx[u] = q.x;
-      ly[u] = q.y;
-      lz[u] = q.z;
-    }
-  }
+  if (staged) stage_fp64<T>(gl, total, pos, lds64);
   __syncthreads();
   const int nrow = bl.nrow[b], row0 = bl.row0[b], vs0 = bl.vslice0[b];
   const int nsl = (nrow + kWave - 1) / kWave;
@@ -365,10 +311,18 @@ __global__ __launch_bounds__(kWave *W) void k_observe_brick(const double4 *__res
     for (int k = 0; k < len; k++) {
       const int e = row[(long long)k * kWave];
       if (e >= total) continue;  // padding
-      double4 qj;
-      if (staged) qj = make_double4(lx[e], ly[e], lz[e], 0.0);
-      else qj = staged_load(e, R, sh.base, ts, pos, gpos);
-      const double dx = qi.x - qj.x, dy = qi.y - qj.y, dz = qi.z - qj.z;
+      double qx, qy, qz;
+      if (staged) {
+        qx = lds64[3 * e];
+        qy = lds64[3 * e + 1];
+        qz = lds64[3 * e + 2];
+      } else {
+        const double4 q = pos[gl[e]];
+        qx = q.x;
+        qy = q.y;
+        qz = q.z;
+      }
+      const double dx = qi.x - qx, dy = qi.y - qy, dz = qi.z - qz;
       const double r2 = (dx * dx + dy * dy + dz * dz);
       if (r2 > CL2) continue;
       const double r6 = r2 * r2 * r2;

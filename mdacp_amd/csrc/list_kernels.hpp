@@ -96,7 +96,8 @@ __global__ void k_slice_brick(const int *__restrict__ vslice0, const int *__rest
 // tbase = offset in the staged array (tbase[R] = staged count = pad slot).
 // The largest staged count goes to maxpart (per workgroup).
 __global__ void k_brick_ranges(Bricks B, CellTables T, int nbrick, int pn, int *__restrict__ tstart,
-                               int *__restrict__ tbase, int *__restrict__ nrange, double *__restrict__ maxpart) {
+                               int *__restrict__ tbase, int *__restrict__ nrange, int *__restrict__ btotal,
+                               double *__restrict__ maxpart) {
   const int b = blockIdx.x * blockDim.x + threadIdx.x;
   double total = 0.0;
   if (b < nbrick) {
@@ -127,155 +128,290 @@ __global__ void k_brick_ranges(Bricks B, CellTables T, int nbrick, int pn, int *
     }
     tb[kRangesPerCell * nc] = base;
     nrange[b] = kRangesPerCell * nc;
+    btotal[b] = base;
     total = (double)base;
   }
   block_max_to_partial(total, maxpart);
 }
 
+// Gather list of each brick: glist[gbase[b] + u] = index (owned, or pn + ghost)
+// of staged entry u, so the force kernels stage with one indexed load per
+// entry.  One workgroup per brick.
+__global__ void k_brick_glist(int rstride, const int *__restrict__ tstart, const int *__restrict__ tbase,
+                              const int *__restrict__ nrange, const long long *__restrict__ gbase,
+                              int *__restrict__ glist) {
+  __shared__ int tb[kRangesPerCell * 216 + 1];
+  const int b = blockIdx.x;
+  const int R = nrange[b];
+  const int *tbg = tbase + (long long)b * (rstride + 1);
+  const int *ts = tstart + (long long)b * rstride;
+  for (int r = threadIdx.x; r <= R; r += blockDim.x) tb[r] = tbg[r];
+  __syncthreads();
+  const int total = tb[R];
+  int *out = glist + gbase[b];
+  for (int u = threadIdx.x; u < total; u += blockDim.x) {
+    int lo = 0, hi = R;  // largest r with tb[r] <= u
+    while (hi - lo > 1) {
+      const int mid = (lo + hi) >> 1;
+      if (tb[mid] <= u) lo = mid;
+      else hi = mid;
+    }
+    out[u] = ts[lo] + (u - tb[lo]);
+  }
+}
+
 // ---- search --------------------------------------------------------------
-struct RowOut {
-  uint16_t *nbr;
-  uint8_t *code;
-  long long stride;
-  int cap;
-  // brick of the row and its staged ranges
-  BrickBox box;
-  const int *ts, *tb;
-};
+constexpr int kMaxRanges = kRangesPerCell * 6 * 6 * 6;  // bricks of at most 4x4x4 cells
+constexpr int kErrSearchCap = 16;  // a row has more candidates than its bitmask holds
 
 // cell offset code of a partner cell relative to the row's cell (each -1..1)
 __device__ __forceinline__ uint8_t offset_code(int ox, int oy, int oz) {
   return (uint8_t)((ox + 1) | ((oy + 1) << 2) | ((oz + 1) << 4));
 }
 
-template <bool kFill>
-__device__ __forceinline__ int search_row(int i, const double4 *__restrict__ pos, const int *__restrict__ cell,
-                                          int pn, Grid g, CellTables T, double S2, RowOut *o,
-                                          int *__restrict__ err) {
-  const double4 qi = pos[i];
-  const int c = cell[i];
-  const int mx = g.m[0], my = g.m[1], mz = g.m[2];
-  const int ix = c % mx, iy = (c / mx) % my, iz = c / (mx * my);
-  int cnt = 0;
-  for (int dz = -1; dz <= 1; dz++) {
-    const int jz = iz + dz;
-    if (jz < 0 || jz >= mz) continue;
-    for (int dy = -1; dy <= 1; dy++) {
-      const int jy = iy + dy;
-      if (jy < 0 || jy >= my) continue;
-      for (int dx = -1; dx <= 1; dx++) {
-        const int jx = ix + dx;
-        if (jx < 0 || jx >= mx) continue;
-        const int c2 = mx * my * jz + mx * jy + jx;
-        for (int pass = 0; pass < 2; pass++) {
-          const int b = pass ? pn + T.gstart[c2] : T.ostart[c2];
-          const int e = b + (pass ? T.gcount[c2] : T.ocount[c2]);
-          for (int j = b; j < e; j++) {
-            if (j == i) continue;
-            const double4 qj = pos[j];
-            const double ex = qi.x - qj.x, ey = qi.y - qj.y, ez = qi.z - qj.z;
-            const double r2 = (ex * ex + ey * ey + ez * ez);
-            if (r2 > S2) continue;
-            if (kFill) {
-              // staged index: range of (partner cell, owned or ghost) + offset
-              const int lx = jx - o->box.lo[0], ly = jy - o->box.lo[1], lz = jz - o->box.lo[2];
-              const int r = kRangesPerCell * ((lz * o->box.n[1] + ly) * o->box.n[0] + lx) + pass;
-              const int off = j - o->ts[r];
-              // always true by construction (the box covers every cell next to a row's cell)
-              const bool good = lx >= 0 && lx < o->box.n[0] && ly >= 0 && ly < o->box.n[1] && lz >= 0 &&
-                                lz < o->box.n[2] && off >= 0 && off < o->tb[r + 1] - o->tb[r];
-              if (!good) {
-                atomicOr(err, kErrLayout);
-              } else if (cnt < o->cap) {
-                o->nbr[(long long)cnt * o->stride] = (uint16_t)(o->tb[r] + off);
-                o->code[(long long)cnt * o->stride] = offset_code(dx, dy, dz);
-              }
-            }
-            cnt++;
+constexpr int kStageBatch = 4;  // staged entries per thread with loads in flight together
+
+// Stage positions (3 doubles per entry) of a brick's gather list into LDS,
+// plus a far sentinel at index `total` (the pad entry of the list).
+template <int T>
+__device__ __forceinline__ void stage_fp64(const int *__restrict__ gl, int total, const double4 *__restrict__ src,
+                                           double *__restrict__ lds) {
+  for (int u0 = threadIdx.x; u0 < total; u0 += T * kStageBatch) {
+    int j[kStageBatch];
+#pragma unroll
+    for (int k = 0; k < kStageBatch; k++) {
+      const int u = u0 + k * T;
+      j[k] = u < total ? gl[u] : 0;
+    }
+    double4 q[kStageBatch];
+#pragma unroll
+    for (int k = 0; k < kStageBatch; k++) q[k] = src[j[k]];
+#pragma unroll
+    for (int k = 0; k < kStageBatch; k++) {
+      const int u = u0 + k * T;
+      if (u < total) {
+        lds[3 * u] = q[k].x;
+        lds[3 * u + 1] = q[k].y;
+        lds[3 * u + 2] = q[k].z;
+      }
+    }
+  }
+  if (threadIdx.x == 0) {
+    lds[3 * total] = 1.0e200;
+    lds[3 * total + 1] = 1.0e200;
+    lds[3 * total + 2] = 1.0e200;
+  }
+}
+
+// Per-brick tables the search and emit kernels read.
+struct SearchTables {
+  const int *tstart, *tbase, *nrange;  // staged ranges (list_kernels: k_brick_ranges)
+  const int *glist;                    // staged entry -> particle index
+  const long long *gbase;              // [brick + 1]
+  const int *row0, *nrow, *vslice0;
+  int rstride;                         // 2 rmax
+  int lds_cap;                         // staged entries that fit the LDS budget (incl. pad)
+};
+
+template <int T>
+__device__ __forceinline__ int load_tb(const SearchTables &st, int b, int *tb) {
+  const int R = st.nrange[b];
+  const int *tbg = st.tbase + (long long)b * (st.rstride + 1);
+  for (int r = threadIdx.x; r <= R; r += T) tb[r] = tbg[r];
+  return R;
+}
+
+// Box coordinates of a row's cell and the staged index of the row itself.
+struct RowCell {
+  int lx, ly, lz, self;
+};
+
+__device__ __forceinline__ RowCell row_cell(int i, int c, const BrickBox &X, const Bricks &B, const int *tb,
+                                            const int *__restrict__ ts) {
+  RowCell rc;
+  rc.lx = c % B.m[0] - X.lo[0];
+  rc.ly = (c / B.m[0]) % B.m[1] - X.lo[1];
+  rc.lz = c / (B.m[0] * B.m[1]) - X.lo[2];
+  const int rs = kRangesPerCell * ((rc.lz * X.n[1] + rc.ly) * X.n[0] + rc.lx);
+  rc.self = tb[rs] + (i - ts[rs]);
+  return rc;
+}
+
+// MeshList::SearchMesh over a brick (meshlist.cc:284-336): each owned row of
+// the brick tests the owned and ghost particles of the 27 cells around its
+// cell, staged in LDS.  For a row the candidates come as 9 contiguous staged
+// runs (x-neighbouring cells are adjacent in the staged order); bit k of the
+// row's mask marks candidate k, in run order, as listed: r^2 = dx*dx + dy*dy +
+// dz*dz without fused multiply-adds, kept when !(r2 > S2), the row itself
+// skipped.  row_len receives the count.
+template <int W>
+__global__ __launch_bounds__(kWave *W) void k_search_brick(const double4 *__restrict__ pos,
+                                                          const int *__restrict__ cell, Bricks B,
+                                                          SearchTables st, double S2, int mcap,
+                                                          int *__restrict__ row_len, uint32_t *__restrict__ mask,
+                                                          int *__restrict__ err) {
+  extern __shared__ double lds64[];
+  __shared__ int tb[kMaxRanges + 1];
+  constexpr int T = kWave * W;
+  const int b = xcd_block(blockIdx.x, gridDim.x);
+  const int R = load_tb<T>(st, b, tb);
+  __syncthreads();
+  const int total = tb[R];
+  const int *gl = st.glist + st.gbase[b];
+  const bool staged = total < st.lds_cap;
+  if (staged) stage_fp64<T>(gl, total, pos, lds64);
+  __syncthreads();
+  const BrickBox X = brick_box(b, B);
+  const int *ts = st.tstart + (long long)b * st.rstride;
+  const int nrow = st.nrow[b], row0 = st.row0[b];
+  for (int v = threadIdx.x; v < nrow; v += T) {
+    const int i = row0 + v;
+    const double4 qi = pos[i];
+    const RowCell rc = row_cell(i, cell[i], X, B, tb, ts);
+    uint32_t *mk = mask + (long long)i * mcap;
+    const int nbits = mcap * 32;
+    int nbit = 0, cnt = 0;
+    uint32_t word = 0;
+    for (int dz = -1; dz <= 1; dz++) {
+      for (int dy = -1; dy <= 1; dy++) {
+        const int r0 = ((rc.lz + dz) * X.n[1] + (rc.ly + dy)) * X.n[0] + rc.lx - 1;
+        const int u0 = tb[kRangesPerCell * r0], u1 = tb[kRangesPerCell * (r0 + 3)];
+        for (int u = u0; u < u1; u++) {
+          double qx, qy, qz;
+          if (staged) {
+            qx = lds64[3 * u];
+            qy = lds64[3 * u + 1];
+            qz = lds64[3 * u + 2];
+          } else {
+            const double4 q = pos[gl[u]];
+            qx = q.x;
+            qy = q.y;
+            qz = q.z;
+          }
+          const double ex = qi.x - qx, ey = qi.y - qy, ez = qi.z - qz;
+          const double r2 = (ex * ex + ey * ey + ez * ez);
+          const bool hit = !(r2 > S2) && u != rc.self;
+          word |= (hit ? 1u : 0u) << (nbit & 31);
+          cnt += hit ? 1 : 0;
+          nbit++;
+          if ((nbit & 31) == 0) {
+            if (nbit <= nbits) mk[(nbit >> 5) - 1] = word;
+            word = 0;
           }
         }
       }
     }
+    if ((nbit & 31) != 0 && nbit <= nbits) mk[nbit >> 5] = word;
+    if (nbit > nbits) atomicOr(err, kErrSearchCap);
+    row_len[i] = cnt;
   }
-  return cnt;
-}
-
-__global__ void k_count_rows(const double4 *__restrict__ pos, const int *__restrict__ cell, int pn, Grid g,
-                             CellTables T, double S2, int *__restrict__ row_len) {
-  const int i = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
-  if (i >= pn) return;
-  row_len[i] = search_row<false>(i, pos, cell, pn, g, T, S2, nullptr, nullptr);
 }
 
 // slice length = longest row of its 64 rows, rounded up to a multiple of
-// kSliceRound so the force loop runs whole batches; slots = 64 * length
+// kSliceRound so the force loop runs whole batches; slots = 64 * length.
+// The largest slice length goes to partial (per workgroup).
 __global__ void k_slice_len(const int *__restrict__ row_len, const int *__restrict__ slice_brick,
                             const int *__restrict__ vslice0, const int *__restrict__ row0,
                             const int *__restrict__ nrow, int nslice, int *__restrict__ slen,
-                            long long *__restrict__ slots) {
+                            long long *__restrict__ slots, double *__restrict__ partial) {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= nslice) return;
-  const int b = slice_brick[s];
-  const int v0 = (s - vslice0[b]) * kWave;
-  const int v1 = min(nrow[b], v0 + kWave);
   int m = 0;
-  for (int v = v0; v < v1; v++) m = max(m, row_len[row0[b] + v]);
-  m = (m + kSliceRound - 1) / kSliceRound * kSliceRound;
-  slen[s] = m;
-  slots[s] = (long long)m * kWave;
-}
-
-struct BrickRows {
-  const int *brick_of_cell, *row0, *vslice0;
-  const int *tstart, *tbase;
-};
-
-__global__ void k_fill_rows(const double4 *__restrict__ pos, const int *__restrict__ cell, int pn, Grid g,
-                            CellTables T, double S2, Bricks B, BrickRows BR, const long long *__restrict__ sbase,
-                            const int *__restrict__ slen, uint16_t *__restrict__ nbr, uint8_t *__restrict__ ncode,
-                            int *__restrict__ err) {
-  const int i = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
-  if (i >= pn) return;
-  const int c = cell[i];
-  const int b = BR.brick_of_cell[c];
-  const int v = i - BR.row0[b];
-  const int s = BR.vslice0[b] + v / kWave, lane = v % kWave;
-  RowOut o;
-  o.box = brick_box(b, B);
-  o.ts = BR.tstart + (long long)b * kRangesPerCell * B.rmax;
-  o.tb = BR.tbase + (long long)b * (kRangesPerCell * B.rmax + 1);
-  const int len = slen[s];
-  o.nbr = nbr + sbase[s] + lane;
-  o.code = ncode + sbase[s] + lane;
-  o.stride = kWave;
-  o.cap = len;
-  const int cnt = search_row<true>(i, pos, cell, pn, g, T, S2, &o, err);
-  if (cnt > len) atomicOr(err, kErrCapacity);
-  const int R = kRangesPerCell * o.box.n[0] * o.box.n[1] * o.box.n[2];
-  const uint16_t pad = (uint16_t)o.tb[R];
-  for (int k = cnt; k < len; k++) {
-    o.nbr[(long long)k * kWave] = pad;
-    o.code[(long long)k * kWave] = (uint8_t)kZeroOffset;
+  if (s < nslice) {
+    const int b = slice_brick[s];
+    const int v0 = (s - vslice0[b]) * kWave;
+    const int v1 = min(nrow[b], v0 + kWave);
+    for (int v = v0; v < v1; v++) m = max(m, row_len[row0[b] + v]);
+    m = (m + kSliceRound - 1) / kSliceRound * kSliceRound;
+    slen[s] = m;
+    slots[s] = (long long)m * kWave;
   }
+  block_max_to_partial((double)m, partial);
 }
 
-// lanes of a brick's last slice past its rows are all padding
-__global__ void k_fill_tail(int nslice, Bricks B, const int *__restrict__ slice_brick,
-                            const int *__restrict__ vslice0, const int *__restrict__ nrow,
-                            const int *__restrict__ nrange, const int *__restrict__ tbase,
-                            const long long *__restrict__ sbase, const int *__restrict__ slen,
-                            uint16_t *__restrict__ nbr, uint8_t *__restrict__ ncode) {
-  const long long gid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  const int s = (int)(gid / kWave), lane = (int)(gid % kWave);
-  if (s >= nslice) return;
-  const int b = slice_brick[s];
-  const int v = (s - vslice0[b]) * kWave + lane;
-  if (v < nrow[b]) return;
-  const uint16_t pad = (uint16_t)tbase[(long long)b * (kRangesPerCell * B.rmax + 1) + nrange[b]];
-  const int len = slen[s];
-  for (int k = 0; k < len; k++) {
-    nbr[sbase[s] + (long long)k * kWave + lane] = pad;
-    ncode[sbase[s] + (long long)k * kWave + lane] = (uint8_t)kZeroOffset;
+// MeshList::MakeList (meshlist.cc:57-87) in sliced-ELL form: one wavefront per
+// 64-row slice turns the rows' candidate masks into staged indices and cell
+// offset codes, pads every row to the slice length with the brick's pad
+// entry, assembles the slice slot-major in LDS and copies it out in 16-byte
+// pieces.  scap = longest slice (LDS: W * scap * 64 * 3 bytes).
+template <int W>
+__global__ __launch_bounds__(kWave *W) void k_emit_brick(const int *__restrict__ cell, Bricks B, SearchTables st,
+                                                        const uint32_t *__restrict__ mask, int mcap,
+                                                        const long long *__restrict__ sbase,
+                                                        const int *__restrict__ slen, int scap,
+                                                        uint16_t *__restrict__ nbr, uint8_t *__restrict__ ncode,
+                                                        int *__restrict__ err) {
+  extern __shared__ uint4 ebuf[];
+  __shared__ int tb[kMaxRanges + 1];
+  constexpr int T = kWave * W;
+  const int b = xcd_block(blockIdx.x, gridDim.x);
+  const int R = load_tb<T>(st, b, tb);
+  __syncthreads();
+  const int total = tb[R];
+  const BrickBox X = brick_box(b, B);
+  const int *ts = st.tstart + (long long)b * st.rstride;
+  const int nrow = st.nrow[b], row0 = st.row0[b], vs0 = st.vslice0[b];
+  const int wv = threadIdx.x / kWave, lane = threadIdx.x % kWave;
+  uint16_t *bn = reinterpret_cast<uint16_t *>(ebuf) + (long long)wv * scap * kWave;
+  uint8_t *bc = reinterpret_cast<uint8_t *>(reinterpret_cast<uint16_t *>(ebuf) + (long long)W * scap * kWave) +
+                (long long)wv * scap * kWave;
+  const int nsl = (nrow + kWave - 1) / kWave;
+  for (int sl = wv; sl < nsl; sl += W) {
+    const int s = vs0 + sl;
+    const int len = slen[s];
+    const int v = sl * kWave + lane;
+    int cnt = 0;
+    if (v < nrow) {
+      const int i = row0 + v;
+      const RowCell rc = row_cell(i, cell[i], X, B, tb, ts);
+      const uint32_t *mk = mask + (long long)i * mcap;
+      int nbit = 0;
+      for (int dz = -1; dz <= 1; dz++) {
+        for (int dy = -1; dy <= 1; dy++) {
+          const int r0 = ((rc.lz + dz) * X.n[1] + (rc.ly + dy)) * X.n[0] + rc.lx - 1;
+          const int u0 = tb[kRangesPerCell * r0];
+          const int t1 = tb[kRangesPerCell * (r0 + 1)], t2 = tb[kRangesPerCell * (r0 + 2)];
+          const int n = tb[kRangesPerCell * (r0 + 3)] - u0;
+          // bits [nbit, nbit + n) of the mask belong to this run
+          for (int k = 0; k < n;) {
+            const int bit = nbit + k;
+            uint32_t w = mk[bit >> 5] >> (bit & 31);
+            const int take = min(32 - (bit & 31), n - k);
+            if (take < 32) w &= (1u << take) - 1u;
+            while (w) {
+              const int p = __ffs(w) - 1;
+              w &= w - 1u;
+              const int u = u0 + k + p;
+              const int dx = u < t1 ? -1 : (u < t2 ? 0 : 1);
+              if (cnt < len) {
+                bn[cnt * kWave + lane] = (uint16_t)u;
+                bc[cnt * kWave + lane] = offset_code(dx, dy, dz);
+              }
+              cnt++;
+            }
+            k += take;
+          }
+          nbit += n;
+        }
+      }
+    }
+    if (cnt > len) atomicOr(err, kErrCapacity);
+    for (int k = cnt; k < len; k++) {
+      bn[k * kWave + lane] = (uint16_t)total;
+      bc[k * kWave + lane] = (uint8_t)kZeroOffset;
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+    // the slice is len*64 entries at sbase (a multiple of 512): 16-byte pieces
+    uint4 *gn = reinterpret_cast<uint4 *>(nbr + sbase[s]);
+    uint4 *gc = reinterpret_cast<uint4 *>(ncode + sbase[s]);
+    const uint4 *ln = reinterpret_cast<const uint4 *>(bn);
+    const uint4 *lc = reinterpret_cast<const uint4 *>(bc);
+    for (int t = lane; t < len * 8; t += kWave) gn[t] = ln[t];
+    for (int t = lane; t < len * 4; t += kWave) gc[t] = lc[t];
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
   }
 }
 

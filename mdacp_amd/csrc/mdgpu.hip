@@ -80,13 +80,20 @@ struct mdgpu_domain {
   // bricks
   int *rank = nullptr, *brick_of_cell = nullptr, *brick_rank0 = nullptr;
   int *row0 = nullptr, *nrow = nullptr, *nsl = nullptr, *vslice0 = nullptr, *slice_brick = nullptr;
-  int *tstart = nullptr, *tbase = nullptr, *nrange = nullptr;
+  int *tstart = nullptr, *tbase = nullptr, *nrange = nullptr, *btotal = nullptr;
+  long long *gbase = nullptr;  // [nbrick + 1] first gather-list entry of each brick
+  int *glist = nullptr;        // staged entry -> particle index, brick by brick
+  long long glist_cap = 0;
   // list
   uint16_t *nbr = nullptr;   // staged partner index per list slot
   uint8_t *ncode = nullptr;  // partner cell offset per list slot
   long long *sbase = nullptr, *slots64 = nullptr;
   int *slen = nullptr, *row_len = nullptr;
   // build scratch
+  uint32_t *mask = nullptr;  // candidate bitmask per row, mcap words
+  size_t mask_cap = 0;
+  int mcap = 32;
+  double *spart = nullptr;  // per-workgroup maxima of slice lengths
   int *slot = nullptr, *perm = nullptr, *nimg = nullptr, *goff = nullptr;
   int *t_gsrc = nullptr, *t_gcode = nullptr, *t_gcell = nullptr;
   long long *key = nullptr, *scan_scratch = nullptr;
@@ -234,6 +241,8 @@ int set_geometry(mdgpu_domain *d) {
   CHK(dalloc(d, &d->vslice0, d->nbrick));
   CHK(dalloc(d, &d->nrange, d->nbrick));
   CHK(dalloc(d, &d->bpart, d->nbrick));
+  CHK(dalloc(d, &d->btotal, d->nbrick));
+  CHK(dalloc(d, &d->gbase, d->nbrick + 1));
   CHK(dalloc(d, &d->tstart, (size_t)d->nbrick * kRangesPerCell * d->rmax));
   CHK(dalloc(d, &d->tbase, (size_t)d->nbrick * (kRangesPerCell * d->rmax + 1)));
   d->initialized = false;
@@ -365,7 +374,7 @@ int read_err_flags(mdgpu_domain *d) {
     d->err = "internal: a listed partner lies outside its brick's staged ranges";
     return MDGPU_ERR_HIP;
   }
-  if (e & kErrCapacity) {
+  if (e & (kErrCapacity | kErrSearchCap)) {
     d->err = "pair list capacity";
     return MDGPU_ERR_CAPACITY;
   }
@@ -408,6 +417,72 @@ int cell_sort(mdgpu_domain *d, const int *cellv, int n, int *counts, int *starts
   hipLaunchKernelGGL(k_sort_cells, dim3(nblk(d->ncell)), dim3(kBlock), 0, st, d->perm, starts, counts, keys,
                      d->ncell);
   return check_launch(d, "cell sort");
+}
+
+template <class K>
+void allow_lds(K kernel, size_t bytes) {
+  if (bytes > 64 * 1024)
+    (void)hipFuncSetAttribute((const void *)kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+}
+
+// staged entries of bytes_per_entry that fit the LDS budget (incl. the pad entry)
+int lds_cap_for(const mdgpu_domain *d, int bytes_per_entry) {
+  const int budget = std::min(d->lds_kb, d->lds_max_kb) * 1024 / bytes_per_entry;
+  return std::max(1, std::min(d->max_staged + 1, budget));
+}
+
+SearchTables search_tables(const mdgpu_domain *d) {
+  SearchTables t;
+  t.tstart = d->tstart;
+  t.tbase = d->tbase;
+  t.nrange = d->nrange;
+  t.glist = d->glist;
+  t.gbase = d->gbase;
+  t.row0 = d->row0;
+  t.nrow = d->nrow;
+  t.vslice0 = d->vslice0;
+  t.rstride = kRangesPerCell * d->rmax;
+  t.lds_cap = lds_cap_for(d, 3 * sizeof(double));
+  return t;
+}
+
+template <int W>
+void launch_search_w(mdgpu_domain *d, const Bricks &B, const SearchTables &t) {
+  const size_t shm = (size_t)t.lds_cap * 3 * sizeof(double);
+  allow_lds(k_search_brick<W>, shm);
+  hipLaunchKernelGGL(k_search_brick<W>, dim3(d->nbrick), dim3(kWave * W), shm, d->stream, d->pos, d->cell, B, t,
+                     d->S2, d->mcap, d->row_len, d->mask, d->i32);
+}
+
+int launch_search(mdgpu_domain *d, const Bricks &B, const SearchTables &t) {
+  switch (d->waves) {
+    case 2: launch_search_w<2>(d, B, t); break;
+    case 4: launch_search_w<4>(d, B, t); break;
+    default: launch_search_w<8>(d, B, t); break;
+  }
+  return check_launch(d, "neighbour search");
+}
+
+template <int W>
+void launch_emit_w(mdgpu_domain *d, const Bricks &B, const SearchTables &t, int scap) {
+  const size_t shm = (size_t)W * scap * kWave * 3;
+  allow_lds(k_emit_brick<W>, shm);
+  hipLaunchKernelGGL(k_emit_brick<W>, dim3(d->nbrick), dim3(kWave * W), shm, d->stream, d->cell, B, t, d->mask,
+                     d->mcap, d->sbase, d->slen, scap, d->nbr, d->ncode, d->i32);
+}
+
+// the emit kernel keeps one slice (scap x 64 entries of 3 bytes) per wavefront in LDS
+int launch_emit(mdgpu_domain *d, const Bricks &B, const SearchTables &t, int scap) {
+  const size_t per_wave = (size_t)scap * kWave * 3;
+  const size_t limit = (size_t)d->lds_max_kb * 1024;
+  if (4 * per_wave <= 80 * 1024) launch_emit_w<4>(d, B, t, scap);
+  else if (2 * per_wave <= limit) launch_emit_w<2>(d, B, t, scap);
+  else if (per_wave <= limit) launch_emit_w<1>(d, B, t, scap);
+  else {
+    d->err = "a pair-list slice is too long for the emit kernel's LDS";
+    return MDGPU_ERR_CAPACITY;
+  }
+  return check_launch(d, "neighbour list emit");
 }
 
 // MDManager::MakePairList for the single domain (mdmanager.cc:261-280).
@@ -493,12 +568,13 @@ int make_pair_list(mdgpu_domain *d) {
     CHK(dalloc(d, &d->sbase, sc));
     CHK(dalloc(d, &d->slots64, sc));
     CHK(dalloc(d, &d->slice_brick, sc));
+    CHK(dalloc(d, &d->spart, nblk(sc)));
     d->slice_cap = sc;
   }
   hipLaunchKernelGGL(k_slice_brick, dim3(nblk(d->nbrick)), dim3(kBlock), 0, st, d->vslice0, d->nsl, d->nbrick,
                      d->slice_brick);
   hipLaunchKernelGGL(k_brick_ranges, dim3(nblk(d->nbrick)), dim3(kBlock), 0, st, B, T, d->nbrick, pn, d->tstart,
-                     d->tbase, d->nrange, d->bpart);
+                     d->tbase, d->nrange, d->btotal, d->bpart);
   hipLaunchKernelGGL(k_reduce_max, dim3(1), dim3(kBlock), 0, st, d->bpart, (int)nblk(d->nbrick), d->u64 + 2);
   double mstaged = 0.0;
   CHK(read_max(d, 2, &mstaged));
@@ -507,19 +583,53 @@ int make_pair_list(mdgpu_domain *d) {
     d->err = "a brick's neighbourhood exceeds 16-bit indexing (extreme local density): use smaller bricks";
     return MDGPU_ERR_CAPACITY;
   }
+  // gather lists: gbase = exclusive scan of the staged counts, gbase[nbrick] = total
+  exclusive_scan<int, long long>(d->btotal, d->gbase, d->nbrick, d->scan_scratch, d->i64, st);
+  HIPCHK(d, hipMemcpyAsync(d->gbase + d->nbrick, d->i64, sizeof(long long), hipMemcpyDeviceToDevice, st));
+  long long gtotal = 0;
+  CHK(read_total(d, &gtotal));
+  if (gtotal > d->glist_cap) {
+    const long long gc = gtotal + gtotal / 8 + 1024;
+    CHK(dalloc(d, &d->glist, (size_t)gc));
+    d->glist_cap = gc;
+  }
+  hipLaunchKernelGGL(k_brick_glist, dim3(d->nbrick), dim3(kBlock), 0, st, kRangesPerCell * d->rmax, d->tstart,
+                     d->tbase, d->nrange, d->gbase, d->glist);
   // 5. MeshList::MakeList: full neighbour list in sliced-ELL layout
   long long slots = 0;
+  int scap = 0;
+  const SearchTables STb = search_tables(d);
   if (pn > 0) {
-    hipLaunchKernelGGL(k_count_rows, dim3(nblk(pn)), dim3(kBlock), 0, st, d->pos, d->cell, pn, g, T, d->S2,
-                       d->row_len);
+    // candidate masks: retried with room for more candidates if a row overflows
+    for (;;) {
+      const size_t need = (size_t)pn * d->mcap;
+      if (need > d->mask_cap) {
+        CHK(dalloc(d, &d->mask, need + need / 8));
+        d->mask_cap = need + need / 8;
+      }
+      HIPCHK(d, hipMemsetAsync(d->i32, 0, sizeof(int), st));
+      CHK(launch_search(d, B, STb));
+      HIPCHK(d, hipMemcpyAsync(d->h_i32, d->i32, sizeof(int) * 4, hipMemcpyDeviceToHost, st));
+      CHK(sync(d));
+      if (!(d->h_i32[0] & kErrSearchCap)) break;
+      if (d->mcap >= (1 << 16)) {
+        d->err = "a row has too many neighbour candidates";
+        return MDGPU_ERR_CAPACITY;
+      }
+      d->mcap *= 2;
+    }
     // number of listed (row, partner) entries, for accounting
     exclusive_scan<int, long long>(d->row_len, d->key, pn, d->scan_scratch, d->i64, st);
     CHK(read_total(d, &d->nbr_entries));
     hipLaunchKernelGGL(k_slice_len, dim3(nblk(d->nslice)), dim3(kBlock), 0, st, d->row_len, d->slice_brick,
-                       d->vslice0, d->row0, d->nrow, d->nslice, d->slen, d->slots64);
+                       d->vslice0, d->row0, d->nrow, d->nslice, d->slen, d->slots64, d->spart);
+    hipLaunchKernelGGL(k_reduce_max, dim3(1), dim3(kBlock), 0, st, d->spart, (int)nblk(d->nslice), d->u64 + 3);
     CHK(ensure_scan(d, d->nslice));
     exclusive_scan<long long, long long>(d->slots64, d->sbase, d->nslice, d->scan_scratch, d->i64, st);
     CHK(read_total(d, &slots));
+    double smax = 0.0;
+    CHK(read_max(d, 3, &smax));
+    scap = (int)smax;
   } else {
     d->nbr_entries = 0;
   }
@@ -530,14 +640,7 @@ int make_pair_list(mdgpu_domain *d) {
     d->nbr_cap = nc;
   }
   d->nbr_slots = slots;
-  if (pn > 0) {
-    const BrickRows BR{d->brick_of_cell, d->row0, d->vslice0, d->tstart, d->tbase};
-    hipLaunchKernelGGL(k_fill_rows, dim3(nblk(pn)), dim3(kBlock), 0, st, d->pos, d->cell, pn, g, T, d->S2, B, BR,
-                       d->sbase, d->slen, d->nbr, d->ncode, d->i32);
-    hipLaunchKernelGGL(k_fill_tail, dim3(nblk((long long)d->nslice * kWave)), dim3(kBlock), 0, st, d->nslice, B,
-                       d->slice_brick, d->vslice0, d->nrow, d->nrange, d->tbase, d->sbase, d->slen, d->nbr,
-                       d->ncode);
-  }
+  if (pn > 0 && scap > 0) CHK(launch_emit(d, B, STb, scap));
   CHK(check_launch(d, "neighbour list"));
   CHK(read_err_flags(d));
   d->initialized = true;
@@ -655,65 +758,55 @@ BrickList brick_list(const mdgpu_domain *d, int bytes_per_entry) {
   bl.code = d->ncode;
   bl.sbase = d->sbase;
   bl.slen = d->slen;
-  bl.tstart = d->tstart;
-  bl.tbase = d->tbase;
-  bl.nrange = d->nrange;
+  bl.glist = d->glist;
+  bl.gbase = d->gbase;
   bl.row0 = d->row0;
   bl.nrow = d->nrow;
   bl.vslice0 = d->vslice0;
-  bl.rstride = kRangesPerCell * d->rmax;
-  const int budget = std::min(d->lds_kb, d->lds_max_kb) * 1024 / bytes_per_entry;
-  bl.lds_cap = std::max(1, std::min(d->max_staged + 1, budget));
+  bl.lds_cap = lds_cap_for(d, bytes_per_entry);
   return bl;
 }
 
-template <class K>
-void allow_lds(K kernel, size_t bytes) {
-  if (bytes > 64 * 1024)
-    (void)hipFuncSetAttribute((const void *)kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
-}
-
 template <int W, int U>
-void launch_fp64(mdgpu_domain *d, const double4 *src, const double4 *gsrc, double4 *pos_out, int epi,
-                 const ForceConsts &fc, const LangevinConsts &lc, double *vb) {
+void launch_fp64(mdgpu_domain *d, const double4 *src, double4 *pos_out, int epi, const ForceConsts &fc,
+                 const LangevinConsts &lc, double *vb) {
   const BrickList bl = brick_list(d, 3 * sizeof(double));
   const size_t shm = (size_t)bl.lds_cap * 3 * sizeof(double);
   allow_lds(k_force_brick_fp64<W, U>, shm);
-  hipLaunchKernelGGL((k_force_brick_fp64<W, U>), dim3(d->nbrick), dim3(kWave * W), shm, d->stream, src, gsrc,
-                     d->mom, pos_out, bl, fc, epi, lc, d->id, vb);
+  hipLaunchKernelGGL((k_force_brick_fp64<W, U>), dim3(d->nbrick), dim3(kWave * W), shm, d->stream, src, d->mom,
+                     pos_out, bl, fc, epi, lc, d->id, vb);
 }
 
 template <int W, int U, bool K>
-void launch_mixed(mdgpu_domain *d, const double4 *src, const float4 *rel, const float4 *grel, double4 *pos_out,
-                  int epi, const ForceConstsF &f32, double dt2, const LangevinConsts &lc, double *vb) {
+void launch_mixed(mdgpu_domain *d, const double4 *src, const float4 *rel, double4 *pos_out, int epi,
+                  const ForceConstsF &f32, double dt2, const LangevinConsts &lc, double *vb) {
   const BrickList bl = brick_list(d, sizeof(float4));
   const size_t shm = (size_t)bl.lds_cap * sizeof(float4);
   allow_lds(k_force_brick_mixed<W, U, K>, shm);
-  hipLaunchKernelGGL((k_force_brick_mixed<W, U, K>), dim3(d->nbrick), dim3(kWave * W), shm, d->stream, rel, grel,
-                     src, d->mom, pos_out, bl, f32, dt2, epi, lc, d->id, vb);
+  hipLaunchKernelGGL((k_force_brick_mixed<W, U, K>), dim3(d->nbrick), dim3(kWave * W), shm, d->stream, rel, src,
+                     d->mom, pos_out, bl, f32, dt2, epi, lc, d->id, vb);
 }
 
 template <int W>
-void launch_force_w(mdgpu_domain *d, const double4 *src, const double4 *gsrc, const float4 *rel,
-                    const float4 *grel, double4 *pos_out, int epi, const ForceConsts &fc, const ForceConstsF &f32,
-                    const LangevinConsts &lc, double *vb) {
+void launch_force_w(mdgpu_domain *d, const double4 *src, const float4 *rel, double4 *pos_out, int epi,
+                    const ForceConsts &fc, const ForceConstsF &f32, const LangevinConsts &lc, double *vb) {
   if (d->precision == MDGPU_FP64) {
-    if (d->unroll == 4) launch_fp64<W, 4>(d, src, gsrc, pos_out, epi, fc, lc, vb);
-    else launch_fp64<W, 8>(d, src, gsrc, pos_out, epi, fc, lc, vb);
+    if (d->unroll == 4) launch_fp64<W, 4>(d, src, pos_out, epi, fc, lc, vb);
+    else launch_fp64<W, 8>(d, src, pos_out, epi, fc, lc, vb);
   } else if (d->kahan) {
-    if (d->unroll == 4) launch_mixed<W, 4, true>(d, src, rel, grel, pos_out, epi, f32, fc.dt2, lc, vb);
-    else launch_mixed<W, 8, true>(d, src, rel, grel, pos_out, epi, f32, fc.dt2, lc, vb);
+    if (d->unroll == 4) launch_mixed<W, 4, true>(d, src, rel, pos_out, epi, f32, fc.dt2, lc, vb);
+    else launch_mixed<W, 8, true>(d, src, rel, pos_out, epi, f32, fc.dt2, lc, vb);
   } else {
-    if (d->unroll == 4) launch_mixed<W, 4, false>(d, src, rel, grel, pos_out, epi, f32, fc.dt2, lc, vb);
-    else launch_mixed<W, 8, false>(d, src, rel, grel, pos_out, epi, f32, fc.dt2, lc, vb);
+    if (d->unroll == 4) launch_mixed<W, 4, false>(d, src, rel, pos_out, epi, f32, fc.dt2, lc, vb);
+    else launch_mixed<W, 8, false>(d, src, rel, pos_out, epi, f32, fc.dt2, lc, vb);
   }
 }
 
-// One pair-force launch.  Rows and owned partners are read from `src`
-// (`rel` in FP32), ghost partners from `gsrc` (`grel`) at [pn, pn + ng);
-// `pos_out` receives the drifted (or copied) owned positions.
-int launch_force(mdgpu_domain *d, const double4 *src, const double4 *gsrc, const float4 *rel, const float4 *grel,
-                 double4 *pos_out, int epilogue, bool want_vmax) {
+// One pair-force launch.  Rows and partners (owned, and ghosts at
+// [pn, pn + ng)) are read from `src` (`rel` in FP32); `pos_out` receives the
+// drifted (or copied) owned positions.
+int launch_force(mdgpu_domain *d, const double4 *src, const float4 *rel, double4 *pos_out, int epilogue,
+                 bool want_vmax) {
   if (d->pn == 0) return MDGPU_OK;
   hipStream_t st = d->stream;
   const ForceConsts fc = fconsts(d);
@@ -723,9 +816,9 @@ int launch_force(mdgpu_domain *d, const double4 *src, const double4 *gsrc, const
   const int ev0 = d->force_launches * 2;
   if (d->timing && ev0 + 1 < (int)d->ev.size()) HIPCHK(d, hipEventRecord(d->ev[ev0], st));
   switch (d->waves) {
-    case 2: launch_force_w<2>(d, src, gsrc, rel, grel, pos_out, epilogue, fc, f32, lc, vb); break;
-    case 8: launch_force_w<8>(d, src, gsrc, rel, grel, pos_out, epilogue, fc, f32, lc, vb); break;
-    default: launch_force_w<4>(d, src, gsrc, rel, grel, pos_out, epilogue, fc, f32, lc, vb); break;
+    case 2: launch_force_w<2>(d, src, rel, pos_out, epilogue, fc, f32, lc, vb); break;
+    case 8: launch_force_w<8>(d, src, rel, pos_out, epilogue, fc, f32, lc, vb); break;
+    default: launch_force_w<4>(d, src, rel, pos_out, epilogue, fc, f32, lc, vb); break;
   }
   if (d->timing && ev0 + 1 < (int)d->ev.size()) {
     HIPCHK(d, hipEventRecord(d->ev[ev0 + 1], st));
@@ -793,7 +886,7 @@ int force_in_place(mdgpu_domain *d) {
   if (d->precision == MDGPU_MIXED && d->tn > 0)
     hipLaunchKernelGGL(k_to_rel, dim3(nblk(d->tn)), dim3(kBlock), 0, d->stream, d->pos, d->cell, d->tn,
                        origins_of(d), d->rel);
-  CHK(launch_force(d, d->pos, d->pos, d->rel, d->rel, d->pos, kKickOnly, false));
+  CHK(launch_force(d, d->pos, d->rel, d->pos, kKickOnly, false));
   d->vmax_valid = false;
   return MDGPU_OK;
 }
@@ -850,7 +943,7 @@ int calculate_step(mdgpu_domain *d) {
       CHK(check_launch(d, "drift+refresh"));
     }
     const int epi = d->thermostat == MDGPU_LANGEVIN ? kKickLangevin : kKickDrift;
-    CHK(launch_force(d, d->pos_tmp, d->pos, d->rel_tmp, d->rel_tmp, d->pos, epi, true));
+    CHK(launch_force(d, d->pos_tmp, d->rel_tmp, d->pos, epi, true));
     d->vmax_valid = true;
   }
   d->sim_time += d->dt;
@@ -861,7 +954,7 @@ int calculate_step(mdgpu_domain *d) {
 template <int W>
 void launch_observe(mdgpu_domain *d, const BrickList &bl, size_t shm) {
   allow_lds(k_observe_brick<W>, shm);
-  hipLaunchKernelGGL(k_observe_brick<W>, dim3(d->nbrick), dim3(kWave * W), shm, d->stream, d->pos, d->pos,
+  hipLaunchKernelGGL(k_observe_brick<W>, dim3(d->nbrick), dim3(kWave * W), shm, d->stream, d->pos,
                      d->mom, bl, d->CL2, d->C2, d->C0, d->partial);
 }
 
@@ -939,7 +1032,7 @@ int mdgpu_destroy(mdgpu_domain *d) {
   void *bufs[] = {d->pos, d->pos_tmp, d->pos_old, d->mom, d->mom_alt, d->rel, d->rel_tmp, d->id, d->id_alt,
                   d->cell, d->cell_alt, d->gsrc, d->gcode, d->c_ocount, d->c_ostart, d->c_gcount, d->c_gstart,
                   d->count_r, d->start_r, d->org, d->rank, d->brick_of_cell, d->brick_rank0, d->row0, d->nrow,
-                  d->nsl, d->vslice0, d->slice_brick, d->tstart, d->tbase, d->nrange, d->nbr, d->ncode,
+                  d->nsl, d->vslice0, d->slice_brick, d->tstart, d->tbase, d->nrange, d->btotal, d->gbase, d->glist, d->mask, d->spart, d->nbr, d->ncode,
                   d->sbase, d->slots64, d->slen, d->row_len, d->slot, d->perm, d->nimg, d->goff, d->t_gsrc,
                   d->t_gcode, d->t_gcell, d->key, d->scan_scratch, d->disp, d->partial, d->maxpart, d->bpart, d->cand,
                   d->u64, d->i64, d->i32, d->d3};
@@ -1226,11 +1319,10 @@ int mdgpu_get_pairs(mdgpu_domain *d, int64_t *npairs, int32_t *out3) {
     d->err = "pair list not built";
     return MDGPU_ERR_ARGUMENT;
   }
-  const int pn = d->pn, ng = d->ng, nbk = d->nbrick, rs = kRangesPerCell * d->rmax;
+  const int pn = d->pn, ng = d->ng, nbk = d->nbrick;
   std::vector<uint16_t> nbr((size_t)d->nbr_slots);
-  std::vector<long long> sb(d->nslice);
-  std::vector<int> sl(d->nslice), ids(pn), cl(pn), r0(nbk), vs0(nbk), nr(nbk), gs(ng), gc(ng);
-  std::vector<int> ts((size_t)nbk * rs), tb((size_t)nbk * (rs + 1));
+  std::vector<long long> sb(d->nslice), gb(nbk + 1);
+  std::vector<int> sl(d->nslice), ids(pn), cl(pn), r0(nbk), vs0(nbk), gs(ng), gc(ng);
   if (d->nbr_slots) HIPCHK(d, hipMemcpyAsync(nbr.data(), d->nbr, nbr.size() * 2, hipMemcpyDeviceToHost, d->stream));
   if (d->nslice) {
     HIPCHK(d, hipMemcpyAsync(sb.data(), d->sbase, d->nslice * 8, hipMemcpyDeviceToHost, d->stream));
@@ -1246,23 +1338,20 @@ int mdgpu_get_pairs(mdgpu_domain *d, int64_t *npairs, int32_t *out3) {
   }
   HIPCHK(d, hipMemcpyAsync(r0.data(), d->row0, nbk * 4, hipMemcpyDeviceToHost, d->stream));
   HIPCHK(d, hipMemcpyAsync(vs0.data(), d->vslice0, nbk * 4, hipMemcpyDeviceToHost, d->stream));
-  HIPCHK(d, hipMemcpyAsync(nr.data(), d->nrange, nbk * 4, hipMemcpyDeviceToHost, d->stream));
-  HIPCHK(d, hipMemcpyAsync(ts.data(), d->tstart, ts.size() * 4, hipMemcpyDeviceToHost, d->stream));
-  HIPCHK(d, hipMemcpyAsync(tb.data(), d->tbase, tb.size() * 4, hipMemcpyDeviceToHost, d->stream));
+  HIPCHK(d, hipMemcpyAsync(gb.data(), d->gbase, (nbk + 1) * 8, hipMemcpyDeviceToHost, d->stream));
   CHK(sync(d));
+  std::vector<int> gl((size_t)gb[nbk]);
+  if (!gl.empty()) HIPCHK(d, hipMemcpy(gl.data(), d->glist, gl.size() * 4, hipMemcpyDeviceToHost));
   int64_t cnt = 0;
   for (int i = 0; i < pn; i++) {
     const int b = d->h_brick_of_cell[cl[i]];
     const int v = i - r0[b];
     const int s = vs0[b] + v / kWave, lane = v % kWave;
-    const int *tbb = tb.data() + (size_t)b * (rs + 1);
-    const int *tsb = ts.data() + (size_t)b * rs;
-    const int R = nr[b];
+    const long long total = gb[b + 1] - gb[b];
     for (int k = 0; k < sl[s]; k++) {
       const int e = nbr[sb[s] + (long long)k * kWave + lane];
-      if (e >= tbb[R]) continue;  // padding
-      const int r = (int)(std::upper_bound(tbb, tbb + R + 1, e) - tbb) - 1;
-      const int j = tsb[r] + (e - tbb[r]);
+      if (e >= total) continue;  // padding
+      const int j = gl[gb[b] + e];
       const int a = ids[i];
       int bb, code;
       if (j < pn) {

@@ -71,11 +71,12 @@ __global__ void k_to_rel(const double4 *__restrict__ pos, const int *__restrict_
 }
 
 // Fused first half drift + halo refresh:
-//   owned:  tmp[i] = q_i + p_i*dt2         (and its cell-relative FP32 copy)
-//   ghost:  pos[g] = (q_src + p_src*dt2) + shift*L   (and its FP32 copy)
+//   owned:  tmp[i] = q_i + p_i*dt2                    (and its cell-relative FP32 copy)
+//   ghost:  pos[g] = tmp[g] = (q_src + p_src*dt2) + shift*L   (and its FP32 copy)
 // (the same two roundings as UpdatePositionHalf then SendBorderParticles).
-// `pos` keeps the owned positions until the force kernel writes the drifted
-// ones from `tmp`; the ghosts stay in `pos` until the next refresh.
+// `tmp` then holds every position the force kernel reads; `pos` keeps the
+// owned positions until the force kernel writes the drifted ones, and the
+// ghosts until the next refresh.
 template <bool kRel>
 __global__ void k_drift_refresh(double4 *__restrict__ pos, const double4 *__restrict__ mom,
                                 double4 *__restrict__ tmp, const int *__restrict__ gsrc,
@@ -102,6 +103,7 @@ __global__ void k_drift_refresh(double4 *__restrict__ pos, const double4 *__rest
     const double3 y = image_of(q, gcode[k], Lx, Ly, Lz);
     const double4 g = make_double4(y.x, y.y, y.z, 0.0);
     pos[i] = g;
+    tmp[i] = g;
     if (kRel) rel[i] = to_rel(g, cell[i], O.x, O.y, O.z, O.mx, O.my);
   }
 }
