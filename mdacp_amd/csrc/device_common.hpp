@@ -11,6 +11,11 @@ constexpr int kXcds = 8;       // MI355X: 8 XCDs, workgroups dispatched round-ro
 constexpr int kMaxParticles = (1 << 26) - 2;        // owned + ghosts + sentinel per domain
 constexpr uint32_t kZeroOffset = 0x15u;             // cell offset (0,0,0): 01 01 01
 constexpr int kSliceRound = 8;                      // slice lengths are multiples of this
+constexpr int kListTail = 2 * kSliceRound * kWave;  // readable padding after the list arrays
+
+// Position of slot k of lane `lane` inside a 64-row slice of the list:
+// slot-major, so slot k of the slice's 64 rows is one contiguous 128-byte run.
+__host__ __device__ __forceinline__ long long slot_of(int k, int lane) { return (long long)k * kWave + lane; }
 
 // Error bits raised by kernels; read back at the existing sync points.
 constexpr int kErrInvalidPosition = 1;

@@ -53,6 +53,113 @@ __device__ __forceinline__ double rcp64(double x) {
   return fma(r, e, r);
 }
 
+// The FP64 kick of one row (one lane of a 64-row slice) over its list
+// entries, U per batch; the next batch's indices are loaded while the current
+// batch is computed.  Partners come from LDS (kStaged) or through the gather
+// list from global memory.  Index loads are unconditional: reading up to U
+// slots past the slice stays inside the next slice or the padding the list
+// arrays carry at their end (kListTail), and those entries are never used.
+// The cutoff is a select, not a branch (padding entries point at a far
+// sentinel).
+template <bool kStaged, int U>
+__device__ __forceinline__ double3 row_kick_fp64(const uint16_t *__restrict__ row, int len, const double4 qi,
+                                                 const double *__restrict__ lds, const double4 *__restrict__ src,
+                                                 const int *__restrict__ gl, int total, const ForceConsts &fc) {
+  double ax = 0.0, ay = 0.0, az = 0.0;
+  int e[U];
+#pragma unroll
+  for (int u = 0; u < U; u++) e[u] = row[slot_of(u, 0)];
+  for (int k = 0; k < len; k += U) {
+    double qx[U], qy[U], qz[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      if (kStaged) {
+        const double *q = lds + 3 * e[u];
+        qx[u] = q[0];
+        qy[u] = q[1];
+        qz[u] = q[2];
+      } else if (e[u] < total) {
+        const double4 q = src[gl[e[u]]];
+        qx[u] = q.x;
+        qy[u] = q.y;
+        qz[u] = q.z;
+      } else {
+        qx[u] = qy[u] = qz[u] = 1.0e200;
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < U; u++) e[u] = row[slot_of(k + U + u, 0)];
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      const double dx = qx[u] - qi.x, dy = qy[u] - qi.y, dz = qz[u] - qi.z;
+      const double r2 = fma(dx, dx, fma(dy, dy, dz * dz));
+      const double ir2 = rcp64(r2);
+      const double ir6 = ir2 * ir2 * ir2;
+      double df = fma(ir2 * ir6, fma(-fc.B, ir6, fc.A), fc.C);
+      df = (r2 <= fc.CL2) ? df : 0.0;
+      ax = fma(df, dx, ax);
+      ay = fma(df, dy, ay);
+      az = fma(df, dz, az);
+    }
+  }
+  return make_double3(ax, ay, az);
+}
+
+// Mixed-precision kick of one row: FP32 cell-relative partner coordinates,
+// FP32 pair forces, sums compensated (kKahan) or plain; same batching and
+// loads as row_kick_fp64.  acc = (sum, compensation) per component.
+template <bool kStaged, bool kKahan, int U>
+__device__ __forceinline__ void row_kick_mixed(const uint16_t *__restrict__ row, const uint8_t *__restrict__ crow,
+                                               int len, const float4 ri, const float4 *__restrict__ lds,
+                                               const float4 *__restrict__ offs, const float4 *__restrict__ rel,
+                                               const int *__restrict__ gl, int total, const ForceConstsF &fc,
+                                               float (&acc)[6]) {
+  float ax = 0.f, ay = 0.f, az = 0.f, cx = 0.f, cy = 0.f, cz = 0.f;
+  int e[U], c[U];
+#pragma unroll
+  for (int u = 0; u < U; u++) {
+    e[u] = row[slot_of(u, 0)];
+    c[u] = crow[slot_of(u, 0)];
+  }
+  for (int k = 0; k < len; k += U) {
+    float4 rj[U];
+    int cc[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      if (kStaged) rj[u] = lds[e[u]];
+      else rj[u] = e[u] < total ? rel[gl[e[u]]] : make_float4(1.0e30f, 1.0e30f, 1.0e30f, 0.0f);
+      cc[u] = c[u];
+    }
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      e[u] = row[slot_of(k + U + u, 0)];
+      c[u] = crow[slot_of(k + U + u, 0)];
+    }
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      const float4 o = offs[cc[u]];
+      const float dx = (rj[u].x - ri.x) + o.x, dy = (rj[u].y - ri.y) + o.y, dz = (rj[u].z - ri.z) + o.z;
+      const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+      const float ir2 = __builtin_amdgcn_rcpf(r2);
+      const float ir6 = ir2 * ir2 * ir2;
+      float df = fmaf(ir2 * ir6, fmaf(-fc.B, ir6, fc.A), fc.C);
+      df = (r2 <= fc.CL2) ? df : 0.0f;
+      if (kKahan) {
+        float y, q;
+        y = fmaf(df, dx, -cx); q = ax + y; cx = (q - ax) - y; ax = q;
+        y = fmaf(df, dy, -cy); q = ay + y; cy = (q - ay) - y; ay = q;
+        y = fmaf(df, dz, -cz); q = az + y; cz = (q - az) - y; az = q;
+      } else {
+        ax = fmaf(df, dx, ax);
+        ay = fmaf(df, dy, ay);
+        az = fmaf(df, dz, az);
+      }
+    }
+  }
+  acc[0] = ax; acc[1] = ay; acc[2] = az;
+  acc[3] = cx; acc[4] = cy; acc[5] = cz;
+}
+
 __device__ __forceinline__ double finish_row(int i, double4 p, const double4 qi, double4 *__restrict__ mom,
                                              double4 *__restrict__ pos_out, int epilogue, double dt2,
                                              const LangevinConsts &lc, const int *__restrict__ id) {
@@ -70,9 +177,7 @@ __device__ __forceinline__ double finish_row(int i, double4 p, const double4 qi,
   return p.x * p.x + p.y * p.y + p.z * p.z;
 }
 
-// FP64 kick.  Entries are consumed in batches of U: the next batch's indices
-// are loaded before the current batch is computed.  The cutoff is a select,
-// not a branch (padding entries point at a far sentinel: r2 = inf).
+// FP64 kick: one workgroup per brick, wavefronts over the brick's slices.
 template <int W, int U>
 __global__ __launch_bounds__(kWave *W) void k_force_brick_fp64(const double4 *__restrict__ src,
                                                               double4 *__restrict__ mom,
@@ -99,58 +204,15 @@ __global__ __launch_bounds__(kWave *W) void k_force_brick_fp64(const double4 *__
     const bool ok = v < nrow;
     const int i = row0 + (ok ? v : nrow - 1);
     const double4 qi = src[i];
-    double ax = 0.0, ay = 0.0, az = 0.0;
     const int len = bl.slen[s];
     const uint16_t *row = bl.nbr + bl.sbase[s] + lane;
-    int e[U];
-    // len is 0 or a multiple of kSliceRound >= U: an empty slice reads nothing
-#pragma unroll
-    for (int u = 0; u < U; u++) e[u] = len > 0 ? row[(long long)u * kWave] : 0;
-    for (int k = 0; k < len; k += U) {
-      double qx[U], qy[U], qz[U];
-      if (staged) {
-#pragma unroll
-        for (int u = 0; u < U; u++) {
-          const double *q = lds64 + 3 * e[u];
-          qx[u] = q[0];
-          qy[u] = q[1];
-          qz[u] = q[2];
-        }
-      } else {
-#pragma unroll
-        for (int u = 0; u < U; u++) {
-          if (e[u] < total) {
-            const double4 q = src[gl[e[u]]];
-            qx[u] = q.x;
-            qy[u] = q.y;
-            qz[u] = q.z;
-          } else {
-            qx[u] = qy[u] = qz[u] = 1.0e200;
-          }
-        }
-      }
-      if (k + U < len) {
-#pragma unroll
-        for (int u = 0; u < U; u++) e[u] = row[(long long)(k + U + u) * kWave];
-      }
-#pragma unroll
-      for (int u = 0; u < U; u++) {
-        const double dx = qx[u] - qi.x, dy = qy[u] - qi.y, dz = qz[u] - qi.z;
-        const double r2 = fma(dx, dx, fma(dy, dy, dz * dz));
-        const double ir2 = rcp64(r2);
-        const double ir6 = ir2 * ir2 * ir2;
-        double df = fma(ir2 * ir6, fma(-fc.B, ir6, fc.A), fc.C);
-        df = (r2 <= fc.CL2) ? df : 0.0;
-        ax = fma(df, dx, ax);
-        ay = fma(df, dy, ay);
-        az = fma(df, dz, az);
-      }
-    }
+    const double3 a = staged ? row_kick_fp64<true, U>(row, len, qi, lds64, src, gl, total, fc)
+                             : row_kick_fp64<false, U>(row, len, qi, lds64, src, gl, total, fc);
     if (ok) {
       double4 p = mom[i];
-      p.x += ax;
-      p.y += ay;
-      p.z += az;
+      p.x += a.x;
+      p.y += a.y;
+      p.z += a.z;
       vmax = fmax(vmax, finish_row(i, p, qi, mom, pos_out, epilogue, fc.dt2, lc, id));
     }
   }
@@ -209,63 +271,17 @@ __global__ __launch_bounds__(kWave *W) void k_force_brick_mixed(
     const bool ok = v < nrow;
     const int i = row0 + (ok ? v : nrow - 1);
     const float4 ri = rel[i];
-    float ax = 0.f, ay = 0.f, az = 0.f, cx = 0.f, cy = 0.f, cz = 0.f;
     const int len = bl.slen[s];
-    const long long base = bl.sbase[s] + lane;
-    const uint16_t *row = bl.nbr + base;
-    const uint8_t *crow = bl.code + base;
-    int e[U], c[U];
-#pragma unroll
-    for (int u = 0; u < U; u++) {
-      e[u] = len > 0 ? row[(long long)u * kWave] : 0;
-      c[u] = len > 0 ? crow[(long long)u * kWave] : 0;
-    }
-    for (int k = 0; k < len; k += U) {
-      float4 rj[U];
-      int cc[U];
-      if (staged) {
-#pragma unroll
-        for (int u = 0; u < U; u++) rj[u] = lds32[e[u]];
-      } else {
-#pragma unroll
-        for (int u = 0; u < U; u++)
-          rj[u] = e[u] < total ? rel[gl[e[u]]] : make_float4(1.0e30f, 1.0e30f, 1.0e30f, 0.0f);
-      }
-#pragma unroll
-      for (int u = 0; u < U; u++) cc[u] = c[u];
-      if (k + U < len) {
-#pragma unroll
-        for (int u = 0; u < U; u++) {
-          e[u] = row[(long long)(k + U + u) * kWave];
-          c[u] = crow[(long long)(k + U + u) * kWave];
-        }
-      }
-#pragma unroll
-      for (int u = 0; u < U; u++) {
-        const float4 o = offs[cc[u]];
-        const float dx = (rj[u].x - ri.x) + o.x, dy = (rj[u].y - ri.y) + o.y, dz = (rj[u].z - ri.z) + o.z;
-        const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
-        const float ir2 = __builtin_amdgcn_rcpf(r2);
-        const float ir6 = ir2 * ir2 * ir2;
-        float df = fmaf(ir2 * ir6, fmaf(-fc.B, ir6, fc.A), fc.C);
-        df = (r2 <= fc.CL2) ? df : 0.0f;
-        if (kKahan) {
-          float y, q;
-          y = fmaf(df, dx, -cx); q = ax + y; cx = (q - ax) - y; ax = q;
-          y = fmaf(df, dy, -cy); q = ay + y; cy = (q - ay) - y; ay = q;
-          y = fmaf(df, dz, -cz); q = az + y; cz = (q - az) - y; az = q;
-        } else {
-          ax = fmaf(df, dx, ax);
-          ay = fmaf(df, dy, ay);
-          az = fmaf(df, dz, az);
-        }
-      }
-    }
+    const uint16_t *row = bl.nbr + bl.sbase[s] + lane;
+    const uint8_t *crow = bl.code + bl.sbase[s] + lane;
+    float acc[6];
+    if (staged) row_kick_mixed<true, kKahan, U>(row, crow, len, ri, lds32, offs, rel, gl, total, fc, acc);
+    else row_kick_mixed<false, kKahan, U>(row, crow, len, ri, lds32, offs, rel, gl, total, fc, acc);
     if (ok) {
       double4 p = mom[i];
-      p.x += (double)ax - (double)cx;
-      p.y += (double)ay - (double)cy;
-      p.z += (double)az - (double)cz;
+      p.x += (double)acc[0] - (double)acc[3];
+      p.y += (double)acc[1] - (double)acc[4];
+      p.z += (double)acc[2] - (double)acc[5];
       vmax = fmax(vmax, finish_row(i, p, pos[i], mom, pos_out, epilogue, dt2, lc, id));
     }
   }
@@ -307,9 +323,9 @@ __global__ __launch_bounds__(kWave *W) void k_observe_brick(const double4 *__res
     ke += 0.5 * p.x * p.x + 0.5 * p.y * p.y + 0.5 * p.z * p.z + 0.5 * p.w * p.w;
     const double4 qi = pos[i];
     const int len = bl.slen[s];
-    const uint16_t *row = bl.nbr + bl.sbase[s] + lane;
+    const uint16_t *row = bl.nbr + bl.sbase[s];
     for (int k = 0; k < len; k++) {
-      const int e = row[(long long)k * kWave];
+      const int e = row[slot_of(k, lane)];
       if (e >= total) continue;  // padding
       double qx, qy, qz;
       if (staged) {

@@ -307,6 +307,72 @@ __global__ __launch_bounds__(kWave *W) void k_search_brick(const double4 *__rest
   }
 }
 
+// ---- row order inside a brick ------------------------------------------
+constexpr int kSortBins = kBlock;  // lengths >= kSortBins - 1 share the last bin
+
+// Order the rows of each brick by decreasing list length, ties by index (a
+// stable counting sort, so the order is deterministic): rowsrc[new] = old,
+// rowinv[old] = new.  Slices of 64 consecutive rows then hold rows of nearly
+// equal length and carry little padding.  One workgroup of kBlock threads per
+// brick.
+__global__ __launch_bounds__(kBlock) void k_brick_row_order(const int *__restrict__ row_len,
+                                                            const int *__restrict__ row0,
+                                                            const int *__restrict__ nrow,
+                                                            int *__restrict__ rowsrc, int *__restrict__ rowinv) {
+  __shared__ int start[kSortBins];
+  __shared__ int scan[kSortBins];
+  __shared__ int bins[kBlock];
+  const int b = blockIdx.x;
+  const int r0 = row0[b], n = nrow[b];
+  const int t = threadIdx.x;
+  start[t] = 0;
+  __syncthreads();
+  // bin of a row: longest rows first
+  for (int v = t; v < n; v += kBlock) atomicAdd(&start[kSortBins - 1 - min(row_len[r0 + v], kSortBins - 1)], 1);
+  __syncthreads();
+  // exclusive scan of the bin counts (Hillis-Steele over kSortBins == kBlock)
+  int x = start[t];
+  scan[t] = x;
+  __syncthreads();
+  for (int o = 1; o < kSortBins; o <<= 1) {
+    const int y = t >= o ? scan[t - o] : 0;
+    __syncthreads();
+    scan[t] += y;
+    __syncthreads();
+  }
+  start[t] = scan[t] - x;
+  __syncthreads();
+  for (int c0 = 0; c0 < n; c0 += kBlock) {
+    const int v = c0 + t;
+    const int bin = v < n ? kSortBins - 1 - min(row_len[r0 + v], kSortBins - 1) : -1;
+    bins[t] = bin;
+    __syncthreads();
+    if (v < n) {
+      int rank = 0;
+      for (int k = 0; k < t; k++) rank += bins[k] == bin ? 1 : 0;
+      const int p = r0 + start[bin] + rank;
+      rowsrc[p] = r0 + v;
+      rowinv[r0 + v] = p;
+    }
+    __syncthreads();
+    if (v < n) atomicAdd(&start[bin], 1);
+    __syncthreads();
+  }
+}
+
+__global__ void k_gather_int(const int *__restrict__ perm, int n, const int *__restrict__ in, int *__restrict__ out) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n) out[k] = in[perm[k]];
+}
+
+// new indices of owned particles (entries >= pn are ghosts and stay)
+__global__ void k_remap_owned(int *__restrict__ idx, long long n, int pn, const int *__restrict__ rowinv) {
+  const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  const int j = idx[k];
+  if (j < pn) idx[k] = rowinv[j];
+}
+
 // slice length = longest row of its 64 rows, rounded up to a multiple of
 // kSliceRound so the force loop runs whole batches; slots = 64 * length.
 // The largest slice length goes to partial (per workgroup).
@@ -329,12 +395,14 @@ __global__ void k_slice_len(const int *__restrict__ row_len, const int *__restri
 }
 
 // MeshList::MakeList (meshlist.cc:57-87) in sliced-ELL form: one wavefront per
-// 64-row slice turns the rows' candidate masks into staged indices and cell
+// 64-row slice (rows in the order of k_brick_row_order, rowsrc[row] = the
+// row's index in the search) turns the rows' candidate masks into staged indices and cell
 // offset codes, pads every row to the slice length with the brick's pad
-// entry, assembles the slice slot-major in LDS and copies it out in 16-byte
-// pieces.  scap = longest slice (LDS: W * scap * 64 * 3 bytes).
+// entry, assembles the slice in LDS in the layout of slot_of() and copies it
+// out in 16-byte pieces.  scap = longest slice (LDS: W * scap * 64 * 3 bytes).
 template <int W>
 __global__ __launch_bounds__(kWave *W) void k_emit_brick(const int *__restrict__ cell, Bricks B, SearchTables st,
+                                                        const int *__restrict__ rowsrc,
                                                         const uint32_t *__restrict__ mask, int mcap,
                                                         const long long *__restrict__ sbase,
                                                         const int *__restrict__ slen, int scap,
@@ -363,7 +431,8 @@ __global__ __launch_bounds__(kWave *W) void k_emit_brick(const int *__restrict__
     if (v < nrow) {
       const int i = row0 + v;
       const RowCell rc = row_cell(i, cell[i], X, B, tb, ts);
-      const uint32_t *mk = mask + (long long)i * mcap;
+      // masks are indexed by the row order of the search
+      const uint32_t *mk = mask + (long long)rowsrc[i] * mcap;
       int nbit = 0;
       for (int dz = -1; dz <= 1; dz++) {
         for (int dy = -1; dy <= 1; dy++) {
@@ -383,8 +452,8 @@ __global__ __launch_bounds__(kWave *W) void k_emit_brick(const int *__restrict__
               const int u = u0 + k + p;
               const int dx = u < t1 ? -1 : (u < t2 ? 0 : 1);
               if (cnt < len) {
-                bn[cnt * kWave + lane] = (uint16_t)u;
-                bc[cnt * kWave + lane] = offset_code(dx, dy, dz);
+                bn[slot_of(cnt, lane)] = (uint16_t)u;
+                bc[slot_of(cnt, lane)] = offset_code(dx, dy, dz);
               }
               cnt++;
             }
@@ -396,8 +465,8 @@ __global__ __launch_bounds__(kWave *W) void k_emit_brick(const int *__restrict__
     }
     if (cnt > len) atomicOr(err, kErrCapacity);
     for (int k = cnt; k < len; k++) {
-      bn[k * kWave + lane] = (uint16_t)total;
-      bc[k * kWave + lane] = (uint8_t)kZeroOffset;
+      bn[slot_of(k, lane)] = (uint16_t)total;
+      bc[slot_of(k, lane)] = (uint8_t)kZeroOffset;
     }
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
     __builtin_amdgcn_wave_barrier();

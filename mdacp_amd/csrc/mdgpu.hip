@@ -90,6 +90,8 @@ struct mdgpu_domain {
   long long *sbase = nullptr, *slots64 = nullptr;
   int *slen = nullptr, *row_len = nullptr;
   // build scratch
+  int *rowsrc = nullptr, *rowinv = nullptr;  // row order inside bricks: new -> old, old -> new
+  long long gtotal = 0;                       // gather-list entries
   uint32_t *mask = nullptr;  // candidate bitmask per row, mcap words
   size_t mask_cap = 0;
   int mcap = 32;
@@ -309,6 +311,8 @@ int ensure_owned(mdgpu_domain *d, int need) {
   CHK(dalloc(d, &d->nimg, nc));
   CHK(dalloc(d, &d->goff, nc));
   CHK(dalloc(d, &d->row_len, nc));
+  CHK(dalloc(d, &d->rowsrc, nc));
+  CHK(dalloc(d, &d->rowinv, nc));
   d->cap = nc;
   return MDGPU_OK;
 }
@@ -467,8 +471,8 @@ template <int W>
 void launch_emit_w(mdgpu_domain *d, const Bricks &B, const SearchTables &t, int scap) {
   const size_t shm = (size_t)W * scap * kWave * 3;
   allow_lds(k_emit_brick<W>, shm);
-  hipLaunchKernelGGL(k_emit_brick<W>, dim3(d->nbrick), dim3(kWave * W), shm, d->stream, d->cell, B, t, d->mask,
-                     d->mcap, d->sbase, d->slen, scap, d->nbr, d->ncode, d->i32);
+  hipLaunchKernelGGL(k_emit_brick<W>, dim3(d->nbrick), dim3(kWave * W), shm, d->stream, d->cell, B, t, d->rowsrc,
+                     d->mask, d->mcap, d->sbase, d->slen, scap, d->nbr, d->ncode, d->i32);
 }
 
 // the emit kernel keeps one slice (scap x 64 entries of 3 bytes) per wavefront in LDS
@@ -483,6 +487,33 @@ int launch_emit(mdgpu_domain *d, const Bricks &B, const SearchTables &t, int sca
     return MDGPU_ERR_CAPACITY;
   }
   return check_launch(d, "neighbour list emit");
+}
+
+// Rows of each brick by decreasing list length: permute the owned particles
+// (positions, momenta, ids, cells, row lengths) and remap the indices that
+// point at them (gather lists, ghost sources).  Ghosts keep their places.
+int order_rows(mdgpu_domain *d) {
+  hipStream_t st = d->stream;
+  const int pn = d->pn, ng = d->ng;
+  hipLaunchKernelGGL(k_brick_row_order, dim3(d->nbrick), dim3(kBlock), 0, st, d->row_len, d->row0, d->nrow,
+                     d->rowsrc, d->rowinv);
+  hipLaunchKernelGGL(k_gather_owned, dim3(nblk(pn)), dim3(kBlock), 0, st, d->rowsrc, pn, d->pos, d->mom, d->id,
+                     d->cell, d->pos_tmp, d->mom_alt, d->id_alt, d->cell_alt);
+  hipLaunchKernelGGL(k_gather_int, dim3(nblk(pn)), dim3(kBlock), 0, st, d->rowsrc, pn, d->row_len, d->nimg);
+  if (ng > 0) {
+    HIPCHK(d, hipMemcpyAsync(d->pos_tmp + pn, d->pos + pn, ng * sizeof(double4), hipMemcpyDeviceToDevice, st));
+    HIPCHK(d, hipMemcpyAsync(d->cell_alt + pn, d->cell + pn, ng * sizeof(int), hipMemcpyDeviceToDevice, st));
+    hipLaunchKernelGGL(k_remap_owned, dim3(nblk(ng)), dim3(kBlock), 0, st, d->gsrc, (long long)ng, pn, d->rowinv);
+  }
+  if (d->gtotal > 0)
+    hipLaunchKernelGGL(k_remap_owned, dim3(nblk(d->gtotal)), dim3(kBlock), 0, st, d->glist, d->gtotal, pn,
+                       d->rowinv);
+  std::swap(d->pos, d->pos_tmp);
+  std::swap(d->mom, d->mom_alt);
+  std::swap(d->id, d->id_alt);
+  std::swap(d->cell, d->cell_alt);
+  std::swap(d->row_len, d->nimg);
+  return check_launch(d, "row order");
 }
 
 // MDManager::MakePairList for the single domain (mdmanager.cc:261-280).
@@ -549,9 +580,6 @@ int make_pair_list(mdgpu_domain *d) {
   }
   d->ng = ng;
   d->tn = pn + ng;
-  // 3. PairList::Init: snapshot of owned + ghost positions
-  if (d->tn > 0)
-    hipLaunchKernelGGL(k_copy_pos, dim3(nblk(d->tn)), dim3(kBlock), 0, st, d->pos, d->pos_old, d->tn);
   // 4. bricks: rows, slices, staged ranges
   const Bricks B = bricks_of(d);
   const CellTables T{d->c_ostart, d->c_ocount, d->c_gstart, d->c_gcount};
@@ -588,6 +616,7 @@ int make_pair_list(mdgpu_domain *d) {
   HIPCHK(d, hipMemcpyAsync(d->gbase + d->nbrick, d->i64, sizeof(long long), hipMemcpyDeviceToDevice, st));
   long long gtotal = 0;
   CHK(read_total(d, &gtotal));
+  d->gtotal = gtotal;
   if (gtotal > d->glist_cap) {
     const long long gc = gtotal + gtotal / 8 + 1024;
     CHK(dalloc(d, &d->glist, (size_t)gc));
@@ -618,6 +647,7 @@ int make_pair_list(mdgpu_domain *d) {
       }
       d->mcap *= 2;
     }
+    CHK(order_rows(d));
     // number of listed (row, partner) entries, for accounting
     exclusive_scan<int, long long>(d->row_len, d->key, pn, d->scan_scratch, d->i64, st);
     CHK(read_total(d, &d->nbr_entries));
@@ -634,14 +664,18 @@ int make_pair_list(mdgpu_domain *d) {
     d->nbr_entries = 0;
   }
   if (slots > d->nbr_cap) {
+    // kListTail readable entries past the last slice (the force kernels read ahead)
     const long long nc = slots + slots / 8 + 1024;
-    CHK(dalloc(d, &d->nbr, (size_t)nc));
-    CHK(dalloc(d, &d->ncode, (size_t)nc));
+    CHK(dalloc(d, &d->nbr, (size_t)(nc + kListTail)));
+    CHK(dalloc(d, &d->ncode, (size_t)(nc + kListTail)));
     d->nbr_cap = nc;
   }
   d->nbr_slots = slots;
   if (pn > 0 && scap > 0) CHK(launch_emit(d, B, STb, scap));
   CHK(check_launch(d, "neighbour list"));
+  // PairList::Init: snapshot of owned + ghost positions (pairlist.cc:10-23)
+  if (d->tn > 0)
+    hipLaunchKernelGGL(k_copy_pos, dim3(nblk(d->tn)), dim3(kBlock), 0, st, d->pos, d->pos_old, d->tn);
   CHK(read_err_flags(d));
   d->initialized = true;
   d->buffer_length = d->margin;
@@ -1029,13 +1063,17 @@ int mdgpu_create(const mdgpu_params *prm, mdgpu_domain **out) {
 int mdgpu_destroy(mdgpu_domain *d) {
   if (!d) return MDGPU_OK;
   if (d->stream) (void)hipStreamSynchronize(d->stream);
-  void *bufs[] = {d->pos, d->pos_tmp, d->pos_old, d->mom, d->mom_alt, d->rel, d->rel_tmp, d->id, d->id_alt,
-                  d->cell, d->cell_alt, d->gsrc, d->gcode, d->c_ocount, d->c_ostart, d->c_gcount, d->c_gstart,
-                  d->count_r, d->start_r, d->org, d->rank, d->brick_of_cell, d->brick_rank0, d->row0, d->nrow,
-                  d->nsl, d->vslice0, d->slice_brick, d->tstart, d->tbase, d->nrange, d->btotal, d->gbase, d->glist, d->mask, d->spart, d->nbr, d->ncode,
-                  d->sbase, d->slots64, d->slen, d->row_len, d->slot, d->perm, d->nimg, d->goff, d->t_gsrc,
-                  d->t_gcode, d->t_gcell, d->key, d->scan_scratch, d->disp, d->partial, d->maxpart, d->bpart, d->cand,
-                  d->u64, d->i64, d->i32, d->d3};
+  void *bufs[] = {d->pos,     d->pos_tmp,     d->pos_old,     d->mom,       d->mom_alt,  d->rel,
+                  d->rel_tmp, d->id,          d->id_alt,      d->cell,      d->cell_alt, d->gsrc,
+                  d->gcode,   d->c_ocount,    d->c_ostart,    d->c_gcount,  d->c_gstart, d->count_r,
+                  d->start_r, d->org,         d->rank,        d->brick_of_cell, d->brick_rank0, d->row0,
+                  d->nrow,    d->nsl,         d->vslice0,     d->slice_brick, d->tstart,  d->tbase,
+                  d->nrange,  d->btotal,      d->gbase,       d->glist,     d->mask,     d->spart,
+                  d->nbr,     d->ncode,       d->sbase,       d->slots64,   d->slen,     d->row_len,
+                  d->rowsrc,  d->rowinv,      d->slot,        d->perm,      d->nimg,     d->goff,
+                  d->t_gsrc,  d->t_gcode,     d->t_gcell,     d->key,       d->scan_scratch, d->disp,
+                  d->partial, d->maxpart,     d->bpart,       d->cand,      d->u64,      d->i64,
+                  d->i32,     d->d3};
   for (void *b : bufs)
     if (b) (void)hipFree(b);
   if (d->h_u64) (void)hipHostFree(d->h_u64);
@@ -1349,7 +1387,7 @@ int mdgpu_get_pairs(mdgpu_domain *d, int64_t *npairs, int32_t *out3) {
     const int s = vs0[b] + v / kWave, lane = v % kWave;
     const long long total = gb[b + 1] - gb[b];
     for (int k = 0; k < sl[s]; k++) {
-      const int e = nbr[sb[s] + (long long)k * kWave + lane];
+      const int e = nbr[sb[s] + slot_of(k, lane)];
       if (e >= total) continue;  // padding
       const int j = gl[gb[b] + e];
       const int a = ids[i];
