@@ -25,7 +25,9 @@
  *   mdgpu_send_border_particles MDManager::SendBorderParticles (mdmanager.cc:282-348)
  *   mdgpu_calculate_force   ForceCalculator::CalculateForce (fcalculator.cc:34-50,
  *                           semantics of CalculateForceNext :228-290)
- *   mdgpu_langevin          ForceCalculator::Langevin (fcalculator.cc:764-778)
+ *   mdgpu_langevin          ForceCalculator::Langevin (fcalculator.cc:764-778); the noise is
+ *                           Philox4x32-10 keyed by (seed, particle id, call count), not
+ *                           the unit's std::mt19937 stream
  *   mdgpu_heatbath_zeta     ForceCalculator::HeatbathZeta (fcalculator.cc:743-749)
  *   mdgpu_heatbath_momenta  ForceCalculator::HeatbathMomenta (fcalculator.cc:751-762)
  *   mdgpu_calculate         MDManager::Calculate (mdmanager.cc:115-192), n steps
@@ -86,12 +88,21 @@ int mdgpu_add_particles(mdgpu_domain *d, int n, const double *q, const double *p
 int mdgpu_particle_number(const mdgpu_domain *d, int *pn, int *tn);
 int mdgpu_get_particles(mdgpu_domain *d, double *q, double *p, int *type);
 int mdgpu_set_momenta(mdgpu_domain *d, const double *p); /* in mdgpu_get_particles order */
+/* overwrite owned positions in mdgpu_get_particles order (writes to Variables::q,
+   variables.h:17, by host code between steps); ghosts follow at the next refresh */
+int mdgpu_set_positions(mdgpu_domain *d, const double *q);
+/* drop all particles (Variables::SetParticleNumber(0), variables.h:31) */
+int mdgpu_clear_particles(mdgpu_domain *d);
+/* ghost images: owner id, shift code, position (the ghost part of Variables::q,
+   indices [pn, tn) in the reference) */
 int mdgpu_get_ghosts(mdgpu_domain *d, int *src_type, int *shift_code, double *q);
 
 /* thermostat and run parameters */
 int mdgpu_set_thermostat(mdgpu_domain *d, int kind, double T, double gamma, double tau);
 int mdgpu_set_seed(mdgpu_domain *d, uint64_t seed);
 int mdgpu_get_zeta(const mdgpu_domain *d, double *zeta);
+/* Variables::Zeta (variables.h:19), used by mdgpu_heatbath_momenta */
+int mdgpu_set_zeta(mdgpu_domain *d, double zeta);
 int mdgpu_get_time(const mdgpu_domain *d, double *sim_time, uint64_t *step);
 
 /* the step, piece by piece (same order as MDManager::Calculate) */
@@ -110,6 +121,8 @@ int mdgpu_calculate(mdgpu_domain *d, int nsteps, int *rebuilds);
 
 /* observers: raw sums as returned by the reference observers */
 int mdgpu_observe(mdgpu_domain *d, double *ke, double *pe, double *virial);
+/* KineticEnergyObserver alone (observer.cc:4-15); needs no pair list */
+int mdgpu_observe_kinetic(mdgpu_domain *d, double *ke);
 
 /* MDManager::ChangeScale: rebuild, scale box and owned positions by alpha, rebuild */
 int mdgpu_change_scale(mdgpu_domain *d, double alpha);
@@ -118,6 +131,11 @@ int mdgpu_change_scale(mdgpu_domain *d, double alpha);
    shift code (sx+1)+3(sy+1)+9(sz+1)); owned-owned pairs as (min id, max id, 13).
    Call with out3 == NULL to get the count. */
 int mdgpu_get_pairs(mdgpu_domain *d, int64_t *npairs, int32_t *out3);
+/* the same pairs by index, as MeshList::GetKeyParticles/GetPartnerParticles
+   hold them (meshlist.h:46-48): key = owned index, partner = owned index
+   (key < partner) or pn + ghost index, in mdgpu_get_particles / mdgpu_get_ghosts
+   order.  Call with key == partner == NULL to get the count. */
+int mdgpu_get_pair_indices(mdgpu_domain *d, int64_t *npairs, int32_t *key, int32_t *partner);
 /* kick of CalculateForce from zero momenta, per owned particle in
    mdgpu_get_particles order (dp = F*dt) */
 int mdgpu_compute_kick(mdgpu_domain *d, double *dp);

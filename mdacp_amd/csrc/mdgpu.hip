@@ -1170,6 +1170,25 @@ int mdgpu_set_momenta(mdgpu_domain *d, const double *p) {
   return MDGPU_OK;
 }
 
+int mdgpu_set_positions(mdgpu_domain *d, const double *q) {
+  if (!d || !q) return MDGPU_ERR_ARGUMENT;
+  const int n = d->pn;
+  std::vector<double4> hq(n);
+  for (int i = 0; i < n; i++) hq[i] = make_double4(q[3 * (size_t)i], q[3 * (size_t)i + 1], q[3 * (size_t)i + 2], 0.0);
+  if (n > 0) HIPCHK(d, hipMemcpyAsync(d->pos, hq.data(), n * sizeof(double4), hipMemcpyHostToDevice, d->stream));
+  CHK(sync(d));
+  return MDGPU_OK;
+}
+
+int mdgpu_clear_particles(mdgpu_domain *d) {
+  if (!d) return MDGPU_ERR_ARGUMENT;
+  CHK(sync(d));
+  d->pn = d->ng = d->tn = 0;
+  d->initialized = false;
+  d->vmax_valid = false;
+  return MDGPU_OK;
+}
+
 int mdgpu_get_ghosts(mdgpu_domain *d, int *src_type, int *shift_code, double *q) {
   if (!d) return MDGPU_ERR_ARGUMENT;
   const int ng = d->ng, pn = d->pn;
@@ -1257,7 +1276,14 @@ int mdgpu_langevin(mdgpu_domain *d) {
     hipLaunchKernelGGL(k_langevin, dim3(nblk(d->pn)), dim3(kBlock), 0, d->stream, d->mom, d->id, d->pn,
                        lconsts(d));
   d->vmax_valid = false;
+  d->step++;  // the next call draws the next numbers of each particle's stream
   return check_launch(d, "langevin");
+}
+
+int mdgpu_set_zeta(mdgpu_domain *d, double zeta) {
+  if (!d) return MDGPU_ERR_ARGUMENT;
+  d->zeta = zeta;
+  return MDGPU_OK;
 }
 
 int mdgpu_heatbath_zeta(mdgpu_domain *d, double t) {
@@ -1336,6 +1362,11 @@ int mdgpu_observe(mdgpu_domain *d, double *ke, double *pe, double *virial) {
   return MDGPU_OK;
 }
 
+int mdgpu_observe_kinetic(mdgpu_domain *d, double *ke) {
+  if (!d || !ke) return MDGPU_ERR_ARGUMENT;
+  return kinetic(d, ke);
+}
+
 int mdgpu_change_scale(mdgpu_domain *d, double alpha) {
   if (!d || !(alpha > 0.0)) return MDGPU_ERR_ARGUMENT;
   CHK(make_pair_list(d));
@@ -1405,6 +1436,48 @@ int mdgpu_get_pairs(mdgpu_domain *d, int64_t *npairs, int32_t *out3) {
         out3[3 * cnt + 1] = bb;
         out3[3 * cnt + 2] = code;
       }
+      cnt++;
+    }
+  }
+  *npairs = cnt;
+  return MDGPU_OK;
+}
+
+int mdgpu_get_pair_indices(mdgpu_domain *d, int64_t *npairs, int32_t *key, int32_t *partner) {
+  if (!d || !npairs) return MDGPU_ERR_ARGUMENT;
+  if (!d->initialized) {
+    d->err = "pair list not built";
+    return MDGPU_ERR_ARGUMENT;
+  }
+  const int pn = d->pn, nbk = d->nbrick;
+  std::vector<uint16_t> nbr((size_t)d->nbr_slots);
+  std::vector<long long> sb(d->nslice), gb(nbk + 1);
+  std::vector<int> sl(d->nslice), cl(pn), r0(nbk), vs0(nbk);
+  if (d->nbr_slots) HIPCHK(d, hipMemcpyAsync(nbr.data(), d->nbr, nbr.size() * 2, hipMemcpyDeviceToHost, d->stream));
+  if (d->nslice) {
+    HIPCHK(d, hipMemcpyAsync(sb.data(), d->sbase, d->nslice * 8, hipMemcpyDeviceToHost, d->stream));
+    HIPCHK(d, hipMemcpyAsync(sl.data(), d->slen, d->nslice * 4, hipMemcpyDeviceToHost, d->stream));
+  }
+  if (pn) HIPCHK(d, hipMemcpyAsync(cl.data(), d->cell, pn * 4, hipMemcpyDeviceToHost, d->stream));
+  HIPCHK(d, hipMemcpyAsync(r0.data(), d->row0, nbk * 4, hipMemcpyDeviceToHost, d->stream));
+  HIPCHK(d, hipMemcpyAsync(vs0.data(), d->vslice0, nbk * 4, hipMemcpyDeviceToHost, d->stream));
+  HIPCHK(d, hipMemcpyAsync(gb.data(), d->gbase, (nbk + 1) * 8, hipMemcpyDeviceToHost, d->stream));
+  CHK(sync(d));
+  std::vector<int> gl((size_t)gb[nbk]);
+  if (!gl.empty()) HIPCHK(d, hipMemcpy(gl.data(), d->glist, gl.size() * 4, hipMemcpyDeviceToHost));
+  int64_t cnt = 0;
+  for (int i = 0; i < pn; i++) {
+    const int b = d->h_brick_of_cell[cl[i]];
+    const int v = i - r0[b];
+    const int s = vs0[b] + v / kWave, lane = v % kWave;
+    const long long total = gb[b + 1] - gb[b];
+    for (int k = 0; k < sl[s]; k++) {
+      const int e = nbr[sb[s] + slot_of(k, lane)];
+      if (e >= total) continue;  // padding
+      const int j = gl[gb[b] + e];
+      if (j < pn && j < i) continue;  // owned-owned pairs once, as (smaller, larger)
+      if (key) key[cnt] = i;
+      if (partner) partner[cnt] = j;
       cnt++;
     }
   }

@@ -25,6 +25,8 @@ EXPORTS = [
     "mdgpu_calculate_force", "mdgpu_langevin", "mdgpu_heatbath_zeta", "mdgpu_heatbath_momenta",
     "mdgpu_calculate", "mdgpu_observe", "mdgpu_change_scale", "mdgpu_get_pairs", "mdgpu_compute_kick",
     "mdgpu_enable_timing", "mdgpu_get_timing", "mdgpu_get_list_stats", "mdgpu_set_option", "mdgpu_get_option",
+    "mdgpu_set_positions", "mdgpu_clear_particles", "mdgpu_set_zeta", "mdgpu_get_pair_indices",
+    "mdgpu_observe_kinetic",
 ]
 OPT_UNROLL, OPT_KAHAN, OPT_WAVES, OPT_LDS_KB, OPT_BRICK = 1, 2, 3, 4, 5
 
@@ -83,6 +85,11 @@ def load():
             "mdgpu_get_list_stats": [P, PL, PL, PI, PI],
             "mdgpu_set_option": [P, I, I],
             "mdgpu_get_option": [P, I, PI],
+            "mdgpu_set_positions": [P, P],
+            "mdgpu_clear_particles": [P],
+            "mdgpu_set_zeta": [P, D],
+            "mdgpu_get_pair_indices": [P, PL, P, P],
+            "mdgpu_observe_kinetic": [P, PD],
         }
         for name, args in sig.items():
             f = getattr(L, name)
@@ -154,6 +161,16 @@ class Domain:
     def set_momenta(self, p):
         p = np.ascontiguousarray(p, dtype=np.float64)
         self._chk(load().mdgpu_set_momenta(self.h, _ptr(p)))
+
+    def set_positions(self, q):
+        q = np.ascontiguousarray(q, dtype=np.float64)
+        self._chk(load().mdgpu_set_positions(self.h, _ptr(q)))
+
+    def clear_particles(self):
+        self._chk(load().mdgpu_clear_particles(self.h))
+
+    def set_zeta(self, zeta):
+        self._chk(load().mdgpu_set_zeta(self.h, zeta))
 
     def ghosts(self):
         pn, tn = self.counts()
@@ -232,6 +249,15 @@ class Domain:
         out = np.zeros((n.value, 3), dtype=np.int32)
         self._chk(load().mdgpu_get_pairs(self.h, ctypes.byref(n), _ptr(out)))
         return out
+
+    def pair_indices(self):
+        """(key, partner) index pairs in particles()/ghosts() order (partner >= pn: ghost)."""
+        n = ctypes.c_int64()
+        self._chk(load().mdgpu_get_pair_indices(self.h, ctypes.byref(n), None, None))
+        key = np.zeros(n.value, dtype=np.int32)
+        partner = np.zeros(n.value, dtype=np.int32)
+        self._chk(load().mdgpu_get_pair_indices(self.h, ctypes.byref(n), _ptr(key), _ptr(partner)))
+        return key, partner
 
     def kick(self):
         """dp = F*dt from zero momenta, in particles() order."""
