@@ -130,8 +130,9 @@ def main():
     ap.add_argument("--margin", type=float, default=0.5)
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py: this build runs one domain on one GPU; --gpus {args.gpus} is not supported "
-              f"(multi-GPU runs would shard by spatial domain decomposition, see DESIGN.md)", file=sys.stderr)
+        print(f"bench.py: one domain runs on one GPU; --gpus {args.gpus} is not supported "
+              f"(several GPUs would each own one box of a spatial domain decomposition, exchanging "
+              f"ghost images every step, which is not built)", file=sys.stderr)
         sys.exit(2)
 
     from mdacp_amd import mdgpu
