@@ -151,7 +151,7 @@ int mdgpu_get_timing(const mdgpu_domain *d, int *force_launches, double *force_m
    them beyond floating-point summation order) */
 #define MDGPU_OPT_UNROLL 1   /* partner entries per batch in the force kernels: 4 or 8 */
 #define MDGPU_OPT_KAHAN 2    /* mixed precision: 1 = compensated FP32 sums, 0 = plain */
-#define MDGPU_OPT_WAVES 3    /* wavefronts per cell-brick workgroup: 2, 4 or 8 */
+#define MDGPU_OPT_WAVES 3    /* wavefronts per cell-brick workgroup: 2, 4, 8 or 16 */
 #define MDGPU_OPT_LDS_KB 4   /* LDS budget for one brick's staged partners, KiB (>= 8) */
 #define MDGPU_OPT_BRICK 5    /* cells per brick: bx | by << 8 | bz << 16, each 1..4
                                 (the next step rebuilds the list) */

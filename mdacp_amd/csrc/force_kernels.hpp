@@ -160,19 +160,38 @@ __device__ __forceinline__ void row_kick_mixed(const uint16_t *__restrict__ row,
   acc[3] = cx; acc[4] = cy; acc[5] = cz;
 }
 
+// Where the epilogue also writes the next step's first half drift (q + p dt/2
+// with the final q and p of this step, the same roundings as the next step's
+// UpdatePositionHalf): pos (owned rows) and, for mixed precision, its
+// cell-relative FP32 copy.  pos == nullptr: not written.
+struct NextDrift {
+  double4 *pos;
+  float4 *rel;
+  const int *cell;
+  Origins O;
+};
+
 __device__ __forceinline__ double finish_row(int i, double4 p, const double4 qi, double4 *__restrict__ mom,
                                              double4 *__restrict__ pos_out, int epilogue, double dt2,
-                                             const LangevinConsts &lc, const int *__restrict__ id) {
+                                             const LangevinConsts &lc, const int *__restrict__ id,
+                                             const NextDrift &nd) {
   if (epilogue == kKickLangevin) langevin_update(p, lc, (uint32_t)id[i]);
   mom[i] = p;
+  double4 q = qi;
   if (epilogue == kKickDrift) {
-    double4 q = qi;
     q.x += p.x * dt2;
     q.y += p.y * dt2;
     q.z += p.z * dt2;
     pos_out[i] = q;
   } else if (epilogue == kKickLangevin) {
     pos_out[i] = qi;  // one half drift per Langevin step (mdmanager.cc:185-192)
+  }
+  if (nd.pos != nullptr && epilogue != kKickOnly) {
+    q.x += p.x * dt2;
+    q.y += p.y * dt2;
+    q.z += p.z * dt2;
+    nd.pos[i] = q;
+    if (nd.rel != nullptr) nd.rel[i] = to_rel(q, nd.cell[i], nd.O.x, nd.O.y, nd.O.z, nd.O.mx, nd.O.my);
   }
   return p.x * p.x + p.y * p.y + p.z * p.z;
 }
@@ -184,7 +203,7 @@ __global__ __launch_bounds__(kWave *W) void k_force_brick_fp64(const double4 *__
                                                               double4 *__restrict__ pos_out, BrickList bl,
                                                               ForceConsts fc, int epilogue, LangevinConsts lc,
                                                               const int *__restrict__ id,
-                                                              double *__restrict__ vpart) {
+                                                              double *__restrict__ vpart, NextDrift nd) {
   extern __shared__ double lds64[];
   constexpr int T = kWave * W;
   const int b = xcd_block(blockIdx.x, gridDim.x);
@@ -213,7 +232,7 @@ __global__ __launch_bounds__(kWave *W) void k_force_brick_fp64(const double4 *__
       p.x += a.x;
       p.y += a.y;
       p.z += a.z;
-      vmax = fmax(vmax, finish_row(i, p, qi, mom, pos_out, epilogue, fc.dt2, lc, id));
+      vmax = fmax(vmax, finish_row(i, p, qi, mom, pos_out, epilogue, fc.dt2, lc, id, nd));
     }
   }
   if (vpart) block_max_to_partial(vmax, vpart);
@@ -229,7 +248,7 @@ template <int W, int U, bool kKahan>
 __global__ __launch_bounds__(kWave *W) void k_force_brick_mixed(
     const float4 *__restrict__ rel, const double4 *__restrict__ pos, double4 *__restrict__ mom,
     double4 *__restrict__ pos_out, BrickList bl, ForceConstsF fc, double dt2, int epilogue, LangevinConsts lc,
-    const int *__restrict__ id, double *__restrict__ vpart) {
+    const int *__restrict__ id, double *__restrict__ vpart, NextDrift nd) {
   extern __shared__ float4 lds32[];
   __shared__ float4 offs[64];
   constexpr int T = kWave * W;
@@ -282,7 +301,7 @@ __global__ __launch_bounds__(kWave *W) void k_force_brick_mixed(
       p.x += (double)acc[0] - (double)acc[3];
       p.y += (double)acc[1] - (double)acc[4];
       p.z += (double)acc[2] - (double)acc[5];
-      vmax = fmax(vmax, finish_row(i, p, pos[i], mom, pos_out, epilogue, dt2, lc, id));
+      vmax = fmax(vmax, finish_row(i, p, pos[i], mom, pos_out, epilogue, dt2, lc, id, nd));
     }
   }
   if (vpart) block_max_to_partial(vmax, vpart);

@@ -33,7 +33,7 @@ struct mdgpu_domain {
   // tuning
   int kahan = 1;          // compensated FP32 sums in the mixed kernel
   int unroll = 8;         // partner entries per batch in the force kernels (4 or 8)
-  int waves = 8;          // wavefronts per brick workgroup: 2, 4 or 8
+  int waves = 8;          // wavefronts per brick workgroup: 2, 4, 8 or 16
   int bd[3] = {4, 4, 4};  // cells per brick
   int lds_kb = 96;        // LDS budget for one brick's staged partners, KiB
   int lds_max_kb = 60;    // device limit per workgroup less the kernels' static LDS
@@ -69,9 +69,12 @@ struct mdgpu_domain {
   // device state: positions as double4 (x, y, z, 0); owned [0, pn), ghosts [pn, tn)
   double4 *pos = nullptr;      // current owned positions; ghosts of the last halo refresh
   double4 *pos_tmp = nullptr;  // half-drifted owned positions inside the fused step; build scratch
+  double4 *pos_next = nullptr; // the next step's half-drifted owned positions (fused epilogue)
   double4 *pos_old = nullptr;  // PairList::Init snapshot
   double4 *mom = nullptr, *mom_alt = nullptr;
   float4 *rel = nullptr, *rel_tmp = nullptr;  // cell-relative FP32 copies (owned + ghosts)
+  float4 *rel_next = nullptr;
+  bool mid_valid = false;  // pos_tmp (rel_tmp) already holds this step's first half drift
   int *id = nullptr, *id_alt = nullptr, *cell = nullptr, *cell_alt = nullptr;
   int *gsrc = nullptr, *gcode = nullptr;
   int *c_ocount = nullptr, *c_ostart = nullptr, *c_gcount = nullptr, *c_gstart = nullptr;
@@ -296,9 +299,11 @@ int ensure_owned(mdgpu_domain *d, int need) {
   CHK(dgrow_copy(d, &d->mom, d->pn, nc));
   CHK(dgrow_copy(d, &d->id, d->pn, nc));
   CHK(dalloc(d, &d->pos_tmp, tc_new));
+  CHK(dalloc(d, &d->pos_next, tc_new));
   CHK(dalloc(d, &d->pos_old, tc_new));
   CHK(dalloc(d, &d->rel, tc_new));
   CHK(dalloc(d, &d->rel_tmp, tc_new));
+  CHK(dalloc(d, &d->rel_next, tc_new));
   CHK(dalloc(d, &d->cell, tc_new));
   CHK(dalloc(d, &d->cell_alt, tc_new));
   CHK(dalloc(d, &d->disp, tc_new));
@@ -329,9 +334,11 @@ int ensure_ghosts(mdgpu_domain *d, int need) {
   CHK(dgrow_copy(d, &d->pos, d->pn, tc_new));
   CHK(dgrow_copy(d, &d->cell, d->pn, tc_new));
   CHK(dalloc(d, &d->pos_tmp, tc_new));
+  CHK(dalloc(d, &d->pos_next, tc_new));
   CHK(dalloc(d, &d->pos_old, tc_new));
   CHK(dalloc(d, &d->rel, tc_new));
   CHK(dalloc(d, &d->rel_tmp, tc_new));
+  CHK(dalloc(d, &d->rel_next, tc_new));
   CHK(dalloc(d, &d->cell_alt, tc_new));
   CHK(dalloc(d, &d->disp, tc_new));
   CHK(dalloc(d, &d->maxpart, nblk((long long)tc_new, kWave)));
@@ -461,6 +468,7 @@ void launch_search_w(mdgpu_domain *d, const Bricks &B, const SearchTables &t) {
 int launch_search(mdgpu_domain *d, const Bricks &B, const SearchTables &t) {
   switch (d->waves) {
     case 2: launch_search_w<2>(d, B, t); break;
+    case 16: launch_search_w<16>(d, B, t); break;
     case 4: launch_search_w<4>(d, B, t); break;
     default: launch_search_w<8>(d, B, t); break;
   }
@@ -522,6 +530,7 @@ int order_rows(mdgpu_domain *d) {
 // indices, and particles are put in brick order here anyway.
 int make_pair_list(mdgpu_domain *d) {
   hipStream_t st = d->stream;
+  d->mid_valid = false;  // particles are reordered and wrapped
   const Grid g = grid_of(d);
   const int pn = d->pn;
   d->ng = 0;
@@ -803,44 +812,46 @@ BrickList brick_list(const mdgpu_domain *d, int bytes_per_entry) {
 
 template <int W, int U>
 void launch_fp64(mdgpu_domain *d, const double4 *src, double4 *pos_out, int epi, const ForceConsts &fc,
-                 const LangevinConsts &lc, double *vb) {
+                 const LangevinConsts &lc, double *vb, const NextDrift &nd) {
   const BrickList bl = brick_list(d, 3 * sizeof(double));
   const size_t shm = (size_t)bl.lds_cap * 3 * sizeof(double);
   allow_lds(k_force_brick_fp64<W, U>, shm);
   hipLaunchKernelGGL((k_force_brick_fp64<W, U>), dim3(d->nbrick), dim3(kWave * W), shm, d->stream, src, d->mom,
-                     pos_out, bl, fc, epi, lc, d->id, vb);
+                     pos_out, bl, fc, epi, lc, d->id, vb, nd);
 }
 
 template <int W, int U, bool K>
 void launch_mixed(mdgpu_domain *d, const double4 *src, const float4 *rel, double4 *pos_out, int epi,
-                  const ForceConstsF &f32, double dt2, const LangevinConsts &lc, double *vb) {
+                  const ForceConstsF &f32, double dt2, const LangevinConsts &lc, double *vb, const NextDrift &nd) {
   const BrickList bl = brick_list(d, sizeof(float4));
   const size_t shm = (size_t)bl.lds_cap * sizeof(float4);
   allow_lds(k_force_brick_mixed<W, U, K>, shm);
   hipLaunchKernelGGL((k_force_brick_mixed<W, U, K>), dim3(d->nbrick), dim3(kWave * W), shm, d->stream, rel, src,
-                     d->mom, pos_out, bl, f32, dt2, epi, lc, d->id, vb);
+                     d->mom, pos_out, bl, f32, dt2, epi, lc, d->id, vb, nd);
 }
 
 template <int W>
 void launch_force_w(mdgpu_domain *d, const double4 *src, const float4 *rel, double4 *pos_out, int epi,
-                    const ForceConsts &fc, const ForceConstsF &f32, const LangevinConsts &lc, double *vb) {
+                    const ForceConsts &fc, const ForceConstsF &f32, const LangevinConsts &lc, double *vb,
+                    const NextDrift &nd) {
   if (d->precision == MDGPU_FP64) {
-    if (d->unroll == 4) launch_fp64<W, 4>(d, src, pos_out, epi, fc, lc, vb);
-    else launch_fp64<W, 8>(d, src, pos_out, epi, fc, lc, vb);
+    if (d->unroll == 4) launch_fp64<W, 4>(d, src, pos_out, epi, fc, lc, vb, nd);
+    else launch_fp64<W, 8>(d, src, pos_out, epi, fc, lc, vb, nd);
   } else if (d->kahan) {
-    if (d->unroll == 4) launch_mixed<W, 4, true>(d, src, rel, pos_out, epi, f32, fc.dt2, lc, vb);
-    else launch_mixed<W, 8, true>(d, src, rel, pos_out, epi, f32, fc.dt2, lc, vb);
+    if (d->unroll == 4) launch_mixed<W, 4, true>(d, src, rel, pos_out, epi, f32, fc.dt2, lc, vb, nd);
+    else launch_mixed<W, 8, true>(d, src, rel, pos_out, epi, f32, fc.dt2, lc, vb, nd);
   } else {
-    if (d->unroll == 4) launch_mixed<W, 4, false>(d, src, rel, pos_out, epi, f32, fc.dt2, lc, vb);
-    else launch_mixed<W, 8, false>(d, src, rel, pos_out, epi, f32, fc.dt2, lc, vb);
+    if (d->unroll == 4) launch_mixed<W, 4, false>(d, src, rel, pos_out, epi, f32, fc.dt2, lc, vb, nd);
+    else launch_mixed<W, 8, false>(d, src, rel, pos_out, epi, f32, fc.dt2, lc, vb, nd);
   }
 }
 
 // One pair-force launch.  Rows and partners (owned, and ghosts at
 // [pn, pn + ng)) are read from `src` (`rel` in FP32); `pos_out` receives the
-// drifted (or copied) owned positions.
+// drifted (or copied) owned positions; `nd` (optional) the next step's first
+// half drift.
 int launch_force(mdgpu_domain *d, const double4 *src, const float4 *rel, double4 *pos_out, int epilogue,
-                 bool want_vmax) {
+                 bool want_vmax, const NextDrift &nd) {
   if (d->pn == 0) return MDGPU_OK;
   hipStream_t st = d->stream;
   const ForceConsts fc = fconsts(d);
@@ -850,9 +861,10 @@ int launch_force(mdgpu_domain *d, const double4 *src, const float4 *rel, double4
   const int ev0 = d->force_launches * 2;
   if (d->timing && ev0 + 1 < (int)d->ev.size()) HIPCHK(d, hipEventRecord(d->ev[ev0], st));
   switch (d->waves) {
-    case 2: launch_force_w<2>(d, src, rel, pos_out, epilogue, fc, f32, lc, vb); break;
-    case 8: launch_force_w<8>(d, src, rel, pos_out, epilogue, fc, f32, lc, vb); break;
-    default: launch_force_w<4>(d, src, rel, pos_out, epilogue, fc, f32, lc, vb); break;
+    case 2: launch_force_w<2>(d, src, rel, pos_out, epilogue, fc, f32, lc, vb, nd); break;
+    case 8: launch_force_w<8>(d, src, rel, pos_out, epilogue, fc, f32, lc, vb, nd); break;
+    case 16: launch_force_w<16>(d, src, rel, pos_out, epilogue, fc, f32, lc, vb, nd); break;
+    default: launch_force_w<4>(d, src, rel, pos_out, epilogue, fc, f32, lc, vb, nd); break;
   }
   if (d->timing && ev0 + 1 < (int)d->ev.size()) {
     HIPCHK(d, hipEventRecord(d->ev[ev0 + 1], st));
@@ -920,7 +932,8 @@ int force_in_place(mdgpu_domain *d) {
   if (d->precision == MDGPU_MIXED && d->tn > 0)
     hipLaunchKernelGGL(k_to_rel, dim3(nblk(d->tn)), dim3(kBlock), 0, d->stream, d->pos, d->cell, d->tn,
                        origins_of(d), d->rel);
-  CHK(launch_force(d, d->pos, d->rel, d->pos, kKickOnly, false));
+  d->mid_valid = false;
+  CHK(launch_force(d, d->pos, d->rel, d->pos, kKickOnly, false, NextDrift{nullptr, nullptr, nullptr, Origins{}}));
   d->vmax_valid = false;
   return MDGPU_OK;
 }
@@ -951,6 +964,7 @@ int calculate_step(mdgpu_domain *d) {
   }
   if (d->thermostat == MDGPU_NOSEHOOVER) {
     // CalculateNoseHoover (mdmanager.cc:159-175)
+    d->mid_valid = false;
     CHK(drift(d));
     CHK(refresh(d));
     double t;
@@ -963,21 +977,41 @@ int calculate_step(mdgpu_domain *d) {
     CHK(heatbath_zeta(d, t));
     CHK(drift(d));
   } else {
-    // fused: drift into pos_tmp + halo refresh into pos, then kick (+ drift or Langevin)
-    const int tot = d->pn + d->ng;
-    if (tot > 0) {
-      if (d->precision == MDGPU_MIXED)
-        hipLaunchKernelGGL((k_drift_refresh<true>), dim3(nblk(tot)), dim3(kBlock), 0, st, d->pos, d->mom,
-                           d->pos_tmp, d->gsrc, d->gcode, d->pn, d->ng, d->dt * 0.5, d->L[0], d->L[1], d->L[2],
-                           d->cell, origins_of(d), d->rel_tmp);
-      else
-        hipLaunchKernelGGL((k_drift_refresh<false>), dim3(nblk(tot)), dim3(kBlock), 0, st, d->pos, d->mom,
-                           d->pos_tmp, d->gsrc, d->gcode, d->pn, d->ng, d->dt * 0.5, d->L[0], d->L[1], d->L[2],
-                           d->cell, origins_of(d), d->rel_tmp);
-      CHK(check_launch(d, "drift+refresh"));
+    // fused: [drift into pos_tmp +] halo refresh, then kick (+ drift or Langevin)
+    // whose epilogue also writes the next step's first half drift into pos_next
+    const bool mixed = d->precision == MDGPU_MIXED;
+    const Origins O = origins_of(d);
+    if (d->mid_valid) {
+      if (d->ng > 0) {
+        if (mixed)
+          hipLaunchKernelGGL((k_refresh_from_drifted<true>), dim3(nblk(d->ng)), dim3(kBlock), 0, st, d->pos,
+                             d->pos_tmp, d->gsrc, d->gcode, d->pn, d->ng, d->L[0], d->L[1], d->L[2], d->cell, O,
+                             d->rel_tmp);
+        else
+          hipLaunchKernelGGL((k_refresh_from_drifted<false>), dim3(nblk(d->ng)), dim3(kBlock), 0, st, d->pos,
+                             d->pos_tmp, d->gsrc, d->gcode, d->pn, d->ng, d->L[0], d->L[1], d->L[2], d->cell, O,
+                             d->rel_tmp);
+      }
+    } else {
+      const int tot = d->pn + d->ng;
+      if (tot > 0) {
+        if (mixed)
+          hipLaunchKernelGGL((k_drift_refresh<true>), dim3(nblk(tot)), dim3(kBlock), 0, st, d->pos, d->mom,
+                             d->pos_tmp, d->gsrc, d->gcode, d->pn, d->ng, d->dt * 0.5, d->L[0], d->L[1], d->L[2],
+                             d->cell, O, d->rel_tmp);
+        else
+          hipLaunchKernelGGL((k_drift_refresh<false>), dim3(nblk(tot)), dim3(kBlock), 0, st, d->pos, d->mom,
+                             d->pos_tmp, d->gsrc, d->gcode, d->pn, d->ng, d->dt * 0.5, d->L[0], d->L[1], d->L[2],
+                             d->cell, O, d->rel_tmp);
+      }
     }
+    CHK(check_launch(d, "drift+refresh"));
     const int epi = d->thermostat == MDGPU_LANGEVIN ? kKickLangevin : kKickDrift;
-    CHK(launch_force(d, d->pos_tmp, d->rel_tmp, d->pos, epi, true));
+    const NextDrift nd{d->pos_next, mixed ? d->rel_next : nullptr, d->cell, O};
+    CHK(launch_force(d, d->pos_tmp, d->rel_tmp, d->pos, epi, true, nd));
+    std::swap(d->pos_tmp, d->pos_next);
+    if (mixed) std::swap(d->rel_tmp, d->rel_next);
+    d->mid_valid = true;
     d->vmax_valid = true;
   }
   d->sim_time += d->dt;
@@ -1063,7 +1097,7 @@ int mdgpu_create(const mdgpu_params *prm, mdgpu_domain **out) {
 int mdgpu_destroy(mdgpu_domain *d) {
   if (!d) return MDGPU_OK;
   if (d->stream) (void)hipStreamSynchronize(d->stream);
-  void *bufs[] = {d->pos,     d->pos_tmp,     d->pos_old,     d->mom,       d->mom_alt,  d->rel,
+  void *bufs[] = {d->pos,     d->pos_tmp,     d->pos_next,    d->rel_next,  d->pos_old,     d->mom,       d->mom_alt,  d->rel,
                   d->rel_tmp, d->id,          d->id_alt,      d->cell,      d->cell_alt, d->gsrc,
                   d->gcode,   d->c_ocount,    d->c_ostart,    d->c_gcount,  d->c_gstart, d->count_r,
                   d->start_r, d->org,         d->rank,        d->brick_of_cell, d->brick_rank0, d->row0,
@@ -1090,12 +1124,14 @@ const char *mdgpu_last_error(const mdgpu_domain *d) { return d ? d->err.c_str() 
 
 int mdgpu_set_precision(mdgpu_domain *d, int precision) {
   if (!d || (precision != MDGPU_FP64 && precision != MDGPU_MIXED)) return MDGPU_ERR_ARGUMENT;
+  d->mid_valid = false;  // the next fused step recomputes its first half drift
   d->precision = precision;
   return MDGPU_OK;
 }
 
 int mdgpu_add_particles(mdgpu_domain *d, int n, const double *q, const double *p, const int *type) {
   if (!d || n < 0 || (n > 0 && !q)) return MDGPU_ERR_ARGUMENT;
+  d->mid_valid = false;  // the next fused step recomputes its first half drift
   std::vector<double4> hq, hp;
   std::vector<int> hid;
   hq.reserve(n);
@@ -1161,6 +1197,7 @@ int mdgpu_get_particles(mdgpu_domain *d, double *q, double *p, int *type) {
 
 int mdgpu_set_momenta(mdgpu_domain *d, const double *p) {
   if (!d || !p) return MDGPU_ERR_ARGUMENT;
+  d->mid_valid = false;  // the next fused step recomputes its first half drift
   const int n = d->pn;
   std::vector<double4> hp(n);
   for (int i = 0; i < n; i++) hp[i] = make_double4(p[3 * (size_t)i], p[3 * (size_t)i + 1], p[3 * (size_t)i + 2], 0.0);
@@ -1172,6 +1209,7 @@ int mdgpu_set_momenta(mdgpu_domain *d, const double *p) {
 
 int mdgpu_set_positions(mdgpu_domain *d, const double *q) {
   if (!d || !q) return MDGPU_ERR_ARGUMENT;
+  d->mid_valid = false;  // the next fused step recomputes its first half drift
   const int n = d->pn;
   std::vector<double4> hq(n);
   for (int i = 0; i < n; i++) hq[i] = make_double4(q[3 * (size_t)i], q[3 * (size_t)i + 1], q[3 * (size_t)i + 2], 0.0);
@@ -1182,6 +1220,7 @@ int mdgpu_set_positions(mdgpu_domain *d, const double *q) {
 
 int mdgpu_clear_particles(mdgpu_domain *d) {
   if (!d) return MDGPU_ERR_ARGUMENT;
+  d->mid_valid = false;  // the next fused step recomputes its first half drift
   CHK(sync(d));
   d->pn = d->ng = d->tn = 0;
   d->initialized = false;
@@ -1215,6 +1254,7 @@ int mdgpu_get_ghosts(mdgpu_domain *d, int *src_type, int *shift_code, double *q)
 
 int mdgpu_set_thermostat(mdgpu_domain *d, int kind, double T, double gamma, double tau) {
   if (!d || kind < MDGPU_NVE || kind > MDGPU_LANGEVIN) return MDGPU_ERR_ARGUMENT;
+  d->mid_valid = false;  // the next fused step recomputes its first half drift
   d->thermostat = kind;
   d->T_aim = T;
   d->gamma = gamma;
@@ -1253,11 +1293,13 @@ int mdgpu_is_pair_list_expired(mdgpu_domain *d, int *expired) {
 
 int mdgpu_update_position_half(mdgpu_domain *d) {
   if (!d) return MDGPU_ERR_ARGUMENT;
+  d->mid_valid = false;  // the next fused step recomputes its first half drift
   return drift(d);
 }
 
 int mdgpu_send_border_particles(mdgpu_domain *d) {
   if (!d) return MDGPU_ERR_ARGUMENT;
+  d->mid_valid = false;  // the next fused step recomputes its first half drift
   return refresh(d);
 }
 
@@ -1272,6 +1314,7 @@ int mdgpu_calculate_force(mdgpu_domain *d) {
 
 int mdgpu_langevin(mdgpu_domain *d) {
   if (!d) return MDGPU_ERR_ARGUMENT;
+  d->mid_valid = false;  // the next fused step recomputes its first half drift
   if (d->pn > 0)
     hipLaunchKernelGGL(k_langevin, dim3(nblk(d->pn)), dim3(kBlock), 0, d->stream, d->mom, d->id, d->pn,
                        lconsts(d));
@@ -1293,6 +1336,7 @@ int mdgpu_heatbath_zeta(mdgpu_domain *d, double t) {
 
 int mdgpu_heatbath_momenta(mdgpu_domain *d) {
   if (!d) return MDGPU_ERR_ARGUMENT;
+  d->mid_valid = false;  // the next fused step recomputes its first half drift
   return heatbath_momenta(d);
 }
 
@@ -1348,6 +1392,7 @@ int mdgpu_observe(mdgpu_domain *d, double *ke, double *pe, double *virial) {
     switch (d->waves) {
       case 2: launch_observe<2>(d, bl, shm); break;
       case 8: launch_observe<8>(d, bl, shm); break;
+      case 16: launch_observe<16>(d, bl, shm); break;
       default: launch_observe<4>(d, bl, shm); break;
     }
     hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(kBlock), 0, d->stream, d->partial, nb, 3, d->d3);
@@ -1369,6 +1414,7 @@ int mdgpu_observe_kinetic(mdgpu_domain *d, double *ke) {
 
 int mdgpu_change_scale(mdgpu_domain *d, double alpha) {
   if (!d || !(alpha > 0.0)) return MDGPU_ERR_ARGUMENT;
+  d->mid_valid = false;  // the next fused step recomputes its first half drift
   CHK(make_pair_list(d));
   for (int k = 0; k < 3; k++) {
     d->L[k] = d->L[k] * alpha;
@@ -1527,6 +1573,7 @@ int mdgpu_get_timing(const mdgpu_domain *d, int *force_launches, double *force_m
 
 int mdgpu_set_option(mdgpu_domain *d, int option, int value) {
   if (!d) return MDGPU_ERR_ARGUMENT;
+  d->mid_valid = false;  // the next fused step recomputes its first half drift
   switch (option) {
     case MDGPU_OPT_UNROLL:
       if (value != 4 && value != 8) return MDGPU_ERR_ARGUMENT;
@@ -1536,7 +1583,7 @@ int mdgpu_set_option(mdgpu_domain *d, int option, int value) {
       d->kahan = value != 0;
       return MDGPU_OK;
     case MDGPU_OPT_WAVES:
-      if (value != 2 && value != 4 && value != 8) return MDGPU_ERR_ARGUMENT;
+      if (value != 2 && value != 4 && value != 8 && value != 16) return MDGPU_ERR_ARGUMENT;
       d->waves = value;
       return MDGPU_OK;
     case MDGPU_OPT_LDS_KB:

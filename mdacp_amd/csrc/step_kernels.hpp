@@ -108,6 +108,25 @@ __global__ void k_drift_refresh(double4 *__restrict__ pos, const double4 *__rest
   }
 }
 
+// Halo refresh when `tmp` already holds the half-drifted owned positions
+// (written by the previous fused step's epilogue): ghosts only,
+// pos[g] = tmp[g] = tmp[src] + shift*L (and the FP32 copy) -- the values
+// k_drift_refresh would write.
+template <bool kRel>
+__global__ void k_refresh_from_drifted(double4 *__restrict__ pos, double4 *__restrict__ tmp,
+                                       const int *__restrict__ gsrc, const int *__restrict__ gcode, int pn,
+                                       int ng, double Lx, double Ly, double Lz, const int *__restrict__ cell,
+                                       Origins O, float4 *__restrict__ rel) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= ng) return;
+  const int i = pn + k;
+  const double3 y = image_of(tmp[gsrc[k]], gcode[k], Lx, Ly, Lz);
+  const double4 g = make_double4(y.x, y.y, y.z, 0.0);
+  pos[i] = g;
+  tmp[i] = g;
+  if (kRel) rel[i] = to_rel(g, cell[i], O.x, O.y, O.z, O.mx, O.my);
+}
+
 // ----------------------------------------------------------- Langevin RNG
 __device__ __forceinline__ void philox4x32_10(uint32_t c[4], uint32_t k0, uint32_t k1) {
 #pragma unroll
