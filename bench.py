@@ -191,7 +191,7 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64",
-        "data": "synthetic: FCC lattice (confmaker.cc:8-44), |v0|=sqrt(3) random directions (variables.cc:195-222)",
+        "data": "synthetic: FCC lattice (confmaker.cc:8-44), |v0|=sqrt(3) random directions (variables.cc:82-108)",
         "config": {"workload": (f"LJ NVE N={n} FCC rho={args.density} L={L:.6g} rc={args.rc} margin={args.margin} "
                                 f"dt={args.dt}, T/P/E observed every 100 steps, rebuilds by the Verlet budget"),
                    "particles": n, "rebuilds_in_timed_steps": r64["rebuilds"]},

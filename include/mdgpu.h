@@ -11,29 +11,29 @@
  * of the last error of a domain is available from mdgpu_last_error().
  *
  * The entry points replace these reference interfaces (kaityo256/mdacp):
- *   mdgpu_create            Variables::Variables (variables.cc:130-142),
+ *   mdgpu_create            Variables::Variables (variables.cc:16-28),
  *                           SimulationInfo (simulationinfo.cc:5-51),
- *                           MeshList::MeshList/ChangeScale (meshlist.cc:73-120)
- *   mdgpu_add_particles     MDUnit::AddParticle (mdunit.cc:113-127)
+ *                           MeshList::MeshList/ChangeScale (meshlist.cc:8-55)
+ *   mdgpu_add_particles     MDUnit::AddParticle (mdunit.cc:36-49)
  *   mdgpu_set_momenta       writes to Variables::p (variables.h:18)
  *   mdgpu_get_particles     reads Variables::q/p/type (variables.h:16-18)
- *   mdgpu_make_pair_list    MDManager::MakePairList (mdmanager.cc:261-280) on one
+ *   mdgpu_make_pair_list    MDManager::MakePairList (mdmanager.cc:262-280) on one
  *                           domain: SendParticles wrap, FindBorderParticles,
  *                           MDUnit::MakePairList -> PairList::Init + MeshList::MakeList
- *   mdgpu_is_pair_list_expired  PairList::IsPairListExpired (pairlist.cc:58-154)
- *   mdgpu_update_position_half  ForceCalculator::UpdatePositionHalf (fcalculator.cc:21-32)
- *   mdgpu_send_border_particles MDManager::SendBorderParticles (mdmanager.cc:282-348)
- *   mdgpu_calculate_force   ForceCalculator::CalculateForce (fcalculator.cc:34-50,
+ *   mdgpu_is_pair_list_expired  PairList::IsPairListExpired (pairlist.cc:26-121)
+ *   mdgpu_update_position_half  ForceCalculator::UpdatePositionHalf (fcalculator.cc:22-32)
+ *   mdgpu_send_border_particles MDManager::SendBorderParticles (mdmanager.cc:283-348)
+ *   mdgpu_calculate_force   ForceCalculator::CalculateForce (fcalculator.cc:35-50,
  *                           semantics of CalculateForceNext :228-290)
- *   mdgpu_langevin          ForceCalculator::Langevin (fcalculator.cc:764-778); the noise is
+ *   mdgpu_langevin          ForceCalculator::Langevin (fcalculator.cc:765-778); the noise is
  *                           Philox4x32-10 keyed by (seed, particle id, call count), not
  *                           the unit's std::mt19937 stream
- *   mdgpu_heatbath_zeta     ForceCalculator::HeatbathZeta (fcalculator.cc:743-749)
- *   mdgpu_heatbath_momenta  ForceCalculator::HeatbathMomenta (fcalculator.cc:751-762)
- *   mdgpu_calculate         MDManager::Calculate (mdmanager.cc:115-192), n steps
+ *   mdgpu_heatbath_zeta     ForceCalculator::HeatbathZeta (fcalculator.cc:744-749)
+ *   mdgpu_heatbath_momenta  ForceCalculator::HeatbathMomenta (fcalculator.cc:752-762)
+ *   mdgpu_calculate         MDManager::Calculate (mdmanager.cc:116-192), n steps
  *   mdgpu_observe           KineticEnergyObserver, PotentialEnergyObserver,
- *                           VirialObserver (observer.cc:4-79)
- *   mdgpu_change_scale      MDManager::ChangeScale (mdmanager.cc:431-443)
+ *                           VirialObserver (observer.cc:5-79)
+ *   mdgpu_change_scale      MDManager::ChangeScale (mdmanager.cc:432-443)
  *   mdgpu_get_pairs         MeshList::GetKeyParticles/GetPartnerParticles
  *                           (meshlist.h:46-48), as canonical pairs
  */
@@ -49,9 +49,9 @@ typedef struct mdgpu_domain mdgpu_domain;
 
 /* status codes */
 #define MDGPU_OK 0
-#define MDGPU_ERR_TOO_FAST 1         /* "Too fast particles exists" (pairlist.cc:92-96) */
-#define MDGPU_ERR_INVALID_POSITION 2 /* "Invalid position" (variables.cc:184-192) */
-#define MDGPU_ERR_INVALID_INDEX 3    /* "Invalid index" (meshlist.cc:244-249) */
+#define MDGPU_ERR_TOO_FAST 1         /* "Too fast particles exists" (pairlist.cc:58-62) */
+#define MDGPU_ERR_INVALID_POSITION 2 /* "Invalid position" (variables.cc:70-78) */
+#define MDGPU_ERR_INVALID_INDEX 3    /* "Invalid index" (meshlist.cc:178-182) */
 #define MDGPU_ERR_CAPACITY 4         /* particle or pair capacity exceeded */
 #define MDGPU_ERR_HIP 5              /* a HIP runtime call failed */
 #define MDGPU_ERR_ARGUMENT 6         /* invalid argument */
@@ -71,7 +71,7 @@ typedef struct {
   double dt;              /* TimeStep */
   double rc;              /* cutoff (CUTOFF_LENGTH, mdconfig.h:15) */
   double margin;          /* BufferLength (simulationinfo.cc:6) */
-  int check_list_length;  /* CheckListLength (simulationinfo.cc:343), 500 */
+  int check_list_length;  /* CheckListLength (simulationinfo.cc:29), 500 */
   int precision;          /* MDGPU_FP64 or MDGPU_MIXED */
   int capacity;           /* initial owned-particle capacity (grows on demand) */
   int device;             /* HIP device ordinal */
@@ -91,7 +91,7 @@ int mdgpu_set_momenta(mdgpu_domain *d, const double *p); /* in mdgpu_get_particl
 /* overwrite owned positions in mdgpu_get_particles order (writes to Variables::q,
    variables.h:17, by host code between steps); ghosts follow at the next refresh */
 int mdgpu_set_positions(mdgpu_domain *d, const double *q);
-/* drop all particles (Variables::SetParticleNumber(0), variables.h:31) */
+/* drop all particles (Variables::SetParticleNumber(0), variables.h:30) */
 int mdgpu_clear_particles(mdgpu_domain *d);
 /* ghost images: owner id, shift code, position (the ghost part of Variables::q,
    indices [pn, tn) in the reference) */
@@ -121,7 +121,7 @@ int mdgpu_calculate(mdgpu_domain *d, int nsteps, int *rebuilds);
 
 /* observers: raw sums as returned by the reference observers */
 int mdgpu_observe(mdgpu_domain *d, double *ke, double *pe, double *virial);
-/* KineticEnergyObserver alone (observer.cc:4-15); needs no pair list */
+/* KineticEnergyObserver alone (observer.cc:5-15); needs no pair list */
 int mdgpu_observe_kinetic(mdgpu_domain *d, double *ke);
 
 /* MDManager::ChangeScale: rebuild, scale box and owned positions by alpha, rebuild */

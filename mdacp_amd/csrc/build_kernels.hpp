@@ -1,13 +1,12 @@
 // build_kernels.hpp -- pair-list construction on gfx950: periodic wrap,
-// counting-sort cell binning (atomics + prefix scan, then a per-cell order by
-// canonical key so the result does not depend on atomic ordering), ghost
-// images of the single-domain halo, and the full neighbour list in sliced-ELL
-// layout (64 rows per slice, one wavefront).
+// counting-sort cell binning (atomics + prefix scan, then each cell's members
+// put back in input order, so the result does not depend on atomic ordering)
+// and the ghost images of the single-domain halo.
 //
 // Reference semantics (kaityo256/mdacp): AdjustPeriodicBoundary
-// (variables.cc:172-193), MakeMesh (meshlist.cc:217-270), FindBorderParticles
-// and MakeBufferForBorderParticles (mdunit.cc:202-254, mdrect.cc:356-380),
-// SearchMesh (meshlist.cc:284-336).  The distance test is evaluated without
+// (variables.cc:59-79), MakeMesh (meshlist.cc:153-205), FindBorderParticles
+// and MakeBufferForBorderParticles (mdunit.cc:125-176, mdrect.cc:24-47),
+// SearchMesh (meshlist.cc:220-271).  The distance test is evaluated without
 // fused multiply-adds (the library is compiled with -ffp-contract=off), so the
 // pair set is bit-exact with the reference's default build.
 #pragma once
@@ -60,30 +59,25 @@ __global__ void k_scatter_perm(const int *__restrict__ cell, const int *__restri
   perm[cstart[cell[i]] + slot[i]] = i;
 }
 
-// Order each cell's members by a canonical 64-bit key (insertion sort; cells
-// hold tens of particles).
+// Order each cell's members by their index in the input order (insertion
+// sort of the cell's contiguous segment of perm; cells hold tens of
+// particles).  With the slots of k_wrap_bin taken by atomics, this makes the
+// order deterministic: a stable counting sort.
 __global__ void k_sort_cells(int *__restrict__ perm, const int *__restrict__ cstart,
-                             const int *__restrict__ ccount, const long long *__restrict__ key,
-                             int ncell) {
+                             const int *__restrict__ ccount, int ncell) {
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= ncell) return;
   int *a = perm + cstart[c];
   const int n = ccount[c];
   for (int t = 1; t < n; t++) {
     const int v = a[t];
-    const long long kv = key[v];
     int u = t - 1;
-    while (u >= 0 && key[a[u]] > kv) {
+    while (u >= 0 && a[u] > v) {
       a[u + 1] = a[u];
       u--;
     }
     a[u + 1] = v;
   }
-}
-
-__global__ void k_owned_key(const int *__restrict__ id, int n, long long *__restrict__ key) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) key[i] = (long long)id[i];
 }
 
 __global__ void k_gather_owned(const int *__restrict__ perm, int n, const double4 *__restrict__ pos_in,
@@ -132,8 +126,7 @@ __global__ void k_emit_images(const double4 *__restrict__ pos, int n, Edges E, d
                               double Lz, Grid g, const int *__restrict__ goff, int ng,
                               int *__restrict__ gsrc, int *__restrict__ gcode,
                               double4 *__restrict__ gpos, int *__restrict__ gcell,
-                              int *__restrict__ gslot, int *__restrict__ gcount,
-                              long long *__restrict__ gkey, const int *__restrict__ id) {
+                              int *__restrict__ gslot, int *__restrict__ gcount) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const double4 q = pos[i];
@@ -160,7 +153,6 @@ __global__ void k_emit_images(const double4 *__restrict__ pos, int n, Edges E, d
         const int cl = cell_of(y.x, y.y, y.z, g);
         gcell[k] = cl;
         gslot[k] = atomicAdd(&gcount[cl], 1);
-        gkey[k] = (long long)id[i] * 32 + code;
         k++;
       }
     }

@@ -72,7 +72,7 @@ __global__ void k_reduce_max(const double *__restrict__ partial, int n, unsigned
 }
 
 // Image displacement of a shift code (sx+1)+3(sy+1)+9(sz+1): one +-L or 0.0
-// per component, added as the reference's halo adds it (mdunit.cc:215-245).
+// per component, added as the reference's halo adds it (mdunit.cc:142-161).
 __device__ __forceinline__ double3 image_of(double4 q, int code, double Lx, double Ly, double Lz) {
   const int sx = code % 3 - 1, sy = (code / 3) % 3 - 1, sz = code / 9 - 1;
   const double dx = sx > 0 ? Lx : (sx < 0 ? -Lx : 0.0);

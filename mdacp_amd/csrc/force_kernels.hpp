@@ -2,10 +2,10 @@
 // (FP64 and mixed FP32) and the pair observers, one workgroup per cell brick.
 //
 // Reference semantics (kaityo256/mdacp):
-//   CalculateForceNext  fcalculator.cc:228-290: for each listed pair within rc,
+//   CalculateForceNext  fcalculator.cc:229-290: for each listed pair within rc,
 //                       p_i += df*(q_j - q_i), df = ((24 r^6 - 48)/(r^6 r^6 r^2) + 8 C2) dt
-//   potential           4(r^-12 - r^-6 + C2 r^2 + C0), C0/C2 from variables.cc:130-142
-//   observers           observer.cc:17-79 (PE and virial, ghost pairs halved)
+//   potential           4(r^-12 - r^-6 + C2 r^2 + C0), C0/C2 from variables.cc:16-28
+//   observers           observer.cc:18-79 (PE and virial, ghost pairs halved)
 //
 // A workgroup first copies its brick's neighbourhood (list_kernels.hpp) into
 // LDS: the owned particles and ghost images of the cells around the brick,
@@ -184,7 +184,7 @@ __device__ __forceinline__ double finish_row(int i, double4 p, const double4 qi,
     q.z += p.z * dt2;
     pos_out[i] = q;
   } else if (epilogue == kKickLangevin) {
-    pos_out[i] = qi;  // one half drift per Langevin step (mdmanager.cc:185-192)
+    pos_out[i] = qi;  // one half drift per Langevin step (mdmanager.cc:186-192)
   }
   if (nd.pos != nullptr && epilogue != kKickOnly) {
     q.x += p.x * dt2;

@@ -1,6 +1,6 @@
 // list_kernels.hpp -- neighbour list on gfx950, organised by cell bricks.
 //
-// The extended cell grid (meshlist.cc:95-120) is cut into bricks of
+// The extended cell grid (meshlist.cc:31-55) is cut into bricks of
 // bd[0] x bd[1] x bd[2] cells.  Cells are ranked brick by brick, and owned
 // particles are stored in that order, so the rows of one brick are contiguous
 // and one workgroup of the force kernels takes one brick.  The partners of a
@@ -14,7 +14,7 @@
 // listed partner is covered whatever the rounding of its image position.
 //
 // Which pairs are listed is decided exactly as in the reference
-// (MeshList::SearchMesh, meshlist.cc:284-336): owned rows against owned and
+// (MeshList::SearchMesh, meshlist.cc:220-271): owned rows against owned and
 // materialised ghost particles binned in the extended grid, r^2 = dx*dx +
 // dy*dy + dz*dz evaluated without fused multiply-adds, kept when !(r2 > S2).
 #pragma once
@@ -237,13 +237,20 @@ __device__ __forceinline__ RowCell row_cell(int i, int c, const BrickBox &X, con
   return rc;
 }
 
-// MeshList::SearchMesh over a brick (meshlist.cc:284-336): each owned row of
+// MeshList::SearchMesh over a brick (meshlist.cc:220-271): each owned row of
 // the brick tests the owned and ghost particles of the 27 cells around its
 // cell, staged in LDS.  For a row the candidates come as 9 contiguous staged
 // runs (x-neighbouring cells are adjacent in the staged order); bit k of the
 // row's mask marks candidate k, in run order, as listed: r^2 = dx*dx + dy*dy +
 // dz*dz without fused multiply-adds, kept when !(r2 > S2), the row itself
 // skipped.  row_len receives the count.
+//
+// Threads take consecutive rows (rows of one cell share their candidates, so
+// most staged reads are broadcasts); each thread tests kSearchBatch
+// candidates at a time and collects the bits in a 64-bit accumulator that is
+// flushed to the mask a word at a time.
+constexpr int kSearchBatch = 4;
+
 template <int W>
 __global__ __launch_bounds__(kWave *W) void k_search_brick(const double4 *__restrict__ pos,
                                                           const int *__restrict__ cell, Bricks B,
@@ -264,45 +271,57 @@ __global__ __launch_bounds__(kWave *W) void k_search_brick(const double4 *__rest
   const BrickBox X = brick_box(b, B);
   const int *ts = st.tstart + (long long)b * st.rstride;
   const int nrow = st.nrow[b], row0 = st.row0[b];
+  const int nwords = mcap;
   for (int v = threadIdx.x; v < nrow; v += T) {
     const int i = row0 + v;
     const double4 qi = pos[i];
     const RowCell rc = row_cell(i, cell[i], X, B, tb, ts);
     uint32_t *mk = mask + (long long)i * mcap;
-    const int nbits = mcap * 32;
-    int nbit = 0, cnt = 0;
-    uint32_t word = 0;
+    unsigned long long acc = 0;  // pending bits, oldest in bit 0
+    int accbits = 0, word = 0, cnt = 0;
     for (int dz = -1; dz <= 1; dz++) {
       for (int dy = -1; dy <= 1; dy++) {
         const int r0 = ((rc.lz + dz) * X.n[1] + (rc.ly + dy)) * X.n[0] + rc.lx - 1;
         const int u0 = tb[kRangesPerCell * r0], u1 = tb[kRangesPerCell * (r0 + 3)];
-        for (int u = u0; u < u1; u++) {
-          double qx, qy, qz;
-          if (staged) {
-            qx = lds64[3 * u];
-            qy = lds64[3 * u + 1];
-            qz = lds64[3 * u + 2];
-          } else {
-            const double4 q = pos[gl[u]];
-            qx = q.x;
-            qy = q.y;
-            qz = q.z;
+        for (int u = u0; u < u1; u += kSearchBatch) {
+          const int nc = min(kSearchBatch, u1 - u);
+          unsigned h = 0;
+#pragma unroll
+          for (int t = 0; t < kSearchBatch; t++) {
+            const int uu = min(u + t, u1 - 1);  // in range; surplus lanes of the batch are masked below
+            double qx, qy, qz;
+            if (staged) {
+              qx = lds64[3 * uu];
+              qy = lds64[3 * uu + 1];
+              qz = lds64[3 * uu + 2];
+            } else {
+              const double4 q = pos[gl[uu]];
+              qx = q.x;
+              qy = q.y;
+              qz = q.z;
+            }
+            const double ex = qi.x - qx, ey = qi.y - qy, ez = qi.z - qz;
+            const double r2 = (ex * ex + ey * ey + ez * ez);
+            const bool hit = t < nc && !(r2 > S2) && uu != rc.self;
+            h |= (hit ? 1u : 0u) << t;
           }
-          const double ex = qi.x - qx, ey = qi.y - qy, ez = qi.z - qz;
-          const double r2 = (ex * ex + ey * ey + ez * ez);
-          const bool hit = !(r2 > S2) && u != rc.self;
-          word |= (hit ? 1u : 0u) << (nbit & 31);
-          cnt += hit ? 1 : 0;
-          nbit++;
-          if ((nbit & 31) == 0) {
-            if (nbit <= nbits) mk[(nbit >> 5) - 1] = word;
-            word = 0;
+          cnt += __popc(h);
+          acc |= (unsigned long long)h << accbits;
+          accbits += nc;
+          if (accbits >= 32) {
+            if (word < nwords) mk[word] = (uint32_t)acc;
+            word++;
+            acc >>= 32;
+            accbits -= 32;
           }
         }
       }
     }
-    if ((nbit & 31) != 0 && nbit <= nbits) mk[nbit >> 5] = word;
-    if (nbit > nbits) atomicOr(err, kErrSearchCap);
+    if (accbits > 0) {
+      if (word < nwords) mk[word] = (uint32_t)acc;
+      word++;
+    }
+    if (word > nwords) atomicOr(err, kErrSearchCap);
     row_len[i] = cnt;
   }
 }
@@ -394,7 +413,7 @@ __global__ void k_slice_len(const int *__restrict__ row_len, const int *__restri
   block_max_to_partial((double)m, partial);
 }
 
-// MeshList::MakeList (meshlist.cc:57-87) in sliced-ELL form: one wavefront per
+// MeshList::MakeList (meshlist.cc:58-87) in sliced-ELL form: one wavefront per
 // 64-row slice (rows in the order of k_brick_row_order, rowsrc[row] = the
 // row's index in the search) turns the rows' candidate masks into staged indices and cell
 // offset codes, pads every row to the slice length with the brick's pad

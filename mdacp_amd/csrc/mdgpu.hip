@@ -182,7 +182,7 @@ int check_launch(mdgpu_domain *d, const char *what) {
   return MDGPU_OK;
 }
 
-// MeshList::ChangeScale (meshlist.cc:95-120), the column origins of the
+// MeshList::ChangeScale (meshlist.cc:31-55), the column origins of the
 // cell-relative FP32 coordinates, and the brick order of the cells.
 int set_geometry(mdgpu_domain *d) {
   for (int k = 0; k < 3; k++) {
@@ -410,9 +410,8 @@ int read_max(mdgpu_domain *d, int k, double *v) {
 
 // Order n binned items (cellv[], slot[] from atomics, counts[] per cell):
 // starts[] over cells (in brick rank order when `ranked`), perm[] = new order,
-// members of a cell ordered by keys[].
-int cell_sort(mdgpu_domain *d, const int *cellv, int n, int *counts, int *starts, const long long *keys,
-              bool ranked) {
+// members of a cell in input order (k_sort_cells).
+int cell_sort(mdgpu_domain *d, const int *cellv, int n, int *counts, int *starts, bool ranked) {
   hipStream_t st = d->stream;
   CHK(ensure_scan(d, d->ncell));
   if (ranked) {
@@ -425,8 +424,7 @@ int cell_sort(mdgpu_domain *d, const int *cellv, int n, int *counts, int *starts
     exclusive_scan<int, int>(counts, starts, d->ncell, d->scan_scratch, nullptr, st);
   }
   hipLaunchKernelGGL(k_scatter_perm, dim3(nblk(n)), dim3(kBlock), 0, st, cellv, d->slot, starts, n, d->perm);
-  hipLaunchKernelGGL(k_sort_cells, dim3(nblk(d->ncell)), dim3(kBlock), 0, st, d->perm, starts, counts, keys,
-                     d->ncell);
+  hipLaunchKernelGGL(k_sort_cells, dim3(nblk(d->ncell)), dim3(kBlock), 0, st, d->perm, starts, counts, d->ncell);
   return check_launch(d, "cell sort");
 }
 
@@ -465,13 +463,10 @@ void launch_search_w(mdgpu_domain *d, const Bricks &B, const SearchTables &t) {
                      d->S2, d->mcap, d->row_len, d->mask, d->i32);
 }
 
+// The search keeps one brick's staged neighbourhood in LDS like the force
+// kernel; with no per-slice work to balance it runs 16 wavefronts per brick.
 int launch_search(mdgpu_domain *d, const Bricks &B, const SearchTables &t) {
-  switch (d->waves) {
-    case 2: launch_search_w<2>(d, B, t); break;
-    case 16: launch_search_w<16>(d, B, t); break;
-    case 4: launch_search_w<4>(d, B, t); break;
-    default: launch_search_w<8>(d, B, t); break;
-  }
+  launch_search_w<16>(d, B, t);
   return check_launch(d, "neighbour search");
 }
 
@@ -524,9 +519,9 @@ int order_rows(mdgpu_domain *d) {
   return check_launch(d, "row order");
 }
 
-// MDManager::MakePairList for the single domain (mdmanager.cc:261-280).
+// MDManager::MakePairList for the single domain (mdmanager.cc:262-280).
 // The reference also permutes particles that cross a face
-// (MakeBufferForSendingParticle, mdunit.cc:155-179); that only renames
+// (MakeBufferForSendingParticle, mdunit.cc:78-101); that only renames
 // indices, and particles are put in brick order here anyway.
 int make_pair_list(mdgpu_domain *d) {
   hipStream_t st = d->stream;
@@ -541,10 +536,9 @@ int make_pair_list(mdgpu_domain *d) {
   if (pn > 0) {
     hipLaunchKernelGGL(k_wrap_bin, dim3(nblk(pn)), dim3(kBlock), 0, st, d->pos, pn, d->L[0], d->L[1], d->L[2],
                        g, d->cell, d->slot, d->c_ocount, d->i32, 1);
-    hipLaunchKernelGGL(k_owned_key, dim3(nblk(pn)), dim3(kBlock), 0, st, d->id, pn, d->key);
   }
   CHK(read_err_flags(d));
-  CHK(cell_sort(d, d->cell, pn, d->c_ocount, d->c_ostart, d->key, true));
+  CHK(cell_sort(d, d->cell, pn, d->c_ocount, d->c_ostart, true));
   if (pn > 0) {
     hipLaunchKernelGGL(k_gather_owned, dim3(nblk(pn)), dim3(kBlock), 0, st, d->perm, pn, d->pos, d->mom, d->id,
                        d->cell, d->pos_tmp, d->mom_alt, d->id_alt, d->cell_alt);
@@ -579,9 +573,9 @@ int make_pair_list(mdgpu_domain *d) {
     // pos_tmp[0..ng) holds the unsorted images
     hipLaunchKernelGGL(k_emit_images, dim3(nblk(pn)), dim3(kBlock), 0, st, d->pos, pn, E, d->L[0], d->L[1],
                        d->L[2], g, d->goff, ng, d->t_gsrc, d->t_gcode, d->pos_tmp, d->t_gcell, d->slot,
-                       d->c_gcount, d->key, d->id);
+                       d->c_gcount);
     CHK(check_launch(d, "emit images"));
-    CHK(cell_sort(d, d->t_gcell, ng, d->c_gcount, d->c_gstart, d->key, false));
+    CHK(cell_sort(d, d->t_gcell, ng, d->c_gcount, d->c_gstart, false));
     hipLaunchKernelGGL(k_gather_ghosts, dim3(nblk(ng)), dim3(kBlock), 0, st, d->perm, ng, pn, d->t_gsrc,
                        d->t_gcode, d->pos_tmp, d->t_gcell, d->gsrc, d->gcode, d->pos, d->cell);
   } else {
@@ -682,7 +676,7 @@ int make_pair_list(mdgpu_domain *d) {
   d->nbr_slots = slots;
   if (pn > 0 && scap > 0) CHK(launch_emit(d, B, STb, scap));
   CHK(check_launch(d, "neighbour list"));
-  // PairList::Init: snapshot of owned + ghost positions (pairlist.cc:10-23)
+  // PairList::Init: snapshot of owned + ghost positions (pairlist.cc:11-23)
   if (d->tn > 0)
     hipLaunchKernelGGL(k_copy_pos, dim3(nblk(d->tn)), dim3(kBlock), 0, st, d->pos, d->pos_old, d->tn);
   CHK(read_err_flags(d));
@@ -702,7 +696,7 @@ int max_p2(mdgpu_domain *d, double *v2) {
   return read_max(d, 0, v2);
 }
 
-// PairList::CheckByDisplacement (pairlist.cc:100-154)
+// PairList::CheckByDisplacement (pairlist.cc:68-121)
 int check_by_displacement(mdgpu_domain *d, int *expired) {
   hipStream_t st = d->stream;
   const int tn = d->tn;
@@ -738,8 +732,8 @@ int check_by_displacement(mdgpu_domain *d, int *expired) {
   return MDGPU_OK;
 }
 
-// PairList::IsPairListExpired (pairlist.cc:58-78) with
-// UpdatePairListValidity (pairlist.cc:80-98)
+// PairList::IsPairListExpired (pairlist.cc:26-45) with
+// UpdatePairListValidity (pairlist.cc:48-65)
 int is_expired(mdgpu_domain *d, int *expired) {
   if (!d->initialized) {
     *expired = 1;
@@ -938,7 +932,7 @@ int force_in_place(mdgpu_domain *d) {
   return MDGPU_OK;
 }
 
-// MDManager::Calculate (mdmanager.cc:115-148) for one domain
+// MDManager::Calculate (mdmanager.cc:116-148) for one domain
 int calculate_step(mdgpu_domain *d) {
   hipStream_t st = d->stream;
   int expired = 0;
@@ -963,7 +957,7 @@ int calculate_step(mdgpu_domain *d) {
     d->rebuild_count++;
   }
   if (d->thermostat == MDGPU_NOSEHOOVER) {
-    // CalculateNoseHoover (mdmanager.cc:159-175)
+    // CalculateNoseHoover (mdmanager.cc:160-175)
     d->mid_valid = false;
     CHK(drift(d));
     CHK(refresh(d));
@@ -1050,7 +1044,7 @@ int mdgpu_create(const mdgpu_params *prm, mdgpu_domain **out) {
   }
   for (int k = 0; k < 3; k++) {
     d->L[k] = prm->L[k];
-    // single domain of a 1x1x1 grid: s = L*0, e = s + L (mdunit.cc:96-101)
+    // single domain of a 1x1x1 grid: s = L*0, e = s + L (mdunit.cc:17-24)
     d->rs[k] = prm->L[k] * 0.0;
     d->re[k] = d->rs[k] + prm->L[k];
   }
@@ -1062,7 +1056,7 @@ int mdgpu_create(const mdgpu_params *prm, mdgpu_domain **out) {
   d->CL2 = prm->rc * prm->rc;
   d->check_list_length = prm->check_list_length > 0 ? prm->check_list_length : 500;
   d->precision = prm->precision;
-  // Variables::Variables (variables.cc:130-142)
+  // Variables::Variables (variables.cc:16-28)
   const double s2 = 1.0 / (prm->rc * prm->rc);
   const double s6 = s2 * s2 * s2;
   const double s8 = s6 * s2;
@@ -1137,7 +1131,7 @@ int mdgpu_add_particles(mdgpu_domain *d, int n, const double *q, const double *p
   hq.reserve(n);
   for (int i = 0; i < n; i++) {
     const double *x = q + 3 * (size_t)i;
-    // MDRect::IsInside (mdrect.h:319-322)
+    // MDRect::IsInside (mdrect.h:35-38)
     if (!(d->rs[0] <= x[0] && x[0] < d->re[0] && d->rs[1] <= x[1] && x[1] < d->re[1] && d->rs[2] <= x[2] &&
           x[2] < d->re[2]))
       continue;

@@ -3,12 +3,12 @@
 // Nose-Hoover scaling, kinetic energy and the Verlet-list expiry test.
 //
 // Reference semantics (kaityo256/mdacp):
-//   UpdatePositionHalf   fcalculator.cc:21-32   q += p*(dt*0.5)
-//   SendBorderParticles  mdmanager.cc:282-348, mdunit.cc:214-269
-//   Langevin             fcalculator.cc:764-778
-//   HeatbathMomenta      fcalculator.cc:751-762
-//   KineticEnergyObserver observer.cc:4-15
-//   expiry               pairlist.cc:58-154
+//   UpdatePositionHalf   fcalculator.cc:22-32   q += p*(dt*0.5)
+//   SendBorderParticles  mdmanager.cc:283-348, mdunit.cc:137-191
+//   Langevin             fcalculator.cc:765-778
+//   HeatbathMomenta      fcalculator.cc:752-762
+//   KineticEnergyObserver observer.cc:5-15
+//   expiry               pairlist.cc:26-121
 #pragma once
 #include "device_common.hpp"
 

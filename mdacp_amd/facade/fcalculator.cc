@@ -14,8 +14,8 @@ void device_kick(Variables *vars) {
 }
 }  // namespace
 //----------------------------------------------------------------------
-// ForceCalculator::CalculateForce (fcalculator.cc:34-50) with the semantics of
-// CalculateForceNext (fcalculator.cc:228-290): p_i += df (q_j - q_i) over the
+// ForceCalculator::CalculateForce (fcalculator.cc:35-50) with the semantics of
+// CalculateForceNext (fcalculator.cc:229-290): p_i += df (q_j - q_i) over the
 // listed partners within the cutoff.  The reference's other variants compute
 // the same kick.
 void
@@ -61,7 +61,7 @@ ForceCalculator::CalculateForceAVX2(Variables *vars, MeshList *mesh, SimulationI
 }
 #endif
 //----------------------------------------------------------------------
-// ForceCalculator::UpdatePositionHalf (fcalculator.cc:21-32)
+// ForceCalculator::UpdatePositionHalf (fcalculator.cc:22-32)
 void
 ForceCalculator::UpdatePositionHalf(Variables *vars, SimulationInfo *sinfo) {
   vars->ToDevice();
@@ -69,7 +69,7 @@ ForceCalculator::UpdatePositionHalf(Variables *vars, SimulationInfo *sinfo) {
   vars->DeviceChanged();
 }
 //----------------------------------------------------------------------
-// ForceCalculator::HeatbathZeta (fcalculator.cc:743-749): a scalar update of
+// ForceCalculator::HeatbathZeta (fcalculator.cc:744-749): a scalar update of
 // Variables::Zeta, done on the host in the reference's order of operations.
 void
 ForceCalculator::HeatbathZeta(Variables *vars, double current_temperature, SimulationInfo *sinfo) {
@@ -79,7 +79,7 @@ ForceCalculator::HeatbathZeta(Variables *vars, double current_temperature, Simul
   vars->Zeta += t1 * dt2;
 }
 //----------------------------------------------------------------------
-// ForceCalculator::HeatbathMomenta (fcalculator.cc:751-762): p *= exp(-dt/2 Zeta)
+// ForceCalculator::HeatbathMomenta (fcalculator.cc:752-762): p *= exp(-dt/2 Zeta)
 // on the device, with the host's Zeta.
 void
 ForceCalculator::HeatbathMomenta(Variables *vars, SimulationInfo *sinfo) {
@@ -92,7 +92,7 @@ ForceCalculator::HeatbathMomenta(Variables *vars, SimulationInfo *sinfo) {
   vars->DeviceChanged();
 }
 //----------------------------------------------------------------------
-// ForceCalculator::Langevin (fcalculator.cc:764-778): p += (-gamma p + N(0, hb_D)) dt
+// ForceCalculator::Langevin (fcalculator.cc:765-778): p += (-gamma p + N(0, hb_D)) dt
 // per component on the device; see fcalculator.h for the noise source.
 void
 ForceCalculator::Langevin(Variables *vars, SimulationInfo *sinfo, std::mt19937 &mt) {

@@ -25,7 +25,7 @@ int env_precision(void) {
 }
 }  // namespace
 //----------------------------------------------------------------------
-// MDUnit::MDUnit (mdunit.cc:8-26): the unit's box from the process grid, then
+// MDUnit::MDUnit (mdunit.cc:8-27): the unit's box from the process grid, then
 // a device domain for it (cutoff CUTOFF_LENGTH, margin BufferLength, the
 // CheckListLength of the displacement check).
 MDUnit::MDUnit(int id_, SimulationInfo *si, ParaInfo *pi):
@@ -99,7 +99,7 @@ MDUnit::AddParticle(double x[D], int type) {
   AddParticle(x, v, type);
 }
 //----------------------------------------------------------------------
-// MDUnit::AddParticle (mdunit.cc:36-41): only particles inside the unit's box.
+// MDUnit::AddParticle (mdunit.cc:36-42): only particles inside the unit's box.
 void
 MDUnit::AddParticle(double x[D], double v[D], int type) {
   if (myrect.IsInside(x)) {
@@ -113,7 +113,7 @@ MDUnit::SetInitialVelocity(double v0) {
   vars->SetInitialVelocity(v0, id);
 }
 //----------------------------------------------------------------------
-// MDUnit::SaveAsCdview (mdunit.cc:43-57): the same line format.
+// MDUnit::SaveAsCdview (mdunit.cc:52-67): the same line format.
 void
 MDUnit::SaveAsCdview(std::ofstream &ofs) {
   vars->ToHost();
@@ -133,7 +133,7 @@ MDUnit::SaveAsCdview(std::ofstream &ofs) {
   }
 }
 //----------------------------------------------------------------------
-// MDUnit::SaveConfiguration (mdunit.cc:59-65): conf<id>.cnf.
+// MDUnit::SaveConfiguration (mdunit.cc:70-75): conf<id>.cnf.
 void
 MDUnit::SaveConfiguration(void) {
   char filename[256];
@@ -148,7 +148,7 @@ MDUnit::SetSeed(const int seed) {
   vars->Check(mdgpu_set_seed(vars->dev, static_cast<uint64_t>(seed + GetID())), "seed");
 }
 //----------------------------------------------------------------------
-// Migration (MakeBufferForSendingParticle / ReceiveParticles, mdunit.cc:67-100):
+// Migration (MakeBufferForSendingParticle / ReceiveParticles, mdunit.cc:78-122):
 // a single domain is its own neighbour on every face, so particles that leave
 // through one face come back through the opposite one -- the periodic wrap,
 // which the device applies at every pair-list construction.  Nothing is sent.
@@ -166,7 +166,7 @@ MDUnit::ReceiveParticles(std::vector<ParticleInfo> &recv_buffer) {
 }
 //----------------------------------------------------------------------
 // Ghosts (FindBorderParticles / MakeBufferForBorderParticles /
-// ReceiveBorderParticles, mdunit.cc:102-191): the device finds the border
+// ReceiveBorderParticles, mdunit.cc:125-191): the device finds the border
 // particles and places their images at each construction, and refreshes the
 // images from the owners' current positions in one pass over all six
 // directions, done here for the first direction of MDManager's sequence.
@@ -215,7 +215,7 @@ MDUnit::Execute(Executor *ex) {
   if (vars->GetParticleNumber() != before) list_built = false;
 }
 //----------------------------------------------------------------------
-// MDUnit::MakePairList (mdunit.cc:193-198): PairList::Init and MeshList::MakeList
+// MDUnit::MakePairList (mdunit.cc:194-198): PairList::Init and MeshList::MakeList
 // on the device (wrap, ghost images, cell binning, Verlet list, snapshot).
 void
 MDUnit::MakePairList(void) {
@@ -224,7 +224,7 @@ MDUnit::MakePairList(void) {
   list_built = true;
 }
 //----------------------------------------------------------------------
-// MDUnit::ChangeScale (mdunit.cc:200-205): positions, the box and the cell
+// MDUnit::ChangeScale (mdunit.cc:201-205): positions, the box and the cell
 // grid scaled by alpha on the device (which rebuilds the list around it).
 void
 MDUnit::ChangeScale(double alpha) {

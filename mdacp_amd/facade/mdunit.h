@@ -1,5 +1,5 @@
 //----------------------------------------------------------------------
-// mdunit.h -- MDACP's MDUnit and Executor (kaityo256/mdacp mdunit.h:17-75)
+// mdunit.h -- MDACP's MDUnit and Executor (kaityo256/mdacp mdunit.h:20-75)
 // on the MI355X.  One MDUnit is one mdgpu domain: its particles, ghost
 // images, cell grid and Verlet list live in HBM and every step operation
 // MDManager calls runs on the GPU through the mdgpu C ABI (include/mdgpu.h).

@@ -2,7 +2,7 @@
 // meshlist.cc -- MeshList facade (interface of kaityo256/mdacp meshlist.h).
 // Construction happens on the device; the COO and CSR lists of the reference
 // (key_particles / partner_particles, number_of_partners / key_pointer /
-// sorted_list, meshlist.cc:57-87) are made on the host from the device list
+// sorted_list, meshlist.cc:58-87) are made on the host from the device list
 // when first asked for after a construction.
 //----------------------------------------------------------------------
 #include <stdio.h>
@@ -27,7 +27,7 @@ void
 MeshList::ChangeScale(SimulationInfo *sinfo, MDRect &myrect) {
 }
 //----------------------------------------------------------------------
-// MeshList::MakeList (meshlist.cc:57-87) and the MDUnit::MakePairList around
+// MeshList::MakeList (meshlist.cc:58-87) and the MDUnit::MakePairList around
 // it: on the device.
 void
 MeshList::MakeList(Variables *v, SimulationInfo *sinfo, MDRect &myrect) {
@@ -59,7 +59,7 @@ MeshList::Materialize(void) {
   vars->Check(mdgpu_get_pair_indices(vars->dev, &n, key_particles.data(), partner_particles.data()),
               "pair list export");
   number_of_pairs = static_cast<int>(n);
-  // CSR by key particle, partners in COO order (meshlist.cc:57-87)
+  // CSR by key particle, partners in COO order (meshlist.cc:58-87)
   const int tn = vars->GetTotalParticleNumber();
   number_of_partners.assign(tn + 1, 0);
   key_pointer.assign(tn + 1, 0);

@@ -1,5 +1,5 @@
 //----------------------------------------------------------------------
-// meshlist.h -- MDACP's MeshList (kaityo256/mdacp meshlist.h:9-61) on the
+// meshlist.h -- MDACP's MeshList (kaityo256/mdacp meshlist.h:10-62) on the
 // MI355X.  The cell grid and the Verlet list are built on the device
 // (MDUnit::MakePairList); this class keeps the reference's public interface.
 // The list accessors (GetKeyParticles, GetPartnerParticles, GetPairNumber,

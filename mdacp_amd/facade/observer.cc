@@ -1,6 +1,6 @@
 //----------------------------------------------------------------------
 // observer.cc -- the device observers (interface of kaityo256/mdacp
-// observer.h; sums of observer.cc:4-79 reduced on the GPU by mdgpu_observe).
+// observer.h; sums of observer.cc:5-79 reduced on the GPU by mdgpu_observe).
 //----------------------------------------------------------------------
 #include "observer.h"
 //----------------------------------------------------------------------
@@ -12,7 +12,7 @@ void observe(Variables *vars, double *ke, double *pe, double *vir) {
 }
 }  // namespace
 //----------------------------------------------------------------------
-// KineticEnergyObserver (observer.cc:4-15): needs no pair list.
+// KineticEnergyObserver (observer.cc:5-15): needs no pair list.
 double
 KineticEnergyObserver::Observe(Variables *vars, MeshList *mesh) {
   vars->ToDevice();

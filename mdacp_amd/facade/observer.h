@@ -1,5 +1,5 @@
 //----------------------------------------------------------------------
-// observer.h -- MDACP's observers (kaityo256/mdacp observer.h:9-39) on the
+// observer.h -- MDACP's observers (kaityo256/mdacp observer.h:8-40) on the
 // MI355X.  The kinetic energy, potential energy and virial observers are
 // device observers: their sums are reduced on the GPU from the resident
 // state and pair list (in a fixed order, so repeated calls agree bitwise).

@@ -1,12 +1,12 @@
 //----------------------------------------------------------------------
 // pairlist.cc -- PairList facade (interface of kaityo256/mdacp pairlist.h).
 // The snapshot, the validity budget and the displacement check
-// (pairlist.cc:10-121) run on the device; this class keeps the lifetime and
+// (pairlist.cc:11-121) run on the device; this class keeps the lifetime and
 // freshness the reference reports.
 //----------------------------------------------------------------------
 #include "pairlist.h"
 //----------------------------------------------------------------------
-// PairList::Init (pairlist.cc:10-23): the device took its snapshot during the
+// PairList::Init (pairlist.cc:11-23): the device took its snapshot during the
 // construction that precedes this call.
 void
 PairList::Init(Variables *vars, SimulationInfo *sinfo) {
@@ -15,7 +15,7 @@ PairList::Init(Variables *vars, SimulationInfo *sinfo) {
   lifetime = 0;
 }
 //----------------------------------------------------------------------
-// PairList::IsPairListExpired (pairlist.cc:25-45)
+// PairList::IsPairListExpired (pairlist.cc:26-45)
 bool
 PairList::IsPairListExpired(Variables *vars, SimulationInfo *sinfo) {
   if (!initialized) {

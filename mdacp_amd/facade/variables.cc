@@ -84,7 +84,7 @@ Variables::Check(int status, const char *what) {
   exit(1);
 }
 //----------------------------------------------------------------------
-// Variables::AddParticle (variables.cc:30-42): appended on the host; pushed to
+// Variables::AddParticle (variables.cc:31-42): appended on the host; pushed to
 // the device before the next device operation.
 void
 Variables::AddParticle(double x[D], double v[D], int t) {
@@ -204,7 +204,7 @@ Variables::ToDevice(void) {
   host_dirty = false;
 }
 //----------------------------------------------------------------------
-// Variables::SaveConfiguration (variables.cc:44-56): owned and ghost rows.
+// Variables::SaveConfiguration (variables.cc:45-56): owned and ghost rows.
 void
 Variables::SaveConfiguration(std::ostream &fs) {
   ToHost();
@@ -219,7 +219,7 @@ Variables::SaveConfiguration(std::ostream &fs) {
   }
 }
 //----------------------------------------------------------------------
-// Variables::AdjustPeriodicBoundary (variables.cc:58-79) on the mirrors; the
+// Variables::AdjustPeriodicBoundary (variables.cc:59-79) on the mirrors; the
 // device applies the same wrap at every pair-list construction.
 void
 Variables::AdjustPeriodicBoundary(SimulationInfo *sinfo) {
@@ -245,7 +245,7 @@ Variables::AdjustPeriodicBoundary(SimulationInfo *sinfo) {
   host_dirty = true;
 }
 //----------------------------------------------------------------------
-// Variables::SetInitialVelocity (variables.cc:81-108): |p| = V0 in a random
+// Variables::SetInitialVelocity (variables.cc:82-108): |p| = V0 in a random
 // direction from std::mt19937(seed) and uniform_real_distribution(0, 1),
 // then the mean removed -- the same library calls in the same order, so the
 // momenta are those of the reference for the same particle order.
@@ -280,7 +280,7 @@ Variables::SetInitialVelocity(const double V0, const int seed) {
   host_dirty = true;
 }
 //----------------------------------------------------------------------
-// Variables::SaveToStream / LoadFromStream (variables.cc:110-134): the same
+// Variables::SaveToStream / LoadFromStream (variables.cc:111-131): the same
 // binary layout (q and p interleaved per component, D components).
 void
 Variables::SaveToStream(std::ostream &fs) {
@@ -318,7 +318,7 @@ Variables::LoadFromStream(std::istream &fs) {
   total_particle_number = pn;
 }
 //----------------------------------------------------------------------
-// Variables::ChangeScale (variables.cc:136-143) on the mirrors.  MDUnit's
+// Variables::ChangeScale (variables.cc:134-140) on the mirrors.  MDUnit's
 // ChangeScale scales the device state itself.
 void
 Variables::ChangeScale(double alpha) {
@@ -331,7 +331,7 @@ Variables::ChangeScale(double alpha) {
   host_dirty = true;
 }
 //----------------------------------------------------------------------
-// Variables::Shuffle (variables.cc:145-175, debugging): a random permutation
+// Variables::Shuffle (variables.cc:145-172, debugging): a random permutation
 // of the owned particles.
 void
 Variables::Shuffle(void) {

@@ -1,7 +1,7 @@
 """Initial configurations on the host (run once per job, not on the step path).
 
 fcc()               ConfigurationMaker::Execute (confmaker.cc:8-44)
-initial_velocity()  Variables::SetInitialVelocity (variables.cc:195-222):
+initial_velocity()  Variables::SetInitialVelocity (variables.cc:82-108):
                     std::mt19937(seed) + uniform_real_distribution<double>(0,1),
                     |p| = V0 in a uniformly random direction, mean removed.
 """

@@ -235,7 +235,7 @@ class Domain:
         return ke.value, pe.value, vir.value
 
     def observables(self):
-        """T, P, E per MDManager::Temperature/Pressure/TotalEnergy (mdmanager.cc:389-429)."""
+        """T, P, E per MDManager::Temperature/Pressure/TotalEnergy (mdmanager.cc:390-429)."""
         ke, pe, vir = self.observe_raw()
         pn = float(self.counts()[0])
         T = (ke / pn) / 1.5

@@ -6,10 +6,10 @@
  * multiply-add is emitted: the reference's default build has none either.
  *
  * Reference files restated here (kaityo256/mdacp):
- *   simulationinfo.cc:5-51, variables.cc:16-28,58-108,133-140,
- *   mdrect.cc:4-60, meshlist.cc:30-87,152-271,287-310, pairlist.cc:10-121,
- *   fcalculator.cc:21-32,228-290,743-778, observer.cc:4-79,
- *   mdunit.cc:77-205, mdmanager.cc:115-192,250-348,368-443, confmaker.cc:8-44.
+ *   simulationinfo.cc:5-51, variables.cc:16-28,59-108,134-140,
+ *   mdrect.cc:5-60, meshlist.cc:31-87,153-271,288-310, pairlist.cc:11-121,
+ *   fcalculator.cc:22-32,229-290,744-778, observer.cc:5-79,
+ *   mdunit.cc:78-205, mdmanager.cc:116-192,251-348,369-443, confmaker.cc:8-44.
  * Third-party arithmetic restated from its published definition:
  *   std::mt19937 (Matsumoto & Nishimura 1998), libstdc++-11
  *   generate_canonical<double,53>, uniform_real_distribution and the Marsaglia
@@ -213,7 +213,7 @@ static void ensure_cap(ora_sys *s, int n) {
   s->cap = c;
 }
 
-/* MeshList::ChangeScale (meshlist.cc:95-120) */
+/* MeshList::ChangeScale (meshlist.cc:31-55) */
 static void mesh_change_scale(ora_sys *s) {
   double w[3];
   int ms[3], m[3];
@@ -238,11 +238,11 @@ ora_sys *ora_create(double Lx, double Ly, double Lz, double dt, double rc, doubl
   s->margin = margin;
   s->SL = rc + margin;          /* simulationinfo.cc:6-7 */
   s->CL2 = rc * rc;             /* fcalculator.cc:231 */
-  s->S2 = s->SL * s->SL;        /* meshlist.cc:312 */
+  s->S2 = s->SL * s->SL;        /* meshlist.cc:247 */
   s->T_aim = 1.0; s->gamma = 0.1; s->tau = 0.1;
-  s->check_list_length = 500;   /* simulationinfo.cc:343 */
+  s->check_list_length = 500;   /* simulationinfo.cc:29 */
   s->thermostat = ORA_NVE;
-  /* Variables::Variables (variables.cc:130-142) */
+  /* Variables::Variables (variables.cc:16-28) */
   const double s2 = 1.0 / (rc * rc);
   const double s6 = s2 * s2 * s2;
   const double s8 = s6 * s2;
@@ -250,13 +250,13 @@ ora_sys *ora_create(double Lx, double Ly, double Lz, double dt, double rc, doubl
   const double s14 = s12 * s2;
   s->C2 = 6.0 * s14 - 3.0 * s8;
   s->C0 = -s12 + s6 - s->C2 / s2;
-  /* single domain: grid 1x1x1, s = L*0, e = s + L (mdunit.cc:96-101) */
+  /* single domain: grid 1x1x1, s = L*0, e = s + L (mdunit.cc:17-24) */
   for (int d = 0; d < 3; d++) {
     s->rs[d] = s->L[d] * 0.0;
     s->re[d] = s->rs[d] + s->L[d];
   }
   mesh_change_scale(s);
-  mt_seed(&s->mt, 0u); /* mdunit.cc:104, id = 0 */
+  mt_seed(&s->mt, 0u); /* mdunit.cc:26, id = 0 */
   ensure_cap(s, 1024);
   return s;
 }
@@ -288,7 +288,7 @@ void ora_set_noise(ora_sys *s, int kind, uint64_t seed) {
 void ora_set_check_list_length(ora_sys *s, int n) { s->check_list_length = n; }
 
 /* MDUnit::AddParticle -> MDRect::IsInside -> Variables::AddParticle
-   (mdunit.cc:122-127, mdrect.h:319-322, variables.cc:144-156) */
+   (mdunit.cc:36-49, mdrect.h:35-38, variables.cc:31-42) */
 int ora_add_particles(ora_sys *s, int n, const double *x, const double *v, const int *type) {
   int added = 0;
   for (int i = 0; i < n; i++) {
@@ -310,7 +310,7 @@ int ora_add_particles(ora_sys *s, int n, const double *x, const double *v, const
   return added;
 }
 
-/* Variables::SetInitialVelocity (variables.cc:195-222) */
+/* Variables::SetInitialVelocity (variables.cc:82-108) */
 void ora_set_initial_velocity(ora_sys *s, double V0, int seed) {
   mt19937 g;
   mt_seed(&g, (uint32_t)seed);
@@ -335,7 +335,7 @@ void ora_set_initial_velocity(ora_sys *s, double V0, int seed) {
 void ora_set_seed(ora_sys *s, int seed) { mt_seed(&s->mt, (uint32_t)(seed + 0)); }
 
 /* ----------------------------------------------------- migration + wrap */
-/* MDRect::IsOverBoundary (mdrect.cc:4-21) */
+/* MDRect::IsOverBoundary (mdrect.cc:5-21) */
 static int over_boundary(const ora_sys *s, int dir, const double *q) {
   switch (dir) {
     case DLEFT: return q[0] < s->rs[0];
@@ -349,7 +349,7 @@ static int over_boundary(const ora_sys *s, int dir, const double *q) {
 }
 
 /* MDUnit::MakeBufferForSendingParticle + ReceiveParticles on the same unit
-   (mdunit.cc:155-200, mdmanager.cc:194-248): stable compaction of the staying
+   (mdunit.cc:78-122, mdmanager.cc:195-248): stable compaction of the staying
    particles, crossing ones appended in order. */
 static void send_particles_sub(ora_sys *s, int dir) {
   int nsend = 0;
@@ -385,7 +385,7 @@ static void send_particles_sub(ora_sys *s, int dir) {
   free(bq); free(bp); free(bt);
 }
 
-/* Variables::AdjustPeriodicBoundary (variables.cc:172-193) */
+/* Variables::AdjustPeriodicBoundary (variables.cc:59-79) */
 static int adjust_periodic(ora_sys *s) {
   for (int i = 0; i < s->pn; i++) {
     double *q = s->q + 4 * (size_t)i;
@@ -403,7 +403,7 @@ static int adjust_periodic(ora_sys *s) {
 }
 
 /* ---------------------------------------------------------------- ghosts */
-/* MDRect::IsInsideEdge (mdrect.cc:23-47) */
+/* MDRect::IsInsideEdge (mdrect.cc:24-47) */
 static int inside_edge(const ora_sys *s, int dir, const double *q) {
   const double SL = s->SL;
   switch (dir) {
@@ -417,7 +417,7 @@ static int inside_edge(const ora_sys *s, int dir, const double *q) {
   return 0;
 }
 
-/* image displacement of MDUnit::MakeBufferForBorderParticles (mdunit.cc:215-241):
+/* image displacement of MDUnit::MakeBufferForBorderParticles (mdunit.cc:142-161):
    a particle leaving through the left face reappears at +L, and so on. */
 static void dir_diff(const ora_sys *s, int dir, double diff[3]) {
   diff[0] = diff[1] = diff[2] = 0.0;
@@ -455,7 +455,7 @@ static void send_border_sub(ora_sys *s, int dir) {
   s->tn += nb;
 }
 
-/* MDUnit::FindBorderParticles (mdunit.cc:202-212) */
+/* MDUnit::FindBorderParticles (mdunit.cc:125-134) */
 static void find_border(ora_sys *s, int dir) {
   s->nborder[dir] = 0;
   for (int i = 0; i < s->tn; i++) {
@@ -469,7 +469,7 @@ static void find_border(ora_sys *s, int dir) {
   }
 }
 
-/* MDManager::SendBorderParticles (mdmanager.cc:282-292) */
+/* MDManager::SendBorderParticles (mdmanager.cc:283-292) */
 void ora_send_border_particles(ora_sys *s) {
   s->tn = s->pn;
   for (int dir = 0; dir < NDIR; dir++) send_border_sub(s, dir);
@@ -490,7 +490,7 @@ static void register_pair(ora_sys *s, int a, int b) {
   s->npairs++;
 }
 
-/* MeshList::MakeMesh (meshlist.cc:217-270): stable counting sort by cell */
+/* MeshList::MakeMesh (meshlist.cc:153-205): stable counting sort by cell */
 static int make_mesh(ora_sys *s) {
   const int pn = s->tn;
   const double imx = 1.0 / s->msize[0], imy = 1.0 / s->msize[1], imz = 1.0 / s->msize[2];
@@ -523,7 +523,7 @@ static int make_mesh(ora_sys *s) {
   return ORA_OK;
 }
 
-/* MeshList::SearchMesh (meshlist.cc:284-336): self cell plus the 13-cell
+/* MeshList::SearchMesh (meshlist.cc:220-271): self cell plus the 13-cell
    half shell, pairs (home particle, later particle), no ghost-ghost pairs */
 static void search_mesh(ora_sys *s, int index, int *v, int vcap) {
   (void)vcap;
@@ -558,7 +558,7 @@ static void search_mesh(ora_sys *s, int index, int *v, int vcap) {
   }
 }
 
-/* MeshList::MakeList (meshlist.cc:122-152): COO via the mesh, then CSR */
+/* MeshList::MakeList (meshlist.cc:117-150): COO via the mesh, then CSR */
 static int make_list(ora_sys *s) {
   s->npairs = 0;
   const int pn = s->tn;
@@ -587,7 +587,7 @@ static int make_list(ora_sys *s) {
   return ORA_OK;
 }
 
-/* PairList::Init (pairlist.cc:43-56) */
+/* PairList::Init (pairlist.cc:11-23) */
 static void pairlist_init(ora_sys *s) {
   s->initialized = 1;
   s->is_fresh = 1;
@@ -597,7 +597,7 @@ static void pairlist_init(ora_sys *s) {
     for (int d = 0; d < 3; d++) s->qb_old[4 * (size_t)i + d] = s->q[4 * (size_t)i + d];
 }
 
-/* MDManager::MakePairList (mdmanager.cc:261-280) for one unit */
+/* MDManager::MakePairList (mdmanager.cc:262-280) for one unit */
 int ora_make_pair_list(ora_sys *s) {
   for (int dir = 0; dir < NDIR; dir++) send_particles_sub(s, dir);
   int err = adjust_periodic(s);
@@ -623,7 +623,7 @@ int ora_make_pair_list(ora_sys *s) {
     }
     send_border_sub(s, dir);
   }
-  /* MDUnit::MakePairList (mdunit.cc:271-276); MeshList::Sort is off */
+  /* MDUnit::MakePairList (mdunit.cc:194-198); MeshList::Sort is off */
   pairlist_init(s);
   err = make_list(s);
   s->rebuilt = 1;
@@ -631,7 +631,7 @@ int ora_make_pair_list(ora_sys *s) {
 }
 
 /* ---------------------------------------------------------------- force */
-/* ForceCalculator::UpdatePositionHalf (fcalculator.cc:21-32) */
+/* ForceCalculator::UpdatePositionHalf (fcalculator.cc:22-32) */
 void ora_update_position_half(ora_sys *s) {
   const double dt2 = s->dt * 0.5;
   for (int i = 0; i < s->pn; i++) {
@@ -642,7 +642,7 @@ void ora_update_position_half(ora_sys *s) {
   }
 }
 
-/* ForceCalculator::CalculateForceNext (fcalculator.cc:228-290): half-list
+/* ForceCalculator::CalculateForceNext (fcalculator.cc:229-290): half-list
    Newton-3 kick.  The reference pipelines the next partner one iteration
    ahead; the arithmetic below is that loop's, pair for pair: each in-cutoff
    partner's p_j is decremented in row order, and p_i receives the running
@@ -696,7 +696,7 @@ void ora_kick_only(ora_sys *s, double *dp_out) {
   free(save);
 }
 
-/* ForceCalculator::Langevin (fcalculator.cc:764-778) */
+/* ForceCalculator::Langevin (fcalculator.cc:765-778) */
 static void langevin(ora_sys *s) {
   const double dt = s->dt;
   const double g = s->gamma;
@@ -721,7 +721,7 @@ static void langevin(ora_sys *s) {
   }
 }
 
-/* ForceCalculator::HeatbathZeta / HeatbathMomenta (fcalculator.cc:743-762) */
+/* ForceCalculator::HeatbathZeta / HeatbathMomenta (fcalculator.cc:744-762) */
 static void heatbath_zeta(ora_sys *s, double t) {
   const double dt2 = s->dt * 0.5;
   double t1 = (t - s->T_aim) / (s->tau * s->tau);
@@ -736,7 +736,7 @@ static void heatbath_momenta(ora_sys *s) {
 }
 
 /* ------------------------------------------------------------ observers */
-/* KineticEnergyObserver (observer.cc:4-15): four lanes incl. padding */
+/* KineticEnergyObserver (observer.cc:5-15): four lanes incl. padding */
 double ora_kinetic_energy_raw(ora_sys *s) {
   double e = 0;
   for (int i = 0; i < s->pn; i++)
@@ -744,7 +744,7 @@ double ora_kinetic_energy_raw(ora_sys *s) {
   return e;
 }
 
-/* PotentialEnergyObserver (observer.cc:17-48) over the COO order */
+/* PotentialEnergyObserver (observer.cc:18-48) over the COO order */
 double ora_potential_energy_raw(ora_sys *s) {
   const double CL2 = s->CL2, C2 = s->C2, C0 = s->C0;
   double energy = 0.0;
@@ -762,7 +762,7 @@ double ora_potential_energy_raw(ora_sys *s) {
   return energy;
 }
 
-/* VirialObserver (observer.cc:50-79) */
+/* VirialObserver (observer.cc:51-79) */
 double ora_virial_raw(ora_sys *s) {
   double phi = 0.0;
   const double C_2 = s->C2, CL2 = s->CL2;
@@ -781,7 +781,7 @@ double ora_virial_raw(ora_sys *s) {
   return phi / 3.0;
 }
 
-/* MDManager observables (mdmanager.cc:389-429); ObserveDouble sums one unit
+/* MDManager observables (mdmanager.cc:390-429); ObserveDouble sums one unit
    onto 0.0 (omp reduction) and through a one-rank MPI_Allreduce */
 static double observe_sum(double unit_value) { return 0.0 + unit_value; }
 
@@ -806,7 +806,7 @@ double ora_total_energy(ora_sys *s) {
 }
 
 /* -------------------------------------------------------------- expiry */
-/* PairList::UpdatePairListValidity (pairlist.cc:80-98) */
+/* PairList::UpdatePairListValidity (pairlist.cc:48-65) */
 static int update_validity(ora_sys *s) {
   const double *p = s->p;
   double vmax = p[0] * p[0] + p[1] * p[1] + p[2] * p[2];
@@ -821,7 +821,7 @@ static int update_validity(ora_sys *s) {
   return ORA_OK;
 }
 
-/* PairList::CheckByDisplacement (pairlist.cc:100-154) */
+/* PairList::CheckByDisplacement (pairlist.cc:68-121) */
 static int check_by_displacement(ora_sys *s) {
   const int pn = s->tn;
   double dr2_max = 0.0;
@@ -871,7 +871,7 @@ static int check_by_displacement(ora_sys *s) {
   return 0;
 }
 
-/* PairList::IsPairListExpired (pairlist.cc:58-78) */
+/* PairList::IsPairListExpired (pairlist.cc:26-45) */
 static int is_expired(ora_sys *s, int *expired) {
   if (!s->initialized) {
     *expired = 1;
@@ -889,7 +889,7 @@ static int is_expired(ora_sys *s, int *expired) {
 }
 
 /* ----------------------------------------------------------------- step */
-/* MDManager::Calculate (mdmanager.cc:115-148) with CalculateForce (:150-157),
+/* MDManager::Calculate (mdmanager.cc:116-148) with CalculateForce (:150-157),
    CalculateNoseHoover (:159-175) and CalculateLangevin (:185-192) */
 int ora_calculate(ora_sys *s) {
   int expired = 0, err;
@@ -925,9 +925,9 @@ int ora_calculate(ora_sys *s) {
 
 int ora_last_step_rebuilt(const ora_sys *s) { return s->rebuilt; }
 
-/* MDManager::ChangeScale (mdmanager.cc:431-443) with MDUnit::ChangeScale
-   (mdunit.cc:278-283), MDRect::ChangeScale (mdrect.cc:54-60) and
-   Variables::ChangeScale (variables.cc:247-254) */
+/* MDManager::ChangeScale (mdmanager.cc:432-443) with MDUnit::ChangeScale
+   (mdunit.cc:201-205), MDRect::ChangeScale (mdrect.cc:55-60) and
+   Variables::ChangeScale (variables.cc:134-140) */
 void ora_change_scale(ora_sys *s, double alpha) {
   ora_make_pair_list(s);
   for (int d = 0; d < 3; d++) s->L[d] = s->L[d] * alpha;

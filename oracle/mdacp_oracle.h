@@ -20,11 +20,11 @@ typedef struct ora_sys ora_sys;
 
 /* error codes returned by the stepping functions (same meaning as mdgpu.h) */
 #define ORA_OK 0
-#define ORA_ERR_TOO_FAST 1        /* pairlist.cc:92-96 */
-#define ORA_ERR_INVALID_POSITION 2 /* variables.cc:184-192 */
-#define ORA_ERR_INVALID_INDEX 3    /* meshlist.cc:244-249 */
+#define ORA_ERR_TOO_FAST 1        /* pairlist.cc:58-62 */
+#define ORA_ERR_INVALID_POSITION 2 /* variables.cc:70-78 */
+#define ORA_ERR_INVALID_INDEX 3    /* meshlist.cc:178-182 */
 
-/* thermostat selectors (mdconfig.h:27, simulationinfo.cc:332-340) */
+/* thermostat selectors (mdconfig.h:27, simulationinfo.cc:17-25) */
 #define ORA_NVE 0
 #define ORA_NOSEHOOVER 1
 #define ORA_LANGEVIN 2

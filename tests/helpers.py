@@ -69,7 +69,7 @@ def lj_C(rc):
 
 
 def wrap_like_reference(q, L):
-    """Variables::AdjustPeriodicBoundary (variables.cc:172-182): one +-L per component."""
+    """Variables::AdjustPeriodicBoundary (variables.cc:59-79): one +-L per component."""
     L = np.broadcast_to(np.asarray(L, dtype=np.float64), q.shape)
     return np.where(q < 0.0, q + L, np.where(q > L, q - L, q))
 
