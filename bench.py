@@ -12,6 +12,12 @@ Reported: particle-updates/s of the FP64 path (`value`) and of the mixed FP32
 path, the energy drift of both over --drift-steps, the pair-force kernel's
 roofline position (HIP events on the domain's stream) and the oracle timed on
 the host as the CPU baseline.
+
+--workload ensemble: the batched-replica workload instead -- 256 independent
+Langevin replicas (N = 4 k^3 drawn log-uniform in [2048, 65536], density in
+[0.3, 0.9], T in [0.7, 1.5], gamma = 1, own key each), rc = 2.5, margin 0.5,
+dt = 0.005, all advanced by one pair-force launch per step replayed from a
+captured graph; particle-updates/s summed over the replicas, FP64 and mixed.
 """
 import argparse
 import json
@@ -115,8 +121,125 @@ def cpu_baseline(x, p, L, args, gpu_rebuilds_per_step):
                         f"rebuild rate ({gpu_rebuilds_per_step:.4f}/step)"))
 
 
+def run_ensemble_gpu(precision, table, built, args):
+    from mdacp_amd import mdgpu
+
+    boxes, counts, q, p, ids = built
+    e = mdgpu.Ensemble(boxes, counts, q, p, ids, T=[r["T"] for r in table], gamma=1.0,
+                       seed=[r["seed"] for r in table], thermostat=mdgpu.LANGEVIN, dt=args.dt, rc=args.rc,
+                       margin=args.margin, precision=precision)
+    e.calculate(args.warmup)
+    e.observables()
+    n = int(counts.sum())
+    rebuilds = 0
+    t0 = time.perf_counter()
+    done = 0
+    while done < args.steps:
+        k = min(100, args.steps - done)
+        rebuilds += e.calculate(k)
+        done += k
+        e.observables()
+    elapsed = time.perf_counter() - t0
+    # the same number of steps again with HIP events around each pair-force launch
+    e.enable_timing(True)
+    e.calculate(args.steps)
+    tm = e.timing()
+    e.enable_timing(False)
+    res = dict(pu_s=args.steps * n / elapsed, ms_per_step=1e3 * elapsed / args.steps, rebuilds=rebuilds,
+               force_avg_ms=tm["force_ms"] / max(1, tm["force_launches"]),
+               rebuild_avg_ms=tm["rebuild_ms"] / max(1, tm["rebuilds"]), list=e.list_stats(),
+               mean_T_over_target=float(np.mean(e.observables()["T"] / np.array([r["T"] for r in table]))))
+    e.close()
+    return res
+
+
+def cpu_baseline_ensemble(table, built, args, sample=16, steps=3):
+    """The oracle on the first `sample` replicas, one after another, `steps` Langevin steps each plus one
+    pair-list build amortised at the GPU run's rebuild rate."""
+    from oracle import oracle as O
+
+    boxes, counts, q, p, ids = built
+    offs = np.concatenate([[0], np.cumsum(counts)])
+    t_steps = t_build = 0.0
+    n = 0
+    for r in range(sample):
+        s = O.OracleSystem(float(boxes[r, 0]), args.dt, args.rc, args.margin)
+        a, b = offs[r], offs[r + 1]
+        s.add(q[a:b], p[a:b], ids[a:b])
+        s.set_thermostat(O.LANGEVIN, table[r]["T"], 1.0, 0.1)
+        s.set_noise(O.NOISE_PHILOX, table[r]["seed"])
+        t0 = time.perf_counter()
+        s.make_pair_list()
+        t_build += time.perf_counter() - t0
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            s.step()
+        t_steps += time.perf_counter() - t0
+        n += int(b - a)
+    return n, t_steps / steps, t_build
+
+
+def main_ensemble(args):
+    from mdacp_amd import mdgpu
+    from mdacp_amd.ensemble import build, replica_table
+
+    table = replica_table(nrep=args.replicas)
+    built = build(table)
+    n = int(built[1].sum())
+    print(f"# ensemble: {len(table)} replicas, {n} particles", file=sys.stderr, flush=True)
+    r64 = run_ensemble_gpu(mdgpu.FP64, table, built, args)
+    print(f"# fp64 {r64}", file=sys.stderr, flush=True)
+    r32 = run_ensemble_gpu(mdgpu.MIXED, table, built, args)
+    print(f"# mixed {r32}", file=sys.stderr, flush=True)
+    cpu = None
+    if not args.no_cpu_baseline:
+        ns, per_step, t_build = cpu_baseline_ensemble(table, built, args)
+        per = per_step + t_build * r64["rebuilds"] / args.steps
+        cpu = dict(value=ns / per, unit="particle-updates/s", cores=1, kind="port",
+                   sample=(f"oracle (C restatement of the reference step, 1 thread), the first 16 replicas "
+                           f"({ns} particles) one after another: 3 Langevin steps each ({per_step:.3f} s per "
+                           f"step of the 16) plus one pair-list build of each ({t_build:.3f} s) amortised at the "
+                           f"GPU run's rebuild rate"))
+    rho_mean = float(np.mean([r["rho"] for r in table]))
+    bytes_pu, _, _, _ = algorithmic_per_update(rho_mean, args.rc, args.margin)
+    avg_s = r64["force_avg_ms"] * 1e-3
+    achieved = bytes_pu * n / avg_s / 1e9
+    line = {
+        "metric": "particle-updates/s, batched Langevin replicas (256, N 2048-62500, rho 0.3-0.9)",
+        "value": r64["pu_s"],
+        "unit": "particle-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": r64["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic: per-replica FCC lattices (confmaker.cc:8-44), |v0|=sqrt(3T) (variables.cc:82-108)",
+        "config": {"workload": (f"{len(table)} Langevin replicas, {n} particles, rc={args.rc} margin={args.margin} "
+                                f"dt={args.dt} gamma=1, one pair-force launch per step (graph replay)"),
+                   "particles": n, "replicas": len(table), "rebuilds_in_timed_steps": r64["rebuilds"]},
+        "roofline": dict(bound="hbm", achieved=achieved, peak=HBM_PEAK_GBS, unit="GB/s",
+                         frac=achieved / HBM_PEAK_GBS, traffic=None, kernel="k_force_brick_fp64",
+                         kernel_avg_ms=r64["force_avg_ms"], bytes_per_particle_update=bytes_pu,
+                         note="algorithmic bytes at the replicas' mean density; kernel time from HIP events over "
+                              "a second pass of the same number of steps"),
+        "cpu_baseline": cpu,
+        "mixed": {"value": r32["pu_s"], "ms_per_step": r32["ms_per_step"],
+                  "speedup_vs_fp64": r32["pu_s"] / r64["pu_s"], "force_kernel_avg_ms": r32["force_avg_ms"],
+                  "rebuilds_in_timed_steps": r32["rebuilds"]},
+        "rebuild_avg_ms": r64["rebuild_avg_ms"],
+        "mean_T_over_target": r64["mean_T_over_target"],
+        "pair_list": r64["list"],
+    }
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
+    ap.add_argument("--workload", choices=["single", "ensemble"], default="single")
+    ap.add_argument("--replicas", type=int, default=256)
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2000)
     ap.add_argument("--warmup", type=int, default=200)
@@ -125,7 +248,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--fcc-cells", type=int, default=100, help="N = 4 k^3")
     ap.add_argument("--density", type=float, default=0.5)
-    ap.add_argument("--dt", type=float, default=0.001)
+    ap.add_argument("--dt", type=float, default=None, help="0.001 (single), 0.005 (ensemble)")
     ap.add_argument("--rc", type=float, default=2.5)
     ap.add_argument("--margin", type=float, default=0.5)
     args = ap.parse_args()
@@ -134,6 +257,15 @@ def main():
               f"(several GPUs would each own one box of a spatial domain decomposition, exchanging "
               f"ghost images every step, which is not built)", file=sys.stderr)
         sys.exit(2)
+    if args.workload == "ensemble":
+        if args.dt is None:
+            args.dt = 0.005
+        if args.steps == 2000:
+            args.steps = 1000
+        main_ensemble(args)
+        return
+    if args.dt is None:
+        args.dt = 0.001
 
     from mdacp_amd import mdgpu
     from mdacp_amd.lattice import fcc, initial_velocity

@@ -36,6 +36,14 @@
  *   mdgpu_change_scale      MDManager::ChangeScale (mdmanager.cc:432-443)
  *   mdgpu_get_pairs         MeshList::GetKeyParticles/GetPartnerParticles
  *                           (meshlist.h:46-48), as canonical pairs
+ *
+ * Ensembles (no counterpart in the reference, which runs one system per
+ * process): mdgpu_create_ensemble makes one domain that advances many
+ * independent replicas -- each its own periodic box, particle count,
+ * temperature and Langevin seed, every one with the single-domain semantics
+ * above -- with one pair-force launch per step for all of them.  Each replica
+ * keeps its own PairList budget and displacement check (pairlist.cc:26-121);
+ * when the list of any replica expires, the lists of all replicas are rebuilt.
  */
 #ifndef MDGPU_H
 #define MDGPU_H
@@ -77,8 +85,30 @@ typedef struct {
   int device;             /* HIP device ordinal */
 } mdgpu_params;
 
+/* one replica of an ensemble: box, Langevin temperature, friction and key */
+typedef struct {
+  double L[3];
+  double T;
+  double gamma;
+  uint64_t seed;
+} mdgpu_replica;
+
 /* lifetime */
 int mdgpu_create(const mdgpu_params *prm, mdgpu_domain **out);
+/* An ensemble of nrep replicas sharing prm's dt, rc, margin, check list length,
+   precision and device (prm->L, prm->capacity unused); thermostat MDGPU_NVE or
+   MDGPU_LANGEVIN.  Replica r owns counts[r] particles, given replica after
+   replica in q, p (n*3 doubles; p may be NULL) and type (NULL: 0..counts[r]-1),
+   each inside [0, L) of its replica.  The pair lists are built before return. */
+int mdgpu_create_ensemble(const mdgpu_params *prm, int thermostat, int nrep, const mdgpu_replica *reps,
+                          const int *counts, const double *q, const double *p, const int *type,
+                          mdgpu_domain **out);
+/* replicas of a domain (1 for mdgpu_create) and owned particles of each;
+   mdgpu_get_particles returns them replica after replica */
+int mdgpu_replica_number(const mdgpu_domain *d, int *nrep);
+int mdgpu_replica_counts(const mdgpu_domain *d, int *counts);
+/* Langevin temperature, friction and key of one replica */
+int mdgpu_set_replica_thermostat(mdgpu_domain *d, int replica, double T, double gamma, uint64_t seed);
 int mdgpu_destroy(mdgpu_domain *d);
 const char *mdgpu_last_error(const mdgpu_domain *d);
 int mdgpu_set_precision(mdgpu_domain *d, int precision);
@@ -123,6 +153,8 @@ int mdgpu_calculate(mdgpu_domain *d, int nsteps, int *rebuilds);
 int mdgpu_observe(mdgpu_domain *d, double *ke, double *pe, double *virial);
 /* KineticEnergyObserver alone (observer.cc:5-15); needs no pair list */
 int mdgpu_observe_kinetic(mdgpu_domain *d, double *ke);
+/* the three observers of every replica (arrays of mdgpu_replica_number entries) */
+int mdgpu_observe_replicas(mdgpu_domain *d, double *ke, double *pe, double *virial);
 
 /* MDManager::ChangeScale: rebuild, scale box and owned positions by alpha, rebuild */
 int mdgpu_change_scale(mdgpu_domain *d, double alpha);
@@ -131,6 +163,8 @@ int mdgpu_change_scale(mdgpu_domain *d, double alpha);
    shift code (sx+1)+3(sy+1)+9(sz+1)); owned-owned pairs as (min id, max id, 13).
    Call with out3 == NULL to get the count. */
 int mdgpu_get_pairs(mdgpu_domain *d, int64_t *npairs, int32_t *out3);
+/* the same for the rows of one replica (ids are those given for that replica) */
+int mdgpu_get_replica_pairs(mdgpu_domain *d, int replica, int64_t *npairs, int32_t *out3);
 /* the same pairs by index, as MeshList::GetKeyParticles/GetPartnerParticles
    hold them (meshlist.h:46-48): key = owned index, partner = owned index
    (key < partner) or pn + ghost index, in mdgpu_get_particles / mdgpu_get_ghosts
@@ -154,7 +188,10 @@ int mdgpu_get_timing(const mdgpu_domain *d, int *force_launches, double *force_m
 #define MDGPU_OPT_WAVES 3    /* wavefronts per cell-brick workgroup: 2, 4, 8 or 16 */
 #define MDGPU_OPT_LDS_KB 4   /* LDS budget for one brick's staged partners, KiB (>= 8) */
 #define MDGPU_OPT_BRICK 5    /* cells per brick: bx | by << 8 | bz << 16, each 1..4
-                                (the next step rebuilds the list) */
+                                (the next step rebuilds the list); an ensemble
+                                picks one per replica from its density */
+#define MDGPU_OPT_GRAPH 6    /* 1 = fused steps replayed from a captured HIP graph
+                                (default; not while timing is enabled), 0 = launched */
 int mdgpu_set_option(mdgpu_domain *d, int option, int value);
 int mdgpu_get_option(const mdgpu_domain *d, int option, int *value);
 

@@ -1,7 +1,8 @@
 // build_kernels.hpp -- pair-list construction on gfx950: periodic wrap,
 // counting-sort cell binning (atomics + prefix scan, then each cell's members
 // put back in input order, so the result does not depend on atomic ordering)
-// and the ghost images of the single-domain halo.
+// and the ghost images of the single-domain halo, for every replica of a
+// domain at once (each particle carries its replica index).
 //
 // Reference semantics (kaityo256/mdacp): AdjustPeriodicBoundary
 // (variables.cc:59-79), MakeMesh (meshlist.cc:153-205), FindBorderParticles
@@ -14,31 +15,30 @@
 
 namespace mdgpu {
 
-struct Grid {
-  double s[3];      // domain start (MDRect::s)
-  double im[3];     // 1 / mesh size
-  int m[3];         // cells per axis incl. one ghost layer per side
-};
-
-__device__ __forceinline__ int cell_of(double x, double y, double z, const Grid &g) {
-  int ix = (int)((x - g.s[0]) * g.im[0]) + 1;
-  int iy = (int)((y - g.s[1]) * g.im[1]) + 1;
-  int iz = (int)((z - g.s[2]) * g.im[2]) + 1;
-  if (ix < 0) ix = g.m[0] - 1; else if (ix >= g.m[0]) ix = 0;
-  if (iy < 0) iy = g.m[1] - 1; else if (iy >= g.m[1]) iy = 0;
-  if (iz < 0) iz = g.m[2] - 1; else if (iz >= g.m[2]) iz = 0;
-  return g.m[0] * g.m[1] * iz + g.m[0] * iy + ix;
+// MeshList's cell of a position (meshlist.cc:164-182): int((q - s)/w) + 1 per
+// axis with s = 0, as a global cell index.
+__device__ __forceinline__ int cell_of(double x, double y, double z, const Replica &R) {
+  int ix = (int)(x * R.im[0]) + 1;
+  int iy = (int)(y * R.im[1]) + 1;
+  int iz = (int)(z * R.im[2]) + 1;
+  if (ix < 0) ix = R.m[0] - 1; else if (ix >= R.m[0]) ix = 0;
+  if (iy < 0) iy = R.m[1] - 1; else if (iy >= R.m[1]) iy = 0;
+  if (iz < 0) iz = R.m[2] - 1; else if (iz >= R.m[2]) iz = 0;
+  return R.cell0 + R.m[0] * R.m[1] * iz + R.m[0] * iy + ix;
 }
 
-// Wrap owned particles into the box (one +-L per component), flag positions
-// still outside, bin, and take a slot in the cell by atomic increment.
-__global__ void k_wrap_bin(double4 *__restrict__ pos, int n, double Lx, double Ly, double Lz,
-                           Grid g, int *__restrict__ cell, int *__restrict__ slot,
+// Wrap owned particles into their replica's box (one +-L per component), flag
+// positions still outside, bin, and take a slot in the cell by atomic
+// increment.
+__global__ void k_wrap_bin(double4 *__restrict__ pos, int n, const int *__restrict__ rep,
+                           const Replica *__restrict__ reps, int *__restrict__ cell, int *__restrict__ slot,
                            int *__restrict__ ccount, int *__restrict__ err, int wrap) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
+  const Replica &R = reps[rep[i]];
   double4 q = pos[i];
   if (wrap) {
+    const double Lx = R.L[0], Ly = R.L[1], Lz = R.L[2];
     if (q.x < 0.0) q.x += Lx; else if (q.x > Lx) q.x -= Lx;
     if (q.y < 0.0) q.y += Ly; else if (q.y > Ly) q.y -= Ly;
     if (q.z < 0.0) q.z += Lz; else if (q.z > Lz) q.z -= Lz;
@@ -47,7 +47,7 @@ __global__ void k_wrap_bin(double4 *__restrict__ pos, int n, double Lx, double L
     if (!(q.x >= 0.0 && q.x <= Lx && q.y >= 0.0 && q.y <= Ly && q.z >= 0.0 && q.z <= Lz))
       atomicOr(err, kErrInvalidPosition);
   }
-  const int c = cell_of(q.x, q.y, q.z, g);
+  const int c = cell_of(q.x, q.y, q.z, R);
   cell[i] = c;
   slot[i] = atomicAdd(&ccount[c], 1);
 }
@@ -59,20 +59,28 @@ __global__ void k_scatter_perm(const int *__restrict__ cell, const int *__restri
   perm[cstart[cell[i]] + slot[i]] = i;
 }
 
-// Order each cell's members by their index in the input order (insertion
-// sort of the cell's contiguous segment of perm; cells hold tens of
-// particles).  With the slots of k_wrap_bin taken by atomics, this makes the
-// order deterministic: a stable counting sort.
+// Order each cell's members (insertion sort of the cell's contiguous segment
+// of perm; cells hold tens of particles) by (key, input index) when `key` is
+// given, else by input index.  With the slots of k_wrap_bin taken by atomics
+// this makes the order deterministic; owned particles are keyed by particle
+// id, so the order after a construction depends only on the particles and
+// their ids, not on the order an earlier construction left.
+__device__ __forceinline__ unsigned long long sort_key(const int *__restrict__ key, int i) {
+  const unsigned long long k = key ? (unsigned long long)((unsigned)key[i] ^ 0x80000000u) : 0ull;
+  return (k << 32) | (unsigned)i;
+}
+
 __global__ void k_sort_cells(int *__restrict__ perm, const int *__restrict__ cstart,
-                             const int *__restrict__ ccount, int ncell) {
+                             const int *__restrict__ ccount, int ncell, const int *__restrict__ key) {
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= ncell) return;
   int *a = perm + cstart[c];
   const int n = ccount[c];
   for (int t = 1; t < n; t++) {
     const int v = a[t];
+    const unsigned long long kv = sort_key(key, v);
     int u = t - 1;
-    while (u >= 0 && a[u] > v) {
+    while (u >= 0 && sort_key(key, a[u]) > kv) {
       a[u + 1] = a[u];
       u--;
     }
@@ -82,9 +90,9 @@ __global__ void k_sort_cells(int *__restrict__ perm, const int *__restrict__ cst
 
 __global__ void k_gather_owned(const int *__restrict__ perm, int n, const double4 *__restrict__ pos_in,
                                const double4 *__restrict__ mom_in, const int *__restrict__ id_in,
-                               const int *__restrict__ cell_in, double4 *__restrict__ pos_out,
-                               double4 *__restrict__ mom_out, int *__restrict__ id_out,
-                               int *__restrict__ cell_out) {
+                               const int *__restrict__ cell_in, const int *__restrict__ rep_in,
+                               double4 *__restrict__ pos_out, double4 *__restrict__ mom_out,
+                               int *__restrict__ id_out, int *__restrict__ cell_out, int *__restrict__ rep_out) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= n) return;
   const int i = perm[k];
@@ -92,6 +100,7 @@ __global__ void k_gather_owned(const int *__restrict__ perm, int n, const double
   mom_out[k] = mom_in[i];
   id_out[k] = id_in[i];
   cell_out[k] = cell_in[i];
+  rep_out[k] = rep_in[i];
 }
 
 // Halo of one domain (MDManager::MakePairList's six sequential
@@ -99,39 +108,38 @@ __global__ void k_gather_owned(const int *__restrict__ perm, int n, const double
 // along each axis a particle strictly within SL of the low face gets an image
 // at +L, strictly within SL of the high face an image at -L; images of images
 // make the edges and corners.  Equivalently, one image per non-zero shift
-// vector whose every non-zero component passes its face test.
-struct Edges {
-  double lo_hi[3];   // s + SL
-  double hi_lo[3];   // e - SL
-  double s[3], e[3];
-};
-
-__device__ __forceinline__ void face_options(const double4 q, const Edges &E, int lo[3], int hi[3]) {
+// vector whose every non-zero component passes its face test.  The box is
+// [s, e) = [0, L) (MDRect::IsInsideEdge, mdrect.cc:24-47: s < q < s + SL and
+// e - SL < q < e).
+__device__ __forceinline__ void face_options(const double4 q, const Replica &R, double SL, int lo[3], int hi[3]) {
   const double v[3] = {q.x, q.y, q.z};
   for (int d = 0; d < 3; d++) {
-    lo[d] = (v[d] < E.lo_hi[d] && v[d] > E.s[d]);
-    hi[d] = (v[d] > E.hi_lo[d] && v[d] < E.e[d]);
+    lo[d] = (v[d] < 0.0 + SL && v[d] > 0.0);
+    hi[d] = (v[d] > R.L[d] - SL && v[d] < R.L[d]);
   }
 }
 
-__global__ void k_count_images(const double4 *__restrict__ pos, int n, Edges E, int *__restrict__ nimg) {
+__global__ void k_count_images(const double4 *__restrict__ pos, int n, const int *__restrict__ rep,
+                               const Replica *__restrict__ reps, double SL, int *__restrict__ nimg) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   int lo[3], hi[3];
-  face_options(pos[i], E, lo, hi);
+  face_options(pos[i], reps[rep[i]], SL, lo, hi);
   nimg[i] = (1 + lo[0] + hi[0]) * (1 + lo[1] + hi[1]) * (1 + lo[2] + hi[2]) - 1;
 }
 
-__global__ void k_emit_images(const double4 *__restrict__ pos, int n, Edges E, double Lx, double Ly,
-                              double Lz, Grid g, const int *__restrict__ goff, int ng,
+__global__ void k_emit_images(const double4 *__restrict__ pos, int n, const int *__restrict__ rep,
+                              const Replica *__restrict__ reps, double SL, const int *__restrict__ goff, int ng,
                               int *__restrict__ gsrc, int *__restrict__ gcode,
                               double4 *__restrict__ gpos, int *__restrict__ gcell,
                               int *__restrict__ gslot, int *__restrict__ gcount) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
+  const int r = rep[i];
+  const Replica &R = reps[r];
   const double4 q = pos[i];
   int lo[3], hi[3];
-  face_options(q, E, lo, hi);
+  face_options(q, R, SL, lo, hi);
   int k = goff[i];
   const int sxs[3] = {0, 1, -1};
   for (int a = 0; a < 3; a++) {
@@ -146,11 +154,11 @@ __global__ void k_emit_images(const double4 *__restrict__ pos, int n, Edges E, d
         if (!sx && !sy && !sz) continue;
         if (k >= ng) return;  // cannot happen: goff is the scan of k_count_images
         const int code = (sx + 1) + 3 * (sy + 1) + 9 * (sz + 1);
-        const double3 y = image_of(q, code, Lx, Ly, Lz);
+        const double3 y = image_of(q, code, R.L);
         gsrc[k] = i;
-        gcode[k] = code;
+        gcode[k] = ghost_word(code, r);
         gpos[k] = make_double4(y.x, y.y, y.z, 0.0);
-        const int cl = cell_of(y.x, y.y, y.z, g);
+        const int cl = cell_of(y.x, y.y, y.z, R);
         gcell[k] = cl;
         gslot[k] = atomicAdd(&gcount[cl], 1);
         k++;

@@ -42,10 +42,10 @@ __device__ __forceinline__ double wave_sum(double v) {
   return v;
 }
 
-// Maximum over the workgroup, written to partial[blockIdx.x] by thread 0
-// (no same-address atomics: those serialise in L2 across ~10^4 workgroups).
+// Maximum over the workgroup, written to partial[slot] by thread 0 (no
+// same-address atomics: those serialise in L2 across ~10^4 workgroups).
 // Every thread of the workgroup must call it.
-__device__ __forceinline__ void block_max_to_partial(double v, double *__restrict__ partial) {
+__device__ __forceinline__ void block_max_to_partial(double v, double *__restrict__ partial, int slot) {
   __shared__ double wm[16];  // up to 1024 threads
   v = wave_max(v);
   if ((threadIdx.x % kWave) == 0) wm[threadIdx.x / kWave] = v;
@@ -53,7 +53,7 @@ __device__ __forceinline__ void block_max_to_partial(double v, double *__restric
   if (threadIdx.x == 0) {
     double m = wm[0];
     for (int k = 1; k < (int)(blockDim.x / kWave); k++) m = fmax(m, wm[k]);
-    partial[blockIdx.x] = m;
+    partial[slot] = m;
   }
 }
 
@@ -71,13 +71,44 @@ __global__ void k_reduce_max(const double *__restrict__ partial, int n, unsigned
   if (threadIdx.x == 0) *out = (unsigned long long)__double_as_longlong(red[0]);
 }
 
+// One replica's periodic box and cell grid.  A domain advances one or more
+// independent replicas (an ensemble); their particles, cells and bricks are
+// concatenated in the domain's arrays, each replica's contiguous, and every
+// kernel that needs geometry finds its replica here.  A single-domain run has
+// one replica.  The box of a replica is [0, L) (MDRect of a 1x1x1 grid,
+// mdunit.cc:17-24); its extended cell grid is MeshList's (meshlist.cc:31-55).
+struct Replica {
+  double L[3];     // box edges
+  double im[3];    // 1 / cell width
+  double w[3];     // cell width
+  double hb_D;     // Langevin noise amplitude sqrt(2 gamma T / dt) (fcalculator.cc:766)
+  double gamma;    // Langevin friction
+  uint64_t seed;   // Langevin key
+  int m[3];        // cells per axis, one ghost layer per side included
+  int nb[3];       // bricks per axis
+  int bd[3];       // cells per brick along each axis
+  int cell0;       // first global cell
+  int brick0;      // first global brick
+  int col0;        // first column origin (x columns, then y, then z) in the origin table
+  int p0, pn;      // owned particles [p0, p0 + pn) once binned
+};
+
+__device__ __forceinline__ int replica_bricks(const Replica &R) { return R.nb[0] * R.nb[1] * R.nb[2]; }
+
+// A ghost's code word: image shift code (sx+1)+3(sy+1)+9(sz+1) in the low
+// five bits, the replica above them.
+constexpr int kShiftBits = 5;
+__host__ __device__ __forceinline__ int ghost_word(int code, int rep) { return code | (rep << kShiftBits); }
+__host__ __device__ __forceinline__ int ghost_shift(int word) { return word & ((1 << kShiftBits) - 1); }
+__host__ __device__ __forceinline__ int ghost_rep(int word) { return word >> kShiftBits; }
+
 // Image displacement of a shift code (sx+1)+3(sy+1)+9(sz+1): one +-L or 0.0
 // per component, added as the reference's halo adds it (mdunit.cc:142-161).
-__device__ __forceinline__ double3 image_of(double4 q, int code, double Lx, double Ly, double Lz) {
+__device__ __forceinline__ double3 image_of(double4 q, int code, const double *L) {
   const int sx = code % 3 - 1, sy = (code / 3) % 3 - 1, sz = code / 9 - 1;
-  const double dx = sx > 0 ? Lx : (sx < 0 ? -Lx : 0.0);
-  const double dy = sy > 0 ? Ly : (sy < 0 ? -Ly : 0.0);
-  const double dz = sz > 0 ? Lz : (sz < 0 ? -Lz : 0.0);
+  const double dx = sx > 0 ? L[0] : (sx < 0 ? -L[0] : 0.0);
+  const double dy = sy > 0 ? L[1] : (sy < 0 ? -L[1] : 0.0);
+  const double dz = sz > 0 ? L[2] : (sz < 0 ? -L[2] : 0.0);
   return make_double3(q.x + dx, q.y + dy, q.z + dz);
 }
 

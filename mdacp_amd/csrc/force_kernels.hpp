@@ -41,8 +41,21 @@ struct BrickList {
   const int *glist;         // staged entry -> particle index (owned, or pn + ghost)
   const long long *gbase;   // [brick + 1] first gather-list entry of each brick
   const int *row0, *nrow, *vslice0;
+  const int *brep;          // replica of each brick
+  const Replica *reps;
   int lds_cap;              // staged entries that fit the LDS budget (incl. the pad entry)
 };
+
+// Time step and Langevin step counter of a pair-force launch: the counter is
+// read from device memory when `clock` is set (a launch replayed from a graph
+// reads the current step), else `step` is used.
+struct StepClock {
+  double dt;
+  uint64_t step;
+  const unsigned long long *clock;
+};
+
+__device__ __forceinline__ uint64_t step_of(const StepClock &sc) { return sc.clock ? sc.clock[0] : sc.step; }
 
 // 1/x to double precision: hardware reciprocal estimate + two Newton steps.
 __device__ __forceinline__ double rcp64(double x) {
@@ -168,13 +181,13 @@ struct NextDrift {
   double4 *pos;
   float4 *rel;
   const int *cell;
-  Origins O;
+  const double *org;  // column origins of the cell-relative coordinates
 };
 
 __device__ __forceinline__ double finish_row(int i, double4 p, const double4 qi, double4 *__restrict__ mom,
                                              double4 *__restrict__ pos_out, int epilogue, double dt2,
                                              const LangevinConsts &lc, const int *__restrict__ id,
-                                             const NextDrift &nd) {
+                                             const NextDrift &nd, const Replica &R) {
   if (epilogue == kKickLangevin) langevin_update(p, lc, (uint32_t)id[i]);
   mom[i] = p;
   double4 q = qi;
@@ -191,7 +204,7 @@ __device__ __forceinline__ double finish_row(int i, double4 p, const double4 qi,
     q.y += p.y * dt2;
     q.z += p.z * dt2;
     nd.pos[i] = q;
-    if (nd.rel != nullptr) nd.rel[i] = to_rel(q, nd.cell[i], nd.O.x, nd.O.y, nd.O.z, nd.O.mx, nd.O.my);
+    if (nd.rel != nullptr) nd.rel[i] = to_rel(q, nd.cell[i], nd.org, R);
   }
   return p.x * p.x + p.y * p.y + p.z * p.z;
 }
@@ -201,12 +214,14 @@ template <int W, int U>
 __global__ __launch_bounds__(kWave *W) void k_force_brick_fp64(const double4 *__restrict__ src,
                                                               double4 *__restrict__ mom,
                                                               double4 *__restrict__ pos_out, BrickList bl,
-                                                              ForceConsts fc, int epilogue, LangevinConsts lc,
+                                                              ForceConsts fc, int epilogue, StepClock sc,
                                                               const int *__restrict__ id,
                                                               double *__restrict__ vpart, NextDrift nd) {
   extern __shared__ double lds64[];
   constexpr int T = kWave * W;
   const int b = xcd_block(blockIdx.x, gridDim.x);
+  const Replica &Rp = bl.reps[bl.brep[b]];
+  const LangevinConsts lc = langevin_consts(Rp, sc.dt, step_of(sc));
   const long long g0 = bl.gbase[b];
   const int total = (int)(bl.gbase[b + 1] - g0);
   const int *gl = bl.glist + g0;
@@ -232,10 +247,10 @@ __global__ __launch_bounds__(kWave *W) void k_force_brick_fp64(const double4 *__
       p.x += a.x;
       p.y += a.y;
       p.z += a.z;
-      vmax = fmax(vmax, finish_row(i, p, qi, mom, pos_out, epilogue, fc.dt2, lc, id, nd));
+      vmax = fmax(vmax, finish_row(i, p, qi, mom, pos_out, epilogue, fc.dt2, lc, id, nd, Rp));
     }
   }
-  if (vpart) block_max_to_partial(vmax, vpart);
+  if (vpart) block_max_to_partial(vmax, vpart, b);
 }
 
 // Mixed precision kick: FP32 cell-relative coordinates (staged as float4),
@@ -247,17 +262,20 @@ __global__ __launch_bounds__(kWave *W) void k_force_brick_fp64(const double4 *__
 template <int W, int U, bool kKahan>
 __global__ __launch_bounds__(kWave *W) void k_force_brick_mixed(
     const float4 *__restrict__ rel, const double4 *__restrict__ pos, double4 *__restrict__ mom,
-    double4 *__restrict__ pos_out, BrickList bl, ForceConstsF fc, double dt2, int epilogue, LangevinConsts lc,
+    double4 *__restrict__ pos_out, BrickList bl, ForceConstsF fc, double dt2, int epilogue, StepClock sc,
     const int *__restrict__ id, double *__restrict__ vpart, NextDrift nd) {
   extern __shared__ float4 lds32[];
   __shared__ float4 offs[64];
   constexpr int T = kWave * W;
+  const int b = xcd_block(blockIdx.x, gridDim.x);
+  const Replica &Rp = bl.reps[bl.brep[b]];
+  const LangevinConsts lc = langevin_consts(Rp, sc.dt, step_of(sc));
   if (threadIdx.x < 64) {
     const int c = threadIdx.x;
     const int ox = (c & 3) - 1, oy = ((c >> 2) & 3) - 1, oz = ((c >> 4) & 3) - 1;
-    offs[c] = make_float4(ox * fc.w[0], oy * fc.w[1], oz * fc.w[2], 0.0f);
+    const float w0 = (float)Rp.w[0], w1 = (float)Rp.w[1], w2 = (float)Rp.w[2];
+    offs[c] = make_float4(ox * w0, oy * w1, oz * w2, 0.0f);
   }
-  const int b = xcd_block(blockIdx.x, gridDim.x);
   const long long g0 = bl.gbase[b];
   const int total = (int)(bl.gbase[b + 1] - g0);
   const int *gl = bl.glist + g0;
@@ -301,14 +319,15 @@ __global__ __launch_bounds__(kWave *W) void k_force_brick_mixed(
       p.x += (double)acc[0] - (double)acc[3];
       p.y += (double)acc[1] - (double)acc[4];
       p.z += (double)acc[2] - (double)acc[5];
-      vmax = fmax(vmax, finish_row(i, p, pos[i], mom, pos_out, epilogue, dt2, lc, id, nd));
+      vmax = fmax(vmax, finish_row(i, p, pos[i], mom, pos_out, epilogue, dt2, lc, id, nd, Rp));
     }
   }
-  if (vpart) block_max_to_partial(vmax, vpart);
+  if (vpart) block_max_to_partial(vmax, vpart, b);
 }
 
-// Observers over the list: kinetic energy, PE and virial sums per workgroup
-// (partial[block*3 + {0,1,2}]), reduced in fixed order by k_reduce_partials,
+// Observers over the list: kinetic energy, PE and virial sums per brick
+// (partial[brick*3 + {0,1,2}]), reduced in fixed order by k_reduce_partials
+// or, per replica, k_replica_sum3,
 // so the results are deterministic.  Each list entry carries weight 1/2:
 // owned-owned pairs appear in two rows, owned-ghost pairs in one row and are
 // halved by the reference (observer.cc:39-41,70-72).  Owned partners are at
@@ -377,7 +396,7 @@ __global__ __launch_bounds__(kWave *W) void k_observe_brick(const double4 *__res
   if (threadIdx.x < 3) {
     double acc = 0.0;
     for (int k = 0; k < W; k++) acc += red[threadIdx.x][k];
-    partial[(long long)blockIdx.x * 3 + threadIdx.x] = acc;
+    partial[(long long)b * 3 + threadIdx.x] = acc;
   }
 }
 

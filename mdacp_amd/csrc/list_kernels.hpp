@@ -1,7 +1,7 @@
 // list_kernels.hpp -- neighbour list on gfx950, organised by cell bricks.
 //
-// The extended cell grid (meshlist.cc:31-55) is cut into bricks of
-// bd[0] x bd[1] x bd[2] cells.  Cells are ranked brick by brick, and owned
+// The extended cell grid (meshlist.cc:31-55) of each replica is cut into
+// bricks of bd[0] x bd[1] x bd[2] cells (Replica::bd).  Cells are ranked brick by brick, and owned
 // particles are stored in that order, so the rows of one brick are contiguous
 // and one workgroup of the force kernels takes one brick.  The partners of a
 // brick's rows lie in its neighbourhood: the brick grown by one cell on every
@@ -26,11 +26,12 @@ struct CellTables {
   const int *ostart, *ocount, *gstart, *gcount;
 };
 
+// Bricks of a domain: each brick's replica, the replica table, and the
+// stride of the per-brick range tables.
 struct Bricks {
-  int bd[3];   // cells per brick along x, y, z
-  int nb[3];   // bricks along x, y, z
-  int m[3];    // cells along x, y, z (extended grid)
-  int rmax;    // neighbourhood cells of a full brick: prod(bd + 2)
+  const int *brep;         // replica of each brick
+  const Replica *reps;
+  int rmax;                // largest neighbourhood of a brick: max over replicas of prod(bd + 2)
 };
 
 // staged ranges of a brick: two per neighbourhood cell (owned, then ghosts)
@@ -41,15 +42,18 @@ struct BrickBox {
   int n[3];   // neighbourhood cells along x, y, z
 };
 
-__device__ __forceinline__ BrickBox brick_box(int b, const Bricks &B) {
+// neighbourhood box of global brick b of replica R, in the replica's cell
+// coordinates
+__device__ __forceinline__ BrickBox brick_box(int b, const Replica &R) {
   BrickBox X;
-  const int bx = b % B.nb[0];
-  const int by = (b / B.nb[0]) % B.nb[1];
-  const int bz = b / (B.nb[0] * B.nb[1]);
+  const int l = b - R.brick0;
+  const int bx = l % R.nb[0];
+  const int by = (l / R.nb[0]) % R.nb[1];
+  const int bz = l / (R.nb[0] * R.nb[1]);
   const int bi[3] = {bx, by, bz};
   for (int d = 0; d < 3; d++) {
-    const int c0 = bi[d] * B.bd[d];
-    const int c1 = min(B.m[d], c0 + B.bd[d]);
+    const int c0 = bi[d] * R.bd[d];
+    const int c1 = min(R.m[d], c0 + R.bd[d]);
     X.lo[d] = c0 - 1;
     X.n[d] = c1 - c0 + 2;
   }
@@ -101,7 +105,8 @@ __global__ void k_brick_ranges(Bricks B, CellTables T, int nbrick, int pn, int *
   const int b = blockIdx.x * blockDim.x + threadIdx.x;
   double total = 0.0;
   if (b < nbrick) {
-    const BrickBox X = brick_box(b, B);
+    const Replica &Rp = B.reps[B.brep[b]];
+    const BrickBox X = brick_box(b, Rp);
     const int nc = X.n[0] * X.n[1] * X.n[2];
     const int stride = kRangesPerCell * B.rmax;
     int *ts = tstart + (long long)b * stride;
@@ -112,8 +117,8 @@ __global__ void k_brick_ranges(Bricks B, CellTables T, int nbrick, int pn, int *
       const int y = X.lo[1] + (r / X.n[0]) % X.n[1];
       const int z = X.lo[2] + r / (X.n[0] * X.n[1]);
       int os = 0, oc = 0, gs = pn, gc = 0;
-      if (x >= 0 && x < B.m[0] && y >= 0 && y < B.m[1] && z >= 0 && z < B.m[2]) {
-        const int c = (z * B.m[1] + y) * B.m[0] + x;
+      if (x >= 0 && x < Rp.m[0] && y >= 0 && y < Rp.m[1] && z >= 0 && z < Rp.m[2]) {
+        const int c = Rp.cell0 + (z * Rp.m[1] + y) * Rp.m[0] + x;
         os = T.ostart[c];
         oc = T.ocount[c];
         gs = pn + T.gstart[c];
@@ -131,7 +136,7 @@ __global__ void k_brick_ranges(Bricks B, CellTables T, int nbrick, int pn, int *
     btotal[b] = base;
     total = (double)base;
   }
-  block_max_to_partial(total, maxpart);
+  block_max_to_partial(total, maxpart, blockIdx.x);
 }
 
 // Gather list of each brick: glist[gbase[b] + u] = index (owned, or pn + ghost)
@@ -226,12 +231,13 @@ struct RowCell {
   int lx, ly, lz, self;
 };
 
-__device__ __forceinline__ RowCell row_cell(int i, int c, const BrickBox &X, const Bricks &B, const int *tb,
+__device__ __forceinline__ RowCell row_cell(int i, int c, const BrickBox &X, const Replica &R, const int *tb,
                                             const int *__restrict__ ts) {
   RowCell rc;
-  rc.lx = c % B.m[0] - X.lo[0];
-  rc.ly = (c / B.m[0]) % B.m[1] - X.lo[1];
-  rc.lz = c / (B.m[0] * B.m[1]) - X.lo[2];
+  c -= R.cell0;
+  rc.lx = c % R.m[0] - X.lo[0];
+  rc.ly = (c / R.m[0]) % R.m[1] - X.lo[1];
+  rc.lz = c / (R.m[0] * R.m[1]) - X.lo[2];
   const int rs = kRangesPerCell * ((rc.lz * X.n[1] + rc.ly) * X.n[0] + rc.lx);
   rc.self = tb[rs] + (i - ts[rs]);
   return rc;
@@ -268,14 +274,15 @@ __global__ __launch_bounds__(kWave *W) void k_search_brick(const double4 *__rest
   const bool staged = total < st.lds_cap;
   if (staged) stage_fp64<T>(gl, total, pos, lds64);
   __syncthreads();
-  const BrickBox X = brick_box(b, B);
+  const Replica &Rp = B.reps[B.brep[b]];
+  const BrickBox X = brick_box(b, Rp);
   const int *ts = st.tstart + (long long)b * st.rstride;
   const int nrow = st.nrow[b], row0 = st.row0[b];
   const int nwords = mcap;
   for (int v = threadIdx.x; v < nrow; v += T) {
     const int i = row0 + v;
     const double4 qi = pos[i];
-    const RowCell rc = row_cell(i, cell[i], X, B, tb, ts);
+    const RowCell rc = row_cell(i, cell[i], X, Rp, tb, ts);
     uint32_t *mk = mask + (long long)i * mcap;
     unsigned long long acc = 0;  // pending bits, oldest in bit 0
     int accbits = 0, word = 0, cnt = 0;
@@ -410,7 +417,7 @@ __global__ void k_slice_len(const int *__restrict__ row_len, const int *__restri
     slen[s] = m;
     slots[s] = (long long)m * kWave;
   }
-  block_max_to_partial((double)m, partial);
+  block_max_to_partial((double)m, partial, blockIdx.x);
 }
 
 // MeshList::MakeList (meshlist.cc:58-87) in sliced-ELL form: one wavefront per
@@ -434,7 +441,8 @@ __global__ __launch_bounds__(kWave *W) void k_emit_brick(const int *__restrict__
   const int R = load_tb<T>(st, b, tb);
   __syncthreads();
   const int total = tb[R];
-  const BrickBox X = brick_box(b, B);
+  const Replica &Rp = B.reps[B.brep[b]];
+  const BrickBox X = brick_box(b, Rp);
   const int *ts = st.tstart + (long long)b * st.rstride;
   const int nrow = st.nrow[b], row0 = st.row0[b], vs0 = st.vslice0[b];
   const int wv = threadIdx.x / kWave, lane = threadIdx.x % kWave;
@@ -449,7 +457,7 @@ __global__ __launch_bounds__(kWave *W) void k_emit_brick(const int *__restrict__
     int cnt = 0;
     if (v < nrow) {
       const int i = row0 + v;
-      const RowCell rc = row_cell(i, cell[i], X, B, tb, ts);
+      const RowCell rc = row_cell(i, cell[i], X, Rp, tb, ts);
       // masks are indexed by the row order of the search
       const uint32_t *mk = mask + (long long)rowsrc[i] * mcap;
       int nbit = 0;

@@ -23,6 +23,25 @@
 
 using namespace mdgpu;
 
+namespace {
+// Parameters and PairList state of one replica (pairlist.h:10-31).
+struct ReplicaState {
+  double T = 1.0, gamma = 0.1;  // Langevin target temperature and friction
+  uint64_t seed = 0;            // Langevin key
+  double buffer_length = 0.0;   // validity budget
+  int lifetime = 0;
+};
+
+// A captured fused step (ghost refresh, pair force, per-replica maxima and
+// their copy to the host), replayed while the pointers it was captured with
+// stay valid.
+struct StepGraph {
+  hipGraphExec_t exec = nullptr;
+  const void *key = nullptr;  // the half-drifted positions it reads
+  long epoch = -1;
+};
+}  // namespace
+
 struct mdgpu_domain {
   mdgpu_params prm{};
   hipStream_t stream = nullptr;
@@ -47,6 +66,27 @@ struct mdgpu_domain {
   // bricks
   int nb[3], nbrick = 0, rmax = 0;
   std::vector<int> h_brick_of_cell;
+
+  // replicas: one for a single domain, several for an ensemble
+  int nrep = 1;
+  std::vector<Replica> rh;         // host copy of the replica table
+  std::vector<ReplicaState> rst;   // per-replica parameters and list budget
+  Replica *reps = nullptr;         // device replica table
+  int *brep = nullptr;             // replica of each brick
+  int *rep = nullptr, *rep_alt = nullptr;  // replica of each owned particle
+  int2 *grng = nullptr;            // ghost range {first, count} of each replica
+  double *rep_v2 = nullptr;        // per replica: max |p|^2 after the last fused step
+  double *rep_out = nullptr;       // per replica observer sums (3 each)
+  int *which = nullptr, *rep_exp = nullptr;  // replicas to check, their answers
+  double *h_rep_v2 = nullptr, *h_rep_out = nullptr;
+  int *h_which = nullptr, *h_rep_exp = nullptr;
+  unsigned long long *clock = nullptr;  // step counter read by replayed force launches
+  bool clock_dirty = true;              // clock differs from `step`
+
+  // captured fused steps, one per parity of the position buffers
+  bool use_graph = true;
+  long epoch = 0;  // bumped whenever a captured step may no longer match
+  StepGraph graphs[2];
 
   // thermostat
   int thermostat = MDGPU_NVE;
@@ -182,64 +222,124 @@ int check_launch(mdgpu_domain *d, const char *what) {
   return MDGPU_OK;
 }
 
-// MeshList::ChangeScale (meshlist.cc:31-55), the column origins of the
-// cell-relative FP32 coordinates, and the brick order of the cells.
+// Invalidate the captured steps and the half drift the next fused step would
+// reuse (state that the step reads has changed).
+void invalidate(mdgpu_domain *d) {
+  d->mid_valid = false;
+  d->epoch++;
+}
+
+// The replica table on the device: Langevin constants from each replica's
+// parameters (hb_D = sqrt(2 gamma T / dt), fcalculator.cc:766) and, for a
+// single domain, its owned count.
+int upload_replicas(mdgpu_domain *d) {
+  if (d->nrep == 1) {
+    d->rst[0].T = d->T_aim;
+    d->rst[0].gamma = d->gamma;
+    d->rst[0].seed = d->seed;
+    d->rh[0].p0 = 0;
+    d->rh[0].pn = d->pn;
+  }
+  for (int r = 0; r < d->nrep; r++) {
+    Replica &R = d->rh[r];
+    const ReplicaState &S = d->rst[r];
+    R.gamma = S.gamma;
+    R.hb_D = std::sqrt(2.0 * S.gamma * S.T / d->dt);
+    R.seed = S.seed;
+  }
+  CHK(sync(d));
+  HIPCHK(d, hipMemcpy(d->reps, d->rh.data(), d->nrep * sizeof(Replica), hipMemcpyHostToDevice));
+  return MDGPU_OK;
+}
+
+// MeshList::ChangeScale (meshlist.cc:31-55) of every replica, the column
+// origins of the cell-relative FP32 coordinates, and the brick order of the
+// cells: cells, bricks and origins of the replicas are concatenated, each
+// replica's cells ranked brick by brick.
 int set_geometry(mdgpu_domain *d) {
-  for (int k = 0; k < 3; k++) {
-    const double w = d->re[k] - d->rs[k];
-    const int ms = (int)(w / d->SL);
-    if (ms < 2) {
-      d->err = "box edge shorter than two search lengths (rc + margin)";
-      return MDGPU_ERR_ARGUMENT;
+  CHK(sync(d));
+  if (d->nrep == 1)
+    for (int k = 0; k < 3; k++) {
+      d->rh[0].L[k] = d->re[k] - d->rs[k];
+      d->rh[0].bd[k] = d->bd[k];
     }
-    d->msize[k] = w / (double)ms;
-    d->m[k] = (int)(w / d->msize[k]) + 2;
+  long long ncell = 0, nbrick = 0;
+  int rmax = 0;
+  std::vector<double> org;
+  for (int r = 0; r < d->nrep; r++) {
+    Replica &R = d->rh[r];
+    for (int k = 0; k < 3; k++) {
+      const double w = R.L[k];
+      const int ms = (int)(w / d->SL);
+      if (ms < 2) {
+        d->err = "box edge shorter than two search lengths (rc + margin)";
+        return MDGPU_ERR_ARGUMENT;
+      }
+      R.w[k] = w / (double)ms;
+      R.im[k] = 1.0 / R.w[k];
+      R.m[k] = (int)(w / R.w[k]) + 2;
+      R.nb[k] = (R.m[k] + R.bd[k] - 1) / R.bd[k];
+    }
+    R.cell0 = (int)ncell;
+    R.brick0 = (int)nbrick;
+    R.col0 = (int)org.size();
+    for (int k = 0; k < 3; k++)
+      for (int c = 0; c < R.m[k]; c++) org.push_back(0.0 + (double)(c - 1) * R.w[k]);
+    ncell += (long long)R.m[0] * R.m[1] * R.m[2];
+    nbrick += (long long)R.nb[0] * R.nb[1] * R.nb[2];
+    rmax = std::max(rmax, (R.bd[0] + 2) * (R.bd[1] + 2) * (R.bd[2] + 2));
+    if (ncell > (1ll << 30)) {
+      d->err = "too many cells";
+      return MDGPU_ERR_CAPACITY;
+    }
   }
-  const long long nc = (long long)d->m[0] * d->m[1] * d->m[2];
-  if (nc > (1ll << 30)) {
-    d->err = "too many cells";
-    return MDGPU_ERR_CAPACITY;
-  }
-  d->ncell = (int)nc;
+  d->ncell = (int)ncell;
+  d->nbrick = (int)nbrick;
+  d->rmax = rmax;
+  if (d->nrep == 1)
+    for (int k = 0; k < 3; k++) {
+      d->msize[k] = d->rh[0].w[k];
+      d->m[k] = d->rh[0].m[k];
+      d->nb[k] = d->rh[0].nb[k];
+    }
   CHK(dalloc(d, &d->c_ocount, d->ncell));
   CHK(dalloc(d, &d->c_ostart, d->ncell));
   CHK(dalloc(d, &d->c_gcount, d->ncell));
   CHK(dalloc(d, &d->c_gstart, d->ncell));
   CHK(dalloc(d, &d->count_r, d->ncell + 1));
   CHK(dalloc(d, &d->start_r, d->ncell + 1));
-  std::vector<double> org(d->m[0] + d->m[1] + d->m[2]);
-  int o = 0;
-  for (int k = 0; k < 3; k++)
-    for (int c = 0; c < d->m[k]; c++) org[o++] = d->rs[k] + (double)(c - 1) * d->msize[k];
   CHK(dalloc(d, &d->org, org.size()));
   HIPCHK(d, hipMemcpy(d->org, org.data(), org.size() * sizeof(double), hipMemcpyHostToDevice));
-  // bricks: bricks z-major, cells z, y, x inside each brick
-  for (int k = 0; k < 3; k++) d->nb[k] = (d->m[k] + d->bd[k] - 1) / d->bd[k];
-  d->nbrick = d->nb[0] * d->nb[1] * d->nb[2];
-  d->rmax = (d->bd[0] + 2) * (d->bd[1] + 2) * (d->bd[2] + 2);
-  std::vector<int> rank(d->ncell), boc(d->ncell), rank0(d->nbrick + 1);
-  int r = 0;
-  for (int bz = 0; bz < d->nb[2]; bz++)
-    for (int by = 0; by < d->nb[1]; by++)
-      for (int bx = 0; bx < d->nb[0]; bx++) {
-        const int b = (bz * d->nb[1] + by) * d->nb[0] + bx;
-        rank0[b] = r;
-        for (int z = bz * d->bd[2]; z < std::min(d->m[2], (bz + 1) * d->bd[2]); z++)
-          for (int y = by * d->bd[1]; y < std::min(d->m[1], (by + 1) * d->bd[1]); y++)
-            for (int x = bx * d->bd[0]; x < std::min(d->m[0], (bx + 1) * d->bd[0]); x++) {
-              const int c = (z * d->m[1] + y) * d->m[0] + x;
-              rank[c] = r++;
-              boc[c] = b;
-            }
-      }
-  rank0[d->nbrick] = r;
+  // bricks z-major inside each replica, cells z, y, x inside each brick
+  std::vector<int> rank(d->ncell), boc(d->ncell), rank0(d->nbrick + 1), brep(d->nbrick);
+  int rk = 0;
+  for (int r = 0; r < d->nrep; r++) {
+    const Replica &R = d->rh[r];
+    for (int bz = 0; bz < R.nb[2]; bz++)
+      for (int by = 0; by < R.nb[1]; by++)
+        for (int bx = 0; bx < R.nb[0]; bx++) {
+          const int b = R.brick0 + (bz * R.nb[1] + by) * R.nb[0] + bx;
+          rank0[b] = rk;
+          brep[b] = r;
+          for (int z = bz * R.bd[2]; z < std::min(R.m[2], (bz + 1) * R.bd[2]); z++)
+            for (int y = by * R.bd[1]; y < std::min(R.m[1], (by + 1) * R.bd[1]); y++)
+              for (int x = bx * R.bd[0]; x < std::min(R.m[0], (bx + 1) * R.bd[0]); x++) {
+                const int c = R.cell0 + (z * R.m[1] + y) * R.m[0] + x;
+                rank[c] = rk++;
+                boc[c] = b;
+              }
+        }
+  }
+  rank0[d->nbrick] = rk;
   d->h_brick_of_cell = boc;
   CHK(dalloc(d, &d->rank, d->ncell));
   CHK(dalloc(d, &d->brick_of_cell, d->ncell));
   CHK(dalloc(d, &d->brick_rank0, d->nbrick + 1));
+  CHK(dalloc(d, &d->brep, d->nbrick));
   HIPCHK(d, hipMemcpy(d->rank, rank.data(), rank.size() * sizeof(int), hipMemcpyHostToDevice));
   HIPCHK(d, hipMemcpy(d->brick_of_cell, boc.data(), boc.size() * sizeof(int), hipMemcpyHostToDevice));
   HIPCHK(d, hipMemcpy(d->brick_rank0, rank0.data(), rank0.size() * sizeof(int), hipMemcpyHostToDevice));
+  HIPCHK(d, hipMemcpy(d->brep, brep.data(), brep.size() * sizeof(int), hipMemcpyHostToDevice));
   CHK(dalloc(d, &d->row0, d->nbrick));
   CHK(dalloc(d, &d->nrow, d->nbrick));
   CHK(dalloc(d, &d->nsl, d->nbrick));
@@ -250,37 +350,16 @@ int set_geometry(mdgpu_domain *d) {
   CHK(dalloc(d, &d->gbase, d->nbrick + 1));
   CHK(dalloc(d, &d->tstart, (size_t)d->nbrick * kRangesPerCell * d->rmax));
   CHK(dalloc(d, &d->tbase, (size_t)d->nbrick * (kRangesPerCell * d->rmax + 1)));
+  CHK(upload_replicas(d));
   d->initialized = false;
+  invalidate(d);
   return MDGPU_OK;
-}
-
-Grid grid_of(const mdgpu_domain *d) {
-  Grid g;
-  for (int k = 0; k < 3; k++) {
-    g.s[k] = d->rs[k];
-    g.im[k] = 1.0 / d->msize[k];
-    g.m[k] = d->m[k];
-  }
-  return g;
-}
-
-Origins origins_of(const mdgpu_domain *d) {
-  Origins O;
-  O.x = d->org;
-  O.y = d->org + d->m[0];
-  O.z = d->org + d->m[0] + d->m[1];
-  O.mx = d->m[0];
-  O.my = d->m[1];
-  return O;
 }
 
 Bricks bricks_of(const mdgpu_domain *d) {
   Bricks B;
-  for (int k = 0; k < 3; k++) {
-    B.bd[k] = d->bd[k];
-    B.nb[k] = d->nb[k];
-    B.m[k] = d->m[k];
-  }
+  B.brep = d->brep;
+  B.reps = d->reps;
   B.rmax = d->rmax;
   return B;
 }
@@ -298,6 +377,8 @@ int ensure_owned(mdgpu_domain *d, int need) {
   CHK(dgrow_copy(d, &d->pos, d->pn, tc_new));
   CHK(dgrow_copy(d, &d->mom, d->pn, nc));
   CHK(dgrow_copy(d, &d->id, d->pn, nc));
+  CHK(dgrow_copy(d, &d->rep, d->pn, nc));
+  CHK(dalloc(d, &d->rep_alt, nc));
   CHK(dalloc(d, &d->pos_tmp, tc_new));
   CHK(dalloc(d, &d->pos_next, tc_new));
   CHK(dalloc(d, &d->pos_old, tc_new));
@@ -319,6 +400,7 @@ int ensure_owned(mdgpu_domain *d, int need) {
   CHK(dalloc(d, &d->rowsrc, nc));
   CHK(dalloc(d, &d->rowinv, nc));
   d->cap = nc;
+  invalidate(d);
   return MDGPU_OK;
 }
 
@@ -351,6 +433,7 @@ int ensure_ghosts(mdgpu_domain *d, int need) {
   CHK(dalloc(d, &d->t_gcode, ngc));
   CHK(dalloc(d, &d->t_gcell, ngc));
   d->gcap = ngc;
+  invalidate(d);
   return MDGPU_OK;
 }
 
@@ -410,8 +493,10 @@ int read_max(mdgpu_domain *d, int k, double *v) {
 
 // Order n binned items (cellv[], slot[] from atomics, counts[] per cell):
 // starts[] over cells (in brick rank order when `ranked`), perm[] = new order,
-// members of a cell in input order (k_sort_cells).
-int cell_sort(mdgpu_domain *d, const int *cellv, int n, int *counts, int *starts, bool ranked) {
+// members of a cell by (key, input index), or input index when key is null
+// (k_sort_cells).
+int cell_sort(mdgpu_domain *d, const int *cellv, int n, int *counts, int *starts, bool ranked,
+              const int *key) {
   hipStream_t st = d->stream;
   CHK(ensure_scan(d, d->ncell));
   if (ranked) {
@@ -424,7 +509,8 @@ int cell_sort(mdgpu_domain *d, const int *cellv, int n, int *counts, int *starts
     exclusive_scan<int, int>(counts, starts, d->ncell, d->scan_scratch, nullptr, st);
   }
   hipLaunchKernelGGL(k_scatter_perm, dim3(nblk(n)), dim3(kBlock), 0, st, cellv, d->slot, starts, n, d->perm);
-  hipLaunchKernelGGL(k_sort_cells, dim3(nblk(d->ncell)), dim3(kBlock), 0, st, d->perm, starts, counts, d->ncell);
+  hipLaunchKernelGGL(k_sort_cells, dim3(nblk(d->ncell)), dim3(kBlock), 0, st, d->perm, starts, counts, d->ncell,
+                     key);
   return check_launch(d, "cell sort");
 }
 
@@ -501,7 +587,7 @@ int order_rows(mdgpu_domain *d) {
   hipLaunchKernelGGL(k_brick_row_order, dim3(d->nbrick), dim3(kBlock), 0, st, d->row_len, d->row0, d->nrow,
                      d->rowsrc, d->rowinv);
   hipLaunchKernelGGL(k_gather_owned, dim3(nblk(pn)), dim3(kBlock), 0, st, d->rowsrc, pn, d->pos, d->mom, d->id,
-                     d->cell, d->pos_tmp, d->mom_alt, d->id_alt, d->cell_alt);
+                     d->cell, d->rep, d->pos_tmp, d->mom_alt, d->id_alt, d->cell_alt, d->rep_alt);
   hipLaunchKernelGGL(k_gather_int, dim3(nblk(pn)), dim3(kBlock), 0, st, d->rowsrc, pn, d->row_len, d->nimg);
   if (ng > 0) {
     HIPCHK(d, hipMemcpyAsync(d->pos_tmp + pn, d->pos + pn, ng * sizeof(double4), hipMemcpyDeviceToDevice, st));
@@ -515,6 +601,7 @@ int order_rows(mdgpu_domain *d) {
   std::swap(d->mom, d->mom_alt);
   std::swap(d->id, d->id_alt);
   std::swap(d->cell, d->cell_alt);
+  std::swap(d->rep, d->rep_alt);
   std::swap(d->row_len, d->nimg);
   return check_launch(d, "row order");
 }
@@ -525,8 +612,7 @@ int order_rows(mdgpu_domain *d) {
 // indices, and particles are put in brick order here anyway.
 int make_pair_list(mdgpu_domain *d) {
   hipStream_t st = d->stream;
-  d->mid_valid = false;  // particles are reordered and wrapped
-  const Grid g = grid_of(d);
+  invalidate(d);  // particles are reordered and wrapped
   const int pn = d->pn;
   d->ng = 0;
   d->tn = pn;
@@ -534,31 +620,26 @@ int make_pair_list(mdgpu_domain *d) {
   // 1. wrap (AdjustPeriodicBoundary) and bin the owned particles, brick order
   HIPCHK(d, hipMemsetAsync(d->c_ocount, 0, sizeof(int) * d->ncell, st));
   if (pn > 0) {
-    hipLaunchKernelGGL(k_wrap_bin, dim3(nblk(pn)), dim3(kBlock), 0, st, d->pos, pn, d->L[0], d->L[1], d->L[2],
-                       g, d->cell, d->slot, d->c_ocount, d->i32, 1);
+    hipLaunchKernelGGL(k_wrap_bin, dim3(nblk(pn)), dim3(kBlock), 0, st, d->pos, pn, d->rep, d->reps, d->cell,
+                       d->slot, d->c_ocount, d->i32, 1);
   }
   CHK(read_err_flags(d));
-  CHK(cell_sort(d, d->cell, pn, d->c_ocount, d->c_ostart, true));
+  CHK(cell_sort(d, d->cell, pn, d->c_ocount, d->c_ostart, true, d->id));
   if (pn > 0) {
     hipLaunchKernelGGL(k_gather_owned, dim3(nblk(pn)), dim3(kBlock), 0, st, d->perm, pn, d->pos, d->mom, d->id,
-                       d->cell, d->pos_tmp, d->mom_alt, d->id_alt, d->cell_alt);
+                       d->cell, d->rep, d->pos_tmp, d->mom_alt, d->id_alt, d->cell_alt, d->rep_alt);
     std::swap(d->pos, d->pos_tmp);
     std::swap(d->mom, d->mom_alt);
     std::swap(d->id, d->id_alt);
     std::swap(d->cell, d->cell_alt);
+    std::swap(d->rep, d->rep_alt);
   }
   // 2. ghost images (FindBorderParticles / SendBorderParticlesSub x 6)
-  Edges E;
-  for (int k = 0; k < 3; k++) {
-    E.s[k] = d->rs[k];
-    E.e[k] = d->re[k];
-    E.lo_hi[k] = d->rs[k] + d->SL;
-    E.hi_lo[k] = d->re[k] - d->SL;
-  }
   CHK(ensure_scan(d, std::max(pn, d->ncell)));
   long long ngl = 0;
   if (pn > 0) {
-    hipLaunchKernelGGL(k_count_images, dim3(nblk(pn)), dim3(kBlock), 0, st, d->pos, pn, E, d->nimg);
+    hipLaunchKernelGGL(k_count_images, dim3(nblk(pn)), dim3(kBlock), 0, st, d->pos, pn, d->rep, d->reps, d->SL,
+                       d->nimg);
     exclusive_scan<int, int>(d->nimg, d->goff, pn, d->scan_scratch, d->i64, st);
     CHK(read_total(d, &ngl));
   }
@@ -571,11 +652,10 @@ int make_pair_list(mdgpu_domain *d) {
   HIPCHK(d, hipMemsetAsync(d->c_gcount, 0, sizeof(int) * d->ncell, st));
   if (ng > 0) {
     // pos_tmp[0..ng) holds the unsorted images
-    hipLaunchKernelGGL(k_emit_images, dim3(nblk(pn)), dim3(kBlock), 0, st, d->pos, pn, E, d->L[0], d->L[1],
-                       d->L[2], g, d->goff, ng, d->t_gsrc, d->t_gcode, d->pos_tmp, d->t_gcell, d->slot,
-                       d->c_gcount);
+    hipLaunchKernelGGL(k_emit_images, dim3(nblk(pn)), dim3(kBlock), 0, st, d->pos, pn, d->rep, d->reps, d->SL,
+                       d->goff, ng, d->t_gsrc, d->t_gcode, d->pos_tmp, d->t_gcell, d->slot, d->c_gcount);
     CHK(check_launch(d, "emit images"));
-    CHK(cell_sort(d, d->t_gcell, ng, d->c_gcount, d->c_gstart, false));
+    CHK(cell_sort(d, d->t_gcell, ng, d->c_gcount, d->c_gstart, false, nullptr));
     hipLaunchKernelGGL(k_gather_ghosts, dim3(nblk(ng)), dim3(kBlock), 0, st, d->perm, ng, pn, d->t_gsrc,
                        d->t_gcode, d->pos_tmp, d->t_gcell, d->gsrc, d->gcode, d->pos, d->cell);
   } else {
@@ -583,6 +663,9 @@ int make_pair_list(mdgpu_domain *d) {
   }
   d->ng = ng;
   d->tn = pn + ng;
+  if (d->nrep > 1 && pn > 0)
+    hipLaunchKernelGGL(k_replica_ghosts, dim3(nblk(d->nrep)), dim3(kBlock), 0, st, d->goff, d->reps, d->nrep, ng,
+                       d->grng);
   // 4. bricks: rows, slices, staged ranges
   const Bricks B = bricks_of(d);
   const CellTables T{d->c_ostart, d->c_ocount, d->c_gstart, d->c_gcount};
@@ -683,16 +766,26 @@ int make_pair_list(mdgpu_domain *d) {
   d->initialized = true;
   d->buffer_length = d->margin;
   d->lifetime = 0;
+  for (ReplicaState &S : d->rst) {
+    S.buffer_length = d->margin;
+    S.lifetime = 0;
+  }
   d->vmax_valid = false;
+  invalidate(d);
   return MDGPU_OK;
 }
 
+// max |p|^2 of a single domain; after a fused step it is the one the step's
+// force launch reduced and copied to the host
 int max_p2(mdgpu_domain *d, double *v2) {
-  if (!d->vmax_valid) {
-    const unsigned nb = nblk(d->pn);
-    hipLaunchKernelGGL(k_max_p2, dim3(nb), dim3(kBlock), 0, d->stream, d->mom, d->pn, d->maxpart);
-    hipLaunchKernelGGL(k_reduce_max, dim3(1), dim3(kBlock), 0, d->stream, d->maxpart, (int)nb, d->u64);
+  if (d->vmax_valid) {
+    CHK(sync(d));
+    *v2 = d->h_rep_v2[0];
+    return MDGPU_OK;
   }
+  const unsigned nb = nblk(d->pn);
+  hipLaunchKernelGGL(k_max_p2, dim3(nb), dim3(kBlock), 0, d->stream, d->mom, d->pn, d->maxpart);
+  hipLaunchKernelGGL(k_reduce_max, dim3(1), dim3(kBlock), 0, d->stream, d->maxpart, (int)nb, d->u64);
   return read_max(d, 0, v2);
 }
 
@@ -757,6 +850,57 @@ int is_expired(mdgpu_domain *d, int *expired) {
   return MDGPU_OK;
 }
 
+// The same test for each replica of an ensemble; *expired = 1 when the list
+// of at least one replica has expired (the ensemble then rebuilds every
+// replica's list).  The maxima of |p|^2 come from the last fused step; the
+// displacement check runs, one workgroup per replica, for the replicas whose
+// budget ran out.
+int is_expired_ensemble(mdgpu_domain *d, int *expired) {
+  if (!d->initialized) {
+    *expired = 1;
+    return MDGPU_OK;
+  }
+  hipStream_t st = d->stream;
+  const int R = d->nrep;
+  if (!d->vmax_valid) {
+    hipLaunchKernelGGL(k_replica_max_p2, dim3(R), dim3(kBlock), 0, st, d->mom, d->reps, d->rep_v2);
+    HIPCHK(d, hipMemcpyAsync(d->h_rep_v2, d->rep_v2, R * sizeof(double), hipMemcpyDeviceToHost, st));
+  }
+  CHK(sync(d));
+  int nflag = 0;
+  for (int r = 0; r < R; r++) {
+    ReplicaState &S = d->rst[r];
+    S.lifetime++;
+    if (S.buffer_length > 0.0) {
+      const double vmax = std::sqrt(d->h_rep_v2[r]);
+      if (vmax * 2.0 * d->dt > d->margin) {
+        char b[200];
+        snprintf(b, sizeof b, "Too fast particles exists. Try smaller time step (replica %d, max_velocity = %f)", r,
+                 vmax);
+        d->err = b;
+        return MDGPU_ERR_TOO_FAST;
+      }
+      S.buffer_length = S.buffer_length - vmax * 2.0 * d->dt;
+    }
+    if (S.buffer_length < 0.0) d->h_which[nflag++] = r;
+  }
+  *expired = 0;
+  if (nflag == 0) return MDGPU_OK;
+  HIPCHK(d, hipMemcpyAsync(d->which, d->h_which, nflag * sizeof(int), hipMemcpyHostToDevice, st));
+  hipLaunchKernelGGL(k_check_replicas, dim3(nflag), dim3(kBlock), 0, st, d->pos, d->pos_old, d->which, d->reps,
+                     d->grng, d->pn, d->margin, d->SL * d->SL, d->CL2, d->check_list_length, d->rep_exp);
+  CHK(check_launch(d, "displacement check"));
+  HIPCHK(d, hipMemcpyAsync(d->h_rep_exp, d->rep_exp, nflag * sizeof(int), hipMemcpyDeviceToHost, st));
+  CHK(sync(d));
+  for (int k = 0; k < nflag; k++)
+    if (d->h_rep_exp[k]) *expired = 1;
+  return MDGPU_OK;
+}
+
+int check_expired(mdgpu_domain *d, int *expired) {
+  return d->nrep == 1 ? is_expired(d, expired) : is_expired_ensemble(d, expired);
+}
+
 ForceConsts fconsts(const mdgpu_domain *d) {
   ForceConsts fc;
   fc.CL2 = d->CL2;
@@ -773,18 +917,7 @@ ForceConstsF fconsts32(const mdgpu_domain *d) {
   f.A = (float)(24.0 * d->dt);
   f.B = (float)(48.0 * d->dt);
   f.C = (float)(8.0 * d->C2 * d->dt);
-  for (int k = 0; k < 3; k++) f.w[k] = (float)d->msize[k];
   return f;
-}
-
-LangevinConsts lconsts(const mdgpu_domain *d) {
-  LangevinConsts lc;
-  lc.gamma = d->gamma;
-  lc.hb_D = std::sqrt(2.0 * d->gamma * d->T_aim / d->dt);
-  lc.dt = d->dt;
-  lc.seed = d->seed;
-  lc.step = d->step;
-  return lc;
 }
 
 // The list as the brick kernels read it; lds_cap = staged entries per brick
@@ -800,72 +933,79 @@ BrickList brick_list(const mdgpu_domain *d, int bytes_per_entry) {
   bl.row0 = d->row0;
   bl.nrow = d->nrow;
   bl.vslice0 = d->vslice0;
+  bl.brep = d->brep;
+  bl.reps = d->reps;
   bl.lds_cap = lds_cap_for(d, bytes_per_entry);
   return bl;
 }
 
 template <int W, int U>
 void launch_fp64(mdgpu_domain *d, const double4 *src, double4 *pos_out, int epi, const ForceConsts &fc,
-                 const LangevinConsts &lc, double *vb, const NextDrift &nd) {
+                 const StepClock &sc, double *vb, const NextDrift &nd) {
   const BrickList bl = brick_list(d, 3 * sizeof(double));
   const size_t shm = (size_t)bl.lds_cap * 3 * sizeof(double);
   allow_lds(k_force_brick_fp64<W, U>, shm);
   hipLaunchKernelGGL((k_force_brick_fp64<W, U>), dim3(d->nbrick), dim3(kWave * W), shm, d->stream, src, d->mom,
-                     pos_out, bl, fc, epi, lc, d->id, vb, nd);
+                     pos_out, bl, fc, epi, sc, d->id, vb, nd);
 }
 
 template <int W, int U, bool K>
 void launch_mixed(mdgpu_domain *d, const double4 *src, const float4 *rel, double4 *pos_out, int epi,
-                  const ForceConstsF &f32, double dt2, const LangevinConsts &lc, double *vb, const NextDrift &nd) {
+                  const ForceConstsF &f32, double dt2, const StepClock &sc, double *vb, const NextDrift &nd) {
   const BrickList bl = brick_list(d, sizeof(float4));
   const size_t shm = (size_t)bl.lds_cap * sizeof(float4);
   allow_lds(k_force_brick_mixed<W, U, K>, shm);
   hipLaunchKernelGGL((k_force_brick_mixed<W, U, K>), dim3(d->nbrick), dim3(kWave * W), shm, d->stream, rel, src,
-                     d->mom, pos_out, bl, f32, dt2, epi, lc, d->id, vb, nd);
+                     d->mom, pos_out, bl, f32, dt2, epi, sc, d->id, vb, nd);
 }
 
 template <int W>
 void launch_force_w(mdgpu_domain *d, const double4 *src, const float4 *rel, double4 *pos_out, int epi,
-                    const ForceConsts &fc, const ForceConstsF &f32, const LangevinConsts &lc, double *vb,
+                    const ForceConsts &fc, const ForceConstsF &f32, const StepClock &sc, double *vb,
                     const NextDrift &nd) {
   if (d->precision == MDGPU_FP64) {
-    if (d->unroll == 4) launch_fp64<W, 4>(d, src, pos_out, epi, fc, lc, vb, nd);
-    else launch_fp64<W, 8>(d, src, pos_out, epi, fc, lc, vb, nd);
+    if (d->unroll == 4) launch_fp64<W, 4>(d, src, pos_out, epi, fc, sc, vb, nd);
+    else launch_fp64<W, 8>(d, src, pos_out, epi, fc, sc, vb, nd);
   } else if (d->kahan) {
-    if (d->unroll == 4) launch_mixed<W, 4, true>(d, src, rel, pos_out, epi, f32, fc.dt2, lc, vb, nd);
-    else launch_mixed<W, 8, true>(d, src, rel, pos_out, epi, f32, fc.dt2, lc, vb, nd);
+    if (d->unroll == 4) launch_mixed<W, 4, true>(d, src, rel, pos_out, epi, f32, fc.dt2, sc, vb, nd);
+    else launch_mixed<W, 8, true>(d, src, rel, pos_out, epi, f32, fc.dt2, sc, vb, nd);
   } else {
-    if (d->unroll == 4) launch_mixed<W, 4, false>(d, src, rel, pos_out, epi, f32, fc.dt2, lc, vb, nd);
-    else launch_mixed<W, 8, false>(d, src, rel, pos_out, epi, f32, fc.dt2, lc, vb, nd);
+    if (d->unroll == 4) launch_mixed<W, 4, false>(d, src, rel, pos_out, epi, f32, fc.dt2, sc, vb, nd);
+    else launch_mixed<W, 8, false>(d, src, rel, pos_out, epi, f32, fc.dt2, sc, vb, nd);
   }
 }
 
 // One pair-force launch.  Rows and partners (owned, and ghosts at
 // [pn, pn + ng)) are read from `src` (`rel` in FP32); `pos_out` receives the
 // drifted (or copied) owned positions; `nd` (optional) the next step's first
-// half drift.
+// half drift.  With want_vmax, each replica's max |p|^2 goes to rep_v2 and its
+// host mirror.  `clock` set: the Langevin step is read from it and advanced
+// (a launch that is captured and replayed).
 int launch_force(mdgpu_domain *d, const double4 *src, const float4 *rel, double4 *pos_out, int epilogue,
-                 bool want_vmax, const NextDrift &nd) {
+                 bool want_vmax, const NextDrift &nd, unsigned long long *clock = nullptr) {
   if (d->pn == 0) return MDGPU_OK;
   hipStream_t st = d->stream;
   const ForceConsts fc = fconsts(d);
   const ForceConstsF f32 = fconsts32(d);
-  const LangevinConsts lc = lconsts(d);
+  const StepClock sc{d->dt, d->step, clock};
   double *vb = want_vmax ? d->bpart : nullptr;
   const int ev0 = d->force_launches * 2;
-  if (d->timing && ev0 + 1 < (int)d->ev.size()) HIPCHK(d, hipEventRecord(d->ev[ev0], st));
+  const bool ev = d->timing && clock == nullptr && ev0 + 1 < (int)d->ev.size();
+  if (ev) HIPCHK(d, hipEventRecord(d->ev[ev0], st));
   switch (d->waves) {
-    case 2: launch_force_w<2>(d, src, rel, pos_out, epilogue, fc, f32, lc, vb, nd); break;
-    case 8: launch_force_w<8>(d, src, rel, pos_out, epilogue, fc, f32, lc, vb, nd); break;
-    case 16: launch_force_w<16>(d, src, rel, pos_out, epilogue, fc, f32, lc, vb, nd); break;
-    default: launch_force_w<4>(d, src, rel, pos_out, epilogue, fc, f32, lc, vb, nd); break;
+    case 2: launch_force_w<2>(d, src, rel, pos_out, epilogue, fc, f32, sc, vb, nd); break;
+    case 8: launch_force_w<8>(d, src, rel, pos_out, epilogue, fc, f32, sc, vb, nd); break;
+    case 16: launch_force_w<16>(d, src, rel, pos_out, epilogue, fc, f32, sc, vb, nd); break;
+    default: launch_force_w<4>(d, src, rel, pos_out, epilogue, fc, f32, sc, vb, nd); break;
   }
-  if (d->timing && ev0 + 1 < (int)d->ev.size()) {
+  if (ev) {
     HIPCHK(d, hipEventRecord(d->ev[ev0 + 1], st));
     d->force_launches++;
   }
-  if (want_vmax)
-    hipLaunchKernelGGL(k_reduce_max, dim3(1), dim3(kBlock), 0, st, d->bpart, d->nbrick, d->u64);
+  if (want_vmax) {
+    hipLaunchKernelGGL(k_replica_max, dim3(d->nrep), dim3(kBlock), 0, st, d->bpart, d->reps, d->rep_v2, clock);
+    HIPCHK(d, hipMemcpyAsync(d->h_rep_v2, d->rep_v2, d->nrep * sizeof(double), hipMemcpyDeviceToHost, st));
+  }
   return check_launch(d, "force");
 }
 
@@ -916,7 +1056,7 @@ int drift(mdgpu_domain *d) {
 int refresh(mdgpu_domain *d) {
   if (d->ng > 0)
     hipLaunchKernelGGL(k_refresh, dim3(nblk(d->ng)), dim3(kBlock), 0, d->stream, d->pos, d->gsrc, d->gcode, d->pn,
-                       d->ng, d->L[0], d->L[1], d->L[2]);
+                       d->ng, d->reps);
   return check_launch(d, "refresh");
 }
 
@@ -924,41 +1064,117 @@ int refresh(mdgpu_domain *d) {
 // halo refresh, all in `pos`)
 int force_in_place(mdgpu_domain *d) {
   if (d->precision == MDGPU_MIXED && d->tn > 0)
-    hipLaunchKernelGGL(k_to_rel, dim3(nblk(d->tn)), dim3(kBlock), 0, d->stream, d->pos, d->cell, d->tn,
-                       origins_of(d), d->rel);
-  d->mid_valid = false;
-  CHK(launch_force(d, d->pos, d->rel, d->pos, kKickOnly, false, NextDrift{nullptr, nullptr, nullptr, Origins{}}));
+    hipLaunchKernelGGL(k_to_rel, dim3(nblk(d->tn)), dim3(kBlock), 0, d->stream, d->pos, d->cell, d->pn, d->tn,
+                       d->rep, d->gcode, d->reps, d->org, d->rel);
+  invalidate(d);
+  CHK(launch_force(d, d->pos, d->rel, d->pos, kKickOnly, false, NextDrift{nullptr, nullptr, nullptr, nullptr}));
   d->vmax_valid = false;
   return MDGPU_OK;
 }
 
-// MDManager::Calculate (mdmanager.cc:116-148) for one domain
+// The launches of one fused step once pos_tmp (rel_tmp) holds the owned
+// particles' first half drift: ghost refresh from those positions, the kick
+// whose epilogue writes the final positions to pos and the next step's first
+// half drift to pos_next (rel_next), and each replica's max |p|^2, copied to
+// the host.  With `clock` the launches read and advance the device step
+// counter (they are being captured into a graph).
+int enqueue_fused(mdgpu_domain *d, unsigned long long *clock) {
+  hipStream_t st = d->stream;
+  const bool mixed = d->precision == MDGPU_MIXED;
+  if (d->ng > 0) {
+    if (mixed)
+      hipLaunchKernelGGL((k_refresh_from_drifted<true>), dim3(nblk(d->ng)), dim3(kBlock), 0, st, d->pos, d->pos_tmp,
+                         d->gsrc, d->gcode, d->pn, d->ng, d->reps, d->cell, d->org, d->rel_tmp);
+    else
+      hipLaunchKernelGGL((k_refresh_from_drifted<false>), dim3(nblk(d->ng)), dim3(kBlock), 0, st, d->pos,
+                         d->pos_tmp, d->gsrc, d->gcode, d->pn, d->ng, d->reps, d->cell, d->org, d->rel_tmp);
+  }
+  CHK(check_launch(d, "refresh"));
+  const int epi = d->thermostat == MDGPU_LANGEVIN ? kKickLangevin : kKickDrift;
+  const NextDrift nd{d->pos_next, mixed ? d->rel_next : nullptr, d->cell, d->org};
+  return launch_force(d, d->pos_tmp, d->rel_tmp, d->pos, epi, true, nd, clock);
+}
+
+void drop_graph(StepGraph &g) {
+  if (g.exec) (void)hipGraphExecDestroy(g.exec);
+  g.exec = nullptr;
+  g.key = nullptr;
+  g.epoch = -1;
+}
+
+// enqueue_fused replayed from a graph.  A graph is captured for the current
+// buffers: one per parity of the swapped position buffers, both recaptured
+// after anything that changes what the launches read (a construction,
+// reallocation, option or thermostat change).
+int launch_fused_graph(mdgpu_domain *d) {
+  StepGraph *g = nullptr;
+  for (StepGraph &c : d->graphs)
+    if (c.exec && c.epoch == d->epoch && c.key == (const void *)d->pos_tmp) g = &c;
+  if (!g) {
+    g = &d->graphs[0];
+    for (StepGraph &c : d->graphs)
+      if (!c.exec || c.epoch != d->epoch) {
+        g = &c;
+        break;
+      }
+    drop_graph(*g);
+    hipGraph_t graph = nullptr;
+    HIPCHK(d, hipStreamBeginCapture(d->stream, hipStreamCaptureModeRelaxed));
+    const int r = enqueue_fused(d, d->clock);
+    const hipError_t e = hipStreamEndCapture(d->stream, &graph);
+    if (r != MDGPU_OK) {
+      if (e == hipSuccess && graph) (void)hipGraphDestroy(graph);
+      return r;
+    }
+    if (e != hipSuccess) {
+      d->err = std::string("step capture: ") + hipGetErrorString(e);
+      return MDGPU_ERR_HIP;
+    }
+    const hipError_t ei = hipGraphInstantiate(&g->exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (ei != hipSuccess) {
+      g->exec = nullptr;
+      d->err = std::string("step graph: ") + hipGetErrorString(ei);
+      return MDGPU_ERR_HIP;
+    }
+    g->key = d->pos_tmp;
+    g->epoch = d->epoch;
+  }
+  HIPCHK(d, hipGraphLaunch(g->exec, d->stream));
+  return MDGPU_OK;
+}
+
+int rebuild_timed(mdgpu_domain *d) {
+  hipStream_t st = d->stream;
+  if (d->timing) {
+    hipEvent_t a, b;
+    HIPCHK(d, hipEventCreate(&a));
+    HIPCHK(d, hipEventCreate(&b));
+    HIPCHK(d, hipEventRecord(a, st));
+    CHK(make_pair_list(d));
+    HIPCHK(d, hipEventRecord(b, st));
+    HIPCHK(d, hipEventSynchronize(b));
+    float ms = 0.f;
+    HIPCHK(d, hipEventElapsedTime(&ms, a, b));
+    d->rebuild_ms += ms;
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+  } else {
+    CHK(make_pair_list(d));
+  }
+  d->rebuild_count++;
+  return MDGPU_OK;
+}
+
+// MDManager::Calculate (mdmanager.cc:116-148): one step of every replica
 int calculate_step(mdgpu_domain *d) {
   hipStream_t st = d->stream;
   int expired = 0;
-  CHK(is_expired(d, &expired));
-  if (expired) {
-    if (d->timing) {
-      hipEvent_t a, b;
-      HIPCHK(d, hipEventCreate(&a));
-      HIPCHK(d, hipEventCreate(&b));
-      HIPCHK(d, hipEventRecord(a, st));
-      CHK(make_pair_list(d));
-      HIPCHK(d, hipEventRecord(b, st));
-      HIPCHK(d, hipEventSynchronize(b));
-      float ms = 0.f;
-      HIPCHK(d, hipEventElapsedTime(&ms, a, b));
-      d->rebuild_ms += ms;
-      (void)hipEventDestroy(a);
-      (void)hipEventDestroy(b);
-    } else {
-      CHK(make_pair_list(d));
-    }
-    d->rebuild_count++;
-  }
+  CHK(check_expired(d, &expired));
+  if (expired) CHK(rebuild_timed(d));
   if (d->thermostat == MDGPU_NOSEHOOVER) {
-    // CalculateNoseHoover (mdmanager.cc:160-175)
-    d->mid_valid = false;
+    // CalculateNoseHoover (mdmanager.cc:160-175), single domain
+    invalidate(d);
     CHK(drift(d));
     CHK(refresh(d));
     double t;
@@ -974,35 +1190,30 @@ int calculate_step(mdgpu_domain *d) {
     // fused: [drift into pos_tmp +] halo refresh, then kick (+ drift or Langevin)
     // whose epilogue also writes the next step's first half drift into pos_next
     const bool mixed = d->precision == MDGPU_MIXED;
-    const Origins O = origins_of(d);
-    if (d->mid_valid) {
-      if (d->ng > 0) {
-        if (mixed)
-          hipLaunchKernelGGL((k_refresh_from_drifted<true>), dim3(nblk(d->ng)), dim3(kBlock), 0, st, d->pos,
-                             d->pos_tmp, d->gsrc, d->gcode, d->pn, d->ng, d->L[0], d->L[1], d->L[2], d->cell, O,
-                             d->rel_tmp);
-        else
-          hipLaunchKernelGGL((k_refresh_from_drifted<false>), dim3(nblk(d->ng)), dim3(kBlock), 0, st, d->pos,
-                             d->pos_tmp, d->gsrc, d->gcode, d->pn, d->ng, d->L[0], d->L[1], d->L[2], d->cell, O,
-                             d->rel_tmp);
-      }
-    } else {
+    if (!d->mid_valid) {
       const int tot = d->pn + d->ng;
       if (tot > 0) {
         if (mixed)
           hipLaunchKernelGGL((k_drift_refresh<true>), dim3(nblk(tot)), dim3(kBlock), 0, st, d->pos, d->mom,
-                             d->pos_tmp, d->gsrc, d->gcode, d->pn, d->ng, d->dt * 0.5, d->L[0], d->L[1], d->L[2],
-                             d->cell, O, d->rel_tmp);
+                             d->pos_tmp, d->gsrc, d->gcode, d->pn, d->ng, d->dt * 0.5, d->rep, d->reps, d->cell,
+                             d->org, d->rel_tmp);
         else
           hipLaunchKernelGGL((k_drift_refresh<false>), dim3(nblk(tot)), dim3(kBlock), 0, st, d->pos, d->mom,
-                             d->pos_tmp, d->gsrc, d->gcode, d->pn, d->ng, d->dt * 0.5, d->L[0], d->L[1], d->L[2],
-                             d->cell, O, d->rel_tmp);
+                             d->pos_tmp, d->gsrc, d->gcode, d->pn, d->ng, d->dt * 0.5, d->rep, d->reps, d->cell,
+                             d->org, d->rel_tmp);
       }
+      CHK(check_launch(d, "drift+refresh"));
     }
-    CHK(check_launch(d, "drift+refresh"));
-    const int epi = d->thermostat == MDGPU_LANGEVIN ? kKickLangevin : kKickDrift;
-    const NextDrift nd{d->pos_next, mixed ? d->rel_next : nullptr, d->cell, O};
-    CHK(launch_force(d, d->pos_tmp, d->rel_tmp, d->pos, epi, true, nd));
+    if (d->use_graph && !d->timing) {
+      if (d->clock_dirty) {
+        hipLaunchKernelGGL(k_set_u64, dim3(1), dim3(1), 0, st, d->clock, (unsigned long long)d->step);
+        d->clock_dirty = false;
+      }
+      CHK(launch_fused_graph(d));
+    } else {
+      CHK(enqueue_fused(d, nullptr));
+      d->clock_dirty = true;
+    }
     std::swap(d->pos_tmp, d->pos_next);
     if (mixed) std::swap(d->rel_tmp, d->rel_next);
     d->mid_valid = true;
@@ -1023,10 +1234,28 @@ void launch_observe(mdgpu_domain *d, const BrickList &bl, size_t shm) {
 }  // namespace
 
 // ======================================================================= C ABI
-extern "C" {
+namespace {
 
-int mdgpu_create(const mdgpu_params *prm, mdgpu_domain **out) {
-  if (!prm || !out) return MDGPU_ERR_ARGUMENT;
+// Cells per brick for a replica of density rho (cubic bricks): the largest
+// edge whose staged neighbourhood (bd + 2 cells per axis, FP64 entries of 24
+// bytes) is expected to fill at most 3/4 of the LDS budget, so that the force
+// kernels stage it.
+int auto_brick(const mdgpu_domain *d, double rho, const double L[3]) {
+  const double budget = 0.75 * std::min(d->lds_kb, d->lds_max_kb) * 1024.0;
+  double w[3];
+  for (int k = 0; k < 3; k++) w[k] = L[k] / std::max(1, (int)(L[k] / d->SL));
+  for (int b = 4; b > 1; b--) {
+    const double staged = rho * (b + 2) * w[0] * (b + 2) * w[1] * (b + 2) * w[2];
+    if (staged * 24.0 <= budget) return b;
+  }
+  return 1;
+}
+
+// A domain of nrep replicas with the common parameters of prm (dt, rc,
+// margin, check list length, precision, device); rl[r] gives replica r's box
+// and Langevin parameters (nullptr: one replica with box prm->L).
+int create_domain(const mdgpu_params *prm, int nrep, const mdgpu_replica *rl, const int *counts,
+                  mdgpu_domain **out) {
   *out = nullptr;
   mdgpu_domain *d = new mdgpu_domain();
   d->prm = *prm;
@@ -1043,10 +1272,10 @@ int mdgpu_create(const mdgpu_params *prm, mdgpu_domain **out) {
       d->lds_max_kb = shm / 1024 - 4;
   }
   for (int k = 0; k < 3; k++) {
-    d->L[k] = prm->L[k];
+    d->L[k] = rl ? rl[0].L[k] : prm->L[k];
     // single domain of a 1x1x1 grid: s = L*0, e = s + L (mdunit.cc:17-24)
-    d->rs[k] = prm->L[k] * 0.0;
-    d->re[k] = d->rs[k] + prm->L[k];
+    d->rs[k] = d->L[k] * 0.0;
+    d->re[k] = d->rs[k] + d->L[k];
   }
   d->dt = prm->dt;
   d->rc = prm->rc;
@@ -1064,9 +1293,44 @@ int mdgpu_create(const mdgpu_params *prm, mdgpu_domain **out) {
   const double s14 = s12 * s2;
   d->C2 = 6.0 * s14 - 3.0 * s8;
   d->C0 = -s12 + s6 - d->C2 / s2;
+  d->nrep = nrep;
+  d->rh.assign(nrep, Replica{});
+  d->rst.assign(nrep, ReplicaState{});
+  long long p0 = 0;
+  for (int r = 0; r < nrep; r++) {
+    Replica &R = d->rh[r];
+    if (rl) {
+      for (int k = 0; k < 3; k++) R.L[k] = rl[r].L[k];
+      d->rst[r].T = rl[r].T;
+      d->rst[r].gamma = rl[r].gamma;
+      d->rst[r].seed = rl[r].seed;
+      const double rho = counts[r] / (R.L[0] * R.L[1] * R.L[2]);
+      const int b = auto_brick(d, rho, R.L);
+      for (int k = 0; k < 3; k++) R.bd[k] = b;
+      R.p0 = (int)p0;
+      R.pn = counts[r];
+      p0 += counts[r];
+    } else {
+      for (int k = 0; k < 3; k++) R.bd[k] = d->bd[k];
+    }
+  }
   int r;
-  if ((r = set_geometry(d)) != MDGPU_OK || (r = ensure_owned(d, std::max(prm->capacity, 1024))) != MDGPU_OK ||
-      (r = ensure_ghosts(d, std::max(1024, prm->capacity / 4))) != MDGPU_OK) {
+  if ((r = dalloc(d, &d->reps, nrep)) || (r = dalloc(d, &d->rep_v2, nrep)) || (r = dalloc(d, &d->rep_out, 3 * nrep)) ||
+      (r = dalloc(d, &d->which, nrep)) || (r = dalloc(d, &d->rep_exp, nrep)) || (r = dalloc(d, &d->grng, nrep)) ||
+      (r = dalloc(d, &d->clock, 1))) {
+    mdgpu_destroy(d);
+    return r;
+  }
+  if (hipHostMalloc((void **)&d->h_rep_v2, nrep * sizeof(double)) != hipSuccess ||
+      hipHostMalloc((void **)&d->h_rep_out, 3 * nrep * sizeof(double)) != hipSuccess ||
+      hipHostMalloc((void **)&d->h_which, nrep * sizeof(int)) != hipSuccess ||
+      hipHostMalloc((void **)&d->h_rep_exp, nrep * sizeof(int)) != hipSuccess) {
+    mdgpu_destroy(d);
+    return MDGPU_ERR_HIP;
+  }
+  const int cap0 = rl ? (int)std::min<long long>(p0, kMaxParticles) : prm->capacity;
+  if ((r = set_geometry(d)) != MDGPU_OK || (r = ensure_owned(d, std::max(cap0, 1024))) != MDGPU_OK ||
+      (r = ensure_ghosts(d, std::max(1024, cap0 / 4))) != MDGPU_OK) {
     mdgpu_destroy(d);
     return r;
   }
@@ -1084,6 +1348,72 @@ int mdgpu_create(const mdgpu_params *prm, mdgpu_domain **out) {
   }
   (void)hipMemset(d->i32, 0, 4 * sizeof(int));
   (void)hipMemset(d->u64, 0, 4 * sizeof(unsigned long long));
+  (void)hipMemset(d->clock, 0, sizeof(unsigned long long));
+  *out = d;
+  return MDGPU_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int mdgpu_create(const mdgpu_params *prm, mdgpu_domain **out) {
+  if (!prm || !out) return MDGPU_ERR_ARGUMENT;
+  return create_domain(prm, 1, nullptr, nullptr, out);
+}
+
+int mdgpu_create_ensemble(const mdgpu_params *prm, int thermostat, int nrep, const mdgpu_replica *reps,
+                          const int *counts, const double *q, const double *p, const int *type, mdgpu_domain **out) {
+  if (!prm || !out || nrep < 1 || !reps || !counts || !q) return MDGPU_ERR_ARGUMENT;
+  *out = nullptr;
+  if (thermostat != MDGPU_NVE && thermostat != MDGPU_LANGEVIN) return MDGPU_ERR_ARGUMENT;
+  if (prm->check_list_length > kCheckCap) return MDGPU_ERR_ARGUMENT;
+  long long total = 0;
+  for (int r = 0; r < nrep; r++) {
+    if (counts[r] < 1) return MDGPU_ERR_ARGUMENT;
+    total += counts[r];
+  }
+  if (total + 1 > kMaxParticles) return MDGPU_ERR_CAPACITY;
+  // every particle inside its replica's box (MDRect::IsInside, mdrect.h:35-38)
+  {
+    long long i = 0;
+    for (int r = 0; r < nrep; r++)
+      for (int k = 0; k < counts[r]; k++, i++)
+        for (int a = 0; a < 3; a++) {
+          const double x = q[3 * i + a];
+          if (!(0.0 <= x && x < reps[r].L[a])) return MDGPU_ERR_ARGUMENT;
+        }
+  }
+  mdgpu_domain *d = nullptr;
+  int rc = create_domain(prm, nrep, reps, counts, &d);
+  if (rc != MDGPU_OK) return rc;
+  d->thermostat = thermostat;
+  const int n = (int)total;
+  std::vector<double4> hq(n), hp(n);
+  std::vector<int> hid(n), hrep(n);
+  {
+    int i = 0;
+    for (int r = 0; r < nrep; r++)
+      for (int k = 0; k < counts[r]; k++, i++) {
+        hq[i] = make_double4(q[3 * (size_t)i], q[3 * (size_t)i + 1], q[3 * (size_t)i + 2], 0.0);
+        hp[i] = p ? make_double4(p[3 * (size_t)i], p[3 * (size_t)i + 1], p[3 * (size_t)i + 2], 0.0)
+                  : make_double4(0.0, 0.0, 0.0, 0.0);
+        hid[i] = type ? type[i] : k;
+        hrep[i] = r;
+      }
+  }
+  if (hipMemcpy(d->pos, hq.data(), n * sizeof(double4), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(d->mom, hp.data(), n * sizeof(double4), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(d->id, hid.data(), n * sizeof(int), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(d->rep, hrep.data(), n * sizeof(int), hipMemcpyHostToDevice) != hipSuccess) {
+    mdgpu_destroy(d);
+    return MDGPU_ERR_HIP;
+  }
+  d->pn = d->tn = n;
+  if ((rc = upload_replicas(d)) != MDGPU_OK || (rc = make_pair_list(d)) != MDGPU_OK) {
+    mdgpu_destroy(d);
+    return rc;
+  }
   *out = d;
   return MDGPU_OK;
 }
@@ -1101,9 +1431,13 @@ int mdgpu_destroy(mdgpu_domain *d) {
                   d->rowsrc,  d->rowinv,      d->slot,        d->perm,      d->nimg,     d->goff,
                   d->t_gsrc,  d->t_gcode,     d->t_gcell,     d->key,       d->scan_scratch, d->disp,
                   d->partial, d->maxpart,     d->bpart,       d->cand,      d->u64,      d->i64,
-                  d->i32,     d->d3};
+                  d->i32,     d->d3,          d->reps,        d->brep,      d->rep,      d->rep_alt,
+                  d->grng,    d->rep_v2,      d->rep_out,     d->which,     d->rep_exp,  d->clock};
+  for (StepGraph &g : d->graphs) drop_graph(g);
   for (void *b : bufs)
     if (b) (void)hipFree(b);
+  for (void *h : {(void *)d->h_rep_v2, (void *)d->h_rep_out, (void *)d->h_which, (void *)d->h_rep_exp})
+    if (h) (void)hipHostFree(h);
   if (d->h_u64) (void)hipHostFree(d->h_u64);
   if (d->h_i64) (void)hipHostFree(d->h_i64);
   if (d->h_i32) (void)hipHostFree(d->h_i32);
@@ -1118,14 +1452,18 @@ const char *mdgpu_last_error(const mdgpu_domain *d) { return d ? d->err.c_str() 
 
 int mdgpu_set_precision(mdgpu_domain *d, int precision) {
   if (!d || (precision != MDGPU_FP64 && precision != MDGPU_MIXED)) return MDGPU_ERR_ARGUMENT;
-  d->mid_valid = false;  // the next fused step recomputes its first half drift
+  invalidate(d);  // the next fused step recomputes its first half drift
   d->precision = precision;
   return MDGPU_OK;
 }
 
 int mdgpu_add_particles(mdgpu_domain *d, int n, const double *q, const double *p, const int *type) {
   if (!d || n < 0 || (n > 0 && !q)) return MDGPU_ERR_ARGUMENT;
-  d->mid_valid = false;  // the next fused step recomputes its first half drift
+  if (d->nrep > 1) {
+    d->err = "an ensemble's particles are set when it is created";
+    return MDGPU_ERR_ARGUMENT;
+  }
+  invalidate(d);  // the next fused step recomputes its first half drift
   std::vector<double4> hq, hp;
   std::vector<int> hid;
   hq.reserve(n);
@@ -1147,10 +1485,12 @@ int mdgpu_add_particles(mdgpu_domain *d, int n, const double *q, const double *p
   HIPCHK(d, hipMemcpyAsync(d->pos + d->pn, hq.data(), add * sizeof(double4), hipMemcpyHostToDevice, d->stream));
   HIPCHK(d, hipMemcpyAsync(d->mom + d->pn, hp.data(), add * sizeof(double4), hipMemcpyHostToDevice, d->stream));
   HIPCHK(d, hipMemcpyAsync(d->id + d->pn, hid.data(), add * sizeof(int), hipMemcpyHostToDevice, d->stream));
+  HIPCHK(d, hipMemsetAsync(d->rep + d->pn, 0, add * sizeof(int), d->stream));
   CHK(sync(d));
   d->pn += add;
   d->tn = d->pn;
   d->ng = 0;
+  CHK(upload_replicas(d));
   d->vmax_valid = false;
   // indices of the old list are stale: the next step rebuilds
   d->initialized = false;
@@ -1191,7 +1531,7 @@ int mdgpu_get_particles(mdgpu_domain *d, double *q, double *p, int *type) {
 
 int mdgpu_set_momenta(mdgpu_domain *d, const double *p) {
   if (!d || !p) return MDGPU_ERR_ARGUMENT;
-  d->mid_valid = false;  // the next fused step recomputes its first half drift
+  invalidate(d);  // the next fused step recomputes its first half drift
   const int n = d->pn;
   std::vector<double4> hp(n);
   for (int i = 0; i < n; i++) hp[i] = make_double4(p[3 * (size_t)i], p[3 * (size_t)i + 1], p[3 * (size_t)i + 2], 0.0);
@@ -1203,7 +1543,7 @@ int mdgpu_set_momenta(mdgpu_domain *d, const double *p) {
 
 int mdgpu_set_positions(mdgpu_domain *d, const double *q) {
   if (!d || !q) return MDGPU_ERR_ARGUMENT;
-  d->mid_valid = false;  // the next fused step recomputes its first half drift
+  invalidate(d);  // the next fused step recomputes its first half drift
   const int n = d->pn;
   std::vector<double4> hq(n);
   for (int i = 0; i < n; i++) hq[i] = make_double4(q[3 * (size_t)i], q[3 * (size_t)i + 1], q[3 * (size_t)i + 2], 0.0);
@@ -1214,12 +1554,16 @@ int mdgpu_set_positions(mdgpu_domain *d, const double *q) {
 
 int mdgpu_clear_particles(mdgpu_domain *d) {
   if (!d) return MDGPU_ERR_ARGUMENT;
-  d->mid_valid = false;  // the next fused step recomputes its first half drift
+  if (d->nrep > 1) {
+    d->err = "an ensemble's particles are set when it is created";
+    return MDGPU_ERR_ARGUMENT;
+  }
+  invalidate(d);  // the next fused step recomputes its first half drift
   CHK(sync(d));
   d->pn = d->ng = d->tn = 0;
   d->initialized = false;
   d->vmax_valid = false;
-  return MDGPU_OK;
+  return upload_replicas(d);
 }
 
 int mdgpu_get_ghosts(mdgpu_domain *d, int *src_type, int *shift_code, double *q) {
@@ -1236,7 +1580,7 @@ int mdgpu_get_ghosts(mdgpu_domain *d, int *src_type, int *shift_code, double *q)
   CHK(sync(d));
   for (int k = 0; k < ng; k++) {
     if (src_type) src_type[k] = ids[gs[k]];
-    if (shift_code) shift_code[k] = gc[k];
+    if (shift_code) shift_code[k] = ghost_shift(gc[k]);
     if (q) {
       q[3 * (size_t)k] = gq[k].x;
       q[3 * (size_t)k + 1] = gq[k].y;
@@ -1248,18 +1592,31 @@ int mdgpu_get_ghosts(mdgpu_domain *d, int *src_type, int *shift_code, double *q)
 
 int mdgpu_set_thermostat(mdgpu_domain *d, int kind, double T, double gamma, double tau) {
   if (!d || kind < MDGPU_NVE || kind > MDGPU_LANGEVIN) return MDGPU_ERR_ARGUMENT;
-  d->mid_valid = false;  // the next fused step recomputes its first half drift
+  if (d->nrep > 1 && kind == MDGPU_NOSEHOOVER) {
+    d->err = "Nose-Hoover needs the temperature of a single domain";
+    return MDGPU_ERR_ARGUMENT;
+  }
+  invalidate(d);  // the next fused step recomputes its first half drift
   d->thermostat = kind;
   d->T_aim = T;
   d->gamma = gamma;
   d->tau = tau;
-  return MDGPU_OK;
+  if (d->nrep > 1)
+    for (ReplicaState &S : d->rst) {
+      S.T = T;
+      S.gamma = gamma;
+    }
+  return upload_replicas(d);
 }
 
 int mdgpu_set_seed(mdgpu_domain *d, uint64_t seed) {
   if (!d) return MDGPU_ERR_ARGUMENT;
+  if (d->nrep > 1) {
+    d->err = "seed the replicas of an ensemble with mdgpu_set_replica_thermostat";
+    return MDGPU_ERR_ARGUMENT;
+  }
   d->seed = seed;
-  return MDGPU_OK;
+  return upload_replicas(d);
 }
 
 int mdgpu_get_zeta(const mdgpu_domain *d, double *zeta) {
@@ -1282,18 +1639,18 @@ int mdgpu_make_pair_list(mdgpu_domain *d) {
 
 int mdgpu_is_pair_list_expired(mdgpu_domain *d, int *expired) {
   if (!d || !expired) return MDGPU_ERR_ARGUMENT;
-  return is_expired(d, expired);
+  return check_expired(d, expired);
 }
 
 int mdgpu_update_position_half(mdgpu_domain *d) {
   if (!d) return MDGPU_ERR_ARGUMENT;
-  d->mid_valid = false;  // the next fused step recomputes its first half drift
+  invalidate(d);  // the next fused step recomputes its first half drift
   return drift(d);
 }
 
 int mdgpu_send_border_particles(mdgpu_domain *d) {
   if (!d) return MDGPU_ERR_ARGUMENT;
-  d->mid_valid = false;  // the next fused step recomputes its first half drift
+  invalidate(d);  // the next fused step recomputes its first half drift
   return refresh(d);
 }
 
@@ -1308,12 +1665,13 @@ int mdgpu_calculate_force(mdgpu_domain *d) {
 
 int mdgpu_langevin(mdgpu_domain *d) {
   if (!d) return MDGPU_ERR_ARGUMENT;
-  d->mid_valid = false;  // the next fused step recomputes its first half drift
+  invalidate(d);  // the next fused step recomputes its first half drift
   if (d->pn > 0)
-    hipLaunchKernelGGL(k_langevin, dim3(nblk(d->pn)), dim3(kBlock), 0, d->stream, d->mom, d->id, d->pn,
-                       lconsts(d));
+    hipLaunchKernelGGL(k_langevin, dim3(nblk(d->pn)), dim3(kBlock), 0, d->stream, d->mom, d->id, d->pn, d->rep,
+                       d->reps, d->dt, (uint64_t)d->step);
   d->vmax_valid = false;
   d->step++;  // the next call draws the next numbers of each particle's stream
+  d->clock_dirty = true;
   return check_launch(d, "langevin");
 }
 
@@ -1324,13 +1682,13 @@ int mdgpu_set_zeta(mdgpu_domain *d, double zeta) {
 }
 
 int mdgpu_heatbath_zeta(mdgpu_domain *d, double t) {
-  if (!d) return MDGPU_ERR_ARGUMENT;
+  if (!d || d->nrep > 1) return MDGPU_ERR_ARGUMENT;
   return heatbath_zeta(d, t);
 }
 
 int mdgpu_heatbath_momenta(mdgpu_domain *d) {
-  if (!d) return MDGPU_ERR_ARGUMENT;
-  d->mid_valid = false;  // the next fused step recomputes its first half drift
+  if (!d || d->nrep > 1) return MDGPU_ERR_ARGUMENT;
+  invalidate(d);  // the next fused step recomputes its first half drift
   return heatbath_momenta(d);
 }
 
@@ -1373,6 +1731,10 @@ int mdgpu_calculate(mdgpu_domain *d, int nsteps, int *rebuilds) {
 
 int mdgpu_observe(mdgpu_domain *d, double *ke, double *pe, double *virial) {
   if (!d) return MDGPU_ERR_ARGUMENT;
+  if (d->nrep > 1) {
+    d->err = "observe the replicas of an ensemble with mdgpu_observe_replicas";
+    return MDGPU_ERR_ARGUMENT;
+  }
   if (!d->initialized) {
     d->err = "pair list not built";
     return MDGPU_ERR_ARGUMENT;
@@ -1402,13 +1764,13 @@ int mdgpu_observe(mdgpu_domain *d, double *ke, double *pe, double *virial) {
 }
 
 int mdgpu_observe_kinetic(mdgpu_domain *d, double *ke) {
-  if (!d || !ke) return MDGPU_ERR_ARGUMENT;
+  if (!d || !ke || d->nrep > 1) return MDGPU_ERR_ARGUMENT;
   return kinetic(d, ke);
 }
 
 int mdgpu_change_scale(mdgpu_domain *d, double alpha) {
-  if (!d || !(alpha > 0.0)) return MDGPU_ERR_ARGUMENT;
-  d->mid_valid = false;  // the next fused step recomputes its first half drift
+  if (!d || !(alpha > 0.0) || d->nrep > 1) return MDGPU_ERR_ARGUMENT;
+  invalidate(d);  // the next fused step recomputes its first half drift
   CHK(make_pair_list(d));
   for (int k = 0; k < 3; k++) {
     d->L[k] = d->L[k] * alpha;
@@ -1422,8 +1784,12 @@ int mdgpu_change_scale(mdgpu_domain *d, double alpha) {
   return make_pair_list(d);
 }
 
-int mdgpu_get_pairs(mdgpu_domain *d, int64_t *npairs, int32_t *out3) {
-  if (!d || !npairs) return MDGPU_ERR_ARGUMENT;
+}  // extern "C"
+
+namespace {
+
+// Canonical pairs of the rows of replica rsel (all rows when rsel < 0).
+int collect_pairs(mdgpu_domain *d, int rsel, int64_t *npairs, int32_t *out3) {
   if (!d->initialized) {
     d->err = "pair list not built";
     return MDGPU_ERR_ARGUMENT;
@@ -1431,7 +1797,7 @@ int mdgpu_get_pairs(mdgpu_domain *d, int64_t *npairs, int32_t *out3) {
   const int pn = d->pn, ng = d->ng, nbk = d->nbrick;
   std::vector<uint16_t> nbr((size_t)d->nbr_slots);
   std::vector<long long> sb(d->nslice), gb(nbk + 1);
-  std::vector<int> sl(d->nslice), ids(pn), cl(pn), r0(nbk), vs0(nbk), gs(ng), gc(ng);
+  std::vector<int> sl(d->nslice), ids(pn), cl(pn), rp(pn), r0(nbk), vs0(nbk), gs(ng), gc(ng);
   if (d->nbr_slots) HIPCHK(d, hipMemcpyAsync(nbr.data(), d->nbr, nbr.size() * 2, hipMemcpyDeviceToHost, d->stream));
   if (d->nslice) {
     HIPCHK(d, hipMemcpyAsync(sb.data(), d->sbase, d->nslice * 8, hipMemcpyDeviceToHost, d->stream));
@@ -1440,6 +1806,7 @@ int mdgpu_get_pairs(mdgpu_domain *d, int64_t *npairs, int32_t *out3) {
   if (pn) {
     HIPCHK(d, hipMemcpyAsync(ids.data(), d->id, pn * 4, hipMemcpyDeviceToHost, d->stream));
     HIPCHK(d, hipMemcpyAsync(cl.data(), d->cell, pn * 4, hipMemcpyDeviceToHost, d->stream));
+    HIPCHK(d, hipMemcpyAsync(rp.data(), d->rep, pn * 4, hipMemcpyDeviceToHost, d->stream));
   }
   if (ng) {
     HIPCHK(d, hipMemcpyAsync(gs.data(), d->gsrc, ng * 4, hipMemcpyDeviceToHost, d->stream));
@@ -1453,6 +1820,7 @@ int mdgpu_get_pairs(mdgpu_domain *d, int64_t *npairs, int32_t *out3) {
   if (!gl.empty()) HIPCHK(d, hipMemcpy(gl.data(), d->glist, gl.size() * 4, hipMemcpyDeviceToHost));
   int64_t cnt = 0;
   for (int i = 0; i < pn; i++) {
+    if (rsel >= 0 && rp[i] != rsel) continue;
     const int b = d->h_brick_of_cell[cl[i]];
     const int v = i - r0[b];
     const int s = vs0[b] + v / kWave, lane = v % kWave;
@@ -1469,7 +1837,7 @@ int mdgpu_get_pairs(mdgpu_domain *d, int64_t *npairs, int32_t *out3) {
         if (bb < a) continue;  // owned-owned pairs appear in both rows
       } else {
         bb = ids[gs[j - pn]];
-        code = gc[j - pn];
+        code = ghost_shift(gc[j - pn]);
       }
       if (out3) {
         out3[3 * cnt] = a;
@@ -1481,6 +1849,24 @@ int mdgpu_get_pairs(mdgpu_domain *d, int64_t *npairs, int32_t *out3) {
   }
   *npairs = cnt;
   return MDGPU_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int mdgpu_get_pairs(mdgpu_domain *d, int64_t *npairs, int32_t *out3) {
+  if (!d || !npairs) return MDGPU_ERR_ARGUMENT;
+  if (d->nrep > 1) {
+    d->err = "pairs of an ensemble are read per replica (mdgpu_get_replica_pairs)";
+    return MDGPU_ERR_ARGUMENT;
+  }
+  return collect_pairs(d, -1, npairs, out3);
+}
+
+int mdgpu_get_replica_pairs(mdgpu_domain *d, int replica, int64_t *npairs, int32_t *out3) {
+  if (!d || !npairs || replica < 0 || replica >= d->nrep) return MDGPU_ERR_ARGUMENT;
+  return collect_pairs(d, replica, npairs, out3);
 }
 
 int mdgpu_get_pair_indices(mdgpu_domain *d, int64_t *npairs, int32_t *key, int32_t *partner) {
@@ -1567,7 +1953,7 @@ int mdgpu_get_timing(const mdgpu_domain *d, int *force_launches, double *force_m
 
 int mdgpu_set_option(mdgpu_domain *d, int option, int value) {
   if (!d) return MDGPU_ERR_ARGUMENT;
-  d->mid_valid = false;  // the next fused step recomputes its first half drift
+  invalidate(d);  // the next fused step recomputes its first half drift
   switch (option) {
     case MDGPU_OPT_UNROLL:
       if (value != 4 && value != 8) return MDGPU_ERR_ARGUMENT;
@@ -1587,12 +1973,19 @@ int mdgpu_set_option(mdgpu_domain *d, int option, int value) {
     case MDGPU_OPT_BRICK: {
       const int bx = value & 0xff, by = (value >> 8) & 0xff, bz = (value >> 16) & 0xff;
       if (bx < 1 || bx > 4 || by < 1 || by > 4 || bz < 1 || bz > 4) return MDGPU_ERR_ARGUMENT;
-      if (bx == d->bd[0] && by == d->bd[1] && bz == d->bd[2]) return MDGPU_OK;
       d->bd[0] = bx;
       d->bd[1] = by;
       d->bd[2] = bz;
+      for (Replica &R : d->rh) {
+        R.bd[0] = bx;
+        R.bd[1] = by;
+        R.bd[2] = bz;
+      }
       return set_geometry(d);  // the next step rebuilds the list
     }
+    case MDGPU_OPT_GRAPH:
+      d->use_graph = value != 0;
+      return MDGPU_OK;
   }
   return MDGPU_ERR_ARGUMENT;
 }
@@ -1605,6 +1998,7 @@ int mdgpu_get_option(const mdgpu_domain *d, int option, int *value) {
     case MDGPU_OPT_WAVES: *value = d->waves; return MDGPU_OK;
     case MDGPU_OPT_LDS_KB: *value = std::min(d->lds_kb, d->lds_max_kb); return MDGPU_OK;
     case MDGPU_OPT_BRICK: *value = d->bd[0] | (d->bd[1] << 8) | (d->bd[2] << 16); return MDGPU_OK;
+    case MDGPU_OPT_GRAPH: *value = d->use_graph ? 1 : 0; return MDGPU_OK;
   }
   return MDGPU_ERR_ARGUMENT;
 }
@@ -1616,6 +2010,66 @@ int mdgpu_get_list_stats(const mdgpu_domain *d, int64_t *list_entries, int64_t *
   if (list_slots) *list_slots = d->nbr_slots;
   if (ncells) *ncells = d->ncell;
   if (ghosts) *ghosts = d->ng;
+  return MDGPU_OK;
+}
+
+int mdgpu_replica_number(const mdgpu_domain *d, int *nrep) {
+  if (!d || !nrep) return MDGPU_ERR_ARGUMENT;
+  *nrep = d->nrep;
+  return MDGPU_OK;
+}
+
+int mdgpu_replica_counts(const mdgpu_domain *d, int *counts) {
+  if (!d || !counts) return MDGPU_ERR_ARGUMENT;
+  if (d->nrep == 1) counts[0] = d->pn;
+  else
+    for (int r = 0; r < d->nrep; r++) counts[r] = d->rh[r].pn;
+  return MDGPU_OK;
+}
+
+int mdgpu_set_replica_thermostat(mdgpu_domain *d, int replica, double T, double gamma, uint64_t seed) {
+  if (!d || replica < 0 || replica >= d->nrep) return MDGPU_ERR_ARGUMENT;
+  if (d->nrep == 1) {
+    d->T_aim = T;
+    d->gamma = gamma;
+    d->seed = seed;
+  } else {
+    d->rst[replica].T = T;
+    d->rst[replica].gamma = gamma;
+    d->rst[replica].seed = seed;
+  }
+  return upload_replicas(d);
+}
+
+int mdgpu_observe_replicas(mdgpu_domain *d, double *ke, double *pe, double *virial) {
+  if (!d) return MDGPU_ERR_ARGUMENT;
+  if (!d->initialized) {
+    d->err = "pair list not built";
+    return MDGPU_ERR_ARGUMENT;
+  }
+  const int nb = d->nbrick, R = d->nrep;
+  CHK(ensure_partial(d, std::max(1, nb)));
+  if (d->pn > 0) {
+    const BrickList bl = brick_list(d, 3 * sizeof(double));
+    const size_t shm = (size_t)bl.lds_cap * 3 * sizeof(double);
+    switch (d->waves) {
+      case 2: launch_observe<2>(d, bl, shm); break;
+      case 8: launch_observe<8>(d, bl, shm); break;
+      case 16: launch_observe<16>(d, bl, shm); break;
+      default: launch_observe<4>(d, bl, shm); break;
+    }
+    hipLaunchKernelGGL(k_replica_sum3, dim3(R), dim3(kBlock), 0, d->stream, d->partial, d->reps, d->rep_out);
+    CHK(check_launch(d, "observe"));
+    HIPCHK(d, hipMemcpyAsync(d->h_rep_out, d->rep_out, 3 * R * sizeof(double), hipMemcpyDeviceToHost, d->stream));
+    CHK(sync(d));
+  } else {
+    for (int k = 0; k < 3 * R; k++) d->h_rep_out[k] = 0.0;
+  }
+  for (int r = 0; r < R; r++) {
+    if (ke) ke[r] = d->h_rep_out[3 * r];
+    if (pe) pe[r] = d->h_rep_out[3 * r + 1];
+    if (virial) virial[r] = d->h_rep_out[3 * r + 2] / 3.0;
+  }
   return MDGPU_OK;
 }
 

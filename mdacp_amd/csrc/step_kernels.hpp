@@ -22,13 +22,23 @@ struct ForceConsts {
 
 struct ForceConstsF {
   float CL2, A, B, C;
-  float w[3];       // cell widths (cell-relative coordinates)
 };
 
 struct LangevinConsts {
   double gamma, hb_D, dt;
   uint64_t seed, step;
 };
+
+// The Langevin constants of a replica at a step.
+__device__ __forceinline__ LangevinConsts langevin_consts(const Replica &R, double dt, uint64_t step) {
+  LangevinConsts lc;
+  lc.gamma = R.gamma;
+  lc.hb_D = R.hb_D;
+  lc.dt = dt;
+  lc.seed = R.seed;
+  lc.step = step;
+  return lc;
+}
 
 // ---------------------------------------------------------------- drift
 __global__ void k_drift(double4 *__restrict__ pos, const double4 *__restrict__ mom, int pn, double dt2) {
@@ -44,30 +54,36 @@ __global__ void k_drift(double4 *__restrict__ pos, const double4 *__restrict__ m
 
 // SendBorderParticles: ghosts [pn, pn + ng) from the current owned positions.
 __global__ void k_refresh(double4 *__restrict__ pos, const int *__restrict__ gsrc, const int *__restrict__ gcode,
-                          int pn, int ng, double Lx, double Ly, double Lz) {
+                          int pn, int ng, const Replica *__restrict__ reps) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= ng) return;
-  const double3 y = image_of(pos[gsrc[k]], gcode[k], Lx, Ly, Lz);
+  const int g = gcode[k];
+  const double3 y = image_of(pos[gsrc[k]], ghost_shift(g), reps[ghost_rep(g)].L);
   pos[pn + k] = make_double4(y.x, y.y, y.z, 0.0);
 }
 
-// Cell-relative FP32 coordinates: q - origin(cell), origin = s + (ix-1)*w.
-__device__ __forceinline__ float4 to_rel(double4 q, int c, const double *org_x, const double *org_y,
-                                         const double *org_z, int mx, int my) {
-  const int ix = c % mx, iy = (c / mx) % my, iz = c / (mx * my);
-  return make_float4((float)(q.x - org_x[ix]), (float)(q.y - org_y[iy]), (float)(q.z - org_z[iz]), 0.0f);
+// Cell-relative FP32 coordinates: q - origin(cell), origin = s + (ix-1)*w per
+// axis, read from the table of column origins (org[R.col0 ...]: the replica's
+// x columns, then its y, then its z columns).
+__device__ __forceinline__ float4 to_rel(double4 q, int c, const double *__restrict__ org, const Replica &R) {
+  const int l = c - R.cell0;
+  const int ix = l % R.m[0], iy = (l / R.m[0]) % R.m[1], iz = l / (R.m[0] * R.m[1]);
+  const double *o = org + R.col0;
+  return make_float4((float)(q.x - o[ix]), (float)(q.y - o[R.m[0] + iy]), (float)(q.z - o[R.m[0] + R.m[1] + iz]),
+                     0.0f);
 }
 
-struct Origins {
-  const double *x, *y, *z;
-  int mx, my;
-};
+// replica of entry i of the owned + ghost arrays
+__device__ __forceinline__ int replica_of(int i, int pn, const int *__restrict__ rep, const int *__restrict__ gcode) {
+  return i < pn ? rep[i] : ghost_rep(gcode[i - pn]);
+}
 
-__global__ void k_to_rel(const double4 *__restrict__ pos, const int *__restrict__ cell, int tn, Origins O,
-                         float4 *__restrict__ rel) {
+__global__ void k_to_rel(const double4 *__restrict__ pos, const int *__restrict__ cell, int pn, int tn,
+                         const int *__restrict__ rep, const int *__restrict__ gcode, const Replica *__restrict__ reps,
+                         const double *__restrict__ org, float4 *__restrict__ rel) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= tn) return;
-  rel[i] = to_rel(pos[i], cell[i], O.x, O.y, O.z, O.mx, O.my);
+  rel[i] = to_rel(pos[i], cell[i], org, reps[replica_of(i, pn, rep, gcode)]);
 }
 
 // Fused first half drift + halo refresh:
@@ -80,8 +96,9 @@ __global__ void k_to_rel(const double4 *__restrict__ pos, const int *__restrict_
 template <bool kRel>
 __global__ void k_drift_refresh(double4 *__restrict__ pos, const double4 *__restrict__ mom,
                                 double4 *__restrict__ tmp, const int *__restrict__ gsrc,
-                                const int *__restrict__ gcode, int pn, int ng, double dt2, double Lx,
-                                double Ly, double Lz, const int *__restrict__ cell, Origins O,
+                                const int *__restrict__ gcode, int pn, int ng, double dt2,
+                                const int *__restrict__ rep, const Replica *__restrict__ reps,
+                                const int *__restrict__ cell, const double *__restrict__ org,
                                 float4 *__restrict__ rel) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < pn) {
@@ -91,20 +108,22 @@ __global__ void k_drift_refresh(double4 *__restrict__ pos, const double4 *__rest
     q.y += p.y * dt2;
     q.z += p.z * dt2;
     tmp[i] = q;
-    if (kRel) rel[i] = to_rel(q, cell[i], O.x, O.y, O.z, O.mx, O.my);
+    if (kRel) rel[i] = to_rel(q, cell[i], org, reps[rep[i]]);
   } else if (i < pn + ng) {
     const int k = i - pn;
     const int src = gsrc[k];
+    const int w = gcode[k];
+    const Replica &R = reps[ghost_rep(w)];
     double4 q = pos[src];
     const double4 p = mom[src];
     q.x += p.x * dt2;
     q.y += p.y * dt2;
     q.z += p.z * dt2;
-    const double3 y = image_of(q, gcode[k], Lx, Ly, Lz);
+    const double3 y = image_of(q, ghost_shift(w), R.L);
     const double4 g = make_double4(y.x, y.y, y.z, 0.0);
     pos[i] = g;
     tmp[i] = g;
-    if (kRel) rel[i] = to_rel(g, cell[i], O.x, O.y, O.z, O.mx, O.my);
+    if (kRel) rel[i] = to_rel(g, cell[i], org, R);
   }
 }
 
@@ -115,16 +134,18 @@ __global__ void k_drift_refresh(double4 *__restrict__ pos, const double4 *__rest
 template <bool kRel>
 __global__ void k_refresh_from_drifted(double4 *__restrict__ pos, double4 *__restrict__ tmp,
                                        const int *__restrict__ gsrc, const int *__restrict__ gcode, int pn,
-                                       int ng, double Lx, double Ly, double Lz, const int *__restrict__ cell,
-                                       Origins O, float4 *__restrict__ rel) {
+                                       int ng, const Replica *__restrict__ reps, const int *__restrict__ cell,
+                                       const double *__restrict__ org, float4 *__restrict__ rel) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= ng) return;
   const int i = pn + k;
-  const double3 y = image_of(tmp[gsrc[k]], gcode[k], Lx, Ly, Lz);
+  const int w = gcode[k];
+  const Replica &R = reps[ghost_rep(w)];
+  const double3 y = image_of(tmp[gsrc[k]], ghost_shift(w), R.L);
   const double4 g = make_double4(y.x, y.y, y.z, 0.0);
   pos[i] = g;
   tmp[i] = g;
-  if (kRel) rel[i] = to_rel(g, cell[i], O.x, O.y, O.z, O.mx, O.my);
+  if (kRel) rel[i] = to_rel(g, cell[i], org, R);
 }
 
 // ----------------------------------------------------------- Langevin RNG
@@ -168,11 +189,12 @@ __device__ __forceinline__ void langevin_update(double4 &p, const LangevinConsts
 }
 
 // ----------------------------------------------------------- thermostats
-__global__ void k_langevin(double4 *__restrict__ mom, const int *__restrict__ id, int pn, LangevinConsts lc) {
+__global__ void k_langevin(double4 *__restrict__ mom, const int *__restrict__ id, int pn, const int *__restrict__ rep,
+                           const Replica *__restrict__ reps, double dt, uint64_t step) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= pn) return;
   double4 p = mom[i];
-  langevin_update(p, lc, (uint32_t)id[i]);
+  langevin_update(p, langevin_consts(reps[rep[i]], dt, step), (uint32_t)id[i]);
   mom[i] = p;
 }
 
@@ -230,7 +252,7 @@ __global__ void k_max_p2(const double4 *__restrict__ mom, int pn, double *__rest
     const double4 p = mom[i];
     v = p.x * p.x + p.y * p.y + p.z * p.z;
   }
-  block_max_to_partial(v, partial);
+  block_max_to_partial(v, partial, blockIdx.x);
 }
 
 // PairList::CheckByDisplacement, first pass: squared displacement since the
@@ -245,7 +267,7 @@ __global__ void k_displacement(const double4 *__restrict__ pos, const double4 *_
     v = dx * dx + dy * dy + dz * dz;
     disp[i] = v;
   }
-  block_max_to_partial(v, partial);
+  block_max_to_partial(v, partial, blockIdx.x);
 }
 
 // second pass: particles displaced by more than (margin - max displacement);
@@ -296,6 +318,188 @@ __global__ void k_scale_pos(double4 *__restrict__ pos, int pn, double alpha) {
   q.y *= alpha;
   q.z *= alpha;
   pos[i] = q;
+}
+
+// ------------------------------------------------------------- replicas
+// Reductions and the expiry test of an ensemble, one workgroup per replica
+// (kBlock threads, fixed order: repeated calls agree bitwise).
+
+__device__ __forceinline__ double block_max_all(double v) {
+  __shared__ double wm[kBlock / kWave];
+  __shared__ double res;
+  v = wave_max(v);
+  if ((threadIdx.x % kWave) == 0) wm[threadIdx.x / kWave] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double m = wm[0];
+    for (int k = 1; k < kBlock / kWave; k++) m = fmax(m, wm[k]);
+    res = m;
+  }
+  __syncthreads();
+  return res;
+}
+
+__device__ __forceinline__ double block_sum0(double v) {
+  __shared__ double ws[kBlock / kWave];
+  v = wave_sum(v);
+  if ((threadIdx.x % kWave) == 0) ws[threadIdx.x / kWave] = v;
+  __syncthreads();
+  double t = 0.0;
+  if (threadIdx.x == 0)
+    for (int k = 0; k < kBlock / kWave; k++) t += ws[k];
+  __syncthreads();
+  return t;  // valid in thread 0
+}
+
+// Largest per-brick value of each replica: out[r] = max of part over the
+// replica's bricks (the force kernel's maxima of |p|^2).  clock (optional) is
+// the step counter of the fused step; it advances by one here.
+__global__ __launch_bounds__(kBlock) void k_replica_max(const double *__restrict__ part,
+                                                        const Replica *__restrict__ reps, double *__restrict__ out,
+                                                        unsigned long long *__restrict__ clock) {
+  const Replica &R = reps[blockIdx.x];
+  const int b0 = R.brick0, nb = replica_bricks(R);
+  double m = 0.0;
+  for (int k = threadIdx.x; k < nb; k += kBlock) m = fmax(m, part[b0 + k]);
+  m = block_max_all(m);
+  if (threadIdx.x == 0) {
+    out[blockIdx.x] = m;
+    if (clock != nullptr && blockIdx.x == 0) clock[0] = clock[0] + 1ull;
+  }
+}
+
+// max |p|^2 over each replica's owned particles (PairList::UpdatePairListValidity,
+// pairlist.cc:48-65, before the square root)
+__global__ __launch_bounds__(kBlock) void k_replica_max_p2(const double4 *__restrict__ mom,
+                                                           const Replica *__restrict__ reps,
+                                                           double *__restrict__ out) {
+  const Replica &R = reps[blockIdx.x];
+  double m = 0.0;
+  for (int k = threadIdx.x; k < R.pn; k += kBlock) {
+    const double4 p = mom[R.p0 + k];
+    m = fmax(m, p.x * p.x + p.y * p.y + p.z * p.z);
+  }
+  m = block_max_all(m);
+  if (threadIdx.x == 0) out[blockIdx.x] = m;
+}
+
+// Per-replica sums of the observers' per-brick partials (3 per brick):
+// out[3r + c] = sum over the replica's bricks of part[3b + c].
+__global__ __launch_bounds__(kBlock) void k_replica_sum3(const double *__restrict__ part,
+                                                         const Replica *__restrict__ reps, double *__restrict__ out) {
+  const Replica &R = reps[blockIdx.x];
+  const int b0 = R.brick0, nb = replica_bricks(R);
+  for (int c = 0; c < 3; c++) {
+    double t = 0.0;
+    for (int k = threadIdx.x; k < nb; k += kBlock) t += part[3 * (long long)(b0 + k) + c];
+    t = block_sum0(t);
+    if (threadIdx.x == 0) out[3 * blockIdx.x + c] = t;
+  }
+}
+
+// KineticEnergyObserver (observer.cc:5-15) of each replica.
+__global__ __launch_bounds__(kBlock) void k_replica_kinetic(const double4 *__restrict__ mom,
+                                                            const Replica *__restrict__ reps,
+                                                            double *__restrict__ out) {
+  const Replica &R = reps[blockIdx.x];
+  double t = 0.0;
+  for (int k = threadIdx.x; k < R.pn; k += kBlock) {
+    const double4 p = mom[R.p0 + k];
+    t += 0.5 * p.x * p.x + 0.5 * p.y * p.y + 0.5 * p.z * p.z + 0.5 * p.w * p.w;
+  }
+  t = block_sum0(t);
+  if (threadIdx.x == 0) out[blockIdx.x] = 0.0 + t;
+}
+
+// PairList::CheckByDisplacement (pairlist.cc:68-121) of the replicas listed in
+// `which`, one workgroup each, over the replica's owned entries [p0, p0 + pn)
+// and its ghosts [pn_all + g0, pn_all + g0 + ng) (grng[r] = {g0, ng}):
+// displacements since the build, candidates displaced by more than
+// margin - max displacement (expired once Nl = min(cap, n) of them are
+// found), then the candidate pairs that were beyond SL at the build and are
+// now inside rc.  expired[k] receives the answer for replica which[k].
+constexpr int kCheckCap = 1024;  // largest CheckListLength an ensemble accepts
+
+__global__ __launch_bounds__(kBlock) void k_check_replicas(const double4 *__restrict__ pos,
+                                                           const double4 *__restrict__ old,
+                                                           const int *__restrict__ which,
+                                                           const Replica *__restrict__ reps,
+                                                           const int2 *__restrict__ grng, int pn_all, double margin,
+                                                           double SL2, double CL2, int cap,
+                                                           int *__restrict__ expired) {
+  __shared__ int cand[kCheckCap];
+  __shared__ int count, flag;
+  const int r = which[blockIdx.x];
+  const Replica &R = reps[r];
+  const int no = R.pn, g0 = grng[r].x, n = no + grng[r].y;
+  double m = 0.0;
+  for (int k = threadIdx.x; k < n; k += kBlock) {
+    const int i = k < no ? R.p0 + k : pn_all + g0 + (k - no);
+    const double4 a = old[i], b = pos[i];
+    const double dx = a.x - b.x, dy = a.y - b.y, dz = a.z - b.z;
+    m = fmax(m, dx * dx + dy * dy + dz * dz);
+  }
+  const double dmax2 = block_max_all(m);
+  const int Nl = n > cap ? cap : n;
+  const double c_distance = margin - sqrt(dmax2);
+  const double c_d2 = c_distance * c_distance;
+  if (threadIdx.x == 0) {
+    count = 0;
+    flag = 0;
+  }
+  __syncthreads();
+  for (int k = threadIdx.x; k < n; k += kBlock) {
+    const int i = k < no ? R.p0 + k : pn_all + g0 + (k - no);
+    const double4 a = old[i], b = pos[i];
+    const double dx = a.x - b.x, dy = a.y - b.y, dz = a.z - b.z;
+    if (dx * dx + dy * dy + dz * dz > c_d2) {
+      const int c = atomicAdd(&count, 1);
+      if (c < Nl) cand[c] = i;
+    }
+  }
+  __syncthreads();
+  const int l = count;
+  if (l > 0 && l >= Nl) {
+    if (threadIdx.x == 0) expired[blockIdx.x] = 1;
+    return;
+  }
+  for (int t = threadIdx.x; t < l * l; t += kBlock) {
+    const int a = t / l, b = t % l;
+    if (b <= a) continue;
+    const double4 o1 = old[cand[a]], o2 = old[cand[b]];
+    double dx = o1.x - o2.x, dy = o1.y - o2.y, dz = o1.z - o2.z;
+    if (dx * dx + dy * dy + dz * dz < SL2) continue;
+    const double4 q1 = pos[cand[a]], q2 = pos[cand[b]];
+    dx = q1.x - q2.x;
+    dy = q1.y - q2.y;
+    dz = q1.z - q2.z;
+    if (dx * dx + dy * dy + dz * dz < CL2) flag = 1;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) expired[blockIdx.x] = flag;
+}
+
+// Ghost ranges of each replica after a construction: replica r's images are
+// [goff[p0], goff[p0 + pn]) (goff = exclusive scan of the image counts over
+// the replica-major owned order; total = the number of images).
+__global__ void k_replica_ghosts(const int *__restrict__ goff, const Replica *__restrict__ reps, int nrep,
+                                 int total, int2 *__restrict__ grng) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= nrep) return;
+  const Replica &R = reps[r];
+  const int a = R.pn > 0 ? goff[R.p0] : 0;
+  int e = total;
+  for (int s = r + 1; s < nrep; s++)
+    if (reps[s].pn > 0) {
+      e = goff[reps[s].p0];
+      break;
+    }
+  const int g0 = R.pn > 0 ? a : e;
+  grng[r] = make_int2(g0, e - g0);
+}
+
+__global__ void k_set_u64(unsigned long long *__restrict__ p, unsigned long long v) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) p[0] = v;
 }
 
 }  // namespace mdgpu

@@ -26,15 +26,21 @@ EXPORTS = [
     "mdgpu_calculate", "mdgpu_observe", "mdgpu_change_scale", "mdgpu_get_pairs", "mdgpu_compute_kick",
     "mdgpu_enable_timing", "mdgpu_get_timing", "mdgpu_get_list_stats", "mdgpu_set_option", "mdgpu_get_option",
     "mdgpu_set_positions", "mdgpu_clear_particles", "mdgpu_set_zeta", "mdgpu_get_pair_indices",
-    "mdgpu_observe_kinetic",
+    "mdgpu_observe_kinetic", "mdgpu_create_ensemble", "mdgpu_replica_number", "mdgpu_replica_counts",
+    "mdgpu_set_replica_thermostat", "mdgpu_observe_replicas", "mdgpu_get_replica_pairs",
 ]
-OPT_UNROLL, OPT_KAHAN, OPT_WAVES, OPT_LDS_KB, OPT_BRICK = 1, 2, 3, 4, 5
+OPT_UNROLL, OPT_KAHAN, OPT_WAVES, OPT_LDS_KB, OPT_BRICK, OPT_GRAPH = 1, 2, 3, 4, 5, 6
 
 
 class Params(ctypes.Structure):
     _fields_ = [("L", ctypes.c_double * 3), ("dt", ctypes.c_double), ("rc", ctypes.c_double),
                 ("margin", ctypes.c_double), ("check_list_length", ctypes.c_int),
                 ("precision", ctypes.c_int), ("capacity", ctypes.c_int), ("device", ctypes.c_int)]
+
+
+class Replica(ctypes.Structure):
+    _fields_ = [("L", ctypes.c_double * 3), ("T", ctypes.c_double), ("gamma", ctypes.c_double),
+                ("seed", ctypes.c_uint64)]
 
 
 class MDGPUError(RuntimeError):
@@ -90,6 +96,13 @@ def load():
             "mdgpu_set_zeta": [P, D],
             "mdgpu_get_pair_indices": [P, PL, P, P],
             "mdgpu_observe_kinetic": [P, PD],
+            "mdgpu_create_ensemble": [ctypes.POINTER(Params), I, I, ctypes.POINTER(Replica), P, P, P, P,
+                                      ctypes.POINTER(ctypes.c_void_p)],
+            "mdgpu_replica_number": [P, PI],
+            "mdgpu_replica_counts": [P, P],
+            "mdgpu_set_replica_thermostat": [P, I, D, D, ctypes.c_uint64],
+            "mdgpu_observe_replicas": [P, P, P, P],
+            "mdgpu_get_replica_pairs": [P, I, PL, P],
         }
         for name, args in sig.items():
             f = getattr(L, name)
@@ -289,3 +302,74 @@ class Domain:
         self._chk(load().mdgpu_get_list_stats(self.h, ctypes.byref(e), ctypes.byref(s), ctypes.byref(c),
                                               ctypes.byref(g)))
         return dict(entries=e.value, slots=s.value, ncell=c.value, ghosts=g.value)
+
+
+class Ensemble(Domain):
+    """Independent replicas advanced together (mdgpu_create_ensemble).
+
+    boxes: (R, 3) box edges; counts: particles per replica; q, p: (sum counts, 3),
+    replica after replica; T, gamma, seed: per-replica Langevin parameters.
+    """
+
+    def __init__(self, boxes, counts, q, p=None, ids=None, T=None, gamma=None, seed=None, thermostat=NVE,
+                 dt=0.005, rc=2.5, margin=0.5, precision=FP64, check_list_length=500, device=0):
+        lib = load()
+        boxes = np.asarray(boxes, dtype=np.float64).reshape(-1, 3)
+        R = len(boxes)
+        counts = np.ascontiguousarray(counts, dtype=np.int32)
+        T = np.ones(R) if T is None else np.broadcast_to(np.asarray(T, dtype=np.float64), (R,))
+        gamma = np.ones(R) if gamma is None else np.broadcast_to(np.asarray(gamma, dtype=np.float64), (R,))
+        seed = np.arange(R) if seed is None else np.broadcast_to(np.asarray(seed), (R,))
+        reps = (Replica * R)()
+        for r in range(R):
+            reps[r].L = (ctypes.c_double * 3)(*boxes[r])
+            reps[r].T = float(T[r])
+            reps[r].gamma = float(gamma[r])
+            reps[r].seed = int(seed[r])
+        q = np.ascontiguousarray(q, dtype=np.float64)
+        p = None if p is None else np.ascontiguousarray(p, dtype=np.float64)
+        ids = None if ids is None else np.ascontiguousarray(ids, dtype=np.int32)
+        prm = Params((ctypes.c_double * 3)(*boxes[0]), dt, rc, margin, check_list_length, precision, 0, device)
+        h = ctypes.c_void_p()
+        rc_ = lib.mdgpu_create_ensemble(ctypes.byref(prm), thermostat, R, reps, _ptr(counts), _ptr(q), _ptr(p),
+                                        _ptr(ids), ctypes.byref(h))
+        if rc_ != OK:
+            raise MDGPUError(rc_, "mdgpu_create_ensemble failed")
+        self.h = h
+        self.boxes = boxes
+        self.L = boxes[0]
+        self.dt = dt
+        self.R = R
+
+    def replica_counts(self):
+        c = np.zeros(self.R, dtype=np.int32)
+        self._chk(load().mdgpu_replica_counts(self.h, _ptr(c)))
+        return c
+
+    def replica_slices(self):
+        c = self.replica_counts()
+        e = np.cumsum(c)
+        return [slice(int(a - n), int(a)) for a, n in zip(e, c)]
+
+    def set_replica_thermostat(self, r, T, gamma, seed):
+        self._chk(load().mdgpu_set_replica_thermostat(self.h, int(r), T, gamma, int(seed)))
+
+    def observe_replicas_raw(self):
+        ke, pe, vir = np.zeros(self.R), np.zeros(self.R), np.zeros(self.R)
+        self._chk(load().mdgpu_observe_replicas(self.h, _ptr(ke), _ptr(pe), _ptr(vir)))
+        return ke, pe, vir
+
+    def observables(self):
+        """T, P, E of every replica (arrays), as MDManager computes them for one system."""
+        ke, pe, vir = self.observe_replicas_raw()
+        pn = self.replica_counts().astype(np.float64)
+        T = (ke / pn) / 1.5
+        V = self.boxes.prod(axis=1)
+        return dict(ke=ke, pe=pe, vir=vir, T=T, P=(T - vir / pn) * pn / V, E=ke / pn + pe / pn)
+
+    def replica_pairs(self, r):
+        n = ctypes.c_int64()
+        self._chk(load().mdgpu_get_replica_pairs(self.h, int(r), ctypes.byref(n), None))
+        out = np.zeros((n.value, 3), dtype=np.int32)
+        self._chk(load().mdgpu_get_replica_pairs(self.h, int(r), ctypes.byref(n), _ptr(out)))
+        return out
