@@ -55,6 +55,7 @@ def run_gpu(precision, x, p, L, args, observe_every=100):
     d = mdgpu.Domain(L, dt=args.dt, rc=args.rc, margin=args.margin, precision=precision, capacity=n)
     d.add_particles(x, p)
     d.make_pair_list()
+    tuned = tune(d, args)
     d.calculate(args.warmup)
     d.observables()
     d.enable_timing(True)
@@ -80,8 +81,21 @@ def run_gpu(precision, x, p, L, args, observe_every=100):
     elapsed = t1 - t0
     res = dict(pu_s=args.steps * n / elapsed, ms_per_step=1e3 * elapsed / args.steps,
                force_avg_ms=force_ms / max(1, force_launches), force_launches=force_launches,
-               rebuilds=rebuilds, rebuild_avg_ms=rebuild_ms / max(1, rebuilds), list=d.list_stats())
+               rebuilds=rebuilds, rebuild_avg_ms=rebuild_ms / max(1, rebuilds), list=d.list_stats(), tuned=tuned)
     return d, res
+
+
+def tune(d, args):
+    """Launch shapes from the autotuner (timed on a clone, outside the timed steps)."""
+    if args.no_autotune:
+        return None
+    from mdacp_amd.autotune import autotune
+
+    t0 = time.perf_counter()
+    r = autotune(d)
+    r["choice"]["seconds"] = time.perf_counter() - t0
+    print(f"# autotune {r['choice']}", file=sys.stderr, flush=True)
+    return r["choice"]
 
 
 def energy_drift(d, steps, every=100):
@@ -128,6 +142,7 @@ def run_ensemble_gpu(precision, table, built, args):
     e = mdgpu.Ensemble(boxes, counts, q, p, ids, T=[r["T"] for r in table], gamma=1.0,
                        seed=[r["seed"] for r in table], thermostat=mdgpu.LANGEVIN, dt=args.dt, rc=args.rc,
                        margin=args.margin, precision=precision)
+    tuned = tune(e, args)
     e.calculate(args.warmup)
     e.observables()
     n = int(counts.sum())
@@ -148,7 +163,8 @@ def run_ensemble_gpu(precision, table, built, args):
     res = dict(pu_s=args.steps * n / elapsed, ms_per_step=1e3 * elapsed / args.steps, rebuilds=rebuilds,
                force_avg_ms=tm["force_ms"] / max(1, tm["force_launches"]),
                rebuild_avg_ms=tm["rebuild_ms"] / max(1, tm["rebuilds"]), list=e.list_stats(),
-               mean_T_over_target=float(np.mean(e.observables()["T"] / np.array([r["T"] for r in table]))))
+               mean_T_over_target=float(np.mean(e.observables()["T"] / np.array([r["T"] for r in table]))),
+               tuned=tuned)
     e.close()
     return res
 
@@ -230,6 +246,7 @@ def main_ensemble(args):
                   "speedup_vs_fp64": r32["pu_s"] / r64["pu_s"], "force_kernel_avg_ms": r32["force_avg_ms"],
                   "rebuilds_in_timed_steps": r32["rebuilds"]},
         "rebuild_avg_ms": r64["rebuild_avg_ms"],
+        "autotune": {"fp64": r64["tuned"], "mixed": r32["tuned"]},
         "mean_T_over_target": r64["mean_T_over_target"],
         "pair_list": r64["list"],
     }
@@ -246,6 +263,7 @@ def main():
     ap.add_argument("--drift-steps", type=int, default=10000)
     ap.add_argument("--cpu-steps", type=int, default=5)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true", help="default launch shapes")
     ap.add_argument("--fcc-cells", type=int, default=100, help="N = 4 k^3")
     ap.add_argument("--density", type=float, default=0.5)
     ap.add_argument("--dt", type=float, default=None, help="0.001 (single), 0.005 (ensemble)")
@@ -336,6 +354,7 @@ def main():
         "energy_drift": {"fp64": drift64, "mixed": drift32},
         "pair_list": r64["list"],
         "rebuild_avg_ms": r64["rebuild_avg_ms"],
+        "autotune": {"fp64": r64["tuned"], "mixed": r32["tuned"]},
     }
     print(json.dumps(line), flush=True)
 

@@ -186,12 +186,17 @@ int mdgpu_get_timing(const mdgpu_domain *d, int *force_launches, double *force_m
 #define MDGPU_OPT_UNROLL 1   /* partner entries per batch in the force kernels: 4 or 8 */
 #define MDGPU_OPT_KAHAN 2    /* mixed precision: 1 = compensated FP32 sums, 0 = plain */
 #define MDGPU_OPT_WAVES 3    /* wavefronts per cell-brick workgroup: 2, 4, 8 or 16 */
-#define MDGPU_OPT_LDS_KB 4   /* LDS budget for one brick's staged partners, KiB (>= 8) */
+#define MDGPU_OPT_LDS_KB 4   /* LDS budget for one brick's staged partners in the force and
+                                observer kernels, KiB (>= 8); bricks with more staged
+                                partners read them from global memory */
 #define MDGPU_OPT_BRICK 5    /* cells per brick: bx | by << 8 | bz << 16, each 1..4
                                 (the next step rebuilds the list); an ensemble
                                 picks one per replica from its density */
 #define MDGPU_OPT_GRAPH 6    /* 1 = fused steps replayed from a captured HIP graph
                                 (default; not while timing is enabled), 0 = launched */
+#define MDGPU_OPT_SEARCH_WAVES 7  /* wavefronts per brick in the neighbour search: 4, 8 or 16 */
+#define MDGPU_OPT_SEARCH_STAGE 8  /* neighbour search candidates staged in LDS (1) or read
+                                     from global memory (0) */
 int mdgpu_set_option(mdgpu_domain *d, int option, int value);
 int mdgpu_get_option(const mdgpu_domain *d, int option, int *value);
 

@@ -1,0 +1,93 @@
+"""Launch-shape autotuner for a domain or an ensemble (host side).
+
+The kernels' launch shapes trade differently with the workload: a large
+uniform domain (4M particles, ~800 rows per cell brick) runs its neighbour
+search best with 16 wavefronts per brick and candidates staged in LDS, while
+an ensemble of small replicas (~200 rows per brick, neighbourhoods of 2-4
+thousand particles) does best with 4 wavefronts reading candidates from L2;
+the pair-force kernel's LDS budget sets how many brick workgroups share a CU.
+autotune() times short runs of the caller's own workload on a clone (the
+caller's trajectory is not touched) and sets the fastest shape on the caller:
+
+- neighbour search: wavefronts per brick {4, 8, 16} x LDS staging {on, off},
+  by the time of a pair-list construction;
+- pair force: wavefronts per brick {4, 8} x LDS budget {48 ... 96} KiB x
+  partner batch {4, 8}, by the force kernel's time (HIP events); the budget
+  decides how many brick workgroups share a CU's 160 KiB and which bricks
+  read their partners from global memory instead;
+- optionally the Verlet margin (the rebuild cadence): each candidate margin's
+  time per step including its rebuilds.  A different margin lists different
+  pairs than the reference's configuration would, so the margin is only
+  reported (the caller creates its domain with it); the default does not
+  try it.
+"""
+import itertools
+import time
+
+from . import mdgpu
+
+SEARCH = [(w, s) for w in (4, 8, 16) for s in (1, 0)]
+FORCE = [(w, kb, u) for w in (4, 8) for kb in (48, 56, 64, 72, 76, 96) for u in (8, 4)]
+
+
+def _rebuild_ms(d, reps=3):
+    d.make_pair_list()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        d.make_pair_list()
+    return 1e3 * (time.perf_counter() - t0) / reps
+
+
+def _force_ms(d, steps):
+    d.enable_timing(True)
+    d.calculate(steps)
+    tm = d.timing()
+    d.enable_timing(False)
+    return tm["force_ms"] / max(1, tm["force_launches"])
+
+
+def autotune(d, steps=30, margins=None, log=None):
+    """Tune d's launch options; returns a dict of the choices and the timings behind them."""
+    say = log or (lambda *a: None)
+    c = d.clone()
+    c.calculate(5)
+    res = {"search": {}, "force": {}}
+    for w, s in SEARCH:
+        c.set_option(mdgpu.OPT_SEARCH_WAVES, w)
+        c.set_option(mdgpu.OPT_SEARCH_STAGE, s)
+        res["search"][f"waves={w} stage={s}"] = t = _rebuild_ms(c)
+        say(f"search waves={w} stage={s}: {t:.3f} ms per construction")
+    best_s = min(SEARCH, key=lambda k: res["search"][f"waves={k[0]} stage={k[1]}"])
+    c.set_option(mdgpu.OPT_SEARCH_WAVES, best_s[0])
+    c.set_option(mdgpu.OPT_SEARCH_STAGE, best_s[1])
+    c.make_pair_list()
+    for w, kb, u in FORCE:
+        c.set_option(mdgpu.OPT_WAVES, w)
+        c.set_option(mdgpu.OPT_LDS_KB, kb)
+        c.set_option(mdgpu.OPT_UNROLL, u)
+        _force_ms(c, 3)
+        res["force"][f"waves={w} lds_kb={kb} unroll={u}"] = t = _force_ms(c, steps)
+        say(f"force waves={w} lds_kb={kb} unroll={u}: {t:.4f} ms")
+    best_f = min(FORCE, key=lambda k: res["force"][f"waves={k[0]} lds_kb={k[1]} unroll={k[2]}"])
+    c.close()
+    choice = dict(search_waves=best_s[0], search_stage=best_s[1], waves=best_f[0], lds_kb=best_f[1],
+                  unroll=best_f[2])
+    d.set_option(mdgpu.OPT_SEARCH_WAVES, choice["search_waves"])
+    d.set_option(mdgpu.OPT_SEARCH_STAGE, choice["search_stage"])
+    d.set_option(mdgpu.OPT_WAVES, choice["waves"])
+    d.set_option(mdgpu.OPT_LDS_KB, choice["lds_kb"])
+    d.set_option(mdgpu.OPT_UNROLL, choice["unroll"])
+    res["choice"] = choice
+    if margins:
+        res["margin"] = {}
+        for m in margins:
+            e = d.clone(margin=m)
+            e.calculate(10)
+            t0 = time.perf_counter()
+            nreb = e.calculate(steps * 10)
+            ms = 1e3 * (time.perf_counter() - t0) / (steps * 10)
+            res["margin"][f"{m}"] = dict(ms_per_step=ms, rebuilds=nreb)
+            say(f"margin {m}: {ms:.4f} ms per step, {nreb} rebuilds")
+            e.close()
+        res["choice"]["margin"] = float(min(margins, key=lambda m: res["margin"][f"{m}"]["ms_per_step"]))
+    return res

@@ -59,33 +59,31 @@ __global__ void k_scatter_perm(const int *__restrict__ cell, const int *__restri
   perm[cstart[cell[i]] + slot[i]] = i;
 }
 
-// Order each cell's members (insertion sort of the cell's contiguous segment
-// of perm; cells hold tens of particles) by (key, input index) when `key` is
-// given, else by input index.  With the slots of k_wrap_bin taken by atomics
-// this makes the order deterministic; owned particles are keyed by particle
-// id, so the order after a construction depends only on the particles and
-// their ids, not on the order an earlier construction left.
+// Order each cell's members by (key, input index) when `key` is given, else
+// by input index: perm_in holds the items grouped by cell (k_scatter_perm, in
+// atomic order); each thread takes one item, counts the members of its cell
+// that sort before it and writes the item at that rank into perm_out.  Keys
+// are unique, so this is a permutation, and with the slots of k_wrap_bin
+// taken by atomics it makes the order deterministic.  Owned particles are
+// keyed by particle id, so the order after a construction depends only on
+// the particles and their ids, not on the order an earlier one left.
 __device__ __forceinline__ unsigned long long sort_key(const int *__restrict__ key, int i) {
   const unsigned long long k = key ? (unsigned long long)((unsigned)key[i] ^ 0x80000000u) : 0ull;
   return (k << 32) | (unsigned)i;
 }
 
-__global__ void k_sort_cells(int *__restrict__ perm, const int *__restrict__ cstart,
-                             const int *__restrict__ ccount, int ncell, const int *__restrict__ key) {
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= ncell) return;
-  int *a = perm + cstart[c];
-  const int n = ccount[c];
-  for (int t = 1; t < n; t++) {
-    const int v = a[t];
-    const unsigned long long kv = sort_key(key, v);
-    int u = t - 1;
-    while (u >= 0 && sort_key(key, a[u]) > kv) {
-      a[u + 1] = a[u];
-      u--;
-    }
-    a[u + 1] = v;
-  }
+__global__ void k_rank_in_cell(const int *__restrict__ perm_in, const int *__restrict__ cellv,
+                               const int *__restrict__ cstart, const int *__restrict__ ccount,
+                               const int *__restrict__ key, int n, int *__restrict__ perm_out) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  const int item = perm_in[p];
+  const int c = cellv[item];
+  const int s0 = cstart[c], m = ccount[c];
+  const unsigned long long mine = sort_key(key, item);
+  int rank = 0;
+  for (int q = 0; q < m; q++) rank += sort_key(key, perm_in[s0 + q]) < mine ? 1 : 0;
+  perm_out[s0 + rank] = item;
 }
 
 __global__ void k_gather_owned(const int *__restrict__ perm, int n, const double4 *__restrict__ pos_in,

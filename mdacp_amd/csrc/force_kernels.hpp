@@ -222,13 +222,17 @@ __global__ __launch_bounds__(kWave *W) void k_force_brick_fp64(const double4 *__
   const int b = xcd_block(blockIdx.x, gridDim.x);
   const Replica &Rp = bl.reps[bl.brep[b]];
   const LangevinConsts lc = langevin_consts(Rp, sc.dt, step_of(sc));
+  const int nrow = bl.nrow[b], row0 = bl.row0[b], vs0 = bl.vslice0[b];
+  if (nrow == 0) {  // a brick of ghost-layer cells only: nothing to stage
+    if (vpart) block_max_to_partial(0.0, vpart, b);
+    return;
+  }
   const long long g0 = bl.gbase[b];
   const int total = (int)(bl.gbase[b + 1] - g0);
   const int *gl = bl.glist + g0;
   const bool staged = total < bl.lds_cap;  // uniform across the workgroup
   if (staged) stage_fp64<T>(gl, total, src, lds64);
   __syncthreads();
-  const int nrow = bl.nrow[b], row0 = bl.row0[b], vs0 = bl.vslice0[b];
   const int nsl = (nrow + kWave - 1) / kWave;
   const int wv = threadIdx.x / kWave, lane = threadIdx.x % kWave;
   double vmax = 0.0;
@@ -276,6 +280,11 @@ __global__ __launch_bounds__(kWave *W) void k_force_brick_mixed(
     const float w0 = (float)Rp.w[0], w1 = (float)Rp.w[1], w2 = (float)Rp.w[2];
     offs[c] = make_float4(ox * w0, oy * w1, oz * w2, 0.0f);
   }
+  const int nrow = bl.nrow[b], row0 = bl.row0[b], vs0 = bl.vslice0[b];
+  if (nrow == 0) {  // a brick of ghost-layer cells only: nothing to stage
+    if (vpart) block_max_to_partial(0.0, vpart, b);
+    return;
+  }
   const long long g0 = bl.gbase[b];
   const int total = (int)(bl.gbase[b + 1] - g0);
   const int *gl = bl.glist + g0;
@@ -298,7 +307,6 @@ __global__ __launch_bounds__(kWave *W) void k_force_brick_mixed(
     if (threadIdx.x == 0) lds32[total] = make_float4(1.0e30f, 1.0e30f, 1.0e30f, 0.0f);
   }
   __syncthreads();
-  const int nrow = bl.nrow[b], row0 = bl.row0[b], vs0 = bl.vslice0[b];
   const int nsl = (nrow + kWave - 1) / kWave;
   const int wv = threadIdx.x / kWave, lane = threadIdx.x % kWave;
   double vmax = 0.0;
@@ -342,13 +350,17 @@ __global__ __launch_bounds__(kWave *W) void k_observe_brick(const double4 *__res
   __shared__ double red[3][W];
   constexpr int T = kWave * W;
   const int b = xcd_block(blockIdx.x, gridDim.x);
+  const int nrow = bl.nrow[b], row0 = bl.row0[b], vs0 = bl.vslice0[b];
+  if (nrow == 0) {
+    if (threadIdx.x < 3) partial[(long long)b * 3 + threadIdx.x] = 0.0;
+    return;
+  }
   const long long g0 = bl.gbase[b];
   const int total = (int)(bl.gbase[b + 1] - g0);
   const int *gl = bl.glist + g0;
   const bool staged = total < bl.lds_cap;
   if (staged) stage_fp64<T>(gl, total, pos, lds64);
   __syncthreads();
-  const int nrow = bl.nrow[b], row0 = bl.row0[b], vs0 = bl.vslice0[b];
   const int nsl = (nrow + kWave - 1) / kWave;
   const int wv = threadIdx.x / kWave, lane = threadIdx.x % kWave;
   double ke = 0.0, pe = 0.0, vir = 0.0;

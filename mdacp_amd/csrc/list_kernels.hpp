@@ -267,6 +267,7 @@ __global__ __launch_bounds__(kWave *W) void k_search_brick(const double4 *__rest
   __shared__ int tb[kMaxRanges + 1];
   constexpr int T = kWave * W;
   const int b = xcd_block(blockIdx.x, gridDim.x);
+  if (st.nrow[b] == 0) return;  // a brick of ghost-layer cells only
   const int R = load_tb<T>(st, b, tb);
   __syncthreads();
   const int total = tb[R];
@@ -438,6 +439,7 @@ __global__ __launch_bounds__(kWave *W) void k_emit_brick(const int *__restrict__
   __shared__ int tb[kMaxRanges + 1];
   constexpr int T = kWave * W;
   const int b = xcd_block(blockIdx.x, gridDim.x);
+  if (st.nrow[b] == 0) return;  // a brick of ghost-layer cells only
   const int R = load_tb<T>(st, b, tb);
   __syncthreads();
   const int total = tb[R];

@@ -56,6 +56,8 @@ struct mdgpu_domain {
   int bd[3] = {4, 4, 4};  // cells per brick
   int lds_kb = 96;        // LDS budget for one brick's staged partners, KiB
   int lds_max_kb = 60;    // device limit per workgroup less the kernels' static LDS
+  int search_waves = 16;  // wavefronts per brick workgroup of the neighbour search: 4, 8 or 16
+  int search_stage = 1;   // neighbour search reads candidates staged in LDS (1) or from global memory (0)
 
   // SimulationInfo / MDRect / MeshList geometry
   double L[3], rs[3], re[3];
@@ -140,6 +142,7 @@ struct mdgpu_domain {
   int mcap = 32;
   double *spart = nullptr;  // per-workgroup maxima of slice lengths
   int *slot = nullptr, *perm = nullptr, *nimg = nullptr, *goff = nullptr;
+  int *perm_in = nullptr;  // binned items grouped by cell, before the in-cell order
   int *t_gsrc = nullptr, *t_gcode = nullptr, *t_gcell = nullptr;
   long long *key = nullptr, *scan_scratch = nullptr;
   long long scan_scratch_n = 0;
@@ -393,6 +396,7 @@ int ensure_owned(mdgpu_domain *d, int need) {
   CHK(dalloc(d, &d->id_alt, nc));
   CHK(dalloc(d, &d->slot, tc_new));
   CHK(dalloc(d, &d->perm, tc_new));
+  CHK(dalloc(d, &d->perm_in, tc_new));
   CHK(dalloc(d, &d->key, tc_new));
   CHK(dalloc(d, &d->nimg, nc));
   CHK(dalloc(d, &d->goff, nc));
@@ -426,6 +430,7 @@ int ensure_ghosts(mdgpu_domain *d, int need) {
   CHK(dalloc(d, &d->maxpart, nblk((long long)tc_new, kWave)));
   CHK(dalloc(d, &d->slot, tc_new));
   CHK(dalloc(d, &d->perm, tc_new));
+  CHK(dalloc(d, &d->perm_in, tc_new));
   CHK(dalloc(d, &d->key, tc_new));
   CHK(dalloc(d, &d->gsrc, ngc));
   CHK(dalloc(d, &d->gcode, ngc));
@@ -494,7 +499,7 @@ int read_max(mdgpu_domain *d, int k, double *v) {
 // Order n binned items (cellv[], slot[] from atomics, counts[] per cell):
 // starts[] over cells (in brick rank order when `ranked`), perm[] = new order,
 // members of a cell by (key, input index), or input index when key is null
-// (k_sort_cells).
+// (k_rank_in_cell).
 int cell_sort(mdgpu_domain *d, const int *cellv, int n, int *counts, int *starts, bool ranked,
               const int *key) {
   hipStream_t st = d->stream;
@@ -508,9 +513,10 @@ int cell_sort(mdgpu_domain *d, const int *cellv, int n, int *counts, int *starts
   } else {
     exclusive_scan<int, int>(counts, starts, d->ncell, d->scan_scratch, nullptr, st);
   }
-  hipLaunchKernelGGL(k_scatter_perm, dim3(nblk(n)), dim3(kBlock), 0, st, cellv, d->slot, starts, n, d->perm);
-  hipLaunchKernelGGL(k_sort_cells, dim3(nblk(d->ncell)), dim3(kBlock), 0, st, d->perm, starts, counts, d->ncell,
-                     key);
+  hipLaunchKernelGGL(k_scatter_perm, dim3(nblk(n)), dim3(kBlock), 0, st, cellv, d->slot, starts, n, d->perm_in);
+  if (n > 0)
+    hipLaunchKernelGGL(k_rank_in_cell, dim3(nblk(n)), dim3(kBlock), 0, st, d->perm_in, cellv, starts, counts, key, n,
+                       d->perm);
   return check_launch(d, "cell sort");
 }
 
@@ -521,10 +527,15 @@ void allow_lds(K kernel, size_t bytes) {
 }
 
 // staged entries of bytes_per_entry that fit the LDS budget (incl. the pad entry)
-int lds_cap_for(const mdgpu_domain *d, int bytes_per_entry) {
-  const int budget = std::min(d->lds_kb, d->lds_max_kb) * 1024 / bytes_per_entry;
+int lds_cap_for(const mdgpu_domain *d, int bytes_per_entry, int kb) {
+  const int budget = std::min(kb, d->lds_max_kb) * 1024 / bytes_per_entry;
   return std::max(1, std::min(d->max_staged + 1, budget));
 }
+
+// LDS budget of the neighbour search's staging (independent of the force
+// kernels' budget: a construction runs one search per brick, without the
+// per-step trade-off between staged bricks and workgroups per CU).
+constexpr int kSearchLdsKb = 96;
 
 SearchTables search_tables(const mdgpu_domain *d) {
   SearchTables t;
@@ -537,22 +548,28 @@ SearchTables search_tables(const mdgpu_domain *d) {
   t.nrow = d->nrow;
   t.vslice0 = d->vslice0;
   t.rstride = kRangesPerCell * d->rmax;
-  t.lds_cap = lds_cap_for(d, 3 * sizeof(double));
+  t.lds_cap = lds_cap_for(d, 3 * sizeof(double), kSearchLdsKb);
   return t;
 }
 
 template <int W>
-void launch_search_w(mdgpu_domain *d, const Bricks &B, const SearchTables &t) {
+void launch_search_w(mdgpu_domain *d, const Bricks &B, SearchTables t) {
+  if (!d->search_stage) t.lds_cap = 0;  // every brick reads its candidates from global memory
   const size_t shm = (size_t)t.lds_cap * 3 * sizeof(double);
   allow_lds(k_search_brick<W>, shm);
   hipLaunchKernelGGL(k_search_brick<W>, dim3(d->nbrick), dim3(kWave * W), shm, d->stream, d->pos, d->cell, B, t,
                      d->S2, d->mcap, d->row_len, d->mask, d->i32);
 }
 
-// The search keeps one brick's staged neighbourhood in LDS like the force
-// kernel; with no per-slice work to balance it runs 16 wavefronts per brick.
+// One workgroup per brick, a thread per row; candidates from the brick's
+// neighbourhood staged in LDS like the force kernel, or read from global
+// memory (L2) when bricks hold few rows for the size of their neighbourhood.
 int launch_search(mdgpu_domain *d, const Bricks &B, const SearchTables &t) {
-  launch_search_w<16>(d, B, t);
+  switch (d->search_waves) {
+    case 4: launch_search_w<4>(d, B, t); break;
+    case 8: launch_search_w<8>(d, B, t); break;
+    default: launch_search_w<16>(d, B, t); break;
+  }
   return check_launch(d, "neighbour search");
 }
 
@@ -887,7 +904,7 @@ int is_expired_ensemble(mdgpu_domain *d, int *expired) {
   *expired = 0;
   if (nflag == 0) return MDGPU_OK;
   HIPCHK(d, hipMemcpyAsync(d->which, d->h_which, nflag * sizeof(int), hipMemcpyHostToDevice, st));
-  hipLaunchKernelGGL(k_check_replicas, dim3(nflag), dim3(kBlock), 0, st, d->pos, d->pos_old, d->which, d->reps,
+  hipLaunchKernelGGL(k_check_replicas, dim3(nflag), dim3(kCheckBlock), 0, st, d->pos, d->pos_old, d->which, d->reps,
                      d->grng, d->pn, d->margin, d->SL * d->SL, d->CL2, d->check_list_length, d->rep_exp);
   CHK(check_launch(d, "displacement check"));
   HIPCHK(d, hipMemcpyAsync(d->h_rep_exp, d->rep_exp, nflag * sizeof(int), hipMemcpyDeviceToHost, st));
@@ -935,7 +952,7 @@ BrickList brick_list(const mdgpu_domain *d, int bytes_per_entry) {
   bl.vslice0 = d->vslice0;
   bl.brep = d->brep;
   bl.reps = d->reps;
-  bl.lds_cap = lds_cap_for(d, bytes_per_entry);
+  bl.lds_cap = lds_cap_for(d, bytes_per_entry, d->lds_kb);
   return bl;
 }
 
@@ -1238,10 +1255,10 @@ namespace {
 
 // Cells per brick for a replica of density rho (cubic bricks): the largest
 // edge whose staged neighbourhood (bd + 2 cells per axis, FP64 entries of 24
-// bytes) is expected to fill at most 3/4 of the LDS budget, so that the force
-// kernels stage it.
+// bytes) is expected to fill at most 90% of 76 KiB, so that the force kernels
+// stage it with two brick workgroups per CU (160 KiB of LDS).
 int auto_brick(const mdgpu_domain *d, double rho, const double L[3]) {
-  const double budget = 0.75 * std::min(d->lds_kb, d->lds_max_kb) * 1024.0;
+  const double budget = 0.9 * std::min(76, d->lds_max_kb) * 1024.0;
   double w[3];
   for (int k = 0; k < 3; k++) w[k] = L[k] / std::max(1, (int)(L[k] / d->SL));
   for (int b = 4; b > 1; b--) {
@@ -1432,7 +1449,8 @@ int mdgpu_destroy(mdgpu_domain *d) {
                   d->t_gsrc,  d->t_gcode,     d->t_gcell,     d->key,       d->scan_scratch, d->disp,
                   d->partial, d->maxpart,     d->bpart,       d->cand,      d->u64,      d->i64,
                   d->i32,     d->d3,          d->reps,        d->brep,      d->rep,      d->rep_alt,
-                  d->grng,    d->rep_v2,      d->rep_out,     d->which,     d->rep_exp,  d->clock};
+                  d->grng,    d->rep_v2,      d->rep_out,     d->which,     d->rep_exp,  d->clock,
+                  d->perm_in};
   for (StepGraph &g : d->graphs) drop_graph(g);
   for (void *b : bufs)
     if (b) (void)hipFree(b);
@@ -1986,6 +2004,13 @@ int mdgpu_set_option(mdgpu_domain *d, int option, int value) {
     case MDGPU_OPT_GRAPH:
       d->use_graph = value != 0;
       return MDGPU_OK;
+    case MDGPU_OPT_SEARCH_WAVES:
+      if (value != 4 && value != 8 && value != 16) return MDGPU_ERR_ARGUMENT;
+      d->search_waves = value;
+      return MDGPU_OK;
+    case MDGPU_OPT_SEARCH_STAGE:
+      d->search_stage = value != 0;
+      return MDGPU_OK;
   }
   return MDGPU_ERR_ARGUMENT;
 }
@@ -1999,6 +2024,8 @@ int mdgpu_get_option(const mdgpu_domain *d, int option, int *value) {
     case MDGPU_OPT_LDS_KB: *value = std::min(d->lds_kb, d->lds_max_kb); return MDGPU_OK;
     case MDGPU_OPT_BRICK: *value = d->bd[0] | (d->bd[1] << 8) | (d->bd[2] << 16); return MDGPU_OK;
     case MDGPU_OPT_GRAPH: *value = d->use_graph ? 1 : 0; return MDGPU_OK;
+    case MDGPU_OPT_SEARCH_WAVES: *value = d->search_waves; return MDGPU_OK;
+    case MDGPU_OPT_SEARCH_STAGE: *value = d->search_stage; return MDGPU_OK;
   }
   return MDGPU_ERR_ARGUMENT;
 }

@@ -324,15 +324,16 @@ __global__ void k_scale_pos(double4 *__restrict__ pos, int pn, double alpha) {
 // Reductions and the expiry test of an ensemble, one workgroup per replica
 // (kBlock threads, fixed order: repeated calls agree bitwise).
 
+template <int T = kBlock>
 __device__ __forceinline__ double block_max_all(double v) {
-  __shared__ double wm[kBlock / kWave];
+  __shared__ double wm[T / kWave];
   __shared__ double res;
   v = wave_max(v);
   if ((threadIdx.x % kWave) == 0) wm[threadIdx.x / kWave] = v;
   __syncthreads();
   if (threadIdx.x == 0) {
     double m = wm[0];
-    for (int k = 1; k < kBlock / kWave; k++) m = fmax(m, wm[k]);
+    for (int k = 1; k < T / kWave; k++) m = fmax(m, wm[k]);
     res = m;
   }
   __syncthreads();
@@ -418,9 +419,10 @@ __global__ __launch_bounds__(kBlock) void k_replica_kinetic(const double4 *__res
 // margin - max displacement (expired once Nl = min(cap, n) of them are
 // found), then the candidate pairs that were beyond SL at the build and are
 // now inside rc.  expired[k] receives the answer for replica which[k].
-constexpr int kCheckCap = 1024;  // largest CheckListLength an ensemble accepts
+constexpr int kCheckCap = 1024;    // largest CheckListLength an ensemble accepts
+constexpr int kCheckBlock = 1024;  // threads per replica
 
-__global__ __launch_bounds__(kBlock) void k_check_replicas(const double4 *__restrict__ pos,
+__global__ __launch_bounds__(kCheckBlock) void k_check_replicas(const double4 *__restrict__ pos,
                                                            const double4 *__restrict__ old,
                                                            const int *__restrict__ which,
                                                            const Replica *__restrict__ reps,
@@ -433,13 +435,13 @@ __global__ __launch_bounds__(kBlock) void k_check_replicas(const double4 *__rest
   const Replica &R = reps[r];
   const int no = R.pn, g0 = grng[r].x, n = no + grng[r].y;
   double m = 0.0;
-  for (int k = threadIdx.x; k < n; k += kBlock) {
+  for (int k = threadIdx.x; k < n; k += kCheckBlock) {
     const int i = k < no ? R.p0 + k : pn_all + g0 + (k - no);
     const double4 a = old[i], b = pos[i];
     const double dx = a.x - b.x, dy = a.y - b.y, dz = a.z - b.z;
     m = fmax(m, dx * dx + dy * dy + dz * dz);
   }
-  const double dmax2 = block_max_all(m);
+  const double dmax2 = block_max_all<kCheckBlock>(m);
   const int Nl = n > cap ? cap : n;
   const double c_distance = margin - sqrt(dmax2);
   const double c_d2 = c_distance * c_distance;
@@ -448,7 +450,7 @@ __global__ __launch_bounds__(kBlock) void k_check_replicas(const double4 *__rest
     flag = 0;
   }
   __syncthreads();
-  for (int k = threadIdx.x; k < n; k += kBlock) {
+  for (int k = threadIdx.x; k < n; k += kCheckBlock) {
     const int i = k < no ? R.p0 + k : pn_all + g0 + (k - no);
     const double4 a = old[i], b = pos[i];
     const double dx = a.x - b.x, dy = a.y - b.y, dz = a.z - b.z;
@@ -463,7 +465,7 @@ __global__ __launch_bounds__(kBlock) void k_check_replicas(const double4 *__rest
     if (threadIdx.x == 0) expired[blockIdx.x] = 1;
     return;
   }
-  for (int t = threadIdx.x; t < l * l; t += kBlock) {
+  for (int t = threadIdx.x; t < l * l; t += kCheckBlock) {
     const int a = t / l, b = t % l;
     if (b <= a) continue;
     const double4 o1 = old[cand[a]], o2 = old[cand[b]];

@@ -30,6 +30,7 @@ EXPORTS = [
     "mdgpu_set_replica_thermostat", "mdgpu_observe_replicas", "mdgpu_get_replica_pairs",
 ]
 OPT_UNROLL, OPT_KAHAN, OPT_WAVES, OPT_LDS_KB, OPT_BRICK, OPT_GRAPH = 1, 2, 3, 4, 5, 6
+OPT_SEARCH_WAVES, OPT_SEARCH_STAGE = 7, 8
 
 
 class Params(ctypes.Structure):
@@ -134,6 +135,9 @@ class Domain:
         self.h = h
         self.L = np.array([Lx, Ly, Lz])
         self.dt = dt
+        self.args = dict(dt=dt, rc=rc, margin=margin, precision=precision, check_list_length=check_list_length,
+                         device=device)
+        self.thermo = (NVE, 1.0, 0.1, 0.1, 0)
 
     def close(self):
         if getattr(self, "h", None):
@@ -197,9 +201,29 @@ class Domain:
 
     def set_thermostat(self, kind, T=1.0, gamma=0.1, tau=0.1):
         self._chk(load().mdgpu_set_thermostat(self.h, kind, T, gamma, tau))
+        self.thermo = (kind, T, gamma, tau, self.thermo[4])
 
     def set_seed(self, seed):
         self._chk(load().mdgpu_set_seed(self.h, seed))
+        self.thermo = self.thermo[:4] + (seed,)
+
+    def clone(self, **override):
+        """A new domain with the same parameters, options and current particles (list built)."""
+        a = dict(self.args, **override)
+        c = Domain(self.L, dt=a["dt"], rc=a["rc"], margin=a["margin"], precision=a["precision"],
+                   capacity=self.counts()[0], check_list_length=a["check_list_length"], device=a["device"])
+        q, p, t = self.particles()
+        c.add_particles(q, p, t)
+        kind, T, gamma, tau, seed = self.thermo
+        c.set_thermostat(kind, T, gamma, tau)
+        c.set_seed(seed)
+        c.copy_options(self)
+        c.make_pair_list()
+        return c
+
+    def copy_options(self, other):
+        for o in (OPT_UNROLL, OPT_KAHAN, OPT_WAVES, OPT_LDS_KB, OPT_GRAPH, OPT_SEARCH_WAVES, OPT_SEARCH_STAGE):
+            self.set_option(o, other.get_option(o))
 
     def zeta(self):
         z = ctypes.c_double()
@@ -340,6 +364,23 @@ class Ensemble(Domain):
         self.L = boxes[0]
         self.dt = dt
         self.R = R
+        self.args = dict(dt=dt, rc=rc, margin=margin, precision=precision, check_list_length=check_list_length,
+                         device=device)
+        self.ens = dict(T=np.array(T, dtype=np.float64), gamma=np.array(gamma, dtype=np.float64),
+                        seed=np.array(seed), thermostat=thermostat)
+
+    def clone(self, **override):
+        """A new ensemble with the same parameters, options and current particles (lists built)."""
+        a = dict(self.args, **override)
+        q, p, t = self.particles()
+        rep_of = np.repeat(np.arange(self.R), self.replica_counts())
+        Lp = self.boxes[rep_of]
+        q = np.where(q < 0.0, q + Lp, np.where(q >= Lp, q - Lp, q))  # into [0, L), as a construction wraps
+        c = Ensemble(self.boxes, self.replica_counts(), q, p, t, T=self.ens["T"], gamma=self.ens["gamma"],
+                     seed=self.ens["seed"], thermostat=self.ens["thermostat"], **a)
+        c.copy_options(self)
+        c.make_pair_list()
+        return c
 
     def replica_counts(self):
         c = np.zeros(self.R, dtype=np.int32)

@@ -285,3 +285,22 @@ def test_langevin_replicas_follow_their_temperatures():
     say("mean T", Tm, "targets", target)
     assert np.all(np.diff(Tm) > 0), Tm
     assert np.all(np.abs(Tm / target - 1.0) < 0.25), (Tm, target)
+
+
+def test_autotune_keeps_results():
+    """The autotuner only changes launch shapes: a tuned ensemble follows an untuned one bit for bit."""
+    say("autotuned ensemble vs default launch shapes")
+    from mdacp_amd.autotune import autotune, FORCE, SEARCH
+
+    a, _ = ensemble(SPEC, melt_seed=8)
+    b, _ = ensemble(SPEC, melt_seed=8)
+    res = autotune(a, steps=5)
+    ch = res["choice"]
+    assert (ch["search_waves"], ch["search_stage"]) in SEARCH
+    assert (ch["waves"], ch["lds_kb"], ch["unroll"]) in FORCE
+    assert a.get_option(mdgpu.OPT_WAVES) == ch["waves"] and a.get_option(mdgpu.OPT_UNROLL) == ch["unroll"]
+    ra, rb = a.calculate(60), b.calculate(60)
+    assert ra == rb
+    qa, pa, ta = a.particles()
+    qb, pb, tb = b.particles()
+    assert np.array_equal(ta, tb) and np.array_equal(qa, qb) and np.array_equal(pa, pb)
