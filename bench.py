@@ -18,6 +18,11 @@ Langevin replicas (N = 4 k^3 drawn log-uniform in [2048, 65536], density in
 [0.3, 0.9], T in [0.7, 1.5], gamma = 1, own key each), rc = 2.5, margin 0.5,
 dt = 0.005, all advanced by one pair-force launch per step replayed from a
 captured graph; particle-updates/s summed over the replicas, FP64 and mixed.
+
+--workload quench: 16,384,000 particles (FCC, 4*160^3) at density 0.75,
+Langevin at T = 0.9 (gamma = 1, dt = 0.005) for --equil steps, then
+MDManager::ChangeScale to density 0.60 and NVE at dt = 0.001 from that state,
+FP64 and mixed; particle-updates/s of the NVE steps and the energy drift.
 """
 import argparse
 import json
@@ -55,8 +60,9 @@ def run_gpu(precision, x, p, L, args, observe_every=100):
     d = mdgpu.Domain(L, dt=args.dt, rc=args.rc, margin=args.margin, precision=precision, capacity=n)
     d.add_particles(x, p)
     d.make_pair_list()
-    tuned = tune(d, args)
     d.calculate(args.warmup)
+    tuned = tune(d, args)
+    d.calculate(20)
     d.observables()
     d.enable_timing(True)
     force_ms = 0.0
@@ -142,8 +148,9 @@ def run_ensemble_gpu(precision, table, built, args):
     e = mdgpu.Ensemble(boxes, counts, q, p, ids, T=[r["T"] for r in table], gamma=1.0,
                        seed=[r["seed"] for r in table], thermostat=mdgpu.LANGEVIN, dt=args.dt, rc=args.rc,
                        margin=args.margin, precision=precision)
-    tuned = tune(e, args)
     e.calculate(args.warmup)
+    tuned = tune(e, args)
+    e.calculate(20)
     e.observables()
     n = int(counts.sum())
     rebuilds = 0
@@ -253,9 +260,96 @@ def main_ensemble(args):
     print(json.dumps(line), flush=True)
 
 
+def main_quench(args):
+    from mdacp_amd import mdgpu
+    from mdacp_amd.lattice import fcc, initial_velocity
+
+    from mdacp_amd.ensemble import box_for
+
+    k = args.fcc_cells
+    L = box_for(k, 0.75)
+    x = fcc(L, 0.75)
+    n = len(x)
+    p = initial_velocity(n, math.sqrt(3.0 * 0.9), 0)
+    print(f"# quench: N={n} L={L} rho=0.75", file=sys.stderr, flush=True)
+    g = mdgpu.Domain(L, dt=0.005, rc=args.rc, margin=args.margin, capacity=n)
+    g.add_particles(x, p)
+    g.set_thermostat(mdgpu.LANGEVIN, 0.9, 1.0, 0.1)
+    g.set_seed(20261202)
+    g.make_pair_list()
+    t0 = time.perf_counter()
+    g.calculate(args.equil)
+    t_eq = time.perf_counter() - t0
+    alpha = (0.75 / 0.6) ** (1.0 / 3.0)
+    g.change_scale(alpha)
+    print(f"# equilibrated {args.equil} Langevin steps in {t_eq:.1f} s; scaled by {alpha}", file=sys.stderr,
+          flush=True)
+    out = {}
+    for name, prec in (("fp64", mdgpu.FP64), ("mixed", mdgpu.MIXED)):
+        d = g.clone(dt=args.dt, precision=prec)
+        d.set_thermostat(mdgpu.NVE)
+        d.calculate(args.warmup)
+        tuned = tune(d, args)
+        d.calculate(20)
+        d.observables()
+        rebuilds = 0
+        t0 = time.perf_counter()
+        done = 0
+        while done < args.steps:
+            kk = min(100, args.steps - done)
+            rebuilds += d.calculate(kk)
+            done += kk
+            d.observables()
+        elapsed = time.perf_counter() - t0
+        d.enable_timing(True)
+        d.calculate(200)
+        tm = d.timing()
+        d.enable_timing(False)
+        drift = energy_drift(d, args.drift_steps) if args.drift_steps else None
+        out[name] = dict(pu_s=args.steps * n / elapsed, ms_per_step=1e3 * elapsed / args.steps, rebuilds=rebuilds,
+                         force_avg_ms=tm["force_ms"] / max(1, tm["force_launches"]),
+                         rebuild_avg_ms=tm["rebuild_ms"] / max(1, tm["rebuilds"]), drift=drift, tuned=tuned)
+        print(f"# {name} {out[name]}", file=sys.stderr, flush=True)
+        d.close()
+    g.close()
+    bytes_pu, _, _, _ = algorithmic_per_update(0.6, args.rc, args.margin)
+    avg_s = out["fp64"]["force_avg_ms"] * 1e-3
+    achieved = bytes_pu * n / avg_s / 1e9
+    line = {
+        "metric": "particle-updates/s, quench 16.4M LJ rho 0.75 -> 0.60, NVE after ChangeScale",
+        "value": out["fp64"]["pu_s"],
+        "unit": "particle-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": out["fp64"]["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic: FCC lattice (confmaker.cc:8-44) at rho 0.75, Langevin T=0.9, ChangeScale to rho 0.60",
+        "config": {"workload": (f"N={n} rho 0.75 -> 0.60 (alpha={alpha:.7f}), {args.equil} Langevin steps "
+                                f"(T=0.9, gamma=1, dt=0.005), then NVE dt={args.dt} rc={args.rc} "
+                                f"margin={args.margin}"), "particles": n,
+                   "rebuilds_in_timed_steps": out["fp64"]["rebuilds"]},
+        "roofline": dict(bound="hbm", achieved=achieved, peak=HBM_PEAK_GBS, unit="GB/s", frac=achieved / HBM_PEAK_GBS,
+                         traffic=None, kernel="k_force_brick_fp64", kernel_avg_ms=out["fp64"]["force_avg_ms"],
+                         bytes_per_particle_update=bytes_pu),
+        "cpu_baseline": None,
+        "mixed": {"value": out["mixed"]["pu_s"], "ms_per_step": out["mixed"]["ms_per_step"],
+                  "speedup_vs_fp64": out["mixed"]["pu_s"] / out["fp64"]["pu_s"],
+                  "force_kernel_avg_ms": out["mixed"]["force_avg_ms"], "rebuilds_in_timed_steps": out["mixed"]["rebuilds"]},
+        "energy_drift": {"fp64": out["fp64"]["drift"], "mixed": out["mixed"]["drift"]},
+        "rebuild_avg_ms": out["fp64"]["rebuild_avg_ms"],
+        "autotune": {"fp64": out["fp64"]["tuned"], "mixed": out["mixed"]["tuned"]},
+    }
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("--workload", choices=["single", "ensemble"], default="single")
+    ap.add_argument("--workload", choices=["single", "ensemble", "quench"], default="single")
+    ap.add_argument("--equil", type=int, default=200, help="quench: Langevin steps before the rescale")
     ap.add_argument("--replicas", type=int, default=256)
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2000)
@@ -264,7 +358,7 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=5)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-autotune", action="store_true", help="default launch shapes")
-    ap.add_argument("--fcc-cells", type=int, default=100, help="N = 4 k^3")
+    ap.add_argument("--fcc-cells", type=int, default=None, help="N = 4 k^3 (100; quench 160)")
     ap.add_argument("--density", type=float, default=0.5)
     ap.add_argument("--dt", type=float, default=None, help="0.001 (single), 0.005 (ensemble)")
     ap.add_argument("--rc", type=float, default=2.5)
@@ -284,12 +378,25 @@ def main():
         return
     if args.dt is None:
         args.dt = 0.001
+    if args.workload == "quench":
+        if args.fcc_cells is None:
+            args.fcc_cells = 160
+        if args.steps == 2000:
+            args.steps = 500
+        if args.drift_steps == 10000:
+            args.drift_steps = 2000
+        main_quench(args)
+        return
+    if args.fcc_cells is None:
+        args.fcc_cells = 100
 
     from mdacp_amd import mdgpu
     from mdacp_amd.lattice import fcc, initial_velocity
 
+    from mdacp_amd.ensemble import box_for
+
     k = args.fcc_cells
-    L = k * (4.0 / args.density) ** (1.0 / 3.0)
+    L = box_for(k, args.density)
     x = fcc(L, args.density)
     n = len(x)
     p = initial_velocity(n, math.sqrt(3.0), 0)

@@ -183,8 +183,11 @@ int mdgpu_get_timing(const mdgpu_domain *d, int *force_launches, double *force_m
 
 /* tuning knobs (no counterpart in the reference; results do not depend on
    them beyond floating-point summation order) */
-#define MDGPU_OPT_UNROLL 1   /* partner entries per batch in the force kernels: 4 or 8 */
-#define MDGPU_OPT_KAHAN 2    /* mixed precision: 1 = compensated FP32 sums, 0 = plain */
+#define MDGPU_OPT_UNROLL 1   /* list entries whose indices the force kernels load ahead of
+                                the 8 being computed: 8, 16 or 32 */
+#define MDGPU_OPT_KAHAN 2    /* mixed precision row sums: 2 = FP32 sums of each 8 list
+                                entries added in FP64 (default), 1 = compensated FP32
+                                (Kahan), 0 = plain FP32 */
 #define MDGPU_OPT_WAVES 3    /* wavefronts per cell-brick workgroup: 2, 4, 8 or 16 */
 #define MDGPU_OPT_LDS_KB 4   /* LDS budget for one brick's staged partners in the force and
                                 observer kernels, KiB (>= 8); bricks with more staged

@@ -6,15 +6,24 @@ search best with 16 wavefronts per brick and candidates staged in LDS, while
 an ensemble of small replicas (~200 rows per brick, neighbourhoods of 2-4
 thousand particles) does best with 4 wavefronts reading candidates from L2;
 the pair-force kernel's LDS budget sets how many brick workgroups share a CU.
-autotune() times short runs of the caller's own workload on a clone (the
-caller's trajectory is not touched) and sets the fastest shape on the caller:
+autotune() times short runs of the caller's own workload on a clone and sets
+the fastest shape on the caller:
 
+- cell bricks of a single domain: edge {4, 3, 2} cells, by the force kernel's
+  time (an ensemble sizes each replica's bricks from its density); a new
+  brick edge rebuilds the caller's list at its next step, which reorders the
+  summation of its kicks, so only this choice changes the caller's bits;
 - neighbour search: wavefronts per brick {4, 8, 16} x LDS staging {on, off},
   by the time of a pair-list construction;
 - pair force: wavefronts per brick {4, 8} x LDS budget {48 ... 96} KiB x
-  partner batch {4, 8}, by the force kernel's time (HIP events); the budget
+  list indices in flight {8, 16, 32}, by the force kernel's time (HIP events); the budget
   decides how many brick workgroups share a CU's 160 KiB and which bricks
   read their partners from global memory instead;
+A candidate replaces the caller's current setting only if it is more than
+2% faster, so timing noise does not move a tuned domain.  Tune on a state
+like the one to be run (after equilibration: a perfect lattice has uniform
+cells and times differently).
+
 - optionally the Verlet margin (the rebuild cadence): each candidate margin's
   time per step including its rebuilds.  A different margin lists different
   pairs than the reference's configuration would, so the margin is only
@@ -27,7 +36,7 @@ import time
 from . import mdgpu
 
 SEARCH = [(w, s) for w in (4, 8, 16) for s in (1, 0)]
-FORCE = [(w, kb, u) for w in (4, 8) for kb in (48, 56, 64, 72, 76, 96) for u in (8, 4)]
+FORCE = [(w, kb, u) for w in (4, 8) for kb in (48, 56, 64, 72, 76, 96) for u in (8, 16, 32)]
 
 
 def _rebuild_ms(d, reps=3):
@@ -46,32 +55,64 @@ def _force_ms(d, steps):
     return tm["force_ms"] / max(1, tm["force_launches"])
 
 
+KEEP = 0.98  # a candidate must take less than this fraction of the current setting's time
+
+
+def _pick(cands, times, current):
+    """The fastest candidate, or the current one unless beaten by more than 2%."""
+    best = min(cands, key=lambda k: times[k])
+    if current in times and times[best] > KEEP * times[current]:
+        return current
+    return best
+
+
 def autotune(d, steps=30, margins=None, log=None):
     """Tune d's launch options; returns a dict of the choices and the timings behind them."""
     say = log or (lambda *a: None)
     c = d.clone()
     c.calculate(5)
     res = {"search": {}, "force": {}}
+    brick = None
+    if not hasattr(d, "R"):
+        res["brick"] = {}
+        bt = {}
+        cur_b = d.get_option(mdgpu.OPT_BRICK) & 0xff
+        for b in (4, 3, 2):
+            c.set_option(mdgpu.OPT_BRICK, b | (b << 8) | (b << 16))
+            c.make_pair_list()
+            _force_ms(c, 3)
+            res["brick"][f"{b}"] = bt[b] = _force_ms(c, steps)
+            say(f"brick {b}: force {bt[b]:.4f} ms")
+        brick = _pick((4, 3, 2), bt, cur_b)
+        c.set_option(mdgpu.OPT_BRICK, brick | (brick << 8) | (brick << 16))
+        c.make_pair_list()
+    st = {}
     for w, s in SEARCH:
         c.set_option(mdgpu.OPT_SEARCH_WAVES, w)
         c.set_option(mdgpu.OPT_SEARCH_STAGE, s)
-        res["search"][f"waves={w} stage={s}"] = t = _rebuild_ms(c)
-        say(f"search waves={w} stage={s}: {t:.3f} ms per construction")
-    best_s = min(SEARCH, key=lambda k: res["search"][f"waves={k[0]} stage={k[1]}"])
+        res["search"][f"waves={w} stage={s}"] = st[(w, s)] = _rebuild_ms(c)
+        say(f"search waves={w} stage={s}: {st[(w, s)]:.3f} ms per construction")
+    best_s = _pick(SEARCH, st, (d.get_option(mdgpu.OPT_SEARCH_WAVES), d.get_option(mdgpu.OPT_SEARCH_STAGE)))
     c.set_option(mdgpu.OPT_SEARCH_WAVES, best_s[0])
     c.set_option(mdgpu.OPT_SEARCH_STAGE, best_s[1])
     c.make_pair_list()
+    ft = {}
     for w, kb, u in FORCE:
         c.set_option(mdgpu.OPT_WAVES, w)
         c.set_option(mdgpu.OPT_LDS_KB, kb)
         c.set_option(mdgpu.OPT_UNROLL, u)
         _force_ms(c, 3)
-        res["force"][f"waves={w} lds_kb={kb} unroll={u}"] = t = _force_ms(c, steps)
-        say(f"force waves={w} lds_kb={kb} unroll={u}: {t:.4f} ms")
-    best_f = min(FORCE, key=lambda k: res["force"][f"waves={k[0]} lds_kb={k[1]} unroll={k[2]}"])
+        res["force"][f"waves={w} lds_kb={kb} unroll={u}"] = ft[(w, kb, u)] = _force_ms(c, steps)
+        say(f"force waves={w} lds_kb={kb} unroll={u}: {ft[(w, kb, u)]:.4f} ms")
+    best_f = _pick(FORCE, ft, (d.get_option(mdgpu.OPT_WAVES), d.get_option(mdgpu.OPT_LDS_KB),
+                               d.get_option(mdgpu.OPT_UNROLL)))
     c.close()
     choice = dict(search_waves=best_s[0], search_stage=best_s[1], waves=best_f[0], lds_kb=best_f[1],
                   unroll=best_f[2])
+    if brick is not None:
+        choice["brick"] = brick
+        if d.get_option(mdgpu.OPT_BRICK) != brick | (brick << 8) | (brick << 16):
+            d.set_option(mdgpu.OPT_BRICK, brick | (brick << 8) | (brick << 16))
     d.set_option(mdgpu.OPT_SEARCH_WAVES, choice["search_waves"])
     d.set_option(mdgpu.OPT_SEARCH_STAGE, choice["search_stage"])
     d.set_option(mdgpu.OPT_WAVES, choice["waves"])

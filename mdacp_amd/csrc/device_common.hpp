@@ -10,12 +10,19 @@ constexpr int kBlock = 256;    // 4 waves per workgroup
 constexpr int kXcds = 8;       // MI355X: 8 XCDs, workgroups dispatched round-robin
 constexpr int kMaxParticles = (1 << 26) - 2;        // owned + ghosts + sentinel per domain
 constexpr uint32_t kZeroOffset = 0x15u;             // cell offset (0,0,0): 01 01 01
-constexpr int kSliceRound = 8;                      // slice lengths are multiples of this
-constexpr int kListTail = 2 * kSliceRound * kWave;  // readable padding after the list arrays
+constexpr int kGroup = 8;                           // list slots a lane holds contiguously
+constexpr int kSliceRound = kGroup;                 // slice lengths are multiples of this
+constexpr int kMaxAhead = 4;                        // list-index groups the force kernels read ahead
+constexpr int kListTail = kMaxAhead * kGroup * kWave;  // readable padding after the list arrays
 
-// Position of slot k of lane `lane` inside a 64-row slice of the list:
-// slot-major, so slot k of the slice's 64 rows is one contiguous 128-byte run.
-__host__ __device__ __forceinline__ long long slot_of(int k, int lane) { return (long long)k * kWave + lane; }
+// Position of slot k of lane `lane` inside a 64-row slice of the list.  Slots
+// come in groups of kGroup: a lane's kGroup consecutive slots are contiguous
+// (16 bytes of uint16 indices, 8 bytes of codes) and the 64 lanes' groups
+// follow one another, so group g of a slice is one contiguous run that a
+// wavefront loads with one 16-byte (8-byte) load per lane.
+__host__ __device__ __forceinline__ long long slot_of(int k, int lane) {
+  return (long long)(k / kGroup) * (kGroup * kWave) + lane * kGroup + (k % kGroup);
+}
 
 // Error bits raised by kernels; read back at the existing sync points.
 constexpr int kErrInvalidPosition = 1;

@@ -66,26 +66,42 @@ __device__ __forceinline__ double rcp64(double x) {
   return fma(r, e, r);
 }
 
+// The 8 staged indices of one group of a lane's list slots (one 16-byte load).
+__device__ __forceinline__ void unpack_group(const uint4 g, int e[kGroup]) {
+  e[0] = g.x & 0xffff; e[1] = g.x >> 16; e[2] = g.y & 0xffff; e[3] = g.y >> 16;
+  e[4] = g.z & 0xffff; e[5] = g.z >> 16; e[6] = g.w & 0xffff; e[7] = g.w >> 16;
+}
+
+__device__ __forceinline__ void unpack_codes(const uint2 g, int c[kGroup]) {
+#pragma unroll
+  for (int u = 0; u < 4; u++) {
+    c[u] = (g.x >> (8 * u)) & 0xff;
+    c[u + 4] = (g.y >> (8 * u)) & 0xff;
+  }
+}
+
 // The FP64 kick of one row (one lane of a 64-row slice) over its list
-// entries, U per batch; the next batch's indices are loaded while the current
-// batch is computed.  Partners come from LDS (kStaged) or through the gather
-// list from global memory.  Index loads are unconditional: reading up to U
-// slots past the slice stays inside the next slice or the padding the list
-// arrays carry at their end (kListTail), and those entries are never used.
-// The cutoff is a select, not a branch (padding entries point at a far
-// sentinel).
-template <bool kStaged, int U>
-__device__ __forceinline__ double3 row_kick_fp64(const uint16_t *__restrict__ row, int len, const double4 qi,
+// entries, one group of kGroup per batch.  `rg` points at the lane's first
+// group (group g at rg[g * kWave]); the indices of the next P groups are in
+// flight while a group is computed (P <= kMaxAhead).  Reading up to P groups past the slice
+// stays inside the next slice or the padding the list arrays carry at their
+// end (kListTail), and those entries are never used.  Partners come from LDS
+// (kStaged) or through the gather list from global memory.  The cutoff is a
+// select, not a branch (padding entries point at a far sentinel).
+template <bool kStaged, int P>
+__device__ __forceinline__ double3 row_kick_fp64(const uint4 *__restrict__ rg, int len, const double4 qi,
                                                  const double *__restrict__ lds, const double4 *__restrict__ src,
                                                  const int *__restrict__ gl, int total, const ForceConsts &fc) {
   double ax = 0.0, ay = 0.0, az = 0.0;
-  int e[U];
+  uint4 ahead[P];
 #pragma unroll
-  for (int u = 0; u < U; u++) e[u] = row[slot_of(u, 0)];
-  for (int k = 0; k < len; k += U) {
-    double qx[U], qy[U], qz[U];
+  for (int j = 0; j < P; j++) ahead[j] = rg[j * kWave];
+  for (int k = 0; k < len; k += kGroup) {
+    int e[kGroup];
+    unpack_group(ahead[0], e);
+    double qx[kGroup], qy[kGroup], qz[kGroup];
 #pragma unroll
-    for (int u = 0; u < U; u++) {
+    for (int u = 0; u < kGroup; u++) {
       if (kStaged) {
         const double *q = lds + 3 * e[u];
         qx[u] = q[0];
@@ -101,9 +117,10 @@ __device__ __forceinline__ double3 row_kick_fp64(const uint16_t *__restrict__ ro
       }
     }
 #pragma unroll
-    for (int u = 0; u < U; u++) e[u] = row[slot_of(k + U + u, 0)];
+    for (int j = 0; j + 1 < P; j++) ahead[j] = ahead[j + 1];
+    ahead[P - 1] = rg[(k / kGroup + P) * kWave];
 #pragma unroll
-    for (int u = 0; u < U; u++) {
+    for (int u = 0; u < kGroup; u++) {
       const double dx = qx[u] - qi.x, dy = qy[u] - qi.y, dz = qz[u] - qi.z;
       const double r2 = fma(dx, dx, fma(dy, dy, dz * dz));
       const double ir2 = rcp64(r2);
@@ -118,38 +135,50 @@ __device__ __forceinline__ double3 row_kick_fp64(const uint16_t *__restrict__ ro
   return make_double3(ax, ay, az);
 }
 
+// How the mixed kernel sums a row's FP32 pair kicks.
+enum MixedSum {
+  kSumPlain = 0,    // one FP32 running sum
+  kSumKahan = 1,    // compensated (Kahan) FP32 running sum
+  kSumGroup64 = 2,  // FP32 sum of each group of kGroup kicks, the group sums added in FP64
+};
+
 // Mixed-precision kick of one row: FP32 cell-relative partner coordinates,
-// FP32 pair forces, sums compensated (kKahan) or plain; same batching and
-// loads as row_kick_fp64.  acc = (sum, compensation) per component.
-template <bool kStaged, bool kKahan, int U>
-__device__ __forceinline__ void row_kick_mixed(const uint16_t *__restrict__ row, const uint8_t *__restrict__ crow,
-                                               int len, const float4 ri, const float4 *__restrict__ lds,
-                                               const float4 *__restrict__ offs, const float4 *__restrict__ rel,
-                                               const int *__restrict__ gl, int total, const ForceConstsF &fc,
-                                               float (&acc)[6]) {
+// FP32 pair forces, summed as kSum says; same groups and loads as
+// row_kick_fp64 (`cg`: the lane's groups of cell-offset codes).
+template <bool kStaged, int kSum, int P>
+__device__ __forceinline__ double3 row_kick_mixed(const uint4 *__restrict__ rg, const uint2 *__restrict__ cg,
+                                                  int len, const float4 ri, const float4 *__restrict__ lds,
+                                                  const float4 *__restrict__ offs, const float4 *__restrict__ rel,
+                                                  const int *__restrict__ gl, int total, const ForceConstsF &fc) {
   float ax = 0.f, ay = 0.f, az = 0.f, cx = 0.f, cy = 0.f, cz = 0.f;
-  int e[U], c[U];
+  double sx = 0.0, sy = 0.0, sz = 0.0;
+  uint4 ahead[P];
+  uint2 cahead[P];
 #pragma unroll
-  for (int u = 0; u < U; u++) {
-    e[u] = row[slot_of(u, 0)];
-    c[u] = crow[slot_of(u, 0)];
+  for (int j = 0; j < P; j++) {
+    ahead[j] = rg[j * kWave];
+    cahead[j] = cg[j * kWave];
   }
-  for (int k = 0; k < len; k += U) {
-    float4 rj[U];
-    int cc[U];
+  for (int k = 0; k < len; k += kGroup) {
+    int e[kGroup], cc[kGroup];
+    unpack_group(ahead[0], e);
+    unpack_codes(cahead[0], cc);
+    float4 rj[kGroup];
 #pragma unroll
-    for (int u = 0; u < U; u++) {
+    for (int u = 0; u < kGroup; u++) {
       if (kStaged) rj[u] = lds[e[u]];
       else rj[u] = e[u] < total ? rel[gl[e[u]]] : make_float4(1.0e30f, 1.0e30f, 1.0e30f, 0.0f);
-      cc[u] = c[u];
     }
 #pragma unroll
-    for (int u = 0; u < U; u++) {
-      e[u] = row[slot_of(k + U + u, 0)];
-      c[u] = crow[slot_of(k + U + u, 0)];
+    for (int j = 0; j + 1 < P; j++) {
+      ahead[j] = ahead[j + 1];
+      cahead[j] = cahead[j + 1];
     }
+    ahead[P - 1] = rg[(k / kGroup + P) * kWave];
+    cahead[P - 1] = cg[(k / kGroup + P) * kWave];
+    if (kSum == kSumGroup64) ax = ay = az = 0.f;
 #pragma unroll
-    for (int u = 0; u < U; u++) {
+    for (int u = 0; u < kGroup; u++) {
       const float4 o = offs[cc[u]];
       const float dx = (rj[u].x - ri.x) + o.x, dy = (rj[u].y - ri.y) + o.y, dz = (rj[u].z - ri.z) + o.z;
       const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
@@ -157,7 +186,7 @@ __device__ __forceinline__ void row_kick_mixed(const uint16_t *__restrict__ row,
       const float ir6 = ir2 * ir2 * ir2;
       float df = fmaf(ir2 * ir6, fmaf(-fc.B, ir6, fc.A), fc.C);
       df = (r2 <= fc.CL2) ? df : 0.0f;
-      if (kKahan) {
+      if (kSum == kSumKahan) {
         float y, q;
         y = fmaf(df, dx, -cx); q = ax + y; cx = (q - ax) - y; ax = q;
         y = fmaf(df, dy, -cy); q = ay + y; cy = (q - ay) - y; ay = q;
@@ -168,9 +197,14 @@ __device__ __forceinline__ void row_kick_mixed(const uint16_t *__restrict__ row,
         az = fmaf(df, dz, az);
       }
     }
+    if (kSum == kSumGroup64) {
+      sx += (double)ax;
+      sy += (double)ay;
+      sz += (double)az;
+    }
   }
-  acc[0] = ax; acc[1] = ay; acc[2] = az;
-  acc[3] = cx; acc[4] = cy; acc[5] = cz;
+  if (kSum == kSumGroup64) return make_double3(sx, sy, sz);
+  return make_double3((double)ax - (double)cx, (double)ay - (double)cy, (double)az - (double)cz);
 }
 
 // Where the epilogue also writes the next step's first half drift (q + p dt/2
@@ -209,7 +243,8 @@ __device__ __forceinline__ double finish_row(int i, double4 p, const double4 qi,
   return p.x * p.x + p.y * p.y + p.z * p.z;
 }
 
-// FP64 kick: one workgroup per brick, wavefronts over the brick's slices.
+// FP64 kick: one workgroup per brick, wavefronts over the brick's slices;
+// U groups of list indices in flight ahead of the group computed.
 template <int W, int U>
 __global__ __launch_bounds__(kWave *W) void k_force_brick_fp64(const double4 *__restrict__ src,
                                                               double4 *__restrict__ mom,
@@ -217,7 +252,7 @@ __global__ __launch_bounds__(kWave *W) void k_force_brick_fp64(const double4 *__
                                                               ForceConsts fc, int epilogue, StepClock sc,
                                                               const int *__restrict__ id,
                                                               double *__restrict__ vpart, NextDrift nd) {
-  extern __shared__ double lds64[];
+  extern __shared__ __align__(16) double lds64[];
   constexpr int T = kWave * W;
   const int b = xcd_block(blockIdx.x, gridDim.x);
   const Replica &Rp = bl.reps[bl.brep[b]];
@@ -243,9 +278,9 @@ __global__ __launch_bounds__(kWave *W) void k_force_brick_fp64(const double4 *__
     const int i = row0 + (ok ? v : nrow - 1);
     const double4 qi = src[i];
     const int len = bl.slen[s];
-    const uint16_t *row = bl.nbr + bl.sbase[s] + lane;
-    const double3 a = staged ? row_kick_fp64<true, U>(row, len, qi, lds64, src, gl, total, fc)
-                             : row_kick_fp64<false, U>(row, len, qi, lds64, src, gl, total, fc);
+    const uint4 *rg = reinterpret_cast<const uint4 *>(bl.nbr + bl.sbase[s] + lane * kGroup);
+    const double3 a = staged ? row_kick_fp64<true, U>(rg, len, qi, lds64, src, gl, total, fc)
+                             : row_kick_fp64<false, U>(rg, len, qi, lds64, src, gl, total, fc);
     if (ok) {
       double4 p = mom[i];
       p.x += a.x;
@@ -263,7 +298,7 @@ __global__ __launch_bounds__(kWave *W) void k_force_brick_fp64(const double4 *__
 // the code array, so dx = (rel_j - rel_i) + off*w stays small.  Ghost images
 // carry coordinates relative to the extended-grid cell they are binned in,
 // like owned particles.
-template <int W, int U, bool kKahan>
+template <int W, int U, int kSum>
 __global__ __launch_bounds__(kWave *W) void k_force_brick_mixed(
     const float4 *__restrict__ rel, const double4 *__restrict__ pos, double4 *__restrict__ mom,
     double4 *__restrict__ pos_out, BrickList bl, ForceConstsF fc, double dt2, int epilogue, StepClock sc,
@@ -317,16 +352,15 @@ __global__ __launch_bounds__(kWave *W) void k_force_brick_mixed(
     const int i = row0 + (ok ? v : nrow - 1);
     const float4 ri = rel[i];
     const int len = bl.slen[s];
-    const uint16_t *row = bl.nbr + bl.sbase[s] + lane;
-    const uint8_t *crow = bl.code + bl.sbase[s] + lane;
-    float acc[6];
-    if (staged) row_kick_mixed<true, kKahan, U>(row, crow, len, ri, lds32, offs, rel, gl, total, fc, acc);
-    else row_kick_mixed<false, kKahan, U>(row, crow, len, ri, lds32, offs, rel, gl, total, fc, acc);
+    const uint4 *rg = reinterpret_cast<const uint4 *>(bl.nbr + bl.sbase[s] + lane * kGroup);
+    const uint2 *cg = reinterpret_cast<const uint2 *>(bl.code + bl.sbase[s] + lane * kGroup);
+    const double3 a = staged ? row_kick_mixed<true, kSum, U>(rg, cg, len, ri, lds32, offs, rel, gl, total, fc)
+                             : row_kick_mixed<false, kSum, U>(rg, cg, len, ri, lds32, offs, rel, gl, total, fc);
     if (ok) {
       double4 p = mom[i];
-      p.x += (double)acc[0] - (double)acc[3];
-      p.y += (double)acc[1] - (double)acc[4];
-      p.z += (double)acc[2] - (double)acc[5];
+      p.x += a.x;
+      p.y += a.y;
+      p.z += a.z;
       vmax = fmax(vmax, finish_row(i, p, pos[i], mom, pos_out, epilogue, dt2, lc, id, nd, Rp));
     }
   }
@@ -346,7 +380,7 @@ __global__ __launch_bounds__(kWave *W) void k_observe_brick(const double4 *__res
                                                           const double4 *__restrict__ mom, BrickList bl,
                                                           double CL2, double C2, double C0,
                                                           double *__restrict__ partial) {
-  extern __shared__ double lds64[];
+  extern __shared__ __align__(16) double lds64[];
   __shared__ double red[3][W];
   constexpr int T = kWave * W;
   const int b = xcd_block(blockIdx.x, gridDim.x);

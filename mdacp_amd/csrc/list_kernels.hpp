@@ -263,7 +263,7 @@ __global__ __launch_bounds__(kWave *W) void k_search_brick(const double4 *__rest
                                                           SearchTables st, double S2, int mcap,
                                                           int *__restrict__ row_len, uint32_t *__restrict__ mask,
                                                           int *__restrict__ err) {
-  extern __shared__ double lds64[];
+  extern __shared__ __align__(16) double lds64[];
   __shared__ int tb[kMaxRanges + 1];
   constexpr int T = kWave * W;
   const int b = xcd_block(blockIdx.x, gridDim.x);

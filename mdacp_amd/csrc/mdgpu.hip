@@ -50,10 +50,11 @@ struct mdgpu_domain {
   int precision = MDGPU_FP64;
 
   // tuning
-  int kahan = 1;          // compensated FP32 sums in the mixed kernel
-  int unroll = 8;         // partner entries per batch in the force kernels (4 or 8)
+  int kahan = 2;          // mixed kernel's row sums: 0 plain FP32, 1 Kahan FP32, 2 FP32 groups summed in FP64
+  int unroll = 8;         // list entries whose indices are loaded ahead in the force kernels (8, 16 or 32)
   int waves = 8;          // wavefronts per brick workgroup: 2, 4, 8 or 16
   int bd[3] = {4, 4, 4};  // cells per brick
+  bool brick_auto = true; // a single domain picks bd from its density at the first construction
   int lds_kb = 96;        // LDS budget for one brick's staged partners, KiB
   int lds_max_kb = 60;    // device limit per workgroup less the kernels' static LDS
   int search_waves = 16;  // wavefronts per brick workgroup of the neighbour search: 4, 8 or 16
@@ -623,12 +624,42 @@ int order_rows(mdgpu_domain *d) {
   return check_launch(d, "row order");
 }
 
+// Cells per brick for a box of density rho (cubic bricks): the largest edge
+// whose staged neighbourhood (bd + 2 cells per axis, FP64 entries of 24
+// bytes) is expected to fill at most 90% of kb KiB of LDS.
+int auto_brick(const mdgpu_domain *d, double rho, const double L[3], int kb) {
+  const double budget = 0.9 * std::min(kb, d->lds_max_kb) * 1024.0;
+  double w[3];
+  for (int k = 0; k < 3; k++) w[k] = L[k] / std::max(1, (int)(L[k] / d->SL));
+  for (int b = 4; b > 1; b--) {
+    const double staged = rho * (b + 2) * w[0] * (b + 2) * w[1] * (b + 2) * w[2];
+    if (staged * 24.0 <= budget) return b;
+  }
+  return 1;
+}
+
+// Ensemble replicas: bricks for two workgroups per CU (160 KiB of LDS).
+constexpr int kReplicaBrickKb = 76;
+// A single domain, unless the caller chose its bricks: the largest bricks the
+// force kernels stage within the default LDS budget, chosen at the first
+// construction from the density.
+constexpr int kDomainBrickKb = 96;
+
 // MDManager::MakePairList for the single domain (mdmanager.cc:262-280).
 // The reference also permutes particles that cross a face
 // (MakeBufferForSendingParticle, mdunit.cc:78-101); that only renames
 // indices, and particles are put in brick order here anyway.
 int make_pair_list(mdgpu_domain *d) {
   hipStream_t st = d->stream;
+  if (d->nrep == 1 && d->brick_auto && d->pn > 0) {
+    d->brick_auto = false;
+    const double rho = d->pn / (d->rh[0].L[0] * d->rh[0].L[1] * d->rh[0].L[2]);
+    const int b = auto_brick(d, rho, d->rh[0].L, kDomainBrickKb);
+    if (b != d->bd[0] || b != d->bd[1] || b != d->bd[2]) {
+      for (int k = 0; k < 3; k++) d->bd[k] = b;
+      CHK(set_geometry(d));
+    }
+  }
   invalidate(d);  // particles are reordered and wrapped
   const int pn = d->pn;
   d->ng = 0;
@@ -966,7 +997,7 @@ void launch_fp64(mdgpu_domain *d, const double4 *src, double4 *pos_out, int epi,
                      pos_out, bl, fc, epi, sc, d->id, vb, nd);
 }
 
-template <int W, int U, bool K>
+template <int W, int U, int K>
 void launch_mixed(mdgpu_domain *d, const double4 *src, const float4 *rel, double4 *pos_out, int epi,
                   const ForceConstsF &f32, double dt2, const StepClock &sc, double *vb, const NextDrift &nd) {
   const BrickList bl = brick_list(d, sizeof(float4));
@@ -976,19 +1007,29 @@ void launch_mixed(mdgpu_domain *d, const double4 *src, const float4 *rel, double
                      d->mom, pos_out, bl, f32, dt2, epi, sc, d->id, vb, nd);
 }
 
+template <int W, int K>
+void launch_mixed_sum(mdgpu_domain *d, const double4 *src, const float4 *rel, double4 *pos_out, int epi,
+                      const ForceConstsF &f32, double dt2, const StepClock &sc, double *vb, const NextDrift &nd) {
+  if (d->unroll == 32) launch_mixed<W, 4, K>(d, src, rel, pos_out, epi, f32, dt2, sc, vb, nd);
+  else if (d->unroll == 16) launch_mixed<W, 2, K>(d, src, rel, pos_out, epi, f32, dt2, sc, vb, nd);
+  else launch_mixed<W, 1, K>(d, src, rel, pos_out, epi, f32, dt2, sc, vb, nd);
+}
+
 template <int W>
 void launch_force_w(mdgpu_domain *d, const double4 *src, const float4 *rel, double4 *pos_out, int epi,
                     const ForceConsts &fc, const ForceConstsF &f32, const StepClock &sc, double *vb,
                     const NextDrift &nd) {
+  // groups of kGroup list indices in flight: unroll / kGroup (1, 2 or 4)
   if (d->precision == MDGPU_FP64) {
-    if (d->unroll == 4) launch_fp64<W, 4>(d, src, pos_out, epi, fc, sc, vb, nd);
-    else launch_fp64<W, 8>(d, src, pos_out, epi, fc, sc, vb, nd);
-  } else if (d->kahan) {
-    if (d->unroll == 4) launch_mixed<W, 4, true>(d, src, rel, pos_out, epi, f32, fc.dt2, sc, vb, nd);
-    else launch_mixed<W, 8, true>(d, src, rel, pos_out, epi, f32, fc.dt2, sc, vb, nd);
+    if (d->unroll == 32) launch_fp64<W, 4>(d, src, pos_out, epi, fc, sc, vb, nd);
+    else if (d->unroll == 16) launch_fp64<W, 2>(d, src, pos_out, epi, fc, sc, vb, nd);
+    else launch_fp64<W, 1>(d, src, pos_out, epi, fc, sc, vb, nd);
   } else {
-    if (d->unroll == 4) launch_mixed<W, 4, false>(d, src, rel, pos_out, epi, f32, fc.dt2, sc, vb, nd);
-    else launch_mixed<W, 8, false>(d, src, rel, pos_out, epi, f32, fc.dt2, sc, vb, nd);
+    switch (d->kahan) {
+      case kSumPlain: launch_mixed_sum<W, kSumPlain>(d, src, rel, pos_out, epi, f32, fc.dt2, sc, vb, nd); break;
+      case kSumKahan: launch_mixed_sum<W, kSumKahan>(d, src, rel, pos_out, epi, f32, fc.dt2, sc, vb, nd); break;
+      default: launch_mixed_sum<W, kSumGroup64>(d, src, rel, pos_out, epi, f32, fc.dt2, sc, vb, nd); break;
+    }
   }
 }
 
@@ -1253,21 +1294,6 @@ void launch_observe(mdgpu_domain *d, const BrickList &bl, size_t shm) {
 // ======================================================================= C ABI
 namespace {
 
-// Cells per brick for a replica of density rho (cubic bricks): the largest
-// edge whose staged neighbourhood (bd + 2 cells per axis, FP64 entries of 24
-// bytes) is expected to fill at most 90% of 76 KiB, so that the force kernels
-// stage it with two brick workgroups per CU (160 KiB of LDS).
-int auto_brick(const mdgpu_domain *d, double rho, const double L[3]) {
-  const double budget = 0.9 * std::min(76, d->lds_max_kb) * 1024.0;
-  double w[3];
-  for (int k = 0; k < 3; k++) w[k] = L[k] / std::max(1, (int)(L[k] / d->SL));
-  for (int b = 4; b > 1; b--) {
-    const double staged = rho * (b + 2) * w[0] * (b + 2) * w[1] * (b + 2) * w[2];
-    if (staged * 24.0 <= budget) return b;
-  }
-  return 1;
-}
-
 // A domain of nrep replicas with the common parameters of prm (dt, rc,
 // margin, check list length, precision, device); rl[r] gives replica r's box
 // and Langevin parameters (nullptr: one replica with box prm->L).
@@ -1322,7 +1348,7 @@ int create_domain(const mdgpu_params *prm, int nrep, const mdgpu_replica *rl, co
       d->rst[r].gamma = rl[r].gamma;
       d->rst[r].seed = rl[r].seed;
       const double rho = counts[r] / (R.L[0] * R.L[1] * R.L[2]);
-      const int b = auto_brick(d, rho, R.L);
+      const int b = auto_brick(d, rho, R.L, kReplicaBrickKb);
       for (int k = 0; k < 3; k++) R.bd[k] = b;
       R.p0 = (int)p0;
       R.pn = counts[r];
@@ -1974,11 +2000,12 @@ int mdgpu_set_option(mdgpu_domain *d, int option, int value) {
   invalidate(d);  // the next fused step recomputes its first half drift
   switch (option) {
     case MDGPU_OPT_UNROLL:
-      if (value != 4 && value != 8) return MDGPU_ERR_ARGUMENT;
+      if (value != 8 && value != 16 && value != 32) return MDGPU_ERR_ARGUMENT;
       d->unroll = value;
       return MDGPU_OK;
     case MDGPU_OPT_KAHAN:
-      d->kahan = value != 0;
+      if (value < kSumPlain || value > kSumGroup64) return MDGPU_ERR_ARGUMENT;
+      d->kahan = value;
       return MDGPU_OK;
     case MDGPU_OPT_WAVES:
       if (value != 2 && value != 4 && value != 8 && value != 16) return MDGPU_ERR_ARGUMENT;
@@ -1991,6 +2018,7 @@ int mdgpu_set_option(mdgpu_domain *d, int option, int value) {
     case MDGPU_OPT_BRICK: {
       const int bx = value & 0xff, by = (value >> 8) & 0xff, bz = (value >> 16) & 0xff;
       if (bx < 1 || bx > 4 || by < 1 || by > 4 || bz < 1 || bz > 4) return MDGPU_ERR_ARGUMENT;
+      d->brick_auto = false;
       d->bd[0] = bx;
       d->bd[1] = by;
       d->bd[2] = bz;

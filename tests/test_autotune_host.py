@@ -1,0 +1,78 @@
+"""CPU checks of the autotuner's selection logic (mdacp_amd.autotune) against a
+mock domain whose kernel times are a known function of the options."""
+from mdacp_amd import autotune as A
+from mdacp_amd import mdgpu
+
+
+class MockDomain:
+    """Times: rebuild fastest at search (8, 0); force fastest at waves 4, 72 KiB, unroll 16, brick 3."""
+
+    def __init__(self, ensemble=False):
+        self.opts = {mdgpu.OPT_SEARCH_WAVES: 16, mdgpu.OPT_SEARCH_STAGE: 1, mdgpu.OPT_WAVES: 8,
+                     mdgpu.OPT_LDS_KB: 96, mdgpu.OPT_UNROLL: 8, mdgpu.OPT_BRICK: 4 | 4 << 8 | 4 << 16}
+        self.clones = []
+        self.closed = False
+        if ensemble:
+            self.R = 3
+
+    def clone(self, **kw):
+        c = MockDomain(hasattr(self, "R"))
+        c.opts = dict(self.opts)
+        self.clones.append(c)
+        return c
+
+    def set_option(self, o, v):
+        self.opts[o] = v
+
+    def get_option(self, o):
+        return self.opts[o]
+
+    def calculate(self, n):
+        return 0
+
+    def make_pair_list(self):
+        pass
+
+    def enable_timing(self, on):
+        pass
+
+    def timing(self):
+        o = self.opts
+        b = o[mdgpu.OPT_BRICK] & 0xff
+        t = 1.0 + 0.1 * abs(b - 3) + 0.05 * (o[mdgpu.OPT_WAVES] != 4) + 0.01 * abs(o[mdgpu.OPT_LDS_KB] - 72) / 8 \
+            + 0.02 * (o[mdgpu.OPT_UNROLL] != 16)
+        return dict(force_ms=t * 10, force_launches=10, total_ms=0.0, rebuilds=0, rebuild_ms=0.0)
+
+    def close(self):
+        self.closed = True
+
+
+def test_picks_the_fastest_shapes(monkeypatch):
+    times = {}
+
+    def fake_rebuild(d, reps=3):
+        w, s = d.get_option(mdgpu.OPT_SEARCH_WAVES), d.get_option(mdgpu.OPT_SEARCH_STAGE)
+        return times.setdefault((w, s), 5.0 + abs(w - 8) * 0.1 + s * 0.3)
+
+    monkeypatch.setattr(A, "_rebuild_ms", fake_rebuild)
+    d = MockDomain()
+    res = A.autotune(d, steps=2)
+    ch = res["choice"]
+    assert (ch["search_waves"], ch["search_stage"]) == (8, 0)
+    assert (ch["waves"], ch["lds_kb"], ch["unroll"]) == (4, 72, 16)
+    assert ch["brick"] == 3
+    # applied to the caller, and the clone was released
+    assert d.opts[mdgpu.OPT_SEARCH_WAVES] == 8 and d.opts[mdgpu.OPT_SEARCH_STAGE] == 0
+    assert d.opts[mdgpu.OPT_WAVES] == 4 and d.opts[mdgpu.OPT_LDS_KB] == 72 and d.opts[mdgpu.OPT_UNROLL] == 16
+    assert d.opts[mdgpu.OPT_BRICK] == 3 | 3 << 8 | 3 << 16
+    assert all(c.closed for c in d.clones)
+    assert set(res["search"]) == {f"waves={w} stage={s}" for w, s in A.SEARCH}
+    assert set(res["force"]) == {f"waves={w} lds_kb={k} unroll={u}" for w, k, u in A.FORCE}
+
+
+def test_ensembles_keep_their_per_replica_bricks(monkeypatch):
+    monkeypatch.setattr(A, "_rebuild_ms", lambda d, reps=3: 1.0)
+    d = MockDomain(ensemble=True)
+    brick0 = d.opts[mdgpu.OPT_BRICK]
+    res = A.autotune(d, steps=2)
+    assert "brick" not in res["choice"] and d.opts[mdgpu.OPT_BRICK] == brick0
