@@ -200,6 +200,10 @@ int mdgpu_get_timing(const mdgpu_domain *d, int *force_launches, double *force_m
 #define MDGPU_OPT_SEARCH_WAVES 7  /* wavefronts per brick in the neighbour search: 4, 8 or 16 */
 #define MDGPU_OPT_SEARCH_STAGE 8  /* neighbour search candidates staged in LDS (1) or read
                                      from global memory (0) */
+#define MDGPU_OPT_PRUNE 9    /* dynamic pruning: the fused step's kick reads an inner list of
+                                the pairs within rc + value/1000, pruned from the full list
+                                whenever the particles could have moved value/1000 since
+                                (0 = off, the default; the same pairs in the same order) */
 int mdgpu_set_option(mdgpu_domain *d, int option, int value);
 int mdgpu_get_option(const mdgpu_domain *d, int option, int *value);
 

@@ -24,6 +24,9 @@ A candidate replaces the caller's current setting only if it is more than
 like the one to be run (after equilibration: a perfect lattice has uniform
 cells and times differently).
 
+- dynamic pruning: the inner-list radius rc + delta, delta in {off, 0.05,
+  0.1, 0.15, 0.2, 0.3}, by the time per step including the pruning passes
+  (the kick sums the same pairs in the same order whatever delta is);
 - optionally the Verlet margin (the rebuild cadence): each candidate margin's
   time per step including its rebuilds.  A different margin lists different
   pairs than the reference's configuration would, so the margin is only
@@ -36,6 +39,7 @@ import time
 from . import mdgpu
 
 SEARCH = [(w, s) for w in (4, 8, 16) for s in (1, 0)]
+PRUNE = [0, 50, 100, 150, 200, 300]  # delta in thousandths
 FORCE = [(w, kb, u) for w in (4, 8) for kb in (48, 56, 64, 72, 76, 96) for u in (8, 16, 32)]
 
 
@@ -106,9 +110,22 @@ def autotune(d, steps=30, margins=None, log=None):
         say(f"force waves={w} lds_kb={kb} unroll={u}: {ft[(w, kb, u)]:.4f} ms")
     best_f = _pick(FORCE, ft, (d.get_option(mdgpu.OPT_WAVES), d.get_option(mdgpu.OPT_LDS_KB),
                                d.get_option(mdgpu.OPT_UNROLL)))
+    c.set_option(mdgpu.OPT_WAVES, best_f[0])
+    c.set_option(mdgpu.OPT_LDS_KB, best_f[1])
+    c.set_option(mdgpu.OPT_UNROLL, best_f[2])
+    res["prune"] = {}
+    pt = {}
+    for m in PRUNE:
+        c.set_option(mdgpu.OPT_PRUNE, m)
+        c.calculate(3)
+        t0 = time.perf_counter()
+        c.calculate(2 * steps)
+        res["prune"][f"{m}"] = pt[m] = 1e3 * (time.perf_counter() - t0) / (2 * steps)
+        say(f"prune delta={m / 1000}: {pt[m]:.4f} ms per step")
+    best_p = _pick(PRUNE, pt, d.get_option(mdgpu.OPT_PRUNE))
     c.close()
     choice = dict(search_waves=best_s[0], search_stage=best_s[1], waves=best_f[0], lds_kb=best_f[1],
-                  unroll=best_f[2])
+                  unroll=best_f[2], prune=best_p)
     if brick is not None:
         choice["brick"] = brick
         if d.get_option(mdgpu.OPT_BRICK) != brick | (brick << 8) | (brick << 16):
@@ -118,6 +135,7 @@ def autotune(d, steps=30, margins=None, log=None):
     d.set_option(mdgpu.OPT_WAVES, choice["waves"])
     d.set_option(mdgpu.OPT_LDS_KB, choice["lds_kb"])
     d.set_option(mdgpu.OPT_UNROLL, choice["unroll"])
+    d.set_option(mdgpu.OPT_PRUNE, choice["prune"])
     res["choice"] = choice
     if margins:
         res["margin"] = {}

@@ -218,11 +218,24 @@ struct NextDrift {
   const double *org;  // column origins of the cell-relative coordinates
 };
 
+// The row's id and cell, loaded with its position before the pair loop so
+// that their latency overlaps it (only what the epilogue needs).
+struct RowExtra {
+  int id, cell;
+};
+
+__device__ __forceinline__ RowExtra row_extra(int i, int epilogue, const int *__restrict__ id, const NextDrift &nd) {
+  RowExtra x;
+  x.id = epilogue == kKickLangevin ? id[i] : 0;
+  x.cell = nd.rel != nullptr ? nd.cell[i] : 0;
+  return x;
+}
+
 __device__ __forceinline__ double finish_row(int i, double4 p, const double4 qi, double4 *__restrict__ mom,
                                              double4 *__restrict__ pos_out, int epilogue, double dt2,
-                                             const LangevinConsts &lc, const int *__restrict__ id,
-                                             const NextDrift &nd, const Replica &R) {
-  if (epilogue == kKickLangevin) langevin_update(p, lc, (uint32_t)id[i]);
+                                             const LangevinConsts &lc, const RowExtra &x, const NextDrift &nd,
+                                             const Replica &R) {
+  if (epilogue == kKickLangevin) langevin_update(p, lc, (uint32_t)x.id);
   mom[i] = p;
   double4 q = qi;
   if (epilogue == kKickDrift) {
@@ -238,7 +251,7 @@ __device__ __forceinline__ double finish_row(int i, double4 p, const double4 qi,
     q.y += p.y * dt2;
     q.z += p.z * dt2;
     nd.pos[i] = q;
-    if (nd.rel != nullptr) nd.rel[i] = to_rel(q, nd.cell[i], nd.org, R);
+    if (nd.rel != nullptr) nd.rel[i] = to_rel(q, x.cell, nd.org, R);
   }
   return p.x * p.x + p.y * p.y + p.z * p.z;
 }
@@ -277,16 +290,17 @@ __global__ __launch_bounds__(kWave *W) void k_force_brick_fp64(const double4 *__
     const bool ok = v < nrow;
     const int i = row0 + (ok ? v : nrow - 1);
     const double4 qi = src[i];
+    double4 p = mom[i];  // loaded before the pair loop, which hides its latency
+    const RowExtra x = row_extra(i, epilogue, id, nd);
     const int len = bl.slen[s];
     const uint4 *rg = reinterpret_cast<const uint4 *>(bl.nbr + bl.sbase[s] + lane * kGroup);
     const double3 a = staged ? row_kick_fp64<true, U>(rg, len, qi, lds64, src, gl, total, fc)
                              : row_kick_fp64<false, U>(rg, len, qi, lds64, src, gl, total, fc);
     if (ok) {
-      double4 p = mom[i];
       p.x += a.x;
       p.y += a.y;
       p.z += a.z;
-      vmax = fmax(vmax, finish_row(i, p, qi, mom, pos_out, epilogue, fc.dt2, lc, id, nd, Rp));
+      vmax = fmax(vmax, finish_row(i, p, qi, mom, pos_out, epilogue, fc.dt2, lc, x, nd, Rp));
     }
   }
   if (vpart) block_max_to_partial(vmax, vpart, b);
@@ -357,14 +371,129 @@ __global__ __launch_bounds__(kWave *W) void k_force_brick_mixed(
     const double3 a = staged ? row_kick_mixed<true, kSum, U>(rg, cg, len, ri, lds32, offs, rel, gl, total, fc)
                              : row_kick_mixed<false, kSum, U>(rg, cg, len, ri, lds32, offs, rel, gl, total, fc);
     if (ok) {
+      // loaded after the pair loop: held across it, they would cost this
+      // kernel a workgroup per CU (registers)
       double4 p = mom[i];
+      const double4 qi = pos[i];
+      const RowExtra x = row_extra(i, epilogue, id, nd);
       p.x += a.x;
       p.y += a.y;
       p.z += a.z;
-      vmax = fmax(vmax, finish_row(i, p, pos[i], mom, pos_out, epilogue, dt2, lc, id, nd, Rp));
+      vmax = fmax(vmax, finish_row(i, p, qi, mom, pos_out, epilogue, dt2, lc, x, nd, Rp));
     }
   }
   if (vpart) block_max_to_partial(vmax, vpart, b);
+}
+
+// Dynamic pruning of the pair list (no counterpart in the reference: it
+// changes which list entries the fused step's kick visits, not which pairs
+// interact).  The entries of each row whose partner lies within rin2 of the
+// row at the current half-drifted positions are copied, in list order, into
+// the inner list, which has the slice layout of the outer list; slen_in[s]
+// receives the slice's inner length (a multiple of kGroup) and the rest of
+// each row up to it is the brick's pad entry.  The host keeps the inner list
+// only while every pair outside it stays beyond rc (the PairList budget
+// argument with rin - rc in place of the margin), so the kick sums the same
+// pairs in the same order.
+__device__ __forceinline__ uint32_t put16(uint32_t w, int half, uint32_t v) {
+  return half ? ((w & 0xffffu) | (v << 16)) : ((w & 0xffff0000u) | v);
+}
+
+__device__ __forceinline__ uint32_t put8(uint32_t w, int byte, uint32_t v) {
+  const int sh = 8 * byte;
+  return (w & ~(0xffu << sh)) | (v << sh);
+}
+
+__device__ __forceinline__ int wave_max_int(int v) {
+  for (int o = kWave / 2; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, kWave));
+  return v;
+}
+
+template <int W>
+__global__ __launch_bounds__(kWave *W) void k_prune_brick(const double4 *__restrict__ src, BrickList bl, double rin2,
+                                                         uint16_t *__restrict__ nbr_in, uint8_t *__restrict__ code_in,
+                                                         int *__restrict__ slen_in) {
+  extern __shared__ __align__(16) double lds64[];
+  constexpr int T = kWave * W;
+  const int b = xcd_block(blockIdx.x, gridDim.x);
+  const int nrow = bl.nrow[b], row0 = bl.row0[b], vs0 = bl.vslice0[b];
+  if (nrow == 0) return;
+  const long long g0 = bl.gbase[b];
+  const int total = (int)(bl.gbase[b + 1] - g0);
+  const int *gl = bl.glist + g0;
+  const bool staged = total < bl.lds_cap;
+  if (staged) stage_fp64<T>(gl, total, src, lds64);
+  __syncthreads();
+  const uint32_t pad16 = (uint32_t)total | ((uint32_t)total << 16);
+  const uint32_t pad8 = kZeroOffset * 0x01010101u;
+  const int nsl = (nrow + kWave - 1) / kWave;
+  const int wv = threadIdx.x / kWave, lane = threadIdx.x % kWave;
+  for (int sl = wv; sl < nsl; sl += W) {
+    const int s = vs0 + sl;
+    const int v = sl * kWave + lane;
+    const bool ok = v < nrow;
+    const int i = row0 + (ok ? v : nrow - 1);
+    const double4 qi = src[i];
+    const int len = bl.slen[s];
+    const long long base = bl.sbase[s] + lane * kGroup;
+    const uint4 *rg = reinterpret_cast<const uint4 *>(bl.nbr + base);
+    const uint2 *cg = reinterpret_cast<const uint2 *>(bl.code + base);
+    uint4 *og = reinterpret_cast<uint4 *>(nbr_in + base);
+    uint2 *oc = reinterpret_cast<uint2 *>(code_in + base);
+    uint32_t w0 = pad16, w1 = pad16, w2 = pad16, w3 = pad16, c0 = pad8, c1 = pad8;
+    int kept = 0;
+    for (int k = 0; k < len; k += kGroup) {
+      int e[kGroup], cc[kGroup];
+      unpack_group(rg[(k / kGroup) * kWave], e);
+      unpack_codes(cg[(k / kGroup) * kWave], cc);
+#pragma unroll
+      for (int u = 0; u < kGroup; u++) {
+        double qx, qy, qz;
+        if (staged) {
+          qx = lds64[3 * e[u]];
+          qy = lds64[3 * e[u] + 1];
+          qz = lds64[3 * e[u] + 2];
+        } else if (e[u] < total) {
+          const double4 q = src[gl[e[u]]];
+          qx = q.x;
+          qy = q.y;
+          qz = q.z;
+        } else {
+          qx = qy = qz = 1.0e200;
+        }
+        const double dx = qx - qi.x, dy = qy - qi.y, dz = qz - qi.z;
+        const double r2 = fma(dx, dx, fma(dy, dy, dz * dz));
+        if (ok && e[u] < total && r2 <= rin2) {
+          const int j = kept % kGroup, wd = j >> 1, hf = j & 1;
+          const uint32_t ev = (uint32_t)e[u], cv = (uint32_t)cc[u];
+          w0 = wd == 0 ? put16(w0, hf, ev) : w0;
+          w1 = wd == 1 ? put16(w1, hf, ev) : w1;
+          w2 = wd == 2 ? put16(w2, hf, ev) : w2;
+          w3 = wd == 3 ? put16(w3, hf, ev) : w3;
+          c0 = j < 4 ? put8(c0, j, cv) : c0;
+          c1 = j >= 4 ? put8(c1, j - 4, cv) : c1;
+          kept++;
+          if (kept % kGroup == 0) {
+            og[(kept / kGroup - 1) * kWave] = make_uint4(w0, w1, w2, w3);
+            oc[(kept / kGroup - 1) * kWave] = make_uint2(c0, c1);
+            w0 = w1 = w2 = w3 = pad16;
+            c0 = c1 = pad8;
+          }
+        }
+      }
+    }
+    const int mine = (kept + kGroup - 1) / kGroup;  // groups this row fills
+    const int groups = wave_max_int(mine);          // groups of the slice's inner list
+    if (kept % kGroup) {
+      og[(kept / kGroup) * kWave] = make_uint4(w0, w1, w2, w3);
+      oc[(kept / kGroup) * kWave] = make_uint2(c0, c1);
+    }
+    for (int g = mine; g < groups; g++) {
+      og[g * kWave] = make_uint4(pad16, pad16, pad16, pad16);
+      oc[g * kWave] = make_uint2(pad8, pad8);
+    }
+    if (lane == 0) slen_in[s] = groups * kGroup;
+  }
 }
 
 // Observers over the list: kinetic energy, PE and virial sums per brick

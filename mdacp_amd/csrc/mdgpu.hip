@@ -133,6 +133,15 @@ struct mdgpu_domain {
   // list
   uint16_t *nbr = nullptr;   // staged partner index per list slot
   uint8_t *ncode = nullptr;  // partner cell offset per list slot
+  // inner (pruned) list of the fused step: same slots, per-slice inner length
+  uint16_t *nbr_in = nullptr;
+  uint8_t *ncode_in = nullptr;
+  int *slen_in = nullptr;
+  double prune_delta = 0.0;    // inner radius rc + prune_delta; 0: the fused step reads the full list
+  bool inner_valid = false;    // the inner list holds every pair that can come within rc
+  double inner_budget = 0.0;   // what is left of prune_delta for particle motion
+  double step_vmax = 0.0;      // max |p| at this step's expiry test (all replicas)
+  int prunes = 0;
   long long *sbase = nullptr, *slots64 = nullptr;
   int *slen = nullptr, *row_len = nullptr;
   // build scratch
@@ -230,6 +239,7 @@ int check_launch(mdgpu_domain *d, const char *what) {
 // reuse (state that the step reads has changed).
 void invalidate(mdgpu_domain *d) {
   d->mid_valid = false;
+  d->inner_valid = false;
   d->epoch++;
 }
 
@@ -727,6 +737,7 @@ int make_pair_list(mdgpu_domain *d) {
   if (d->nslice + 1 > d->slice_cap) {
     const int sc = d->nslice + d->nslice / 4 + 64;
     CHK(dalloc(d, &d->slen, sc));
+    CHK(dalloc(d, &d->slen_in, sc));
     CHK(dalloc(d, &d->sbase, sc));
     CHK(dalloc(d, &d->slots64, sc));
     CHK(dalloc(d, &d->slice_brick, sc));
@@ -802,6 +813,8 @@ int make_pair_list(mdgpu_domain *d) {
     const long long nc = slots + slots / 8 + 1024;
     CHK(dalloc(d, &d->nbr, (size_t)(nc + kListTail)));
     CHK(dalloc(d, &d->ncode, (size_t)(nc + kListTail)));
+    CHK(dalloc(d, &d->nbr_in, (size_t)(nc + kListTail)));
+    CHK(dalloc(d, &d->ncode_in, (size_t)(nc + kListTail)));
     d->nbr_cap = nc;
   }
   d->nbr_slots = slots;
@@ -881,10 +894,17 @@ int is_expired(mdgpu_domain *d, int *expired) {
     return MDGPU_OK;
   }
   d->lifetime++;
+  if (d->buffer_length <= 0.0 && d->prune_delta > 0.0 && d->inner_valid) {
+    // the budget is spent, so the reference skips max|p|; the inner list's budget still needs it
+    double v2 = 0.0;
+    CHK(max_p2(d, &v2));
+    d->step_vmax = std::sqrt(v2);
+  }
   if (d->buffer_length > 0.0) {
     double v2 = 0.0;
     CHK(max_p2(d, &v2));
     const double vmax = std::sqrt(v2);
+    d->step_vmax = vmax;
     if (vmax * 2.0 * d->dt > d->margin) {
       char b[160];
       snprintf(b, sizeof b, "Too fast particles exists. Try smaller time step (max_velocity = %f)", vmax);
@@ -915,6 +935,9 @@ int is_expired_ensemble(mdgpu_domain *d, int *expired) {
     HIPCHK(d, hipMemcpyAsync(d->h_rep_v2, d->rep_v2, R * sizeof(double), hipMemcpyDeviceToHost, st));
   }
   CHK(sync(d));
+  double v2max = 0.0;
+  for (int r = 0; r < R; r++) v2max = std::max(v2max, d->h_rep_v2[r]);
+  d->step_vmax = std::sqrt(v2max);
   int nflag = 0;
   for (int r = 0; r < R; r++) {
     ReplicaState &S = d->rst[r];
@@ -970,12 +993,12 @@ ForceConstsF fconsts32(const mdgpu_domain *d) {
 
 // The list as the brick kernels read it; lds_cap = staged entries per brick
 // that fit the LDS budget (bricks with more read partners from global memory).
-BrickList brick_list(const mdgpu_domain *d, int bytes_per_entry) {
+BrickList brick_list(const mdgpu_domain *d, int bytes_per_entry, bool inner = false) {
   BrickList bl;
-  bl.nbr = d->nbr;
-  bl.code = d->ncode;
+  bl.nbr = inner ? d->nbr_in : d->nbr;
+  bl.code = inner ? d->ncode_in : d->ncode;
   bl.sbase = d->sbase;
-  bl.slen = d->slen;
+  bl.slen = inner ? d->slen_in : d->slen;
   bl.glist = d->glist;
   bl.gbase = d->gbase;
   bl.row0 = d->row0;
@@ -989,8 +1012,8 @@ BrickList brick_list(const mdgpu_domain *d, int bytes_per_entry) {
 
 template <int W, int U>
 void launch_fp64(mdgpu_domain *d, const double4 *src, double4 *pos_out, int epi, const ForceConsts &fc,
-                 const StepClock &sc, double *vb, const NextDrift &nd) {
-  const BrickList bl = brick_list(d, 3 * sizeof(double));
+                 const StepClock &sc, double *vb, const NextDrift &nd, bool inner) {
+  const BrickList bl = brick_list(d, 3 * sizeof(double), inner);
   const size_t shm = (size_t)bl.lds_cap * 3 * sizeof(double);
   allow_lds(k_force_brick_fp64<W, U>, shm);
   hipLaunchKernelGGL((k_force_brick_fp64<W, U>), dim3(d->nbrick), dim3(kWave * W), shm, d->stream, src, d->mom,
@@ -999,8 +1022,9 @@ void launch_fp64(mdgpu_domain *d, const double4 *src, double4 *pos_out, int epi,
 
 template <int W, int U, int K>
 void launch_mixed(mdgpu_domain *d, const double4 *src, const float4 *rel, double4 *pos_out, int epi,
-                  const ForceConstsF &f32, double dt2, const StepClock &sc, double *vb, const NextDrift &nd) {
-  const BrickList bl = brick_list(d, sizeof(float4));
+                  const ForceConstsF &f32, double dt2, const StepClock &sc, double *vb, const NextDrift &nd,
+                  bool inner) {
+  const BrickList bl = brick_list(d, sizeof(float4), inner);
   const size_t shm = (size_t)bl.lds_cap * sizeof(float4);
   allow_lds(k_force_brick_mixed<W, U, K>, shm);
   hipLaunchKernelGGL((k_force_brick_mixed<W, U, K>), dim3(d->nbrick), dim3(kWave * W), shm, d->stream, rel, src,
@@ -1009,26 +1033,27 @@ void launch_mixed(mdgpu_domain *d, const double4 *src, const float4 *rel, double
 
 template <int W, int K>
 void launch_mixed_sum(mdgpu_domain *d, const double4 *src, const float4 *rel, double4 *pos_out, int epi,
-                      const ForceConstsF &f32, double dt2, const StepClock &sc, double *vb, const NextDrift &nd) {
-  if (d->unroll == 32) launch_mixed<W, 4, K>(d, src, rel, pos_out, epi, f32, dt2, sc, vb, nd);
-  else if (d->unroll == 16) launch_mixed<W, 2, K>(d, src, rel, pos_out, epi, f32, dt2, sc, vb, nd);
-  else launch_mixed<W, 1, K>(d, src, rel, pos_out, epi, f32, dt2, sc, vb, nd);
+                      const ForceConstsF &f32, double dt2, const StepClock &sc, double *vb, const NextDrift &nd,
+                      bool inner) {
+  if (d->unroll == 32) launch_mixed<W, 4, K>(d, src, rel, pos_out, epi, f32, dt2, sc, vb, nd, inner);
+  else if (d->unroll == 16) launch_mixed<W, 2, K>(d, src, rel, pos_out, epi, f32, dt2, sc, vb, nd, inner);
+  else launch_mixed<W, 1, K>(d, src, rel, pos_out, epi, f32, dt2, sc, vb, nd, inner);
 }
 
 template <int W>
 void launch_force_w(mdgpu_domain *d, const double4 *src, const float4 *rel, double4 *pos_out, int epi,
                     const ForceConsts &fc, const ForceConstsF &f32, const StepClock &sc, double *vb,
-                    const NextDrift &nd) {
+                    const NextDrift &nd, bool in) {
   // groups of kGroup list indices in flight: unroll / kGroup (1, 2 or 4)
   if (d->precision == MDGPU_FP64) {
-    if (d->unroll == 32) launch_fp64<W, 4>(d, src, pos_out, epi, fc, sc, vb, nd);
-    else if (d->unroll == 16) launch_fp64<W, 2>(d, src, pos_out, epi, fc, sc, vb, nd);
-    else launch_fp64<W, 1>(d, src, pos_out, epi, fc, sc, vb, nd);
+    if (d->unroll == 32) launch_fp64<W, 4>(d, src, pos_out, epi, fc, sc, vb, nd, in);
+    else if (d->unroll == 16) launch_fp64<W, 2>(d, src, pos_out, epi, fc, sc, vb, nd, in);
+    else launch_fp64<W, 1>(d, src, pos_out, epi, fc, sc, vb, nd, in);
   } else {
     switch (d->kahan) {
-      case kSumPlain: launch_mixed_sum<W, kSumPlain>(d, src, rel, pos_out, epi, f32, fc.dt2, sc, vb, nd); break;
-      case kSumKahan: launch_mixed_sum<W, kSumKahan>(d, src, rel, pos_out, epi, f32, fc.dt2, sc, vb, nd); break;
-      default: launch_mixed_sum<W, kSumGroup64>(d, src, rel, pos_out, epi, f32, fc.dt2, sc, vb, nd); break;
+      case kSumPlain: launch_mixed_sum<W, kSumPlain>(d, src, rel, pos_out, epi, f32, fc.dt2, sc, vb, nd, in); break;
+      case kSumKahan: launch_mixed_sum<W, kSumKahan>(d, src, rel, pos_out, epi, f32, fc.dt2, sc, vb, nd, in); break;
+      default: launch_mixed_sum<W, kSumGroup64>(d, src, rel, pos_out, epi, f32, fc.dt2, sc, vb, nd, in); break;
     }
   }
 }
@@ -1040,7 +1065,7 @@ void launch_force_w(mdgpu_domain *d, const double4 *src, const float4 *rel, doub
 // host mirror.  `clock` set: the Langevin step is read from it and advanced
 // (a launch that is captured and replayed).
 int launch_force(mdgpu_domain *d, const double4 *src, const float4 *rel, double4 *pos_out, int epilogue,
-                 bool want_vmax, const NextDrift &nd, unsigned long long *clock = nullptr) {
+                 bool want_vmax, const NextDrift &nd, unsigned long long *clock = nullptr, bool inner = false) {
   if (d->pn == 0) return MDGPU_OK;
   hipStream_t st = d->stream;
   const ForceConsts fc = fconsts(d);
@@ -1051,10 +1076,10 @@ int launch_force(mdgpu_domain *d, const double4 *src, const float4 *rel, double4
   const bool ev = d->timing && clock == nullptr && ev0 + 1 < (int)d->ev.size();
   if (ev) HIPCHK(d, hipEventRecord(d->ev[ev0], st));
   switch (d->waves) {
-    case 2: launch_force_w<2>(d, src, rel, pos_out, epilogue, fc, f32, sc, vb, nd); break;
-    case 8: launch_force_w<8>(d, src, rel, pos_out, epilogue, fc, f32, sc, vb, nd); break;
-    case 16: launch_force_w<16>(d, src, rel, pos_out, epilogue, fc, f32, sc, vb, nd); break;
-    default: launch_force_w<4>(d, src, rel, pos_out, epilogue, fc, f32, sc, vb, nd); break;
+    case 2: launch_force_w<2>(d, src, rel, pos_out, epilogue, fc, f32, sc, vb, nd, inner); break;
+    case 8: launch_force_w<8>(d, src, rel, pos_out, epilogue, fc, f32, sc, vb, nd, inner); break;
+    case 16: launch_force_w<16>(d, src, rel, pos_out, epilogue, fc, f32, sc, vb, nd, inner); break;
+    default: launch_force_w<4>(d, src, rel, pos_out, epilogue, fc, f32, sc, vb, nd, inner); break;
   }
   if (ev) {
     HIPCHK(d, hipEventRecord(d->ev[ev0 + 1], st));
@@ -1136,7 +1161,31 @@ int force_in_place(mdgpu_domain *d) {
 // half drift to pos_next (rel_next), and each replica's max |p|^2, copied to
 // the host.  With `clock` the launches read and advance the device step
 // counter (they are being captured into a graph).
-int enqueue_fused(mdgpu_domain *d, unsigned long long *clock) {
+template <int W>
+void launch_prune_w(mdgpu_domain *d) {
+  const BrickList bl = brick_list(d, 3 * sizeof(double));
+  const size_t shm = (size_t)bl.lds_cap * 3 * sizeof(double);
+  const double rin = d->rc + d->prune_delta;
+  allow_lds(k_prune_brick<W>, shm);
+  hipLaunchKernelGGL(k_prune_brick<W>, dim3(d->nbrick), dim3(kWave * W), shm, d->stream, d->pos_tmp, bl, rin * rin,
+                     d->nbr_in, d->ncode_in, d->slen_in);
+}
+
+// The inner list from the full list at the half-drifted positions (pos_tmp,
+// ghosts refreshed): every listed pair within rc + prune_delta.
+int launch_prune(mdgpu_domain *d) {
+  if (d->pn == 0) return MDGPU_OK;
+  switch (d->waves) {
+    case 2: launch_prune_w<2>(d); break;
+    case 8: launch_prune_w<8>(d); break;
+    case 16: launch_prune_w<16>(d); break;
+    default: launch_prune_w<4>(d); break;
+  }
+  d->prunes++;
+  return check_launch(d, "prune");
+}
+
+int enqueue_fused(mdgpu_domain *d, unsigned long long *clock, bool prune = false) {
   hipStream_t st = d->stream;
   const bool mixed = d->precision == MDGPU_MIXED;
   if (d->ng > 0) {
@@ -1148,9 +1197,10 @@ int enqueue_fused(mdgpu_domain *d, unsigned long long *clock) {
                          d->pos_tmp, d->gsrc, d->gcode, d->pn, d->ng, d->reps, d->cell, d->org, d->rel_tmp);
   }
   CHK(check_launch(d, "refresh"));
+  if (prune) CHK(launch_prune(d));
   const int epi = d->thermostat == MDGPU_LANGEVIN ? kKickLangevin : kKickDrift;
   const NextDrift nd{d->pos_next, mixed ? d->rel_next : nullptr, d->cell, d->org};
-  return launch_force(d, d->pos_tmp, d->rel_tmp, d->pos, epi, true, nd, clock);
+  return launch_force(d, d->pos_tmp, d->rel_tmp, d->pos, epi, true, nd, clock, d->prune_delta > 0.0);
 }
 
 void drop_graph(StepGraph &g) {
@@ -1262,14 +1312,25 @@ int calculate_step(mdgpu_domain *d) {
       }
       CHK(check_launch(d, "drift+refresh"));
     }
-    if (d->use_graph && !d->timing) {
+    // the inner list: pruned again once the motion since the last pruning
+    // could have brought a pair from beyond rc + delta to within rc
+    bool prune = false;
+    if (d->prune_delta > 0.0) {
+      if (d->inner_valid) d->inner_budget -= d->step_vmax * 2.0 * d->dt;
+      if (!d->inner_valid || !(d->inner_budget > 0.0)) {
+        prune = true;
+        d->inner_valid = true;
+        d->inner_budget = d->prune_delta;
+      }
+    }
+    if (d->use_graph && !d->timing && !prune) {
       if (d->clock_dirty) {
         hipLaunchKernelGGL(k_set_u64, dim3(1), dim3(1), 0, st, d->clock, (unsigned long long)d->step);
         d->clock_dirty = false;
       }
       CHK(launch_fused_graph(d));
     } else {
-      CHK(enqueue_fused(d, nullptr));
+      CHK(enqueue_fused(d, nullptr, prune));
       d->clock_dirty = true;
     }
     std::swap(d->pos_tmp, d->pos_next);
@@ -1476,7 +1537,7 @@ int mdgpu_destroy(mdgpu_domain *d) {
                   d->partial, d->maxpart,     d->bpart,       d->cand,      d->u64,      d->i64,
                   d->i32,     d->d3,          d->reps,        d->brep,      d->rep,      d->rep_alt,
                   d->grng,    d->rep_v2,      d->rep_out,     d->which,     d->rep_exp,  d->clock,
-                  d->perm_in};
+                  d->perm_in, d->nbr_in,      d->ncode_in,    d->slen_in};
   for (StepGraph &g : d->graphs) drop_graph(g);
   for (void *b : bufs)
     if (b) (void)hipFree(b);
@@ -2039,6 +2100,10 @@ int mdgpu_set_option(mdgpu_domain *d, int option, int value) {
     case MDGPU_OPT_SEARCH_STAGE:
       d->search_stage = value != 0;
       return MDGPU_OK;
+    case MDGPU_OPT_PRUNE:
+      if (value < 0 || value > 1000) return MDGPU_ERR_ARGUMENT;
+      d->prune_delta = value * 1.0e-3;
+      return MDGPU_OK;
   }
   return MDGPU_ERR_ARGUMENT;
 }
@@ -2054,6 +2119,7 @@ int mdgpu_get_option(const mdgpu_domain *d, int option, int *value) {
     case MDGPU_OPT_GRAPH: *value = d->use_graph ? 1 : 0; return MDGPU_OK;
     case MDGPU_OPT_SEARCH_WAVES: *value = d->search_waves; return MDGPU_OK;
     case MDGPU_OPT_SEARCH_STAGE: *value = d->search_stage; return MDGPU_OK;
+    case MDGPU_OPT_PRUNE: *value = (int)std::lround(d->prune_delta * 1.0e3); return MDGPU_OK;
   }
   return MDGPU_ERR_ARGUMENT;
 }

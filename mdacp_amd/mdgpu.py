@@ -30,7 +30,7 @@ EXPORTS = [
     "mdgpu_set_replica_thermostat", "mdgpu_observe_replicas", "mdgpu_get_replica_pairs",
 ]
 OPT_UNROLL, OPT_KAHAN, OPT_WAVES, OPT_LDS_KB, OPT_BRICK, OPT_GRAPH = 1, 2, 3, 4, 5, 6
-OPT_SEARCH_WAVES, OPT_SEARCH_STAGE = 7, 8
+OPT_SEARCH_WAVES, OPT_SEARCH_STAGE, OPT_PRUNE = 7, 8, 9
 
 
 class Params(ctypes.Structure):
@@ -222,7 +222,8 @@ class Domain:
         return c
 
     def copy_options(self, other):
-        for o in (OPT_UNROLL, OPT_KAHAN, OPT_WAVES, OPT_LDS_KB, OPT_GRAPH, OPT_SEARCH_WAVES, OPT_SEARCH_STAGE):
+        for o in (OPT_UNROLL, OPT_KAHAN, OPT_WAVES, OPT_LDS_KB, OPT_GRAPH, OPT_SEARCH_WAVES, OPT_SEARCH_STAGE,
+                  OPT_PRUNE):
             self.set_option(o, other.get_option(o))
 
     def zeta(self):

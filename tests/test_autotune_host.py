@@ -9,7 +9,8 @@ class MockDomain:
 
     def __init__(self, ensemble=False):
         self.opts = {mdgpu.OPT_SEARCH_WAVES: 16, mdgpu.OPT_SEARCH_STAGE: 1, mdgpu.OPT_WAVES: 8,
-                     mdgpu.OPT_LDS_KB: 96, mdgpu.OPT_UNROLL: 8, mdgpu.OPT_BRICK: 4 | 4 << 8 | 4 << 16}
+                     mdgpu.OPT_LDS_KB: 96, mdgpu.OPT_UNROLL: 8, mdgpu.OPT_BRICK: 4 | 4 << 8 | 4 << 16,
+                     mdgpu.OPT_PRUNE: 0}
         self.clones = []
         self.closed = False
         if ensemble:

@@ -304,3 +304,17 @@ def test_autotune_keeps_results():
     qa, pa, ta = a.particles()
     qb, pb, tb = b.particles()
     assert np.array_equal(ta, tb) and np.array_equal(qa, qb) and np.array_equal(pa, pb)
+
+
+def test_pruned_ensemble_is_bitwise_neutral():
+    """Dynamic pruning in an ensemble (budget from the largest |p| of all replicas), replayed from graphs."""
+    say("pruned ensemble vs full list")
+    a, _ = ensemble(SPEC, melt_seed=9)
+    b, _ = ensemble(SPEC, melt_seed=9)
+    b.set_option(mdgpu.OPT_PRUNE, 200)
+    ra = sum(a.calculate(20) for _ in range(4))
+    rb = sum(b.calculate(20) for _ in range(4))
+    assert ra == rb and ra >= 2
+    qa, pa, ta = a.particles()
+    qb, pb, tb = b.particles()
+    assert np.array_equal(ta, tb) and np.array_equal(qa, qb) and np.array_equal(pa, pb)
