@@ -66,8 +66,9 @@ typedef struct mdgpu_domain mdgpu_domain;
 
 /* force precision */
 #define MDGPU_FP64 0   /* FP64 throughout */
-#define MDGPU_MIXED 1  /* FP32 cell-relative pair forces, compensated FP32 sums,
-                          FP64 positions, momenta, integration and reductions */
+#define MDGPU_MIXED 1  /* FP32 cell-relative pair forces, FP32 sums of each 8 list
+                          entries added in FP64 (MDGPU_OPT_KAHAN), FP64 positions,
+                          momenta, integration and reductions */
 
 /* thermostat (SimulationInfo::ControlTemperature / HeatbathType) */
 #define MDGPU_NVE 0
