@@ -191,11 +191,15 @@ int mdgpu_get_timing(const mdgpu_domain *d, int *force_launches, double *force_m
                                 (Kahan), 0 = plain FP32 */
 #define MDGPU_OPT_WAVES 3    /* wavefronts per cell-brick workgroup: 2, 4, 8 or 16 */
 #define MDGPU_OPT_LDS_KB 4   /* LDS budget for one brick's staged partners in the force and
-                                observer kernels, KiB (>= 8); bricks with more staged
-                                partners read them from global memory */
-#define MDGPU_OPT_BRICK 5    /* cells per brick: bx | by << 8 | bz << 16, each 1..4
-                                (the next step rebuilds the list); an ensemble
-                                picks one per replica from its density */
+                                observer kernels, KiB (>= 8; default: MDGPU_OPT_BRICK_KB's
+                                value); bricks with more staged partners read them from
+                                global memory */
+#define MDGPU_OPT_BRICK 5    /* cells per brick: bx | by << 8 | bz << 16, each 1..8 with
+                                (bx+2)(by+2)(bz+2) <= 216 (the next step rebuilds the
+                                list); by default a domain picks, at its first
+                                construction, and an ensemble, per replica from its
+                                density, the bricks that stage the fewest cells within
+                                the LDS budget */
 #define MDGPU_OPT_GRAPH 6    /* 1 = fused steps replayed from a captured HIP graph
                                 (default; not while timing is enabled), 0 = launched */
 #define MDGPU_OPT_SEARCH_WAVES 7  /* wavefronts per brick in the neighbour search: 4, 8 or 16 */
@@ -205,6 +209,12 @@ int mdgpu_get_timing(const mdgpu_domain *d, int *force_launches, double *force_m
                                 the pairs within rc + value/1000, pruned from the full list
                                 whenever the particles could have moved value/1000 since
                                 (0 = off, the default; the same pairs in the same order) */
+#define MDGPU_OPT_ROW_SORT 10  /* rows of a brick ordered by list length (1, the default: less
+                                  slice padding) or kept in cell order (0: a slice's partners lie
+                                  close together in LDS); the next step rebuilds the list */
+#define MDGPU_OPT_BRICK_KB 11  /* LDS budget, KiB (8..160), that the automatic bricks are sized
+                                  for (default 96 for a domain, 76 for an ensemble); setting it
+                                  chooses the bricks again and the next step rebuilds the list */
 int mdgpu_set_option(mdgpu_domain *d, int option, int value);
 int mdgpu_get_option(const mdgpu_domain *d, int option, int *value);
 

@@ -9,10 +9,12 @@ the pair-force kernel's LDS budget sets how many brick workgroups share a CU.
 autotune() times short runs of the caller's own workload on a clone and sets
 the fastest shape on the caller:
 
-- cell bricks of a single domain: edge {4, 3, 2} cells, by the force kernel's
-  time (an ensemble sizes each replica's bricks from its density); a new
-  brick edge rebuilds the caller's list at its next step, which reorders the
-  summation of its kicks, so only this choice changes the caller's bits;
+- cell bricks of a single domain: the domain's own choice (the bricks that
+  stage the fewest cells within the LDS budget) against cubes of edge
+  {4, 3, 2} cells, by the force kernel's time (an ensemble sizes each
+  replica's bricks from its density); new bricks rebuild the caller's list at
+  its next step, which reorders the summation of its kicks, so only this
+  choice changes the caller's bits;
 - neighbour search: wavefronts per brick {4, 8, 16} x LDS staging {on, off},
   by the time of a pair-list construction;
 - pair force: wavefronts per brick {4, 8} x LDS budget {48 ... 96} KiB x
@@ -62,6 +64,14 @@ def _force_ms(d, steps):
 KEEP = 0.98  # a candidate must take less than this fraction of the current setting's time
 
 
+def _brick_code(b):
+    return b[0] | (b[1] << 8) | (b[2] << 16)
+
+
+def _brick_dims(v):
+    return (v & 0xff, (v >> 8) & 0xff, (v >> 16) & 0xff)
+
+
 def _pick(cands, times, current):
     """The fastest candidate, or the current one unless beaten by more than 2%."""
     best = min(cands, key=lambda k: times[k])
@@ -80,15 +90,16 @@ def autotune(d, steps=30, margins=None, log=None):
     if not hasattr(d, "R"):
         res["brick"] = {}
         bt = {}
-        cur_b = d.get_option(mdgpu.OPT_BRICK) & 0xff
-        for b in (4, 3, 2):
-            c.set_option(mdgpu.OPT_BRICK, b | (b << 8) | (b << 16))
+        cur_b = _brick_dims(d.get_option(mdgpu.OPT_BRICK))
+        cands = [cur_b] + [(b, b, b) for b in (4, 3, 2) if (b, b, b) != cur_b]
+        for b in cands:
+            c.set_option(mdgpu.OPT_BRICK, _brick_code(b))
             c.make_pair_list()
             _force_ms(c, 3)
-            res["brick"][f"{b}"] = bt[b] = _force_ms(c, steps)
+            res["brick"]["x".join(map(str, b))] = bt[b] = _force_ms(c, steps)
             say(f"brick {b}: force {bt[b]:.4f} ms")
-        brick = _pick((4, 3, 2), bt, cur_b)
-        c.set_option(mdgpu.OPT_BRICK, brick | (brick << 8) | (brick << 16))
+        brick = _pick(cands, bt, cur_b)
+        c.set_option(mdgpu.OPT_BRICK, _brick_code(brick))
         c.make_pair_list()
     st = {}
     for w, s in SEARCH:
@@ -128,8 +139,8 @@ def autotune(d, steps=30, margins=None, log=None):
                   unroll=best_f[2], prune=best_p)
     if brick is not None:
         choice["brick"] = brick
-        if d.get_option(mdgpu.OPT_BRICK) != brick | (brick << 8) | (brick << 16):
-            d.set_option(mdgpu.OPT_BRICK, brick | (brick << 8) | (brick << 16))
+        if d.get_option(mdgpu.OPT_BRICK) != _brick_code(brick):
+            d.set_option(mdgpu.OPT_BRICK, _brick_code(brick))
     d.set_option(mdgpu.OPT_SEARCH_WAVES, choice["search_waves"])
     d.set_option(mdgpu.OPT_SEARCH_STAGE, choice["search_stage"])
     d.set_option(mdgpu.OPT_WAVES, choice["waves"])

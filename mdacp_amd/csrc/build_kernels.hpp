@@ -48,6 +48,11 @@ __global__ void k_wrap_bin(double4 *__restrict__ pos, int n, const int *__restri
       atomicOr(err, kErrInvalidPosition);
   }
   const int c = cell_of(q.x, q.y, q.z, R);
+  // 0 <= q <= L bins into cells 1 .. m - 1, which bricks cover; the lower
+  // ghost layer is refused like a position outside the box
+  const int l = c - R.cell0;
+  if (l % R.m[0] == 0 || (l / R.m[0]) % R.m[1] == 0 || l / (R.m[0] * R.m[1]) == 0)
+    atomicOr(err, kErrInvalidPosition);
   cell[i] = c;
   slot[i] = atomicAdd(&ccount[c], 1);
 }

@@ -1,7 +1,11 @@
 // list_kernels.hpp -- neighbour list on gfx950, organised by cell bricks.
 //
-// The extended cell grid (meshlist.cc:31-55) of each replica is cut into
-// bricks of bd[0] x bd[1] x bd[2] cells (Replica::bd).  Cells are ranked brick by brick, and owned
+// The extended cell grid (meshlist.cc:31-55) of each replica, m cells per
+// axis, is cut into bricks of bd[0] x bd[1] x bd[2] cells (Replica::bd) from
+// cell 1 to cell m - 1: the owned cells and the upper ghost layer, which holds
+// an owned particle that sits exactly on the upper face (q == L bins there).
+// The lower ghost layer (cell 0) only ever holds ghost images and belongs to
+// no brick.  Cells are ranked brick by brick, and owned
 // particles are stored in that order, so the rows of one brick are contiguous
 // and one workgroup of the force kernels takes one brick.  The partners of a
 // brick's rows lie in its neighbourhood: the brick grown by one cell on every
@@ -36,6 +40,10 @@ struct Bricks {
 
 // staged ranges of a brick: two per neighbourhood cell (owned, then ghosts)
 constexpr int kRangesPerCell = 2;
+// cells of a brick's neighbourhood, (bd[0] + 2)(bd[1] + 2)(bd[2] + 2), at most,
+// and the longest brick edge in cells
+constexpr int kMaxBrickCells = 216;
+constexpr int kMaxBrickEdge = 8;
 
 struct BrickBox {
   int lo[3];  // first neighbourhood cell (may be -1)
@@ -43,7 +51,8 @@ struct BrickBox {
 };
 
 // neighbourhood box of global brick b of replica R, in the replica's cell
-// coordinates
+// coordinates (the last box along an axis reaches one cell past the grid;
+// that cell is empty)
 __device__ __forceinline__ BrickBox brick_box(int b, const Replica &R) {
   BrickBox X;
   const int l = b - R.brick0;
@@ -52,7 +61,7 @@ __device__ __forceinline__ BrickBox brick_box(int b, const Replica &R) {
   const int bz = l / (R.nb[0] * R.nb[1]);
   const int bi[3] = {bx, by, bz};
   for (int d = 0; d < 3; d++) {
-    const int c0 = bi[d] * R.bd[d];
+    const int c0 = 1 + bi[d] * R.bd[d];
     const int c1 = min(R.m[d], c0 + R.bd[d]);
     X.lo[d] = c0 - 1;
     X.n[d] = c1 - c0 + 2;
@@ -145,7 +154,7 @@ __global__ void k_brick_ranges(Bricks B, CellTables T, int nbrick, int pn, int *
 __global__ void k_brick_glist(int rstride, const int *__restrict__ tstart, const int *__restrict__ tbase,
                               const int *__restrict__ nrange, const long long *__restrict__ gbase,
                               int *__restrict__ glist) {
-  __shared__ int tb[kRangesPerCell * 216 + 1];
+  __shared__ int tb[kRangesPerCell * kMaxBrickCells + 1];
   const int b = blockIdx.x;
   const int R = nrange[b];
   const int *tbg = tbase + (long long)b * (rstride + 1);
@@ -166,7 +175,7 @@ __global__ void k_brick_glist(int rstride, const int *__restrict__ tstart, const
 }
 
 // ---- search --------------------------------------------------------------
-constexpr int kMaxRanges = kRangesPerCell * 6 * 6 * 6;  // bricks of at most 4x4x4 cells
+constexpr int kMaxRanges = kRangesPerCell * kMaxBrickCells;
 constexpr int kErrSearchCap = 16;  // a row has more candidates than its bitmask holds
 
 // cell offset code of a partner cell relative to the row's cell (each -1..1)
@@ -340,18 +349,28 @@ constexpr int kSortBins = kBlock;  // lengths >= kSortBins - 1 share the last bi
 // Order the rows of each brick by decreasing list length, ties by index (a
 // stable counting sort, so the order is deterministic): rowsrc[new] = old,
 // rowinv[old] = new.  Slices of 64 consecutive rows then hold rows of nearly
-// equal length and carry little padding.  One workgroup of kBlock threads per
-// brick.
+// equal length and carry little padding.  With sort == 0 the rows keep their
+// cell order (rowsrc and rowinv are the identity): a slice then holds the
+// rows of a few neighbouring cells, whose partners lie close together in the
+// staged array.  One workgroup of kBlock threads per brick.
 __global__ __launch_bounds__(kBlock) void k_brick_row_order(const int *__restrict__ row_len,
                                                             const int *__restrict__ row0,
                                                             const int *__restrict__ nrow,
-                                                            int *__restrict__ rowsrc, int *__restrict__ rowinv) {
+                                                            int *__restrict__ rowsrc, int *__restrict__ rowinv,
+                                                            int sort) {
   __shared__ int start[kSortBins];
   __shared__ int scan[kSortBins];
   __shared__ int bins[kBlock];
   const int b = blockIdx.x;
   const int r0 = row0[b], n = nrow[b];
   const int t = threadIdx.x;
+  if (!sort) {
+    for (int v = t; v < n; v += kBlock) {
+      rowsrc[r0 + v] = r0 + v;
+      rowinv[r0 + v] = r0 + v;
+    }
+    return;
+  }
   start[t] = 0;
   __syncthreads();
   // bin of a row: longest rows first

@@ -55,10 +55,12 @@ struct mdgpu_domain {
   int waves = 8;          // wavefronts per brick workgroup: 2, 4, 8 or 16
   int bd[3] = {4, 4, 4};  // cells per brick
   bool brick_auto = true; // a single domain picks bd from its density at the first construction
-  int lds_kb = 96;        // LDS budget for one brick's staged partners, KiB
+  int brick_kb = 0;         // LDS budget the automatic bricks are sized for, KiB (0: the default)
+  int lds_kb = 0;         // LDS budget for one brick's staged partners, KiB (0: the bricks' budget)
   int lds_max_kb = 60;    // device limit per workgroup less the kernels' static LDS
   int search_waves = 16;  // wavefronts per brick workgroup of the neighbour search: 4, 8 or 16
   int search_stage = 1;   // neighbour search reads candidates staged in LDS (1) or from global memory (0)
+  int row_sort = 1;       // rows of a brick ordered by list length (1) or kept in cell order (0)
 
   // SimulationInfo / MDRect / MeshList geometry
   double L[3], rs[3], re[3];
@@ -84,6 +86,8 @@ struct mdgpu_domain {
   double *h_rep_v2 = nullptr, *h_rep_out = nullptr;
   int *h_which = nullptr, *h_rep_exp = nullptr;
   unsigned long long *clock = nullptr;  // step counter read by replayed force launches
+  unsigned long long *chk_dmax = nullptr;  // displacement check of listed replicas: max |dq|^2 bits,
+  int *chk_cnt = nullptr, *chk_cand = nullptr;  // candidate counts, candidates (cap per replica)
   bool clock_dirty = true;              // clock differs from `step`
 
   // captured fused steps, one per parity of the position buffers
@@ -292,7 +296,7 @@ int set_geometry(mdgpu_domain *d) {
       R.w[k] = w / (double)ms;
       R.im[k] = 1.0 / R.w[k];
       R.m[k] = (int)(w / R.w[k]) + 2;
-      R.nb[k] = (R.m[k] + R.bd[k] - 1) / R.bd[k];
+      R.nb[k] = (R.m[k] - 1 + R.bd[k] - 1) / R.bd[k];  // bricks tile cells 1 .. m - 1
     }
     R.cell0 = (int)ncell;
     R.brick0 = (int)nbrick;
@@ -324,8 +328,10 @@ int set_geometry(mdgpu_domain *d) {
   CHK(dalloc(d, &d->start_r, d->ncell + 1));
   CHK(dalloc(d, &d->org, org.size()));
   HIPCHK(d, hipMemcpy(d->org, org.data(), org.size() * sizeof(double), hipMemcpyHostToDevice));
-  // bricks z-major inside each replica, cells z, y, x inside each brick
-  std::vector<int> rank(d->ncell), boc(d->ncell), rank0(d->nbrick + 1), brep(d->nbrick);
+  // bricks z-major inside each replica, cells z, y, x inside each brick; then
+  // the replica's lower ghost-layer cells, which belong to no brick (no owned
+  // particle bins there: k_wrap_bin refuses one)
+  std::vector<int> rank(d->ncell), boc(d->ncell, -1), rank0(d->nbrick + 1), brep(d->nbrick);
   int rk = 0;
   for (int r = 0; r < d->nrep; r++) {
     const Replica &R = d->rh[r];
@@ -335,14 +341,18 @@ int set_geometry(mdgpu_domain *d) {
           const int b = R.brick0 + (bz * R.nb[1] + by) * R.nb[0] + bx;
           rank0[b] = rk;
           brep[b] = r;
-          for (int z = bz * R.bd[2]; z < std::min(R.m[2], (bz + 1) * R.bd[2]); z++)
-            for (int y = by * R.bd[1]; y < std::min(R.m[1], (by + 1) * R.bd[1]); y++)
-              for (int x = bx * R.bd[0]; x < std::min(R.m[0], (bx + 1) * R.bd[0]); x++) {
+          for (int z = 1 + bz * R.bd[2]; z < std::min(R.m[2], 1 + (bz + 1) * R.bd[2]); z++)
+            for (int y = 1 + by * R.bd[1]; y < std::min(R.m[1], 1 + (by + 1) * R.bd[1]); y++)
+              for (int x = 1 + bx * R.bd[0]; x < std::min(R.m[0], 1 + (bx + 1) * R.bd[0]); x++) {
                 const int c = R.cell0 + (z * R.m[1] + y) * R.m[0] + x;
                 rank[c] = rk++;
                 boc[c] = b;
               }
         }
+    for (int z = 0; z < R.m[2]; z++)
+      for (int y = 0; y < R.m[1]; y++)
+        for (int x = 0; x < R.m[0]; x++)
+          if (x == 0 || y == 0 || z == 0) rank[R.cell0 + (z * R.m[1] + y) * R.m[0] + x] = rk++;
   }
   rank0[d->nbrick] = rk;
   d->h_brick_of_cell = boc;
@@ -613,7 +623,7 @@ int order_rows(mdgpu_domain *d) {
   hipStream_t st = d->stream;
   const int pn = d->pn, ng = d->ng;
   hipLaunchKernelGGL(k_brick_row_order, dim3(d->nbrick), dim3(kBlock), 0, st, d->row_len, d->row0, d->nrow,
-                     d->rowsrc, d->rowinv);
+                     d->rowsrc, d->rowinv, d->row_sort);
   hipLaunchKernelGGL(k_gather_owned, dim3(nblk(pn)), dim3(kBlock), 0, st, d->rowsrc, pn, d->pos, d->mom, d->id,
                      d->cell, d->rep, d->pos_tmp, d->mom_alt, d->id_alt, d->cell_alt, d->rep_alt);
   hipLaunchKernelGGL(k_gather_int, dim3(nblk(pn)), dim3(kBlock), 0, st, d->rowsrc, pn, d->row_len, d->nimg);
@@ -637,15 +647,50 @@ int order_rows(mdgpu_domain *d) {
 // Cells per brick for a box of density rho (cubic bricks): the largest edge
 // whose staged neighbourhood (bd + 2 cells per axis, FP64 entries of 24
 // bytes) is expected to fill at most 90% of kb KiB of LDS.
-int auto_brick(const mdgpu_domain *d, double rho, const double L[3], int kb) {
-  const double budget = 0.9 * std::min(kb, d->lds_max_kb) * 1024.0;
-  double w[3];
-  for (int k = 0; k < 3; k++) w[k] = L[k] / std::max(1, (int)(L[k] / d->SL));
-  for (int b = 4; b > 1; b--) {
-    const double staged = rho * (b + 2) * w[0] * (b + 2) * w[1] * (b + 2) * w[2];
-    if (staged * 24.0 <= budget) return b;
+// Cells of the neighbourhoods of all bricks along one axis, counting only
+// cells inside the grid: bricks of bd cells tile cells 1 .. ms + 1 of a grid
+// of ms + 2 (ms owned cells and the two ghost layers).
+long long axis_staged_cells(int ms, int bd) {
+  long long t = 0;
+  for (int c0 = 1; c0 <= ms + 1; c0 += bd) {
+    const int c1 = std::min(ms + 2, c0 + bd);
+    t += std::min(c1, ms + 1) - (c0 - 1) + 1;
   }
-  return 1;
+  return t;
+}
+
+// Brick dimensions for a box of edges L at density rho: among the bricks
+// whose neighbourhood fits kMaxBrickCells cells and, at the mean density,
+// 90% of kb KiB of LDS (positions, 24 B per staged entry), the ones that
+// stage the fewest cells in all (ties: fewer bricks).  Staging the
+// neighbourhood is the per-brick overhead of the search and force kernels.
+void auto_brick(const mdgpu_domain *d, double rho, const double L[3], int kb, int bd[3]) {
+  const double budget = 0.9 * std::min(kb, d->lds_max_kb) * 1024.0;
+  int ms[3];
+  double vol = 1.0;
+  for (int k = 0; k < 3; k++) {
+    ms[k] = std::max(1, (int)(L[k] / d->SL));
+    vol *= L[k] / ms[k];
+  }
+  long long best = -1, best_n = 0;
+  for (int k = 0; k < 3; k++) bd[k] = 1;
+  for (int bz = 1; bz <= kMaxBrickEdge; bz++)
+    for (int by = 1; by <= kMaxBrickEdge; by++)
+      for (int bx = 1; bx <= kMaxBrickEdge; bx++) {
+        const int e[3] = {bx, by, bz};
+        const int cells = (bx + 2) * (by + 2) * (bz + 2);
+        if (cells > kMaxBrickCells || rho * vol * cells * 24.0 > budget) continue;
+        long long staged = 1, n = 1;
+        for (int k = 0; k < 3; k++) {
+          staged *= axis_staged_cells(ms[k], e[k]);
+          n *= (ms[k] + e[k]) / e[k];
+        }
+        if (best < 0 || staged < best || (staged == best && n < best_n)) {
+          best = staged;
+          best_n = n;
+          for (int k = 0; k < 3; k++) bd[k] = e[k];
+        }
+      }
 }
 
 // Ensemble replicas: bricks for two workgroups per CU (160 KiB of LDS).
@@ -654,6 +699,15 @@ constexpr int kReplicaBrickKb = 76;
 // force kernels stage within the default LDS budget, chosen at the first
 // construction from the density.
 constexpr int kDomainBrickKb = 96;
+
+int brick_budget(const mdgpu_domain *d) {
+  return d->brick_kb ? d->brick_kb : (d->nrep == 1 ? kDomainBrickKb : kReplicaBrickKb);
+}
+
+// The force kernels' LDS budget: unless set, the one the bricks are sized
+// for, so that the largest bricks of a state do not push a workgroup past the
+// occupancy the bricks were sized for (two per CU at 76 KiB, one at 96).
+int force_lds_kb(const mdgpu_domain *d) { return d->lds_kb ? d->lds_kb : brick_budget(d); }
 
 // MDManager::MakePairList for the single domain (mdmanager.cc:262-280).
 // The reference also permutes particles that cross a face
@@ -664,9 +718,10 @@ int make_pair_list(mdgpu_domain *d) {
   if (d->nrep == 1 && d->brick_auto && d->pn > 0) {
     d->brick_auto = false;
     const double rho = d->pn / (d->rh[0].L[0] * d->rh[0].L[1] * d->rh[0].L[2]);
-    const int b = auto_brick(d, rho, d->rh[0].L, kDomainBrickKb);
-    if (b != d->bd[0] || b != d->bd[1] || b != d->bd[2]) {
-      for (int k = 0; k < 3; k++) d->bd[k] = b;
+    int b[3];
+    auto_brick(d, rho, d->rh[0].L, brick_budget(d), b);
+    if (b[0] != d->bd[0] || b[1] != d->bd[1] || b[2] != d->bd[2]) {
+      for (int k = 0; k < 3; k++) d->bd[k] = b[k];
       CHK(set_geometry(d));
     }
   }
@@ -958,8 +1013,14 @@ int is_expired_ensemble(mdgpu_domain *d, int *expired) {
   *expired = 0;
   if (nflag == 0) return MDGPU_OK;
   HIPCHK(d, hipMemcpyAsync(d->which, d->h_which, nflag * sizeof(int), hipMemcpyHostToDevice, st));
-  hipLaunchKernelGGL(k_check_replicas, dim3(nflag), dim3(kCheckBlock), 0, st, d->pos, d->pos_old, d->which, d->reps,
-                     d->grng, d->pn, d->margin, d->SL * d->SL, d->CL2, d->check_list_length, d->rep_exp);
+  const int cap = d->check_list_length;
+  hipLaunchKernelGGL(k_zero_check, dim3(nblk(nflag)), dim3(kBlock), 0, st, d->chk_dmax, d->chk_cnt, nflag);
+  hipLaunchKernelGGL(k_check_dmax, dim3(nflag * kCheckBlocks), dim3(kBlock), 0, st, d->pos, d->pos_old, d->which,
+                     d->reps, d->grng, d->pn, d->chk_dmax);
+  hipLaunchKernelGGL(k_check_cand, dim3(nflag * kCheckBlocks), dim3(kBlock), 0, st, d->pos, d->pos_old, d->which,
+                     d->reps, d->grng, d->pn, d->margin, cap, d->chk_dmax, d->chk_cnt, d->chk_cand);
+  hipLaunchKernelGGL(k_check_pairs, dim3(nflag), dim3(kBlock), 0, st, d->pos, d->pos_old, d->which, d->reps,
+                     d->grng, cap, d->SL * d->SL, d->CL2, d->chk_cnt, d->chk_cand, d->rep_exp);
   CHK(check_launch(d, "displacement check"));
   HIPCHK(d, hipMemcpyAsync(d->h_rep_exp, d->rep_exp, nflag * sizeof(int), hipMemcpyDeviceToHost, st));
   CHK(sync(d));
@@ -1006,7 +1067,7 @@ BrickList brick_list(const mdgpu_domain *d, int bytes_per_entry, bool inner = fa
   bl.vslice0 = d->vslice0;
   bl.brep = d->brep;
   bl.reps = d->reps;
-  bl.lds_cap = lds_cap_for(d, bytes_per_entry, d->lds_kb);
+  bl.lds_cap = lds_cap_for(d, bytes_per_entry, force_lds_kb(d));
   return bl;
 }
 
@@ -1409,8 +1470,7 @@ int create_domain(const mdgpu_params *prm, int nrep, const mdgpu_replica *rl, co
       d->rst[r].gamma = rl[r].gamma;
       d->rst[r].seed = rl[r].seed;
       const double rho = counts[r] / (R.L[0] * R.L[1] * R.L[2]);
-      const int b = auto_brick(d, rho, R.L, kReplicaBrickKb);
-      for (int k = 0; k < 3; k++) R.bd[k] = b;
+      auto_brick(d, rho, R.L, brick_budget(d), R.bd);
       R.p0 = (int)p0;
       R.pn = counts[r];
       p0 += counts[r];
@@ -1421,7 +1481,8 @@ int create_domain(const mdgpu_params *prm, int nrep, const mdgpu_replica *rl, co
   int r;
   if ((r = dalloc(d, &d->reps, nrep)) || (r = dalloc(d, &d->rep_v2, nrep)) || (r = dalloc(d, &d->rep_out, 3 * nrep)) ||
       (r = dalloc(d, &d->which, nrep)) || (r = dalloc(d, &d->rep_exp, nrep)) || (r = dalloc(d, &d->grng, nrep)) ||
-      (r = dalloc(d, &d->clock, 1))) {
+      (r = dalloc(d, &d->clock, 1)) || (r = dalloc(d, &d->chk_dmax, nrep)) || (r = dalloc(d, &d->chk_cnt, nrep)) ||
+      (r = dalloc(d, &d->chk_cand, (size_t)nrep * d->check_list_length))) {
     mdgpu_destroy(d);
     return r;
   }
@@ -1537,7 +1598,8 @@ int mdgpu_destroy(mdgpu_domain *d) {
                   d->partial, d->maxpart,     d->bpart,       d->cand,      d->u64,      d->i64,
                   d->i32,     d->d3,          d->reps,        d->brep,      d->rep,      d->rep_alt,
                   d->grng,    d->rep_v2,      d->rep_out,     d->which,     d->rep_exp,  d->clock,
-                  d->perm_in, d->nbr_in,      d->ncode_in,    d->slen_in};
+                  d->perm_in, d->nbr_in,      d->ncode_in,    d->slen_in,   d->chk_dmax, d->chk_cnt,
+                  d->chk_cand};
   for (StepGraph &g : d->graphs) drop_graph(g);
   for (void *b : bufs)
     if (b) (void)hipFree(b);
@@ -2078,7 +2140,9 @@ int mdgpu_set_option(mdgpu_domain *d, int option, int value) {
       return MDGPU_OK;
     case MDGPU_OPT_BRICK: {
       const int bx = value & 0xff, by = (value >> 8) & 0xff, bz = (value >> 16) & 0xff;
-      if (bx < 1 || bx > 4 || by < 1 || by > 4 || bz < 1 || bz > 4) return MDGPU_ERR_ARGUMENT;
+      if (bx < 1 || bx > kMaxBrickEdge || by < 1 || by > kMaxBrickEdge || bz < 1 || bz > kMaxBrickEdge ||
+          (bx + 2) * (by + 2) * (bz + 2) > kMaxBrickCells)
+        return MDGPU_ERR_ARGUMENT;
       d->brick_auto = false;
       d->bd[0] = bx;
       d->bd[1] = by;
@@ -2104,6 +2168,20 @@ int mdgpu_set_option(mdgpu_domain *d, int option, int value) {
       if (value < 0 || value > 1000) return MDGPU_ERR_ARGUMENT;
       d->prune_delta = value * 1.0e-3;
       return MDGPU_OK;
+    case MDGPU_OPT_ROW_SORT:
+      d->row_sort = value != 0;
+      d->initialized = false;  // the next step rebuilds the list
+      return MDGPU_OK;
+    case MDGPU_OPT_BRICK_KB:
+      if (value < 8 || value > 160) return MDGPU_ERR_ARGUMENT;
+      d->brick_kb = value;
+      if (d->nrep == 1) {
+        d->brick_auto = true;  // chosen at the next construction, which the next step runs
+        d->initialized = false;
+        return MDGPU_OK;
+      }
+      for (Replica &R : d->rh) auto_brick(d, R.pn / (R.L[0] * R.L[1] * R.L[2]), R.L, value, R.bd);
+      return set_geometry(d);  // the next step rebuilds the list
   }
   return MDGPU_ERR_ARGUMENT;
 }
@@ -2114,12 +2192,14 @@ int mdgpu_get_option(const mdgpu_domain *d, int option, int *value) {
     case MDGPU_OPT_UNROLL: *value = d->unroll; return MDGPU_OK;
     case MDGPU_OPT_KAHAN: *value = d->kahan; return MDGPU_OK;
     case MDGPU_OPT_WAVES: *value = d->waves; return MDGPU_OK;
-    case MDGPU_OPT_LDS_KB: *value = std::min(d->lds_kb, d->lds_max_kb); return MDGPU_OK;
+    case MDGPU_OPT_LDS_KB: *value = std::min(force_lds_kb(d), d->lds_max_kb); return MDGPU_OK;
     case MDGPU_OPT_BRICK: *value = d->bd[0] | (d->bd[1] << 8) | (d->bd[2] << 16); return MDGPU_OK;
     case MDGPU_OPT_GRAPH: *value = d->use_graph ? 1 : 0; return MDGPU_OK;
     case MDGPU_OPT_SEARCH_WAVES: *value = d->search_waves; return MDGPU_OK;
     case MDGPU_OPT_SEARCH_STAGE: *value = d->search_stage; return MDGPU_OK;
     case MDGPU_OPT_PRUNE: *value = (int)std::lround(d->prune_delta * 1.0e3); return MDGPU_OK;
+    case MDGPU_OPT_ROW_SORT: *value = d->row_sort; return MDGPU_OK;
+    case MDGPU_OPT_BRICK_KB: *value = brick_budget(d); return MDGPU_OK;
   }
   return MDGPU_ERR_ARGUMENT;
 }

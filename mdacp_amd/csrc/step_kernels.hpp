@@ -412,73 +412,114 @@ __global__ __launch_bounds__(kBlock) void k_replica_kinetic(const double4 *__res
   if (threadIdx.x == 0) out[blockIdx.x] = 0.0 + t;
 }
 
-// PairList::CheckByDisplacement (pairlist.cc:68-121) of the replicas listed in
-// `which`, one workgroup each, over the replica's owned entries [p0, p0 + pn)
-// and its ghosts [pn_all + g0, pn_all + g0 + ng) (grng[r] = {g0, ng}):
-// displacements since the build, candidates displaced by more than
-// margin - max displacement (expired once Nl = min(cap, n) of them are
-// found), then the candidate pairs that were beyond SL at the build and are
-// now inside rc.  expired[k] receives the answer for replica which[k].
-constexpr int kCheckCap = 1024;    // largest CheckListLength an ensemble accepts
-constexpr int kCheckBlock = 1024;  // threads per replica
+// PairList::CheckByDisplacement (pairlist.cc:68-121) of the replicas listed
+// in `which`, over each replica's owned entries [p0, p0 + pn) and its ghosts
+// [pn_all + g0, pn_all + g0 + ng) (grng[r] = {g0, ng}), in three launches:
+//  1. k_check_dmax, kCheckBlocks workgroups per listed replica: the largest
+//     displacement since the build, into dmax[k] as the bits of a double
+//     (non-negative doubles order like their bits);
+//  2. k_check_cand, as many workgroups: the entries displaced by more than
+//     margin - max displacement, up to Nl = min(cap, n) of them stored, all of
+//     them counted in cnt[k];
+//  3. k_check_pairs, one workgroup per listed replica: expired once Nl
+//     candidates were found, else when a candidate pair that was beyond SL at
+//     the build is now inside rc.
+// dmax and cnt start at zero (k_zero_check); expired[k] receives the answer
+// for replica which[k].  The decision does not depend on the order in which
+// candidates are stored.
+constexpr int kCheckCap = 1024;   // largest CheckListLength an ensemble accepts (cap² pair tests per workgroup)
+constexpr int kCheckBlocks = 32;  // workgroups per listed replica in the first two launches
 
-__global__ __launch_bounds__(kCheckBlock) void k_check_replicas(const double4 *__restrict__ pos,
-                                                           const double4 *__restrict__ old,
-                                                           const int *__restrict__ which,
-                                                           const Replica *__restrict__ reps,
-                                                           const int2 *__restrict__ grng, int pn_all, double margin,
-                                                           double SL2, double CL2, int cap,
-                                                           int *__restrict__ expired) {
-  __shared__ int cand[kCheckCap];
-  __shared__ int count, flag;
-  const int r = which[blockIdx.x];
+__device__ __forceinline__ int check_entry(int k, const Replica &R, int no, int g0, int pn_all) {
+  return k < no ? R.p0 + k : pn_all + g0 + (k - no);
+}
+
+__global__ __launch_bounds__(kBlock) void k_check_dmax(const double4 *__restrict__ pos,
+                                                       const double4 *__restrict__ old,
+                                                       const int *__restrict__ which, const Replica *__restrict__ reps,
+                                                       const int2 *__restrict__ grng, int pn_all,
+                                                       unsigned long long *__restrict__ dmax) {
+  const int kr = blockIdx.x / kCheckBlocks, part = blockIdx.x % kCheckBlocks;
+  const int r = which[kr];
   const Replica &R = reps[r];
   const int no = R.pn, g0 = grng[r].x, n = no + grng[r].y;
   double m = 0.0;
-  for (int k = threadIdx.x; k < n; k += kCheckBlock) {
-    const int i = k < no ? R.p0 + k : pn_all + g0 + (k - no);
+  for (int k = part * kBlock + threadIdx.x; k < n; k += kCheckBlocks * kBlock) {
+    const int i = check_entry(k, R, no, g0, pn_all);
     const double4 a = old[i], b = pos[i];
     const double dx = a.x - b.x, dy = a.y - b.y, dz = a.z - b.z;
     m = fmax(m, dx * dx + dy * dy + dz * dz);
   }
-  const double dmax2 = block_max_all<kCheckBlock>(m);
+  m = block_max_all(m);
+  if (threadIdx.x == 0) atomicMax(&dmax[kr], (unsigned long long)__double_as_longlong(m));
+}
+
+__global__ __launch_bounds__(kBlock) void k_check_cand(const double4 *__restrict__ pos,
+                                                       const double4 *__restrict__ old,
+                                                       const int *__restrict__ which, const Replica *__restrict__ reps,
+                                                       const int2 *__restrict__ grng, int pn_all, double margin,
+                                                       int cap, const unsigned long long *__restrict__ dmax,
+                                                       int *__restrict__ cnt, int *__restrict__ cand) {
+  const int kr = blockIdx.x / kCheckBlocks, part = blockIdx.x % kCheckBlocks;
+  const int r = which[kr];
+  const Replica &R = reps[r];
+  const int no = R.pn, g0 = grng[r].x, n = no + grng[r].y;
   const int Nl = n > cap ? cap : n;
-  const double c_distance = margin - sqrt(dmax2);
+  const double c_distance = margin - sqrt(__longlong_as_double((long long)dmax[kr]));
   const double c_d2 = c_distance * c_distance;
-  if (threadIdx.x == 0) {
-    count = 0;
-    flag = 0;
-  }
-  __syncthreads();
-  for (int k = threadIdx.x; k < n; k += kCheckBlock) {
-    const int i = k < no ? R.p0 + k : pn_all + g0 + (k - no);
+  for (int k = part * kBlock + threadIdx.x; k < n; k += kCheckBlocks * kBlock) {
+    const int i = check_entry(k, R, no, g0, pn_all);
     const double4 a = old[i], b = pos[i];
     const double dx = a.x - b.x, dy = a.y - b.y, dz = a.z - b.z;
     if (dx * dx + dy * dy + dz * dz > c_d2) {
-      const int c = atomicAdd(&count, 1);
-      if (c < Nl) cand[c] = i;
+      const int c = atomicAdd(&cnt[kr], 1);
+      if (c < Nl) cand[(long long)kr * cap + c] = i;
     }
   }
-  __syncthreads();
-  const int l = count;
+}
+
+__global__ __launch_bounds__(kBlock) void k_check_pairs(const double4 *__restrict__ pos,
+                                                        const double4 *__restrict__ old,
+                                                        const int *__restrict__ which,
+                                                        const Replica *__restrict__ reps,
+                                                        const int2 *__restrict__ grng, int cap, double SL2,
+                                                        double CL2, const int *__restrict__ cnt,
+                                                        const int *__restrict__ cand, int *__restrict__ expired) {
+  __shared__ int flag;
+  const int kr = blockIdx.x;
+  const int r = which[kr];
+  const int n = reps[r].pn + grng[r].y;
+  const int Nl = n > cap ? cap : n;
+  const int l = cnt[kr];
   if (l > 0 && l >= Nl) {
-    if (threadIdx.x == 0) expired[blockIdx.x] = 1;
+    if (threadIdx.x == 0) expired[kr] = 1;
     return;
   }
-  for (int t = threadIdx.x; t < l * l; t += kCheckBlock) {
+  if (threadIdx.x == 0) flag = 0;
+  __syncthreads();
+  const int *c = cand + (long long)kr * cap;
+  for (int t = threadIdx.x; t < l * l; t += kBlock) {
     const int a = t / l, b = t % l;
     if (b <= a) continue;
-    const double4 o1 = old[cand[a]], o2 = old[cand[b]];
+    const double4 o1 = old[c[a]], o2 = old[c[b]];
     double dx = o1.x - o2.x, dy = o1.y - o2.y, dz = o1.z - o2.z;
     if (dx * dx + dy * dy + dz * dz < SL2) continue;
-    const double4 q1 = pos[cand[a]], q2 = pos[cand[b]];
+    const double4 q1 = pos[c[a]], q2 = pos[c[b]];
     dx = q1.x - q2.x;
     dy = q1.y - q2.y;
     dz = q1.z - q2.z;
     if (dx * dx + dy * dy + dz * dz < CL2) flag = 1;
   }
   __syncthreads();
-  if (threadIdx.x == 0) expired[blockIdx.x] = flag;
+  if (threadIdx.x == 0) expired[kr] = flag;
+}
+
+__global__ void k_zero_check(unsigned long long *__restrict__ dmax, int *__restrict__ cnt, int n) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n) {
+    dmax[k] = 0ull;
+    cnt[k] = 0;
+  }
 }
 
 // Ghost ranges of each replica after a construction: replica r's images are

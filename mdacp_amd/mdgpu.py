@@ -30,7 +30,7 @@ EXPORTS = [
     "mdgpu_set_replica_thermostat", "mdgpu_observe_replicas", "mdgpu_get_replica_pairs",
 ]
 OPT_UNROLL, OPT_KAHAN, OPT_WAVES, OPT_LDS_KB, OPT_BRICK, OPT_GRAPH = 1, 2, 3, 4, 5, 6
-OPT_SEARCH_WAVES, OPT_SEARCH_STAGE, OPT_PRUNE = 7, 8, 9
+OPT_SEARCH_WAVES, OPT_SEARCH_STAGE, OPT_PRUNE, OPT_ROW_SORT, OPT_BRICK_KB = 7, 8, 9, 10, 11
 
 
 class Params(ctypes.Structure):
@@ -380,6 +380,8 @@ class Ensemble(Domain):
         c = Ensemble(self.boxes, self.replica_counts(), q, p, t, T=self.ens["T"], gamma=self.ens["gamma"],
                      seed=self.ens["seed"], thermostat=self.ens["thermostat"], **a)
         c.copy_options(self)
+        if c.get_option(OPT_BRICK_KB) != self.get_option(OPT_BRICK_KB):
+            c.set_option(OPT_BRICK_KB, self.get_option(OPT_BRICK_KB))
         c.make_pair_list()
         return c
 

@@ -61,7 +61,7 @@ def test_picks_the_fastest_shapes(monkeypatch):
     ch = res["choice"]
     assert (ch["search_waves"], ch["search_stage"]) == (8, 0)
     assert (ch["waves"], ch["lds_kb"], ch["unroll"]) == (4, 72, 16)
-    assert ch["brick"] == 3
+    assert ch["brick"] == (3, 3, 3)
     # applied to the caller, and the clone was released
     assert d.opts[mdgpu.OPT_SEARCH_WAVES] == 8 and d.opts[mdgpu.OPT_SEARCH_STAGE] == 0
     assert d.opts[mdgpu.OPT_WAVES] == 4 and d.opts[mdgpu.OPT_LDS_KB] == 72 and d.opts[mdgpu.OPT_UNROLL] == 16
