@@ -343,6 +343,165 @@ __global__ __launch_bounds__(kWave *W) void k_search_brick(const double4 *__rest
   }
 }
 
+__device__ __forceinline__ double readlane_f64(double v, int l) {
+  const long long b = __double_as_longlong(v);
+  const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)b, l);
+  const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(b >> 32), l);
+  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
+// Rows ur0 .. ur0 + nr - 1 (staged indices) of a cell against this lane's
+// candidate (cx, cy, cz): r^2 without fused multiply-adds, kept when
+// !(r2 > S2).  Lane r keeps row r's ballot in (lo, hi).  Rows come from
+// LDS (kStaged) or from lane r's registers (ax, ay, az), four at a time.
+template <bool kStaged>
+__device__ __forceinline__ void row_position(int r, int ur0, const double *__restrict__ lds, double ax, double ay,
+                                             double az, double &qx, double &qy, double &qz) {
+  if (kStaged) {
+    qx = lds[3 * (ur0 + r)];
+    qy = lds[3 * (ur0 + r) + 1];
+    qz = lds[3 * (ur0 + r) + 2];
+  } else {
+    qx = readlane_f64(ax, r);
+    qy = readlane_f64(ay, r);
+    qz = readlane_f64(az, r);
+  }
+}
+
+__device__ __forceinline__ void test_row(int r, int lane, double qx, double qy, double qz, double cx, double cy,
+                                         double cz, double S2, uint32_t &lo, uint32_t &hi) {
+  const double ex = qx - cx, ey = qy - cy, ez = qz - cz;
+  const double r2 = (ex * ex + ey * ey + ez * ez);
+  const unsigned long long m = __ballot(!(r2 > S2));
+  if (lane == r) {
+    lo = (uint32_t)m;
+    hi = (uint32_t)(m >> 32);
+  }
+}
+
+template <bool kStaged>
+__device__ __forceinline__ void test_rows(int nr, int ur0, const double *__restrict__ lds, double ax, double ay,
+                                          double az, double cx, double cy, double cz, double S2, int lane,
+                                          uint32_t &lo, uint32_t &hi) {
+  int r = 0;
+  for (; r + 4 <= nr; r += 4) {
+    double q[4][3];
+#pragma unroll
+    for (int j = 0; j < 4; j++) row_position<kStaged>(r + j, ur0, lds, ax, ay, az, q[j][0], q[j][1], q[j][2]);
+#pragma unroll
+    for (int j = 0; j < 4; j++) test_row(r + j, lane, q[j][0], q[j][1], q[j][2], cx, cy, cz, S2, lo, hi);
+  }
+  for (; r < nr; r++) {
+    double qx, qy, qz;
+    row_position<kStaged>(r, ur0, lds, ax, ay, az, qx, qy, qz);
+    test_row(r, lane, qx, qy, qz, cx, cy, cz, S2, lo, hi);
+  }
+}
+
+// The same search and the same masks, one wavefront per cell of the brick:
+// the cell's candidates (its 9 runs, concatenated) are taken 64 at a time, one
+// per lane, and each row of the cell is tested against all 64 at once; the
+// wavefront's ballot of the test is bits [64c, 64c + 64) of that row's mask.
+// Rows are read at one address for the whole wavefront (an LDS broadcast) and
+// every lane does the same work.  Lane r collects the mask words of row r of
+// each block of up to 64 rows of the cell, clears the row's own bit (the
+// row sits in the middle run, at its offset in its cell's owned range) and
+// counts the rest.
+template <int W>
+__global__ __launch_bounds__(kWave *W) void k_search_cells(const double4 *__restrict__ pos, Bricks B,
+                                                          SearchTables st, double S2, int mcap,
+                                                          int *__restrict__ row_len, uint32_t *__restrict__ mask,
+                                                          int *__restrict__ err) {
+  extern __shared__ __align__(16) double lds64[];
+  __shared__ int tb[kMaxRanges + 1];
+  constexpr int T = kWave * W;
+  const int b = xcd_block(blockIdx.x, gridDim.x);
+  if (st.nrow[b] == 0) return;  // a brick without rows
+  const int R = load_tb<T>(st, b, tb);
+  __syncthreads();
+  const int total = tb[R];
+  const int *gl = st.glist + st.gbase[b];
+  const bool staged = total < st.lds_cap;
+  if (staged) stage_fp64<T>(gl, total, pos, lds64);
+  __syncthreads();
+  const Replica &Rp = B.reps[B.brep[b]];
+  const BrickBox X = brick_box(b, Rp);
+  const int *ts = st.tstart + (long long)b * st.rstride;
+  const int wv = threadIdx.x / kWave, lane = threadIdx.x % kWave;
+  // the brick's own cells are the neighbourhood less its outer layer
+  const int ox = X.n[0] - 2, oy = X.n[1] - 2, oz = X.n[2] - 2;
+  for (int c = wv; c < ox * oy * oz; c += W) {
+    const int lx = 1 + c % ox, ly = 1 + (c / ox) % oy, lz = 1 + c / (ox * oy);
+    const int rs = kRangesPerCell * ((lz * X.n[1] + ly) * X.n[0] + lx);
+    const int urow0 = __builtin_amdgcn_readfirstlane(tb[rs]);  // staged index of the cell's first row
+    const int nrows = __builtin_amdgcn_readfirstlane(tb[rs + 1]) - urow0;
+    if (nrows == 0) continue;
+    const int irow0 = __builtin_amdgcn_readfirstlane(ts[rs]);  // particle index of that row
+    // runs of the candidate sequence: staged start ru[q], first candidate rp[q]
+    int ru[9], rp[10];
+    rp[0] = 0;
+#pragma unroll
+    for (int q = 0; q < 9; q++) {
+      const int r0 = ((lz + q / 3 - 1) * X.n[1] + (ly + q % 3 - 1)) * X.n[0] + lx - 1;
+      ru[q] = __builtin_amdgcn_readfirstlane(tb[kRangesPerCell * r0]);
+      rp[q + 1] = rp[q] + __builtin_amdgcn_readfirstlane(tb[kRangesPerCell * (r0 + 3)]) - ru[q];
+    }
+    const int ncand = rp[9];
+    if (ncand > 32 * mcap) {  // the host retries with longer masks
+      if (lane == 0) atomicOr(err, kErrSearchCap);
+      continue;
+    }
+    const int nw = (ncand + 31) / 32;
+    for (int rb = 0; rb < nrows; rb += kWave) {
+      const int nr = min(kWave, nrows - rb);
+      int cnt = 0;
+      double ax = 0.0, ay = 0.0, az = 0.0;  // without staging, lane r holds row rb + r
+      if (!staged && lane < nr) {
+        const double4 q = pos[irow0 + rb + lane];
+        ax = q.x;
+        ay = q.y;
+        az = q.z;
+      }
+      for (int c0 = 0; c0 < ncand; c0 += kWave) {
+        const int k = c0 + lane;
+        int u = ru[0] + k;
+#pragma unroll
+        for (int q = 1; q < 9; q++)
+          if (k >= rp[q]) u = ru[q] + (k - rp[q]);
+        double cx = 1.0e200, cy = 1.0e200, cz = 1.0e200;  // lanes past the last candidate
+        if (k < ncand) {
+          if (staged) {
+            cx = lds64[3 * u];
+            cy = lds64[3 * u + 1];
+            cz = lds64[3 * u + 2];
+          } else {
+            const double4 q = pos[gl[u]];
+            cx = q.x;
+            cy = q.y;
+            cz = q.z;
+          }
+        }
+        uint32_t lo = 0, hi = 0;
+        if (staged)
+          test_rows<true>(nr, urow0 + rb, lds64, 0.0, 0.0, 0.0, cx, cy, cz, S2, lane, lo, hi);
+        else
+          test_rows<false>(nr, urow0 + rb, nullptr, ax, ay, az, cx, cy, cz, S2, lane, lo, hi);
+        if (lane < nr) {
+          // the row itself is candidate kself (the middle run holds the row's cell)
+          const int kself = rp[4] + (urow0 + rb + lane - ru[4]) - c0;
+          if (kself >= 0 && kself < 32) lo &= ~(1u << kself);
+          else if (kself >= 32 && kself < 64) hi &= ~(1u << (kself - 32));
+          cnt += __popc(lo) + __popc(hi);
+          uint32_t *mk = mask + (long long)(irow0 + rb + lane) * mcap + c0 / 32;
+          mk[0] = lo;
+          if (c0 / 32 + 1 < nw) mk[1] = hi;
+        }
+      }
+      if (lane < nr) row_len[irow0 + rb + lane] = cnt;
+    }
+  }
+}
+
 // ---- row order inside a brick ------------------------------------------
 constexpr int kSortBins = kBlock;  // lengths >= kSortBins - 1 share the last bin
 
@@ -440,41 +599,62 @@ __global__ void k_slice_len(const int *__restrict__ row_len, const int *__restri
   block_max_to_partial((double)m, partial, blockIdx.x);
 }
 
+// One 8-entry group of a row's list: staged indices (16 bits each) and cell
+// offset codes (8 bits each), packed in registers as the entries come and
+// stored as one 16-byte and one 8-byte piece (the layout of slot_of(): group g
+// of lane l is 16-byte piece g * 64 + l of the slice).
+struct GroupPack {
+  unsigned long long lo, hi, cd;
+  __device__ __forceinline__ void clear() { lo = hi = cd = 0ull; }
+  __device__ __forceinline__ void put(int j, unsigned u, unsigned c) {
+    const unsigned long long v = (unsigned long long)u << (16 * (j & 3));
+    if (j < 4) lo |= v;
+    else hi |= v;
+    cd |= (unsigned long long)c << (8 * j);
+  }
+  __device__ __forceinline__ void store(uint4 *gn, uint2 *gc, int piece) {
+    gn[piece] = make_uint4((unsigned)lo, (unsigned)(lo >> 32), (unsigned)hi, (unsigned)(hi >> 32));
+    gc[piece] = make_uint2((unsigned)cd, (unsigned)(cd >> 32));
+    clear();
+  }
+};
+
+constexpr int kEmitWaves = 4;  // wavefronts per brick workgroup of k_emit_brick
+
 // MeshList::MakeList (meshlist.cc:58-87) in sliced-ELL form: one wavefront per
 // 64-row slice (rows in the order of k_brick_row_order, rowsrc[row] = the
-// row's index in the search) turns the rows' candidate masks into staged indices and cell
-// offset codes, pads every row to the slice length with the brick's pad
-// entry, assembles the slice in LDS in the layout of slot_of() and copies it
-// out in 16-byte pieces.  scap = longest slice (LDS: W * scap * 64 * 3 bytes).
+// row's index in the search) turns the rows' candidate masks into staged
+// indices and cell offset codes, each lane packing its row's entries 8 at a
+// time and storing each group as it fills; every row is padded to the slice
+// length with the brick's pad entry.
 template <int W>
 __global__ __launch_bounds__(kWave *W) void k_emit_brick(const int *__restrict__ cell, Bricks B, SearchTables st,
                                                         const int *__restrict__ rowsrc,
                                                         const uint32_t *__restrict__ mask, int mcap,
                                                         const long long *__restrict__ sbase,
-                                                        const int *__restrict__ slen, int scap,
-                                                        uint16_t *__restrict__ nbr, uint8_t *__restrict__ ncode,
-                                                        int *__restrict__ err) {
-  extern __shared__ uint4 ebuf[];
+                                                        const int *__restrict__ slen, uint16_t *__restrict__ nbr,
+                                                        uint8_t *__restrict__ ncode, int *__restrict__ err) {
   __shared__ int tb[kMaxRanges + 1];
   constexpr int T = kWave * W;
   const int b = xcd_block(blockIdx.x, gridDim.x);
   if (st.nrow[b] == 0) return;  // a brick of ghost-layer cells only
   const int R = load_tb<T>(st, b, tb);
   __syncthreads();
-  const int total = tb[R];
+  const unsigned total = (unsigned)tb[R];
   const Replica &Rp = B.reps[B.brep[b]];
   const BrickBox X = brick_box(b, Rp);
   const int *ts = st.tstart + (long long)b * st.rstride;
   const int nrow = st.nrow[b], row0 = st.row0[b], vs0 = st.vslice0[b];
   const int wv = threadIdx.x / kWave, lane = threadIdx.x % kWave;
-  uint16_t *bn = reinterpret_cast<uint16_t *>(ebuf) + (long long)wv * scap * kWave;
-  uint8_t *bc = reinterpret_cast<uint8_t *>(reinterpret_cast<uint16_t *>(ebuf) + (long long)W * scap * kWave) +
-                (long long)wv * scap * kWave;
   const int nsl = (nrow + kWave - 1) / kWave;
   for (int sl = wv; sl < nsl; sl += W) {
     const int s = vs0 + sl;
-    const int len = slen[s];
+    const int len = slen[s];  // a multiple of kGroup
     const int v = sl * kWave + lane;
+    uint4 *gn = reinterpret_cast<uint4 *>(nbr + sbase[s]);  // sbase: a multiple of 512
+    uint2 *gc = reinterpret_cast<uint2 *>(ncode + sbase[s]);
+    GroupPack pk;
+    pk.clear();
     int cnt = 0;
     if (v < nrow) {
       const int i = row0 + v;
@@ -500,8 +680,8 @@ __global__ __launch_bounds__(kWave *W) void k_emit_brick(const int *__restrict__
               const int u = u0 + k + p;
               const int dx = u < t1 ? -1 : (u < t2 ? 0 : 1);
               if (cnt < len) {
-                bn[slot_of(cnt, lane)] = (uint16_t)u;
-                bc[slot_of(cnt, lane)] = offset_code(dx, dy, dz);
+                pk.put(cnt & (kGroup - 1), (unsigned)u, offset_code(dx, dy, dz));
+                if ((cnt & (kGroup - 1)) == kGroup - 1) pk.store(gn, gc, (cnt / kGroup) * kWave + lane);
               }
               cnt++;
             }
@@ -512,23 +692,18 @@ __global__ __launch_bounds__(kWave *W) void k_emit_brick(const int *__restrict__
       }
     }
     if (cnt > len) atomicOr(err, kErrCapacity);
-    for (int k = cnt; k < len; k++) {
-      bn[slot_of(k, lane)] = (uint16_t)total;
-      bc[slot_of(k, lane)] = (uint8_t)kZeroOffset;
+    int k = min(cnt, len);
+    if (k & (kGroup - 1)) {  // pad the open group
+      for (int j = k & (kGroup - 1); j < kGroup; j++) pk.put(j, total, kZeroOffset);
+      pk.store(gn, gc, (k / kGroup) * kWave + lane);
+      k = (k / kGroup + 1) * kGroup;
     }
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
-    // the slice is len*64 entries at sbase (a multiple of 512): 16-byte pieces
-    uint4 *gn = reinterpret_cast<uint4 *>(nbr + sbase[s]);
-    uint4 *gc = reinterpret_cast<uint4 *>(ncode + sbase[s]);
-    const uint4 *ln = reinterpret_cast<const uint4 *>(bn);
-    const uint4 *lc = reinterpret_cast<const uint4 *>(bc);
-    for (int t = lane; t < len * 8; t += kWave) gn[t] = ln[t];
-    for (int t = lane; t < len * 4; t += kWave) gc[t] = lc[t];
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+    const uint4 pad = make_uint4(total | total << 16, total | total << 16, total | total << 16, total | total << 16);
+    const unsigned zc = kZeroOffset * 0x01010101u;
+    for (; k < len; k += kGroup) {
+      gn[(k / kGroup) * kWave + lane] = pad;
+      gc[(k / kGroup) * kWave + lane] = make_uint2(zc, zc);
+    }
   }
 }
 

@@ -61,6 +61,7 @@ struct mdgpu_domain {
   int search_waves = 16;  // wavefronts per brick workgroup of the neighbour search: 4, 8 or 16
   int search_stage = 1;   // neighbour search reads candidates staged in LDS (1) or from global memory (0)
   int row_sort = 1;       // rows of a brick ordered by list length (1) or kept in cell order (0)
+  int search_cells = 1;   // neighbour search by a wavefront per cell (1) or a thread per row (0)
 
   // SimulationInfo / MDRect / MeshList geometry
   double L[3], rs[3], re[3];
@@ -577,14 +578,20 @@ template <int W>
 void launch_search_w(mdgpu_domain *d, const Bricks &B, SearchTables t) {
   if (!d->search_stage) t.lds_cap = 0;  // every brick reads its candidates from global memory
   const size_t shm = (size_t)t.lds_cap * 3 * sizeof(double);
-  allow_lds(k_search_brick<W>, shm);
-  hipLaunchKernelGGL(k_search_brick<W>, dim3(d->nbrick), dim3(kWave * W), shm, d->stream, d->pos, d->cell, B, t,
-                     d->S2, d->mcap, d->row_len, d->mask, d->i32);
+  if (d->search_cells) {
+    allow_lds(k_search_cells<W>, shm);
+    hipLaunchKernelGGL(k_search_cells<W>, dim3(d->nbrick), dim3(kWave * W), shm, d->stream, d->pos, B, t, d->S2,
+                       d->mcap, d->row_len, d->mask, d->i32);
+  } else {
+    allow_lds(k_search_brick<W>, shm);
+    hipLaunchKernelGGL(k_search_brick<W>, dim3(d->nbrick), dim3(kWave * W), shm, d->stream, d->pos, d->cell, B, t,
+                       d->S2, d->mcap, d->row_len, d->mask, d->i32);
+  }
 }
 
-// One workgroup per brick, a thread per row; candidates from the brick's
-// neighbourhood staged in LDS like the force kernel, or read from global
-// memory (L2) when bricks hold few rows for the size of their neighbourhood.
+// One workgroup per brick, a wavefront per cell (or a thread per row);
+// candidates from the brick's neighbourhood staged in LDS like the force
+// kernel, or read from global memory (L2).
 int launch_search(mdgpu_domain *d, const Bricks &B, const SearchTables &t) {
   switch (d->search_waves) {
     case 4: launch_search_w<4>(d, B, t); break;
@@ -594,25 +601,10 @@ int launch_search(mdgpu_domain *d, const Bricks &B, const SearchTables &t) {
   return check_launch(d, "neighbour search");
 }
 
-template <int W>
-void launch_emit_w(mdgpu_domain *d, const Bricks &B, const SearchTables &t, int scap) {
-  const size_t shm = (size_t)W * scap * kWave * 3;
-  allow_lds(k_emit_brick<W>, shm);
-  hipLaunchKernelGGL(k_emit_brick<W>, dim3(d->nbrick), dim3(kWave * W), shm, d->stream, d->cell, B, t, d->rowsrc,
-                     d->mask, d->mcap, d->sbase, d->slen, scap, d->nbr, d->ncode, d->i32);
-}
-
-// the emit kernel keeps one slice (scap x 64 entries of 3 bytes) per wavefront in LDS
-int launch_emit(mdgpu_domain *d, const Bricks &B, const SearchTables &t, int scap) {
-  const size_t per_wave = (size_t)scap * kWave * 3;
-  const size_t limit = (size_t)d->lds_max_kb * 1024;
-  if (4 * per_wave <= 80 * 1024) launch_emit_w<4>(d, B, t, scap);
-  else if (2 * per_wave <= limit) launch_emit_w<2>(d, B, t, scap);
-  else if (per_wave <= limit) launch_emit_w<1>(d, B, t, scap);
-  else {
-    d->err = "a pair-list slice is too long for the emit kernel's LDS";
-    return MDGPU_ERR_CAPACITY;
-  }
+// one wavefront per 64-row slice, four per brick workgroup
+int launch_emit(mdgpu_domain *d, const Bricks &B, const SearchTables &t) {
+  hipLaunchKernelGGL(k_emit_brick<kEmitWaves>, dim3(d->nbrick), dim3(kWave * kEmitWaves), 0, d->stream, d->cell, B,
+                     t, d->rowsrc, d->mask, d->mcap, d->sbase, d->slen, d->nbr, d->ncode, d->i32);
   return check_launch(d, "neighbour list emit");
 }
 
@@ -873,7 +865,7 @@ int make_pair_list(mdgpu_domain *d) {
     d->nbr_cap = nc;
   }
   d->nbr_slots = slots;
-  if (pn > 0 && scap > 0) CHK(launch_emit(d, B, STb, scap));
+  if (pn > 0 && scap > 0) CHK(launch_emit(d, B, STb));
   CHK(check_launch(d, "neighbour list"));
   // PairList::Init: snapshot of owned + ghost positions (pairlist.cc:11-23)
   if (d->tn > 0)
@@ -2172,6 +2164,9 @@ int mdgpu_set_option(mdgpu_domain *d, int option, int value) {
       d->row_sort = value != 0;
       d->initialized = false;  // the next step rebuilds the list
       return MDGPU_OK;
+    case MDGPU_OPT_SEARCH_CELLS:
+      d->search_cells = value != 0;
+      return MDGPU_OK;
     case MDGPU_OPT_BRICK_KB:
       if (value < 8 || value > 160) return MDGPU_ERR_ARGUMENT;
       d->brick_kb = value;
@@ -2200,6 +2195,7 @@ int mdgpu_get_option(const mdgpu_domain *d, int option, int *value) {
     case MDGPU_OPT_PRUNE: *value = (int)std::lround(d->prune_delta * 1.0e3); return MDGPU_OK;
     case MDGPU_OPT_ROW_SORT: *value = d->row_sort; return MDGPU_OK;
     case MDGPU_OPT_BRICK_KB: *value = brick_budget(d); return MDGPU_OK;
+    case MDGPU_OPT_SEARCH_CELLS: *value = d->search_cells; return MDGPU_OK;
   }
   return MDGPU_ERR_ARGUMENT;
 }

@@ -30,7 +30,7 @@ EXPORTS = [
     "mdgpu_set_replica_thermostat", "mdgpu_observe_replicas", "mdgpu_get_replica_pairs",
 ]
 OPT_UNROLL, OPT_KAHAN, OPT_WAVES, OPT_LDS_KB, OPT_BRICK, OPT_GRAPH = 1, 2, 3, 4, 5, 6
-OPT_SEARCH_WAVES, OPT_SEARCH_STAGE, OPT_PRUNE, OPT_ROW_SORT, OPT_BRICK_KB = 7, 8, 9, 10, 11
+OPT_SEARCH_WAVES, OPT_SEARCH_STAGE, OPT_PRUNE, OPT_ROW_SORT, OPT_BRICK_KB, OPT_SEARCH_CELLS = 7, 8, 9, 10, 11, 12
 
 
 class Params(ctypes.Structure):
@@ -223,7 +223,7 @@ class Domain:
 
     def copy_options(self, other):
         for o in (OPT_UNROLL, OPT_KAHAN, OPT_WAVES, OPT_LDS_KB, OPT_GRAPH, OPT_SEARCH_WAVES, OPT_SEARCH_STAGE,
-                  OPT_PRUNE):
+                  OPT_PRUNE, OPT_SEARCH_CELLS):
             self.set_option(o, other.get_option(o))
 
     def zeta(self):

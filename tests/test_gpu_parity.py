@@ -106,6 +106,32 @@ def test_built_state_parity_uneven_box_and_edges():
     compare_state(d, o, np.array(L), 2.5, 0.001, "uneven")
 
 
+@pytest.mark.parametrize("cells,stage", [(1, 1), (1, 0), (0, 1), (0, 0)])
+def test_search_kernels_agree_with_oracle(cells, stage):
+    """Both neighbour-search kernels, with and without LDS staging, list the oracle's pairs on a
+    fluid with a dense cluster (a cell of more than 64 rows, masks longer than the first guess)
+    and particles on the upper faces (binned into the upper ghost layer)."""
+    say("case: search kernel cells", cells, "stage", stage)
+    L = 16.0
+    q, p, ids = melted_state(L, 0.5)
+    rng = np.random.default_rng(5)
+    cl = 7.3 + rng.uniform(0.0, 0.6, (100, 3))
+    q = np.concatenate([q, cl])
+    q[:6, 2] = L
+    q[6:9, 0] = L
+    p = np.concatenate([p, np.zeros((100, 3))])
+    ids = np.concatenate([ids, np.arange(len(ids), len(ids) + 100, dtype=np.int32)])
+    d, o = make_pair(L, q, p, ids)
+    d.set_option(mdgpu.OPT_SEARCH_CELLS, cells)
+    d.set_option(mdgpu.OPT_SEARCH_STAGE, stage)
+    d.make_pair_list()
+    o.make_pair_list()
+    gp, op = d.pairs(), o.pairs()
+    assert np.array_equal(O.canonical_pairs(gp), O.canonical_pairs(op))
+    say("pairs", len(gp))
+    d.close()
+
+
 def test_golden_pairs_and_kicks_fcc2048(golden_dir):
     say("case: golden fixture fcc2048 step 0")
     g = load_golden("fcc2048_nve")
