@@ -599,34 +599,44 @@ __global__ void k_slice_len(const int *__restrict__ row_len, const int *__restri
   block_max_to_partial((double)m, partial, blockIdx.x);
 }
 
-// One 8-entry group of a row's list: staged indices (16 bits each) and cell
-// offset codes (8 bits each), packed in registers as the entries come and
-// stored as one 16-byte and one 8-byte piece (the layout of slot_of(): group g
-// of lane l is 16-byte piece g * 64 + l of the slice).
-struct GroupPack {
-  unsigned long long lo, hi, cd;
-  __device__ __forceinline__ void clear() { lo = hi = cd = 0ull; }
-  __device__ __forceinline__ void put(int j, unsigned u, unsigned c) {
-    const unsigned long long v = (unsigned long long)u << (16 * (j & 3));
-    if (j < 4) lo |= v;
-    else hi |= v;
-    cd |= (unsigned long long)c << (8 * j);
-  }
-  __device__ __forceinline__ void store(uint4 *gn, uint2 *gc, int piece) {
-    gn[piece] = make_uint4((unsigned)lo, (unsigned)(lo >> 32), (unsigned)hi, (unsigned)(hi >> 32));
-    gc[piece] = make_uint2((unsigned)cd, (unsigned)(cd >> 32));
-    clear();
-  }
+constexpr int kEmitWaves = 4;  // wavefronts per brick workgroup of k_emit_brick
+
+// A row's walk through its candidate mask during the emit: the current
+// 64-bit mask word (bits already taken cleared) and the next one, loaded ahead;
+// the current run q of the candidate sequence with its first candidate kb, its
+// end ke, its first staged entry u0 and the starts t1, t2 of its second and
+// third cells (x-neighbouring cells are adjacent in the staged order).
+struct MaskCursor {
+  const unsigned long long *mk;
+  unsigned long long w, wn;
+  int wi, nw, ncand;
+  int q, kb, ke, u0, t1, t2;
 };
 
-constexpr int kEmitWaves = 4;  // wavefronts per brick workgroup of k_emit_brick
+__device__ __forceinline__ unsigned long long mask_word(const MaskCursor &c, int wi) {
+  if (wi >= c.nw) return 0ull;
+  const unsigned long long v = c.mk[wi];
+  const int valid = c.ncand - 64 * wi;  // the search writes whole 32-bit words up to the last candidate
+  return valid >= 64 ? v : v & ((1ull << valid) - 1ull);
+}
+
+__device__ __forceinline__ void cursor_run(MaskCursor &c, const RowCell &rc, const BrickBox &X, const int *tb) {
+  const int r0 = ((rc.lz + c.q / 3 - 1) * X.n[1] + (rc.ly + c.q % 3 - 1)) * X.n[0] + rc.lx - 1;
+  c.u0 = tb[kRangesPerCell * r0];
+  c.t1 = tb[kRangesPerCell * (r0 + 1)];
+  c.t2 = tb[kRangesPerCell * (r0 + 2)];
+  c.kb = c.ke;
+  c.ke = c.kb + tb[kRangesPerCell * (r0 + 3)] - c.u0;
+}
 
 // MeshList::MakeList (meshlist.cc:58-87) in sliced-ELL form: one wavefront per
 // 64-row slice (rows in the order of k_brick_row_order, rowsrc[row] = the
 // row's index in the search) turns the rows' candidate masks into staged
-// indices and cell offset codes, each lane packing its row's entries 8 at a
-// time and storing each group as it fills; every row is padded to the slice
-// length with the brick's pad entry.
+// indices and cell offset codes.  All lanes produce entry j of group g
+// together, each from its own row's mask (or the brick's pad entry once the
+// row is exhausted), and store group g as one 16-byte piece of indices and one
+// 8-byte piece of codes per lane: piece g * 64 + lane of the slice, the layout
+// of slot_of().
 template <int W>
 __global__ __launch_bounds__(kWave *W) void k_emit_brick(const int *__restrict__ cell, Bricks B, SearchTables st,
                                                         const int *__restrict__ rowsrc,
@@ -637,7 +647,7 @@ __global__ __launch_bounds__(kWave *W) void k_emit_brick(const int *__restrict__
   __shared__ int tb[kMaxRanges + 1];
   constexpr int T = kWave * W;
   const int b = xcd_block(blockIdx.x, gridDim.x);
-  if (st.nrow[b] == 0) return;  // a brick of ghost-layer cells only
+  if (st.nrow[b] == 0) return;  // a brick without rows
   const int R = load_tb<T>(st, b, tb);
   __syncthreads();
   const unsigned total = (unsigned)tb[R];
@@ -651,58 +661,66 @@ __global__ __launch_bounds__(kWave *W) void k_emit_brick(const int *__restrict__
     const int s = vs0 + sl;
     const int len = slen[s];  // a multiple of kGroup
     const int v = sl * kWave + lane;
-    uint4 *gn = reinterpret_cast<uint4 *>(nbr + sbase[s]);  // sbase: a multiple of 512
-    uint2 *gc = reinterpret_cast<uint2 *>(ncode + sbase[s]);
-    GroupPack pk;
-    pk.clear();
-    int cnt = 0;
+    MaskCursor c;
+    RowCell rc = {0, 0, 0, 0};
+    c.ncand = 0;
+    c.nw = 0;
+    c.mk = nullptr;
     if (v < nrow) {
       const int i = row0 + v;
-      const RowCell rc = row_cell(i, cell[i], X, Rp, tb, ts);
-      // masks are indexed by the row order of the search
-      const uint32_t *mk = mask + (long long)rowsrc[i] * mcap;
-      int nbit = 0;
-      for (int dz = -1; dz <= 1; dz++) {
-        for (int dy = -1; dy <= 1; dy++) {
-          const int r0 = ((rc.lz + dz) * X.n[1] + (rc.ly + dy)) * X.n[0] + rc.lx - 1;
-          const int u0 = tb[kRangesPerCell * r0];
-          const int t1 = tb[kRangesPerCell * (r0 + 1)], t2 = tb[kRangesPerCell * (r0 + 2)];
-          const int n = tb[kRangesPerCell * (r0 + 3)] - u0;
-          // bits [nbit, nbit + n) of the mask belong to this run
-          for (int k = 0; k < n;) {
-            const int bit = nbit + k;
-            uint32_t w = mk[bit >> 5] >> (bit & 31);
-            const int take = min(32 - (bit & 31), n - k);
-            if (take < 32) w &= (1u << take) - 1u;
-            while (w) {
-              const int p = __ffs(w) - 1;
-              w &= w - 1u;
-              const int u = u0 + k + p;
-              const int dx = u < t1 ? -1 : (u < t2 ? 0 : 1);
-              if (cnt < len) {
-                pk.put(cnt & (kGroup - 1), (unsigned)u, offset_code(dx, dy, dz));
-                if ((cnt & (kGroup - 1)) == kGroup - 1) pk.store(gn, gc, (cnt / kGroup) * kWave + lane);
-              }
-              cnt++;
-            }
-            k += take;
+      rc = row_cell(i, cell[i], X, Rp, tb, ts);
+      // masks are indexed by the row order of the search; mcap is even
+      c.mk = reinterpret_cast<const unsigned long long *>(mask + (long long)rowsrc[i] * mcap);
+      int n = 0;
+      for (int q = 0; q < 9; q++) {
+        const int r0 = ((rc.lz + q / 3 - 1) * X.n[1] + (rc.ly + q % 3 - 1)) * X.n[0] + rc.lx - 1;
+        n += tb[kRangesPerCell * (r0 + 3)] - tb[kRangesPerCell * r0];
+      }
+      c.ncand = n;
+      c.nw = (n + 63) / 64;
+    }
+    c.wi = 0;
+    c.w = mask_word(c, 0);
+    c.wn = mask_word(c, 1);
+    c.q = 0;
+    c.ke = 0;
+    if (c.nw > 0) cursor_run(c, rc, X, tb);
+    uint4 *gn = reinterpret_cast<uint4 *>(nbr + sbase[s]);  // sbase: a multiple of 512
+    uint2 *gc = reinterpret_cast<uint2 *>(ncode + sbase[s]);
+    for (int g = 0; g < len / kGroup; g++) {
+      unsigned e[kGroup], cd[kGroup];
+#pragma unroll
+      for (int j = 0; j < kGroup; j++) {
+        while (c.w == 0ull && c.wi + 1 < c.nw) {
+          c.wi++;
+          c.w = c.wn;
+          c.wn = mask_word(c, c.wi + 1);
+        }
+        e[j] = total;
+        cd[j] = kZeroOffset;
+        if (c.w != 0ull) {
+          const int p = __ffsll((long long)c.w) - 1;
+          c.w &= c.w - 1ull;
+          const int kc = 64 * c.wi + p;
+          while (kc >= c.ke) {
+            c.q++;
+            cursor_run(c, rc, X, tb);
           }
-          nbit += n;
+          const int u = c.u0 + (kc - c.kb);
+          const int dx = u < c.t1 ? -1 : (u < c.t2 ? 0 : 1);
+          e[j] = (unsigned)u;
+          cd[j] = offset_code(dx, c.q % 3 - 1, c.q / 3 - 1);
         }
       }
+      gn[g * kWave + lane] = make_uint4(e[0] | e[1] << 16, e[2] | e[3] << 16, e[4] | e[5] << 16, e[6] | e[7] << 16);
+      gc[g * kWave + lane] = make_uint2(cd[0] | cd[1] << 8 | cd[2] << 16 | cd[3] << 24,
+                                        cd[4] | cd[5] << 8 | cd[6] << 16 | cd[7] << 24);
     }
-    if (cnt > len) atomicOr(err, kErrCapacity);
-    int k = min(cnt, len);
-    if (k & (kGroup - 1)) {  // pad the open group
-      for (int j = k & (kGroup - 1); j < kGroup; j++) pk.put(j, total, kZeroOffset);
-      pk.store(gn, gc, (k / kGroup) * kWave + lane);
-      k = (k / kGroup + 1) * kGroup;
-    }
-    const uint4 pad = make_uint4(total | total << 16, total | total << 16, total | total << 16, total | total << 16);
-    const unsigned zc = kZeroOffset * 0x01010101u;
-    for (; k < len; k += kGroup) {
-      gn[(k / kGroup) * kWave + lane] = pad;
-      gc[(k / kGroup) * kWave + lane] = make_uint2(zc, zc);
+    // entries left over: the slice is shorter than the row
+    if (c.w != 0ull || c.wn != 0ull || c.wi + 2 < c.nw) {
+      bool more = c.w != 0ull || c.wn != 0ull;
+      for (int k = c.wi + 2; k < c.nw && !more; k++) more = mask_word(c, k) != 0ull;
+      if (more) atomicOr(err, kErrCapacity);
     }
   }
 }

@@ -415,12 +415,13 @@ __global__ __launch_bounds__(kBlock) void k_replica_kinetic(const double4 *__res
 // PairList::CheckByDisplacement (pairlist.cc:68-121) of the replicas listed
 // in `which`, over each replica's owned entries [p0, p0 + pn) and its ghosts
 // [pn_all + g0, pn_all + g0 + ng) (grng[r] = {g0, ng}), in three launches:
-//  1. k_check_dmax, kCheckBlocks workgroups per listed replica: the largest
-//     displacement since the build, into dmax[k] as the bits of a double
-//     (non-negative doubles order like their bits);
+//  1. k_check_dmax, kCheckBlocks workgroups per listed replica: each entry's
+//     squared displacement since the build into disp, and the largest into
+//     dmax[k] as the bits of a double (non-negative doubles order like their
+//     bits);
 //  2. k_check_cand, as many workgroups: the entries displaced by more than
-//     margin - max displacement, up to Nl = min(cap, n) of them stored, all of
-//     them counted in cnt[k];
+//     margin - max displacement (read back from disp), up to Nl = min(cap, n)
+//     of them stored, all of them counted in cnt[k];
 //  3. k_check_pairs, one workgroup per listed replica: expired once Nl
 //     candidates were found, else when a candidate pair that was beyond SL at
 //     the build is now inside rc.
@@ -438,6 +439,7 @@ __global__ __launch_bounds__(kBlock) void k_check_dmax(const double4 *__restrict
                                                        const double4 *__restrict__ old,
                                                        const int *__restrict__ which, const Replica *__restrict__ reps,
                                                        const int2 *__restrict__ grng, int pn_all,
+                                                       double *__restrict__ disp,
                                                        unsigned long long *__restrict__ dmax) {
   const int kr = blockIdx.x / kCheckBlocks, part = blockIdx.x % kCheckBlocks;
   const int r = which[kr];
@@ -448,14 +450,15 @@ __global__ __launch_bounds__(kBlock) void k_check_dmax(const double4 *__restrict
     const int i = check_entry(k, R, no, g0, pn_all);
     const double4 a = old[i], b = pos[i];
     const double dx = a.x - b.x, dy = a.y - b.y, dz = a.z - b.z;
-    m = fmax(m, dx * dx + dy * dy + dz * dz);
+    const double d2 = dx * dx + dy * dy + dz * dz;
+    disp[i] = d2;
+    m = fmax(m, d2);
   }
   m = block_max_all(m);
   if (threadIdx.x == 0) atomicMax(&dmax[kr], (unsigned long long)__double_as_longlong(m));
 }
 
-__global__ __launch_bounds__(kBlock) void k_check_cand(const double4 *__restrict__ pos,
-                                                       const double4 *__restrict__ old,
+__global__ __launch_bounds__(kBlock) void k_check_cand(const double *__restrict__ disp,
                                                        const int *__restrict__ which, const Replica *__restrict__ reps,
                                                        const int2 *__restrict__ grng, int pn_all, double margin,
                                                        int cap, const unsigned long long *__restrict__ dmax,
@@ -469,9 +472,7 @@ __global__ __launch_bounds__(kBlock) void k_check_cand(const double4 *__restrict
   const double c_d2 = c_distance * c_distance;
   for (int k = part * kBlock + threadIdx.x; k < n; k += kCheckBlocks * kBlock) {
     const int i = check_entry(k, R, no, g0, pn_all);
-    const double4 a = old[i], b = pos[i];
-    const double dx = a.x - b.x, dy = a.y - b.y, dz = a.z - b.z;
-    if (dx * dx + dy * dy + dz * dz > c_d2) {
+    if (disp[i] > c_d2) {
       const int c = atomicAdd(&cnt[kr], 1);
       if (c < Nl) cand[(long long)kr * cap + c] = i;
     }
