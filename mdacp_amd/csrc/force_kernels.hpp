@@ -80,6 +80,19 @@ __device__ __forceinline__ void unpack_codes(const uint2 g, int c[kGroup]) {
   }
 }
 
+// Keeps a prefetched list group at its load: an empty asm that takes the
+// group as an operand together with values computed from the current group
+// (the row's running sums), so the compiler cannot move the load down to the
+// iteration that uses it, where its latency would stall the wavefront.
+template <class A>
+__device__ __forceinline__ void hold_prefetch(uint4 &g, const A &a, const A &b, const A &c) {
+  asm volatile("" : "+v"(g.x), "+v"(g.y), "+v"(g.z), "+v"(g.w) : "v"(a), "v"(b), "v"(c));
+}
+template <class A>
+__device__ __forceinline__ void hold_prefetch(uint2 &g, const A &a, const A &b, const A &c) {
+  asm volatile("" : "+v"(g.x), "+v"(g.y) : "v"(a), "v"(b), "v"(c));
+}
+
 // The FP64 kick of one row (one lane of a 64-row slice) over its list
 // entries, one group of kGroup per batch.  `rg` points at the lane's first
 // group (group g at rg[g * kWave]); the indices of the next P groups are in
@@ -99,6 +112,10 @@ __device__ __forceinline__ double3 row_kick_fp64(const uint4 *__restrict__ rg, i
   for (int k = 0; k < len; k += kGroup) {
     int e[kGroup];
     unpack_group(ahead[0], e);
+#pragma unroll
+    for (int j = 0; j + 1 < P; j++) ahead[j] = ahead[j + 1];
+    ahead[P - 1] = rg[(k / kGroup + P) * kWave];
+    __builtin_amdgcn_sched_barrier(0);  // the prefetch is issued before this group's work
     double qx[kGroup], qy[kGroup], qz[kGroup];
 #pragma unroll
     for (int u = 0; u < kGroup; u++) {
@@ -117,9 +134,6 @@ __device__ __forceinline__ double3 row_kick_fp64(const uint4 *__restrict__ rg, i
       }
     }
 #pragma unroll
-    for (int j = 0; j + 1 < P; j++) ahead[j] = ahead[j + 1];
-    ahead[P - 1] = rg[(k / kGroup + P) * kWave];
-#pragma unroll
     for (int u = 0; u < kGroup; u++) {
       const double dx = qx[u] - qi.x, dy = qy[u] - qi.y, dz = qz[u] - qi.z;
       const double r2 = fma(dx, dx, fma(dy, dy, dz * dz));
@@ -131,6 +145,7 @@ __device__ __forceinline__ double3 row_kick_fp64(const uint4 *__restrict__ rg, i
       ay = fma(df, dy, ay);
       az = fma(df, dz, az);
     }
+    hold_prefetch(ahead[P - 1], ax, ay, az);
   }
   return make_double3(ax, ay, az);
 }
@@ -163,12 +178,6 @@ __device__ __forceinline__ double3 row_kick_mixed(const uint4 *__restrict__ rg, 
     int e[kGroup], cc[kGroup];
     unpack_group(ahead[0], e);
     unpack_codes(cahead[0], cc);
-    float4 rj[kGroup];
-#pragma unroll
-    for (int u = 0; u < kGroup; u++) {
-      if (kStaged) rj[u] = lds[e[u]];
-      else rj[u] = e[u] < total ? rel[gl[e[u]]] : make_float4(1.0e30f, 1.0e30f, 1.0e30f, 0.0f);
-    }
 #pragma unroll
     for (int j = 0; j + 1 < P; j++) {
       ahead[j] = ahead[j + 1];
@@ -176,6 +185,13 @@ __device__ __forceinline__ double3 row_kick_mixed(const uint4 *__restrict__ rg, 
     }
     ahead[P - 1] = rg[(k / kGroup + P) * kWave];
     cahead[P - 1] = cg[(k / kGroup + P) * kWave];
+    __builtin_amdgcn_sched_barrier(0);  // the prefetch is issued before this group's work
+    float4 rj[kGroup];
+#pragma unroll
+    for (int u = 0; u < kGroup; u++) {
+      if (kStaged) rj[u] = lds[e[u]];
+      else rj[u] = e[u] < total ? rel[gl[e[u]]] : make_float4(1.0e30f, 1.0e30f, 1.0e30f, 0.0f);
+    }
     if (kSum == kSumGroup64) ax = ay = az = 0.f;
 #pragma unroll
     for (int u = 0; u < kGroup; u++) {
@@ -202,6 +218,8 @@ __device__ __forceinline__ double3 row_kick_mixed(const uint4 *__restrict__ rg, 
       sy += (double)ay;
       sz += (double)az;
     }
+    hold_prefetch(ahead[P - 1], ax, ay, az);
+    hold_prefetch(cahead[P - 1], ax, ay, az);
   }
   if (kSum == kSumGroup64) return make_double3(sx, sy, sz);
   return make_double3((double)ax - (double)cx, (double)ay - (double)cy, (double)az - (double)cz);
