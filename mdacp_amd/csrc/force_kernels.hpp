@@ -57,13 +57,13 @@ struct StepClock {
 
 __device__ __forceinline__ uint64_t step_of(const StepClock &sc) { return sc.clock ? sc.clock[0] : sc.step; }
 
-// 1/x to double precision: hardware reciprocal estimate + two Newton steps.
+// 1/x for the pair force: the hardware reciprocal (relative error below
+// 5e-8 on gfx950) refined by one Newton step, which squares the error: below
+// 2.3e-15 relative, about 10 ulp (measured over 1e-3 <= x <= 1e3).  A second
+// step would give the correctly rounded quotient at two more FMAs per pair.
 __device__ __forceinline__ double rcp64(double x) {
-  double r = __builtin_amdgcn_rcp(x);
-  double e = fma(-x, r, 1.0);
-  r = fma(r, e, r);
-  e = fma(-x, r, 1.0);
-  return fma(r, e, r);
+  const double r = __builtin_amdgcn_rcp(x);
+  return fma(r, fma(-x, r, 1.0), r);
 }
 
 // The 8 staged indices of one group of a lane's list slots (one 16-byte load).
