@@ -218,6 +218,12 @@ int mdgpu_get_timing(const mdgpu_domain *d, int *force_launches, double *force_m
 #define MDGPU_OPT_SEARCH_CELLS 12  /* neighbour search with a wavefront per cell, lanes over 64
                                       candidates at a time (1, the default), or a thread per
                                       row (0); the same masks either way */
+#define MDGPU_OPT_RUN_AHEAD 13  /* mdgpu_calculate queues up to this many fused steps without
+                                   waiting for the host (default 64; 0 = every step waits): a
+                                   device-side gate does each step's validity budget and stops
+                                   the run at the step whose displacement check or construction
+                                   needs the host, which then takes that step; the same
+                                   trajectory either way */
 int mdgpu_set_option(mdgpu_domain *d, int option, int value);
 int mdgpu_get_option(const mdgpu_domain *d, int option, int *value);
 

@@ -102,6 +102,19 @@ struct Replica {
 
 __device__ __forceinline__ int replica_bricks(const Replica &R) { return R.nb[0] * R.nb[1] * R.nb[2]; }
 
+// A run of fused steps queued without waiting for the host.  The gate kernel
+// at the head of each step halts the run when that step's validity test needs
+// the host (a budget spent: the displacement check or a construction is due;
+// or a particle too fast); the step's other kernels then return at once, and
+// so does every step queued after it, which leaves the state of the last
+// completed step for the host to take the halted step from.
+struct RunGate {
+  int halted;  // 0 while the run goes on
+  int done;    // steps completed in this run
+};
+
+__device__ __forceinline__ bool run_halted(const RunGate *g) { return g != nullptr && g->halted != 0; }
+
 // A ghost's code word: image shift code (sx+1)+3(sy+1)+9(sz+1) in the low
 // five bits, the replica above them.
 constexpr int kShiftBits = 5;

@@ -53,6 +53,7 @@ struct StepClock {
   double dt;
   uint64_t step;
   const unsigned long long *clock;
+  const RunGate *gate;  // a step of a run: nothing to do once the run has halted
 };
 
 __device__ __forceinline__ uint64_t step_of(const StepClock &sc) { return sc.clock ? sc.clock[0] : sc.step; }
@@ -285,11 +286,12 @@ __global__ __launch_bounds__(kWave *W) void k_force_brick_fp64(const double4 *__
                                                               double *__restrict__ vpart, NextDrift nd) {
   extern __shared__ __align__(16) double lds64[];
   constexpr int T = kWave * W;
+  if (run_halted(sc.gate)) return;
   const int b = xcd_block(blockIdx.x, gridDim.x);
   const Replica &Rp = bl.reps[bl.brep[b]];
   const LangevinConsts lc = langevin_consts(Rp, sc.dt, step_of(sc));
   const int nrow = bl.nrow[b], row0 = bl.row0[b], vs0 = bl.vslice0[b];
-  if (nrow == 0) {  // a brick of ghost-layer cells only: nothing to stage
+  if (nrow == 0) {  // a brick without rows: nothing to stage
     if (vpart) block_max_to_partial(0.0, vpart, b);
     return;
   }
@@ -338,6 +340,7 @@ __global__ __launch_bounds__(kWave *W) void k_force_brick_mixed(
   extern __shared__ float4 lds32[];
   __shared__ float4 offs[64];
   constexpr int T = kWave * W;
+  if (run_halted(sc.gate)) return;
   const int b = xcd_block(blockIdx.x, gridDim.x);
   const Replica &Rp = bl.reps[bl.brep[b]];
   const LangevinConsts lc = langevin_consts(Rp, sc.dt, step_of(sc));

@@ -95,6 +95,13 @@ struct mdgpu_domain {
   bool use_graph = true;
   long epoch = 0;  // bumped whenever a captured step may no longer match
   StepGraph graphs[2];
+  // runs of steps queued without waiting for the host (k_gate at the head of
+  // each step): at most run_ahead steps per run (0: every step waits)
+  int run_ahead = 64;
+  StepGraph gated[2];
+  RunGate *gate = nullptr, *h_gate = nullptr;  // device state and its pinned mirror
+  double *gbudget = nullptr, *h_gbudget = nullptr;  // per replica: validity budget
+  int *glife = nullptr, *h_glife = nullptr;         // per replica: list lifetime
 
   // thermostat
   int thermostat = MDGPU_NVE;
@@ -905,15 +912,19 @@ int check_by_displacement(mdgpu_domain *d, int *expired) {
   hipLaunchKernelGGL(k_displacement, dim3(nblk(tn)), dim3(kBlock), 0, st, d->pos, d->pos_old, tn, d->disp,
                      d->maxpart);
   hipLaunchKernelGGL(k_reduce_max, dim3(1), dim3(kBlock), 0, st, d->maxpart, (int)nblk(tn), d->u64 + 1);
-  double dr2_max;
-  CHK(read_max(d, 1, &dr2_max));
+  // the three passes run back to back, each taking the last one's result from
+  // device memory, and the host reads the outcome once
   const int Nl = (tn > d->check_list_length) ? d->check_list_length : tn;
-  const double c_distance = d->margin - std::sqrt(dr2_max);
-  const double c_d2 = c_distance * c_distance;
   HIPCHK(d, hipMemsetAsync(d->i32 + 1, 0, sizeof(int) * 2, st));
   if (tn > 0)
-    hipLaunchKernelGGL(k_candidates, dim3(nblk(tn)), dim3(kBlock), 0, st, d->disp, tn, c_d2, d->i32 + 1,
-                       d->cand, d->check_list_length);
+    hipLaunchKernelGGL(k_candidates, dim3(nblk(tn)), dim3(kBlock), 0, st, d->disp, tn, d->u64 + 1, d->margin,
+                       d->i32 + 1, d->cand, d->check_list_length);
+  if (Nl > 1) {
+    const long long total = (long long)Nl * Nl;
+    hipLaunchKernelGGL(k_candidate_pairs, dim3(nblk(total)), dim3(kBlock), 0, st, d->pos, d->pos_old, d->cand,
+                       d->i32 + 1, Nl, d->SL * d->SL, d->CL2, d->i32 + 2);
+  }
+  CHK(check_launch(d, "displacement check"));
   HIPCHK(d, hipMemcpyAsync(d->h_i32, d->i32, sizeof(int) * 4, hipMemcpyDeviceToHost, st));
   CHK(sync(d));
   const int count = d->h_i32[1];
@@ -921,15 +932,7 @@ int check_by_displacement(mdgpu_domain *d, int *expired) {
     *expired = 1;
     return MDGPU_OK;
   }
-  const int l = count;
-  if (l > 1) {
-    const long long total = (long long)l * l;
-    hipLaunchKernelGGL(k_candidate_pairs, dim3(nblk(total)), dim3(kBlock), 0, st, d->pos, d->pos_old, d->cand, l,
-                       d->SL * d->SL, d->CL2, d->i32 + 2);
-    HIPCHK(d, hipMemcpyAsync(d->h_i32, d->i32, sizeof(int) * 4, hipMemcpyDeviceToHost, st));
-    CHK(sync(d));
-    *expired = d->h_i32[2] != 0;
-  }
+  *expired = count > 1 && d->h_i32[2] != 0;
   return MDGPU_OK;
 }
 
@@ -1118,12 +1121,13 @@ void launch_force_w(mdgpu_domain *d, const double4 *src, const float4 *rel, doub
 // host mirror.  `clock` set: the Langevin step is read from it and advanced
 // (a launch that is captured and replayed).
 int launch_force(mdgpu_domain *d, const double4 *src, const float4 *rel, double4 *pos_out, int epilogue,
-                 bool want_vmax, const NextDrift &nd, unsigned long long *clock = nullptr, bool inner = false) {
+                 bool want_vmax, const NextDrift &nd, unsigned long long *clock = nullptr, bool inner = false,
+                 RunGate *gate = nullptr) {
   if (d->pn == 0) return MDGPU_OK;
   hipStream_t st = d->stream;
   const ForceConsts fc = fconsts(d);
   const ForceConstsF f32 = fconsts32(d);
-  const StepClock sc{d->dt, d->step, clock};
+  const StepClock sc{d->dt, d->step, clock, gate};
   double *vb = want_vmax ? d->bpart : nullptr;
   const int ev0 = d->force_launches * 2;
   const bool ev = d->timing && clock == nullptr && ev0 + 1 < (int)d->ev.size();
@@ -1139,8 +1143,10 @@ int launch_force(mdgpu_domain *d, const double4 *src, const float4 *rel, double4
     d->force_launches++;
   }
   if (want_vmax) {
-    hipLaunchKernelGGL(k_replica_max, dim3(d->nrep), dim3(kBlock), 0, st, d->bpart, d->reps, d->rep_v2, clock);
-    HIPCHK(d, hipMemcpyAsync(d->h_rep_v2, d->rep_v2, d->nrep * sizeof(double), hipMemcpyDeviceToHost, st));
+    hipLaunchKernelGGL(k_replica_max, dim3(d->nrep), dim3(kBlock), 0, st, d->bpart, d->reps, d->rep_v2, clock,
+                       gate);
+    // a run's steps keep the maxima on the device for the next step's gate
+    if (!gate) HIPCHK(d, hipMemcpyAsync(d->h_rep_v2, d->rep_v2, d->nrep * sizeof(double), hipMemcpyDeviceToHost, st));
   }
   return check_launch(d, "force");
 }
@@ -1238,22 +1244,28 @@ int launch_prune(mdgpu_domain *d) {
   return check_launch(d, "prune");
 }
 
-int enqueue_fused(mdgpu_domain *d, unsigned long long *clock, bool prune = false) {
+// The fused step's launches.  gated: a step of a run, headed by the validity
+// gate (k_gate), whose launches do nothing once the run has halted.
+int enqueue_fused(mdgpu_domain *d, unsigned long long *clock, bool prune = false, bool gated = false) {
   hipStream_t st = d->stream;
   const bool mixed = d->precision == MDGPU_MIXED;
+  RunGate *gate = gated ? d->gate : nullptr;
+  if (gated)
+    hipLaunchKernelGGL(k_gate, dim3(1), dim3(kBlock), 0, st, d->gate, d->gbudget, d->glife, d->rep_v2, d->nrep, d->dt,
+                       d->margin);
   if (d->ng > 0) {
     if (mixed)
       hipLaunchKernelGGL((k_refresh_from_drifted<true>), dim3(nblk(d->ng)), dim3(kBlock), 0, st, d->pos, d->pos_tmp,
-                         d->gsrc, d->gcode, d->pn, d->ng, d->reps, d->cell, d->org, d->rel_tmp);
+                         d->gsrc, d->gcode, d->pn, d->ng, d->reps, d->cell, d->org, d->rel_tmp, gate);
     else
       hipLaunchKernelGGL((k_refresh_from_drifted<false>), dim3(nblk(d->ng)), dim3(kBlock), 0, st, d->pos,
-                         d->pos_tmp, d->gsrc, d->gcode, d->pn, d->ng, d->reps, d->cell, d->org, d->rel_tmp);
+                         d->pos_tmp, d->gsrc, d->gcode, d->pn, d->ng, d->reps, d->cell, d->org, d->rel_tmp, gate);
   }
   CHK(check_launch(d, "refresh"));
   if (prune) CHK(launch_prune(d));
   const int epi = d->thermostat == MDGPU_LANGEVIN ? kKickLangevin : kKickDrift;
   const NextDrift nd{d->pos_next, mixed ? d->rel_next : nullptr, d->cell, d->org};
-  return launch_force(d, d->pos_tmp, d->rel_tmp, d->pos, epi, true, nd, clock, d->prune_delta > 0.0);
+  return launch_force(d, d->pos_tmp, d->rel_tmp, d->pos, epi, true, nd, clock, d->prune_delta > 0.0, gate);
 }
 
 void drop_graph(StepGraph &g) {
@@ -1267,21 +1279,22 @@ void drop_graph(StepGraph &g) {
 // buffers: one per parity of the swapped position buffers, both recaptured
 // after anything that changes what the launches read (a construction,
 // reallocation, option or thermostat change).
-int launch_fused_graph(mdgpu_domain *d) {
+int launch_fused_graph(mdgpu_domain *d, bool gated = false) {
+  StepGraph *gs = gated ? d->gated : d->graphs;
   StepGraph *g = nullptr;
-  for (StepGraph &c : d->graphs)
-    if (c.exec && c.epoch == d->epoch && c.key == (const void *)d->pos_tmp) g = &c;
+  for (int k = 0; k < 2; k++)
+    if (gs[k].exec && gs[k].epoch == d->epoch && gs[k].key == (const void *)d->pos_tmp) g = &gs[k];
   if (!g) {
-    g = &d->graphs[0];
-    for (StepGraph &c : d->graphs)
-      if (!c.exec || c.epoch != d->epoch) {
-        g = &c;
+    g = &gs[0];
+    for (int k = 0; k < 2; k++)
+      if (!gs[k].exec || gs[k].epoch != d->epoch) {
+        g = &gs[k];
         break;
       }
     drop_graph(*g);
     hipGraph_t graph = nullptr;
     HIPCHK(d, hipStreamBeginCapture(d->stream, hipStreamCaptureModeRelaxed));
-    const int r = enqueue_fused(d, d->clock);
+    const int r = enqueue_fused(d, d->clock, false, gated);
     const hipError_t e = hipStreamEndCapture(d->stream, &graph);
     if (r != MDGPU_OK) {
       if (e == hipSuccess && graph) (void)hipGraphDestroy(graph);
@@ -1396,6 +1409,88 @@ int calculate_step(mdgpu_domain *d) {
   return MDGPU_OK;
 }
 
+// Whether the next steps may be queued without waiting for the host: fused
+// NVE or Langevin steps replayed from graphs, no inner list, and the last step
+// a fused one (its maxima of |p|^2 and the next first half drift are on the
+// device).
+bool can_run_ahead(const mdgpu_domain *d) {
+  return d->run_ahead > 0 && d->use_graph && !d->timing && d->thermostat != MDGPU_NOSEHOOVER &&
+         !(d->prune_delta > 0.0) && d->initialized && d->mid_valid && d->vmax_valid && d->pn > 0;
+}
+
+double &budget_of(mdgpu_domain *d, int r) { return d->nrep == 1 ? d->buffer_length : d->rst[r].buffer_length; }
+int &lifetime_of(mdgpu_domain *d, int r) { return d->nrep == 1 ? d->lifetime : d->rst[r].lifetime; }
+
+// Steps every validity budget lasts at the last maxima of |p| (0: the next
+// step needs the host).  The maxima change from step to step, so a run may
+// still halt earlier, or end before the budgets do.
+int run_length(mdgpu_domain *d, int limit, int *k) {
+  *k = 0;
+  for (int r = 0; r < d->nrep; r++)
+    if (!(budget_of(d, r) > 0.0)) return MDGPU_OK;  // a displacement check is due: no wait for the maxima
+  CHK(sync(d));  // the last step's maxima
+  double steps = (double)limit;
+  for (int r = 0; r < d->nrep; r++) {
+    const double b = budget_of(d, r);
+    const double v = std::sqrt(d->h_rep_v2[r]);
+    if (v > 0.0) steps = std::min(steps, b / (v * 2.0 * d->dt));
+  }
+  if (steps >= 1.0) *k = (int)steps;
+  return MDGPU_OK;
+}
+
+// A run of k fused steps queued at once, each headed by k_gate; *done
+// receives the steps completed, fewer than k when a step's validity test
+// needs the host.  The budgets and lifetimes go to the device before the run
+// and come back after it, as do the last maxima of |p|^2.
+int run_gated(mdgpu_domain *d, int k, int *done) {
+  hipStream_t st = d->stream;
+  const int R = d->nrep;
+  const bool mixed = d->precision == MDGPU_MIXED;
+  for (int r = 0; r < R; r++) {
+    d->h_gbudget[r] = budget_of(d, r);
+    d->h_glife[r] = lifetime_of(d, r);
+  }
+  d->h_gate->halted = 0;
+  d->h_gate->done = 0;
+  HIPCHK(d, hipMemcpyAsync(d->gbudget, d->h_gbudget, R * sizeof(double), hipMemcpyHostToDevice, st));
+  HIPCHK(d, hipMemcpyAsync(d->glife, d->h_glife, R * sizeof(int), hipMemcpyHostToDevice, st));
+  HIPCHK(d, hipMemcpyAsync(d->gate, d->h_gate, sizeof(RunGate), hipMemcpyHostToDevice, st));
+  if (d->clock_dirty) {
+    hipLaunchKernelGGL(k_set_u64, dim3(1), dim3(1), 0, st, d->clock, (unsigned long long)d->step);
+    d->clock_dirty = false;
+  }
+  for (int j = 0; j < k; j++) {
+    CHK(launch_fused_graph(d, true));
+    std::swap(d->pos_tmp, d->pos_next);
+    if (mixed) std::swap(d->rel_tmp, d->rel_next);
+  }
+  HIPCHK(d, hipMemcpyAsync(d->h_gate, d->gate, sizeof(RunGate), hipMemcpyDeviceToHost, st));
+  HIPCHK(d, hipMemcpyAsync(d->h_gbudget, d->gbudget, R * sizeof(double), hipMemcpyDeviceToHost, st));
+  HIPCHK(d, hipMemcpyAsync(d->h_glife, d->glife, R * sizeof(int), hipMemcpyDeviceToHost, st));
+  HIPCHK(d, hipMemcpyAsync(d->h_rep_v2, d->rep_v2, R * sizeof(double), hipMemcpyDeviceToHost, st));
+  CHK(sync(d));
+  const int n = d->h_gate->done;
+  if (n < 0 || n > k) {
+    d->err = "step run: inconsistent count of completed steps";
+    return MDGPU_ERR_HIP;
+  }
+  if ((k - n) % 2) {  // the halted steps' buffer swaps did not happen on the device
+    std::swap(d->pos_tmp, d->pos_next);
+    if (mixed) std::swap(d->rel_tmp, d->rel_next);
+  }
+  for (int r = 0; r < R; r++) {
+    budget_of(d, r) = d->h_gbudget[r];
+    lifetime_of(d, r) = d->h_glife[r];
+  }
+  for (int j = 0; j < n; j++) {
+    d->sim_time += d->dt;
+    d->step++;
+  }
+  *done = n;
+  return MDGPU_OK;
+}
+
 template <int W>
 void launch_observe(mdgpu_domain *d, const BrickList &bl, size_t shm) {
   allow_lds(k_observe_brick<W>, shm);
@@ -1474,6 +1569,7 @@ int create_domain(const mdgpu_params *prm, int nrep, const mdgpu_replica *rl, co
   if ((r = dalloc(d, &d->reps, nrep)) || (r = dalloc(d, &d->rep_v2, nrep)) || (r = dalloc(d, &d->rep_out, 3 * nrep)) ||
       (r = dalloc(d, &d->which, nrep)) || (r = dalloc(d, &d->rep_exp, nrep)) || (r = dalloc(d, &d->grng, nrep)) ||
       (r = dalloc(d, &d->clock, 1)) || (r = dalloc(d, &d->chk_dmax, nrep)) || (r = dalloc(d, &d->chk_cnt, nrep)) ||
+      (r = dalloc(d, &d->gate, 1)) || (r = dalloc(d, &d->gbudget, nrep)) || (r = dalloc(d, &d->glife, nrep)) ||
       (r = dalloc(d, &d->chk_cand, (size_t)nrep * d->check_list_length))) {
     mdgpu_destroy(d);
     return r;
@@ -1481,7 +1577,10 @@ int create_domain(const mdgpu_params *prm, int nrep, const mdgpu_replica *rl, co
   if (hipHostMalloc((void **)&d->h_rep_v2, nrep * sizeof(double)) != hipSuccess ||
       hipHostMalloc((void **)&d->h_rep_out, 3 * nrep * sizeof(double)) != hipSuccess ||
       hipHostMalloc((void **)&d->h_which, nrep * sizeof(int)) != hipSuccess ||
-      hipHostMalloc((void **)&d->h_rep_exp, nrep * sizeof(int)) != hipSuccess) {
+      hipHostMalloc((void **)&d->h_rep_exp, nrep * sizeof(int)) != hipSuccess ||
+      hipHostMalloc((void **)&d->h_gate, sizeof(RunGate)) != hipSuccess ||
+      hipHostMalloc((void **)&d->h_gbudget, nrep * sizeof(double)) != hipSuccess ||
+      hipHostMalloc((void **)&d->h_glife, nrep * sizeof(int)) != hipSuccess) {
     mdgpu_destroy(d);
     return MDGPU_ERR_HIP;
   }
@@ -1591,11 +1690,13 @@ int mdgpu_destroy(mdgpu_domain *d) {
                   d->i32,     d->d3,          d->reps,        d->brep,      d->rep,      d->rep_alt,
                   d->grng,    d->rep_v2,      d->rep_out,     d->which,     d->rep_exp,  d->clock,
                   d->perm_in, d->nbr_in,      d->ncode_in,    d->slen_in,   d->chk_dmax, d->chk_cnt,
-                  d->chk_cand};
+                  d->chk_cand, d->gate,       d->gbudget,     d->glife};
   for (StepGraph &g : d->graphs) drop_graph(g);
+  for (StepGraph &g : d->gated) drop_graph(g);
   for (void *b : bufs)
     if (b) (void)hipFree(b);
-  for (void *h : {(void *)d->h_rep_v2, (void *)d->h_rep_out, (void *)d->h_which, (void *)d->h_rep_exp})
+  for (void *h : {(void *)d->h_rep_v2, (void *)d->h_rep_out, (void *)d->h_which, (void *)d->h_rep_exp,
+                  (void *)d->h_gate, (void *)d->h_gbudget, (void *)d->h_glife})
     if (h) (void)hipHostFree(h);
   if (d->h_u64) (void)hipHostFree(d->h_u64);
   if (d->h_i64) (void)hipHostFree(d->h_i64);
@@ -1869,7 +1970,27 @@ int mdgpu_calculate(mdgpu_domain *d, int nsteps, int *rebuilds) {
     HIPCHK(d, hipEventRecord(t0, d->stream));
   }
   int r = MDGPU_OK;
-  for (int s = 0; s < nsteps && r == MDGPU_OK; s++) r = calculate_step(d);
+  int s = 0;
+  while (s < nsteps && r == MDGPU_OK) {
+    // steps whose validity the budgets settle run ahead of the host; a step
+    // that needs the host (the displacement check, a construction, an error)
+    // is taken here, one at a time
+    if (can_run_ahead(d) && nsteps - s > 1) {
+      int k = 0;
+      r = run_length(d, std::min(nsteps - s, d->run_ahead), &k);
+      if (r != MDGPU_OK) break;
+      if (k > 0) {
+        int done = 0;
+        r = run_gated(d, k, &done);
+        s += done;
+        if (r != MDGPU_OK || done == k) continue;
+      }
+    }
+    if (s < nsteps) {
+      r = calculate_step(d);
+      s++;
+    }
+  }
   if (r == MDGPU_OK) r = sync(d);
   if (d->timing && t0) {
     HIPCHK(d, hipEventRecord(t1, d->stream));
@@ -2167,6 +2288,10 @@ int mdgpu_set_option(mdgpu_domain *d, int option, int value) {
     case MDGPU_OPT_SEARCH_CELLS:
       d->search_cells = value != 0;
       return MDGPU_OK;
+    case MDGPU_OPT_RUN_AHEAD:
+      if (value < 0) return MDGPU_ERR_ARGUMENT;
+      d->run_ahead = value;
+      return MDGPU_OK;
     case MDGPU_OPT_BRICK_KB:
       if (value < 8 || value > 160) return MDGPU_ERR_ARGUMENT;
       d->brick_kb = value;
@@ -2196,6 +2321,7 @@ int mdgpu_get_option(const mdgpu_domain *d, int option, int *value) {
     case MDGPU_OPT_ROW_SORT: *value = d->row_sort; return MDGPU_OK;
     case MDGPU_OPT_BRICK_KB: *value = brick_budget(d); return MDGPU_OK;
     case MDGPU_OPT_SEARCH_CELLS: *value = d->search_cells; return MDGPU_OK;
+    case MDGPU_OPT_RUN_AHEAD: *value = d->run_ahead; return MDGPU_OK;
   }
   return MDGPU_ERR_ARGUMENT;
 }

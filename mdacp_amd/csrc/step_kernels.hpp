@@ -135,7 +135,9 @@ template <bool kRel>
 __global__ void k_refresh_from_drifted(double4 *__restrict__ pos, double4 *__restrict__ tmp,
                                        const int *__restrict__ gsrc, const int *__restrict__ gcode, int pn,
                                        int ng, const Replica *__restrict__ reps, const int *__restrict__ cell,
-                                       const double *__restrict__ org, float4 *__restrict__ rel) {
+                                       const double *__restrict__ org, float4 *__restrict__ rel,
+                                       const RunGate *__restrict__ gate) {
+  if (run_halted(gate)) return;
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= ng) return;
   const int i = pn + k;
@@ -272,21 +274,28 @@ __global__ void k_displacement(const double4 *__restrict__ pos, const double4 *_
 
 // second pass: particles displaced by more than (margin - max displacement);
 // up to `cap` of them are collected (the reference stops at CheckListLength).
-__global__ void k_candidates(const double *__restrict__ disp, int tn, double c_d2, int *__restrict__ count,
-                             int *__restrict__ list, int cap) {
+// The threshold comes from the maximum of the first pass (the bits of a
+// double in dmax), c = margin - sqrt(dmax), compared squared.
+__global__ void k_candidates(const double *__restrict__ disp, int tn, const unsigned long long *__restrict__ dmax,
+                             double margin, int *__restrict__ count, int *__restrict__ list, int cap) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= tn) return;
+  const double c_distance = margin - sqrt(__longlong_as_double((long long)dmax[0]));
+  const double c_d2 = c_distance * c_distance;
   if (disp[i] > c_d2) {
     const int k = atomicAdd(count, 1);
     if (k < cap) list[k] = i;
   }
 }
 
-// third pass: a candidate pair that was beyond SL at the build and is now
-// inside rc expires the list.
+// third pass, for 1 < l < Nl candidates (l = count[0]; Nl or more expire the
+// list without it): a candidate pair that was beyond SL at the build and is
+// now inside rc expires the list.  Launched for Nl * Nl pairs.
 __global__ void k_candidate_pairs(const double4 *__restrict__ pos, const double4 *__restrict__ old,
-                                  const int *__restrict__ list, int l, double SL2, double CL2,
-                                  int *__restrict__ flag) {
+                                  const int *__restrict__ list, const int *__restrict__ count, int Nl, double SL2,
+                                  double CL2, int *__restrict__ flag) {
+  const int l = count[0];
+  if (l <= 1 || l >= Nl) return;
   const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   const long long total = (long long)l * l;
   if (t >= total) return;
@@ -357,15 +366,68 @@ __device__ __forceinline__ double block_sum0(double v) {
 // the step counter of the fused step; it advances by one here.
 __global__ __launch_bounds__(kBlock) void k_replica_max(const double *__restrict__ part,
                                                         const Replica *__restrict__ reps, double *__restrict__ out,
-                                                        unsigned long long *__restrict__ clock) {
+                                                        unsigned long long *__restrict__ clock,
+                                                        RunGate *__restrict__ gate) {
+  if (run_halted(gate)) return;
   const Replica &R = reps[blockIdx.x];
   const int b0 = R.brick0, nb = replica_bricks(R);
-  double m = 0.0;
-  for (int k = threadIdx.x; k < nb; k += kBlock) m = fmax(m, part[b0 + k]);
+  double m0 = 0.0, m1 = 0.0, m2 = 0.0, m3 = 0.0;  // four loads in flight per thread
+  int k = threadIdx.x;
+  for (; k + 3 * kBlock < nb; k += 4 * kBlock) {
+    const double a = part[b0 + k], b = part[b0 + k + kBlock];
+    const double c = part[b0 + k + 2 * kBlock], e = part[b0 + k + 3 * kBlock];
+    m0 = fmax(m0, a);
+    m1 = fmax(m1, b);
+    m2 = fmax(m2, c);
+    m3 = fmax(m3, e);
+  }
+  for (; k < nb; k += kBlock) m0 = fmax(m0, part[b0 + k]);
+  double m = fmax(fmax(m0, m1), fmax(m2, m3));
   m = block_max_all(m);
   if (threadIdx.x == 0) {
     out[blockIdx.x] = m;
     if (clock != nullptr && blockIdx.x == 0) clock[0] = clock[0] + 1ull;
+    if (gate != nullptr && blockIdx.x == 0) gate->done = gate->done + 1;
+  }
+}
+
+// PairList::IsPairListExpired with UpdatePairListValidity (pairlist.cc:26-65)
+// for the next step of a run, every replica at once: the lifetime counts up
+// and a budget still positive shrinks by 2 max|p| dt, from the replica's
+// max |p|^2 after the last step (v2).  When some replica's particles are too
+// fast or its budget ends below zero, nothing is committed and the run halts:
+// the host takes that step (and its displacement check) from the same values.
+// One workgroup.
+__device__ __forceinline__ bool gate_budget(double v2, double dt, double margin, double &b) {
+  if (b > 0.0) {
+    const double vmax = sqrt(v2);
+    if (vmax * 2.0 * dt > margin) return true;
+    b = b - vmax * 2.0 * dt;
+  }
+  return b < 0.0;
+}
+
+__global__ __launch_bounds__(kBlock) void k_gate(RunGate *__restrict__ gate, double *__restrict__ budget,
+                                                 int *__restrict__ lifetime, const double *__restrict__ v2,
+                                                 int nrep, double dt, double margin) {
+  __shared__ int stop;
+  if (gate->halted) return;
+  if (threadIdx.x == 0) stop = 0;
+  __syncthreads();
+  for (int r = threadIdx.x; r < nrep; r += kBlock) {
+    double b = budget[r];
+    if (gate_budget(v2[r], dt, margin, b)) stop = 1;
+  }
+  __syncthreads();
+  if (stop) {
+    if (threadIdx.x == 0) gate->halted = 1;
+    return;
+  }
+  for (int r = threadIdx.x; r < nrep; r += kBlock) {
+    double b = budget[r];
+    gate_budget(v2[r], dt, margin, b);
+    budget[r] = b;
+    lifetime[r] = lifetime[r] + 1;
   }
 }
 

@@ -31,6 +31,7 @@ EXPORTS = [
 ]
 OPT_UNROLL, OPT_KAHAN, OPT_WAVES, OPT_LDS_KB, OPT_BRICK, OPT_GRAPH = 1, 2, 3, 4, 5, 6
 OPT_SEARCH_WAVES, OPT_SEARCH_STAGE, OPT_PRUNE, OPT_ROW_SORT, OPT_BRICK_KB, OPT_SEARCH_CELLS = 7, 8, 9, 10, 11, 12
+OPT_RUN_AHEAD = 13
 
 
 class Params(ctypes.Structure):
@@ -223,7 +224,7 @@ class Domain:
 
     def copy_options(self, other):
         for o in (OPT_UNROLL, OPT_KAHAN, OPT_WAVES, OPT_LDS_KB, OPT_GRAPH, OPT_SEARCH_WAVES, OPT_SEARCH_STAGE,
-                  OPT_PRUNE, OPT_SEARCH_CELLS):
+                  OPT_PRUNE, OPT_SEARCH_CELLS, OPT_RUN_AHEAD):
             self.set_option(o, other.get_option(o))
 
     def zeta(self):

@@ -254,6 +254,44 @@ def test_piecewise_api_equals_fused_step():
     assert np.array_equal(qa, qb) and np.array_equal(pa, pb)
 
 
+@pytest.mark.parametrize("thermostat,precision", [("nve", mdgpu.FP64), ("langevin", mdgpu.FP64),
+                                                  ("nve", mdgpu.MIXED)])
+def test_runs_ahead_match_steps_one_at_a_time(thermostat, precision):
+    """Steps queued ahead of the host (the device-side validity gate, MDGPU_OPT_RUN_AHEAD) give
+    the same bits, time and constructions as steps that each wait for the host: the gate does
+    the budget arithmetic the host does and stops the run where the host must take over."""
+    say("case: run ahead", thermostat, precision)
+    L = 16.0
+    x = fcc(L, 0.5)
+    n = len(x)
+    p = initial_velocity(n, math.sqrt(3.0 * 1.5), 3)
+    ds = []
+    for ahead in (64, 0):
+        d = mdgpu.Domain(L, dt=0.005, precision=precision, capacity=n)
+        d.add_particles(x, p)
+        if thermostat == "langevin":
+            d.set_thermostat(mdgpu.LANGEVIN, 1.2, 1.0, 0.1)
+            d.set_seed(11)
+        d.set_option(mdgpu.OPT_RUN_AHEAD, ahead)
+        ds.append(d)
+    a, b = ds
+    assert a.get_option(mdgpu.OPT_RUN_AHEAD) == 64 and b.get_option(mdgpu.OPT_RUN_AHEAD) == 0
+    ra = rb = 0
+    for chunk in (1, 37, 100, 2, 60):
+        ra += a.calculate(chunk)
+        rb += b.calculate(chunk)
+        assert ra == rb, (chunk, ra, rb)
+    assert ra >= 5, ra
+    qa, pa, ta = a.particles()
+    qb, pb, tb = b.particles()
+    assert np.array_equal(ta, tb)
+    assert np.array_equal(qa, qb) and np.array_equal(pa, pb)
+    assert a.time() == b.time()
+    say("constructions", ra)
+    a.close()
+    b.close()
+
+
 def test_runs_are_bitwise_deterministic():
     say("case: determinism")
     L = 20.0
