@@ -209,9 +209,11 @@ int mdgpu_get_timing(const mdgpu_domain *d, int *force_launches, double *force_m
                                 the pairs within rc + value/1000, pruned from the full list
                                 whenever the particles could have moved value/1000 since
                                 (0 = off, the default; the same pairs in the same order) */
-#define MDGPU_OPT_ROW_SORT 10  /* rows of a brick ordered by list length (1, the default: less
-                                  slice padding) or kept in cell order (0: a slice's partners lie
-                                  close together in LDS); the next step rebuilds the list */
+#define MDGPU_OPT_ROW_SORT 10  /* rows of a brick ordered by list length counted in steps of
+                                  this value, 1..64 (default 8, the slices' rounding: no
+                                  more padding than exact lengths, and rows of one cell stay
+                                  together), or kept in cell order (0); the next step
+                                  rebuilds the list */
 #define MDGPU_OPT_BRICK_KB 11  /* LDS budget, KiB (8..160), that the automatic bricks are sized
                                   for (default 96 for a domain, 76 for an ensemble); setting it
                                   chooses the bricks again and the next step rebuilds the list */

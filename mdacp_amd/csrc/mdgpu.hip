@@ -60,7 +60,7 @@ struct mdgpu_domain {
   int lds_max_kb = 60;    // device limit per workgroup less the kernels' static LDS
   int search_waves = 16;  // wavefronts per brick workgroup of the neighbour search: 4, 8 or 16
   int search_stage = 1;   // neighbour search reads candidates staged in LDS (1) or from global memory (0)
-  int row_sort = 1;       // rows of a brick ordered by list length (1) or kept in cell order (0)
+  int row_sort = kSliceRound;  // rows of a brick by list length in steps of this (0: cell order)
   int search_cells = 1;   // neighbour search by a wavefront per cell (1) or a thread per row (0)
 
   // SimulationInfo / MDRect / MeshList geometry
@@ -2282,7 +2282,8 @@ int mdgpu_set_option(mdgpu_domain *d, int option, int value) {
       d->prune_delta = value * 1.0e-3;
       return MDGPU_OK;
     case MDGPU_OPT_ROW_SORT:
-      d->row_sort = value != 0;
+      if (value < 0 || value > 64) return MDGPU_ERR_ARGUMENT;
+      d->row_sort = value;
       d->initialized = false;  // the next step rebuilds the list
       return MDGPU_OK;
     case MDGPU_OPT_SEARCH_CELLS:
