@@ -107,25 +107,29 @@ __global__ void k_slice_brick(const int *__restrict__ vslice0, const int *__rest
 // cell r, range 2r+1 = its ghosts (cells outside the extended grid are
 // empty).  tstart = first index in the owned+ghost arrays (ghosts from pn on),
 // tbase = offset in the staged array (tbase[R] = staged count = pad slot).
-// The largest staged count goes to maxpart (per workgroup).
-__global__ void k_brick_ranges(Bricks B, CellTables T, int nbrick, int pn, int *__restrict__ tstart,
-                               int *__restrict__ tbase, int *__restrict__ nrange, int *__restrict__ btotal,
-                               double *__restrict__ maxpart) {
-  const int b = blockIdx.x * blockDim.x + threadIdx.x;
-  double total = 0.0;
-  if (b < nbrick) {
-    const Replica &Rp = B.reps[B.brep[b]];
-    const BrickBox X = brick_box(b, Rp);
-    const int nc = X.n[0] * X.n[1] * X.n[2];
-    const int stride = kRangesPerCell * B.rmax;
-    int *ts = tstart + (long long)b * stride;
-    int *tb = tbase + (long long)b * (stride + 1);
-    int base = 0;
-    for (int r = 0; r < nc; r++) {
+// One wavefront per brick, a lane per neighbourhood cell, the offsets from a
+// wavefront prefix sum; each brick's staged count also goes to total_d.
+__global__ __launch_bounds__(kBlock) void k_brick_ranges(Bricks B, CellTables T, int nbrick, int pn,
+                                                         int *__restrict__ tstart, int *__restrict__ tbase,
+                                                         int *__restrict__ nrange, int *__restrict__ btotal,
+                                                         double *__restrict__ total_d) {
+  const int b = blockIdx.x * (kBlock / kWave) + threadIdx.x / kWave;
+  const int lane = threadIdx.x % kWave;
+  if (b >= nbrick) return;  // the whole wavefront
+  const Replica &Rp = B.reps[B.brep[b]];
+  const BrickBox X = brick_box(b, Rp);
+  const int nc = X.n[0] * X.n[1] * X.n[2];
+  const int stride = kRangesPerCell * B.rmax;
+  int *ts = tstart + (long long)b * stride;
+  int *tb = tbase + (long long)b * (stride + 1);
+  int base = 0;
+  for (int r0 = 0; r0 < nc; r0 += kWave) {
+    const int r = r0 + lane;
+    int os = 0, oc = 0, gs = pn, gc = 0;
+    if (r < nc) {
       const int x = X.lo[0] + r % X.n[0];
       const int y = X.lo[1] + (r / X.n[0]) % X.n[1];
       const int z = X.lo[2] + r / (X.n[0] * X.n[1]);
-      int os = 0, oc = 0, gs = pn, gc = 0;
       if (x >= 0 && x < Rp.m[0] && y >= 0 && y < Rp.m[1] && z >= 0 && z < Rp.m[2]) {
         const int c = Rp.cell0 + (z * Rp.m[1] + y) * Rp.m[0] + x;
         os = T.ostart[c];
@@ -133,19 +137,28 @@ __global__ void k_brick_ranges(Bricks B, CellTables T, int nbrick, int pn, int *
         gs = pn + T.gstart[c];
         gc = T.gcount[c];
       }
-      ts[2 * r] = os;
-      tb[2 * r] = base;
-      base += oc;
-      ts[2 * r + 1] = gs;
-      tb[2 * r + 1] = base;
-      base += gc;
     }
+    const int cnt = oc + gc;
+    int inc = cnt;  // inclusive prefix sum over the wavefront
+    for (int o = 1; o < kWave; o <<= 1) {
+      const int v = __shfl_up(inc, o, kWave);
+      if (lane >= o) inc += v;
+    }
+    const int ex = base + inc - cnt;
+    if (r < nc) {
+      ts[2 * r] = os;
+      tb[2 * r] = ex;
+      ts[2 * r + 1] = gs;
+      tb[2 * r + 1] = ex + oc;
+    }
+    base += __shfl(inc, kWave - 1, kWave);
+  }
+  if (lane == 0) {
     tb[kRangesPerCell * nc] = base;
     nrange[b] = kRangesPerCell * nc;
     btotal[b] = base;
-    total = (double)base;
+    total_d[b] = (double)base;
   }
-  block_max_to_partial(total, maxpart, blockIdx.x);
 }
 
 // Gather list of each brick: glist[gbase[b] + u] = index (owned, or pn + ghost)

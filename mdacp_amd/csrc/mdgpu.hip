@@ -800,9 +800,9 @@ int make_pair_list(mdgpu_domain *d) {
   }
   hipLaunchKernelGGL(k_slice_brick, dim3(nblk(d->nbrick)), dim3(kBlock), 0, st, d->vslice0, d->nsl, d->nbrick,
                      d->slice_brick);
-  hipLaunchKernelGGL(k_brick_ranges, dim3(nblk(d->nbrick)), dim3(kBlock), 0, st, B, T, d->nbrick, pn, d->tstart,
-                     d->tbase, d->nrange, d->btotal, d->bpart);
-  hipLaunchKernelGGL(k_reduce_max, dim3(1), dim3(kBlock), 0, st, d->bpart, (int)nblk(d->nbrick), d->u64 + 2);
+  hipLaunchKernelGGL(k_brick_ranges, dim3(nblk((long long)d->nbrick * kWave)), dim3(kBlock), 0, st, B, T,
+                     d->nbrick, pn, d->tstart, d->tbase, d->nrange, d->btotal, d->bpart);
+  hipLaunchKernelGGL(k_reduce_max, dim3(1), dim3(kBlock), 0, st, d->bpart, d->nbrick, d->u64 + 2);
   double mstaged = 0.0;
   CHK(read_max(d, 2, &mstaged));
   d->max_staged = (int)mstaged;
