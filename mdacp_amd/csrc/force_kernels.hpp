@@ -524,7 +524,11 @@ __global__ __launch_bounds__(kWave *W) void k_prune_brick(const double4 *__restr
 // owned-owned pairs appear in two rows, owned-ghost pairs in one row and are
 // halved by the reference (observer.cc:39-41,70-72).  Owned partners are at
 // their current positions, ghosts where the last halo refresh put them, as
-// the reference's are.
+// the reference's are.  The pair terms come from one division per pair,
+// ir2 = 1/r^2 and ir6 = ir2^3: PE 4 (ir6^2 - ir6 + C2 r^2 + C0) and virial
+// ir6 (24 - 48 ir6) + 8 C2 r^2, which are the reference's
+// 4 (1/r^12 - 1/r^6 + C2 r^2 + C0) and ((24 r^6 - 48)/(r^6 r^6 r^2) + 8 C2) r^2
+// to a few roundings.  Indices come 8 per 16-byte load, as in the kick.
 template <int W>
 __global__ __launch_bounds__(kWave *W) void k_observe_brick(const double4 *__restrict__ pos,
                                                           const double4 *__restrict__ mom, BrickList bl,
@@ -557,27 +561,36 @@ __global__ __launch_bounds__(kWave *W) void k_observe_brick(const double4 *__res
     ke += 0.5 * p.x * p.x + 0.5 * p.y * p.y + 0.5 * p.z * p.z + 0.5 * p.w * p.w;
     const double4 qi = pos[i];
     const int len = bl.slen[s];
-    const uint16_t *row = bl.nbr + bl.sbase[s];
-    for (int k = 0; k < len; k++) {
-      const int e = row[slot_of(k, lane)];
-      if (e >= total) continue;  // padding
-      double qx, qy, qz;
-      if (staged) {
-        qx = lds64[3 * e];
-        qy = lds64[3 * e + 1];
-        qz = lds64[3 * e + 2];
-      } else {
-        const double4 q = pos[gl[e]];
-        qx = q.x;
-        qy = q.y;
-        qz = q.z;
+    const uint4 *rg = reinterpret_cast<const uint4 *>(bl.nbr + bl.sbase[s]) + lane;  // group g at rg[g * kWave]
+    for (int k = 0; k < len; k += kGroup) {
+      int e[kGroup];
+      unpack_group(rg[(k / kGroup) * kWave], e);
+      double qx[kGroup], qy[kGroup], qz[kGroup];
+#pragma unroll
+      for (int u = 0; u < kGroup; u++) {
+        if (staged) {  // padding reads the staged far sentinel
+          qx[u] = lds64[3 * e[u]];
+          qy[u] = lds64[3 * e[u] + 1];
+          qz[u] = lds64[3 * e[u] + 2];
+        } else if (e[u] < total) {
+          const double4 q = pos[gl[e[u]]];
+          qx[u] = q.x;
+          qy[u] = q.y;
+          qz[u] = q.z;
+        } else {
+          qx[u] = qy[u] = qz[u] = 1.0e200;
+        }
       }
-      const double dx = qi.x - qx, dy = qi.y - qy, dz = qi.z - qz;
-      const double r2 = (dx * dx + dy * dy + dz * dz);
-      if (r2 > CL2) continue;
-      const double r6 = r2 * r2 * r2;
-      pe += 0.5 * (4.0 * (1.0 / (r2 * r2 * r2 * r2 * r2 * r2) - 1.0 / r6 + C2 * r2 + C0));
-      vir += 0.5 * (((24.0 * r6 - 48.0) / (r6 * r6 * r2) + C2 * 8.0) * r2);
+#pragma unroll
+      for (int u = 0; u < kGroup; u++) {
+        const double dx = qi.x - qx[u], dy = qi.y - qy[u], dz = qi.z - qz[u];
+        const double r2 = (dx * dx + dy * dy + dz * dz);
+        const double ir2 = 1.0 / r2;
+        const double ir6 = ir2 * ir2 * ir2;
+        const bool in = !(r2 > CL2);
+        pe += in ? 0.5 * (4.0 * (ir6 * ir6 - ir6 + C2 * r2 + C0)) : 0.0;
+        vir += in ? 0.5 * (ir6 * (24.0 - 48.0 * ir6) + C2 * 8.0 * r2) : 0.0;
+      }
     }
   }
   ke = wave_sum(ke);

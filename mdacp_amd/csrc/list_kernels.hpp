@@ -411,6 +411,13 @@ __device__ __forceinline__ void test_rows(int nr, int ur0, const double *__restr
   }
 }
 
+// The next item of a workgroup's shared counter, for a whole wavefront.
+__device__ __forceinline__ int take_next(int *counter, int lane) {
+  int t = 0;
+  if (lane == 0) t = atomicAdd(counter, 1);
+  return __shfl(t, 0, kWave);
+}
+
 // The same search and the same masks, one wavefront per cell of the brick:
 // the cell's candidates (its 9 runs, concatenated) are taken 64 at a time, one
 // per lane, and each row of the cell is tested against all 64 at once; the
@@ -427,10 +434,12 @@ __global__ __launch_bounds__(kWave *W) void k_search_cells(const double4 *__rest
                                                           int *__restrict__ err) {
   extern __shared__ __align__(16) double lds64[];
   __shared__ int tb[kMaxRanges + 1];
+  __shared__ int cell_next;  // the next cell a wavefront takes
   constexpr int T = kWave * W;
   const int b = xcd_block(blockIdx.x, gridDim.x);
   if (st.nrow[b] == 0) return;  // a brick without rows
   const int R = load_tb<T>(st, b, tb);
+  if (threadIdx.x == 0) cell_next = W;
   __syncthreads();
   const int total = tb[R];
   const int *gl = st.glist + st.gbase[b];
@@ -443,7 +452,8 @@ __global__ __launch_bounds__(kWave *W) void k_search_cells(const double4 *__rest
   const int wv = threadIdx.x / kWave, lane = threadIdx.x % kWave;
   // the brick's own cells are the neighbourhood less its outer layer
   const int ox = X.n[0] - 2, oy = X.n[1] - 2, oz = X.n[2] - 2;
-  for (int c = wv; c < ox * oy * oz; c += W) {
+  // cells differ in rows and candidates: each wavefront takes the next one as it finishes
+  for (int c = wv; c < ox * oy * oz; c = take_next(&cell_next, lane)) {
     const int lx = 1 + c % ox, ly = 1 + (c / ox) % oy, lz = 1 + c / (ox * oy);
     const int rs = kRangesPerCell * ((lz * X.n[1] + ly) * X.n[0] + lx);
     const int urow0 = __builtin_amdgcn_readfirstlane(tb[rs]);  // staged index of the cell's first row
