@@ -444,7 +444,10 @@ int ensure_ghosts(mdgpu_domain *d, int need) {
     d->err = "particle capacity exceeded (owned + ghosts)";
     return MDGPU_ERR_CAPACITY;
   }
-  const int ngc = std::max(need, d->gcap + d->gcap / 4);
+  // room for the ghost count to fluctuate between constructions: growing
+  // reallocates and copies every per-entry array
+  const long long want = std::max<long long>((long long)need + need / 8 + 1024, (long long)d->gcap + d->gcap / 4);
+  const int ngc = (int)std::min<long long>(want, kMaxParticles - 1 - (long long)d->cap);
   const size_t tc_new = (size_t)d->cap + ngc + 1;
   CHK(dgrow_copy(d, &d->pos, d->pn, tc_new));
   CHK(dgrow_copy(d, &d->cell, d->pn, tc_new));
