@@ -285,6 +285,7 @@ __global__ __launch_bounds__(kWave *W) void k_force_brick_fp64(const double4 *__
                                                               const int *__restrict__ id,
                                                               double *__restrict__ vpart, NextDrift nd) {
   extern __shared__ __align__(16) double lds64[];
+  __shared__ int slice_next;  // the next slice a wavefront takes
   constexpr int T = kWave * W;
   if (run_halted(sc.gate)) return;
   const int b = xcd_block(blockIdx.x, gridDim.x);
@@ -299,12 +300,14 @@ __global__ __launch_bounds__(kWave *W) void k_force_brick_fp64(const double4 *__
   const int total = (int)(bl.gbase[b + 1] - g0);
   const int *gl = bl.glist + g0;
   const bool staged = total < bl.lds_cap;  // uniform across the workgroup
+  if (threadIdx.x == 0) slice_next = W;
   if (staged) stage_fp64<T>(gl, total, src, lds64);
   __syncthreads();
   const int nsl = (nrow + kWave - 1) / kWave;
   const int wv = threadIdx.x / kWave, lane = threadIdx.x % kWave;
   double vmax = 0.0;
-  for (int sl = wv; sl < nsl; sl += W) {
+  // slices come longest first: each wavefront takes the next one as it finishes
+  for (int sl = wv; sl < nsl; sl = take_next(&slice_next, lane)) {
     const int s = vs0 + sl;
     const int v = sl * kWave + lane;
     const bool ok = v < nrow;
@@ -339,6 +342,7 @@ __global__ __launch_bounds__(kWave *W) void k_force_brick_mixed(
     const int *__restrict__ id, double *__restrict__ vpart, NextDrift nd) {
   extern __shared__ float4 lds32[];
   __shared__ float4 offs[64];
+  __shared__ int slice_next;  // the next slice a wavefront takes
   constexpr int T = kWave * W;
   if (run_halted(sc.gate)) return;
   const int b = xcd_block(blockIdx.x, gridDim.x);
@@ -351,7 +355,7 @@ __global__ __launch_bounds__(kWave *W) void k_force_brick_mixed(
     offs[c] = make_float4(ox * w0, oy * w1, oz * w2, 0.0f);
   }
   const int nrow = bl.nrow[b], row0 = bl.row0[b], vs0 = bl.vslice0[b];
-  if (nrow == 0) {  // a brick of ghost-layer cells only: nothing to stage
+  if (nrow == 0) {  // a brick without rows: nothing to stage
     if (vpart) block_max_to_partial(0.0, vpart, b);
     return;
   }
@@ -376,11 +380,13 @@ __global__ __launch_bounds__(kWave *W) void k_force_brick_mixed(
     }
     if (threadIdx.x == 0) lds32[total] = make_float4(1.0e30f, 1.0e30f, 1.0e30f, 0.0f);
   }
+  if (threadIdx.x == 0) slice_next = W;
   __syncthreads();
   const int nsl = (nrow + kWave - 1) / kWave;
   const int wv = threadIdx.x / kWave, lane = threadIdx.x % kWave;
   double vmax = 0.0;
-  for (int sl = wv; sl < nsl; sl += W) {
+  // slices come longest first: each wavefront takes the next one as it finishes
+  for (int sl = wv; sl < nsl; sl = take_next(&slice_next, lane)) {
     const int s = vs0 + sl;
     const int v = sl * kWave + lane;
     const bool ok = v < nrow;
