@@ -132,6 +132,42 @@ def test_search_kernels_agree_with_oracle(cells, stage):
     d.close()
 
 
+def test_layout_options_keep_pairs_and_kicks():
+    """Row order (MDGPU_OPT_ROW_SORT), per-axis brick edges (MDGPU_OPT_BRICK) and the brick LDS
+    budget (MDGPU_OPT_BRICK_KB) change where rows and partners sit, never the pairs: the oracle's
+    pair set every time; kicks bit-identical across row orders (a row sums its partners in the
+    same order wherever it sits) and within 1e-12 across bricks (the partners' order changes)."""
+    say("case: layout options")
+    L = 21.0
+    q, p, ids = melted_state(L, 0.6, steps=150, seed=4)
+    d, o = make_pair(L, q, p, ids)
+    o.make_pair_list()
+    op = O.canonical_pairs(o.pairs())
+    ko = o.kick()[np.argsort(o.state()[2])]
+    C0, C2 = lj_C(2.5)
+    qo, _, to = o.state()
+    S = pair_force_sums(qo[np.argsort(to)], o.pairs().astype(np.int64), L, 2.5, 0.001, C2)
+    kicks = {}
+    for name, opt, value in [("sort8", mdgpu.OPT_ROW_SORT, 8), ("sort1", mdgpu.OPT_ROW_SORT, 1),
+                             ("sort0", mdgpu.OPT_ROW_SORT, 0), ("sort16", mdgpu.OPT_ROW_SORT, 16),
+                             ("brick345", mdgpu.OPT_BRICK, 3 | 4 << 8 | 5 << 16),
+                             ("brick2", mdgpu.OPT_BRICK, 2 | 2 << 8 | 2 << 16),
+                             ("kb40", mdgpu.OPT_BRICK_KB, 40), ("kb128", mdgpu.OPT_BRICK_KB, 128)]:
+        d.set_option(opt, value)
+        d.make_pair_list()
+        assert np.array_equal(O.canonical_pairs(d.pairs()), op), name
+        t = d.particles()[2]
+        kicks[name] = d.kick()[np.argsort(t)]
+        err = np.abs(kicks[name] - ko).max(axis=1)
+        assert np.all(err <= 1e-12 * np.maximum(S, 1e-300) + 1e-300), name
+        if name.startswith("sort"):
+            d.set_option(mdgpu.OPT_ROW_SORT, 8)
+    for name in ("sort1", "sort0", "sort16"):
+        assert np.array_equal(kicks[name], kicks["sort8"]), name
+    say("pairs", len(op), "bricks", d.get_option(mdgpu.OPT_BRICK) & 0xff)
+    d.close()
+
+
 def test_golden_pairs_and_kicks_fcc2048(golden_dir):
     say("case: golden fixture fcc2048 step 0")
     g = load_golden("fcc2048_nve")
