@@ -27,9 +27,30 @@ __device__ __forceinline__ int cell_of(double x, double y, double z, const Repli
   return R.cell0 + R.m[0] * R.m[1] * iz + R.m[0] * iy + ix;
 }
 
+// A slot in cell c's counter for each active lane of the wavefront: one
+// atomic add per distinct cell among the active lanes, whose lanes take
+// consecutive slots in lane order.  Particles arrive mostly in cell order, so
+// a wavefront meets few cells, and per-particle atomics on one counter would
+// serialise.  The order within a cell is settled later by particle id.
+__device__ __forceinline__ int cell_slot(int *__restrict__ counts, int c) {
+  unsigned long long todo = __ballot(1);
+  const int lane = __lane_id();
+  int slot = 0;
+  while (todo) {
+    const int leader = __ffsll((long long)todo) - 1;
+    const int cl = __shfl(c, leader, kWave);
+    const unsigned long long same = __ballot(c == cl) & todo;
+    int base = 0;
+    if (lane == leader) base = atomicAdd(&counts[cl], __popcll(same));
+    base = __shfl(base, leader, kWave);
+    if (c == cl) slot = base + __popcll(same & ((1ull << lane) - 1ull));
+    todo &= ~same;
+  }
+  return slot;
+}
+
 // Wrap owned particles into their replica's box (one +-L per component), flag
-// positions still outside, bin, and take a slot in the cell by atomic
-// increment.
+// positions still outside, bin, and take a slot in the cell (cell_slot).
 __global__ void k_wrap_bin(double4 *__restrict__ pos, int n, const int *__restrict__ rep,
                            const Replica *__restrict__ reps, int *__restrict__ cell, int *__restrict__ slot,
                            int *__restrict__ ccount, int *__restrict__ err, int wrap) {
@@ -54,7 +75,7 @@ __global__ void k_wrap_bin(double4 *__restrict__ pos, int n, const int *__restri
   if (l % R.m[0] == 0 || (l / R.m[0]) % R.m[1] == 0 || l / (R.m[0] * R.m[1]) == 0)
     atomicOr(err, kErrInvalidPosition);
   cell[i] = c;
-  slot[i] = atomicAdd(&ccount[c], 1);
+  slot[i] = cell_slot(ccount, c);
 }
 
 __global__ void k_scatter_perm(const int *__restrict__ cell, const int *__restrict__ slot,
@@ -163,7 +184,7 @@ __global__ void k_emit_images(const double4 *__restrict__ pos, int n, const int 
         gpos[k] = make_double4(y.x, y.y, y.z, 0.0);
         const int cl = cell_of(y.x, y.y, y.z, R);
         gcell[k] = cl;
-        gslot[k] = atomicAdd(&gcount[cl], 1);
+        gslot[k] = cell_slot(gcount, cl);
         k++;
       }
     }
