@@ -29,24 +29,30 @@ __device__ __forceinline__ int cell_of(double x, double y, double z, const Repli
 
 // A slot in cell c's counter for each active lane of the wavefront: one
 // atomic add per distinct cell among the active lanes, whose lanes take
-// consecutive slots in lane order.  Particles arrive mostly in cell order, so
-// a wavefront meets few cells, and per-particle atomics on one counter would
+// consecutive slots in lane order.  The lanes are first grouped by cell with
+// ballots (no memory traffic); then the first lane of every group adds its
+// group's size in the same instruction, so the wavefront waits for one atomic
+// round trip however many cells it meets (rows sorted by list length put many
+// cells in one wavefront).  Per-particle atomics on one counter would
 // serialise.  The order within a cell is settled later by particle id.
 __device__ __forceinline__ int cell_slot(int *__restrict__ counts, int c) {
   unsigned long long todo = __ballot(1);
   const int lane = __lane_id();
-  int slot = 0;
+  int leader_of = lane, rank = 0, size = 0;
   while (todo) {
     const int leader = __ffsll((long long)todo) - 1;
     const int cl = __shfl(c, leader, kWave);
     const unsigned long long same = __ballot(c == cl) & todo;
-    int base = 0;
-    if (lane == leader) base = atomicAdd(&counts[cl], __popcll(same));
-    base = __shfl(base, leader, kWave);
-    if (c == cl) slot = base + __popcll(same & ((1ull << lane) - 1ull));
+    if (c == cl) {
+      leader_of = leader;
+      rank = __popcll(same & ((1ull << lane) - 1ull));
+    }
+    if (lane == leader) size = __popcll(same);
     todo &= ~same;
   }
-  return slot;
+  int base = 0;
+  if (lane == leader_of) base = atomicAdd(&counts[c], size);
+  return __shfl(base, leader_of, kWave) + rank;
 }
 
 // Wrap owned particles into their replica's box (one +-L per component), flag
