@@ -67,6 +67,14 @@ __device__ __forceinline__ double rcp64(double x) {
   return fma(r, fma(-x, r, 1.0), r);
 }
 
+// 1/x with a second Newton step: within about an ulp of the quotient, for
+// sums that must agree with divisions closely (the observers' PE and virial
+// sum r^-12 terms, which would scale the one-step error by 6 with one sign).
+__device__ __forceinline__ double rcp64_exact(double x) {
+  const double r = rcp64(x);
+  return fma(r, fma(-x, r, 1.0), r);
+}
+
 // The 8 staged indices of one group of a lane's list slots (one 16-byte load).
 __device__ __forceinline__ void unpack_group(const uint4 g, int e[kGroup]) {
   e[0] = g.x & 0xffff; e[1] = g.x >> 16; e[2] = g.y & 0xffff; e[3] = g.y >> 16;
@@ -530,11 +538,15 @@ __global__ __launch_bounds__(kWave *W) void k_prune_brick(const double4 *__restr
 // owned-owned pairs appear in two rows, owned-ghost pairs in one row and are
 // halved by the reference (observer.cc:39-41,70-72).  Owned partners are at
 // their current positions, ghosts where the last halo refresh put them, as
-// the reference's are.  The pair terms come from one division per pair,
-// ir2 = 1/r^2 and ir6 = ir2^3: PE 4 (ir6^2 - ir6 + C2 r^2 + C0) and virial
-// ir6 (24 - 48 ir6) + 8 C2 r^2, which are the reference's
-// 4 (1/r^12 - 1/r^6 + C2 r^2 + C0) and ((24 r^6 - 48)/(r^6 r^6 r^2) + 8 C2) r^2
-// to a few roundings.  Indices come 8 per 16-byte load, as in the kick.
+// the reference's are.  The pair terms come from one reciprocal per pair,
+// ir2 = 1/r^2 (rcp64_exact) and ir6 = ir2^3: half of PE 4 (ir6^2 - ir6 + C2 r^2
+// + C0) and half of the virial ir6 (24 - 48 ir6) + 8 C2 r^2, which are the
+// reference's 4 (1/r^12 - 1/r^6 + C2 r^2 + C0) and
+// ((24 r^6 - 48)/(r^6 r^6 r^2) + 8 C2) r^2 to a few roundings.  A row sums
+// ir6^2 - ir6 + C2 r^2 + C0 and 3 ir6 - 6 ir6^2 + C2 r^2 and scales the sums
+// by 2 and 4 (exact) at the end.  r^2 is summed without fused multiply-adds,
+// so the cutoff keeps the reference's pairs.  Indices come 8 per 16-byte load,
+// the next group in flight while a group is computed, as in the kick.
 template <int W>
 __global__ __launch_bounds__(kWave *W) void k_observe_brick(const double4 *__restrict__ pos,
                                                           const double4 *__restrict__ mom, BrickList bl,
@@ -568,9 +580,13 @@ __global__ __launch_bounds__(kWave *W) void k_observe_brick(const double4 *__res
     const double4 qi = pos[i];
     const int len = bl.slen[s];
     const uint4 *rg = reinterpret_cast<const uint4 *>(bl.nbr + bl.sbase[s]) + lane;  // group g at rg[g * kWave]
+    double spe = 0.0, svir = 0.0;
+    uint4 ahead = rg[0];  // one group past the slice stays inside the list arrays (kListTail)
     for (int k = 0; k < len; k += kGroup) {
       int e[kGroup];
-      unpack_group(rg[(k / kGroup) * kWave], e);
+      unpack_group(ahead, e);
+      ahead = rg[(k / kGroup + 1) * kWave];
+      __builtin_amdgcn_sched_barrier(0);  // the prefetch is issued before this group's work
       double qx[kGroup], qy[kGroup], qz[kGroup];
 #pragma unroll
       for (int u = 0; u < kGroup; u++) {
@@ -591,13 +607,19 @@ __global__ __launch_bounds__(kWave *W) void k_observe_brick(const double4 *__res
       for (int u = 0; u < kGroup; u++) {
         const double dx = qi.x - qx[u], dy = qi.y - qy[u], dz = qi.z - qz[u];
         const double r2 = (dx * dx + dy * dy + dz * dz);
-        const double ir2 = 1.0 / r2;
+        const double ir2 = rcp64_exact(r2);  // padding: r2 = inf, NaN terms, dropped by the select
         const double ir6 = ir2 * ir2 * ir2;
+        const double c2r2 = C2 * r2;
         const bool in = !(r2 > CL2);
-        pe += in ? 0.5 * (4.0 * (ir6 * ir6 - ir6 + C2 * r2 + C0)) : 0.0;
-        vir += in ? 0.5 * (ir6 * (24.0 - 48.0 * ir6) + C2 * 8.0 * r2) : 0.0;
+        const double tpe = fma(ir6, ir6, -ir6) + (c2r2 + C0);
+        const double tvir = fma(ir6, fma(-6.0, ir6, 3.0), c2r2);
+        spe += in ? tpe : 0.0;
+        svir += in ? tvir : 0.0;
       }
+      hold_prefetch(ahead, spe, svir, ke);
     }
+    pe += 2.0 * spe;
+    vir += 4.0 * svir;
   }
   ke = wave_sum(ke);
   pe = wave_sum(pe);
