@@ -168,6 +168,8 @@ struct mdgpu_domain {
   int *t_gsrc = nullptr, *t_gcode = nullptr, *t_gcell = nullptr;
   long long *key = nullptr, *scan_scratch = nullptr;
   long long scan_scratch_n = 0;
+  double *xfer = nullptr;  // 3 doubles per particle: host buffers in the boundary's layout
+  size_t xfer_cap = 0;
   double *disp = nullptr, *partial = nullptr, *maxpart = nullptr;
   double *bpart = nullptr;  // one maximum per brick workgroup
   int partial_n = 0;
@@ -1693,7 +1695,7 @@ int mdgpu_destroy(mdgpu_domain *d) {
                   d->i32,     d->d3,          d->reps,        d->brep,      d->rep,      d->rep_alt,
                   d->grng,    d->rep_v2,      d->rep_out,     d->which,     d->rep_exp,  d->clock,
                   d->perm_in, d->nbr_in,      d->ncode_in,    d->slen_in,   d->chk_dmax, d->chk_cnt,
-                  d->chk_cand, d->gate,       d->gbudget,     d->glife};
+                  d->chk_cand, d->gate,       d->gbudget,     d->glife,     d->xfer};
   for (StepGraph &g : d->graphs) drop_graph(g);
   for (StepGraph &g : d->gated) drop_graph(g);
   for (void *b : bufs)
@@ -1767,38 +1769,49 @@ int mdgpu_particle_number(const mdgpu_domain *d, int *pn, int *tn) {
   return MDGPU_OK;
 }
 
+// The owned particles' x, y, z (of q or p) between the device's 32-byte
+// entries and a caller's array of 3 doubles per particle: packed or unpacked
+// on the device in `xfer`, so that one copy moves exactly the caller's bytes
+// and no host loop touches them.  The stream orders the reuse of `xfer`.
+int ensure_xfer(mdgpu_domain *d, size_t count) {
+  if (count <= d->xfer_cap) return MDGPU_OK;
+  CHK(dalloc(d, &d->xfer, count));
+  d->xfer_cap = count;
+  return MDGPU_OK;
+}
+
+int copy_out3(mdgpu_domain *d, const double4 *src, int n, double *dst) {
+  CHK(ensure_xfer(d, 3 * (size_t)n));
+  hipLaunchKernelGGL(k_pack3, dim3(nblk(n)), dim3(kBlock), 0, d->stream, src, n, d->xfer);
+  HIPCHK(d, hipMemcpyAsync(dst, d->xfer, 3 * (size_t)n * sizeof(double), hipMemcpyDeviceToHost, d->stream));
+  return MDGPU_OK;
+}
+
+int copy_in3(mdgpu_domain *d, const double *src, int n, double4 *dst) {
+  CHK(ensure_xfer(d, 3 * (size_t)n));
+  HIPCHK(d, hipMemcpyAsync(d->xfer, src, 3 * (size_t)n * sizeof(double), hipMemcpyHostToDevice, d->stream));
+  hipLaunchKernelGGL(k_unpack3, dim3(nblk(n)), dim3(kBlock), 0, d->stream, d->xfer, n, dst);
+  return MDGPU_OK;
+}
+
 int mdgpu_get_particles(mdgpu_domain *d, double *q, double *p, int *type) {
   if (!d) return MDGPU_ERR_ARGUMENT;
   const int n = d->pn;
-  std::vector<double4> hq(n), hp(n);
   if (n > 0) {
-    HIPCHK(d, hipMemcpyAsync(hq.data(), d->pos, n * sizeof(double4), hipMemcpyDeviceToHost, d->stream));
-    HIPCHK(d, hipMemcpyAsync(hp.data(), d->mom, n * sizeof(double4), hipMemcpyDeviceToHost, d->stream));
+    if (q) CHK(copy_out3(d, d->pos, n, q));
+    if (p) CHK(copy_out3(d, d->mom, n, p));
     if (type) HIPCHK(d, hipMemcpyAsync(type, d->id, n * sizeof(int), hipMemcpyDeviceToHost, d->stream));
   }
-  CHK(sync(d));
-  for (int i = 0; i < n; i++) {
-    if (q) {
-      q[3 * (size_t)i] = hq[i].x;
-      q[3 * (size_t)i + 1] = hq[i].y;
-      q[3 * (size_t)i + 2] = hq[i].z;
-    }
-    if (p) {
-      p[3 * (size_t)i] = hp[i].x;
-      p[3 * (size_t)i + 1] = hp[i].y;
-      p[3 * (size_t)i + 2] = hp[i].z;
-    }
-  }
-  return MDGPU_OK;
+  CHK(check_launch(d, "copy to host"));
+  return sync(d);
 }
 
 int mdgpu_set_momenta(mdgpu_domain *d, const double *p) {
   if (!d || !p) return MDGPU_ERR_ARGUMENT;
   invalidate(d);  // the next fused step recomputes its first half drift
   const int n = d->pn;
-  std::vector<double4> hp(n);
-  for (int i = 0; i < n; i++) hp[i] = make_double4(p[3 * (size_t)i], p[3 * (size_t)i + 1], p[3 * (size_t)i + 2], 0.0);
-  if (n > 0) HIPCHK(d, hipMemcpyAsync(d->mom, hp.data(), n * sizeof(double4), hipMemcpyHostToDevice, d->stream));
+  if (n > 0) CHK(copy_in3(d, p, n, d->mom));
+  CHK(check_launch(d, "copy from host"));
   CHK(sync(d));
   d->vmax_valid = false;
   return MDGPU_OK;
@@ -1808,9 +1821,8 @@ int mdgpu_set_positions(mdgpu_domain *d, const double *q) {
   if (!d || !q) return MDGPU_ERR_ARGUMENT;
   invalidate(d);  // the next fused step recomputes its first half drift
   const int n = d->pn;
-  std::vector<double4> hq(n);
-  for (int i = 0; i < n; i++) hq[i] = make_double4(q[3 * (size_t)i], q[3 * (size_t)i + 1], q[3 * (size_t)i + 2], 0.0);
-  if (n > 0) HIPCHK(d, hipMemcpyAsync(d->pos, hq.data(), n * sizeof(double4), hipMemcpyHostToDevice, d->stream));
+  if (n > 0) CHK(copy_in3(d, q, n, d->pos));
+  CHK(check_launch(d, "copy from host"));
   CHK(sync(d));
   return MDGPU_OK;
 }

@@ -210,6 +210,24 @@ __global__ void k_scale_momenta(double4 *__restrict__ mom, int pn, double f) {
 
 // ------------------------------------------------------------- reductions
 // One workgroup reduces nblk tuples of partial sums in a fixed order.
+// x, y, z of n 32-byte entries to and from 3 doubles per entry (host transfers)
+__global__ void k_pack3(const double4 *__restrict__ src, int n, double *__restrict__ out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double4 v = src[i];
+  double *o = out + 3 * (size_t)i;
+  o[0] = v.x;
+  o[1] = v.y;
+  o[2] = v.z;
+}
+
+__global__ void k_unpack3(const double *__restrict__ in, int n, double4 *__restrict__ dst) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double *v = in + 3 * (size_t)i;
+  dst[i] = make_double4(v[0], v[1], v[2], 0.0);
+}
+
 __global__ void k_reduce_partials(const double *__restrict__ partial, int nblk, int ncomp,
                                   double *__restrict__ out) {
   __shared__ double red[kBlock];
