@@ -478,14 +478,16 @@ int ensure_ghosts(mdgpu_domain *d, int need) {
 
 int ensure_scan(mdgpu_domain *d, long long n) {
   const long long need = scan_scratch_size(n);
-  if (need <= d->scan_scratch_n) return MDGPU_OK;
+  if (d->scan_scratch && need <= d->scan_scratch_n) return MDGPU_OK;
+  d->scan_scratch_n = 0;  // dalloc frees the old buffer first; a failed allocation leaves none
   CHK(dalloc(d, &d->scan_scratch, (size_t)need));
   d->scan_scratch_n = need;
   return MDGPU_OK;
 }
 
 int ensure_partial(mdgpu_domain *d, int nb) {
-  if (nb * 3 <= d->partial_n) return MDGPU_OK;
+  if (d->partial && nb * 3 <= d->partial_n) return MDGPU_OK;
+  d->partial_n = 0;
   CHK(dalloc(d, &d->partial, (size_t)nb * 3));
   d->partial_n = nb * 3;
   return MDGPU_OK;
@@ -836,7 +838,8 @@ int make_pair_list(mdgpu_domain *d) {
     // candidate masks: retried with room for more candidates if a row overflows
     for (;;) {
       const size_t need = (size_t)pn * d->mcap;
-      if (need > d->mask_cap) {
+      if (!d->mask || need > d->mask_cap) {
+        d->mask_cap = 0;
         CHK(dalloc(d, &d->mask, need + need / 8));
         d->mask_cap = need + need / 8;
       }
@@ -1774,7 +1777,8 @@ int mdgpu_particle_number(const mdgpu_domain *d, int *pn, int *tn) {
 // on the device in `xfer`, so that one copy moves exactly the caller's bytes
 // and no host loop touches them.  The stream orders the reuse of `xfer`.
 int ensure_xfer(mdgpu_domain *d, size_t count) {
-  if (count <= d->xfer_cap) return MDGPU_OK;
+  if (d->xfer && count <= d->xfer_cap) return MDGPU_OK;
+  d->xfer_cap = 0;  // dalloc frees the old buffer first; a failed allocation leaves none
   CHK(dalloc(d, &d->xfer, count));
   d->xfer_cap = count;
   return MDGPU_OK;
