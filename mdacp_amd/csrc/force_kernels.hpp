@@ -296,7 +296,7 @@ __global__ __launch_bounds__(kWave *W) void k_force_brick_fp64(const double4 *__
   __shared__ int slice_next;  // the next slice a wavefront takes
   constexpr int T = kWave * W;
   if (run_halted(sc.gate)) return;
-  const int b = xcd_block(blockIdx.x, gridDim.x);
+  const int b = blockIdx.x;
   const Replica &Rp = bl.reps[bl.brep[b]];
   const LangevinConsts lc = langevin_consts(Rp, sc.dt, step_of(sc));
   const int nrow = bl.nrow[b], row0 = bl.row0[b], vs0 = bl.vslice0[b];
@@ -353,7 +353,7 @@ __global__ __launch_bounds__(kWave *W) void k_force_brick_mixed(
   __shared__ int slice_next;  // the next slice a wavefront takes
   constexpr int T = kWave * W;
   if (run_halted(sc.gate)) return;
-  const int b = xcd_block(blockIdx.x, gridDim.x);
+  const int b = blockIdx.x;
   const Replica &Rp = bl.reps[bl.brep[b]];
   const LangevinConsts lc = langevin_consts(Rp, sc.dt, step_of(sc));
   if (threadIdx.x < 64) {
@@ -450,7 +450,7 @@ __global__ __launch_bounds__(kWave *W) void k_prune_brick(const double4 *__restr
                                                          int *__restrict__ slen_in) {
   extern __shared__ __align__(16) double lds64[];
   constexpr int T = kWave * W;
-  const int b = xcd_block(blockIdx.x, gridDim.x);
+  const int b = blockIdx.x;
   const int nrow = bl.nrow[b], row0 = bl.row0[b], vs0 = bl.vslice0[b];
   if (nrow == 0) return;
   const long long g0 = bl.gbase[b];
@@ -555,7 +555,7 @@ __global__ __launch_bounds__(kWave *W) void k_observe_brick(const double4 *__res
   extern __shared__ __align__(16) double lds64[];
   __shared__ double red[3][W];
   constexpr int T = kWave * W;
-  const int b = xcd_block(blockIdx.x, gridDim.x);
+  const int b = blockIdx.x;
   const int nrow = bl.nrow[b], row0 = bl.row0[b], vs0 = bl.vslice0[b];
   if (nrow == 0) {
     if (threadIdx.x < 3) partial[(long long)b * 3 + threadIdx.x] = 0.0;

@@ -288,7 +288,7 @@ __global__ __launch_bounds__(kWave *W) void k_search_brick(const double4 *__rest
   extern __shared__ __align__(16) double lds64[];
   __shared__ int tb[kMaxRanges + 1];
   constexpr int T = kWave * W;
-  const int b = xcd_block(blockIdx.x, gridDim.x);
+  const int b = blockIdx.x;
   if (st.nrow[b] == 0) return;  // a brick of ghost-layer cells only
   const int R = load_tb<T>(st, b, tb);
   __syncthreads();
@@ -436,7 +436,7 @@ __global__ __launch_bounds__(kWave *W) void k_search_cells(const double4 *__rest
   __shared__ int tb[kMaxRanges + 1];
   __shared__ int cell_next;  // the next cell a wavefront takes
   constexpr int T = kWave * W;
-  const int b = xcd_block(blockIdx.x, gridDim.x);
+  const int b = blockIdx.x;
   if (st.nrow[b] == 0) return;  // a brick without rows
   const int R = load_tb<T>(st, b, tb);
   if (threadIdx.x == 0) cell_next = W;
@@ -675,7 +675,7 @@ __global__ __launch_bounds__(kWave *W) void k_emit_brick(const int *__restrict__
                                                         uint8_t *__restrict__ ncode, int *__restrict__ err) {
   __shared__ int tb[kMaxRanges + 1];
   constexpr int T = kWave * W;
-  const int b = xcd_block(blockIdx.x, gridDim.x);
+  const int b = blockIdx.x;
   if (st.nrow[b] == 0) return;  // a brick without rows
   const int R = load_tb<T>(st, b, tb);
   __syncthreads();
