@@ -274,12 +274,15 @@ class Domain:
         return ke.value, pe.value, vir.value
 
     def observables(self):
-        """T, P, E per MDManager::Temperature/Pressure/TotalEnergy (mdmanager.cc:390-429)."""
+        """T, P, E per MDManager::Temperature/Pressure/TotalEnergy (mdmanager.cc:390-429), in IEEE
+        double arithmetic as there: NaN for a domain without particles."""
         ke, pe, vir = self.observe_raw()
-        pn = float(self.counts()[0])
-        T = (ke / pn) / 1.5
-        V = self.L[0] * self.L[1] * self.L[2]
-        return dict(ke=ke, pe=pe, vir=vir, T=T, P=(T - vir / pn) * pn / V, E=ke / pn + pe / pn)
+        pn = np.float64(self.counts()[0])
+        with np.errstate(divide="ignore", invalid="ignore"):
+            T = (ke / pn) / 1.5
+            V = self.L[0] * self.L[1] * self.L[2]
+            return dict(ke=ke, pe=pe, vir=vir, T=float(T), P=float((T - vir / pn) * pn / V),
+                        E=float(ke / pn + pe / pn))
 
     # -------------------------------------------------------------- export
     def pairs(self):

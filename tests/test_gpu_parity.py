@@ -399,6 +399,82 @@ def test_error_paths_raise():
     assert e.value.code == mdgpu.ERR_INVALID_POSITION
 
 
+
+# smoothed LJ constants at rc = 2.5 from C2 = 6 s^14 - 3 s^8, C0 = -s^12 + s^6 - C2 rc^2, s = 1/rc
+# (variables.cc:16-28), evaluated in exact arithmetic
+C2_25, C0_25 = -0.0019499738726400002, 0.016266559488
+
+
+def test_empty_domain_and_add_filter():
+    """No particles: a build, steps and observers run and give zeros.  Particles outside
+    [0, L) are not added (MDRect::IsInside), as in the oracle."""
+    say("case: empty domain, add filter")
+    L = 16.0
+    d = mdgpu.Domain(L, dt=0.001)
+    d.make_pair_list()
+    assert d.calculate(5) >= 0
+    assert d.counts() == (0, 0)
+    ob = d.observables()
+    assert ob["ke"] == 0.0 and ob["pe"] == 0.0 and ob["vir"] == 0.0
+    assert np.isnan(ob["T"]) and np.isnan(ob["E"])  # 0 / 0, as the reference's double division gives
+    q, p, t = d.particles()
+    assert q.shape == (0, 3) and p.shape == (0, 3) and t.shape == (0,)
+    x = np.array([[0.0, 1.0, 1.0], [L, 1.0, 1.0], [np.nextafter(L, 0.0), 2.0, 2.0], [-1e-12, 3.0, 3.0],
+                  [5.0, 5.0, L], [8.0, 8.0, 8.0]])
+    d.add_particles(x, None, np.arange(len(x), dtype=np.int32))
+    o = O.OracleSystem(L, 0.001, 2.5, 0.5)
+    o.add(x, None, np.arange(len(x), dtype=np.int32))
+    pn, _ = d.counts()
+    assert pn == 3 == len(o.state()[2])
+    assert np.array_equal(np.sort(d.particles()[2]), np.array([0, 2, 5]))
+    d.close()
+
+
+@pytest.mark.parametrize("across", [False, True])
+def test_dimer_closed_form(across):
+    """Two particles 1.0 apart, inside the box or through the periodic boundary: kick
+    (24 - 8 C2) dt along the separation, PE 4 (C2 + C0), virial (-24 + 8 C2) / 3; an NVE run
+    follows the oracle, energy included (the pair flies apart from r = 1, where the repulsion is
+    steep: position Verlet at dt = 0.001 ends 1.7e-3 above the initial energy, in both).  Inside the box the two rows see exactly
+    opposite separations, so the kicks are exact negatives and the total momentum stays exactly
+    zero; through the boundary each row pairs with the other particle's image, whose position is
+    rounded differently (15.3 - 16 against 0.3 + 16), and the kicks are opposite to rounding."""
+    say("case: dimer", "across the boundary" if across else "inside the box")
+    L, dt = 16.0, 0.001
+    x = np.array([[0.3, 8.0, 8.0], [L - 0.7, 8.0, 8.0]]) if across else np.array([[8.0, 8.0, 8.0], [9.0, 8.0, 8.0]])
+    sign = 1.0 if across else -1.0  # particle 0's partner lies at -x (its image) or at +x
+    d = mdgpu.Domain(L, dt=dt)
+    d.add_particles(x, None, np.array([0, 1], dtype=np.int32))
+    d.make_pair_list()
+    q, p, t = d.particles()
+    k = d.kick()[np.argsort(t)]
+    want = sign * (24.0 - 8.0 * C2_25) * dt
+    assert k[0] == pytest.approx([want, 0.0, 0.0], rel=1e-13, abs=1e-300)
+    if across:
+        assert k[1] == pytest.approx(-k[0], rel=1e-13, abs=1e-300)
+    else:
+        assert np.array_equal(k[1], -k[0])
+    ob = d.observables()
+    assert ob["pe"] == pytest.approx(4.0 * (C2_25 + C0_25), rel=1e-12)
+    assert ob["vir"] == pytest.approx((-24.0 + 8.0 * C2_25) / 3.0, rel=1e-12)
+    d.calculate(2000)
+    q, p, t = d.particles()
+    if across:
+        assert np.abs(p.sum(axis=0)).max() <= 1e-12 * np.abs(p).max()
+    else:
+        assert np.array_equal(p.sum(axis=0), np.zeros(3))
+    ob = d.observables()
+    o = O.OracleSystem(L, dt, 2.5, 0.5)
+    o.add(x, None, np.array([0, 1], dtype=np.int32))
+    o.make_pair_list()
+    for _ in range(2000):
+        o.step()
+    oq, op, ot = o.state()
+    ia, ib = np.argsort(t), np.argsort(ot)
+    assert np.abs(q[ia] - oq[ib]).max() < 1e-10 and np.abs(p[ia] - op[ib]).max() < 1e-10
+    assert ob["E"] == pytest.approx(o.observables()["E"], rel=1e-10)
+    d.close()
+
 def test_full_size_4M_pairs_exact_and_closed_form():
     """4,000,000 particles, rho 0.5, L 200, rc 2.5, margin 0.5 (perfect FCC):
     27 half pairs per particle, bit-exact with the oracle; again after 1000 steps."""
