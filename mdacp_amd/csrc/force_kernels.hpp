@@ -43,6 +43,8 @@ struct BrickList {
   const int *row0, *nrow, *vslice0;
   const int *brep;          // replica of each brick
   const Replica *reps;
+  const double4 *ghost;     // FP64 positions of the ghost images (gather entries from pn on)
+  int pn;                   // owned particles
   int lds_cap;              // staged entries that fit the LDS budget (incl. the pad entry)
 };
 
@@ -113,7 +115,8 @@ __device__ __forceinline__ void hold_prefetch(uint2 &g, const A &a, const A &b, 
 template <bool kStaged, int P>
 __device__ __forceinline__ double3 row_kick_fp64(const uint4 *__restrict__ rg, int len, const double4 qi,
                                                  const double *__restrict__ lds, const double4 *__restrict__ src,
-                                                 const int *__restrict__ gl, int total, const ForceConsts &fc) {
+                                                 const int *__restrict__ gl, int total, const ForceConsts &fc,
+                                                 int pn, const double4 *__restrict__ ghost) {
   double ax = 0.0, ay = 0.0, az = 0.0;
   uint4 ahead[P];
 #pragma unroll
@@ -134,7 +137,7 @@ __device__ __forceinline__ double3 row_kick_fp64(const uint4 *__restrict__ rg, i
         qy[u] = q[1];
         qz[u] = q[2];
       } else if (e[u] < total) {
-        const double4 q = src[gl[e[u]]];
+        const double4 q = gather_pos(src, ghost, pn, gl[e[u]]);
         qx[u] = q.x;
         qy[u] = q.y;
         qz[u] = q.z;
@@ -309,7 +312,7 @@ __global__ __launch_bounds__(kWave *W) void k_force_brick_fp64(const double4 *__
   const int *gl = bl.glist + g0;
   const bool staged = total < bl.lds_cap;  // uniform across the workgroup
   if (threadIdx.x == 0) slice_next = W;
-  if (staged) stage_fp64<T>(gl, total, src, lds64);
+  if (staged) stage_fp64<T>(gl, total, src, lds64, bl.pn, bl.ghost);
   __syncthreads();
   const int nsl = (nrow + kWave - 1) / kWave;
   const int wv = threadIdx.x / kWave, lane = threadIdx.x % kWave;
@@ -325,8 +328,8 @@ __global__ __launch_bounds__(kWave *W) void k_force_brick_fp64(const double4 *__
     const RowExtra x = row_extra(i, epilogue, id, nd);
     const int len = bl.slen[s];
     const uint4 *rg = reinterpret_cast<const uint4 *>(bl.nbr + bl.sbase[s] + lane * kGroup);
-    const double3 a = staged ? row_kick_fp64<true, U>(rg, len, qi, lds64, src, gl, total, fc)
-                             : row_kick_fp64<false, U>(rg, len, qi, lds64, src, gl, total, fc);
+    const double3 a = staged ? row_kick_fp64<true, U>(rg, len, qi, lds64, src, gl, total, fc, bl.pn, bl.ghost)
+                             : row_kick_fp64<false, U>(rg, len, qi, lds64, src, gl, total, fc, bl.pn, bl.ghost);
     if (ok) {
       p.x += a.x;
       p.y += a.y;
@@ -457,7 +460,7 @@ __global__ __launch_bounds__(kWave *W) void k_prune_brick(const double4 *__restr
   const int total = (int)(bl.gbase[b + 1] - g0);
   const int *gl = bl.glist + g0;
   const bool staged = total < bl.lds_cap;
-  if (staged) stage_fp64<T>(gl, total, src, lds64);
+  if (staged) stage_fp64<T>(gl, total, src, lds64, bl.pn, bl.ghost);
   __syncthreads();
   const uint32_t pad16 = (uint32_t)total | ((uint32_t)total << 16);
   const uint32_t pad8 = kZeroOffset * 0x01010101u;
@@ -489,7 +492,7 @@ __global__ __launch_bounds__(kWave *W) void k_prune_brick(const double4 *__restr
           qy = lds64[3 * e[u] + 1];
           qz = lds64[3 * e[u] + 2];
         } else if (e[u] < total) {
-          const double4 q = src[gl[e[u]]];
+          const double4 q = gather_pos(src, bl.ghost, bl.pn, gl[e[u]]);
           qx = q.x;
           qy = q.y;
           qz = q.z;
@@ -565,7 +568,7 @@ __global__ __launch_bounds__(kWave *W) void k_observe_brick(const double4 *__res
   const int total = (int)(bl.gbase[b + 1] - g0);
   const int *gl = bl.glist + g0;
   const bool staged = total < bl.lds_cap;
-  if (staged) stage_fp64<T>(gl, total, pos, lds64);
+  if (staged) stage_fp64<T>(gl, total, pos, lds64, 0, pos);
   __syncthreads();
   const int nsl = (nrow + kWave - 1) / kWave;
   const int wv = threadIdx.x / kWave, lane = threadIdx.x % kWave;

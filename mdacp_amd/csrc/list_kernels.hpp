@@ -198,11 +198,21 @@ __device__ __forceinline__ uint8_t offset_code(int ox, int oy, int oz) {
 
 constexpr int kStageBatch = 4;  // staged entries per thread with loads in flight together
 
+// Position of gather-list entry j: an owned particle (j < pn) from src, a
+// ghost image from `ghost`.  The pair kernels of a fused step read the owned
+// particles' half drift from one array and the images, which the halo refresh
+// writes once, from the current positions.
+__device__ __forceinline__ double4 gather_pos(const double4 *__restrict__ src, const double4 *__restrict__ ghost,
+                                              int pn, int j) {
+  return (j < pn ? src : ghost)[j];
+}
+
 // Stage positions (3 doubles per entry) of a brick's gather list into LDS,
-// plus a far sentinel at index `total` (the pad entry of the list).
+// plus a far sentinel at index `total` (the pad entry of the list); entries
+// from pn on come from `ghost`.
 template <int T>
 __device__ __forceinline__ void stage_fp64(const int *__restrict__ gl, int total, const double4 *__restrict__ src,
-                                           double *__restrict__ lds) {
+                                           double *__restrict__ lds, int pn, const double4 *__restrict__ ghost) {
   for (int u0 = threadIdx.x; u0 < total; u0 += T * kStageBatch) {
     int j[kStageBatch];
 #pragma unroll
@@ -212,7 +222,7 @@ __device__ __forceinline__ void stage_fp64(const int *__restrict__ gl, int total
     }
     double4 q[kStageBatch];
 #pragma unroll
-    for (int k = 0; k < kStageBatch; k++) q[k] = src[j[k]];
+    for (int k = 0; k < kStageBatch; k++) q[k] = gather_pos(src, ghost, pn, j[k]);
 #pragma unroll
     for (int k = 0; k < kStageBatch; k++) {
       const int u = u0 + k * T;
@@ -295,7 +305,7 @@ __global__ __launch_bounds__(kWave *W) void k_search_brick(const double4 *__rest
   const int total = tb[R];
   const int *gl = st.glist + st.gbase[b];
   const bool staged = total < st.lds_cap;
-  if (staged) stage_fp64<T>(gl, total, pos, lds64);
+  if (staged) stage_fp64<T>(gl, total, pos, lds64, 0, pos);
   __syncthreads();
   const Replica &Rp = B.reps[B.brep[b]];
   const BrickBox X = brick_box(b, Rp);
@@ -444,7 +454,7 @@ __global__ __launch_bounds__(kWave *W) void k_search_cells(const double4 *__rest
   const int total = tb[R];
   const int *gl = st.glist + st.gbase[b];
   const bool staged = total < st.lds_cap;
-  if (staged) stage_fp64<T>(gl, total, pos, lds64);
+  if (staged) stage_fp64<T>(gl, total, pos, lds64, 0, pos);
   __syncthreads();
   const Replica &Rp = B.reps[B.brep[b]];
   const BrickBox X = brick_box(b, Rp);

@@ -1070,6 +1070,8 @@ BrickList brick_list(const mdgpu_domain *d, int bytes_per_entry, bool inner = fa
   bl.vslice0 = d->vslice0;
   bl.brep = d->brep;
   bl.reps = d->reps;
+  bl.ghost = d->pos;  // the halo refresh writes the images there only
+  bl.pn = d->pn;
   bl.lds_cap = lds_cap_for(d, bytes_per_entry, force_lds_kb(d));
   return bl;
 }

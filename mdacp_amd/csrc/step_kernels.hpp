@@ -88,11 +88,11 @@ __global__ void k_to_rel(const double4 *__restrict__ pos, const int *__restrict_
 
 // Fused first half drift + halo refresh:
 //   owned:  tmp[i] = q_i + p_i*dt2                    (and its cell-relative FP32 copy)
-//   ghost:  pos[g] = tmp[g] = (q_src + p_src*dt2) + shift*L   (and its FP32 copy)
+//   ghost:  pos[g] = (q_src + p_src*dt2) + shift*L   (and its FP32 copy)
 // (the same two roundings as UpdatePositionHalf then SendBorderParticles).
-// `tmp` then holds every position the force kernel reads; `pos` keeps the
-// owned positions until the force kernel writes the drifted ones, and the
-// ghosts until the next refresh.
+// The force kernel reads the owned particles from `tmp` and the ghosts from
+// `pos` (gather_pos); `pos` keeps the owned positions until the force kernel
+// writes the drifted ones, and the ghosts until the next refresh.
 template <bool kRel>
 __global__ void k_drift_refresh(double4 *__restrict__ pos, const double4 *__restrict__ mom,
                                 double4 *__restrict__ tmp, const int *__restrict__ gsrc,
@@ -122,17 +122,16 @@ __global__ void k_drift_refresh(double4 *__restrict__ pos, const double4 *__rest
     const double3 y = image_of(q, ghost_shift(w), R.L);
     const double4 g = make_double4(y.x, y.y, y.z, 0.0);
     pos[i] = g;
-    tmp[i] = g;
     if (kRel) rel[i] = to_rel(g, cell[i], org, R);
   }
 }
 
 // Halo refresh when `tmp` already holds the half-drifted owned positions
 // (written by the previous fused step's epilogue): ghosts only,
-// pos[g] = tmp[g] = tmp[src] + shift*L (and the FP32 copy) -- the values
+// pos[g] = tmp[src] + shift*L (and the FP32 copy) -- the values
 // k_drift_refresh would write.
 template <bool kRel>
-__global__ void k_refresh_from_drifted(double4 *__restrict__ pos, double4 *__restrict__ tmp,
+__global__ void k_refresh_from_drifted(double4 *__restrict__ pos, const double4 *__restrict__ tmp,
                                        const int *__restrict__ gsrc, const int *__restrict__ gcode, int pn,
                                        int ng, const Replica *__restrict__ reps, const int *__restrict__ cell,
                                        const double *__restrict__ org, float4 *__restrict__ rel,
@@ -146,7 +145,6 @@ __global__ void k_refresh_from_drifted(double4 *__restrict__ pos, double4 *__res
   const double3 y = image_of(tmp[gsrc[k]], ghost_shift(w), R.L);
   const double4 g = make_double4(y.x, y.y, y.z, 0.0);
   pos[i] = g;
-  tmp[i] = g;
   if (kRel) rel[i] = to_rel(g, cell[i], org, R);
 }
 
