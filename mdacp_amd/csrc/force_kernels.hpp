@@ -272,9 +272,9 @@ __device__ __forceinline__ double finish_row(int i, double4 p, const double4 qi,
     q.x += p.x * dt2;
     q.y += p.y * dt2;
     q.z += p.z * dt2;
-    pos_out[i] = q;
+    if (pos_out != nullptr) pos_out[i] = q;
   } else if (epilogue == kKickLangevin) {
-    pos_out[i] = qi;  // one half drift per Langevin step (mdmanager.cc:186-192)
+    if (pos_out != nullptr) pos_out[i] = qi;  // one half drift per Langevin step (mdmanager.cc:186-192)
   }
   if (nd.pos != nullptr && epilogue != kKickOnly) {
     q.x += p.x * dt2;

@@ -1275,7 +1275,9 @@ int enqueue_fused(mdgpu_domain *d, unsigned long long *clock, bool prune = false
   if (prune) CHK(launch_prune(d));
   const int epi = d->thermostat == MDGPU_LANGEVIN ? kKickLangevin : kKickDrift;
   const NextDrift nd{d->pos_next, mixed ? d->rel_next : nullptr, d->cell, d->org};
-  return launch_force(d, d->pos_tmp, d->rel_tmp, d->pos, epi, true, nd, clock, d->prune_delta > 0.0, gate);
+  // a run's steps leave the final owned positions out (run_gated writes them after the run)
+  return launch_force(d, d->pos_tmp, d->rel_tmp, gated ? nullptr : d->pos, epi, true, nd, clock,
+                      d->prune_delta > 0.0, gate);
 }
 
 void drop_graph(StepGraph &g) {
@@ -1492,6 +1494,13 @@ int run_gated(mdgpu_domain *d, int k, int *done) {
   for (int r = 0; r < R; r++) {
     budget_of(d, r) = d->h_gbudget[r];
     lifetime_of(d, r) = d->h_glife[r];
+  }
+  if (n > 0 && d->pn > 0) {
+    // the final owned positions of the last completed step, whose first half
+    // drift is now in pos_next (the next step's is in pos_tmp)
+    hipLaunchKernelGGL(k_final_positions, dim3(nblk(d->pn)), dim3(kBlock), 0, st, d->pos, d->pos_next, d->mom,
+                       d->pn, d->dt * 0.5, d->thermostat == MDGPU_LANGEVIN ? 0 : 1);
+    CHK(check_launch(d, "final positions"));
   }
   for (int j = 0; j < n; j++) {
     d->sim_time += d->dt;

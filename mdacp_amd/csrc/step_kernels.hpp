@@ -126,6 +126,25 @@ __global__ void k_drift_refresh(double4 *__restrict__ pos, const double4 *__rest
   }
 }
 
+// The owned particles' positions at the end of a step, from that step's first
+// half drift qh and final momenta: qh + p dt/2 (drift) or qh (Langevin), the
+// operations and roundings of the kick's epilogue (finish_row).  A run's steps
+// leave them out, since no step of the run reads them; the host writes them
+// once after the run, from the last completed step.
+__global__ void k_final_positions(double4 *__restrict__ pos, const double4 *__restrict__ qh,
+                                  const double4 *__restrict__ mom, int pn, double dt2, int drift) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= pn) return;
+  double4 q = qh[i];
+  if (drift) {
+    const double4 p = mom[i];
+    q.x += p.x * dt2;
+    q.y += p.y * dt2;
+    q.z += p.z * dt2;
+  }
+  pos[i] = q;
+}
+
 // Halo refresh when `tmp` already holds the half-drifted owned positions
 // (written by the previous fused step's epilogue): ghosts only,
 // pos[g] = tmp[src] + shift*L (and the FP32 copy) -- the values
