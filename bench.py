@@ -42,6 +42,7 @@ HBM_PEAK_GBS = 8000.0
 # FETCH_SIZE and WRITE_SIZE counters (rocprofv3 --pmc) scaled by a copy of known
 # size; written from the profile summaries.
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_force_brick_fp64_4M_traffic.json")
+ENS_TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_force_brick_fp64_ensemble256_traffic.json")
 FP64_VECTOR_PEAK_TFLOPS = 78.6
 FP32_VECTOR_PEAK_TFLOPS = 157.3
 
@@ -213,6 +214,15 @@ def cpu_baseline_ensemble(table, built, args, sample=16, steps=3):
     return n, t_steps / steps, t_build
 
 
+def ensemble_traffic(n, nrep):
+    """Measured L2-to-memory bytes per force launch for the committed 256-replica table, else None."""
+    if not os.path.exists(ENS_TRAFFIC_FILE):
+        return None
+    with open(ENS_TRAFFIC_FILE) as f:
+        t = json.load(f)
+    return t["hbm_bytes_per_launch"] if (t.get("particles"), t.get("replicas")) == (n, nrep) else None
+
+
 def main_ensemble(args):
     from mdacp_amd import mdgpu
     from mdacp_amd.ensemble import build, replica_table
@@ -255,7 +265,7 @@ def main_ensemble(args):
                                 f"dt={args.dt} gamma=1, one pair-force launch per step (graph replay)"),
                    "particles": n, "replicas": len(table), "rebuilds_in_timed_steps": r64["rebuilds"]},
         "roofline": dict(bound="hbm", achieved=achieved, peak=HBM_PEAK_GBS, unit="GB/s",
-                         frac=achieved / HBM_PEAK_GBS, traffic=None, kernel="k_force_brick_fp64",
+                         frac=achieved / HBM_PEAK_GBS, traffic=ensemble_traffic(n, len(table)), kernel="k_force_brick_fp64",
                          kernel_avg_ms=r64["force_avg_ms"], bytes_per_particle_update=bytes_pu,
                          note="algorithmic bytes at the replicas' mean density; kernel time from HIP events over "
                               "a second pass of the same number of steps"),
