@@ -43,6 +43,7 @@ HBM_PEAK_GBS = 8000.0
 # size; written from the profile summaries.
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_force_brick_fp64_4M_traffic.json")
 ENS_TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_force_brick_fp64_ensemble256_traffic.json")
+QUENCH_TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_force_brick_fp64_quench16M_traffic.json")
 FP64_VECTOR_PEAK_TFLOPS = 78.6
 FP32_VECTOR_PEAK_TFLOPS = 157.3
 
@@ -214,13 +215,18 @@ def cpu_baseline_ensemble(table, built, args, sample=16, steps=3):
     return n, t_steps / steps, t_build
 
 
-def ensemble_traffic(n, nrep):
-    """Measured L2-to-memory bytes per force launch for the committed 256-replica table, else None."""
-    if not os.path.exists(ENS_TRAFFIC_FILE):
+def measured_traffic(path, **match):
+    """Measured L2-to-memory bytes per force launch from a committed profile, if it was taken
+    on this workload (its recorded particle and replica counts match), else None."""
+    if not os.path.exists(path):
         return None
-    with open(ENS_TRAFFIC_FILE) as f:
+    with open(path) as f:
         t = json.load(f)
-    return t["hbm_bytes_per_launch"] if (t.get("particles"), t.get("replicas")) == (n, nrep) else None
+    return t["hbm_bytes_per_launch"] if all(t.get(k) == v for k, v in match.items()) else None
+
+
+def ensemble_traffic(n, nrep):
+    return measured_traffic(ENS_TRAFFIC_FILE, particles=n, replicas=nrep)
 
 
 def main_ensemble(args):
@@ -354,7 +360,8 @@ def main_quench(args):
                                 f"margin={args.margin}"), "particles": n,
                    "rebuilds_in_timed_steps": out["fp64"]["rebuilds"]},
         "roofline": dict(bound="hbm", achieved=achieved, peak=HBM_PEAK_GBS, unit="GB/s", frac=achieved / HBM_PEAK_GBS,
-                         traffic=None, kernel="k_force_brick_fp64", kernel_avg_ms=out["fp64"]["force_avg_ms"],
+                         traffic=measured_traffic(QUENCH_TRAFFIC_FILE, particles=n), kernel="k_force_brick_fp64",
+                         kernel_avg_ms=out["fp64"]["force_avg_ms"],
                          bytes_per_particle_update=bytes_pu),
         "cpu_baseline": None,
         "mixed": {"value": out["mixed"]["pu_s"], "ms_per_step": out["mixed"]["ms_per_step"],
