@@ -197,8 +197,12 @@ def ref_available():
 
 
 def run_reference(L, x, v, ids, steps, dump_every=1, dt=0.001, thermostat="nve", T=1.0,
-                  gamma=0.1, tau=0.1, seed=1, init_v0=None, timeout=600):
-    """Run the reference's MDManager (compiled unmodified, rc=3.0, margin=0.3)."""
+                  gamma=0.1, tau=0.1, seed=1, init_v0=None, change_scale=None, timeout=600):
+    """Run the reference's MDManager (compiled unmodified, rc=3.0, margin=0.3).
+
+    init_v0: MDManager::SetInitialVelocity(init_v0) after loading (the given
+    momenta are replaced).  change_scale = (alpha, step): MDManager::ChangeScale(alpha)
+    after that step, followed by a record with scaled=True."""
     if not ref_available():
         raise RuntimeError("oracle/_ref/mdacp_ref is not built (run `make -C oracle ref`)")
     Lx, Ly, Lz = (float(L),) * 3 if np.isscalar(L) else (float(v) for v in L)
@@ -214,6 +218,9 @@ def run_reference(L, x, v, ids, steps, dump_every=1, dt=0.001, thermostat="nve",
             lines.append("HeatbathType=" + ("Langevin" if thermostat == "langevin" else "NoseHoover"))
         if init_v0 is not None:
             lines.append(f"InitV0={init_v0!r}")
+        if change_scale is not None:
+            lines.append(f"ChangeScaleAlpha={float(change_scale[0])!r}")
+            lines.append(f"ChangeScaleStep={int(change_scale[1])}")
         with open(spec, "w") as f:
             f.write("\n".join(lines) + "\n")
         pf = os.path.join(td, "particles.bin")
@@ -238,7 +245,7 @@ def parse_records(buf):
     off = 0
     while off < len(buf):
         hdr = np.frombuffer(buf, dtype=np.int64, count=8, offset=off); off += 64
-        step, pn, tn, npairs, rebuilt, ncons = (int(v) for v in hdr[:6])
+        step, pn, tn, npairs, rebuilt, ncons, scaled = (int(v) for v in hdr[:7])
         if pn < 0:
             recs.append(dict(step=step, rebuilt=bool(rebuilt), dump=False))
             continue
@@ -259,7 +266,7 @@ def parse_records(buf):
         gq = take(np.float64, 3 * ng).reshape(ng, 3)
         pairs = take(np.int32, 3 * npairs).reshape(npairs, 3)
         kick = take(np.float64, 3 * pn).reshape(pn, 3)
-        recs.append(dict(step=step, rebuilt=bool(rebuilt), dump=True, pn=pn, tn=tn, ncons=ncons,
+        recs.append(dict(step=step, rebuilt=bool(rebuilt), dump=True, scaled=bool(scaled), pn=pn, tn=tn, ncons=ncons,
                          ke=obs[0], pe=obs[1], vir=obs[2], T=obs[3], P=obs[4], E=obs[5],
                          time=obs[6], q=q, p=p, type=t, gsrc=gsrc, gcode=gcode, gq=gq,
                          pairs=pairs, kick=kick))

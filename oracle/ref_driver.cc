@@ -8,7 +8,10 @@
 // usage: mdacp_ref <spec.cfg> <particles.bin> <out.bin>
 //   spec.cfg holds the reference's own keys (SystemSizeX/Y/Z, TimeStep,
 //   ControlTemperature, HeatbathType, HeatbathGamma, AimedTemperature,
-//   HeatbathTau) plus driver keys read here: Steps, DumpEvery, InitV0, Seed.
+//   HeatbathTau) plus driver keys read here: Steps, DumpEvery, InitV0, Seed,
+//   ChangeScaleAlpha and ChangeScaleStep (MDManager::ChangeScale(alpha) called
+//   after that step, mdmanager.cc:432-443, followed by a record flagged in
+//   header[6]).
 //   particles.bin: int32 n, then n*(x,y,z,vx,vy,vz) doubles, then n int32 ids.
 //   out.bin: a sequence of records (see write_record).
 #include <omp.h>
@@ -29,8 +32,8 @@
 namespace {
 
 struct Spec {
-  int steps = 0, dump_every = 1, seed = 1;
-  double init_v0 = -1.0;
+  int steps = 0, dump_every = 1, seed = 1, scale_step = -1;
+  double init_v0 = -1.0, scale_alpha = 1.0;
 };
 
 Spec read_spec(const char *path) {
@@ -45,6 +48,8 @@ Spec read_spec(const char *path) {
     if (key == "DumpEvery") s.dump_every = std::stoi(val);
     if (key == "Seed") s.seed = std::stoi(val);
     if (key == "InitV0") s.init_v0 = std::stod(val);
+    if (key == "ChangeScaleAlpha") s.scale_alpha = std::stod(val);
+    if (key == "ChangeScaleStep") s.scale_step = std::stoi(val);
   }
   return s;
 }
@@ -130,11 +135,12 @@ void put(std::ofstream &o, const T *p, size_t n) {
   o.write(reinterpret_cast<const char *>(p), sizeof(T) * n);
 }
 
-// record: int64 header[8] = {step, pn, tn, npairs, rebuilt, nconstructions, 0, 0}
+// record: int64 header[8] = {step, pn, tn, npairs, rebuilt, nconstructions, scaled, 0}
+//         (scaled = 1: the record written right after MDManager::ChangeScale)
 //         double obs[8] = {ke_raw, pe_raw, vir_raw, T, P, E, sim_time, 0}
 //         q[pn*3], p[pn*3], type[pn] (int32), ghost_src[tn-pn], ghost_code[tn-pn],
 //         ghost_q[(tn-pn)*3], pairs[npairs*3] (int32), kick[pn*3]
-void write_record(std::ofstream &o, MDManager *mdm, int step, int rebuilt, double SL) {
+void write_record(std::ofstream &o, MDManager *mdm, int step, int rebuilt, double SL, int scaled = 0) {
   MDUnit *u = mdm->GetMDUnit(0);
   Variables *vars = u->GetVariables();
   MeshGrab grab;
@@ -147,7 +153,7 @@ void write_record(std::ofstream &o, MDManager *mdm, int step, int rebuilt, doubl
   KineticEnergyObserver ke;
   PotentialEnergyObserver pe(nullptr);
   VirialObserver vir;
-  int64_t hdr[8] = {step, pn, tn, np, rebuilt, mesh->GetNumberOfConstructions(), 0, 0};
+  int64_t hdr[8] = {step, pn, tn, np, rebuilt, mesh->GetNumberOfConstructions(), scaled, 0};
   double obs[8] = {u->ObserveDouble(&ke), u->ObserveDouble(&pe), u->ObserveDouble(&vir),
                    mdm->Temperature(), mdm->Pressure(), mdm->TotalEnergy(),
                    mdm->GetSimulationTime(), 0.0};
@@ -231,6 +237,10 @@ int main(int argc, char **argv) {
   write_record(out, mdm, 0, 1, SL);
   MeshGrab grab;
   for (int step = 1; step <= spec.steps; step++) {
+    if (step - 1 == spec.scale_step) {
+      mdm->ChangeScale(spec.scale_alpha);
+      write_record(out, mdm, step - 1, 1, SL, 1);
+    }
     mdm->GetMDUnit(0)->ObserveDouble(&grab);
     const int before = grab.mesh->GetNumberOfConstructions();
     mdm->Calculate();

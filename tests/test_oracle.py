@@ -18,39 +18,56 @@ from oracle import oracle as O
 
 
 def run_oracle_case(g, steps=None):
+    """Replays a reference fixture on the oracle and compares every dump bitwise.
+    Fixtures with ref_init_v0 take their momenta from the oracle's SetInitialVelocity
+    (variables.cc:82-108); with change_scale = (alpha, step) the oracle's ChangeScale
+    (mdmanager.cc:432-443) runs after that step and is compared with the record the
+    reference wrote right after its own."""
     steps = g["steps"] if steps is None else steps
     x = fcc(g["L"], g["density"])
     n = len(x)
-    p = initial_velocity(n, g["v0"], 0)
     ids = np.arange(n, dtype=np.int32)
     if g["reverse_ids"]:
         ids = ids[::-1].copy()
     s = O.OracleSystem(g["L"], g["dt"], g["rc"], g["margin"])
-    s.add(x, p, ids)
+    if g.get("ref_init_v0"):
+        s.add(x, None, ids)
+        s.set_initial_velocity(g["v0"], 0)
+    else:
+        s.add(x, initial_velocity(n, g["v0"], 0), ids)
     kind = {"nve": O.NVE, "langevin": O.LANGEVIN, "nosehoover": O.NOSEHOOVER}[g["thermostat"]]
     s.set_thermostat(kind, g["T"], g["gamma"], g["tau"])
     s.set_seed(g["seed"])
     s.make_pair_list()
-    dumps = {d["step"]: d for d in g["dumps"]}
+    dumps = {(d["step"], d.get("scaled", False)): d for d in g["dumps"]}
+    scale = g.get("change_scale")
     rebuilt = [0]
     checked = 0
+
+    def compare(d, step):
+        q, pp, t = s.state()
+        ob = s.observables()
+        assert pair_hash(s.pairs()) == (d["pair_sha256"], d["pair_count"]), step
+        assert state_hash_by_id(q, pp, t) == d["state_sha256"], step
+        for k in ("ke", "pe", "vir", "T", "P", "E", "time"):
+            assert ob[k] == d[k], (step, k)
+
     for step in range(0, steps + 1):
         if step > 0 and s.step():
             rebuilt.append(step)
-        if step in dumps:
-            d = dumps[step]
-            q, pp, t = s.state()
-            ob = s.observables()
-            assert pair_hash(s.pairs()) == (d["pair_sha256"], d["pair_count"]), step
-            assert state_hash_by_id(q, pp, t) == d["state_sha256"], step
-            for k in ("ke", "pe", "vir", "T", "P", "E", "time"):
-                assert ob[k] == d[k], (step, k)
+        if (step, False) in dumps:
+            compare(dumps[(step, False)], step)
             checked += 1
+        if scale is not None and step == scale[1]:
+            s.change_scale(scale[0])
+            if (step, True) in dumps:
+                compare(dumps[(step, True)], step)
+                checked += 1
     assert rebuilt == [r for r in g["rebuild_steps"] if r <= steps]
     return checked
 
 
-@pytest.mark.parametrize("name", ["fcc2048_nve", "fcc2048_langevin", "fcc2048_nosehoover"])
+@pytest.mark.parametrize("name", ["fcc2048_nve", "fcc2048_langevin", "fcc2048_nosehoover", "fcc2048_quench"])
 def test_oracle_matches_reference_fixture_bitwise(name):
     g = load_golden(name)
     assert run_oracle_case(g) == len(g["dumps"])
@@ -304,3 +321,85 @@ def test_langevin_stationary_variance_closed_form():
             acc.append(s.state()[1].ravel() ** 2)
     m = np.mean(acc)
     assert m == pytest.approx(1.3 / (1 - 1.0 * 0.005 / 2), rel=0.05)
+
+
+# ------------------------------------------------- ChangeScale, SetInitialVelocity
+def test_quench_fixture_covers_initial_velocity_and_change_scale():
+    """The fixture that pins ora_set_initial_velocity and ora_change_scale holds what
+    it is meant to: the reference's own SetInitialVelocity momenta at step 0, one
+    ChangeScale record, and rebuilds after it."""
+    g = load_golden("fcc2048_quench")
+    assert g["ref_init_v0"] and g["change_scale"] is not None
+    alpha, at = g["change_scale"]
+    scaled = [d for d in g["dumps"] if d["scaled"]]
+    assert len(scaled) == 1 and scaled[0]["step"] == at
+    assert any(r > at for r in g["rebuild_steps"])
+    # the host's initial momenta (mdacp_amd.lattice, used by the GPU tests and the
+    # bench) equal the reference's SetInitialVelocity bit for bit
+    x = fcc(g["L"], g["density"])
+    n = len(x)
+    ids = np.arange(n, dtype=np.int32)[::-1].copy()
+    d0 = [d for d in g["dumps"] if d["step"] == 0][0]
+    assert state_hash_by_id(x, initial_velocity(n, g["v0"], 0), ids) == d0["state_sha256"]
+
+
+def test_initial_velocity_mathematics():
+    """Every particle gets speed V0 in a random direction and then the mean is
+    removed: so there is one vector c (the removed mean) with |p_i + c| = V0 for all
+    i, found here by least squares from 2 p_i.c + |c|^2 = V0^2 - |p_i|^2; the result
+    has zero mean, and the directions are uniform on the sphere (<z> = 0, <z^2> = 1/3)."""
+    n, v0 = 20000, 1.7
+    s = O.OracleSystem(40.0, 0.001, 2.5, 0.5)
+    s.add(np.random.default_rng(0).uniform(0, 40.0, (n, 3)))
+    s.set_initial_velocity(v0, 3)
+    p = s.state()[1]
+    assert np.abs(p.mean(axis=0)).max() < 1e-15
+    A = np.concatenate([2.0 * p, np.ones((n, 1))], axis=1)
+    b = v0 * v0 - np.einsum("ij,ij->i", p, p)
+    sol = np.linalg.lstsq(A, b, rcond=None)[0]
+    c = sol[:3]
+    assert sol[3] == pytest.approx(c @ c, rel=1e-6, abs=1e-12)
+    speed = np.linalg.norm(p + c, axis=1)
+    assert np.abs(speed - v0).max() < 1e-12
+    u = (p + c) / v0
+    assert abs(np.mean(u[:, 2])) < 0.02 and abs(np.mean(u[:, 2] ** 2) - 1.0 / 3.0) < 0.02
+    # a plausible slip -- a direction drawn from z = u instead of z = 2u - 1 -- would
+    # put every z >= 0: the test above rejects it; so does the reference fixture
+
+
+def test_change_scale_mathematics():
+    """ora_change_scale: positions become fl(q * alpha) bit for bit (by id), momenta do
+    not change, the box becomes alpha L, the pair set equals brute force at the new
+    geometry, and a perfect FCC crystal at spacing a becomes one at spacing a alpha
+    with the closed-form lattice sums."""
+    rng = np.random.default_rng(11)
+    L, alpha = 9.3, 1.0772173450159417
+    n = 300
+    x = rng.uniform(0, L, (n, 3))
+    ids = rng.permutation(n).astype(np.int32)
+    p = rng.normal(0, 1, (n, 3))
+    s = O.OracleSystem(L, 0.001, 2.5, 0.5)
+    s.add(x, p, ids)
+    s.make_pair_list()
+    q0, p0, t0 = s.state()
+    s.change_scale(alpha)
+    q1, p1, t1 = s.state()
+    o0, o1 = np.argsort(t0), np.argsort(t1)
+    assert np.array_equal(q1[o1], q0[o0] * alpha)
+    assert np.array_equal(p1[o1], p0[o0])
+    assert np.array_equal(s.L, np.full(3, L * alpha))
+    bf = brute_force_pairs(q0[o0] * alpha, np.sort(t0), L * alpha, 3.0)
+    assert np.array_equal(O.canonical_pairs(s.pairs()), O.canonical_pairs(bf))
+    # rescaled perfect crystal: rho 0.5 (a = 2) to a = 2 alpha
+    Lf = 16.0
+    xf = fcc(Lf, 0.5)
+    nf = len(xf)
+    for rc, margin in [(2.5, 0.5), (3.0, 0.3)]:
+        f = O.OracleSystem(Lf, 0.001, rc, margin)
+        f.add(xf)
+        f.change_scale(alpha)
+        pe_sum, vir_sum, _ = fcc_lattice_sums(2.0 * alpha, rc, rc + margin)
+        ob = f.observables()
+        assert ob["pe"] / nf == pytest.approx(pe_sum, rel=1e-12)
+        assert ob["vir"] / nf == pytest.approx(vir_sum, rel=1e-12)
+        assert np.max(np.abs(f.kick())) < 1e-12
