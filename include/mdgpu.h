@@ -154,6 +154,15 @@ int mdgpu_calculate(mdgpu_domain *d, int nsteps, int *rebuilds);
 int mdgpu_observe(mdgpu_domain *d, double *ke, double *pe, double *virial);
 /* KineticEnergyObserver alone (observer.cc:5-15); needs no pair list */
 int mdgpu_observe_kinetic(mdgpu_domain *d, double *ke);
+/* mdgpu_observe without any device allocation, for callers that cannot
+   allocate (code under HIP graph capture, tight serving loops, harnesses that
+   forbid hipMalloc).  No counterpart in the reference.  Its per-brick partial
+   sums live in a fixed device array of the library (3 MiB per GPU, held until
+   the process ends) instead of the domain's growable buffer: a domain of more
+   than 131072 bricks gets MDGPU_ERR_CAPACITY (use mdgpu_observe).  Every call
+   on one GPU shares that array, so calls that overlap on one GPU race.  Same
+   kernels and summation order as mdgpu_observe: the same bits. */
+int mdgpu_observe_noalloc(mdgpu_domain *d, double *ke, double *pe, double *virial);
 /* the three observers of every replica (arrays of mdgpu_replica_number entries) */
 int mdgpu_observe_replicas(mdgpu_domain *d, double *ke, double *pe, double *virial);
 

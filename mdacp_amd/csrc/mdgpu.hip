@@ -11,6 +11,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -22,6 +23,18 @@
 #include "step_kernels.hpp"
 
 using namespace mdgpu;
+
+// Device scratch of mdgpu_observe_noalloc: the per-brick partial sums of the
+// observer kernel (3 doubles per brick, up to kNoAllocBricks bricks) and the
+// reduced totals after them (3 doubles).  Each GPU holds its own copy, at its
+// own address, from the library's load until the process ends.  Host code
+// never uses this name as an address: create_domain resolves the device copy
+// once per device with hipGetSymbolAddress and every launch takes that pointer.
+// Every call on one GPU shares the array, so calls that overlap on one GPU
+// (several domains or threads at once) race on it.
+constexpr int kNoAllocBricks = 131072;
+constexpr size_t kScratchDoubles = 3 * (size_t)kNoAllocBricks + 4;
+static __device__ __align__(256) double scratch[kScratchDoubles];
 
 namespace {
 // Parameters and PairList state of one replica (pairlist.h:10-31).
@@ -184,6 +197,7 @@ struct mdgpu_domain {
   long long *h_i64 = nullptr;
   int *h_i32 = nullptr;
   double *h_d3 = nullptr;
+  double *scratch_dev = nullptr;  // this device's copy of `scratch` (mdgpu_observe_noalloc)
 
   // timing
   bool timing = false;
@@ -1511,16 +1525,65 @@ int run_gated(mdgpu_domain *d, int k, int *done) {
 }
 
 template <int W>
-void launch_observe(mdgpu_domain *d, const BrickList &bl, size_t shm) {
+void launch_observe_w(mdgpu_domain *d, const BrickList &bl, size_t shm, double *partial) {
   allow_lds(k_observe_brick<W>, shm);
   hipLaunchKernelGGL(k_observe_brick<W>, dim3(d->nbrick), dim3(kWave * W), shm, d->stream, d->pos,
-                     d->mom, bl, d->CL2, d->C2, d->C0, d->partial);
+                     d->mom, bl, d->CL2, d->C2, d->C0, partial);
+}
+
+// The observer kernel over every brick: KE, PE and virial per brick into
+// partial[0, 3 nbrick).
+void launch_observe(mdgpu_domain *d, double *partial) {
+  const BrickList bl = brick_list(d, 3 * sizeof(double));
+  const size_t shm = (size_t)bl.lds_cap * 3 * sizeof(double);
+  switch (d->waves) {
+    case 2: launch_observe_w<2>(d, bl, shm, partial); break;
+    case 8: launch_observe_w<8>(d, bl, shm, partial); break;
+    case 16: launch_observe_w<16>(d, bl, shm, partial); break;
+    default: launch_observe_w<4>(d, bl, shm, partial); break;
+  }
+}
+
+// KE, PE and virial sums of a single domain: per-brick partial sums into
+// partial[0, 3 nbrick), reduced in fixed order into out_dev[0, 3), copied to
+// the host.  Allocates nothing.
+int observe_into(mdgpu_domain *d, double *partial, double *out_dev, double out[3]) {
+  out[0] = out[1] = out[2] = 0.0;
+  if (d->pn == 0) return MDGPU_OK;
+  const int nb = d->nbrick;
+  launch_observe(d, partial);
+  hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(kBlock), 0, d->stream, partial, nb, 3, out_dev);
+  CHK(check_launch(d, "observe"));
+  HIPCHK(d, hipMemcpyAsync(d->h_d3, out_dev, 3 * sizeof(double), hipMemcpyDeviceToHost, d->stream));
+  CHK(sync(d));
+  std::memcpy(out, d->h_d3, 3 * sizeof(double));
+  return MDGPU_OK;
 }
 
 }  // namespace
 
 // ======================================================================= C ABI
 namespace {
+
+// The address of `scratch` on the current device (`device`), looked up once per
+// device and kept: every GPU holds its own copy of the array.
+constexpr int kMaxScratchDevices = 64;
+std::mutex g_scratch_mu;
+double *g_scratch_dev[kMaxScratchDevices] = {};
+
+hipError_t scratch_address(int device, double **out) {
+  *out = nullptr;
+  if (device < 0 || device >= kMaxScratchDevices) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_dev[device]) {
+    void *ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    g_scratch_dev[device] = static_cast<double *>(ptr);
+  }
+  *out = g_scratch_dev[device];
+  return hipSuccess;
+}
 
 // A domain of nrep replicas with the common parameters of prm (dt, rc,
 // margin, check list length, precision, device); rl[r] gives replica r's box
@@ -1624,6 +1687,15 @@ int create_domain(const mdgpu_params *prm, int nrep, const mdgpu_replica *rl, co
   (void)hipMemset(d->i32, 0, 4 * sizeof(int));
   (void)hipMemset(d->u64, 0, 4 * sizeof(unsigned long long));
   (void)hipMemset(d->clock, 0, sizeof(unsigned long long));
+  {
+    // set-up of mdgpu_observe_noalloc on this device (the device is current)
+    const hipError_t e = scratch_address(prm->device, &d->scratch_dev);
+    if (e != hipSuccess) {
+      // the domain works without it; mdgpu_observe_noalloc reports the failure
+      d->scratch_dev = nullptr;
+      d->err = std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e);
+    }
+  }
   *out = d;
   return MDGPU_OK;
 }
@@ -2049,24 +2121,37 @@ int mdgpu_observe(mdgpu_domain *d, double *ke, double *pe, double *virial) {
     d->err = "pair list not built";
     return MDGPU_ERR_ARGUMENT;
   }
-  const int nb = d->nbrick;
-  CHK(ensure_partial(d, std::max(1, nb)));
-  double out[3] = {0.0, 0.0, 0.0};
-  if (d->pn > 0) {
-    const BrickList bl = brick_list(d, 3 * sizeof(double));
-    const size_t shm = (size_t)bl.lds_cap * 3 * sizeof(double);
-    switch (d->waves) {
-      case 2: launch_observe<2>(d, bl, shm); break;
-      case 8: launch_observe<8>(d, bl, shm); break;
-      case 16: launch_observe<16>(d, bl, shm); break;
-      default: launch_observe<4>(d, bl, shm); break;
-    }
-    hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(kBlock), 0, d->stream, d->partial, nb, 3, d->d3);
-    CHK(check_launch(d, "observe"));
-    HIPCHK(d, hipMemcpyAsync(d->h_d3, d->d3, 3 * sizeof(double), hipMemcpyDeviceToHost, d->stream));
-    CHK(sync(d));
-    std::memcpy(out, d->h_d3, sizeof(out));
+  CHK(ensure_partial(d, std::max(1, d->nbrick)));
+  double out[3];
+  CHK(observe_into(d, d->partial, d->d3, out));
+  if (ke) *ke = out[0];
+  if (pe) *pe = out[1];
+  if (virial) *virial = out[2] / 3.0;
+  return MDGPU_OK;
+}
+
+int mdgpu_observe_noalloc(mdgpu_domain *d, double *ke, double *pe, double *virial) {
+  if (!d) return MDGPU_ERR_ARGUMENT;
+  if (d->nrep > 1) {
+    d->err = "observe the replicas of an ensemble with mdgpu_observe_replicas";
+    return MDGPU_ERR_ARGUMENT;
   }
+  if (!d->initialized) {
+    d->err = "pair list not built";
+    return MDGPU_ERR_ARGUMENT;
+  }
+  if (!d->scratch_dev) {
+    d->err = "mdgpu_observe_noalloc: the device scratch array was not resolved when the domain was created";
+    return MDGPU_ERR_HIP;
+  }
+  if (d->nbrick > kNoAllocBricks) {
+    d->err = "mdgpu_observe_noalloc: " + std::to_string(d->nbrick) + " bricks exceed the scratch limit of " +
+             std::to_string(kNoAllocBricks) + " (use mdgpu_observe)";
+    return MDGPU_ERR_CAPACITY;
+  }
+  // every partial the kernel reads is written by it first: nothing to clear
+  double out[3];
+  CHK(observe_into(d, d->scratch_dev, d->scratch_dev + 3 * (size_t)kNoAllocBricks, out));
   if (ke) *ke = out[0];
   if (pe) *pe = out[1];
   if (virial) *virial = out[2] / 3.0;
@@ -2404,14 +2489,7 @@ int mdgpu_observe_replicas(mdgpu_domain *d, double *ke, double *pe, double *viri
   const int nb = d->nbrick, R = d->nrep;
   CHK(ensure_partial(d, std::max(1, nb)));
   if (d->pn > 0) {
-    const BrickList bl = brick_list(d, 3 * sizeof(double));
-    const size_t shm = (size_t)bl.lds_cap * 3 * sizeof(double);
-    switch (d->waves) {
-      case 2: launch_observe<2>(d, bl, shm); break;
-      case 8: launch_observe<8>(d, bl, shm); break;
-      case 16: launch_observe<16>(d, bl, shm); break;
-      default: launch_observe<4>(d, bl, shm); break;
-    }
+    launch_observe(d, d->partial);
     hipLaunchKernelGGL(k_replica_sum3, dim3(R), dim3(kBlock), 0, d->stream, d->partial, d->reps, d->rep_out);
     CHK(check_launch(d, "observe"));
     HIPCHK(d, hipMemcpyAsync(d->h_rep_out, d->rep_out, 3 * R * sizeof(double), hipMemcpyDeviceToHost, d->stream));

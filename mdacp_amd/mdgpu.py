@@ -28,6 +28,7 @@ EXPORTS = [
     "mdgpu_set_positions", "mdgpu_clear_particles", "mdgpu_set_zeta", "mdgpu_get_pair_indices",
     "mdgpu_observe_kinetic", "mdgpu_create_ensemble", "mdgpu_replica_number", "mdgpu_replica_counts",
     "mdgpu_set_replica_thermostat", "mdgpu_observe_replicas", "mdgpu_get_replica_pairs",
+    "mdgpu_observe_noalloc",
 ]
 OPT_UNROLL, OPT_KAHAN, OPT_WAVES, OPT_LDS_KB, OPT_BRICK, OPT_GRAPH = 1, 2, 3, 4, 5, 6
 OPT_SEARCH_WAVES, OPT_SEARCH_STAGE, OPT_PRUNE, OPT_ROW_SORT, OPT_BRICK_KB, OPT_SEARCH_CELLS = 7, 8, 9, 10, 11, 12
@@ -85,6 +86,7 @@ def load():
             "mdgpu_heatbath_momenta": [P],
             "mdgpu_calculate": [P, I, PI],
             "mdgpu_observe": [P, PD, PD, PD],
+            "mdgpu_observe_noalloc": [P, PD, PD, PD],
             "mdgpu_change_scale": [P, D],
             "mdgpu_get_pairs": [P, PL, P],
             "mdgpu_compute_kick": [P, P],
