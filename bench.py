@@ -55,6 +55,59 @@ def algorithmic_per_update(rho, rc, margin):
     return 96.0 + 4.0 + 4.0 * n_L, 26.0 * n_C + 8.0 * (n_L - n_C) + 15.0, n_L, n_C
 
 
+def thermalize(d, steps):
+    """Steps taken before tuning and timing, as the reference's Benchmark project does
+    (ThermalizeLoop, benchmark.cc:20-27), so that the timed window sees the melted fluid
+    and not the starting lattice.  Returns the constructions per step over the second
+    half of these steps: the Verlet cadence the timed window is amortised at."""
+    first = steps - steps // 2
+    if first:
+        d.calculate(first)
+    rebuilds = d.calculate(steps // 2) if steps // 2 else 0
+    return rebuilds / max(1, steps // 2), rebuilds, steps // 2
+
+
+def construction_ms(d, reps=3, wait=None):
+    """Wall time of one pair-list construction (MDManager::MakePairList on the device, host
+    round trips included): the median of `reps` constructions, each followed by a
+    reduction that waits for the stream (`wait`, default the kinetic energy), less that
+    reduction alone."""
+    wait = wait or d.kinetic_energy
+    wait()
+    ts, ks = [], []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        d.make_pair_list()
+        wait()
+        t1 = time.perf_counter()
+        wait()
+        t2 = time.perf_counter()
+        ts.append(t1 - t0)
+        ks.append(t2 - t1)
+    return 1e3 * (float(np.median(ts)) - float(np.median(ks)))
+
+
+def observation_ms(d, reps=3):
+    """Wall time of one observation (T, P, E: the three observers and their reductions)."""
+    d.observables()
+    ts = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        d.observables()
+        ts.append(time.perf_counter() - t0)
+    return 1e3 * float(np.median(ts))
+
+
+def amortised(n, steps, elapsed_s, rebuilds, n_obs, rate, t_build_ms, t_obs_ms, observe_every=100):
+    """Particle-updates/s of the timed window with constructions and observations at the
+    workload's own cadence: the window's own constructions and observations are replaced
+    by steps * rate constructions and steps / observe_every observations (MUPS,
+    benchmark.cc:45-51, over a run long enough to hold them).  A long window holds about
+    that many already, and the correction vanishes."""
+    t = elapsed_s + 1e-3 * ((steps * rate - rebuilds) * t_build_ms + (steps / observe_every - n_obs) * t_obs_ms)
+    return steps * n / t, 1e3 * t / steps
+
+
 def run_gpu(precision, x, p, L, args, observe_every=100):
     from mdacp_amd import mdgpu
 
@@ -62,23 +115,30 @@ def run_gpu(precision, x, p, L, args, observe_every=100):
     d = mdgpu.Domain(L, dt=args.dt, rc=args.rc, margin=args.margin, precision=precision, capacity=n)
     d.add_particles(x, p)
     d.make_pair_list()
+    rate, therm_rebuilds, therm_steps = thermalize(d, args.thermalize)
     d.calculate(args.warmup)
     tuned = tune(d, args)
     d.calculate(20)
     d.observables()
     # the timed steps: graph-replayed fused steps, queued ahead of the host while the
-    # validity budgets allow (the library's defaults), observed every 100 steps
+    # validity budgets allow (the library's defaults), observed after the first step
+    # and every observe_every steps (benchmark.cc:32-40)
     rebuilds = 0
-    trace = []
+    n_obs = 0
     t0 = time.perf_counter()
     done = 0
     while done < args.steps:
-        k = min(observe_every, args.steps - done)
+        k = 1 if done == 0 else min(observe_every, args.steps - done)
         rebuilds += d.calculate(k)
         done += k
-        trace.append(d.observables())
+        if (done - 1) % observe_every == 0:
+            d.observables()
+            n_obs += 1
     t1 = time.perf_counter()
     elapsed = t1 - t0
+    t_build = construction_ms(d)
+    t_obs = observation_ms(d)
+    pu_s, ms_step = amortised(n, args.steps, elapsed, rebuilds, n_obs, rate, t_build, t_obs, observe_every)
     # host buffers at the boundary: the state read back into and written from host
     # arrays (mdgpu_get_particles, mdgpu_set_positions / mdgpu_set_momenta), as a
     # caller holding the state on the host pays once per timed window
@@ -89,18 +149,21 @@ def run_gpu(precision, x, p, L, args, observe_every=100):
     d.set_momenta(ph)
     t4 = time.perf_counter()
     pcie = dict(download_ms=1e3 * (t3 - t2), upload_ms=1e3 * (t4 - t3), bytes_each_way=int(qh.nbytes + ph.nbytes),
-                pu_s=args.steps * n / (elapsed + t4 - t2))
+                pu_s=args.steps * n / (args.steps * ms_step * 1e-3 + t4 - t2))
     del qh, ph
     # the same number of steps again with HIP events around each pair-force launch and
     # each construction (launched directly, so slower per step than the timed pass)
     d.enable_timing(True)
-    d.calculate(args.steps)
+    d.calculate(max(args.steps, 100))
     tm = d.timing()
     d.enable_timing(False)
-    res = dict(pu_s=args.steps * n / elapsed, ms_per_step=1e3 * elapsed / args.steps,
+    res = dict(pu_s=pu_s, ms_per_step=ms_step, window_pu_s=args.steps * n / elapsed,
+               window_ms_per_step=1e3 * elapsed / args.steps,
                force_avg_ms=tm["force_ms"] / max(1, tm["force_launches"]), force_launches=tm["force_launches"],
-               rebuilds=rebuilds, rebuild_avg_ms=tm["rebuild_ms"] / max(1, tm["rebuilds"]), list=d.list_stats(),
-               tuned=tuned, pcie=pcie)
+               rebuilds=rebuilds, observations=n_obs, construction_ms=t_build, observation_ms=t_obs,
+               rebuilds_per_step=rate, cadence_sample=f"{therm_rebuilds} constructions in the last {therm_steps} "
+                                                      f"of {args.thermalize} thermalisation steps",
+               list=d.list_stats(), tuned=tuned, pcie=pcie)
     return d, res
 
 
@@ -117,20 +180,63 @@ def tune(d, args):
     return r["choice"]
 
 
-def energy_drift(d, steps, every=100):
-    """Slope of E/N (linear fit) scaled to 10^4 steps, and the RMS residual."""
+def energy_drift(d, steps, every=10, blocks=20):
+    """Energy drift of E/N over `steps` steps: E sampled every `every` steps, averaged
+    over `blocks` consecutive blocks; the slope of a least-squares line through the block
+    means, scaled to 10^4 steps, with its standard error from the scatter of the block
+    means about the line (block averages absorb the correlation of nearby samples).  The
+    detection floor is twice the standard error: a slope below it is not distinguishable
+    from zero on this run.  rms_fluct is the RMS of the samples about the line."""
     e, t = [], []
     for k in range(0, steps + 1, every):
         if k:
             d.calculate(every)
-        ob = d.observables()
-        e.append(ob["E"])
+        e.append(d.observables()["E"])
         t.append(k)
     e, t = np.array(e), np.array(t, dtype=float)
-    a, b = np.polyfit(t, e, 1)
-    resid = e - (a * t + b)
-    return dict(slope_per_1e4_steps=float(a * 1e4), rms_fluct=float(np.sqrt(np.mean(resid ** 2))),
-                e_start=float(e[0]), e_end=float(e[-1]), steps=steps)
+    nb = len(e) // blocks
+    eb = e[: nb * blocks].reshape(blocks, nb).mean(axis=1)
+    tb = t[: nb * blocks].reshape(blocks, nb).mean(axis=1)
+    A = np.stack([tb - tb.mean(), np.ones_like(tb)], 1)
+    (a, b), res, _, _ = np.linalg.lstsq(A, eb, rcond=None)
+    resid = eb - A @ np.array([a, b])
+    se = math.sqrt(float(resid @ resid) / (blocks - 2) / float(np.sum((tb - tb.mean()) ** 2)))
+    a_all, b_all = np.polyfit(t, e, 1)
+    fl = e - (a_all * t + b_all)
+    return dict(slope_per_1e4_steps=float(a * 1e4), stderr_per_1e4_steps=float(se * 1e4),
+                floor_per_1e4_steps=float(2.0 * se * 1e4), rms_fluct=float(np.sqrt(np.mean(fl ** 2))),
+                e_start=float(e[0]), e_end=float(e[-1]), steps=steps, sample_every=every, blocks=blocks,
+                estimator="least-squares slope of E/N block means; stderr from block residuals")
+
+
+def drift_comparison(d64, d32):
+    """The north star's drift criterion, |slope(mixed)| <= 2 |slope(FP64)|, judged with the
+    FP64 slope replaced by its detection floor when it is below that floor."""
+    if not d64 or not d32:
+        return None
+    ref = max(abs(d64["slope_per_1e4_steps"]), d64["floor_per_1e4_steps"])
+    return dict(fp64_reference_per_1e4_steps=ref,
+                fp64_below_floor=abs(d64["slope_per_1e4_steps"]) < d64["floor_per_1e4_steps"],
+                mixed_over_fp64=abs(d32["slope_per_1e4_steps"]) / ref,
+                mixed_within_2x=abs(d32["slope_per_1e4_steps"]) <= 2.0 * ref)
+
+
+def reference_cpu():
+    """The reference's own CPU program timed in the build container (oracle/time_reference.py:
+    its Benchmark project, unmodified, rc 3.0 / margin 0.3 as compiled), for the record next
+    to the port timed here."""
+    path = os.path.join(ROOT, "profiles", "reference_cpu_benchmark.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        r = json.load(f)
+    return [dict(value=run["particle_updates_per_s"], unit="particle-updates/s", cores=run["threads"],
+                 kind="reference", host=r["host"], particles=run["particles"],
+                 sample=(f"kaityo256/mdacp Benchmark project compiled unmodified, {run['threads']} OpenMP "
+                         f"thread(s), N={run['particles']} vacancy FCC rho 0.5, rc {r['rc']} margin {r['margin']} "
+                         f"(compiled in), {run['total_loop']} timed steps, MUPS line of benchmark.cc:45-51; "
+                         f"timed in the build container, not on this machine"))
+            for run in r["runs"]]
 
 
 def cpu_baseline(x, p, L, args, gpu_rebuilds_per_step):
@@ -151,7 +257,8 @@ def cpu_baseline(x, p, L, args, gpu_rebuilds_per_step):
                 sample=(f"oracle (C restatement of the reference step, 1 thread) on the same "
                         f"{n}-particle workload: {args.cpu_steps} steps timed ({t_steps:.2f} s/step) "
                         f"plus one pair-list build ({t_build:.2f} s) amortised at the GPU run's "
-                        f"rebuild rate ({gpu_rebuilds_per_step:.4f}/step)"))
+                        f"rebuild rate ({gpu_rebuilds_per_step:.4f}/step)"),
+                reference=reference_cpu())
 
 
 def run_ensemble_gpu(precision, table, built, args):
@@ -161,28 +268,38 @@ def run_ensemble_gpu(precision, table, built, args):
     e = mdgpu.Ensemble(boxes, counts, q, p, ids, T=[r["T"] for r in table], gamma=1.0,
                        seed=[r["seed"] for r in table], thermostat=mdgpu.LANGEVIN, dt=args.dt, rc=args.rc,
                        margin=args.margin, precision=precision)
+    rate, therm_rebuilds, therm_steps = thermalize(e, args.thermalize)
     e.calculate(args.warmup)
     tuned = tune(e, args)
     e.calculate(20)
     e.observables()
     n = int(counts.sum())
     rebuilds = 0
+    n_obs = 0
     t0 = time.perf_counter()
     done = 0
     while done < args.steps:
-        k = min(100, args.steps - done)
+        k = 1 if done == 0 else min(100, args.steps - done)
         rebuilds += e.calculate(k)
         done += k
-        e.observables()
+        if (done - 1) % 100 == 0:
+            e.observables()
+            n_obs += 1
     elapsed = time.perf_counter() - t0
+    t_build = construction_ms(e, wait=e.observe_replicas_raw)
+    t_obs = observation_ms(e)
+    pu_s, ms_step = amortised(n, args.steps, elapsed, rebuilds, n_obs, rate, t_build, t_obs)
     # the same number of steps again with HIP events around each pair-force launch
     e.enable_timing(True)
-    e.calculate(args.steps)
+    e.calculate(max(args.steps, 100))
     tm = e.timing()
     e.enable_timing(False)
-    res = dict(pu_s=args.steps * n / elapsed, ms_per_step=1e3 * elapsed / args.steps, rebuilds=rebuilds,
+    res = dict(pu_s=pu_s, ms_per_step=ms_step, window_pu_s=args.steps * n / elapsed, rebuilds=rebuilds,
+               rebuilds_per_step=rate, construction_ms=t_build, observation_ms=t_obs,
+               cadence_sample=f"{therm_rebuilds} constructions in the last {therm_steps} of {args.thermalize} "
+                              f"thermalisation steps",
                force_avg_ms=tm["force_ms"] / max(1, tm["force_launches"]),
-               rebuild_avg_ms=tm["rebuild_ms"] / max(1, tm["rebuilds"]), list=e.list_stats(),
+               rebuild_avg_ms=t_build, list=e.list_stats(),
                mean_T_over_target=float(np.mean(e.observables()["T"] / np.array([r["T"] for r in table]))),
                tuned=tuned)
     e.close()
@@ -244,7 +361,7 @@ def main_ensemble(args):
     cpu = None
     if not args.no_cpu_baseline:
         ns, per_step, t_build = cpu_baseline_ensemble(table, built, args)
-        per = per_step + t_build * r64["rebuilds"] / args.steps
+        per = per_step + t_build * r64["rebuilds_per_step"]
         cpu = dict(value=ns / per, unit="particle-updates/s", cores=1, kind="port",
                    sample=(f"oracle (C restatement of the reference step, 1 thread), the first 16 replicas "
                            f"({ns} particles) one after another: 3 Langevin steps each ({per_step:.3f} s per "
@@ -269,7 +386,11 @@ def main_ensemble(args):
         "data": "synthetic: per-replica FCC lattices (confmaker.cc:8-44), |v0|=sqrt(3T) (variables.cc:82-108)",
         "config": {"workload": (f"{len(table)} Langevin replicas, {n} particles, rc={args.rc} margin={args.margin} "
                                 f"dt={args.dt} gamma=1, one pair-force launch per step (graph replay)"),
-                   "particles": n, "replicas": len(table), "rebuilds_in_timed_steps": r64["rebuilds"]},
+                   "particles": n, "replicas": len(table), "thermalize": args.thermalize,
+                   "rebuilds_in_timed_steps": r64["rebuilds"]},
+        "amortisation": {"rebuilds_per_step": r64["rebuilds_per_step"], "cadence_sample": r64["cadence_sample"],
+                         "construction_ms": r64["construction_ms"], "observation_ms": r64["observation_ms"],
+                         "window_pu_s": r64["window_pu_s"]},
         "roofline": dict(bound="hbm", achieved=achieved, peak=HBM_PEAK_GBS, unit="GB/s",
                          frac=achieved / HBM_PEAK_GBS, traffic=ensemble_traffic(n, len(table)), kernel="k_force_brick_fp64",
                          kernel_avg_ms=r64["force_avg_ms"], bytes_per_particle_update=bytes_pu,
@@ -382,6 +503,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2000)
     ap.add_argument("--warmup", type=int, default=200)
+    ap.add_argument("--thermalize", type=int, default=None,
+                    help="steps before tuning and timing (default 3000 single, 400 ensemble, 0 quench)")
     ap.add_argument("--drift-steps", type=int, default=10000)
     ap.add_argument("--cpu-steps", type=int, default=5)
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -397,6 +520,8 @@ def main():
               f"(several GPUs would each own one box of a spatial domain decomposition, exchanging "
               f"ghost images every step, which is not built)", file=sys.stderr)
         sys.exit(2)
+    if args.thermalize is None:
+        args.thermalize = {"single": 3000, "ensemble": 400, "quench": 0}[args.workload]
     if args.workload == "ensemble":
         if args.dt is None:
             args.dt = 0.005
@@ -461,7 +586,7 @@ def main():
                     vector_flop_frac=flops_pu * n / avg_s / 1e12 / FP64_VECTOR_PEAK_TFLOPS)
     cpu = None
     if not args.no_cpu_baseline:
-        cpu = cpu_baseline(x, p, L, args, r64["rebuilds"] / args.steps)
+        cpu = cpu_baseline(x, p, L, args, r64["rebuilds_per_step"])
 
     avg32 = r32["force_avg_ms"] * 1e-3
     line = {
@@ -478,17 +603,28 @@ def main():
         "dtype": "f64",
         "data": "synthetic: FCC lattice (confmaker.cc:8-44), |v0|=sqrt(3) random directions (variables.cc:82-108)",
         "config": {"workload": (f"LJ NVE N={n} FCC rho={args.density} L={L:.6g} rc={args.rc} margin={args.margin} "
-                                f"dt={args.dt}, T/P/E observed every 100 steps, rebuilds by the Verlet budget"),
-                   "particles": n, "rebuilds_in_timed_steps": r64["rebuilds"]},
+                                f"dt={args.dt}, T/P/E observed every 100 steps, rebuilds by the Verlet budget; "
+                                f"{args.thermalize} thermalisation steps before tuning and timing"),
+                   "particles": n, "thermalize": args.thermalize, "rebuilds_in_timed_steps": r64["rebuilds"]},
+        "amortisation": {"rebuilds_per_step": r64["rebuilds_per_step"], "cadence_sample": r64["cadence_sample"],
+                         "construction_ms": r64["construction_ms"], "observation_ms": r64["observation_ms"],
+                         "observations_in_timed_steps": r64["observations"],
+                         "window_pu_s": r64["window_pu_s"], "window_ms_per_step": r64["window_ms_per_step"],
+                         "note": "value = particle-updates/s of the timed steps with constructions at the measured "
+                                 "Verlet cadence and observations every 100 steps (the window's own replaced by "
+                                 "that many); window_pu_s is the timed steps as they fell"},
         "roofline": roofline,
         "cpu_baseline": cpu,
         "mixed": {"value": r32["pu_s"], "ms_per_step": r32["ms_per_step"], "speedup_vs_fp64": r32["pu_s"] / r64["pu_s"],
                   "force_kernel_avg_ms": r32["force_avg_ms"],
+                  "force_kernel_speedup_vs_fp64": r64["force_avg_ms"] / r32["force_avg_ms"],
                   "force_kernel_frac_hbm": bytes_pu * n / avg32 / 1e9 / HBM_PEAK_GBS,
-                  "force_rms_rel_err_vs_fp64": force_rms, "rebuilds_in_timed_steps": r32["rebuilds"]},
-        "energy_drift": {"fp64": drift64, "mixed": drift32},
+                  "force_rms_rel_err_vs_fp64": force_rms, "rebuilds_in_timed_steps": r32["rebuilds"],
+                  "construction_ms": r32["construction_ms"], "rebuilds_per_step": r32["rebuilds_per_step"],
+                  "window_pu_s": r32["window_pu_s"]},
+        "energy_drift": {"fp64": drift64, "mixed": drift32, "criterion": drift_comparison(drift64, drift32)},
         "pair_list": r64["list"],
-        "rebuild_avg_ms": r64["rebuild_avg_ms"],
+        "rebuild_avg_ms": r64["construction_ms"],
         "pcie_inclusive": r64["pcie"],
         "autotune": {"fp64": r64["tuned"], "mixed": r32["tuned"]},
     }

@@ -325,8 +325,16 @@ int set_geometry(mdgpu_domain *d) {
     R.cell0 = (int)ncell;
     R.brick0 = (int)nbrick;
     R.col0 = (int)org.size();
-    for (int k = 0; k < 3; k++)
-      for (int c = 0; c < R.m[k]; c++) org.push_back(0.0 + (double)(c - 1) * R.w[k]);
+    // Origins of the FP32 cell-relative coordinates: multiples of the cell width
+    // rounded to FP32, exact in FP64 (24-bit width times an integer below 2^12).
+    // The mixed kernels add (cell_j - cell_i) * (float)w for a partner one cell
+    // over, which is then exactly the difference of the two origins: a partner
+    // separation carries no per-cell bias, only the rounding of the two
+    // coordinates.  The cells themselves (binning) keep the FP64 width.
+    for (int k = 0; k < 3; k++) {
+      const double w32 = (double)(float)R.w[k];
+      for (int c = 0; c < R.m[k]; c++) org.push_back((double)(c - 1) * w32);
+    }
     ncell += (long long)R.m[0] * R.m[1] * R.m[2];
     nbrick += (long long)R.nb[0] * R.nb[1] * R.nb[2];
     rmax = std::max(rmax, (R.bd[0] + 2) * (R.bd[1] + 2) * (R.bd[2] + 2));
@@ -1060,12 +1068,19 @@ ForceConsts fconsts(const mdgpu_domain *d) {
   return fc;
 }
 
+// FP32 constants of the mixed kick.  C is chosen so that the kick evaluated in
+// FP32 with the rounded A, B at r^2 = rc^2 (rounded) is zero to one rounding:
+// the smoothed force vanishes at the cutoff in the arithmetic the kernel uses,
+// not only in exact arithmetic (8 C2 dt rounded to FP32 would leave a step in
+// the force where pairs cross rc).
 ForceConstsF fconsts32(const mdgpu_domain *d) {
   ForceConstsF f;
   f.CL2 = (float)d->CL2;
   f.A = (float)(24.0 * d->dt);
   f.B = (float)(48.0 * d->dt);
-  f.C = (float)(8.0 * d->C2 * d->dt);
+  const double ir2 = 1.0 / (double)f.CL2;
+  const double ir6 = ir2 * ir2 * ir2;
+  f.C = (float)(-(ir2 * ir6) * ((double)f.A - (double)f.B * ir6));
   return f;
 }
 

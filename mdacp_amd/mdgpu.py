@@ -270,6 +270,12 @@ class Domain:
         self.L = self.L * alpha
 
     # ------------------------------------------------------------- observe
+    def kinetic_energy(self):
+        """KineticEnergyObserver alone (no pair list needed); waits for the domain's stream."""
+        ke = ctypes.c_double()
+        self._chk(load().mdgpu_observe_kinetic(self.h, ctypes.byref(ke)))
+        return ke.value
+
     def observe_raw(self):
         ke, pe, vir = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
         self._chk(load().mdgpu_observe(self.h, ctypes.byref(ke), ctypes.byref(pe), ctypes.byref(vir)))

@@ -13,6 +13,9 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "liboracle.so")
 REF_BIN = os.path.join(HERE, "_ref", "mdacp_ref")
+# the same driver linked against the MI355X facade instead of the reference's
+# MDUnit / MeshList / ForceCalculator (make -C oracle facade_driver)
+FACADE_DRIVER_BIN = os.path.join(HERE, "_ref", "mdacp_gpu_driver")
 
 NVE, NOSEHOOVER, LANGEVIN = 0, 1, 2
 NOISE_MT19937, NOISE_PHILOX = 0, 1
@@ -197,14 +200,17 @@ def ref_available():
 
 
 def run_reference(L, x, v, ids, steps, dump_every=1, dt=0.001, thermostat="nve", T=1.0,
-                  gamma=0.1, tau=0.1, seed=1, init_v0=None, change_scale=None, timeout=600):
+                  gamma=0.1, tau=0.1, seed=1, init_v0=None, change_scale=None, timeout=600,
+                  binary=None, kick=True):
     """Run the reference's MDManager (compiled unmodified, rc=3.0, margin=0.3).
 
     init_v0: MDManager::SetInitialVelocity(init_v0) after loading (the given
     momenta are replaced).  change_scale = (alpha, step): MDManager::ChangeScale(alpha)
-    after that step, followed by a record with scaled=True."""
-    if not ref_available():
-        raise RuntimeError("oracle/_ref/mdacp_ref is not built (run `make -C oracle ref`)")
+    after that step, followed by a record with scaled=True.  binary: another build of
+    the driver (FACADE_DRIVER_BIN); kick=False leaves the kick section zero."""
+    binary = REF_BIN if binary is None else binary
+    if not os.path.exists(binary):
+        raise RuntimeError(f"{binary} is not built (run `make -C oracle ref` / `facade_driver`)")
     Lx, Ly, Lz = (float(L),) * 3 if np.isscalar(L) else (float(v) for v in L)
     dt, T, gamma, tau = float(dt), float(T), float(gamma), float(tau)
     init_v0 = None if init_v0 is None else float(init_v0)
@@ -218,6 +224,8 @@ def run_reference(L, x, v, ids, steps, dump_every=1, dt=0.001, thermostat="nve",
             lines.append("HeatbathType=" + ("Langevin" if thermostat == "langevin" else "NoseHoover"))
         if init_v0 is not None:
             lines.append(f"InitV0={init_v0!r}")
+        if not kick:
+            lines.append("Kick=0")
         if change_scale is not None:
             lines.append(f"ChangeScaleAlpha={float(change_scale[0])!r}")
             lines.append(f"ChangeScaleStep={int(change_scale[1])}")
@@ -235,7 +243,7 @@ def run_reference(L, x, v, ids, steps, dump_every=1, dt=0.001, thermostat="nve",
             np.asarray(ids, dtype=np.int32).tofile(f)
         out = os.path.join(td, "out.bin")
         env = dict(os.environ, OMP_NUM_THREADS="1")
-        subprocess.run([REF_BIN, spec, pf, out], check=True, env=env, cwd=td,
+        subprocess.run([binary, spec, pf, out], check=True, env=env, cwd=td,
                        stdout=subprocess.DEVNULL, timeout=timeout)
         return parse_records(open(out, "rb").read())
 
@@ -265,9 +273,10 @@ def parse_records(buf):
         gcode = take(np.int32, ng)
         gq = take(np.float64, 3 * ng).reshape(ng, 3)
         pairs = take(np.int32, 3 * npairs).reshape(npairs, 3)
+        pairs_csr = take(np.int32, 3 * npairs).reshape(npairs, 3)
         kick = take(np.float64, 3 * pn).reshape(pn, 3)
         recs.append(dict(step=step, rebuilt=bool(rebuilt), dump=True, scaled=bool(scaled), pn=pn, tn=tn, ncons=ncons,
                          ke=obs[0], pe=obs[1], vir=obs[2], T=obs[3], P=obs[4], E=obs[5],
                          time=obs[6], q=q, p=p, type=t, gsrc=gsrc, gcode=gcode, gq=gq,
-                         pairs=pairs, kick=kick))
+                         pairs=pairs, pairs_csr=pairs_csr, kick=kick))
     return recs

@@ -11,7 +11,11 @@
 //   HeatbathTau) plus driver keys read here: Steps, DumpEvery, InitV0, Seed,
 //   ChangeScaleAlpha and ChangeScaleStep (MDManager::ChangeScale(alpha) called
 //   after that step, mdmanager.cc:432-443, followed by a record flagged in
-//   header[6]).
+//   header[6]), Kick (0: the kick section of each record is left zero).
+//
+// The same source also links against the MI355X facade (mdacp_amd/facade,
+// `make -C oracle facade_driver` -> oracle/_ref/mdacp_gpu_driver), where it
+// reads the facade's list accessors through the same calls.
 //   particles.bin: int32 n, then n*(x,y,z,vx,vy,vz) doubles, then n int32 ids.
 //   out.bin: a sequence of records (see write_record).
 #include <omp.h>
@@ -32,7 +36,7 @@
 namespace {
 
 struct Spec {
-  int steps = 0, dump_every = 1, seed = 1, scale_step = -1;
+  int steps = 0, dump_every = 1, seed = 1, scale_step = -1, kick = 1;
   double init_v0 = -1.0, scale_alpha = 1.0;
 };
 
@@ -50,6 +54,7 @@ Spec read_spec(const char *path) {
     if (key == "InitV0") s.init_v0 = std::stod(val);
     if (key == "ChangeScaleAlpha") s.scale_alpha = std::stod(val);
     if (key == "ChangeScaleStep") s.scale_step = std::stoi(val);
+    if (key == "Kick") s.kick = std::stoi(val);
   }
   return s;
 }
@@ -139,8 +144,13 @@ void put(std::ofstream &o, const T *p, size_t n) {
 //         (scaled = 1: the record written right after MDManager::ChangeScale)
 //         double obs[8] = {ke_raw, pe_raw, vir_raw, T, P, E, sim_time, 0}
 //         q[pn*3], p[pn*3], type[pn] (int32), ghost_src[tn-pn], ghost_code[tn-pn],
-//         ghost_q[(tn-pn)*3], pairs[npairs*3] (int32), kick[pn*3]
-void write_record(std::ofstream &o, MDManager *mdm, int step, int rebuilt, double SL, int scaled = 0) {
+//         ghost_q[(tn-pn)*3], pairs[npairs*3] (int32), pairs_csr[npairs*3] (int32),
+//         kick[pn*3]
+// pairs come from the COO accessors (GetKeyParticles/GetPartnerParticles,
+// meshlist.h:46-48), pairs_csr from the CSR ones (GetPartnerNumber,
+// GetKeyPointer, GetSortedList, meshlist.h:49-53), each as canonical triples.
+void write_record(std::ofstream &o, MDManager *mdm, int step, int rebuilt, double SL, bool kick_on,
+                  int scaled = 0) {
   MDUnit *u = mdm->GetMDUnit(0);
   Variables *vars = u->GetVariables();
   MeshGrab grab;
@@ -175,10 +185,7 @@ void write_record(std::ofstream &o, MDManager *mdm, int step, int rebuilt, doubl
     for (int d = 0; d < 3; d++) gq[3 * (g - pn) + d] = vars->q[g][d];
   put(o, gq.data(), gq.size());
   // canonical pairs: (owned key id, partner owner id, partner image shift)
-  std::vector<int32_t> pr(3 * (size_t)np);
-  int *key = mesh->GetKeyParticles(), *par = mesh->GetPartnerParticles();
-  for (int k = 0; k < np; k++) {
-    int i = key[k], j = par[k];
+  auto canon = [&](int i, int j, int32_t *out) {
     int a = vars->type[i], b, sh = 13;
     if (j < pn) {
       b = vars->type[j];
@@ -187,20 +194,34 @@ void write_record(std::ofstream &o, MDManager *mdm, int step, int rebuilt, doubl
       b = gsrc[j - pn];
       sh = gcode[j - pn];
     }
-    pr[3 * k] = a;
-    pr[3 * k + 1] = b;
-    pr[3 * k + 2] = sh;
-  }
+    out[0] = a;
+    out[1] = b;
+    out[2] = sh;
+  };
+  std::vector<int32_t> pr(3 * (size_t)np);
+  int *key = mesh->GetKeyParticles(), *par = mesh->GetPartnerParticles();
+  for (int k = 0; k < np; k++) canon(key[k], par[k], &pr[3 * (size_t)k]);
   put(o, pr.data(), pr.size());
+  // the same list through the CSR accessors, row by row
+  std::vector<int32_t> pc(3 * (size_t)np, -1);
+  int *sorted = mesh->GetSortedList();
+  size_t m = 0;
+  for (int i = 0; i < pn; i++) {
+    const int kp = mesh->GetKeyPointer(i), nn = mesh->GetPartnerNumber(i);
+    for (int k = 0; k < nn && m < (size_t)np; k++, m++) canon(i, sorted[kp + k], &pc[3 * m]);
+  }
+  put(o, pc.data(), pc.size());
   // kick from zero momenta through the reference's own force routine
-  std::vector<double> save(4 * (size_t)tn);
-  std::memcpy(save.data(), vars->p, sizeof(double) * 4 * tn);
-  std::memset(vars->p, 0, sizeof(double) * 4 * tn);
-  u->CalculateForce();
-  std::vector<double> kick(3 * pn);
-  for (int i = 0; i < pn; i++)
-    for (int d = 0; d < 3; d++) kick[3 * i + d] = vars->p[i][d];
-  std::memcpy(vars->p, save.data(), sizeof(double) * 4 * tn);
+  std::vector<double> kick(3 * pn, 0.0);
+  if (kick_on) {
+    std::vector<double> save(4 * (size_t)tn);
+    std::memcpy(save.data(), vars->p, sizeof(double) * 4 * tn);
+    std::memset(vars->p, 0, sizeof(double) * 4 * tn);
+    u->CalculateForce();
+    for (int i = 0; i < pn; i++)
+      for (int d = 0; d < 3; d++) kick[3 * i + d] = vars->p[i][d];
+    std::memcpy(vars->p, save.data(), sizeof(double) * 4 * tn);
+  }
   put(o, kick.data(), kick.size());
 }
 
@@ -234,18 +255,19 @@ int main(int argc, char **argv) {
   mdm->SetSeed(spec.seed);
   std::ofstream out(argv[3], std::ios::binary);
   mdm->MakePairList();
-  write_record(out, mdm, 0, 1, SL);
+  const bool kick_on = spec.kick != 0;
+  write_record(out, mdm, 0, 1, SL, kick_on);
   MeshGrab grab;
   for (int step = 1; step <= spec.steps; step++) {
     if (step - 1 == spec.scale_step) {
       mdm->ChangeScale(spec.scale_alpha);
-      write_record(out, mdm, step - 1, 1, SL, 1);
+      write_record(out, mdm, step - 1, 1, SL, kick_on, 1);
     }
     mdm->GetMDUnit(0)->ObserveDouble(&grab);
     const int before = grab.mesh->GetNumberOfConstructions();
     mdm->Calculate();
     const int rebuilt = grab.mesh->GetNumberOfConstructions() != before;
-    if (step % spec.dump_every == 0) write_record(out, mdm, step, rebuilt, SL);
+    if (step % spec.dump_every == 0) write_record(out, mdm, step, rebuilt, SL, kick_on);
     else {
       int64_t hdr[8] = {step, -1, 0, 0, rebuilt, 0, 0, 0};
       put(out, hdr, 8);

@@ -20,6 +20,33 @@ def pair_hash(pairs3):
     return hashlib.sha256(keys.tobytes()).hexdigest(), int(len(keys))
 
 
+def _mix64(x):
+    """splitmix64 finaliser on a uint64 array (wrapping arithmetic)."""
+    with np.errstate(over="ignore"):
+        z = x + np.uint64(0x9E3779B97F4A7C15)
+        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+    return z ^ (z >> np.uint64(31))
+
+
+def pair_checksum(pairs3, chunk=1 << 24):
+    """Order-independent checksum of a canonical pair set, for sets too large to sort
+    quickly (4M and 16.4M particles): the count, and the sum (mod 2^64) and the xor of
+    splitmix64 of each canonical key.  Two sets with equal checksums are equal but for
+    a 2^-64-scale accident; a missing, extra or changed pair changes the sum."""
+    n = len(pairs3)
+    tot = np.uint64(0)
+    xor = np.uint64(0)
+    for a in range(0, n, chunk):
+        p = np.asarray(pairs3[a:a + chunk], dtype=np.int64)
+        key = ((p[:, 0] << 36) | (p[:, 1] << 5) | p[:, 2]).astype(np.uint64)
+        h = _mix64(key)
+        with np.errstate(over="ignore"):
+            tot = tot + np.sum(h, dtype=np.uint64)
+        xor ^= np.bitwise_xor.reduce(h)
+    return dict(count=int(n), sum=f"{int(tot):016x}", xor=f"{int(xor):016x}")
+
+
 def state_hash_by_id(q, p, ids):
     o = np.argsort(ids, kind="stable")
     h = hashlib.sha256()
@@ -78,3 +105,24 @@ def oracle_system(L, x, p, ids, dt, rc, margin):
     s = O.OracleSystem(L, dt, rc, margin)
     s.add(x, p, ids)
     return s
+
+
+def fcc_lattice_sums(a, rc, SL):
+    """Exact lattice sums of the smoothed LJ potential on an infinite FCC crystal of cubic
+    cell a: PE per particle, virial per particle (VirialObserver / N), and the number of
+    half pairs per particle within SL."""
+    import math
+    C0, C2 = lj_C(rc)
+    m = int(math.ceil(SL / a)) + 2
+    r = np.arange(-m, m + 1)
+    X, Y, Z = np.meshgrid(r, r, r, indexing="ij")
+    base = np.stack([X.ravel(), Y.ravel(), Z.ravel()], 1).astype(float) * a
+    pts = np.concatenate([base + np.array(b) * a for b in [(0, 0, 0), (0, .5, .5), (.5, 0, .5), (.5, .5, 0)]])
+    r2 = np.einsum("ij,ij->i", pts, pts)
+    r2 = r2[r2 > 0]
+    inside = r2[r2 <= rc * rc]
+    pe = 0.5 * np.sum(4.0 * (inside ** -6 - inside ** -3 + C2 * inside + C0))
+    r6 = inside ** 3
+    vir = 0.5 * np.sum(((24.0 * r6 - 48.0) / (r6 * r6 * inside) + C2 * 8.0) * inside) / 3.0
+    half_pairs_sl = int(np.sum(r2 <= SL * SL)) // 2
+    return pe, vir, half_pairs_sl

@@ -1,13 +1,17 @@
 """The reference's own program (main.cc, MDManager, ProjectManager and its
 projects, unmodified) running on the MI355X facade (mdacp_amd/facade ->
-libmdgpu.so), against the same program on the CPU.
+libmdgpu.so), against what the same program printed and dumped on the CPU.
 
-Both binaries are built from the reference sources by `make -C oracle programs`
-(oracle/_ref/mdacp_gpu and oracle/_ref/mdacp_cpu); the tests are skipped where
-they were not built.  They read no reference file at run time: the configs are
-written here.  The programs print observables with 6 significant digits
-(std::ostream defaults), which bounds the comparison.
+oracle/_ref/mdacp_gpu is built from the reference sources by
+`make -C oracle programs`; the CPU program's output is a fixture written in the
+build container by tests/golden/make_program_fixtures.py (tests/golden/
+programs.npz), so the reference never runs here.  The programs print observables
+with 6 significant digits (std::ostream defaults), which bounds the comparison;
+cdview dumps (conf*.cd: "0 type x y z px py pz" per particle, mdunit.cc:52-67)
+are matched particle by particle by position, since the facade stores particles
+in its own order.
 """
+import json
 import os
 import subprocess
 
@@ -15,28 +19,22 @@ import numpy as np
 import pytest
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-REF = os.path.join(ROOT, "oracle", "_ref")
-GPU = os.path.join(REF, "mdacp_gpu")
-CPU = os.path.join(REF, "mdacp_cpu")
+GPU = os.path.join(ROOT, "oracle", "_ref", "mdacp_gpu")
+GOLDEN = os.path.join(ROOT, "tests", "golden")
 
 pytestmark = [
     pytest.mark.gpu,
-    pytest.mark.skipif(not (os.path.exists(GPU) and os.path.exists(CPU)),
-                       reason="oracle/_ref programs not built (make -C oracle programs)"),
+    pytest.mark.skipif(not os.path.exists(GPU), reason="oracle/_ref/mdacp_gpu not built (make -C oracle programs)"),
 ]
 
 
-def run(binary, cfg_text, tmp_path, tag, env_extra=None, timeout=240):
-    d = tmp_path / tag
-    d.mkdir()
-    cfg = d / "input.cfg"
-    cfg.write_text(cfg_text)
-    env = dict(os.environ, OMP_NUM_THREADS="1")
-    env.update(env_extra or {})
-    print(f"running {os.path.basename(binary)} {tag}", flush=True)
-    r = subprocess.run([binary, str(cfg)], cwd=d, env=env, capture_output=True, text=True, timeout=timeout)
-    assert r.returncode == 0, f"{binary} failed ({r.returncode}):\n{r.stdout[-2000:]}\n{r.stderr[-2000:]}"
-    return r.stdout
+def fixture(name):
+    with open(os.path.join(GOLDEN, "programs.json")) as f:
+        meta = json.load(f)[name]
+    arrs = np.load(os.path.join(GOLDEN, "programs.npz"))
+    rows = arrs[f"{name}/rows"]
+    dumps = {fn: arrs[f"{name}/{fn}"] for fn in meta["dumps"]}
+    return meta, rows, dumps
 
 
 def observe_rows(out, marker):
@@ -52,78 +50,133 @@ def observe_rows(out, marker):
     return np.array(rows)
 
 
-BENCH = """Mode=Benchmark
-ThermalizeLoop=20
-SystemSize=20
-TimeStep=0.001
-TotalLoop=400
-ObserveLoop=100
-InitialVelocity=1.0
-"""
+def run_gpu(name, tmp_path, env_extra=None, timeout=240):
+    meta, rows, dumps = fixture(name)
+    d = tmp_path / name
+    d.mkdir()
+    cfg = d / "input.cfg"
+    cfg.write_text(meta["cfg"])
+    env = dict(os.environ, OMP_NUM_THREADS="1")
+    env.update(env_extra or {})
+    print(f"running mdacp_gpu {name} {env_extra or ''}", flush=True)
+    r = subprocess.run([GPU, str(cfg)], cwd=d, env=env, capture_output=True, text=True, timeout=timeout)
+    assert r.returncode == 0, f"mdacp_gpu failed ({r.returncode}):\n{r.stdout[-2000:]}\n{r.stderr[-2000:]}"
+    got = {fn: np.loadtxt(d / fn, ndmin=2) for fn in sorted(os.listdir(d)) if fn.endswith(".cd")}
+    return observe_rows(r.stdout, meta["marker"]), rows, got, dumps
+
+
+def match_dump(got, want, pos_tol, mom_tol):
+    """Pairs every particle of one dump with the nearest particle of the other by position
+    (a bijection is required), then bounds the position and momentum differences."""
+    from scipy.spatial import cKDTree
+
+    assert got.shape == want.shape, (got.shape, want.shape)
+    assert np.all(got[:, 0] == 0.0) and np.all(want[:, 0] == 0.0)
+    dist, idx = cKDTree(want[:, 2:5]).query(got[:, 2:5])
+    assert len(np.unique(idx)) == len(idx), "positions do not pair one to one"
+    w = want[idx]
+    assert np.array_equal(got[:, 1], w[:, 1])  # particle types
+    dq = np.abs(got[:, 2:5] - w[:, 2:5]).max()
+    dp = np.abs(got[:, 5:8] - w[:, 5:8]).max()
+    assert dq <= pos_tol and dp <= mom_tol, (dq, dp)
+    return dq, dp
 
 
 def test_benchmark_project_matches_reference_program(tmp_path):
-    """Benchmark project (NVE, vacancy FCC, |v0| = 1): t, T, P, E every 100 steps."""
-    cpu = observe_rows(run(CPU, BENCH, tmp_path, "cpu"), "#observe")
-    gpu = observe_rows(run(GPU, BENCH, tmp_path, "gpu"), "#observe")
+    """Benchmark project (NVE, vacancy FCC, |v0| = 1): t, T, P, E every 100 steps to the
+    last printed digit (relative 1e-5 of each value's scale)."""
+    gpu, cpu, _, _ = run_gpu("benchmark", tmp_path)
     assert cpu.shape == gpu.shape == (4, 4), (cpu, gpu)
-    # 6 printed digits: agreement to the last printed digit (relative 1e-5 of each value's scale)
     np.testing.assert_allclose(gpu, cpu, rtol=2e-5, atol=2e-5)
 
 
 def test_benchmark_project_mixed_precision(tmp_path):
-    cpu = observe_rows(run(CPU, BENCH, tmp_path, "cpu"), "#observe")
-    gpu = observe_rows(run(GPU, BENCH, tmp_path, "gpu", {"MDACP_GPU_PRECISION": "mixed"}), "#observe")
+    """The same with FP32 pair forces: the trajectory leaves the FP64 one slowly, and T, P,
+    E agree to within 1e-4 of their scale after 400 steps."""
+    gpu, cpu, _, _ = run_gpu("benchmark", tmp_path, {"MDACP_GPU_PRECISION": "mixed"})
     assert cpu.shape == gpu.shape
     np.testing.assert_allclose(gpu[:, 0], cpu[:, 0])
-    # FP32 pair forces: the trajectory separates slowly from the FP64 one
-    np.testing.assert_allclose(gpu[:, 1:], cpu[:, 1:], rtol=2e-3, atol=2e-3)
-
-
-NOSEHOOVER = """Mode=Benchmark
-ThermalizeLoop=10
-SystemSize=20
-TimeStep=0.002
-TotalLoop=300
-ObserveLoop=100
-InitialVelocity=1.0
-ControlTemperature=yes
-AimedTemperature=0.9
-HeatbathType=NoseHoover
-"""
+    np.testing.assert_allclose(gpu[:, 1:], cpu[:, 1:], rtol=1e-4, atol=1e-4)
 
 
 def test_nose_hoover_benchmark_matches_reference_program(tmp_path):
     """Benchmark project with the Nose-Hoover heat bath (two temperature reductions a step)."""
-    cpu = observe_rows(run(CPU, NOSEHOOVER, tmp_path, "cpu"), "#observe")
-    gpu = observe_rows(run(GPU, NOSEHOOVER, tmp_path, "gpu"), "#observe")
+    gpu, cpu, _, _ = run_gpu("nosehoover", tmp_path)
     assert cpu.shape == gpu.shape == (3, 4)
     np.testing.assert_allclose(gpu, cpu, rtol=2e-5, atol=2e-5)
-
-
-LANGEVIN = """Mode=Langevin
-InitialVelocity=1.8
-TimeStep=0.005
-UnitLength=16
-ObserveLoop=100
-TotalLoop=3000
-AimedTemperature=0.9
-Density=0.70
-HeatbathTau=0.1
-HeatbathGamma=1.0
-HeatbathType=Langevin
-Seed=12345
-"""
 
 
 def test_langevin_project_thermalises(tmp_path):
     """Langevin project: the noise stream differs from the reference's (Philox on the
     device, std::mt19937 on the host), so the runs agree in distribution only: same
     first observation, and both relax towards T = 0.9."""
-    cpu = observe_rows(run(CPU, LANGEVIN, tmp_path, "cpu"), None)
-    gpu = observe_rows(run(GPU, LANGEVIN, tmp_path, "gpu"), None)
+    gpu, cpu, _, _ = run_gpu("langevin", tmp_path)
     assert cpu.shape == gpu.shape == (30, 4)
     np.testing.assert_allclose(gpu[:, 0], cpu[:, 0])
     assert abs(gpu[0, 1] - cpu[0, 1]) < 0.02 * cpu[0, 1]
     t_gpu, t_cpu = gpu[15:, 1].mean(), cpu[15:, 1].mean()
     assert abs(t_gpu - 0.9) < 0.06 and abs(t_cpu - 0.9) < 0.06, (t_gpu, t_cpu)
+
+
+def test_collision_project_and_cdview_dumps(tmp_path):
+    """Collision project (collision.cc:53-79): two droplets at +-1 along x, cdview dumps at
+    the start and at every observation.  The initial dump is the same particles to the printed
+    digit; later dumps pair one to one within the printed precision."""
+    gpu, cpu, got, want = run_gpu("collision", tmp_path)
+    assert cpu.shape == gpu.shape == (4, 2)
+    np.testing.assert_allclose(gpu, cpu, rtol=2e-5, atol=2e-5)
+    assert sorted(got) == sorted(want) == [f"conf{i:04d}.cd" for i in range(5)]
+    for fn in sorted(want):
+        dq, dp = match_dump(got[fn], want[fn], pos_tol=1.5e-4, mom_tol=1.5e-5)
+        print(fn, "max |dq|", dq, "max |dp|", dp, flush=True)
+
+
+def test_burst_project_and_cdview_dumps(tmp_path):
+    """Burst project (burst.cc:103-142): a droplet thermalised with the Nose-Hoover bath, then
+    NVE: t, T, P, E rows and the cdview dumps of both phases."""
+    gpu, cpu, got, want = run_gpu("burst", tmp_path)
+    assert cpu.shape == gpu.shape == (3, 4)
+    np.testing.assert_allclose(gpu, cpu, rtol=2e-5, atol=2e-5)
+    assert sorted(got) == sorted(want)
+    for fn in sorted(want):
+        dq, dp = match_dump(got[fn], want[fn], pos_tol=1.5e-4, mom_tol=1.5e-5)
+        print(fn, "max |dq|", dq, "max |dp|", dp, flush=True)
+
+
+def test_facade_list_accessors_match_reference_fixture():
+    """The reference's MDManager on the facade, driven by oracle/ref_driver.cc linked against
+    it (oracle/_ref/mdacp_gpu_driver): the pair lists read through the facade's COO accessors
+    (GetKeyParticles / GetPartnerParticles) and its CSR accessors (GetPartnerNumber /
+    GetKeyPointer / GetSortedList, meshlist.h:46-53) both equal the reference's own pair sets
+    (SHA-256 of the canonical set, tests/golden/fcc2048_nve.json) at every dump, with the
+    reference's rebuild steps and T, P, E within 1e-9."""
+    import math
+    import sys
+
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from helpers import load_golden, pair_hash
+    from mdacp_amd.lattice import fcc, initial_velocity
+    from oracle import oracle as O
+
+    if not os.path.exists(O.FACADE_DRIVER_BIN):
+        pytest.skip("oracle/_ref/mdacp_gpu_driver not built (make -C oracle facade_driver)")
+    g = load_golden("fcc2048_nve")
+    x = fcc(g["L"], g["density"])
+    n = len(x)
+    print("running mdacp_gpu_driver fcc2048_nve", flush=True)
+    recs = O.run_reference(g["L"], x, initial_velocity(n, g["v0"], 0), np.arange(n, dtype=np.int32),
+                           steps=g["steps"], dump_every=50, dt=g["dt"], binary=O.FACADE_DRIVER_BIN, kick=False,
+                           timeout=240)
+    dumps = {d["step"]: d for d in g["dumps"]}
+    assert [r["step"] for r in recs if r["rebuilt"]] == g["rebuild_steps"]
+    seen = 0
+    for r in recs:
+        if not r["dump"]:
+            continue
+        want = dumps[r["step"]]
+        assert pair_hash(r["pairs"]) == (want["pair_sha256"], want["pair_count"]), r["step"]
+        assert pair_hash(r["pairs_csr"]) == (want["pair_sha256"], want["pair_count"]), r["step"]
+        for k in ("T", "P", "E"):
+            assert r[k] == pytest.approx(want[k], rel=1e-9), (r["step"], k)
+        seen += 1
+    assert seen == len(g["dumps"])

@@ -12,7 +12,7 @@ import math
 import numpy as np
 import pytest
 
-from helpers import lj_C, load_golden, pair_hash, state_hash_by_id
+from helpers import fcc_lattice_sums, lj_C, load_golden, pair_checksum, pair_hash, state_hash_by_id
 from mdacp_amd.lattice import fcc, initial_velocity
 from oracle import oracle as O
 
@@ -161,24 +161,6 @@ def test_oracle_pairs_equal_brute_force(rc, margin, seed):
     s.make_pair_list()
     bf = brute_force_pairs(x, ids, L, rc + margin)
     assert np.array_equal(O.canonical_pairs(s.pairs()), O.canonical_pairs(bf))
-
-
-def fcc_lattice_sums(a, rc, SL):
-    """Exact lattice sums of the smoothed LJ potential on an infinite FCC crystal."""
-    C0, C2 = lj_C(rc)
-    m = int(math.ceil(SL / a)) + 2
-    r = np.arange(-m, m + 1)
-    X, Y, Z = np.meshgrid(r, r, r, indexing="ij")
-    base = np.stack([X.ravel(), Y.ravel(), Z.ravel()], 1).astype(float) * a
-    pts = np.concatenate([base + np.array(b) * a for b in [(0, 0, 0), (0, .5, .5), (.5, 0, .5), (.5, .5, 0)]])
-    r2 = np.einsum("ij,ij->i", pts, pts)
-    r2 = r2[r2 > 0]
-    inside = r2[r2 <= rc * rc]
-    pe = 0.5 * np.sum(4.0 * (inside ** -6 - inside ** -3 + C2 * inside + C0))
-    r6 = inside ** 3
-    vir = 0.5 * np.sum(((24.0 * r6 - 48.0) / (r6 * r6 * inside) + C2 * 8.0) * inside) / 3.0
-    half_pairs_sl = int(np.sum(r2 <= SL * SL)) // 2
-    return pe, vir, half_pairs_sl
 
 
 @pytest.mark.parametrize("rc,margin,pe_ref,vir_ref,pairs_ref", [
@@ -403,3 +385,23 @@ def test_change_scale_mathematics():
         assert ob["pe"] / nf == pytest.approx(pe_sum, rel=1e-12)
         assert ob["vir"] / nf == pytest.approx(vir_sum, rel=1e-12)
         assert np.max(np.abs(f.kick())) < 1e-12
+
+
+def test_pair_checksum_detects_changes():
+    """The order-independent checksum used for the 4M and 16.4M pair sets: invariant under
+    reordering; a dropped, added or altered pair (partner id or image code) changes it."""
+    rng = np.random.default_rng(9)
+    a = rng.integers(0, 1 << 24, 5000)
+    b = rng.integers(0, 1 << 24, 5000)
+    c = rng.integers(0, 27, 5000)
+    p = np.stack([np.minimum(a, b), np.maximum(a, b), c], 1).astype(np.int32)
+    ref = pair_checksum(p)
+    assert pair_checksum(p[rng.permutation(len(p))]) == ref
+    assert pair_checksum(p, chunk=77) == ref
+    assert pair_checksum(p[1:]) != ref
+    q = p.copy(); q[10, 1] += 1
+    assert pair_checksum(q) != ref
+    q = p.copy(); q[10, 2] = (q[10, 2] + 1) % 27
+    assert pair_checksum(q) != ref
+    q = np.concatenate([p, p[:1]])
+    assert pair_checksum(q) != ref
