@@ -66,9 +66,10 @@ typedef struct mdgpu_domain mdgpu_domain;
 
 /* force precision */
 #define MDGPU_FP64 0   /* FP64 throughout */
-#define MDGPU_MIXED 1  /* FP32 cell-relative pair forces, FP32 sums of each 8 list
-                          entries added in FP64 (MDGPU_OPT_KAHAN), FP64 positions,
-                          momenta, integration and reductions */
+#define MDGPU_MIXED 1  /* FP32 pair forces from FP32 coordinates relative to the row's
+                          cell brick (or its cell, MDGPU_OPT_MIXED_FRAME), FP32 sums of
+                          each 8 list entries added in FP64 (MDGPU_OPT_KAHAN), FP64
+                          positions, momenta, integration and reductions */
 
 /* thermostat (SimulationInfo::ControlTemperature / HeatbathType) */
 #define MDGPU_NVE 0
@@ -235,6 +236,23 @@ int mdgpu_get_timing(const mdgpu_domain *d, int *force_launches, double *force_m
                                    the run at the step whose displacement check or construction
                                    needs the host, which then takes that step; the same
                                    trajectory either way */
+#define MDGPU_OPT_PIPELINE 14  /* FP64 kick with 4, 8 or 16 wavefronts: 1 = persistent workgroups (one
+                                  per CU) whose producer wavefronts stage the next brick's
+                                  neighbourhood into a second LDS buffer with LDS-direct loads while
+                                  the others compute the current brick; 0 = one workgroup per brick
+                                  (default); the same sums in the same order */
+#define MDGPU_OPT_MIXED_FRAME 15  /* mixed precision's FP32 coordinates: 1 = relative to the centre of
+                                     the row's cell brick, staged from the FP64 positions (default: no
+                                     offsets to read; coordinates up to about half a brick
+                                     neighbourhood, so rounding about 2x coarser); 0 = relative to each
+                                     particle's cell, with the partner's cell offset per list entry */
+#define MDGPU_OPT_DISPLACEMENT_CHECK 16  /* rebuild cadence: 1 = once the Verlet budget is spent, the
+                                            exact displacement check each step (pairlist.cc:26-121;
+                                            the default and the reference's schedule); 0 = the list
+                                            expires as soon as the budget is spent (more constructions,
+                                            no checks).  The pairs within rc are the same either way;
+                                            only the construction steps, and with them the summation
+                                            order of the kicks, differ */
 int mdgpu_set_option(mdgpu_domain *d, int option, int value);
 int mdgpu_get_option(const mdgpu_domain *d, int option, int *value);
 

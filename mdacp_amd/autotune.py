@@ -29,6 +29,15 @@ cells and times differently).
 - dynamic pruning: the inner-list radius rc + delta, delta in {off, 0.05,
   0.1, 0.15, 0.2, 0.3}, by the time per step including the pruning passes
   (the kick sums the same pairs in the same order whatever delta is);
+- list layout (MDGPU_OPT_ROW_SORT): padded 64-row slices in cell order (0),
+  slices of rows sorted by list length in steps of 8 (8), or by exact length,
+  the CSR-like layout with the least padding (1), by the force kernel's time
+  (a row sums its partners in the same order in every layout);
+- rebuild cadence (MDGPU_OPT_DISPLACEMENT_CHECK): once the Verlet budget is
+  spent, the exact displacement check every step until it expires the list
+  (the reference's schedule, 1), or a construction at once (0), by the time
+  per step over a window holding several constructions; the pairs within rc
+  are the same, so forces change only in summation order;
 - optionally the Verlet margin (the rebuild cadence): each candidate margin's
   time per step including its rebuilds.  A different margin lists different
   pairs than the reference's configuration would, so the margin is only
@@ -41,6 +50,8 @@ import time
 from . import mdgpu
 
 SEARCH = [(w, s) for w in (4, 8, 16) for s in (1, 0)]
+LAYOUT = [8, 1, 0]  # MDGPU_OPT_ROW_SORT values
+CADENCE = [1, 0]    # MDGPU_OPT_DISPLACEMENT_CHECK values
 PRUNE = [0, 50, 100, 150, 200, 300]  # delta in thousandths
 FORCE = [(w, kb, u) for w in (4, 8) for kb in (48, 56, 64, 72, 76, 96) for u in (8, 16, 32)]
 
@@ -80,7 +91,17 @@ def _pick(cands, times, current):
     return best
 
 
-def autotune(d, steps=30, margins=None, log=None):
+def _step_ms(d, steps):
+    """Wall time per step over `steps` steps, constructions included."""
+    d.calculate(3)
+    d.kinetic_energy() if not hasattr(d, "R") else d.observe_replicas_raw()
+    t0 = time.perf_counter()
+    d.calculate(steps)
+    d.kinetic_energy() if not hasattr(d, "R") else d.observe_replicas_raw()
+    return 1e3 * (time.perf_counter() - t0) / steps
+
+
+def autotune(d, steps=30, margins=None, log=None, cadence_steps=300):
     """Tune d's launch options; returns a dict of the choices and the timings behind them."""
     say = log or (lambda *a: None)
     c = d.clone()
@@ -134,9 +155,28 @@ def autotune(d, steps=30, margins=None, log=None):
         res["prune"][f"{m}"] = pt[m] = 1e3 * (time.perf_counter() - t0) / (2 * steps)
         say(f"prune delta={m / 1000}: {pt[m]:.4f} ms per step")
     best_p = _pick(PRUNE, pt, d.get_option(mdgpu.OPT_PRUNE))
+    c.set_option(mdgpu.OPT_PRUNE, best_p)
+    res["layout"] = {}
+    lt = {}
+    for v in LAYOUT:
+        c.set_option(mdgpu.OPT_ROW_SORT, v)
+        c.make_pair_list()
+        _force_ms(c, 3)
+        res["layout"][f"{v}"] = lt[v] = _force_ms(c, steps)
+        say(f"layout row_sort={v}: force {lt[v]:.4f} ms")
+    best_l = _pick(LAYOUT, lt, d.get_option(mdgpu.OPT_ROW_SORT))
+    c.set_option(mdgpu.OPT_ROW_SORT, best_l)
+    c.make_pair_list()
+    res["cadence"] = {}
+    ct = {}
+    for v in CADENCE:
+        c.set_option(mdgpu.OPT_DISPLACEMENT_CHECK, v)
+        res["cadence"][f"{v}"] = ct[v] = _step_ms(c, cadence_steps)
+        say(f"cadence displacement_check={v}: {ct[v]:.4f} ms per step")
+    best_c = _pick(CADENCE, ct, d.get_option(mdgpu.OPT_DISPLACEMENT_CHECK))
     c.close()
     choice = dict(search_waves=best_s[0], search_stage=best_s[1], waves=best_f[0], lds_kb=best_f[1],
-                  unroll=best_f[2], prune=best_p)
+                  unroll=best_f[2], prune=best_p, row_sort=best_l, displacement_check=best_c)
     if brick is not None:
         choice["brick"] = brick
         if d.get_option(mdgpu.OPT_BRICK) != _brick_code(brick):
@@ -147,6 +187,9 @@ def autotune(d, steps=30, margins=None, log=None):
     d.set_option(mdgpu.OPT_LDS_KB, choice["lds_kb"])
     d.set_option(mdgpu.OPT_UNROLL, choice["unroll"])
     d.set_option(mdgpu.OPT_PRUNE, choice["prune"])
+    if d.get_option(mdgpu.OPT_ROW_SORT) != choice["row_sort"]:
+        d.set_option(mdgpu.OPT_ROW_SORT, choice["row_sort"])
+    d.set_option(mdgpu.OPT_DISPLACEMENT_CHECK, choice["displacement_check"])
     res["choice"] = choice
     if margins:
         res["margin"] = {}

@@ -237,6 +237,71 @@ __device__ __forceinline__ double3 row_kick_mixed(const uint4 *__restrict__ rg, 
   return make_double3((double)ax - (double)cx, (double)ay - (double)cy, (double)az - (double)cz);
 }
 
+// Mixed-precision kick of one row in a brick-centred FP32 frame: partners and
+// the row are (float)(q - c) with c the centre of the row's brick, so dx =
+// rj - ri needs no cell offset.  Same groups, loads and sums as row_kick_mixed.
+template <bool kStaged, int kSum, int P>
+__device__ __forceinline__ double3 row_kick_mixed_brick(const uint4 *__restrict__ rg, int len, const float4 ri,
+                                                        const float4 *__restrict__ lds,
+                                                        const double4 *__restrict__ src,
+                                                        const double4 *__restrict__ ghost, int pn,
+                                                        const int *__restrict__ gl, int total,
+                                                        const ForceConstsF &fc, const double3 c) {
+  float ax = 0.f, ay = 0.f, az = 0.f, cx = 0.f, cy = 0.f, cz = 0.f;
+  double sx = 0.0, sy = 0.0, sz = 0.0;
+  uint4 ahead[P];
+#pragma unroll
+  for (int j = 0; j < P; j++) ahead[j] = rg[j * kWave];
+  for (int k = 0; k < len; k += kGroup) {
+    int e[kGroup];
+    unpack_group(ahead[0], e);
+#pragma unroll
+    for (int j = 0; j + 1 < P; j++) ahead[j] = ahead[j + 1];
+    ahead[P - 1] = rg[(k / kGroup + P) * kWave];
+    __builtin_amdgcn_sched_barrier(0);  // the prefetch is issued before this group's work
+    float4 rj[kGroup];
+#pragma unroll
+    for (int u = 0; u < kGroup; u++) {
+      if (kStaged) {
+        rj[u] = lds[e[u]];
+      } else if (e[u] < total) {
+        const double4 q = gather_pos(src, ghost, pn, gl[e[u]]);
+        rj[u] = make_float4((float)(q.x - c.x), (float)(q.y - c.y), (float)(q.z - c.z), 0.0f);
+      } else {
+        rj[u] = make_float4(1.0e30f, 1.0e30f, 1.0e30f, 0.0f);
+      }
+    }
+    if (kSum == kSumGroup64) ax = ay = az = 0.f;
+#pragma unroll
+    for (int u = 0; u < kGroup; u++) {
+      const float dx = rj[u].x - ri.x, dy = rj[u].y - ri.y, dz = rj[u].z - ri.z;
+      const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+      const float ir2 = __builtin_amdgcn_rcpf(r2);
+      const float ir6 = ir2 * ir2 * ir2;
+      float df = fmaf(ir2 * ir6, fmaf(-fc.B, ir6, fc.A), fc.C);
+      df = (r2 <= fc.CL2) ? df : 0.0f;
+      if (kSum == kSumKahan) {
+        float y, q;
+        y = fmaf(df, dx, -cx); q = ax + y; cx = (q - ax) - y; ax = q;
+        y = fmaf(df, dy, -cy); q = ay + y; cy = (q - ay) - y; ay = q;
+        y = fmaf(df, dz, -cz); q = az + y; cz = (q - az) - y; az = q;
+      } else {
+        ax = fmaf(df, dx, ax);
+        ay = fmaf(df, dy, ay);
+        az = fmaf(df, dz, az);
+      }
+    }
+    if (kSum == kSumGroup64) {
+      sx += (double)ax;
+      sy += (double)ay;
+      sz += (double)az;
+    }
+    hold_prefetch(ahead[P - 1], ax, ay, az);
+  }
+  if (kSum == kSumGroup64) return make_double3(sx, sy, sz);
+  return make_double3((double)ax - (double)cx, (double)ay - (double)cy, (double)az - (double)cz);
+}
+
 // Where the epilogue also writes the next step's first half drift (q + p dt/2
 // with the final q and p of this step, the same roundings as the next step's
 // UpdatePositionHalf): pos (owned rows) and, for mixed precision, its
@@ -340,6 +405,197 @@ __global__ __launch_bounds__(kWave *W) void k_force_brick_fp64(const double4 *__
   if (vpart) block_max_to_partial(vmax, vpart, b);
 }
 
+// ---------------------------------------------------------------------------
+// Pipelined FP64 kick: the same per-brick work as k_force_brick_fp64, with the
+// staging of the next brick overlapped with the pair loop of the current one.
+//
+// The grid is persistent: workgroup g serves bricks g, g + G, g + 2G, ... (G =
+// gridDim.x, one workgroup per CU).  Each workgroup has W compute wavefronts
+// and P producer wavefronts and two LDS buffers.  While the compute wavefronts
+// walk the slices of brick b from one buffer, the producers copy the
+// neighbourhood of brick b + G into the other with LDS-direct loads
+// (global_load_lds_dword{,x4}: lane l of one load writes at base + l * size,
+// for size 4 or 16 bytes).  A buffer keeps x, y of entry e at xy[e] (16 bytes,
+// one x4 load per 64 entries) and z at z[e] (8 bytes, two lanes per entry, one
+// dword load per 32 entries): 24 bytes per entry, as the one-workgroup-per-brick
+// kernel stages them.  These loads hold no registers, and a producer waits only
+// for its own loads, so the compute wavefronts never wait for the staging
+// (their memory waits count their own loads only).  One barrier per brick hands
+// the buffers over.  Every workgroup runs the same number of iterations, so
+// every wavefront reaches every barrier.
+// ---------------------------------------------------------------------------
+constexpr int kStageChunk = 64;  // entries one producer x4 load moves
+
+struct PipeBuf {
+  double2 *xy;  // [cap] x, y
+  double *z;    // [cap] z
+};
+
+// Copy the positions of gather-list entries [0, total) into buf with LDS-direct
+// loads, chunks of kStageChunk entries spread over the P producer wavefronts
+// (this one is producer pw), plus the far sentinel at entry `total` (the pad
+// entry of the list).  Returns after this wavefront's loads have landed.
+__device__ __forceinline__ void stage_fp64_dma(const int *__restrict__ gl, int total, const double4 *__restrict__ src,
+                                               const double4 *__restrict__ ghost, int pn, PipeBuf buf, int pw,
+                                               int P, int lane) {
+  constexpr int kBatch = 8;  // chunks whose indices are loaded together
+  const int nchunk = (total + kStageChunk - 1) / kStageChunk;
+  for (int c0 = pw; c0 < nchunk; c0 += P * kBatch) {
+    int j[kBatch];
+#pragma unroll
+    for (int k = 0; k < kBatch; k++) {
+      const int u = (c0 + k * P) * kStageChunk + lane;
+      j[k] = u < total ? gl[u] : -1;
+    }
+#pragma unroll
+    for (int k = 0; k < kBatch; k++) {
+      const int c = c0 + k * P;
+      if (c >= nchunk) break;  // uniform
+      const double4 *rec = j[k] >= 0 ? (j[k] < pn ? src : ghost) + j[k] : nullptr;
+      if (rec != nullptr)
+        __builtin_amdgcn_global_load_lds((const void __attribute__((address_space(1))) *)rec,
+                                         (void __attribute__((address_space(3))) *)(buf.xy + kStageChunk * c),
+                                         16, 0, 0);
+      // z: lane l moves dword l & 1 of entry (l >> 1) of each half of the chunk
+#pragma unroll
+      for (int h = 0; h < 2; h++) {
+        const int jz = __shfl(j[k], 32 * h + (lane >> 1), kWave);
+        if (jz >= 0) {
+          const char *g = reinterpret_cast<const char *>((jz < pn ? src : ghost) + jz) + 16 + 4 * (lane & 1);
+          __builtin_amdgcn_global_load_lds((const void __attribute__((address_space(1))) *)g,
+                                           (void __attribute__((address_space(3))) *)(buf.z + kStageChunk * c + 32 * h),
+                                           4, 0, 0);
+        }
+      }
+    }
+  }
+  if (pw == 0 && lane == 0) {
+    buf.xy[total] = make_double2(1.0e200, 1.0e200);
+    buf.z[total] = 1.0e200;
+  }
+  __builtin_amdgcn_s_waitcnt(0);  // this wavefront's loads have landed in LDS
+}
+
+// row_kick_fp64 over a pipelined buffer (x, y and z in two arrays).
+template <int P>
+__device__ __forceinline__ double3 row_kick_fp64_buf(const uint4 *__restrict__ rg, int len, const double4 qi,
+                                                     const PipeBuf buf, const ForceConsts &fc) {
+  double ax = 0.0, ay = 0.0, az = 0.0;
+  uint4 ahead[P];
+#pragma unroll
+  for (int j = 0; j < P; j++) ahead[j] = rg[j * kWave];
+  for (int k = 0; k < len; k += kGroup) {
+    int e[kGroup];
+    unpack_group(ahead[0], e);
+#pragma unroll
+    for (int j = 0; j + 1 < P; j++) ahead[j] = ahead[j + 1];
+    ahead[P - 1] = rg[(k / kGroup + P) * kWave];
+    __builtin_amdgcn_sched_barrier(0);  // the prefetch is issued before this group's work
+    double qx[kGroup], qy[kGroup], qz[kGroup];
+#pragma unroll
+    for (int u = 0; u < kGroup; u++) {
+      const double2 q = buf.xy[e[u]];
+      qx[u] = q.x;
+      qy[u] = q.y;
+      qz[u] = buf.z[e[u]];
+    }
+#pragma unroll
+    for (int u = 0; u < kGroup; u++) {
+      const double dx = qx[u] - qi.x, dy = qy[u] - qi.y, dz = qz[u] - qi.z;
+      const double r2 = fma(dx, dx, fma(dy, dy, dz * dz));
+      const double ir2 = rcp64(r2);
+      const double ir6 = ir2 * ir2 * ir2;
+      double df = fma(ir2 * ir6, fma(-fc.B, ir6, fc.A), fc.C);
+      df = (r2 <= fc.CL2) ? df : 0.0;
+      ax = fma(df, dx, ax);
+      ay = fma(df, dy, ay);
+      az = fma(df, dz, az);
+    }
+    hold_prefetch(ahead[P - 1], ax, ay, az);
+  }
+  return make_double3(ax, ay, az);
+}
+
+template <int W, int U, int P>
+__global__ __launch_bounds__(kWave *(W + P)) void k_force_pipe_fp64(const double4 *__restrict__ src,
+                                                                   double4 *__restrict__ mom,
+                                                                   double4 *__restrict__ pos_out, BrickList bl,
+                                                                   ForceConsts fc, int epilogue, StepClock sc,
+                                                                   const int *__restrict__ id,
+                                                                   double *__restrict__ vpart, NextDrift nd,
+                                                                   int nbrick) {
+  extern __shared__ __align__(16) double lds64[];
+  __shared__ int slice_next[2];
+  __shared__ double wmax[2][W];
+  if (run_halted(sc.gate)) return;
+  const int cap = bl.lds_cap;  // entries per buffer, pad entry included
+  const PipeBuf buf0{reinterpret_cast<double2 *>(lds64), lds64 + 2 * (size_t)cap};
+  const PipeBuf buf1{reinterpret_cast<double2 *>(lds64 + 3 * (size_t)cap), lds64 + 5 * (size_t)cap};
+  const int wv = threadIdx.x / kWave, lane = threadIdx.x % kWave;
+  const bool producer = wv >= W;
+  const int G = gridDim.x;
+  int b = blockIdx.x;
+  // brick b's staging: skipped for a brick without rows or one too large for a buffer
+  auto stage = [&](int bb, const PipeBuf &buf) {
+    if (bb >= nbrick || bl.nrow[bb] == 0) return;
+    const long long g0 = bl.gbase[bb];
+    const int total = (int)(bl.gbase[bb + 1] - g0);
+    if (total >= cap) return;
+    stage_fp64_dma(bl.glist + g0, total, src, bl.ghost, bl.pn, buf, wv - W, P, lane);
+  };
+  if (producer) stage(b, buf0);
+  if (threadIdx.x == 0) slice_next[0] = W;
+  __syncthreads();
+  for (int it = 0; b < nbrick; it++, b += G) {
+    const int cur = it & 1;
+    const PipeBuf lbuf = cur ? buf1 : buf0;
+    if (producer) {
+      stage(b + G, cur ? buf0 : buf1);
+      if (threadIdx.x == W * kWave) slice_next[cur ^ 1] = W;
+    } else {
+      double vmax = 0.0;
+      const int nrow = bl.nrow[b];
+      if (nrow > 0) {
+        const Replica &Rp = bl.reps[bl.brep[b]];
+        const LangevinConsts lc = langevin_consts(Rp, sc.dt, step_of(sc));
+        const int row0 = bl.row0[b], vs0 = bl.vslice0[b];
+        const long long g0 = bl.gbase[b];
+        const int total = (int)(bl.gbase[b + 1] - g0);
+        const int *gl = bl.glist + g0;
+        const bool staged = total < cap;  // uniform across the workgroup
+        const int nsl = (nrow + kWave - 1) / kWave;
+        for (int sl = wv; sl < nsl; sl = take_next(&slice_next[cur], lane)) {
+          const int s = vs0 + sl;
+          const int v = sl * kWave + lane;
+          const bool ok = v < nrow;
+          const int i = row0 + (ok ? v : nrow - 1);
+          const double4 qi = src[i];
+          double4 p = mom[i];
+          const RowExtra x = row_extra(i, epilogue, id, nd);
+          const int len = bl.slen[s];
+          const uint4 *rg = reinterpret_cast<const uint4 *>(bl.nbr + bl.sbase[s] + lane * kGroup);
+          const double3 a = staged ? row_kick_fp64_buf<U>(rg, len, qi, lbuf, fc)
+                                   : row_kick_fp64<false, U>(rg, len, qi, nullptr, src, gl, total, fc, bl.pn, bl.ghost);
+          if (ok) {
+            p.x += a.x;
+            p.y += a.y;
+            p.z += a.z;
+            vmax = fmax(vmax, finish_row(i, p, qi, mom, pos_out, epilogue, fc.dt2, lc, x, nd, Rp));
+          }
+        }
+      }
+      vmax = wave_max(vmax);
+      if (lane == 0) wmax[cur][wv] = vmax;
+    }
+    __syncthreads();  // brick b done; brick b + G staged in the other buffer
+    if (vpart && threadIdx.x == 0) {
+      double m = wmax[cur][0];
+      for (int k = 1; k < W; k++) m = fmax(m, wmax[cur][k]);
+      vpart[b] = m;
+    }
+  }
+}
+
 // Mixed precision kick: FP32 cell-relative coordinates (staged as float4),
 // FP32 pair forces, compensated (Kahan) FP32 sums; the sum is added to the
 // FP64 momentum and the drift is FP64.  The partner's cell offset comes from
@@ -413,6 +669,93 @@ __global__ __launch_bounds__(kWave *W) void k_force_brick_mixed(
       // kernel a workgroup per CU (registers)
       double4 p = mom[i];
       const double4 qi = pos[i];
+      const RowExtra x = row_extra(i, epilogue, id, nd);
+      p.x += a.x;
+      p.y += a.y;
+      p.z += a.z;
+      vmax = fmax(vmax, finish_row(i, p, qi, mom, pos_out, epilogue, dt2, lc, x, nd, Rp));
+    }
+  }
+  if (vpart) block_max_to_partial(vmax, vpart, b);
+}
+
+// Mixed-precision kick in a brick-centred FP32 frame: the workgroup stages its
+// brick's neighbourhood from the FP64 positions as (float)(q - c), c the
+// centre of the brick's cells, 16 bytes per entry; rows use the same frame, so
+// a partner separation is one FP32 subtraction per axis, with no cell offset,
+// and the list's cell-offset codes are not read.  The coordinates stay within
+// about half a neighbourhood (9 at 4x4x4-cell bricks of 3-wide cells), so
+// each carries a rounding of at most 2^-21 to 2^-20; the cell-relative frame
+// (k_force_brick_mixed) rounds 4 times finer at the cost of one offset lookup
+// per pair.  Sums, epilogue and FP64 momenta as in k_force_brick_mixed.
+template <int W, int U, int kSum>
+__global__ __launch_bounds__(kWave *W) void k_force_brick_mixed_brick(
+    const double4 *__restrict__ src, double4 *__restrict__ mom, double4 *__restrict__ pos_out, BrickList bl,
+    ForceConstsF fc, double dt2, int epilogue, StepClock sc, const int *__restrict__ id, double *__restrict__ vpart,
+    NextDrift nd) {
+  extern __shared__ float4 lds32[];
+  __shared__ int slice_next;  // the next slice a wavefront takes
+  constexpr int T = kWave * W;
+  if (run_halted(sc.gate)) return;
+  const int b = blockIdx.x;
+  const Replica &Rp = bl.reps[bl.brep[b]];
+  const LangevinConsts lc = langevin_consts(Rp, sc.dt, step_of(sc));
+  const int nrow = bl.nrow[b], row0 = bl.row0[b], vs0 = bl.vslice0[b];
+  if (nrow == 0) {  // a brick without rows: nothing to stage
+    if (vpart) block_max_to_partial(0.0, vpart, b);
+    return;
+  }
+  const BrickBox X = brick_box(b, Rp);
+  const double3 c = make_double3((X.lo[0] + 0.5 * (X.n[0] - 2)) * Rp.w[0], (X.lo[1] + 0.5 * (X.n[1] - 2)) * Rp.w[1],
+                                 (X.lo[2] + 0.5 * (X.n[2] - 2)) * Rp.w[2]);
+  const long long g0 = bl.gbase[b];
+  const int total = (int)(bl.gbase[b + 1] - g0);
+  const int *gl = bl.glist + g0;
+  const bool staged = total < bl.lds_cap;
+  if (staged) {
+    for (int u0 = threadIdx.x; u0 < total; u0 += T * kStageBatch) {
+      int j[kStageBatch];
+#pragma unroll
+      for (int k = 0; k < kStageBatch; k++) {
+        const int u = u0 + k * T;
+        j[k] = u < total ? gl[u] : 0;
+      }
+      double4 q[kStageBatch];
+#pragma unroll
+      for (int k = 0; k < kStageBatch; k++) q[k] = gather_pos(src, bl.ghost, bl.pn, j[k]);
+#pragma unroll
+      for (int k = 0; k < kStageBatch; k++)
+        if (u0 + k * T < total)
+          lds32[u0 + k * T] =
+              make_float4((float)(q[k].x - c.x), (float)(q[k].y - c.y), (float)(q[k].z - c.z), 0.0f);
+    }
+    if (threadIdx.x == 0) lds32[total] = make_float4(1.0e30f, 1.0e30f, 1.0e30f, 0.0f);
+  }
+  if (threadIdx.x == 0) slice_next = W;
+  __syncthreads();
+  const int nsl = (nrow + kWave - 1) / kWave;
+  const int wv = threadIdx.x / kWave, lane = threadIdx.x % kWave;
+  double vmax = 0.0;
+  // slices come longest first: each wavefront takes the next one as it finishes
+  for (int sl = wv; sl < nsl; sl = take_next(&slice_next, lane)) {
+    const int s = vs0 + sl;
+    const int v = sl * kWave + lane;
+    const bool ok = v < nrow;
+    const int i = row0 + (ok ? v : nrow - 1);
+    float4 ri;
+    {
+      const double4 q = src[i];
+      ri = make_float4((float)(q.x - c.x), (float)(q.y - c.y), (float)(q.z - c.z), 0.0f);
+    }
+    const int len = bl.slen[s];
+    const uint4 *rg = reinterpret_cast<const uint4 *>(bl.nbr + bl.sbase[s] + lane * kGroup);
+    const double3 a =
+        staged ? row_kick_mixed_brick<true, kSum, U>(rg, len, ri, lds32, src, bl.ghost, bl.pn, gl, total, fc, c)
+               : row_kick_mixed_brick<false, kSum, U>(rg, len, ri, lds32, src, bl.ghost, bl.pn, gl, total, fc, c);
+    if (ok) {
+      // loaded after the pair loop (registers)
+      double4 p = mom[i];
+      const double4 qi = src[i];
       const RowExtra x = row_extra(i, epilogue, id, nd);
       p.x += a.x;
       p.y += a.y;

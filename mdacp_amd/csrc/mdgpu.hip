@@ -71,6 +71,13 @@ struct mdgpu_domain {
   int brick_kb = 0;         // LDS budget the automatic bricks are sized for, KiB (0: the default)
   int lds_kb = 0;         // LDS budget for one brick's staged partners, KiB (0: the bricks' budget)
   int lds_max_kb = 60;    // device limit per workgroup less the kernels' static LDS
+  int num_cu = 256;       // compute units (the pipelined kick's persistent grid)
+  int disp_check = 1;     // 1 = PairList::CheckByDisplacement once the budget is spent (pairlist.cc:26-45),
+                          // 0 = the list expires as soon as the budget is spent
+  int mixed_frame = 1;    // mixed kick: 0 = cell-relative FP32 copies with cell-offset codes,
+                          // 1 = brick-centred FP32 frame staged from the FP64 positions (default)
+  int pipeline = 0;       // FP64 kick: 1 = persistent workgroups staging the next brick while
+                          // computing the current one (k_force_pipe_fp64), 0 = a workgroup per brick
   int search_waves = 16;  // wavefronts per brick workgroup of the neighbour search: 4, 8 or 16
   int search_stage = 1;   // neighbour search reads candidates staged in LDS (1) or from global memory (0)
   int row_sort = kSliceRound;  // rows of a brick by list length in steps of this (0: cell order)
@@ -222,6 +229,10 @@ namespace {
     int r_ = (call);               \
     if (r_ != MDGPU_OK) return r_; \
   } while (0)
+
+// The cell-relative FP32 copies (rel, rel_tmp, rel_next) are kept only for
+// the mixed kick in the cell-relative frame.
+inline bool rel_frame(const mdgpu_domain *d) { return d->precision == MDGPU_MIXED && d->mixed_frame == 0; }
 
 inline unsigned nblk(long long n, int b = kBlock) { return (unsigned)std::max<long long>(1, (n + b - 1) / b); }
 
@@ -994,7 +1005,10 @@ int is_expired(mdgpu_domain *d, int *expired) {
     d->buffer_length = d->buffer_length - vmax * 2.0 * d->dt;
   }
   *expired = 0;
-  if (d->buffer_length < 0.0) CHK(check_by_displacement(d, expired));
+  if (d->buffer_length < 0.0) {
+    if (d->disp_check) CHK(check_by_displacement(d, expired));
+    else *expired = 1;  // rebuild cadence without the exact check: the budget alone
+  }
   return MDGPU_OK;
 }
 
@@ -1037,6 +1051,10 @@ int is_expired_ensemble(mdgpu_domain *d, int *expired) {
   }
   *expired = 0;
   if (nflag == 0) return MDGPU_OK;
+  if (!d->disp_check) {  // the budget alone: a spent budget expires the list
+    *expired = 1;
+    return MDGPU_OK;
+  }
   HIPCHK(d, hipMemcpyAsync(d->which, d->h_which, nflag * sizeof(int), hipMemcpyHostToDevice, st));
   const int cap = d->check_list_length;
   hipLaunchKernelGGL(k_zero_check, dim3(nblk(nflag)), dim3(kBlock), 0, st, d->chk_dmax, d->chk_cnt, nflag);
@@ -1115,6 +1133,29 @@ void launch_fp64(mdgpu_domain *d, const double4 *src, double4 *pos_out, int epi,
                      pos_out, bl, fc, epi, sc, d->id, vb, nd);
 }
 
+// Staged entries per buffer of the pipelined kick (two buffers in the LDS of one
+// workgroup), pad entry included.
+int pipe_cap(const mdgpu_domain *d) {
+  const int kb = d->lds_kb ? std::min(d->lds_kb, d->lds_max_kb) : d->lds_max_kb;
+  const int budget = (kb * 1024 / (2 * 3 * (int)sizeof(double))) & ~1;
+  // even: the second buffer's 16-byte xy entries start 16-byte aligned
+  return std::max(2, std::min((d->max_staged + 2) & ~1, budget));
+}
+
+template <int W, int U>
+void launch_fp64_pipe(mdgpu_domain *d, const double4 *src, double4 *pos_out, int epi, const ForceConsts &fc,
+                      const StepClock &sc, double *vb, const NextDrift &nd, bool inner) {
+  constexpr int P = 2;  // producer wavefronts
+  static_assert(W + P <= 16, "at most 1024 threads per workgroup");
+  BrickList bl = brick_list(d, 3 * sizeof(double), inner);
+  bl.lds_cap = pipe_cap(d);
+  const size_t shm = 2 * (size_t)bl.lds_cap * 3 * sizeof(double);
+  allow_lds(k_force_pipe_fp64<W, U, P>, shm);
+  const int grid = std::max(1, std::min(d->nbrick, d->num_cu));
+  hipLaunchKernelGGL((k_force_pipe_fp64<W, U, P>), dim3(grid), dim3(kWave * (W + P)), shm, d->stream, src,
+                     d->mom, pos_out, bl, fc, epi, sc, d->id, vb, nd, d->nbrick);
+}
+
 template <int W, int U, int K>
 void launch_mixed(mdgpu_domain *d, const double4 *src, const float4 *rel, double4 *pos_out, int epi,
                   const ForceConstsF &f32, double dt2, const StepClock &sc, double *vb, const NextDrift &nd,
@@ -1126,10 +1167,28 @@ void launch_mixed(mdgpu_domain *d, const double4 *src, const float4 *rel, double
                      d->mom, pos_out, bl, f32, dt2, epi, sc, d->id, vb, nd);
 }
 
+// the mixed kick in the brick-centred frame (no FP32 copies, no cell codes)
+template <int W, int U, int K>
+void launch_mixed_brick(mdgpu_domain *d, const double4 *src, double4 *pos_out, int epi, const ForceConstsF &f32,
+                        double dt2, const StepClock &sc, double *vb, const NextDrift &nd, bool inner) {
+  const BrickList bl = brick_list(d, sizeof(float4), inner);
+  const size_t shm = (size_t)bl.lds_cap * sizeof(float4);
+  const NextDrift nd64{nd.pos, nullptr, nd.cell, nd.org};
+  allow_lds(k_force_brick_mixed_brick<W, U, K>, shm);
+  hipLaunchKernelGGL((k_force_brick_mixed_brick<W, U, K>), dim3(d->nbrick), dim3(kWave * W), shm, d->stream, src,
+                     d->mom, pos_out, bl, f32, dt2, epi, sc, d->id, vb, nd64);
+}
+
 template <int W, int K>
 void launch_mixed_sum(mdgpu_domain *d, const double4 *src, const float4 *rel, double4 *pos_out, int epi,
                       const ForceConstsF &f32, double dt2, const StepClock &sc, double *vb, const NextDrift &nd,
                       bool inner) {
+  if (d->mixed_frame == 1) {
+    if (d->unroll == 32) launch_mixed_brick<W, 4, K>(d, src, pos_out, epi, f32, dt2, sc, vb, nd, inner);
+    else if (d->unroll == 16) launch_mixed_brick<W, 2, K>(d, src, pos_out, epi, f32, dt2, sc, vb, nd, inner);
+    else launch_mixed_brick<W, 1, K>(d, src, pos_out, epi, f32, dt2, sc, vb, nd, inner);
+    return;
+  }
   if (d->unroll == 32) launch_mixed<W, 4, K>(d, src, rel, pos_out, epi, f32, dt2, sc, vb, nd, inner);
   else if (d->unroll == 16) launch_mixed<W, 2, K>(d, src, rel, pos_out, epi, f32, dt2, sc, vb, nd, inner);
   else launch_mixed<W, 1, K>(d, src, rel, pos_out, epi, f32, dt2, sc, vb, nd, inner);
@@ -1140,7 +1199,12 @@ void launch_force_w(mdgpu_domain *d, const double4 *src, const float4 *rel, doub
                     const ForceConsts &fc, const ForceConstsF &f32, const StepClock &sc, double *vb,
                     const NextDrift &nd, bool in) {
   // groups of kGroup list indices in flight: unroll / kGroup (1, 2 or 4)
-  if (d->precision == MDGPU_FP64) {
+  if (d->precision == MDGPU_FP64 && d->pipeline && W >= 4) {
+    // compute wavefronts: 14 with 16 requested (two producers make 16), else as requested
+    constexpr int WP = W > 14 ? 14 : W;
+    if (d->unroll == 16) launch_fp64_pipe<WP, 2>(d, src, pos_out, epi, fc, sc, vb, nd, in);
+    else launch_fp64_pipe<WP, 1>(d, src, pos_out, epi, fc, sc, vb, nd, in);
+  } else if (d->precision == MDGPU_FP64) {
     if (d->unroll == 32) launch_fp64<W, 4>(d, src, pos_out, epi, fc, sc, vb, nd, in);
     else if (d->unroll == 16) launch_fp64<W, 2>(d, src, pos_out, epi, fc, sc, vb, nd, in);
     else launch_fp64<W, 1>(d, src, pos_out, epi, fc, sc, vb, nd, in);
@@ -1244,7 +1308,7 @@ int refresh(mdgpu_domain *d) {
 // CalculateForce on the current positions (owned and the ghosts of the last
 // halo refresh, all in `pos`)
 int force_in_place(mdgpu_domain *d) {
-  if (d->precision == MDGPU_MIXED && d->tn > 0)
+  if (rel_frame(d) && d->tn > 0)
     hipLaunchKernelGGL(k_to_rel, dim3(nblk(d->tn)), dim3(kBlock), 0, d->stream, d->pos, d->cell, d->pn, d->tn,
                        d->rep, d->gcode, d->reps, d->org, d->rel);
   invalidate(d);
@@ -1287,7 +1351,7 @@ int launch_prune(mdgpu_domain *d) {
 // gate (k_gate), whose launches do nothing once the run has halted.
 int enqueue_fused(mdgpu_domain *d, unsigned long long *clock, bool prune = false, bool gated = false) {
   hipStream_t st = d->stream;
-  const bool mixed = d->precision == MDGPU_MIXED;
+  const bool mixed = rel_frame(d);
   RunGate *gate = gated ? d->gate : nullptr;
   if (gated)
     hipLaunchKernelGGL(k_gate, dim3(1), dim3(kBlock), 0, st, d->gate, d->gbudget, d->glife, d->rep_v2, d->nrep, d->dt,
@@ -1404,7 +1468,7 @@ int calculate_step(mdgpu_domain *d) {
   } else {
     // fused: [drift into pos_tmp +] halo refresh, then kick (+ drift or Langevin)
     // whose epilogue also writes the next step's first half drift into pos_next
-    const bool mixed = d->precision == MDGPU_MIXED;
+    const bool mixed = rel_frame(d);
     if (!d->mid_valid) {
       const int tot = d->pn + d->ng;
       if (tot > 0) {
@@ -1487,7 +1551,7 @@ int run_length(mdgpu_domain *d, int limit, int *k) {
 int run_gated(mdgpu_domain *d, int k, int *done) {
   hipStream_t st = d->stream;
   const int R = d->nrep;
-  const bool mixed = d->precision == MDGPU_MIXED;
+  const bool mixed = rel_frame(d);
   for (int r = 0; r < R; r++) {
     d->h_gbudget[r] = budget_of(d, r);
     d->h_glife[r] = lifetime_of(d, r);
@@ -1619,6 +1683,9 @@ int create_domain(const mdgpu_params *prm, int nrep, const mdgpu_replica *rl, co
     if (hipDeviceGetAttribute(&shm, hipDeviceAttributeMaxSharedMemoryPerBlock, prm->device) == hipSuccess &&
         shm > 8 * 1024)
       d->lds_max_kb = shm / 1024 - 4;
+    int ncu = 0;
+    if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, prm->device) == hipSuccess && ncu > 0)
+      d->num_cu = ncu;
   }
   for (int k = 0; k < 3; k++) {
     d->L[k] = rl ? rl[0].L[k] : prm->L[k];
@@ -2377,6 +2444,18 @@ int mdgpu_set_option(mdgpu_domain *d, int option, int value) {
       if (value != 2 && value != 4 && value != 8 && value != 16) return MDGPU_ERR_ARGUMENT;
       d->waves = value;
       return MDGPU_OK;
+    case MDGPU_OPT_PIPELINE:
+      if (value != 0 && value != 1) return MDGPU_ERR_ARGUMENT;
+      d->pipeline = value;
+      return MDGPU_OK;
+    case MDGPU_OPT_MIXED_FRAME:
+      if (value != 0 && value != 1) return MDGPU_ERR_ARGUMENT;
+      d->mixed_frame = value;
+      return MDGPU_OK;
+    case MDGPU_OPT_DISPLACEMENT_CHECK:
+      if (value != 0 && value != 1) return MDGPU_ERR_ARGUMENT;
+      d->disp_check = value;
+      return MDGPU_OK;
     case MDGPU_OPT_LDS_KB:
       if (value < 8) return MDGPU_ERR_ARGUMENT;
       d->lds_kb = value;
@@ -2453,6 +2532,9 @@ int mdgpu_get_option(const mdgpu_domain *d, int option, int *value) {
     case MDGPU_OPT_BRICK_KB: *value = brick_budget(d); return MDGPU_OK;
     case MDGPU_OPT_SEARCH_CELLS: *value = d->search_cells; return MDGPU_OK;
     case MDGPU_OPT_RUN_AHEAD: *value = d->run_ahead; return MDGPU_OK;
+    case MDGPU_OPT_PIPELINE: *value = d->pipeline; return MDGPU_OK;
+    case MDGPU_OPT_MIXED_FRAME: *value = d->mixed_frame; return MDGPU_OK;
+    case MDGPU_OPT_DISPLACEMENT_CHECK: *value = d->disp_check; return MDGPU_OK;
   }
   return MDGPU_ERR_ARGUMENT;
 }

@@ -32,7 +32,7 @@ EXPORTS = [
 ]
 OPT_UNROLL, OPT_KAHAN, OPT_WAVES, OPT_LDS_KB, OPT_BRICK, OPT_GRAPH = 1, 2, 3, 4, 5, 6
 OPT_SEARCH_WAVES, OPT_SEARCH_STAGE, OPT_PRUNE, OPT_ROW_SORT, OPT_BRICK_KB, OPT_SEARCH_CELLS = 7, 8, 9, 10, 11, 12
-OPT_RUN_AHEAD = 13
+OPT_RUN_AHEAD, OPT_PIPELINE, OPT_MIXED_FRAME, OPT_DISPLACEMENT_CHECK = 13, 14, 15, 16
 
 
 class Params(ctypes.Structure):

@@ -10,7 +10,7 @@ class MockDomain:
     def __init__(self, ensemble=False):
         self.opts = {mdgpu.OPT_SEARCH_WAVES: 16, mdgpu.OPT_SEARCH_STAGE: 1, mdgpu.OPT_WAVES: 8,
                      mdgpu.OPT_LDS_KB: 96, mdgpu.OPT_UNROLL: 8, mdgpu.OPT_BRICK: 4 | 4 << 8 | 4 << 16,
-                     mdgpu.OPT_PRUNE: 0}
+                     mdgpu.OPT_PRUNE: 0, mdgpu.OPT_ROW_SORT: 8, mdgpu.OPT_DISPLACEMENT_CHECK: 1}
         self.clones = []
         self.closed = False
         if ensemble:
@@ -34,6 +34,12 @@ class MockDomain:
     def make_pair_list(self):
         pass
 
+    def kinetic_energy(self):
+        return 0.0
+
+    def observe_replicas_raw(self):
+        return None
+
     def enable_timing(self, on):
         pass
 
@@ -41,7 +47,7 @@ class MockDomain:
         o = self.opts
         b = o[mdgpu.OPT_BRICK] & 0xff
         t = 1.0 + 0.1 * abs(b - 3) + 0.05 * (o[mdgpu.OPT_WAVES] != 4) + 0.01 * abs(o[mdgpu.OPT_LDS_KB] - 72) / 8 \
-            + 0.02 * (o[mdgpu.OPT_UNROLL] != 16)
+            + 0.02 * (o[mdgpu.OPT_UNROLL] != 16) + 0.03 * (o[mdgpu.OPT_ROW_SORT] != 1)
         return dict(force_ms=t * 10, force_launches=10, total_ms=0.0, rebuilds=0, rebuild_ms=0.0)
 
     def close(self):
@@ -56,9 +62,14 @@ def test_picks_the_fastest_shapes(monkeypatch):
         return times.setdefault((w, s), 5.0 + abs(w - 8) * 0.1 + s * 0.3)
 
     monkeypatch.setattr(A, "_rebuild_ms", fake_rebuild)
+    # the cadence without displacement checks is 5% faster per step on this mock
+    monkeypatch.setattr(A, "_step_ms", lambda d, steps: 1.0 - 0.05 * (d.get_option(mdgpu.OPT_DISPLACEMENT_CHECK) == 0))
     d = MockDomain()
     res = A.autotune(d, steps=2)
     ch = res["choice"]
+    assert ch["row_sort"] == 1 and d.opts[mdgpu.OPT_ROW_SORT] == 1
+    assert ch["displacement_check"] == 0 and d.opts[mdgpu.OPT_DISPLACEMENT_CHECK] == 0
+    assert set(res["layout"]) == {str(v) for v in A.LAYOUT} and set(res["cadence"]) == {str(v) for v in A.CADENCE}
     assert (ch["search_waves"], ch["search_stage"]) == (8, 0)
     assert (ch["waves"], ch["lds_kb"], ch["unroll"]) == (4, 72, 16)
     assert ch["brick"] == (3, 3, 3)
@@ -73,6 +84,7 @@ def test_picks_the_fastest_shapes(monkeypatch):
 
 def test_ensembles_keep_their_per_replica_bricks(monkeypatch):
     monkeypatch.setattr(A, "_rebuild_ms", lambda d, reps=3: 1.0)
+    monkeypatch.setattr(A, "_step_ms", lambda d, steps: 1.0)
     d = MockDomain(ensemble=True)
     brick0 = d.opts[mdgpu.OPT_BRICK]
     res = A.autotune(d, steps=2)

@@ -288,9 +288,11 @@ def test_langevin_replicas_follow_their_temperatures():
 
 
 def test_autotune_keeps_results():
-    """The autotuner only changes launch shapes: a tuned ensemble follows an untuned one bit for bit."""
+    """Besides the list layout and the rebuild cadence, which decide when the list is rebuilt,
+    the autotuner changes launch shapes only: an ensemble with the tuned launch shapes follows
+    one with the default shapes bit for bit when both use the tuned layout and cadence."""
     say("autotuned ensemble vs default launch shapes")
-    from mdacp_amd.autotune import autotune, FORCE, SEARCH
+    from mdacp_amd.autotune import autotune, CADENCE, FORCE, LAYOUT, SEARCH
 
     a, _ = ensemble(SPEC, melt_seed=8)
     b, _ = ensemble(SPEC, melt_seed=8)
@@ -298,7 +300,13 @@ def test_autotune_keeps_results():
     ch = res["choice"]
     assert (ch["search_waves"], ch["search_stage"]) in SEARCH
     assert (ch["waves"], ch["lds_kb"], ch["unroll"]) in FORCE
+    assert ch["row_sort"] in LAYOUT and ch["displacement_check"] in CADENCE
     assert a.get_option(mdgpu.OPT_WAVES) == ch["waves"] and a.get_option(mdgpu.OPT_UNROLL) == ch["unroll"]
+    assert a.get_option(mdgpu.OPT_DISPLACEMENT_CHECK) == ch["displacement_check"]
+    say("cadence timings", res["cadence"], "layout timings", res["layout"])
+    if b.get_option(mdgpu.OPT_ROW_SORT) != ch["row_sort"]:
+        b.set_option(mdgpu.OPT_ROW_SORT, ch["row_sort"])
+    b.set_option(mdgpu.OPT_DISPLACEMENT_CHECK, ch["displacement_check"])
     ra, rb = a.calculate(60), b.calculate(60)
     assert ra == rb
     qa, pa, ta = a.particles()

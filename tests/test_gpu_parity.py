@@ -290,13 +290,14 @@ def test_piecewise_api_equals_fused_step():
     assert np.array_equal(qa, qb) and np.array_equal(pa, pb)
 
 
-@pytest.mark.parametrize("thermostat,precision", [("nve", mdgpu.FP64), ("langevin", mdgpu.FP64),
-                                                  ("nve", mdgpu.MIXED)])
-def test_runs_ahead_match_steps_one_at_a_time(thermostat, precision):
+@pytest.mark.parametrize("thermostat,precision,frame", [("nve", mdgpu.FP64, 0), ("langevin", mdgpu.FP64, 0),
+                                                        ("nve", mdgpu.MIXED, 0), ("nve", mdgpu.MIXED, 1),
+                                                        ("langevin", mdgpu.MIXED, 1)])
+def test_runs_ahead_match_steps_one_at_a_time(thermostat, precision, frame):
     """Steps queued ahead of the host (the device-side validity gate, MDGPU_OPT_RUN_AHEAD) give
     the same bits, time and constructions as steps that each wait for the host: the gate does
     the budget arithmetic the host does and stops the run where the host must take over."""
-    say("case: run ahead", thermostat, precision)
+    say("case: run ahead", thermostat, precision, frame)
     L = 16.0
     x = fcc(L, 0.5)
     n = len(x)
@@ -304,6 +305,7 @@ def test_runs_ahead_match_steps_one_at_a_time(thermostat, precision):
     ds = []
     for ahead in (64, 0):
         d = mdgpu.Domain(L, dt=0.005, precision=precision, capacity=n)
+        d.set_option(mdgpu.OPT_MIXED_FRAME, frame)
         d.add_particles(x, p)
         if thermostat == "langevin":
             d.set_thermostat(mdgpu.LANGEVIN, 1.2, 1.0, 0.1)
@@ -343,12 +345,17 @@ def test_runs_are_bitwise_deterministic():
     assert all(np.array_equal(a, b) for a, b in zip(outs[0], outs[1]))
 
 
-def test_mixed_precision_force_error_is_bounded():
-    say("case: mixed precision force error")
+@pytest.mark.parametrize("frame,bound", [(0, 2e-6), (1, 4e-6)])
+def test_mixed_precision_force_error_is_bounded(frame, bound):
+    """Mixed-precision kicks against FP64 on a fluid: RMS relative error below 2e-6 in the
+    cell-relative frame and below 4e-6 in the brick-centred frame, whose coordinates are up to
+    about half a neighbourhood wide (rounding 2^-21..2^-20 instead of 2^-23..2^-22)."""
+    say("case: mixed precision force error, frame", frame)
     L = 20.0
     xx, pp, ids = melted_state(L, 0.5, steps=200, seed=4)
     d64 = mdgpu.Domain(L, dt=0.001)
     d32 = mdgpu.Domain(L, dt=0.001, precision=mdgpu.MIXED)
+    d32.set_option(mdgpu.OPT_MIXED_FRAME, frame)
     for d in (d64, d32):
         d.add_particles(xx, pp, ids)
         d.make_pair_list()
@@ -356,7 +363,7 @@ def test_mixed_precision_force_error_is_bounded():
     k32 = d32.kick()
     rms = math.sqrt(float(np.sum((k32 - k64) ** 2) / np.sum(k64 ** 2)))
     say("mixed force RMS relative error", rms)
-    assert rms < 2e-6
+    assert rms < bound
     assert np.array_equal(O.canonical_pairs(d64.pairs()), O.canonical_pairs(d32.pairs()))
 
 
@@ -562,3 +569,72 @@ def test_pruned_inner_list_langevin_and_nose_hoover():
         qa, pa, ta = a.particles()
         qb, pb, tb = b.particles()
         assert np.array_equal(qa, qb) and np.array_equal(pa, pb), kind
+
+
+@pytest.mark.parametrize("lds_kb", [0, 24])
+def test_pipelined_kick_is_bitwise_identical(lds_kb):
+    """The pipelined FP64 kick (MDGPU_OPT_PIPELINE: persistent workgroups whose producer
+    wavefronts stage the next brick with LDS-direct loads) sums every row's partners in the
+    same order as the one-workgroup-per-brick kick: NVE and Langevin runs with rebuilds give
+    the same bits, also where bricks overflow the LDS buffers (a 24 KiB budget: partners read
+    from global memory) and with the pruned inner list."""
+    say("case: pipelined kick, lds_kb", lds_kb)
+    L = 30.0
+    xx, pp, ids = melted_state(L, 0.55, steps=150, seed=8)
+    for kind, prune in ((mdgpu.NVE, 0), (mdgpu.LANGEVIN, 0), (mdgpu.NVE, 150)):
+        a = mdgpu.Domain(L, dt=0.002)
+        b = mdgpu.Domain(L, dt=0.002)
+        for d in (a, b):
+            d.add_particles(xx, pp, ids)
+            d.set_thermostat(kind, 1.1, 0.5, 0.1)
+            d.set_seed(5)
+            if lds_kb:
+                d.set_option(mdgpu.OPT_LDS_KB, lds_kb)
+            if prune:
+                d.set_option(mdgpu.OPT_PRUNE, prune)
+            d.make_pair_list()
+        b.set_option(mdgpu.OPT_PIPELINE, 1)
+        assert b.get_option(mdgpu.OPT_PIPELINE) == 1
+        ra = sum(a.calculate(60) for _ in range(5))
+        rb = sum(b.calculate(60) for _ in range(5))
+        assert ra == rb and ra >= 2, (ra, rb)
+        qa, pa, _ = a.particles_by_id()
+        qb, pb, _ = b.particles_by_id()
+        assert np.array_equal(qa, qb) and np.array_equal(pa, pb), (kind, prune)
+        assert np.array_equal(a.kick(), b.kick())
+        a.close()
+        b.close()
+
+
+def test_rebuild_cadence_without_displacement_check():
+    """MDGPU_OPT_DISPLACEMENT_CHECK 0: the list expires as soon as the Verlet budget is spent,
+    without the exact displacement check.  That rebuilds at least as often and lists the same
+    pairs within rc, so the trajectory differs only by the summation order of the kicks, and
+    a list built at any step is the oracle's."""
+    say("case: rebuild cadence without displacement check")
+    L = 20.0
+    xx, pp, ids = melted_state(L, 0.5, steps=200, seed=6)
+    a = mdgpu.Domain(L, dt=0.002)
+    b = mdgpu.Domain(L, dt=0.002)
+    for d in (a, b):
+        d.add_particles(xx, pp, ids)
+        d.make_pair_list()
+    b.set_option(mdgpu.OPT_DISPLACEMENT_CHECK, 0)
+    assert b.get_option(mdgpu.OPT_DISPLACEMENT_CHECK) == 0
+    ra = rb = 0
+    for _ in range(4):
+        ra += a.calculate(50)
+        rb += b.calculate(50)
+    assert rb > ra >= 1, (ra, rb)
+    qa, pa, _ = a.particles_by_id()
+    qb, pb, _ = b.particles_by_id()
+    # positions are wrapped at constructions, which fall on different steps here
+    dq = np.abs(np.remainder(qa - qb + 0.5 * L, L) - 0.5 * L)
+    assert dq.max() < 1e-9 and np.abs(pa - pb).max() < 1e-8
+    q, p, t = b.particles()
+    o = O.OracleSystem(L, 0.002, 2.5, 0.5)
+    o.add(wrap_like_reference(q, L), p, t)
+    b.make_pair_list()
+    o.make_pair_list()
+    assert np.array_equal(O.canonical_pairs(b.pairs()), O.canonical_pairs(o.pairs()))
+    say("constructions with the check", ra, "without", rb)
