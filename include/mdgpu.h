@@ -44,6 +44,34 @@
  * above -- with one pair-force launch per step for all of them.  Each replica
  * keeps its own PairList budget and displacement check (pairlist.cc:26-121);
  * when the list of any replica expires, the lists of all replicas are rebuilt.
+ *
+ * Units of a process grid (MDManager with several MDUnits in one process,
+ * mdmanager.cc:34-45, e.g. OMP_NUM_THREADS > 1): mdgpu_create_unit makes one
+ * domain per unit, each the box [s, e) of the grid, joined in a group that
+ * shares one device and one stream.  Particles migrate and ghosts are placed
+ * device to device between the units of the group:
+ *   mdgpu_unit_send_particles     MDUnit::MakeBufferForSendingParticle (mdunit.cc:78-101)
+ *   mdgpu_unit_receive_particles  MDUnit::ReceiveParticles (mdunit.cc:104-122) from the
+ *                                 sending unit's buffer
+ *   mdgpu_unit_adjust_periodic_boundary  MDUnit::AdjustPeriodicBoundary
+ *                                 (variables.cc:59-79), then the unit's owned particles
+ *                                 binned (MeshList::MakeMesh, meshlist.cc:153-205)
+ *   mdgpu_unit_find_border_particles  the six FindBorderParticles /
+ *                                 MakeBufferForBorderParticles / ReceiveBorderParticles
+ *                                 passes of MDManager::MakePairList (mdmanager.cc:269-275,
+ *                                 mdunit.cc:125-191): every ghost the relay would bring,
+ *                                 straight from the unit that owns the particle
+ *   mdgpu_make_pair_list          MDUnit::MakePairList (mdunit.cc:194-198) of the unit
+ *   mdgpu_send_border_particles   MDManager::SendBorderParticles (mdmanager.cc:283-348)
+ *                                 for the unit: its ghosts from their owners' positions
+ *   mdgpu_group_make_pair_list    MDManager::MakePairList (mdmanager.cc:262-280) for every
+ *                                 unit of the group, in the reference's order
+ * MDManager calls each unit's operations between its phase barriers
+ * (omp parallel for): a unit's work is queued on the group's stream after the
+ * work of every unit's previous phase.  The other step operations
+ * (mdgpu_update_position_half, mdgpu_calculate_force, the thermostats,
+ * mdgpu_observe) act on one unit as on a single domain; the fused
+ * mdgpu_calculate and mdgpu_change_scale are refused for a unit.
  */
 #ifndef MDGPU_H
 #define MDGPU_H
@@ -105,6 +133,30 @@ int mdgpu_create(const mdgpu_params *prm, mdgpu_domain **out);
 int mdgpu_create_ensemble(const mdgpu_params *prm, int thermostat, int nrep, const mdgpu_replica *reps,
                           const int *counts, const double *q, const double *p, const int *type,
                           mdgpu_domain **out);
+/* One unit of a process grid (ParaInfo's grid, parainfo.cc): units per axis,
+   the unit's grid position and its box [s, e) (MDUnit::MDUnit, mdunit.cc:8-27). */
+typedef struct {
+  int grid[3];
+  int pos[3];
+  double s[3];
+  double e[3];
+} mdgpu_unit;
+
+/* A unit of the grid, with prm's parameters (prm->L the whole box).  join:
+   NULL for the first unit of a group, else any unit of the group (same grid,
+   box and device, another grid position; at most 64 units).  Each edge of the
+   unit's box must hold two search lengths (rc + margin).  A unit keeps its
+   rows in cell order (MDGPU_OPT_ROW_SORT 0). */
+int mdgpu_create_unit(const mdgpu_params *prm, const mdgpu_unit *u, mdgpu_domain *join, mdgpu_domain **out);
+/* dir: D_LEFT, D_RIGHT, D_BACK, D_FORWARD, D_DOWN, D_UP = 0..5 (mdconfig.h:25);
+   nsend: particles now in the unit's send buffer */
+int mdgpu_unit_send_particles(mdgpu_domain *d, int dir, int *nsend);
+int mdgpu_unit_receive_particles(mdgpu_domain *d, mdgpu_domain *src);
+int mdgpu_unit_adjust_periodic_boundary(mdgpu_domain *d);
+/* needs every unit of the group through mdgpu_unit_adjust_periodic_boundary */
+int mdgpu_unit_find_border_particles(mdgpu_domain *d);
+int mdgpu_group_make_pair_list(mdgpu_domain *d);
+
 /* replicas of a domain (1 for mdgpu_create) and owned particles of each;
    mdgpu_get_particles returns them replica after replica */
 int mdgpu_replica_number(const mdgpu_domain *d, int *nrep);

@@ -15,12 +15,13 @@
 
 namespace mdgpu {
 
-// MeshList's cell of a position (meshlist.cc:164-182): int((q - s)/w) + 1 per
-// axis with s = 0, as a global cell index.
+// MeshList's cell of a position (meshlist.cc:164-182): int((q - s) * (1/w)) + 1
+// per axis with s the box origin (0 for a periodic box, where q - 0.0 is q), as
+// a global cell index.
 __device__ __forceinline__ int cell_of(double x, double y, double z, const Replica &R) {
-  int ix = (int)(x * R.im[0]) + 1;
-  int iy = (int)(y * R.im[1]) + 1;
-  int iz = (int)(z * R.im[2]) + 1;
+  int ix = (int)((x - R.o[0]) * R.im[0]) + 1;
+  int iy = (int)((y - R.o[1]) * R.im[1]) + 1;
+  int iz = (int)((z - R.o[2]) * R.im[2]) + 1;
   if (ix < 0) ix = R.m[0] - 1; else if (ix >= R.m[0]) ix = 0;
   if (iy < 0) iy = R.m[1] - 1; else if (iy >= R.m[1]) iy = 0;
   if (iz < 0) iz = R.m[2] - 1; else if (iz >= R.m[2]) iz = 0;
@@ -219,6 +220,189 @@ __global__ void k_set_sentinel(double4 *a, double4 *b, float4 *ra, float4 *rb, i
     if (ra) ra[idx] = make_float4(1.0e30f, 1.0e30f, 1.0e30f, 0.0f);
     if (rb) rb[idx] = make_float4(1.0e30f, 1.0e30f, 1.0e30f, 0.0f);
   }
+}
+
+// ---- units of a process grid: MDManager with several MDUnits in one process
+// (mdmanager.cc:194-348), each unit a box [s, e) of the grid on the same device.
+
+// Variables::AdjustPeriodicBoundary (variables.cc:59-79) with the whole box's
+// edges: one +-L per component, then a position still outside [0, L] (or NaN)
+// is flagged.
+__global__ void k_unit_wrap(double4 *__restrict__ pos, int n, double Lx, double Ly, double Lz,
+                            int *__restrict__ err) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double4 q = pos[i];
+  if (q.x < 0.0) q.x += Lx; else if (q.x > Lx) q.x -= Lx;
+  if (q.y < 0.0) q.y += Ly; else if (q.y > Ly) q.y -= Ly;
+  if (q.z < 0.0) q.z += Lz; else if (q.z > Lz) q.z -= Lz;
+  pos[i] = q;
+  if (!(q.x >= 0.0 && q.x <= Lx && q.y >= 0.0 && q.y <= Ly && q.z >= 0.0 && q.z <= Lz))
+    atomicOr(err, kErrInvalidPosition);
+}
+
+// MDRect::IsOverBoundary (mdrect.cc:4-21) of direction dir (D_LEFT, D_RIGHT,
+// D_BACK, D_FORWARD, D_DOWN, D_UP = 0..5: axis dir / 2, the low face for even
+// dir) for a unit whose box spans [s, e) along that axis.
+__global__ void k_leave_flags(const double4 *__restrict__ pos, int n, int dir, double s, double e,
+                              int *__restrict__ flag) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double4 q = pos[i];
+  const int a = dir >> 1;
+  const double v = a == 0 ? q.x : (a == 1 ? q.y : q.z);
+  flag[i] = (dir & 1) ? (v >= e) : (v < s);
+}
+
+// MDUnit::MakeBufferForSendingParticle (mdunit.cc:78-101): the particles that
+// leave go to the send buffer in their order (off = exclusive scan of the
+// flags), the others are compacted in theirs.
+__global__ void k_partition(const int *__restrict__ flag, const int *__restrict__ off, int n,
+                            const double4 *__restrict__ pos, const double4 *__restrict__ mom,
+                            const int *__restrict__ id, double4 *__restrict__ pos_keep,
+                            double4 *__restrict__ mom_keep, int *__restrict__ id_keep,
+                            double4 *__restrict__ pos_send, double4 *__restrict__ mom_send,
+                            int *__restrict__ id_send) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int o = off[i];
+  if (flag[i]) {
+    pos_send[o] = pos[i];
+    mom_send[o] = mom[i];
+    id_send[o] = id[i];
+  } else {
+    pos_keep[i - o] = pos[i];
+    mom_keep[i - o] = mom[i];
+    id_keep[i - o] = id[i];
+  }
+}
+
+// One unit of the group as a source of another unit's ghosts.  Along each
+// axis the source is in the receiver's slab (same grid coordinate), or its
+// left or right neighbour, or several of these at once (a grid of one or two
+// units along the axis).  What the reference relays through six sequential
+// FindBorderParticles / SendBorderParticlesSub passes (mdmanager.cc:269-275,
+// mdunit.cc:125-191) is then, per axis and independently: the position
+// itself from the same slab; from the left neighbour, a particle strictly
+// within SL of that neighbour's high face (MDRect::IsInsideEdge D_RIGHT,
+// mdrect.cc:30-31), shifted by -L when the neighbour's D_RIGHT crosses the
+// periodic boundary (ParaInfo::IsOverBoundary, parainfo.cc:256-275); from the
+// right neighbour, strictly within SL of its low face (D_LEFT), shifted by +L
+// when that crosses.  The ghosts are every combination of one option per axis
+// except the particle itself; a ghost relayed through other units carries the
+// same position, since each relay adds 0 or +-L to one component only.
+struct SrcUnit {
+  const double4 *pos;     // the source's owned particles
+  int n, base;            // their number and first entry in the concatenation
+  int same[3];            // same grid coordinate as the receiver along the axis
+  int left[3], right[3];  // the source is the receiver's left / right neighbour
+  int lsh[3], rsh[3];     // image shift of what comes from the left / right: -1, 0, +1
+  double s[3], e[3];      // the source's box, for its border tests
+};
+constexpr int kMaxUnits = 64;  // units in one group
+
+// The options of one axis in the order k_emit_images takes its shifts (0,
+// +1, -1 for a unit that is its own neighbour), so that a grid of one unit
+// places its ghosts exactly as a periodic domain does.
+__device__ __forceinline__ int unit_axis_options(const SrcUnit &S, int a, double v, double SL, int sh[3]) {
+  int k = 0;
+  if (S.same[a]) sh[k++] = 0;
+  if (S.right[a] && v < S.s[a] + SL && v > S.s[a]) sh[k++] = S.rsh[a];
+  if (S.left[a] && v > S.e[a] - SL && v < S.e[a]) sh[k++] = S.lsh[a];
+  return k;
+}
+
+__device__ __forceinline__ int unit_source(const SrcUnit *__restrict__ tab, int nsrc, int t) {
+  int w = nsrc - 1;
+  while (w > 0 && t < tab[w].base) w--;
+  return w;
+}
+
+__global__ void k_unit_count(const SrcUnit *__restrict__ tab, int nsrc, int total, int self, double SL,
+                             int *__restrict__ cnt) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= total) return;
+  const int w = unit_source(tab, nsrc, t);
+  const SrcUnit &S = tab[w];
+  const double4 q = S.pos[t - S.base];
+  int sh[3];
+  const int nx = unit_axis_options(S, 0, q.x, SL, sh);
+  const int ny = unit_axis_options(S, 1, q.y, SL, sh);
+  const int nz = unit_axis_options(S, 2, q.z, SL, sh);
+  cnt[t] = nx * ny * nz - (w == self ? 1 : 0);
+}
+
+// The images of k_unit_count's count, at off[t]: source index and a word
+// holding the shift code and the source unit (ghost_word), the position, the
+// receiver's cell and a slot in it.  The shift is added as image_of adds it.
+__global__ void k_unit_emit(const SrcUnit *__restrict__ tab, int nsrc, int total, int self, double SL,
+                            const int *__restrict__ off, int ng, double Lx, double Ly, double Lz,
+                            const Replica *__restrict__ reps, int *__restrict__ gsrc, int *__restrict__ gword,
+                            double4 *__restrict__ gpos, int *__restrict__ gcell, int *__restrict__ gslot,
+                            int *__restrict__ gcount) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= total) return;
+  const int w = unit_source(tab, nsrc, t);
+  const SrcUnit &S = tab[w];
+  const int i = t - S.base;
+  const double4 q = S.pos[i];
+  int ox[3], oy[3], oz[3];
+  const int nx = unit_axis_options(S, 0, q.x, SL, ox);
+  const int ny = unit_axis_options(S, 1, q.y, SL, oy);
+  const int nz = unit_axis_options(S, 2, q.z, SL, oz);
+  const Replica &R = reps[0];
+  const double L[3] = {Lx, Ly, Lz};
+  int k = off[t];
+  for (int a = 0; a < nx; a++)
+    for (int b = 0; b < ny; b++)
+      for (int c = 0; c < nz; c++) {
+        const int sx = ox[a], sy = oy[b], sz = oz[c];
+        if (w == self && !sx && !sy && !sz) continue;  // the particle itself
+        if (k >= ng) return;  // cannot happen: off is the scan of k_unit_count
+        const int code = (sx + 1) + 3 * (sy + 1) + 9 * (sz + 1);
+        const double3 y = image_of(q, code, L);
+        gsrc[k] = i;
+        gword[k] = ghost_word(code, w);
+        gpos[k] = make_double4(y.x, y.y, y.z, 0.0);
+        const int cl = cell_of(y.x, y.y, y.z, R);
+        gcell[k] = cl;
+        gslot[k] = cell_slot(gcount, cl);
+        k++;
+      }
+}
+
+// k_gather_ghosts for a unit: the shift code goes to gcode (replica 0), the
+// source unit to gunit.
+__global__ void k_gather_unit_ghosts(const int *__restrict__ perm, int ng, int pn, const int *__restrict__ src_in,
+                                     const int *__restrict__ word_in, const double4 *__restrict__ gpos_in,
+                                     const int *__restrict__ gcell_in, int *__restrict__ gsrc,
+                                     int *__restrict__ gcode, int *__restrict__ gunit, double4 *__restrict__ pos,
+                                     int *__restrict__ cell) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= ng) return;
+  const int g = perm[k];
+  gsrc[k] = src_in[g];
+  gcode[k] = ghost_shift(word_in[g]);
+  gunit[k] = ghost_rep(word_in[g]);
+  pos[pn + k] = gpos_in[g];
+  cell[pn + k] = gcell_in[g];
+}
+
+// Owned positions of every unit of a group, by unit slot.
+struct UnitPos {
+  const double4 *pos[kMaxUnits];
+};
+
+// MDManager::SendBorderParticles for a unit (mdmanager.cc:283-348): every
+// ghost from its source unit's current position plus its shift.
+__global__ void k_unit_refresh(double4 *__restrict__ pos, const int *__restrict__ gsrc,
+                               const int *__restrict__ gcode, const int *__restrict__ gunit, int pn, int ng,
+                               UnitPos P, double Lx, double Ly, double Lz) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= ng) return;
+  const double L[3] = {Lx, Ly, Lz};
+  const double3 y = image_of(P.pos[gunit[k]][gsrc[k]], gcode[k], L);
+  pos[pn + k] = make_double4(y.x, y.y, y.z, 0.0);
 }
 
 }  // namespace mdgpu

@@ -84,6 +84,7 @@ __global__ void k_reduce_max(const double *__restrict__ partial, int n, unsigned
 // mdunit.cc:17-24); its extended cell grid is MeshList's (meshlist.cc:31-55).
 struct Replica {
   double L[3];     // box edges
+  double o[3];     // box origin (MDRect's start): cells count from it; 0 for a periodic box
   double im[3];    // 1 / cell width
   double w[3];     // cell width
   double hb_D;     // Langevin noise amplitude sqrt(2 gamma T / dt) (fcalculator.cc:766)

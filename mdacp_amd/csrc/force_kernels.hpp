@@ -706,8 +706,9 @@ __global__ __launch_bounds__(kWave *W) void k_force_brick_mixed_brick(
     return;
   }
   const BrickBox X = brick_box(b, Rp);
-  const double3 c = make_double3((X.lo[0] + 0.5 * (X.n[0] - 2)) * Rp.w[0], (X.lo[1] + 0.5 * (X.n[1] - 2)) * Rp.w[1],
-                                 (X.lo[2] + 0.5 * (X.n[2] - 2)) * Rp.w[2]);
+  const double3 c = make_double3(Rp.o[0] + (X.lo[0] + 0.5 * (X.n[0] - 2)) * Rp.w[0],
+                                 Rp.o[1] + (X.lo[1] + 0.5 * (X.n[1] - 2)) * Rp.w[1],
+                                 Rp.o[2] + (X.lo[2] + 0.5 * (X.n[2] - 2)) * Rp.w[2]);
   const long long g0 = bl.gbase[b];
   const int total = (int)(bl.gbase[b + 1] - g0);
   const int *gl = bl.glist + g0;

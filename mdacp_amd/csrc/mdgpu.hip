@@ -53,6 +53,22 @@ struct StepGraph {
   const void *key = nullptr;  // the half-drifted positions it reads
   long epoch = -1;
 };
+
+// The units of one process grid (MDManager's MDUnits of one process,
+// mdmanager.cc:34-45) on one device: they share one stream, so that work
+// each unit queues after MDManager's phase barriers runs after what every
+// unit queued before them, and each unit's ghosts come straight from the
+// owned particles of the units they image.
+struct UnitGroup {
+  std::mutex mu;  // the slot table, and the host-side pointers of units read by other units
+  hipStream_t stream = nullptr;
+  int device = 0;
+  double L[3] = {0.0, 0.0, 0.0};  // the whole box
+  int grid[3] = {1, 1, 1};        // units per axis
+  std::vector<mdgpu_domain *> units;  // by slot; nullptr once destroyed
+  int live = 0;
+  long epoch = 0;  // bumped when a unit is destroyed: ghosts placed before then may image it
+};
 }  // namespace
 
 struct mdgpu_domain {
@@ -110,6 +126,21 @@ struct mdgpu_domain {
   unsigned long long *chk_dmax = nullptr;  // displacement check of listed replicas: max |dq|^2 bits,
   int *chk_cnt = nullptr, *chk_cand = nullptr;  // candidate counts, candidates (cap per replica)
   bool clock_dirty = true;              // clock differs from `step`
+
+  // unit of a process grid (mdgpu_create_unit); none for a periodic domain or an ensemble
+  UnitGroup *group = nullptr;
+  int unit_slot = -1;          // index in the group: a ghost's source unit
+  int upos[3] = {0, 0, 0};     // position in the grid
+  bool own_stream = true;      // false: the group's stream
+  bool ghosts_ready = false;   // ghosts placed for the current owned particles and their order
+  long ghost_epoch = -1;       // the group's epoch when they were placed
+  int *gunit = nullptr;        // source unit of each ghost
+  double4 *st_pos = nullptr, *st_mom = nullptr;  // send buffer of the last MakeBufferForSendingParticle
+  int *st_id = nullptr;
+  int st_n = 0, st_cap = 0;
+  SrcUnit *src_tab = nullptr, *h_src_tab = nullptr;  // ghost sources (device; pinned mirror)
+  int *ucnt = nullptr, *uoff = nullptr;  // per source particle: images, first image
+  long long ucap = 0;
 
   // captured fused steps, one per parity of the position buffers
   bool use_graph = true;
@@ -314,6 +345,7 @@ int set_geometry(mdgpu_domain *d) {
   if (d->nrep == 1)
     for (int k = 0; k < 3; k++) {
       d->rh[0].L[k] = d->re[k] - d->rs[k];
+      d->rh[0].o[k] = d->rs[k];
       d->rh[0].bd[k] = d->bd[k];
     }
   long long ncell = 0, nbrick = 0;
@@ -342,9 +374,11 @@ int set_geometry(mdgpu_domain *d) {
     // over, which is then exactly the difference of the two origins: a partner
     // separation carries no per-cell bias, only the rounding of the two
     // coordinates.  The cells themselves (binning) keep the FP64 width.
+    // A unit's box starts at R.o: its origins are R.o + (c - 1) * w32, and a
+    // partner separation then carries at most the rounding of those sums.
     for (int k = 0; k < 3; k++) {
       const double w32 = (double)(float)R.w[k];
-      for (int c = 0; c < R.m[k]; c++) org.push_back((double)(c - 1) * w32);
+      for (int c = 0; c < R.m[k]; c++) org.push_back(R.o[k] + (double)(c - 1) * w32);
     }
     ncell += (long long)R.m[0] * R.m[1] * R.m[2];
     nbrick += (long long)R.nb[0] * R.nb[1] * R.nb[2];
@@ -501,6 +535,7 @@ int ensure_ghosts(mdgpu_domain *d, int need) {
   CHK(dalloc(d, &d->key, tc_new));
   CHK(dalloc(d, &d->gsrc, ngc));
   CHK(dalloc(d, &d->gcode, ngc));
+  CHK(dalloc(d, &d->gunit, ngc));
   CHK(dalloc(d, &d->t_gsrc, ngc));
   CHK(dalloc(d, &d->t_gcode, ngc));
   CHK(dalloc(d, &d->t_gcell, ngc));
@@ -658,11 +693,15 @@ int launch_emit(mdgpu_domain *d, const Bricks &B, const SearchTables &t) {
 // Rows of each brick by decreasing list length: permute the owned particles
 // (positions, momenta, ids, cells, row lengths) and remap the indices that
 // point at them (gather lists, ghost sources).  Ghosts keep their places.
+// With row_sort 0 the rows stay in cell order (rowsrc is the identity) and
+// nothing moves: a unit's owned particles keep the indices other units'
+// ghosts refer to.
 int order_rows(mdgpu_domain *d) {
   hipStream_t st = d->stream;
   const int pn = d->pn, ng = d->ng;
   hipLaunchKernelGGL(k_brick_row_order, dim3(d->nbrick), dim3(kBlock), 0, st, d->row_len, d->row0, d->nrow,
                      d->rowsrc, d->rowinv, d->row_sort);
+  if (d->row_sort == 0) return check_launch(d, "row order");
   hipLaunchKernelGGL(k_gather_owned, dim3(nblk(pn)), dim3(kBlock), 0, st, d->rowsrc, pn, d->pos, d->mom, d->id,
                      d->cell, d->rep, d->pos_tmp, d->mom_alt, d->id_alt, d->cell_alt, d->rep_alt);
   hipLaunchKernelGGL(k_gather_int, dim3(nblk(pn)), dim3(kBlock), 0, st, d->rowsrc, pn, d->row_len, d->nimg);
@@ -748,12 +787,11 @@ int brick_budget(const mdgpu_domain *d) {
 // occupancy the bricks were sized for (two per CU at 76 KiB, one at 96).
 int force_lds_kb(const mdgpu_domain *d) { return d->lds_kb ? d->lds_kb : brick_budget(d); }
 
-// MDManager::MakePairList for the single domain (mdmanager.cc:262-280).
-// The reference also permutes particles that cross a face
-// (MakeBufferForSendingParticle, mdunit.cc:78-101); that only renames
-// indices, and particles are put in brick order here anyway.
-int make_pair_list(mdgpu_domain *d) {
-  hipStream_t st = d->stream;
+bool is_unit(const mdgpu_domain *d) { return d->group != nullptr; }
+
+// A single domain (or unit) picks its bricks from its density at its first
+// construction, unless the caller chose them.
+int auto_bricks(mdgpu_domain *d) {
   if (d->nrep == 1 && d->brick_auto && d->pn > 0) {
     d->brick_auto = false;
     const double rho = d->pn / (d->rh[0].L[0] * d->rh[0].L[1] * d->rh[0].L[2]);
@@ -764,16 +802,23 @@ int make_pair_list(mdgpu_domain *d) {
       CHK(set_geometry(d));
     }
   }
+  return MDGPU_OK;
+}
+
+// Bin the owned particles (after the periodic wrap of each replica's box when
+// `wrap`) and put them in brick order, each cell's members by particle id.
+int bin_owned(mdgpu_domain *d, int wrap) {
+  hipStream_t st = d->stream;
   invalidate(d);  // particles are reordered and wrapped
   const int pn = d->pn;
   d->ng = 0;
   d->tn = pn;
-  HIPCHK(d, hipMemsetAsync(d->i32, 0, sizeof(int) * 4, st));
-  // 1. wrap (AdjustPeriodicBoundary) and bin the owned particles, brick order
+  d->initialized = false;
+  d->ghosts_ready = false;
   HIPCHK(d, hipMemsetAsync(d->c_ocount, 0, sizeof(int) * d->ncell, st));
   if (pn > 0) {
     hipLaunchKernelGGL(k_wrap_bin, dim3(nblk(pn)), dim3(kBlock), 0, st, d->pos, pn, d->rep, d->reps, d->cell,
-                       d->slot, d->c_ocount, d->i32, 1);
+                       d->slot, d->c_ocount, d->i32, wrap);
   }
   CHK(read_err_flags(d));
   CHK(cell_sort(d, d->cell, pn, d->c_ocount, d->c_ostart, true, d->id));
@@ -786,7 +831,15 @@ int make_pair_list(mdgpu_domain *d) {
     std::swap(d->cell, d->cell_alt);
     std::swap(d->rep, d->rep_alt);
   }
-  // 2. ghost images (FindBorderParticles / SendBorderParticlesSub x 6)
+  return MDGPU_OK;
+}
+
+// The ghost images of a periodic domain or of every replica of an ensemble
+// (FindBorderParticles / SendBorderParticlesSub x 6), cell-sorted after the
+// owned particles.
+int place_images(mdgpu_domain *d) {
+  hipStream_t st = d->stream;
+  const int pn = d->pn;
   CHK(ensure_scan(d, std::max(pn, d->ncell)));
   long long ngl = 0;
   if (pn > 0) {
@@ -818,6 +871,17 @@ int make_pair_list(mdgpu_domain *d) {
   if (d->nrep > 1 && pn > 0)
     hipLaunchKernelGGL(k_replica_ghosts, dim3(nblk(d->nrep)), dim3(kBlock), 0, st, d->goff, d->reps, d->nrep, ng,
                        d->grng);
+  d->ghosts_ready = true;
+  return check_launch(d, "ghost images");
+}
+
+// Bricks, the neighbour list and the PairList::Init snapshot for the owned
+// particles and ghosts in place (bin_owned, then place_images or a unit's
+// ghost placement).
+int build_lists(mdgpu_domain *d) {
+  hipStream_t st = d->stream;
+  const int pn = d->pn;
+  HIPCHK(d, hipMemsetAsync(d->i32, 0, sizeof(int) * 4, st));
   // 4. bricks: rows, slices, staged ranges
   const Bricks B = bricks_of(d);
   const CellTables T{d->c_ostart, d->c_ocount, d->c_gstart, d->c_gcount};
@@ -929,6 +993,27 @@ int make_pair_list(mdgpu_domain *d) {
   d->vmax_valid = false;
   invalidate(d);
   return MDGPU_OK;
+}
+
+// MDManager::MakePairList for a periodic domain or an ensemble
+// (mdmanager.cc:262-280).  The reference also permutes particles that cross
+// a face (MakeBufferForSendingParticle, mdunit.cc:78-101); that only renames
+// indices, and particles are put in brick order here anyway.  A unit builds
+// its lists from the ghosts its group placed (mdgpu_unit_find_border_particles).
+int make_pair_list(mdgpu_domain *d) {
+  if (is_unit(d)) {
+    if (!d->ghosts_ready) {
+      d->err = "a unit's pair list needs its group's migration and ghosts first "
+               "(mdgpu_group_make_pair_list, or the mdgpu_unit_* sequence)";
+      return MDGPU_ERR_ARGUMENT;
+    }
+    return build_lists(d);
+  }
+  CHK(auto_bricks(d));
+  HIPCHK(d, hipMemsetAsync(d->i32, 0, sizeof(int) * 4, d->stream));
+  CHK(bin_owned(d, 1));
+  CHK(place_images(d));
+  return build_lists(d);
 }
 
 // max |p|^2 of a single domain; after a fused step it is the one the step's
@@ -1667,14 +1752,22 @@ hipError_t scratch_address(int device, double **out) {
 // A domain of nrep replicas with the common parameters of prm (dt, rc,
 // margin, check list length, precision, device); rl[r] gives replica r's box
 // and Langevin parameters (nullptr: one replica with box prm->L).
+// A unit (`unit`) has the box [unit->s, unit->e) of the whole box prm->L and
+// queues its work on the group's stream `shared`.
 int create_domain(const mdgpu_params *prm, int nrep, const mdgpu_replica *rl, const int *counts,
-                  mdgpu_domain **out) {
+                  mdgpu_domain **out, const mdgpu_unit *unit = nullptr, hipStream_t shared = nullptr) {
   *out = nullptr;
   mdgpu_domain *d = new mdgpu_domain();
   d->prm = *prm;
   d->device = prm->device;
-  if (hipSetDevice(prm->device) != hipSuccess ||
-      hipStreamCreateWithFlags(&d->stream, hipStreamNonBlocking) != hipSuccess) {
+  if (hipSetDevice(prm->device) != hipSuccess) {
+    delete d;
+    return MDGPU_ERR_HIP;
+  }
+  if (shared) {
+    d->stream = shared;
+    d->own_stream = false;
+  } else if (hipStreamCreateWithFlags(&d->stream, hipStreamNonBlocking) != hipSuccess) {
     delete d;
     return MDGPU_ERR_HIP;
   }
@@ -1689,10 +1782,17 @@ int create_domain(const mdgpu_params *prm, int nrep, const mdgpu_replica *rl, co
   }
   for (int k = 0; k < 3; k++) {
     d->L[k] = rl ? rl[0].L[k] : prm->L[k];
-    // single domain of a 1x1x1 grid: s = L*0, e = s + L (mdunit.cc:17-24)
-    d->rs[k] = d->L[k] * 0.0;
-    d->re[k] = d->rs[k] + d->L[k];
+    if (unit) {
+      d->rs[k] = unit->s[k];
+      d->re[k] = unit->e[k];
+      d->upos[k] = unit->pos[k];
+    } else {
+      // single domain of a 1x1x1 grid: s = L*0, e = s + L (mdunit.cc:17-24)
+      d->rs[k] = d->L[k] * 0.0;
+      d->re[k] = d->rs[k] + d->L[k];
+    }
   }
+  if (unit) d->row_sort = 0;  // rows stay in cell order: other units' ghosts index a unit's particles
   d->dt = prm->dt;
   d->rc = prm->rc;
   d->margin = prm->margin;
@@ -1759,6 +1859,11 @@ int create_domain(const mdgpu_params *prm, int nrep, const mdgpu_replica *rl, co
     mdgpu_destroy(d);
     return r;
   }
+  if (unit && ((r = dalloc(d, &d->src_tab, kMaxUnits)) != MDGPU_OK ||
+               hipHostMalloc((void **)&d->h_src_tab, kMaxUnits * sizeof(SrcUnit)) != hipSuccess)) {
+    mdgpu_destroy(d);
+    return r ? r : MDGPU_ERR_HIP;
+  }
   if (hipHostMalloc((void **)&d->h_u64, 4 * sizeof(unsigned long long)) != hipSuccess ||
       hipHostMalloc((void **)&d->h_i64, 4 * sizeof(long long)) != hipSuccess ||
       hipHostMalloc((void **)&d->h_i32, 4 * sizeof(int)) != hipSuccess ||
@@ -1782,6 +1887,253 @@ int create_domain(const mdgpu_params *prm, int nrep, const mdgpu_replica *rl, co
   return MDGPU_OK;
 }
 
+// ---- units of a process grid ------------------------------------------------
+
+bool same_group(const mdgpu_domain *a, const mdgpu_domain *b) { return a->group && a->group == b->group; }
+
+// MDUnit::MakeBufferForSendingParticle (mdunit.cc:78-101): the owned
+// particles over the face of direction dir (MDRect::IsOverBoundary) move to
+// the unit's send buffer, the others are compacted in their order.
+int unit_send(mdgpu_domain *d, int dir, int *nsend) {
+  hipStream_t st = d->stream;
+  invalidate(d);
+  const int pn = d->pn;
+  d->ng = 0;
+  d->tn = pn;
+  d->initialized = false;
+  d->ghosts_ready = false;
+  d->st_n = 0;
+  if (pn > 0) {
+    const int a = dir >> 1;
+    CHK(ensure_scan(d, pn));
+    hipLaunchKernelGGL(k_leave_flags, dim3(nblk(pn)), dim3(kBlock), 0, st, d->pos, pn, dir, d->rs[a], d->re[a],
+                       d->nimg);
+    exclusive_scan<int, int>(d->nimg, d->goff, pn, d->scan_scratch, d->i64, st);
+    long long ns = 0;
+    CHK(read_total(d, &ns));
+    if (ns > d->st_cap) {
+      const int c = (int)std::min<long long>(ns + ns / 4 + 256, kMaxParticles);
+      d->st_cap = 0;
+      CHK(dalloc(d, &d->st_pos, c));
+      CHK(dalloc(d, &d->st_mom, c));
+      CHK(dalloc(d, &d->st_id, c));
+      d->st_cap = c;
+    }
+    if (ns > 0) {
+      hipLaunchKernelGGL(k_partition, dim3(nblk(pn)), dim3(kBlock), 0, st, d->nimg, d->goff, pn, d->pos, d->mom,
+                         d->id, d->pos_tmp, d->mom_alt, d->id_alt, d->st_pos, d->st_mom, d->st_id);
+      std::swap(d->pos, d->pos_tmp);
+      std::swap(d->mom, d->mom_alt);
+      std::swap(d->id, d->id_alt);
+      d->pn = pn - (int)ns;
+      d->tn = d->pn;
+    }
+    d->st_n = (int)ns;
+  }
+  *nsend = d->st_n;
+  d->vmax_valid = false;
+  CHK(check_launch(d, "send particles"));
+  return upload_replicas(d);
+}
+
+// MDUnit::ReceiveParticles (mdunit.cc:104-122): src's send buffer appended to
+// the unit's owned particles, device to device.
+int unit_receive(mdgpu_domain *d, mdgpu_domain *src) {
+  const int n = src->st_n;
+  if (n == 0) return MDGPU_OK;
+  hipStream_t st = d->stream;
+  invalidate(d);
+  CHK(ensure_owned(d, d->pn + n));
+  HIPCHK(d, hipMemcpyAsync(d->pos + d->pn, src->st_pos, n * sizeof(double4), hipMemcpyDeviceToDevice, st));
+  HIPCHK(d, hipMemcpyAsync(d->mom + d->pn, src->st_mom, n * sizeof(double4), hipMemcpyDeviceToDevice, st));
+  HIPCHK(d, hipMemcpyAsync(d->id + d->pn, src->st_id, n * sizeof(int), hipMemcpyDeviceToDevice, st));
+  HIPCHK(d, hipMemsetAsync(d->rep + d->pn, 0, n * sizeof(int), st));
+  d->pn += n;
+  d->tn = d->pn;
+  d->ng = 0;
+  d->initialized = false;
+  d->ghosts_ready = false;
+  d->vmax_valid = false;
+  return upload_replicas(d);
+}
+
+// MDUnit::AdjustPeriodicBoundary (mdunit.h:71 -> variables.cc:59-79, the whole
+// box's edges), then the owned particles binned in the unit's cells
+// (MeshList::MakeMesh, meshlist.cc:153-205) and put in brick order, so that
+// the indices other units' ghosts take from this unit are final.
+int unit_adjust(mdgpu_domain *d) {
+  hipStream_t st = d->stream;
+  CHK(auto_bricks(d));
+  HIPCHK(d, hipMemsetAsync(d->i32, 0, sizeof(int) * 4, st));
+  if (d->pn > 0)
+    hipLaunchKernelGGL(k_unit_wrap, dim3(nblk(d->pn)), dim3(kBlock), 0, st, d->pos, d->pn, d->L[0], d->L[1],
+                       d->L[2], d->i32);
+  return bin_owned(d, 0);
+}
+
+// The unit's ghosts: what MDManager's six FindBorderParticles /
+// SendBorderParticlesSub passes bring it (mdmanager.cc:269-275), taken from
+// the owned particles of every unit of the group (SrcUnit), cell-sorted after
+// its own.  Every unit of the group must have been through unit_adjust.  The
+// group's lock keeps the units' positions where this reads them: another
+// unit placing its ghosts may reallocate its arrays.
+int unit_ghosts(mdgpu_domain *d) {
+  UnitGroup *g = d->group;
+  std::lock_guard<std::mutex> lock(g->mu);
+  hipStream_t st = d->stream;
+  const int pn = d->pn;
+  const int nsrc = (int)g->units.size();
+  long long total = 0;
+  for (int w = 0; w < nsrc; w++) {
+    const mdgpu_domain *W = g->units[w];
+    SrcUnit &S = d->h_src_tab[w];
+    S = SrcUnit{};
+    S.base = (int)total;
+    if (!W) continue;
+    S.pos = W->pos;
+    S.n = W->pn;
+    for (int a = 0; a < 3; a++) {
+      const int G = g->grid[a], u = d->upos[a], v = W->upos[a];
+      S.same[a] = v == u;
+      S.left[a] = v == (u - 1 + G) % G;   // W sends to u through its D_RIGHT-type face
+      S.right[a] = v == (u + 1) % G;      // ... through its D_LEFT-type face
+      S.lsh[a] = v + 1 >= G ? -1 : 0;     // ParaInfo::IsOverBoundary(W, high face)
+      S.rsh[a] = v - 1 < 0 ? 1 : 0;       // ParaInfo::IsOverBoundary(W, low face)
+      S.s[a] = W->rs[a];
+      S.e[a] = W->re[a];
+    }
+    total += W->pn;
+  }
+  if (total > (long long)INT32_MAX - 1) {
+    d->err = "too many particles in the group";
+    return MDGPU_ERR_CAPACITY;
+  }
+  if (total + 1 > d->ucap) {
+    const long long c = total + total / 8 + 1024;
+    d->ucap = 0;
+    CHK(dalloc(d, &d->ucnt, (size_t)c));
+    CHK(dalloc(d, &d->uoff, (size_t)c));
+    d->ucap = c;
+  }
+  HIPCHK(d, hipMemcpyAsync(d->src_tab, d->h_src_tab, nsrc * sizeof(SrcUnit), hipMemcpyHostToDevice, st));
+  CHK(ensure_scan(d, std::max<long long>(total, d->ncell)));
+  long long ngl = 0;
+  if (total > 0) {
+    hipLaunchKernelGGL(k_unit_count, dim3(nblk(total)), dim3(kBlock), 0, st, d->src_tab, nsrc, (int)total,
+                       d->unit_slot, d->SL, d->ucnt);
+    exclusive_scan<int, int>(d->ucnt, d->uoff, (int)total, d->scan_scratch, d->i64, st);
+    CHK(read_total(d, &ngl));
+  }
+  if (ngl + pn + 1 > kMaxParticles) {
+    d->err = "particle capacity exceeded (owned + ghosts)";
+    return MDGPU_ERR_CAPACITY;
+  }
+  const int ng = (int)ngl;
+  const double4 *own = d->pos;
+  CHK(ensure_ghosts(d, ng));
+  if (d->pos != own) {
+    // the unit's own positions moved: the table's entry for it follows
+    d->h_src_tab[d->unit_slot].pos = d->pos;
+    HIPCHK(d, hipMemcpyAsync(d->src_tab, d->h_src_tab, nsrc * sizeof(SrcUnit), hipMemcpyHostToDevice, st));
+  }
+  HIPCHK(d, hipMemsetAsync(d->c_gcount, 0, sizeof(int) * d->ncell, st));
+  if (ng > 0) {
+    // pos_tmp[0..ng) holds the unsorted images
+    hipLaunchKernelGGL(k_unit_emit, dim3(nblk(total)), dim3(kBlock), 0, st, d->src_tab, nsrc, (int)total,
+                       d->unit_slot, d->SL, d->uoff, ng, g->L[0], g->L[1], g->L[2], d->reps, d->t_gsrc, d->t_gcode,
+                       d->pos_tmp, d->t_gcell, d->slot, d->c_gcount);
+    CHK(check_launch(d, "unit ghosts"));
+    CHK(cell_sort(d, d->t_gcell, ng, d->c_gcount, d->c_gstart, false, nullptr));
+    hipLaunchKernelGGL(k_gather_unit_ghosts, dim3(nblk(ng)), dim3(kBlock), 0, st, d->perm, ng, pn, d->t_gsrc,
+                       d->t_gcode, d->pos_tmp, d->t_gcell, d->gsrc, d->gcode, d->gunit, d->pos, d->cell);
+  } else {
+    exclusive_scan<int, int>(d->c_gcount, d->c_gstart, d->ncell, d->scan_scratch, nullptr, st);
+  }
+  d->ng = ng;
+  d->tn = pn + ng;
+  d->ghosts_ready = true;
+  d->ghost_epoch = g->epoch;
+  // the host table is read by the copies above before it is written again:
+  // the next write follows this unit's next read_total
+  return check_launch(d, "unit ghosts");
+}
+
+// MDManager::SendBorderParticles for a unit (mdmanager.cc:283-348): each ghost
+// from its source unit's current owned position.
+int unit_refresh(mdgpu_domain *d) {
+  if (d->ng == 0) return MDGPU_OK;
+  UnitGroup *g = d->group;
+  UnitPos P{};
+  {
+    std::lock_guard<std::mutex> lock(g->mu);
+    if (d->ghost_epoch != g->epoch) {
+      d->err = "a unit of the group was destroyed after this unit's ghosts were placed: rebuild the group's lists";
+      return MDGPU_ERR_ARGUMENT;
+    }
+    for (size_t w = 0; w < g->units.size(); w++) P.pos[w] = g->units[w] ? g->units[w]->pos : nullptr;
+  }
+  hipLaunchKernelGGL(k_unit_refresh, dim3(nblk(d->ng)), dim3(kBlock), 0, d->stream, d->pos, d->gsrc, d->gcode,
+                     d->gunit, d->pn, d->ng, P, g->L[0], g->L[1], g->L[2]);
+  return check_launch(d, "unit halo");
+}
+
+// The unit of the group at grid position p (nullptr if there is none).
+mdgpu_domain *unit_at(UnitGroup *g, const int p[3]) {
+  for (mdgpu_domain *u : g->units)
+    if (u && u->upos[0] == p[0] && u->upos[1] == p[1] && u->upos[2] == p[2]) return u;
+  return nullptr;
+}
+
+// The neighbour of unit u in direction dir (ParaInfo::GetNeighborID, parainfo.cc:191-224).
+mdgpu_domain *unit_neighbor(mdgpu_domain *u, int dir) {
+  UnitGroup *g = u->group;
+  int p[3] = {u->upos[0], u->upos[1], u->upos[2]};
+  const int a = dir >> 1;
+  p[a] += (dir & 1) ? 1 : -1;
+  p[a] = (p[a] + g->grid[a]) % g->grid[a];
+  return unit_at(g, p);
+}
+
+// Ghost source ids: the id of the owned particle each ghost images (the
+// unit's own, or the source unit's for a unit).
+int ghost_source_ids(mdgpu_domain *d, std::vector<int> &out) {
+  const int ng = d->ng;
+  out.assign(ng, 0);
+  if (ng == 0) return MDGPU_OK;
+  std::vector<int> gs(ng), gu(ng, 0);
+  HIPCHK(d, hipMemcpyAsync(gs.data(), d->gsrc, ng * sizeof(int), hipMemcpyDeviceToHost, d->stream));
+  if (is_unit(d)) HIPCHK(d, hipMemcpyAsync(gu.data(), d->gunit, ng * sizeof(int), hipMemcpyDeviceToHost, d->stream));
+  CHK(sync(d));
+  std::vector<std::vector<int>> ids;
+  if (!is_unit(d)) {
+    ids.resize(1);
+    ids[0].resize(d->pn);
+    if (d->pn) HIPCHK(d, hipMemcpy(ids[0].data(), d->id, d->pn * sizeof(int), hipMemcpyDeviceToHost));
+  } else {
+    UnitGroup *g = d->group;
+    std::lock_guard<std::mutex> lock(g->mu);
+    if (d->ghost_epoch != g->epoch) {
+      d->err = "a unit of the group was destroyed after this unit's ghosts were placed: rebuild the group's lists";
+      return MDGPU_ERR_ARGUMENT;
+    }
+    ids.resize(g->units.size());
+    for (size_t w = 0; w < g->units.size(); w++) {
+      const mdgpu_domain *W = g->units[w];
+      if (!W || W->pn == 0) continue;
+      ids[w].resize(W->pn);
+      HIPCHK(d, hipMemcpy(ids[w].data(), W->id, W->pn * sizeof(int), hipMemcpyDeviceToHost));
+    }
+  }
+  for (int k = 0; k < ng; k++) {
+    if (gu[k] < 0 || gu[k] >= (int)ids.size() || gs[k] < 0 || gs[k] >= (int)ids[gu[k]].size()) {
+      d->err = "ghost source out of range: a source unit changed since this unit's construction";
+      return MDGPU_ERR_ARGUMENT;
+    }
+    out[k] = ids[gu[k]][gs[k]];
+  }
+  return MDGPU_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -1789,6 +2141,146 @@ extern "C" {
 int mdgpu_create(const mdgpu_params *prm, mdgpu_domain **out) {
   if (!prm || !out) return MDGPU_ERR_ARGUMENT;
   return create_domain(prm, 1, nullptr, nullptr, out);
+}
+
+int mdgpu_create_unit(const mdgpu_params *prm, const mdgpu_unit *u, mdgpu_domain *join, mdgpu_domain **out) {
+  if (!prm || !u || !out) return MDGPU_ERR_ARGUMENT;
+  *out = nullptr;
+  for (int a = 0; a < 3; a++)
+    if (u->grid[a] < 1 || u->pos[a] < 0 || u->pos[a] >= u->grid[a] || !(u->s[a] < u->e[a]) || !(prm->L[a] > 0.0))
+      return MDGPU_ERR_ARGUMENT;
+  UnitGroup *g = nullptr;
+  bool fresh = false;
+  if (join) {
+    g = join->group;
+    if (!g) {
+      join->err = "mdgpu_create_unit: the domain to join is not a unit";
+      return MDGPU_ERR_ARGUMENT;
+    }
+  } else {
+    g = new UnitGroup();
+    fresh = true;
+    g->device = prm->device;
+    for (int a = 0; a < 3; a++) {
+      g->L[a] = prm->L[a];
+      g->grid[a] = u->grid[a];
+    }
+    if (hipSetDevice(prm->device) != hipSuccess ||
+        hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking) != hipSuccess) {
+      delete g;
+      return MDGPU_ERR_HIP;
+    }
+  }
+  mdgpu_domain *d = nullptr;
+  const int r = create_domain(prm, 1, nullptr, nullptr, &d, u, g->stream);
+  if (r != MDGPU_OK) {
+    if (fresh) {
+      (void)hipStreamDestroy(g->stream);
+      delete g;
+    }
+    return r;
+  }
+  std::string why;
+  {
+    std::lock_guard<std::mutex> lock(g->mu);
+    for (int a = 0; a < 3; a++)
+      if (g->grid[a] != u->grid[a] || g->L[a] != prm->L[a]) why = "the units of a group share the grid and the box";
+    if (g->device != prm->device) why = "the units of a group share the device";
+    if ((int)g->units.size() >= kMaxUnits) why = "too many units in one group";
+    if (unit_at(g, u->pos)) why = "a unit of the group already has this grid position";
+    if (why.empty()) {
+      d->group = g;
+      d->unit_slot = (int)g->units.size();
+      g->units.push_back(d);
+      g->live++;
+    }
+  }
+  if (!why.empty()) {
+    mdgpu_destroy(d);  // not in the group: leaves the group's stream alone
+    if (join) join->err = "mdgpu_create_unit: " + why;
+    return MDGPU_ERR_ARGUMENT;
+  }
+  *out = d;
+  return MDGPU_OK;
+}
+
+int mdgpu_unit_send_particles(mdgpu_domain *d, int dir, int *nsend) {
+  if (!d || dir < 0 || dir > 5 || !nsend) return MDGPU_ERR_ARGUMENT;
+  if (!is_unit(d)) {
+    d->err = "not a unit (mdgpu_create_unit)";
+    return MDGPU_ERR_ARGUMENT;
+  }
+  return unit_send(d, dir, nsend);
+}
+
+int mdgpu_unit_receive_particles(mdgpu_domain *d, mdgpu_domain *src) {
+  if (!d || !src) return MDGPU_ERR_ARGUMENT;
+  if (!same_group(d, src)) {
+    d->err = "particles are received from a unit of the same group";
+    return MDGPU_ERR_ARGUMENT;
+  }
+  return unit_receive(d, src);
+}
+
+int mdgpu_unit_adjust_periodic_boundary(mdgpu_domain *d) {
+  if (!d) return MDGPU_ERR_ARGUMENT;
+  if (!is_unit(d)) {
+    d->err = "not a unit (mdgpu_create_unit)";
+    return MDGPU_ERR_ARGUMENT;
+  }
+  return unit_adjust(d);
+}
+
+int mdgpu_unit_find_border_particles(mdgpu_domain *d) {
+  if (!d) return MDGPU_ERR_ARGUMENT;
+  if (!is_unit(d)) {
+    d->err = "not a unit (mdgpu_create_unit)";
+    return MDGPU_ERR_ARGUMENT;
+  }
+  return unit_ghosts(d);
+}
+
+int mdgpu_group_make_pair_list(mdgpu_domain *d) {
+  if (!d) return MDGPU_ERR_ARGUMENT;
+  if (!is_unit(d)) {
+    d->err = "not a unit (mdgpu_create_unit)";
+    return MDGPU_ERR_ARGUMENT;
+  }
+  UnitGroup *g = d->group;
+  std::vector<mdgpu_domain *> us;
+  {
+    std::lock_guard<std::mutex> lock(g->mu);
+    for (mdgpu_domain *u : g->units)
+      if (u) us.push_back(u);
+  }
+  auto fail = [&](mdgpu_domain *u, int r) {
+    if (u != d) d->err = u->err;
+    return r;
+  };
+  int r;
+  // MDManager::SendParticles (mdmanager.cc:194-260): every unit fills its
+  // buffer for a direction, then every unit takes its opposite neighbour's
+  for (int dir = 0; dir < 6; dir++) {
+    for (mdgpu_domain *u : us) {
+      int n = 0;
+      if ((r = unit_send(u, dir, &n)) != MDGPU_OK) return fail(u, r);
+    }
+    for (mdgpu_domain *u : us) {
+      mdgpu_domain *src = unit_neighbor(u, dir ^ 1);
+      if (!src) {
+        d->err = "the group does not fill its grid";
+        return MDGPU_ERR_ARGUMENT;
+      }
+      if ((r = unit_receive(u, src)) != MDGPU_OK) return fail(u, r);
+    }
+  }
+  for (mdgpu_domain *u : us)
+    if ((r = unit_adjust(u)) != MDGPU_OK) return fail(u, r);
+  for (mdgpu_domain *u : us)
+    if ((r = unit_ghosts(u)) != MDGPU_OK) return fail(u, r);
+  for (mdgpu_domain *u : us)
+    if ((r = make_pair_list(u)) != MDGPU_OK) return fail(u, r);
+  return MDGPU_OK;
 }
 
 int mdgpu_create_ensemble(const mdgpu_params *prm, int thermostat, int nrep, const mdgpu_replica *reps,
@@ -1863,7 +2355,8 @@ int mdgpu_destroy(mdgpu_domain *d) {
                   d->i32,     d->d3,          d->reps,        d->brep,      d->rep,      d->rep_alt,
                   d->grng,    d->rep_v2,      d->rep_out,     d->which,     d->rep_exp,  d->clock,
                   d->perm_in, d->nbr_in,      d->ncode_in,    d->slen_in,   d->chk_dmax, d->chk_cnt,
-                  d->chk_cand, d->gate,       d->gbudget,     d->glife,     d->xfer};
+                  d->chk_cand, d->gate,       d->gbudget,     d->glife,     d->xfer,     d->gunit,
+                  d->st_pos,  d->st_mom,      d->st_id,       d->src_tab,   d->ucnt,     d->uoff};
   for (StepGraph &g : d->graphs) drop_graph(g);
   for (StepGraph &g : d->gated) drop_graph(g);
   for (void *b : bufs)
@@ -1875,8 +2368,24 @@ int mdgpu_destroy(mdgpu_domain *d) {
   if (d->h_i64) (void)hipHostFree(d->h_i64);
   if (d->h_i32) (void)hipHostFree(d->h_i32);
   if (d->h_d3) (void)hipHostFree(d->h_d3);
+  if (d->h_src_tab) (void)hipHostFree(d->h_src_tab);
   for (hipEvent_t e : d->ev) (void)hipEventDestroy(e);
-  if (d->stream) (void)hipStreamDestroy(d->stream);
+  if (d->stream && d->own_stream) (void)hipStreamDestroy(d->stream);
+  if (d->group) {
+    // the last unit of a group takes the group and its stream with it
+    UnitGroup *g = d->group;
+    bool last = false;
+    {
+      std::lock_guard<std::mutex> lock(g->mu);
+      if (d->unit_slot >= 0 && d->unit_slot < (int)g->units.size()) g->units[d->unit_slot] = nullptr;
+      g->epoch++;
+      last = --g->live == 0;
+    }
+    if (last) {
+      (void)hipStreamDestroy(g->stream);
+      delete g;
+    }
+  }
   delete d;
   return MDGPU_OK;
 }
@@ -1923,6 +2432,7 @@ int mdgpu_add_particles(mdgpu_domain *d, int n, const double *q, const double *p
   d->pn += add;
   d->tn = d->pn;
   d->ng = 0;
+  d->ghosts_ready = false;
   CHK(upload_replicas(d));
   d->vmax_valid = false;
   // indices of the old list are stale: the next step rebuilds
@@ -2006,6 +2516,7 @@ int mdgpu_clear_particles(mdgpu_domain *d) {
   CHK(sync(d));
   d->pn = d->ng = d->tn = 0;
   d->initialized = false;
+  d->ghosts_ready = false;
   d->vmax_valid = false;
   return upload_replicas(d);
 }
@@ -2013,17 +2524,15 @@ int mdgpu_clear_particles(mdgpu_domain *d) {
 int mdgpu_get_ghosts(mdgpu_domain *d, int *src_type, int *shift_code, double *q) {
   if (!d) return MDGPU_ERR_ARGUMENT;
   const int ng = d->ng, pn = d->pn;
-  std::vector<int> gs(ng), gc(ng), ids(pn);
+  std::vector<int> gc(ng), gid;
   std::vector<double4> gq(ng);
   if (ng > 0) {
-    HIPCHK(d, hipMemcpyAsync(gs.data(), d->gsrc, ng * sizeof(int), hipMemcpyDeviceToHost, d->stream));
     HIPCHK(d, hipMemcpyAsync(gc.data(), d->gcode, ng * sizeof(int), hipMemcpyDeviceToHost, d->stream));
     HIPCHK(d, hipMemcpyAsync(gq.data(), d->pos + pn, ng * sizeof(double4), hipMemcpyDeviceToHost, d->stream));
-    HIPCHK(d, hipMemcpyAsync(ids.data(), d->id, pn * sizeof(int), hipMemcpyDeviceToHost, d->stream));
   }
-  CHK(sync(d));
+  CHK(ghost_source_ids(d, gid));  // synchronises
   for (int k = 0; k < ng; k++) {
-    if (src_type) src_type[k] = ids[gs[k]];
+    if (src_type) src_type[k] = gid[k];
     if (shift_code) shift_code[k] = ghost_shift(gc[k]);
     if (q) {
       q[3 * (size_t)k] = gq[k].x;
@@ -2095,7 +2604,7 @@ int mdgpu_update_position_half(mdgpu_domain *d) {
 int mdgpu_send_border_particles(mdgpu_domain *d) {
   if (!d) return MDGPU_ERR_ARGUMENT;
   invalidate(d);  // the next fused step recomputes its first half drift
-  return refresh(d);
+  return is_unit(d) ? unit_refresh(d) : refresh(d);
 }
 
 int mdgpu_calculate_force(mdgpu_domain *d) {
@@ -2138,6 +2647,10 @@ int mdgpu_heatbath_momenta(mdgpu_domain *d) {
 
 int mdgpu_calculate(mdgpu_domain *d, int nsteps, int *rebuilds) {
   if (!d || nsteps < 0) return MDGPU_ERR_ARGUMENT;
+  if (is_unit(d)) {
+    d->err = "mdgpu_calculate: the units of a group step through MDManager's sequence (the mdgpu_unit_* calls)";
+    return MDGPU_ERR_ARGUMENT;
+  }
   hipEvent_t t0 = nullptr, t1 = nullptr;
   d->force_launches = 0;
   d->force_ms = d->total_ms = d->rebuild_ms = 0.0;
@@ -2247,6 +2760,10 @@ int mdgpu_observe_kinetic(mdgpu_domain *d, double *ke) {
 
 int mdgpu_change_scale(mdgpu_domain *d, double alpha) {
   if (!d || !(alpha > 0.0) || d->nrep > 1) return MDGPU_ERR_ARGUMENT;
+  if (is_unit(d)) {
+    d->err = "mdgpu_change_scale: a unit of a process grid does not rescale";
+    return MDGPU_ERR_ARGUMENT;
+  }
   invalidate(d);  // the next fused step recomputes its first half drift
   CHK(make_pair_list(d));
   for (int k = 0; k < 3; k++) {
@@ -2274,7 +2791,7 @@ int collect_pairs(mdgpu_domain *d, int rsel, int64_t *npairs, int32_t *out3) {
   const int pn = d->pn, ng = d->ng, nbk = d->nbrick;
   std::vector<uint16_t> nbr((size_t)d->nbr_slots);
   std::vector<long long> sb(d->nslice), gb(nbk + 1);
-  std::vector<int> sl(d->nslice), ids(pn), cl(pn), rp(pn), r0(nbk), vs0(nbk), gs(ng), gc(ng);
+  std::vector<int> sl(d->nslice), ids(pn), cl(pn), rp(pn), r0(nbk), vs0(nbk), gc(ng), gid;
   if (d->nbr_slots) HIPCHK(d, hipMemcpyAsync(nbr.data(), d->nbr, nbr.size() * 2, hipMemcpyDeviceToHost, d->stream));
   if (d->nslice) {
     HIPCHK(d, hipMemcpyAsync(sb.data(), d->sbase, d->nslice * 8, hipMemcpyDeviceToHost, d->stream));
@@ -2285,14 +2802,11 @@ int collect_pairs(mdgpu_domain *d, int rsel, int64_t *npairs, int32_t *out3) {
     HIPCHK(d, hipMemcpyAsync(cl.data(), d->cell, pn * 4, hipMemcpyDeviceToHost, d->stream));
     HIPCHK(d, hipMemcpyAsync(rp.data(), d->rep, pn * 4, hipMemcpyDeviceToHost, d->stream));
   }
-  if (ng) {
-    HIPCHK(d, hipMemcpyAsync(gs.data(), d->gsrc, ng * 4, hipMemcpyDeviceToHost, d->stream));
-    HIPCHK(d, hipMemcpyAsync(gc.data(), d->gcode, ng * 4, hipMemcpyDeviceToHost, d->stream));
-  }
+  if (ng) HIPCHK(d, hipMemcpyAsync(gc.data(), d->gcode, ng * 4, hipMemcpyDeviceToHost, d->stream));
   HIPCHK(d, hipMemcpyAsync(r0.data(), d->row0, nbk * 4, hipMemcpyDeviceToHost, d->stream));
   HIPCHK(d, hipMemcpyAsync(vs0.data(), d->vslice0, nbk * 4, hipMemcpyDeviceToHost, d->stream));
   HIPCHK(d, hipMemcpyAsync(gb.data(), d->gbase, (nbk + 1) * 8, hipMemcpyDeviceToHost, d->stream));
-  CHK(sync(d));
+  CHK(ghost_source_ids(d, gid));  // synchronises
   std::vector<int> gl((size_t)gb[nbk]);
   if (!gl.empty()) HIPCHK(d, hipMemcpy(gl.data(), d->glist, gl.size() * 4, hipMemcpyDeviceToHost));
   int64_t cnt = 0;
@@ -2313,7 +2827,7 @@ int collect_pairs(mdgpu_domain *d, int rsel, int64_t *npairs, int32_t *out3) {
         code = 13;
         if (bb < a) continue;  // owned-owned pairs appear in both rows
       } else {
-        bb = ids[gs[j - pn]];
+        bb = gid[j - pn];
         code = ghost_shift(gc[j - pn]);
       }
       if (out3) {
@@ -2492,6 +3006,10 @@ int mdgpu_set_option(mdgpu_domain *d, int option, int value) {
       return MDGPU_OK;
     case MDGPU_OPT_ROW_SORT:
       if (value < 0 || value > 64) return MDGPU_ERR_ARGUMENT;
+      if (is_unit(d) && value != 0) {
+        d->err = "a unit keeps its rows in cell order: other units' ghosts index its particles";
+        return MDGPU_ERR_ARGUMENT;
+      }
       d->row_sort = value;
       d->initialized = false;  // the next step rebuilds the list
       return MDGPU_OK;

@@ -5,13 +5,22 @@
 //----------------------------------------------------------------------
 #include <iostream>
 #include <fstream>
+#include <mutex>
 #include <string>
+#include <vector>
 #include <stdlib.h>
 #include <string.h>
+#include <mpi.h>
 #include "mpistream.h"
 #include "mdunit.h"
 //----------------------------------------------------------------------
 namespace {
+// The units of this process that are device units of one group, in the order
+// they were made (MDManager makes them from its OpenMP threads).
+std::mutex g_units_mu;
+std::vector<MDUnit *> g_units;
+std::vector<mdgpu_domain *> g_unit_devs;
+
 int env_int(const char *name, int def) {
   const char *v = getenv(name);
   return v ? atoi(v) : def;
@@ -38,9 +47,14 @@ MDUnit::MDUnit(int id_, SimulationInfo *si, ParaInfo *pi):
   int grid_position[D];
   pinfo->GetGridSize(grid_size);
   pinfo->GetGridPosition(id, grid_position);
-  if (grid_size[X] != 1 || grid_size[Y] != 1 || grid_size[Z] != 1) {
-    show_error("the GPU build runs one domain per process: use mpirun -np 1 and OMP_NUM_THREADS=1");
-    exit(1);
+  unit_mode = grid_size[X] != 1 || grid_size[Y] != 1 || grid_size[Z] != 1;
+  if (unit_mode) {
+    int procs = 1;
+    MPI_Comm_size(MPI_COMM_WORLD, &procs);
+    if (procs != 1) {
+      show_error("the GPU build runs the units of one process on its GPU: use mpirun -np 1");
+      exit(1);
+    }
   }
   double s[D], e[D];
   for (int d = 0; d < D; d++) {
@@ -64,9 +78,30 @@ MDUnit::MDUnit(int id_, SimulationInfo *si, ParaInfo *pi):
   prm.capacity = 0;
   prm.device = env_int("MDACP_GPU_DEVICE", 0);
   mdgpu_domain *d = NULL;
-  if (mdgpu_create(&prm, &d) != MDGPU_OK) {
-    show_error("could not create the GPU domain (no HIP device, or out of memory)");
-    exit(1);
+  if (!unit_mode) {
+    if (mdgpu_create(&prm, &d) != MDGPU_OK) {
+      show_error("could not create the GPU domain (no HIP device, or out of memory)");
+      exit(1);
+    }
+  } else {
+    // a device unit of the process's group: the first unit makes the group
+    mdgpu_unit u;
+    for (int k = 0; k < 3; k++) {
+      u.grid[k] = grid_size[k];
+      u.pos[k] = grid_position[k];
+      u.s[k] = s[k];
+      u.e[k] = e[k];
+    }
+    std::lock_guard<std::mutex> lock(g_units_mu);
+    mdgpu_domain *join = g_unit_devs.empty() ? NULL : g_unit_devs[0];
+    if (mdgpu_create_unit(&prm, &u, join, &d) != MDGPU_OK) {
+      show_error("could not create the GPU unit (no HIP device, out of memory, or a unit box under two search lengths)");
+      if (join) mout << mdgpu_last_error(join) << std::endl;
+      exit(1);
+    }
+    g_units.push_back(this);
+    g_unit_devs.push_back(d);
+    vars->SetUnitTag(GetID());
   }
   vars->AttachDevice(d);
   mesh->Attach(vars);
@@ -75,6 +110,15 @@ MDUnit::MDUnit(int id_, SimulationInfo *si, ParaInfo *pi):
 //----------------------------------------------------------------------
 MDUnit::~MDUnit(void) {
   mdgpu_domain *d = vars->dev;
+  if (unit_mode) {
+    std::lock_guard<std::mutex> lock(g_units_mu);
+    for (size_t k = 0; k < g_units.size(); k++)
+      if (g_units[k] == this) {
+        g_units.erase(g_units.begin() + k);
+        g_unit_devs.erase(g_unit_devs.begin() + k);
+        break;
+      }
+  }
   delete vars;
   delete mesh;
   delete plist;
@@ -148,30 +192,82 @@ MDUnit::SetSeed(const int seed) {
   vars->Check(mdgpu_set_seed(vars->dev, static_cast<uint64_t>(seed + GetID())), "seed");
 }
 //----------------------------------------------------------------------
-// Migration (MakeBufferForSendingParticle / ReceiveParticles, mdunit.cc:78-122):
-// a single domain is its own neighbour on every face, so particles that leave
+// The unit of this process whose send_buffer this is (NULL: another buffer).
+MDUnit *
+MDUnit::Sender(const std::vector<ParticleInfo> &buffer) {
+  std::lock_guard<std::mutex> lock(g_units_mu);
+  for (size_t k = 0; k < g_units.size(); k++)
+    if (&g_units[k]->send_buffer == &buffer) return g_units[k];
+  return NULL;
+}
+//----------------------------------------------------------------------
+// Migration (MakeBufferForSendingParticle / ReceiveParticles, mdunit.cc:78-122).
+// A single domain is its own neighbour on every face, so particles that leave
 // through one face come back through the opposite one -- the periodic wrap,
-// which the device applies at every pair-list construction.  Nothing is sent.
+// which the device applies at every pair-list construction: nothing is sent.
+// A device unit moves the particles over the face of `dir` into its device
+// send buffer; the unit MDManager hands that send_buffer to takes them from
+// there, device to device.
 void
 MDUnit::MakeBufferForSendingParticle(const int dir) {
   send_buffer.clear();
+  if (!unit_mode) return;
+  vars->ToDevice();
+  int n = 0;
+  vars->Check(mdgpu_unit_send_particles(vars->dev, dir, &n), "send particles");
+  vars->DeviceChanged();
+  list_built = false;
 }
 //----------------------------------------------------------------------
 void
 MDUnit::ReceiveParticles(std::vector<ParticleInfo> &recv_buffer) {
+  if (unit_mode) {
+    MDUnit *src = Sender(recv_buffer);
+    if (src != NULL) {
+      vars->ToDevice();
+      vars->Check(mdgpu_unit_receive_particles(vars->dev, src->vars->dev), "receive particles");
+      vars->DeviceChanged();
+      list_built = false;
+      return;
+    }
+    if (!recv_buffer.empty()) {
+      show_error("the GPU build runs the units of one process on its GPU: particles from another process");
+      exit(1);
+    }
+    return;
+  }
   for (size_t k = 0; k < recv_buffer.size(); k++) {
     ParticleInfo &pi = recv_buffer[k];
     AddParticle(pi.q, pi.p, pi.type);
   }
 }
 //----------------------------------------------------------------------
+// MDUnit::AdjustPeriodicBoundary (mdunit.h:71): a device unit wraps its
+// particles into the box and bins them in its cells, so that the indices the
+// other units' ghosts take from it are final; a single domain wraps at its
+// pair-list construction.
+void
+MDUnit::AdjustPeriodicBoundary(void) {
+  if (!unit_mode) return;
+  vars->ToDevice();
+  vars->Check(mdgpu_unit_adjust_periodic_boundary(vars->dev), "periodic boundary");
+  vars->DeviceChanged();
+  list_built = false;
+}
+//----------------------------------------------------------------------
 // Ghosts (FindBorderParticles / MakeBufferForBorderParticles /
 // ReceiveBorderParticles, mdunit.cc:125-191): the device finds the border
 // particles and places their images at each construction, and refreshes the
 // images from the owners' current positions in one pass over all six
-// directions, done here for the first direction of MDManager's sequence.
+// directions, done here for the first direction of MDManager's sequence.  A
+// device unit places, at its first direction, every ghost the six passes
+// bring it, from the units that own them (all units are through
+// AdjustPeriodicBoundary by then: MDManager's phases are separated).
 void
 MDUnit::FindBorderParticles(const int dir) {
+  if (!unit_mode || dir != D_LEFT) return;
+  vars->Check(mdgpu_unit_find_border_particles(vars->dev), "border particles");
+  vars->DeviceChanged();
 }
 //----------------------------------------------------------------------
 void
@@ -187,7 +283,7 @@ MDUnit::MakeBufferForBorderParticles(const int dir) {
 void
 MDUnit::ReceiveBorderParticles(std::vector<ParticleInfo> &recv_buffer) {
   if (!recv_buffer.empty()) {
-    show_error("the GPU build runs one domain per process: ghost images from another unit");
+    show_error("the GPU build runs the units of one process on its GPU: ghost images from another process");
     exit(1);
   }
 }
@@ -228,6 +324,10 @@ MDUnit::MakePairList(void) {
 // grid scaled by alpha on the device (which rebuilds the list around it).
 void
 MDUnit::ChangeScale(double alpha) {
+  if (unit_mode) {
+    show_error("ChangeScale with several units in a process is not supported by the GPU build");
+    exit(1);
+  }
   vars->ToDevice();
   myrect.ChangeScale(alpha);
   vars->Check(mdgpu_change_scale(vars->dev, alpha), "change scale");

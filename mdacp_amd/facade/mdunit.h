@@ -5,12 +5,23 @@
 // MDManager calls runs on the GPU through the mdgpu C ABI (include/mdgpu.h).
 //
 // The public interface is the reference's, so MDManager, ProjectManager and
-// the projects compile and run unchanged on top of it.  One domain per
-// process (mpirun -np 1, OMP_NUM_THREADS=1): the particle migration and halo
+// the projects compile and run unchanged on top of it.
+//
+// One unit per process (OMP_NUM_THREADS=1): the particle migration and halo
 // exchange MDManager drives through send_buffer / Receive* reduce, for a
 // domain that is its own neighbour on all six faces, to the periodic wrap and
 // ghost images the device makes itself (at MakePairList, and at the halo
 // refresh MakeBufferForBorderParticles(D_LEFT) triggers).
+//
+// Several units in one process (OMP_NUM_THREADS > 1, one MPI process): each
+// unit is a device unit of one group (mdgpu_create_unit) on the process's
+// GPU, and MDManager's calls move particles and ghosts device to device:
+// MakeBufferForSendingParticle fills the unit's device send buffer,
+// ReceiveParticles(sender's send_buffer) appends the sender's device buffer
+// (the host vectors stay empty; the sender is found by the address of its
+// send_buffer), AdjustPeriodicBoundary wraps and bins on the device, and
+// FindBorderParticles(D_LEFT) places every ghost the six passes would bring,
+// from the units that own the particles.
 //
 // Environment (read when the unit is created):
 //   MDACP_GPU_PRECISION  "fp64" (default) or "mixed"
@@ -50,6 +61,8 @@ private:
   MDRect myrect;
   std::mt19937 mt;
   bool list_built;  // the device holds a list for its current particles
+  bool unit_mode;   // one of several units of the process (a device unit)
+  static MDUnit *Sender(const std::vector<ParticleInfo> &buffer);
 public:
   MDUnit (int id_, SimulationInfo *si, ParaInfo *pi);
   ~MDUnit(void);
@@ -64,7 +77,7 @@ public:
   double * GetSystemSize(void) {return sinfo->L;}
   double GetTimeStep(void) {return sinfo->TimeStep;}
   void SetInitialVelocity(double v0);
-  int GetParticleNumber(void) {return vars->GetParticleNumber();}
+  int GetParticleNumber(void) {return vars->DeviceParticleNumber();}
   int GetTotalParticleNumber(void) {return vars->GetTotalParticleNumber();}
   void SetTotalParticleNumber(int n) {vars->SetTotalParticleNumber(n);}
 
@@ -79,7 +92,7 @@ public:
   void FindBorderParticles(const int dir);
   void MakeBufferForBorderParticles(const int dir);
   void ReceiveParticles(std::vector<ParticleInfo> &recv_buffer);
-  void AdjustPeriodicBoundary(void) {}
+  void AdjustPeriodicBoundary(void);
   void ReceiveBorderParticles(std::vector<ParticleInfo> &recv_buffer);
   double ObserveDouble(DoubleObserver *obs);
   int IntegerDouble(IntegerObserver *obs);

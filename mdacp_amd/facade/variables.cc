@@ -9,12 +9,28 @@
 #include <math.h>
 #include <algorithm>
 #include <iostream>
+#include <mutex>
 #include <random>
 #include <vector>
 #include "mpistream.h"
 #include "variables.h"
 //----------------------------------------------------------------------
 namespace {
+// The caller's type of every particle id, by the tag of the unit that named
+// it (ids of a plain counter: tag 0).  Units of one process add particles
+// from their own threads (MDManager::ExecuteAll) and read each other's
+// entries once particles migrate, so the tables share one lock.
+const int kIdBits = 24;
+const int kMaxTags = 128;
+std::vector<int> g_types[kMaxTags];
+std::mutex g_types_mu;
+
+int type_of_id(int id, bool tagged) {
+  const int tag = tagged ? (id >> kIdBits) : 0;
+  const int local = tagged ? (id & ((1 << kIdBits) - 1)) : id;
+  return g_types[tag][local];
+}
+
 template <class T>
 T *grow_aligned(T *old, size_t keep, size_t n) {
   void *p = NULL;
@@ -50,6 +66,7 @@ Variables::Variables(void) {
   p = NULL;
   type = NULL;
   ids = NULL;
+  unit_tag = -1;
   dev = NULL;
   dev_count = 0;
   host_stale = false;
@@ -84,6 +101,27 @@ Variables::Check(int status, const char *what) {
   exit(1);
 }
 //----------------------------------------------------------------------
+int
+Variables::NewId(int t) {
+  std::lock_guard<std::mutex> lock(g_types_mu);
+  std::vector<int> &v = g_types[unit_tag < 0 ? 0 : unit_tag];
+  const size_t local = v.size();
+  if (unit_tag >= 0 && local >= (size_t(1) << kIdBits)) {
+    show_error("a unit of a process grid names at most 2^24 particles");
+    exit(1);
+  }
+  v.push_back(t);
+  return unit_tag < 0 ? static_cast<int>(local) : ((unit_tag << kIdBits) | static_cast<int>(local));
+}
+//----------------------------------------------------------------------
+int
+Variables::DeviceParticleNumber(void) {
+  if (dev == NULL || !host_stale) return particle_number;
+  int pn = 0, tn = 0;
+  Check(mdgpu_particle_number(dev, &pn, &tn), "particle number");
+  return pn;
+}
+//----------------------------------------------------------------------
 // Variables::AddParticle (variables.cc:31-42): appended on the host; pushed to
 // the device before the next device operation.
 void
@@ -98,8 +136,7 @@ Variables::AddParticle(double x[D], double v[D], int t) {
   p[i][Y] = v[Y];
   p[i][Z] = v[Z];
   type[i] = t;
-  ids[i] = static_cast<int>(type_of_id.size());
-  type_of_id.push_back(t);
+  ids[i] = NewId(t);
   particle_number++;
   total_particle_number = particle_number;
 }
@@ -134,25 +171,33 @@ Variables::ToHost(void) {
   std::vector<int> gsrc(ng), gcode(ng);
   std::vector<double> gq((size_t)ng * 3);
   if (ng > 0) Check(mdgpu_get_ghosts(dev, gsrc.data(), gcode.data(), gq.data()), "download ghosts");
-  std::vector<int> row_of_id(type_of_id.size(), -1);
+  const bool tagged = unit_tag >= 0;
+  std::lock_guard<std::mutex> lock(g_types_mu);
   for (int i = 0; i < pn; i++) {
     for (int d = 0; d < 3; d++) {
       q[i][d] = hq[3 * (size_t)i + d];
       p[i][d] = hp[3 * (size_t)i + d];
     }
     ids[i] = hid[i];
-    type[i] = type_of_id[hid[i]];
-    row_of_id[hid[i]] = i;
+    type[i] = type_of_id(hid[i], tagged);
+  }
+  // ghosts: the owner's type; the owner's momentum when the owner is here
+  // (a ghost of another unit of a process grid gets 0: the reference's ghost
+  // momentum rows hold nothing either)
+  std::vector<int> row_of_id;
+  if (!tagged && ng > 0) {
+    row_of_id.assign(g_types[0].size(), -1);
+    for (int i = 0; i < pn; i++) row_of_id[hid[i]] = i;
   }
   for (int k = 0; k < ng; k++) {
     const int i = pn + k;
-    const int src = row_of_id[gsrc[k]];
+    const int src = tagged ? -1 : row_of_id[gsrc[k]];
     for (int d = 0; d < 3; d++) {
       q[i][d] = gq[3 * (size_t)k + d];
-      p[i][d] = p[src][d];
+      p[i][d] = src >= 0 ? p[src][d] : 0.0;
     }
     ids[i] = gsrc[k];
-    type[i] = type[src];
+    type[i] = type_of_id(gsrc[k], tagged);
   }
   particle_number = pn;
   total_particle_number = tn;
@@ -311,8 +356,7 @@ Variables::LoadFromStream(std::istream &fs) {
       fs.read((char *)&p[i][d], sizeof(double));
     }
     type[i] = 0;
-    ids[i] = static_cast<int>(type_of_id.size());
-    type_of_id.push_back(0);
+    ids[i] = NewId(0);
   }
   particle_number = pn;
   total_particle_number = pn;

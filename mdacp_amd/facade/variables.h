@@ -12,6 +12,9 @@
 //    read as before and there is no compile-time limit N;
 //  - each particle also carries a unique id used by the device to order
 //    particles inside a cell deterministically; `type` is the caller's value.
+//    The unit that adds a particle names it: its own counter, or, for the
+//    units of a process grid, unit id * 2^24 + counter, so that ids stay
+//    unique and reproducible when particles migrate between units.
 //----------------------------------------------------------------------
 #ifndef variables_h
 #define variables_h
@@ -28,9 +31,10 @@ private:
   int total_particle_number;
   double C0, C2;
   int capacity;                 // rows of the host mirrors (owned + ghosts)
-  std::vector<int> type_of_id;  // caller's type of each particle id
   int *ids;                     // particle id per mirror row
+  int unit_tag;                 // -1: ids are a plain counter; else the unit id of a process grid
   void Reserve(int n);
+  int NewId(int t);             // the next id of this unit, with its caller's type
 
 public:
   Variables(void);
@@ -65,6 +69,8 @@ public:
   bool host_stale;    // the device changed since the mirrors were filled
   bool host_dirty;    // the mirrors may hold changes the device lacks
   void AttachDevice(mdgpu_domain *d) {dev = d;}
+  void SetUnitTag(int tag) {unit_tag = tag;}  // before the first AddParticle
+  int DeviceParticleNumber(void);  // owned particles, current with the device
   void ToHost(void);    // refresh the mirrors (owned and ghosts) if stale
   void ToDevice(void);  // push particles added on the host and host changes
   void DeviceChanged(void) {host_stale = true;}

@@ -28,8 +28,11 @@ EXPORTS = [
     "mdgpu_set_positions", "mdgpu_clear_particles", "mdgpu_set_zeta", "mdgpu_get_pair_indices",
     "mdgpu_observe_kinetic", "mdgpu_create_ensemble", "mdgpu_replica_number", "mdgpu_replica_counts",
     "mdgpu_set_replica_thermostat", "mdgpu_observe_replicas", "mdgpu_get_replica_pairs",
-    "mdgpu_observe_noalloc",
+    "mdgpu_observe_noalloc", "mdgpu_create_unit", "mdgpu_unit_send_particles", "mdgpu_unit_receive_particles",
+    "mdgpu_unit_adjust_periodic_boundary", "mdgpu_unit_find_border_particles", "mdgpu_group_make_pair_list",
 ]
+# directions of a unit's faces (mdconfig.h:25)
+D_LEFT, D_RIGHT, D_BACK, D_FORWARD, D_DOWN, D_UP = range(6)
 OPT_UNROLL, OPT_KAHAN, OPT_WAVES, OPT_LDS_KB, OPT_BRICK, OPT_GRAPH = 1, 2, 3, 4, 5, 6
 OPT_SEARCH_WAVES, OPT_SEARCH_STAGE, OPT_PRUNE, OPT_ROW_SORT, OPT_BRICK_KB, OPT_SEARCH_CELLS = 7, 8, 9, 10, 11, 12
 OPT_RUN_AHEAD, OPT_PIPELINE, OPT_MIXED_FRAME, OPT_DISPLACEMENT_CHECK = 13, 14, 15, 16
@@ -44,6 +47,11 @@ class Params(ctypes.Structure):
 class Replica(ctypes.Structure):
     _fields_ = [("L", ctypes.c_double * 3), ("T", ctypes.c_double), ("gamma", ctypes.c_double),
                 ("seed", ctypes.c_uint64)]
+
+
+class UnitSpec(ctypes.Structure):
+    _fields_ = [("grid", ctypes.c_int * 3), ("pos", ctypes.c_int * 3), ("s", ctypes.c_double * 3),
+                ("e", ctypes.c_double * 3)]
 
 
 class MDGPUError(RuntimeError):
@@ -107,6 +115,13 @@ def load():
             "mdgpu_set_replica_thermostat": [P, I, D, D, ctypes.c_uint64],
             "mdgpu_observe_replicas": [P, P, P, P],
             "mdgpu_get_replica_pairs": [P, I, PL, P],
+            "mdgpu_create_unit": [ctypes.POINTER(Params), ctypes.POINTER(UnitSpec), P,
+                                  ctypes.POINTER(ctypes.c_void_p)],
+            "mdgpu_unit_send_particles": [P, I, PI],
+            "mdgpu_unit_receive_particles": [P, P],
+            "mdgpu_unit_adjust_periodic_boundary": [P],
+            "mdgpu_unit_find_border_particles": [P],
+            "mdgpu_group_make_pair_list": [P],
         }
         for name, args in sig.items():
             f = getattr(L, name)
@@ -429,3 +444,74 @@ class Ensemble(Domain):
         out = np.zeros((n.value, 3), dtype=np.int32)
         self._chk(load().mdgpu_get_replica_pairs(self.h, int(r), ctypes.byref(n), _ptr(out)))
         return out
+
+
+class Unit(Domain):
+    """One unit of a process grid on the GPU (mdgpu_create_unit): the box [s, e) of a grid
+    of units over the box L, in the group of `join` (None: a new group)."""
+
+    def __init__(self, L, grid, pos, s, e, join=None, dt=0.001, rc=2.5, margin=0.5, precision=FP64, capacity=0,
+                 check_list_length=500, device=0):
+        lib = load()
+        Lx, Ly, Lz = (float(L),) * 3 if np.isscalar(L) else (float(v) for v in L)
+        prm = Params((ctypes.c_double * 3)(Lx, Ly, Lz), dt, rc, margin, check_list_length, precision,
+                     int(capacity), device)
+        spec = UnitSpec((ctypes.c_int * 3)(*grid), (ctypes.c_int * 3)(*pos), (ctypes.c_double * 3)(*s),
+                        (ctypes.c_double * 3)(*e))
+        h = ctypes.c_void_p()
+        rc_ = lib.mdgpu_create_unit(ctypes.byref(prm), ctypes.byref(spec), join.h if join is not None else None,
+                                    ctypes.byref(h))
+        if rc_ != OK:
+            msg = lib.mdgpu_last_error(join.h).decode() if join is not None else "mdgpu_create_unit failed"
+            raise MDGPUError(rc_, msg)
+        self.h = h
+        self.L = np.array([Lx, Ly, Lz])
+        self.grid, self.pos = tuple(grid), tuple(pos)
+        self.s, self.e = np.array(s, dtype=float), np.array(e, dtype=float)
+        self.dt = dt
+        self.args = dict(dt=dt, rc=rc, margin=margin, precision=precision, check_list_length=check_list_length,
+                         device=device)
+        self.thermo = (NVE, 1.0, 0.1, 0.1, 0)
+
+    def send_particles(self, direction):
+        n = ctypes.c_int()
+        self._chk(load().mdgpu_unit_send_particles(self.h, direction, ctypes.byref(n)))
+        return n.value
+
+    def receive_particles(self, src):
+        self._chk(load().mdgpu_unit_receive_particles(self.h, src.h))
+
+    def adjust_periodic_boundary(self):
+        self._chk(load().mdgpu_unit_adjust_periodic_boundary(self.h))
+
+    def find_border_particles(self):
+        self._chk(load().mdgpu_unit_find_border_particles(self.h))
+
+    def group_make_pair_list(self):
+        self._chk(load().mdgpu_group_make_pair_list(self.h))
+
+
+def unit_boxes(L, grid):
+    """Boxes of a grid of units over the box L, unit id = x + gx * (y + gy * z), each
+    [ul * pos, ul * pos + ul) per axis with ul = L / g (MDUnit::MDUnit, mdunit.cc:13-24)."""
+    L = [float(L)] * 3 if np.isscalar(L) else [float(v) for v in L]
+    out = []
+    for z in range(grid[2]):
+        for y in range(grid[1]):
+            for x in range(grid[0]):
+                p = (x, y, z)
+                s, e = [], []
+                for a in range(3):
+                    ul = L[a] / float(grid[a])
+                    s.append(ul * float(p[a]))
+                    e.append(s[-1] + ul)
+                out.append((p, s, e))
+    return out
+
+
+def unit_grid(L, grid, **kw):
+    """One Unit per box of unit_boxes, all in one group."""
+    units = []
+    for p, s, e in unit_boxes(L, grid):
+        units.append(Unit(L, grid, p, s, e, join=units[0] if units else None, **kw))
+    return units

@@ -5,7 +5,8 @@ container only; tests/test_gpu_programs.py runs the same program on the MI355X
 facade (oracle/_ref/mdacp_gpu) and compares with these, so the reference itself
 never runs on the GPU machine.
 
-For each project configuration in PROGRAMS: the rows the program prints
+For each project configuration in PROGRAMS (OMP_NUM_THREADS = its `threads`, 1 by
+default): the rows the program prints
 (time, T[, P, E] per observation) and every cdview dump it writes
 (MDManager::SaveAsCdviewSequential, mdmanager.cc:81-105; lines
 "0 type x y z px py pz", mdunit.cc:52-67), as float arrays in
@@ -86,6 +87,12 @@ InitialVelocity=0.9
 """),
 }
 
+# The same programs with several units in one process (OMP_NUM_THREADS > 1: ParaInfo
+# splits the box into one unit per thread; each unit draws its own initial velocities
+# with its id as the seed, so these runs differ from the one-unit runs above)
+for _name, _threads in (("benchmark", 8), ("nosehoover", 8), ("collision", 4), ("burst", 8)):
+    PROGRAMS[f"{_name}_omp{_threads}"] = dict(PROGRAMS[_name], threads=_threads)
+
 
 def observe_rows(out, marker):
     """Rows of numbers the program printed (the observation lines)."""
@@ -124,13 +131,15 @@ def main():
         sys.exit("oracle/_ref/mdacp_cpu is not built: make -C oracle programs")
     arrays, meta = {}, {}
     for name, spec in PROGRAMS.items():
+        threads = spec.get("threads", 1)
         with tempfile.TemporaryDirectory() as td:
-            out, dumps = run_program(CPU, spec["cfg"], td)
+            out, dumps = run_program(CPU, spec["cfg"], td, {"OMP_NUM_THREADS": str(threads)})
         rows = observe_rows(out, spec["marker"])
         arrays[f"{name}/rows"] = rows
         for fn, a in dumps.items():
             arrays[f"{name}/{fn}"] = a
-        meta[name] = dict(cfg=spec["cfg"], marker=spec["marker"], rows=list(rows.shape), dumps=sorted(dumps))
+        meta[name] = dict(cfg=spec["cfg"], marker=spec["marker"], rows=list(rows.shape), dumps=sorted(dumps),
+                          threads=threads)
         print(name, rows.shape, {k: v.shape for k, v in dumps.items()}, flush=True)
     np.savez_compressed(os.path.join(HERE, "programs.npz"), **arrays)
     with open(os.path.join(HERE, "programs.json"), "w") as f:

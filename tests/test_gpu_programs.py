@@ -56,9 +56,9 @@ def run_gpu(name, tmp_path, env_extra=None, timeout=240):
     d.mkdir()
     cfg = d / "input.cfg"
     cfg.write_text(meta["cfg"])
-    env = dict(os.environ, OMP_NUM_THREADS="1")
+    env = dict(os.environ, OMP_NUM_THREADS=str(meta.get("threads", 1)))
     env.update(env_extra or {})
-    print(f"running mdacp_gpu {name} {env_extra or ''}", flush=True)
+    print(f"running mdacp_gpu {name} threads={env['OMP_NUM_THREADS']} {env_extra or ''}", flush=True)
     r = subprocess.run([GPU, str(cfg)], cwd=d, env=env, capture_output=True, text=True, timeout=timeout)
     assert r.returncode == 0, f"mdacp_gpu failed ({r.returncode}):\n{r.stdout[-2000:]}\n{r.stderr[-2000:]}"
     got = {fn: np.loadtxt(d / fn, ndmin=2) for fn in sorted(os.listdir(d)) if fn.endswith(".cd")}
@@ -135,6 +135,48 @@ def test_burst_project_and_cdview_dumps(tmp_path):
     """Burst project (burst.cc:103-142): a droplet thermalised with the Nose-Hoover bath, then
     NVE: t, T, P, E rows and the cdview dumps of both phases."""
     gpu, cpu, got, want = run_gpu("burst", tmp_path)
+    assert cpu.shape == gpu.shape == (3, 4)
+    np.testing.assert_allclose(gpu, cpu, rtol=2e-5, atol=2e-5)
+    assert sorted(got) == sorted(want)
+    for fn in sorted(want):
+        dq, dp = match_dump(got[fn], want[fn], pos_tol=1.5e-4, mom_tol=1.5e-5)
+        print(fn, "max |dq|", dq, "max |dp|", dp, flush=True)
+
+
+def test_benchmark_project_with_eight_units(tmp_path):
+    """OMP_NUM_THREADS=8: MDManager makes 2x2x2 units in the process (ParaInfo), each a device
+    unit of one group on the GPU; particles migrate and ghosts come device to device.  The
+    Benchmark project's configuration and initial velocities are per unit
+    (ExtractConfigurationMaker seeds each unit's lattice vacancies with its id,
+    confmaker.cc:174-230), so this run is its own trajectory: the reference's own 8-thread run
+    prints the fixture's rows, to the last printed digit."""
+    gpu, cpu, _, _ = run_gpu("benchmark_omp8", tmp_path)
+    assert cpu.shape == gpu.shape == (4, 4), (cpu, gpu)
+    np.testing.assert_allclose(gpu, cpu, rtol=2e-5, atol=2e-5)
+
+
+def test_nose_hoover_with_eight_units(tmp_path):
+    """The same with the Nose-Hoover bath: zeta from the temperature summed over the units."""
+    gpu, cpu, _, _ = run_gpu("nosehoover_omp8", tmp_path)
+    assert cpu.shape == gpu.shape == (3, 4)
+    np.testing.assert_allclose(gpu, cpu, rtol=2e-5, atol=2e-5)
+
+
+def test_collision_with_four_units(tmp_path):
+    """Collision project on four units: the droplets cross unit faces; rows and cdview dumps
+    (written unit after unit, matched by position)."""
+    gpu, cpu, got, want = run_gpu("collision_omp4", tmp_path)
+    assert cpu.shape == gpu.shape == (4, 2)
+    np.testing.assert_allclose(gpu, cpu, rtol=2e-5, atol=2e-5)
+    assert sorted(got) == sorted(want)
+    for fn in sorted(want):
+        dq, dp = match_dump(got[fn], want[fn], pos_tol=1.5e-4, mom_tol=1.5e-5)
+        print(fn, "max |dq|", dq, "max |dp|", dp, flush=True)
+
+
+def test_burst_with_eight_units(tmp_path):
+    """Burst project on eight units: Nose-Hoover thermalisation, then NVE, rows and dumps."""
+    gpu, cpu, got, want = run_gpu("burst_omp8", tmp_path)
     assert cpu.shape == gpu.shape == (3, 4)
     np.testing.assert_allclose(gpu, cpu, rtol=2e-5, atol=2e-5)
     assert sorted(got) == sorted(want)
