@@ -235,7 +235,9 @@ def reference_cpu():
                  sample=(f"kaityo256/mdacp Benchmark project compiled unmodified, {run['threads']} OpenMP "
                          f"thread(s), N={run['particles']} vacancy FCC rho 0.5, rc {r['rc']} margin {r['margin']} "
                          f"(compiled in), {run['total_loop']} timed steps, MUPS line of benchmark.cc:45-51; "
-                         f"timed in the build container, not on this machine"))
+                         f"timed in the build container, not on this machine; the fastest of "
+                         f"{len(run.get('samples_mups', [run['mups']]))} runs on shared virtual CPUs "
+                         f"(MUPS {run.get('samples_mups', [run['mups']])})"))
             for run in r["runs"]]
 
 

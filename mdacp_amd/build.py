@@ -33,28 +33,13 @@ def sources():
     return [os.path.join(CSRC, "mdgpu.hip")]
 
 
-PROBE = os.path.join(HERE, "copy_probe")
-
-
 def build(verbose=True):
     cmd = [HIPCC, *HIP_FLAGS, "-I", os.path.join(ROOT, "include"), "-I", CSRC, *sources(), "-o", LIB + ".tmp"]
     if verbose:
         print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True)
     os.replace(LIB + ".tmp", LIB)
-    build_probe(verbose)
     return LIB
-
-
-def build_probe(verbose=True):
-    """The copy kernel that calibrates the FETCH_SIZE / WRITE_SIZE counters."""
-    flags = [f for f in HIP_FLAGS if f not in ("-fPIC", "-shared")]
-    cmd = [HIPCC, *flags, os.path.join(CSRC, "copy_probe.hip"), "-o", PROBE + ".tmp"]
-    if verbose:
-        print(" ".join(cmd), flush=True)
-    subprocess.run(cmd, check=True)
-    os.replace(PROBE + ".tmp", PROBE)
-    return PROBE
 
 
 if __name__ == "__main__":

@@ -5,10 +5,19 @@
 #include "observer.h"
 //----------------------------------------------------------------------
 namespace {
-// pe and virial over the pair list of the resident state
+// pe and virial over the pair list of the resident state: one pass per state,
+// shared by the observers that ask for it (MDManager::Pressure and
+// TotalEnergy each ask at every observation)
 void observe(Variables *vars, double *ke, double *pe, double *vir) {
   vars->ToDevice();
-  vars->Check(mdgpu_observe(vars->dev, ke, pe, vir), "observer");
+  if (!vars->pair_sums_valid) {
+    double *s = vars->pair_sums;
+    vars->Check(mdgpu_observe(vars->dev, &s[0], &s[1], &s[2]), "observer");
+    vars->pair_sums_valid = true;
+  }
+  *ke = vars->pair_sums[0];
+  *pe = vars->pair_sums[1];
+  *vir = vars->pair_sums[2];
 }
 }  // namespace
 //----------------------------------------------------------------------

@@ -71,6 +71,7 @@ Variables::Variables(void) {
   dev_count = 0;
   host_stale = false;
   host_dirty = false;
+  pair_sums_valid = false;
   Reserve(1024);
 }
 //----------------------------------------------------------------------
@@ -211,6 +212,7 @@ Variables::ToHost(void) {
 void
 Variables::ToDevice(void) {
   if (dev == NULL) return;
+  if (host_dirty || particle_number > dev_count) pair_sums_valid = false;
   if (host_dirty && dev_count > 0) {
     std::vector<double> hq((size_t)dev_count * 3), hp((size_t)dev_count * 3);
     for (int i = 0; i < dev_count; i++) {

@@ -73,7 +73,12 @@ public:
   int DeviceParticleNumber(void);  // owned particles, current with the device
   void ToHost(void);    // refresh the mirrors (owned and ghosts) if stale
   void ToDevice(void);  // push particles added on the host and host changes
-  void DeviceChanged(void) {host_stale = true;}
+  void DeviceChanged(void) {host_stale = true; pair_sums_valid = false;}
+  // the last mdgpu_observe pair sums of the resident state (kinetic energy,
+  // potential energy, virial), kept until the state changes so that the
+  // potential-energy and virial observers of one observation share one pass
+  bool pair_sums_valid;
+  double pair_sums[3];
   void Check(int status, const char *what);  // report a device error and exit(1)
 };
 //----------------------------------------------------------------------

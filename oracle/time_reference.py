@@ -61,17 +61,28 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--loops", type=int, default=300)
     ap.add_argument("--thermalize", type=int, default=150)
+    ap.add_argument("--repeat", type=int, default=2,
+                    help="runs per configuration; the build container's virtual CPUs are shared, so the "
+                         "fastest run is the one reported")
     args = ap.parse_args()
     if not os.path.exists(BIN):
         raise SystemExit("oracle/_ref/mdacp_cpu is not built: make -C oracle programs")
-    runs = [run(1, 100, args.loops, args.thermalize), run(8, 200, args.loops, args.thermalize)]
+    runs = []
+    for threads, size in ((1, 100), (8, 200)):
+        samples = [run(threads, size, args.loops, args.thermalize) for _ in range(args.repeat)]
+        best = max(samples, key=lambda r: r["mups"])
+        best = dict(best, samples_mups=[r["mups"] for r in samples])
+        runs.append(best)
     res = dict(program="kaityo256/mdacp Benchmark project, compiled unmodified (g++ -O3 -fopenmp, makefile:14), "
                        "CalculateForceNext kernel (default build)",
-               rc=3.0, margin=0.3, dt=0.001, host=cpu_model(), host_cores=os.cpu_count(), runs=runs)
+               rc=3.0, margin=0.3, dt=0.001, host=cpu_model(), host_cores=os.cpu_count(),
+               note="virtual CPUs of a shared host: runs of one configuration vary; the fastest is reported",
+               runs=runs)
     with open(OUT, "w") as f:
         json.dump(res, f, indent=1)
     for r in runs:
-        print(f"threads={r['threads']} N={r['particles']} {r['seconds']:.2f} s {r['mups']:.3f} MUPS", flush=True)
+        print(f"threads={r['threads']} N={r['particles']} {r['seconds']:.2f} s {r['mups']:.3f} MUPS "
+              f"(samples {r['samples_mups']})", flush=True)
 
 
 if __name__ == "__main__":
