@@ -19,6 +19,10 @@ Langevin replicas (N = 4 k^3 drawn log-uniform in [2048, 65536], density in
 dt = 0.005, all advanced by one pair-force launch per step replayed from a
 captured graph; particle-updates/s summed over the replicas, FP64 and mixed.
 
+--workload units: the default workload on the units of a process grid (--grid, default
+2,2,2: MDManager's decomposition for OMP_NUM_THREADS=8), all on this GPU, with
+device-to-device migration and halo between them.
+
 --workload quench: 16,384,000 particles (FCC, 4*160^3) at density 0.75,
 Langevin at T = 0.9 (gamma = 1, dt = 0.005) for --equil steps, then
 MDManager::ChangeScale to density 0.60 and NVE at dt = 0.001 from that state,
@@ -497,9 +501,112 @@ def main_quench(args):
     print(json.dumps(line), flush=True)
 
 
+def run_units_gpu(precision, x, p, L, grid, args, observe_every=100):
+    """The default workload on the units of a process grid (mdgpu_create_unit): MDManager's
+    decomposition with OMP_NUM_THREADS = gx*gy*gz threads, every unit on this GPU, stepped
+    by mdgpu_group_calculate; particle-updates/s amortised as for the single domain."""
+    from mdacp_amd import mdgpu
+
+    n = len(x)
+    g = mdgpu.UnitGroup(L, grid, dt=args.dt, rc=args.rc, margin=args.margin, precision=precision, capacity=n)
+    g.add_particles(x, p)
+    g.make_pair_list()
+    rate, therm_rebuilds, therm_steps = thermalize(g, args.thermalize)
+    g.calculate(args.warmup)
+    g.calculate(20)
+    g.observables()
+    rebuilds = 0
+    n_obs = 0
+    t0 = time.perf_counter()
+    done = 0
+    while done < args.steps:
+        k = 1 if done == 0 else min(observe_every, args.steps - done)
+        rebuilds += g.calculate(k)
+        done += k
+        if (done - 1) % observe_every == 0:
+            g.observables()
+            n_obs += 1
+    elapsed = time.perf_counter() - t0
+    t_build = construction_ms(g)
+    t_obs = observation_ms(g)
+    pu_s, ms_step = amortised(n, args.steps, elapsed, rebuilds, n_obs, rate, t_build, t_obs)
+    g.enable_timing(True)
+    g.calculate(args.steps)
+    tm = g.timing()
+    g.enable_timing(False)
+    res = dict(pu_s=pu_s, ms_per_step=ms_step, window_pu_s=args.steps * n / elapsed, rebuilds=rebuilds,
+               rebuilds_per_step=rate, construction_ms=t_build, observation_ms=t_obs,
+               cadence_sample=f"{therm_rebuilds} constructions in the last {therm_steps} of {args.thermalize} "
+                              f"thermalisation steps",
+               force_ms_per_step=tm["force_ms"] / args.steps, force_launches_per_step=tm["force_launches"] / args.steps,
+               step_wall_ms_timed_pass=tm["total_ms"] / args.steps, list=g.list_stats(),
+               units=[u.counts()[0] for u in g.units])
+    g.close()
+    return res
+
+
+def main_units(args):
+    from mdacp_amd import mdgpu
+    from mdacp_amd.ensemble import box_for
+    from mdacp_amd.lattice import fcc, initial_velocity
+
+    grid = tuple(int(v) for v in args.grid.split(","))
+    k = args.fcc_cells or 100
+    L = box_for(k, args.density)
+    x = fcc(L, args.density)
+    n = len(x)
+    p = initial_velocity(n, math.sqrt(3.0), 0)
+    print(f"# units {grid}: N={n} L={L} rho={args.density}", file=sys.stderr, flush=True)
+    r64 = run_units_gpu(mdgpu.FP64, x, p, L, grid, args)
+    print(f"# fp64 {r64}", file=sys.stderr, flush=True)
+    r32 = run_units_gpu(mdgpu.MIXED, x, p, L, grid, args)
+    print(f"# mixed {r32}", file=sys.stderr, flush=True)
+    bytes_pu, _, _, _ = algorithmic_per_update(args.density, args.rc, args.margin)
+    achieved = bytes_pu * n / (r64["force_ms_per_step"] * 1e-3) / 1e9
+    roofline = dict(bound="hbm", achieved=achieved, peak=HBM_PEAK_GBS, unit="GB/s", frac=achieved / HBM_PEAK_GBS,
+                    traffic=None, kernel="k_force_brick_fp64 (one launch per unit, summed over the units of a step)",
+                    kernel_avg_ms=r64["force_ms_per_step"], bytes_per_particle_update=bytes_pu)
+    ref = reference_cpu()
+    ref8 = [r for r in (ref or []) if r["cores"] == 8]
+    cpu = None
+    if ref8:
+        cpu = dict(ref8[0], sample="the reference's own program with the same decomposition: " + ref8[0]["sample"])
+    line = {
+        "metric": METRIC,
+        "value": r64["pu_s"],
+        "unit": "particle-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": r64["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic: FCC lattice (confmaker.cc:8-44), |v0|=sqrt(3) random directions (variables.cc:82-108)",
+        "config": {"workload": (f"LJ NVE N={n} FCC rho={args.density} L={L:.6g} rc={args.rc} margin={args.margin} "
+                                f"dt={args.dt} on {grid[0]}x{grid[1]}x{grid[2]} units of a process grid on one GPU "
+                                f"(MDManager with OMP_NUM_THREADS={grid[0] * grid[1] * grid[2]}: device-to-device "
+                                f"migration and halo); {args.thermalize} thermalisation steps"),
+                   "particles": n, "grid": list(grid), "units": r64["units"],
+                   "rebuilds_in_timed_steps": r64["rebuilds"]},
+        "amortisation": {"rebuilds_per_step": r64["rebuilds_per_step"], "cadence_sample": r64["cadence_sample"],
+                         "construction_ms": r64["construction_ms"], "observation_ms": r64["observation_ms"],
+                         "window_pu_s": r64["window_pu_s"]},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "force_launches_per_step": r64["force_launches_per_step"],
+        "mixed": {"value": r32["pu_s"], "ms_per_step": r32["ms_per_step"], "speedup_vs_fp64": r32["pu_s"] / r64["pu_s"],
+                  "force_ms_per_step": r32["force_ms_per_step"]},
+        "pair_list": r64["list"],
+    }
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("--workload", choices=["single", "ensemble", "quench"], default="single")
+    ap.add_argument("--workload", choices=["single", "ensemble", "quench", "units"], default="single")
+    ap.add_argument("--grid", default="2,2,2", help="units: units per axis (the reference's OMP_NUM_THREADS=8 grid)")
     ap.add_argument("--equil", type=int, default=200, help="quench: Langevin steps before the rescale")
     ap.add_argument("--replicas", type=int, default=256)
     ap.add_argument("--gpus", type=int, default=1)
@@ -523,7 +630,7 @@ def main():
               f"ghost images every step, which is not built)", file=sys.stderr)
         sys.exit(2)
     if args.thermalize is None:
-        args.thermalize = {"single": 3000, "ensemble": 400, "quench": 0}[args.workload]
+        args.thermalize = {"single": 3000, "ensemble": 400, "quench": 0, "units": 3000}[args.workload]
     if args.workload == "ensemble":
         if args.dt is None:
             args.dt = 0.005
@@ -533,6 +640,9 @@ def main():
         return
     if args.dt is None:
         args.dt = 0.001
+    if args.workload == "units":
+        main_units(args)
+        return
     if args.workload == "quench":
         if args.fcc_cells is None:
             args.fcc_cells = 160

@@ -66,6 +66,8 @@
  *                                 for the unit: its ghosts from their owners' positions
  *   mdgpu_group_make_pair_list    MDManager::MakePairList (mdmanager.cc:262-280) for every
  *                                 unit of the group, in the reference's order
+ *   mdgpu_group_calculate         MDManager::Calculate (mdmanager.cc:116-192) for every unit
+ *                                 of the group, n steps
  * MDManager calls each unit's operations between its phase barriers
  * (omp parallel for): a unit's work is queued on the group's stream after the
  * work of every unit's previous phase.  The other step operations
@@ -145,8 +147,7 @@ typedef struct {
 /* A unit of the grid, with prm's parameters (prm->L the whole box).  join:
    NULL for the first unit of a group, else any unit of the group (same grid,
    box and device, another grid position; at most 64 units).  Each edge of the
-   unit's box must hold two search lengths (rc + margin).  A unit keeps its
-   rows in cell order (MDGPU_OPT_ROW_SORT 0). */
+   unit's box must hold two search lengths (rc + margin). */
 int mdgpu_create_unit(const mdgpu_params *prm, const mdgpu_unit *u, mdgpu_domain *join, mdgpu_domain **out);
 /* dir: D_LEFT, D_RIGHT, D_BACK, D_FORWARD, D_DOWN, D_UP = 0..5 (mdconfig.h:25);
    nsend: particles now in the unit's send buffer */
@@ -156,6 +157,13 @@ int mdgpu_unit_adjust_periodic_boundary(mdgpu_domain *d);
 /* needs every unit of the group through mdgpu_unit_adjust_periodic_boundary */
 int mdgpu_unit_find_border_particles(mdgpu_domain *d);
 int mdgpu_group_make_pair_list(mdgpu_domain *d);
+/* nsteps of MDManager::Calculate (mdmanager.cc:116-192) for every unit of d's
+   group: each unit's expiry test (OR'ed), the group's construction when any
+   expired, then the half drifts, halo and force of every unit, with the
+   thermostat of d (NVE, Langevin, or Nose-Hoover on the group's temperature).
+   With mdgpu_enable_timing on d, every unit records its force launches
+   (mdgpu_get_timing of each unit) and d the wall time of the steps. */
+int mdgpu_group_calculate(mdgpu_domain *d, int nsteps, int *rebuilds);
 
 /* replicas of a domain (1 for mdgpu_create) and owned particles of each;
    mdgpu_get_particles returns them replica after replica */

@@ -393,6 +393,112 @@ struct UnitPos {
   const double4 *pos[kMaxUnits];
 };
 
+// The row order each unit of a group gave its owned particles at its last
+// construction (old index -> new), by unit slot.
+struct UnitRowInv {
+  const int *inv[kMaxUnits];
+};
+
+// Every unit of a group in one launch: the flat index t of a launch over all
+// units' items (owned particles, ghosts or blocks) belongs to the last unit u
+// with base[u] <= t (units without items share their successor's base).
+__device__ __forceinline__ int group_unit_of(const int *base, int n, int t) {
+  int lo = 0, hi = n - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (base[mid] <= t) lo = mid; else hi = mid - 1;
+  }
+  return lo;
+}
+
+// The owned particles of every unit (UpdatePositionHalf, fcalculator.cc:22-32,
+// with k_drift's operations).
+struct GroupDrift {
+  double4 *pos[kMaxUnits];
+  const double4 *mom[kMaxUnits];
+  int base[kMaxUnits + 1];
+  int n;
+};
+
+__global__ void k_group_drift(GroupDrift G, double dt2) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= G.base[G.n]) return;
+  const int u = group_unit_of(G.base, G.n, t);
+  const int i = t - G.base[u];
+  double4 q = G.pos[u][i];
+  const double4 p = G.mom[u][i];
+  q.x += p.x * dt2;
+  q.y += p.y * dt2;
+  q.z += p.z * dt2;
+  G.pos[u][i] = q;
+}
+
+// The ghosts of every unit from their sources' current positions (k_unit_refresh
+// for the whole group).
+struct GroupHalo {
+  double4 *pos[kMaxUnits];
+  const int *gsrc[kMaxUnits];
+  const int *gcode[kMaxUnits];
+  const int *gunit[kMaxUnits];
+  int pn[kMaxUnits];
+  int base[kMaxUnits + 1];
+  int n;
+};
+
+__global__ void k_group_refresh(GroupHalo G, double Lx, double Ly, double Lz) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= G.base[G.n]) return;
+  const int u = group_unit_of(G.base, G.n, t);
+  const int k = t - G.base[u];
+  const double L[3] = {Lx, Ly, Lz};
+  const double3 y = image_of(G.pos[G.gunit[u][k]][G.gsrc[u][k]], G.gcode[u][k], L);
+  G.pos[u][G.pn[u] + k] = make_double4(y.x, y.y, y.z, 0.0);
+}
+
+// max |p|^2 of every unit: blocks aligned to units (bbase: first block of each
+// unit), a partial maximum per block, then one workgroup per unit.
+struct GroupMomenta {
+  const double4 *mom[kMaxUnits];
+  int pn[kMaxUnits];
+  int bbase[kMaxUnits + 1];
+  int n;
+};
+
+__global__ void k_group_max_p2(GroupMomenta G, double *__restrict__ partial) {
+  const int b = blockIdx.x;
+  const int u = group_unit_of(G.bbase, G.n, b);
+  const int i = (b - G.bbase[u]) * blockDim.x + threadIdx.x;
+  double v = 0.0;
+  if (i < G.pn[u]) {
+    const double4 p = G.mom[u][i];
+    v = p.x * p.x + p.y * p.y + p.z * p.z;
+  }
+  block_max_to_partial(v, partial, b);
+}
+
+__global__ void k_group_reduce_max(GroupMomenta G, const double *__restrict__ partial,
+                                   unsigned long long *__restrict__ out) {
+  __shared__ double red[kBlock];
+  const int u = blockIdx.x;
+  double m = 0.0;
+  for (int k = G.bbase[u] + threadIdx.x; k < G.bbase[u + 1]; k += blockDim.x) m = fmax(m, partial[k]);
+  red[threadIdx.x] = m;
+  __syncthreads();
+  for (int o = blockDim.x / 2; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + o]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[u] = (unsigned long long)__double_as_longlong(red[0]);
+}
+
+// A unit's ghost sources follow their source units' row orders: the indices
+// were taken before the sources ordered their rows.
+__global__ void k_remap_unit_ghosts(int *__restrict__ gsrc, const int *__restrict__ gunit, int ng, UnitRowInv R) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= ng) return;
+  gsrc[k] = R.inv[gunit[k]][gsrc[k]];
+}
+
 // MDManager::SendBorderParticles for a unit (mdmanager.cc:283-348): every
 // ghost from its source unit's current position plus its shift.
 __global__ void k_unit_refresh(double4 *__restrict__ pos, const int *__restrict__ gsrc,

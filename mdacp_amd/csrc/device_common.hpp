@@ -81,7 +81,8 @@ __global__ void k_reduce_max(const double *__restrict__ partial, int n, unsigned
 // concatenated in the domain's arrays, each replica's contiguous, and every
 // kernel that needs geometry finds its replica here.  A single-domain run has
 // one replica.  The box of a replica is [0, L) (MDRect of a 1x1x1 grid,
-// mdunit.cc:17-24); its extended cell grid is MeshList's (meshlist.cc:31-55).
+// mdunit.cc:17-24), or [o, o + L) for a unit of a larger grid; its extended
+// cell grid is MeshList's (meshlist.cc:31-55).
 struct Replica {
   double L[3];     // box edges
   double o[3];     // box origin (MDRect's start): cells count from it; 0 for a periodic box

@@ -68,6 +68,10 @@ struct UnitGroup {
   std::vector<mdgpu_domain *> units;  // by slot; nullptr once destroyed
   int live = 0;
   long epoch = 0;  // bumped when a unit is destroyed: ghosts placed before then may image it
+  // the whole-group expiry test: per-block maxima, then each unit's max |p|^2 (bits) and its pinned copy
+  double *part = nullptr;
+  size_t part_n = 0;
+  unsigned long long *vmax2 = nullptr, *h_vmax2 = nullptr;
 };
 }  // namespace
 
@@ -134,6 +138,7 @@ struct mdgpu_domain {
   bool own_stream = true;      // false: the group's stream
   bool ghosts_ready = false;   // ghosts placed for the current owned particles and their order
   long ghost_epoch = -1;       // the group's epoch when they were placed
+  bool ghost_remap = false;    // ghost sources still index the sources' rows as they were before their row order
   int *gunit = nullptr;        // source unit of each ghost
   double4 *st_pos = nullptr, *st_mom = nullptr;  // send buffer of the last MakeBufferForSendingParticle
   int *st_id = nullptr;
@@ -708,7 +713,10 @@ int order_rows(mdgpu_domain *d) {
   if (ng > 0) {
     HIPCHK(d, hipMemcpyAsync(d->pos_tmp + pn, d->pos + pn, ng * sizeof(double4), hipMemcpyDeviceToDevice, st));
     HIPCHK(d, hipMemcpyAsync(d->cell_alt + pn, d->cell + pn, ng * sizeof(int), hipMemcpyDeviceToDevice, st));
-    hipLaunchKernelGGL(k_remap_owned, dim3(nblk(ng)), dim3(kBlock), 0, st, d->gsrc, (long long)ng, pn, d->rowinv);
+    // a unit's ghost sources index other units' rows: remapped once every
+    // unit of the group has ordered its own (remap_unit_ghosts)
+    if (!d->group)
+      hipLaunchKernelGGL(k_remap_owned, dim3(nblk(ng)), dim3(kBlock), 0, st, d->gsrc, (long long)ng, pn, d->rowinv);
   }
   if (d->gtotal > 0)
     hipLaunchKernelGGL(k_remap_owned, dim3(nblk(d->gtotal)), dim3(kBlock), 0, st, d->glist, d->gtotal, pn,
@@ -1030,11 +1038,11 @@ int max_p2(mdgpu_domain *d, double *v2) {
   return read_max(d, 0, v2);
 }
 
-// PairList::CheckByDisplacement (pairlist.cc:68-121)
-int check_by_displacement(mdgpu_domain *d, int *expired) {
+// PairList::CheckByDisplacement (pairlist.cc:68-121): the passes queued (the
+// outcome copied to h_i32), then read once the stream has run them.
+int enqueue_displacement_check(mdgpu_domain *d) {
   hipStream_t st = d->stream;
   const int tn = d->tn;
-  *expired = 0;
   hipLaunchKernelGGL(k_displacement, dim3(nblk(tn)), dim3(kBlock), 0, st, d->pos, d->pos_old, tn, d->disp,
                      d->maxpart);
   hipLaunchKernelGGL(k_reduce_max, dim3(1), dim3(kBlock), 0, st, d->maxpart, (int)nblk(tn), d->u64 + 1);
@@ -1052,7 +1060,12 @@ int check_by_displacement(mdgpu_domain *d, int *expired) {
   }
   CHK(check_launch(d, "displacement check"));
   HIPCHK(d, hipMemcpyAsync(d->h_i32, d->i32, sizeof(int) * 4, hipMemcpyDeviceToHost, st));
-  CHK(sync(d));
+  return MDGPU_OK;
+}
+
+int displacement_check_result(const mdgpu_domain *d, int *expired) {
+  const int tn = d->tn;
+  const int Nl = (tn > d->check_list_length) ? d->check_list_length : tn;
   const int count = d->h_i32[1];
   if (count > 0 && count >= Nl) {
     *expired = 1;
@@ -1060,6 +1073,13 @@ int check_by_displacement(mdgpu_domain *d, int *expired) {
   }
   *expired = count > 1 && d->h_i32[2] != 0;
   return MDGPU_OK;
+}
+
+int check_by_displacement(mdgpu_domain *d, int *expired) {
+  *expired = 0;
+  CHK(enqueue_displacement_check(d));
+  CHK(sync(d));
+  return displacement_check_result(d, expired);
 }
 
 // PairList::IsPairListExpired (pairlist.cc:26-45) with
@@ -1792,7 +1812,6 @@ int create_domain(const mdgpu_params *prm, int nrep, const mdgpu_replica *rl, co
       d->re[k] = d->rs[k] + d->L[k];
     }
   }
-  if (unit) d->row_sort = 0;  // rows stay in cell order: other units' ghosts index a unit's particles
   d->dt = prm->dt;
   d->rc = prm->rc;
   d->margin = prm->margin;
@@ -2053,9 +2072,36 @@ int unit_ghosts(mdgpu_domain *d) {
   d->tn = pn + ng;
   d->ghosts_ready = true;
   d->ghost_epoch = g->epoch;
+  d->ghost_remap = true;
   // the host table is read by the copies above before it is written again:
   // the next write follows this unit's next read_total
   return check_launch(d, "unit ghosts");
+}
+
+// The unit's ghost sources after every source unit has built its list and
+// ordered its rows (order_rows permutes a unit's owned particles): each
+// source index mapped through its unit's row order, once per construction.
+// The group's lock is held.
+int remap_unit_ghosts_locked(mdgpu_domain *d) {
+  UnitGroup *g = d->group;
+  if (!d->ghost_remap) return MDGPU_OK;
+  if (d->ghost_epoch != g->epoch) {
+    d->err = "a unit of the group was destroyed after this unit's ghosts were placed: rebuild the group's lists";
+    return MDGPU_ERR_ARGUMENT;
+  }
+  UnitRowInv R{};
+  for (size_t w = 0; w < g->units.size(); w++) {
+    const mdgpu_domain *W = g->units[w];
+    if (W && !W->initialized) {
+      d->err = "a unit's halo before every unit of its group built its pair list";
+      return MDGPU_ERR_ARGUMENT;
+    }
+    R.inv[w] = W ? W->rowinv : nullptr;
+  }
+  if (d->ng > 0)
+    hipLaunchKernelGGL(k_remap_unit_ghosts, dim3(nblk(d->ng)), dim3(kBlock), 0, d->stream, d->gsrc, d->gunit, d->ng, R);
+  d->ghost_remap = false;
+  return check_launch(d, "unit ghost sources");
 }
 
 // MDManager::SendBorderParticles for a unit (mdmanager.cc:283-348): each ghost
@@ -2070,6 +2116,7 @@ int unit_refresh(mdgpu_domain *d) {
       d->err = "a unit of the group was destroyed after this unit's ghosts were placed: rebuild the group's lists";
       return MDGPU_ERR_ARGUMENT;
     }
+    CHK(remap_unit_ghosts_locked(d));
     for (size_t w = 0; w < g->units.size(); w++) P.pos[w] = g->units[w] ? g->units[w]->pos : nullptr;
   }
   hipLaunchKernelGGL(k_unit_refresh, dim3(nblk(d->ng)), dim3(kBlock), 0, d->stream, d->pos, d->gsrc, d->gcode,
@@ -2100,6 +2147,10 @@ int ghost_source_ids(mdgpu_domain *d, std::vector<int> &out) {
   const int ng = d->ng;
   out.assign(ng, 0);
   if (ng == 0) return MDGPU_OK;
+  if (is_unit(d)) {
+    std::lock_guard<std::mutex> lock(d->group->mu);
+    CHK(remap_unit_ghosts_locked(d));
+  }
   std::vector<int> gs(ng), gu(ng, 0);
   HIPCHK(d, hipMemcpyAsync(gs.data(), d->gsrc, ng * sizeof(int), hipMemcpyDeviceToHost, d->stream));
   if (is_unit(d)) HIPCHK(d, hipMemcpyAsync(gu.data(), d->gunit, ng * sizeof(int), hipMemcpyDeviceToHost, d->stream));
@@ -2132,6 +2183,303 @@ int ghost_source_ids(mdgpu_domain *d, std::vector<int> &out) {
     out[k] = ids[gu[k]][gs[k]];
   }
   return MDGPU_OK;
+}
+
+// The live units of d's group, in slot order.
+std::vector<mdgpu_domain *> group_units(mdgpu_domain *d) {
+  std::vector<mdgpu_domain *> us;
+  std::lock_guard<std::mutex> lock(d->group->mu);
+  for (mdgpu_domain *u : d->group->units)
+    if (u) us.push_back(u);
+  return us;
+}
+
+// A unit's error, reported on the unit the caller called.
+int group_fail(mdgpu_domain *d, mdgpu_domain *u, int r) {
+  if (u != d) d->err = u->err;
+  return r;
+}
+
+// MDManager::MakePairList (mdmanager.cc:262-280) for every unit of the group.
+int group_make_pair_list(mdgpu_domain *d, const std::vector<mdgpu_domain *> &us) {
+  int r;
+  // MDManager::SendParticles (mdmanager.cc:194-260): every unit fills its
+  // buffer for a direction, then every unit takes its opposite neighbour's
+  for (int dir = 0; dir < 6; dir++) {
+    for (mdgpu_domain *u : us) {
+      int n = 0;
+      if ((r = unit_send(u, dir, &n)) != MDGPU_OK) return group_fail(d, u, r);
+    }
+    for (mdgpu_domain *u : us) {
+      mdgpu_domain *src = unit_neighbor(u, dir ^ 1);
+      if (!src) {
+        d->err = "the group does not fill its grid";
+        return MDGPU_ERR_ARGUMENT;
+      }
+      if ((r = unit_receive(u, src)) != MDGPU_OK) return group_fail(d, u, r);
+    }
+  }
+  for (mdgpu_domain *u : us)
+    if ((r = unit_adjust(u)) != MDGPU_OK) return group_fail(d, u, r);
+  for (mdgpu_domain *u : us)
+    if ((r = unit_ghosts(u)) != MDGPU_OK) return group_fail(d, u, r);
+  for (mdgpu_domain *u : us)
+    if ((r = make_pair_list(u)) != MDGPU_OK) return group_fail(d, u, r);
+  std::lock_guard<std::mutex> lock(d->group->mu);
+  for (mdgpu_domain *u : us)
+    if ((r = remap_unit_ghosts_locked(u)) != MDGPU_OK) return group_fail(d, u, r);
+  return MDGPU_OK;
+}
+
+// MDManager::IsPairListExpired (mdmanager.cc:369-377): every unit's
+// PairList::IsPairListExpired (pairlist.cc:26-65, is_expired), OR'ed.  The
+// units share a stream, so their maxima of |p| are reduced together and read
+// after one wait, and so are the displacement checks of the units whose budget
+// ran out.
+int group_expired(mdgpu_domain *d, const std::vector<mdgpu_domain *> &us, bool *any) {
+  *any = false;
+  std::vector<char> vmax_of(us.size(), 0), check_of(us.size(), 0);
+  bool queued = false;
+  for (size_t k = 0; k < us.size(); k++) {
+    mdgpu_domain *u = us[k];
+    if (!u->initialized) {
+      *any = true;
+      continue;
+    }
+    u->lifetime++;
+    if (u->buffer_length > 0.0) {
+      vmax_of[k] = 1;
+      queued = true;
+    }
+  }
+  if (queued) {
+    // every unit's max |p|^2 in two launches (the maximum does not depend on
+    // the order it is taken in), read after one wait
+    UnitGroup *g = d->group;
+    const int ns = (int)g->units.size();
+    GroupMomenta G{};
+    int nb = 0;
+    for (int w = 0; w < ns; w++) {
+      const mdgpu_domain *W = g->units[w];
+      G.mom[w] = W ? W->mom : nullptr;
+      G.pn[w] = W ? W->pn : 0;
+      G.bbase[w] = nb;
+      nb += (int)nblk(G.pn[w]);
+    }
+    G.bbase[ns] = nb;
+    G.n = ns;
+    if ((size_t)nb > g->part_n) {
+      if (g->part) (void)hipFree(g->part);
+      g->part = nullptr;
+      g->part_n = 0;
+      HIPCHK(d, hipMalloc((void **)&g->part, (size_t)nb * 2 * sizeof(double)));
+      g->part_n = (size_t)nb * 2;
+    }
+    if (!g->vmax2) {
+      HIPCHK(d, hipMalloc((void **)&g->vmax2, kMaxUnits * sizeof(unsigned long long)));
+      HIPCHK(d, hipHostMalloc((void **)&g->h_vmax2, kMaxUnits * sizeof(unsigned long long)));
+    }
+    hipLaunchKernelGGL(k_group_max_p2, dim3(nb), dim3(kBlock), 0, d->stream, G, g->part);
+    hipLaunchKernelGGL(k_group_reduce_max, dim3(ns), dim3(kBlock), 0, d->stream, G, g->part, g->vmax2);
+    CHK(check_launch(d, "group max |p|"));
+    HIPCHK(d, hipMemcpyAsync(g->h_vmax2, g->vmax2, ns * sizeof(unsigned long long), hipMemcpyDeviceToHost,
+                             d->stream));
+    CHK(sync(d));
+  }
+  for (size_t k = 0; k < us.size(); k++) {
+    if (!vmax_of[k]) continue;
+    mdgpu_domain *u = us[k];
+    double v2 = 0.0;
+    std::memcpy(&v2, d->group->h_vmax2 + u->unit_slot, sizeof(double));
+    const double vmax = std::sqrt(v2);
+    u->step_vmax = vmax;
+    if (vmax * 2.0 * u->dt > u->margin) {
+      char b[160];
+      snprintf(b, sizeof b, "Too fast particles exists. Try smaller time step (max_velocity = %f)", vmax);
+      u->err = b;
+      return group_fail(d, u, MDGPU_ERR_TOO_FAST);
+    }
+    u->buffer_length = u->buffer_length - vmax * 2.0 * u->dt;
+  }
+  queued = false;
+  for (size_t k = 0; k < us.size(); k++) {
+    mdgpu_domain *u = us[k];
+    if (!u->initialized || !(u->buffer_length < 0.0)) continue;
+    if (!u->disp_check) {
+      *any = true;
+      continue;
+    }
+    const int r = enqueue_displacement_check(u);
+    if (r != MDGPU_OK) return group_fail(d, u, r);
+    check_of[k] = 1;
+    queued = true;
+  }
+  if (queued) CHK(sync(d));
+  for (size_t k = 0; k < us.size(); k++) {
+    if (!check_of[k]) continue;
+    int e = 0;
+    CHK(displacement_check_result(us[k], &e));
+    if (e) *any = true;
+  }
+  return MDGPU_OK;
+}
+
+// UpdatePositionHalf of every unit in one launch.
+int group_drift(mdgpu_domain *d, const std::vector<mdgpu_domain *> &us) {
+  UnitGroup *g = d->group;
+  const int ns = (int)g->units.size();
+  GroupDrift G{};
+  int t = 0;
+  for (int w = 0; w < ns; w++) {
+    mdgpu_domain *W = g->units[w];
+    G.pos[w] = W ? W->pos : nullptr;
+    G.mom[w] = W ? W->mom : nullptr;
+    G.base[w] = t;
+    t += W ? W->pn : 0;
+  }
+  G.base[ns] = t;
+  G.n = ns;
+  for (mdgpu_domain *u : us) invalidate(u);
+  if (t > 0) hipLaunchKernelGGL(k_group_drift, dim3(nblk(t)), dim3(kBlock), 0, d->stream, G, d->dt * 0.5);
+  return check_launch(d, "group drift");
+}
+
+// MDManager::SendBorderParticles for every unit in one launch (each unit's
+// ghost sources remapped first where its construction left that pending).
+int group_refresh(mdgpu_domain *d) {
+  UnitGroup *g = d->group;
+  std::lock_guard<std::mutex> lock(g->mu);
+  const int ns = (int)g->units.size();
+  GroupHalo G{};
+  int t = 0;
+  for (int w = 0; w < ns; w++) {
+    mdgpu_domain *W = g->units[w];
+    G.pos[w] = W ? W->pos : nullptr;
+    G.base[w] = t;
+    if (!W || W->ng == 0) continue;
+    if (W->ghost_epoch != g->epoch) {
+      d->err = "a unit of the group was destroyed after the ghosts were placed: rebuild the group's lists";
+      return MDGPU_ERR_ARGUMENT;
+    }
+    const int r = remap_unit_ghosts_locked(W);
+    if (r != MDGPU_OK) return group_fail(d, W, r);
+    G.gsrc[w] = W->gsrc;
+    G.gcode[w] = W->gcode;
+    G.gunit[w] = W->gunit;
+    G.pn[w] = W->pn;
+    t += W->ng;
+  }
+  G.base[ns] = t;
+  G.n = ns;
+  if (t > 0)
+    hipLaunchKernelGGL(k_group_refresh, dim3(nblk(t)), dim3(kBlock), 0, d->stream, G, g->L[0], g->L[1], g->L[2]);
+  return check_launch(d, "group halo");
+}
+
+// Temperature over the whole group (MDManager::Temperature, mdmanager.cc:390-397).
+int group_temperature(mdgpu_domain *d, const std::vector<mdgpu_domain *> &us, double *t) {
+  double ke = 0.0;
+  long long n = 0;
+  for (mdgpu_domain *u : us) {
+    double k = 0.0;
+    const int r = kinetic(u, &k);
+    if (r != MDGPU_OK) return group_fail(d, u, r);
+    ke += k;
+    n += u->pn;
+  }
+  *t = (ke / (double)n) / 1.5;
+  return MDGPU_OK;
+}
+
+// MDManager::Calculate (mdmanager.cc:116-192) for every unit of the group,
+// nsteps times: the expiry test of every unit, the group's construction if any
+// expired, the first half drift of every unit, every unit's halo, then the
+// force and the second half drift (NVE), the force and the Langevin update
+// (Langevin), or the Nose-Hoover sequence with the group's temperature.  The
+// units' thermostat is the one d has.  With timing on (mdgpu_enable_timing on
+// d), every unit records its force launches and d the wall time of the steps.
+int group_calculate(mdgpu_domain *d, int nsteps, int *rebuilds) {
+  const std::vector<mdgpu_domain *> us = group_units(d);
+  for (mdgpu_domain *u : us)
+    if (u->thermostat != d->thermostat) {
+      d->err = "the units of a group step with one thermostat";
+      return MDGPU_ERR_ARGUMENT;
+    }
+  hipEvent_t t0 = nullptr, t1 = nullptr;
+  for (mdgpu_domain *u : us) {
+    u->force_launches = 0;
+    u->force_ms = u->total_ms = u->rebuild_ms = 0.0;
+    u->rebuild_count = 0;
+    u->timing = d->timing;
+    if (d->timing) {
+      const size_t need = (size_t)nsteps * 2 + 2;
+      while (u->ev.size() < need) {
+        hipEvent_t e;
+        HIPCHK(d, hipEventCreate(&e));
+        u->ev.push_back(e);
+      }
+    }
+  }
+  if (d->timing) {
+    HIPCHK(d, hipEventCreate(&t0));
+    HIPCHK(d, hipEventCreate(&t1));
+    HIPCHK(d, hipEventRecord(t0, d->stream));
+  }
+  int count = 0;
+  int r = MDGPU_OK;
+  for (int s = 0; s < nsteps && r == MDGPU_OK; s++) {
+    bool any = false;
+    if ((r = group_expired(d, us, &any)) != MDGPU_OK) break;
+    if (any) {
+      if ((r = group_make_pair_list(d, us)) != MDGPU_OK) break;
+      count++;
+    }
+    if ((r = group_drift(d, us)) != MDGPU_OK || (r = group_refresh(d)) != MDGPU_OK) break;
+    if (d->thermostat == MDGPU_NOSEHOOVER) {
+      double t = 0.0;
+      if ((r = group_temperature(d, us, &t)) != MDGPU_OK) break;
+      for (mdgpu_domain *u : us) {
+        if ((r = heatbath_zeta(u, t)) != MDGPU_OK || (r = heatbath_momenta(u)) != MDGPU_OK ||
+            (r = force_in_place(u)) != MDGPU_OK || (r = heatbath_momenta(u)) != MDGPU_OK)
+          break;
+      }
+      if (r != MDGPU_OK) break;
+      if ((r = group_temperature(d, us, &t)) != MDGPU_OK) break;
+      for (mdgpu_domain *u : us)
+        if ((r = heatbath_zeta(u, t)) != MDGPU_OK) break;
+      if (r != MDGPU_OK || (r = group_drift(d, us)) != MDGPU_OK) break;
+    } else if (d->thermostat == MDGPU_LANGEVIN) {
+      for (mdgpu_domain *u : us)
+        if ((r = force_in_place(u)) != MDGPU_OK || (r = mdgpu_langevin(u)) != MDGPU_OK) break;
+    } else {
+      for (mdgpu_domain *u : us)
+        if ((r = force_in_place(u)) != MDGPU_OK) break;
+      if (r != MDGPU_OK || (r = group_drift(d, us)) != MDGPU_OK) break;
+    }
+    if (r != MDGPU_OK) break;
+    for (mdgpu_domain *u : us) {
+      u->sim_time += u->dt;
+      if (d->thermostat != MDGPU_LANGEVIN) u->step++;  // mdgpu_langevin counts its own calls
+    }
+  }
+  if (r == MDGPU_OK) r = sync(d);
+  if (d->timing && t0) {
+    HIPCHK(d, hipEventRecord(t1, d->stream));
+    HIPCHK(d, hipEventSynchronize(t1));
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, t0, t1);
+    d->total_ms = ms;
+    for (mdgpu_domain *u : us)
+      for (int k = 0; k < u->force_launches; k++) {
+        (void)hipEventElapsedTime(&ms, u->ev[2 * k], u->ev[2 * k + 1]);
+        u->force_ms += ms;
+      }
+    (void)hipEventDestroy(t0);
+    (void)hipEventDestroy(t1);
+  }
+  d->rebuild_count = count;
+  if (rebuilds) *rebuilds = count;
+  return r;
 }
 
 }  // namespace
@@ -2246,41 +2594,16 @@ int mdgpu_group_make_pair_list(mdgpu_domain *d) {
     d->err = "not a unit (mdgpu_create_unit)";
     return MDGPU_ERR_ARGUMENT;
   }
-  UnitGroup *g = d->group;
-  std::vector<mdgpu_domain *> us;
-  {
-    std::lock_guard<std::mutex> lock(g->mu);
-    for (mdgpu_domain *u : g->units)
-      if (u) us.push_back(u);
+  return group_make_pair_list(d, group_units(d));
+}
+
+int mdgpu_group_calculate(mdgpu_domain *d, int nsteps, int *rebuilds) {
+  if (!d || nsteps < 0) return MDGPU_ERR_ARGUMENT;
+  if (!is_unit(d)) {
+    d->err = "not a unit (mdgpu_create_unit)";
+    return MDGPU_ERR_ARGUMENT;
   }
-  auto fail = [&](mdgpu_domain *u, int r) {
-    if (u != d) d->err = u->err;
-    return r;
-  };
-  int r;
-  // MDManager::SendParticles (mdmanager.cc:194-260): every unit fills its
-  // buffer for a direction, then every unit takes its opposite neighbour's
-  for (int dir = 0; dir < 6; dir++) {
-    for (mdgpu_domain *u : us) {
-      int n = 0;
-      if ((r = unit_send(u, dir, &n)) != MDGPU_OK) return fail(u, r);
-    }
-    for (mdgpu_domain *u : us) {
-      mdgpu_domain *src = unit_neighbor(u, dir ^ 1);
-      if (!src) {
-        d->err = "the group does not fill its grid";
-        return MDGPU_ERR_ARGUMENT;
-      }
-      if ((r = unit_receive(u, src)) != MDGPU_OK) return fail(u, r);
-    }
-  }
-  for (mdgpu_domain *u : us)
-    if ((r = unit_adjust(u)) != MDGPU_OK) return fail(u, r);
-  for (mdgpu_domain *u : us)
-    if ((r = unit_ghosts(u)) != MDGPU_OK) return fail(u, r);
-  for (mdgpu_domain *u : us)
-    if ((r = make_pair_list(u)) != MDGPU_OK) return fail(u, r);
-  return MDGPU_OK;
+  return group_calculate(d, nsteps, rebuilds);
 }
 
 int mdgpu_create_ensemble(const mdgpu_params *prm, int thermostat, int nrep, const mdgpu_replica *reps,
@@ -2383,6 +2706,9 @@ int mdgpu_destroy(mdgpu_domain *d) {
     }
     if (last) {
       (void)hipStreamDestroy(g->stream);
+      if (g->part) (void)hipFree(g->part);
+      if (g->vmax2) (void)hipFree(g->vmax2);
+      if (g->h_vmax2) (void)hipHostFree(g->h_vmax2);
       delete g;
     }
   }
@@ -3006,10 +3332,6 @@ int mdgpu_set_option(mdgpu_domain *d, int option, int value) {
       return MDGPU_OK;
     case MDGPU_OPT_ROW_SORT:
       if (value < 0 || value > 64) return MDGPU_ERR_ARGUMENT;
-      if (is_unit(d) && value != 0) {
-        d->err = "a unit keeps its rows in cell order: other units' ghosts index its particles";
-        return MDGPU_ERR_ARGUMENT;
-      }
       d->row_sort = value;
       d->initialized = false;  // the next step rebuilds the list
       return MDGPU_OK;

@@ -30,6 +30,7 @@ EXPORTS = [
     "mdgpu_set_replica_thermostat", "mdgpu_observe_replicas", "mdgpu_get_replica_pairs",
     "mdgpu_observe_noalloc", "mdgpu_create_unit", "mdgpu_unit_send_particles", "mdgpu_unit_receive_particles",
     "mdgpu_unit_adjust_periodic_boundary", "mdgpu_unit_find_border_particles", "mdgpu_group_make_pair_list",
+    "mdgpu_group_calculate",
 ]
 # directions of a unit's faces (mdconfig.h:25)
 D_LEFT, D_RIGHT, D_BACK, D_FORWARD, D_DOWN, D_UP = range(6)
@@ -122,6 +123,7 @@ def load():
             "mdgpu_unit_adjust_periodic_boundary": [P],
             "mdgpu_unit_find_border_particles": [P],
             "mdgpu_group_make_pair_list": [P],
+            "mdgpu_group_calculate": [P, I, PI],
         }
         for name, args in sig.items():
             f = getattr(L, name)
@@ -490,6 +492,12 @@ class Unit(Domain):
     def group_make_pair_list(self):
         self._chk(load().mdgpu_group_make_pair_list(self.h))
 
+    def group_calculate(self, nsteps=1):
+        """nsteps of MDManager::Calculate for every unit of the group; returns the constructions."""
+        r = ctypes.c_int()
+        self._chk(load().mdgpu_group_calculate(self.h, nsteps, ctypes.byref(r)))
+        return r.value
+
 
 def unit_boxes(L, grid):
     """Boxes of a grid of units over the box L, unit id = x + gx * (y + gy * z), each
@@ -515,3 +523,77 @@ def unit_grid(L, grid, **kw):
     for p, s, e in unit_boxes(L, grid):
         units.append(Unit(L, grid, p, s, e, join=units[0] if units else None, **kw))
     return units
+
+
+class UnitGroup:
+    """The units of a process grid as one system, as MDManager sees its units: particles added
+    to every unit (each keeps its box), the group's construction and steps
+    (mdgpu_group_make_pair_list, mdgpu_group_calculate), observers summed over the units."""
+
+    def __init__(self, L, grid, **kw):
+        self.units = unit_grid(L, grid, **kw)
+        self.L = self.units[0].L
+        self.grid = tuple(grid)
+
+    def add_particles(self, q, p=None, ids=None):
+        for u in self.units:
+            u.add_particles(q, p, ids)
+
+    def make_pair_list(self):
+        self.units[0].group_make_pair_list()
+
+    def calculate(self, nsteps=1):
+        return self.units[0].group_calculate(nsteps)
+
+    def counts(self):
+        c = [u.counts() for u in self.units]
+        return sum(x[0] for x in c), sum(x[1] for x in c)
+
+    def particles(self):
+        q, p, t = zip(*(u.particles() for u in self.units))
+        return np.concatenate(q), np.concatenate(p), np.concatenate(t)
+
+    def particles_by_id(self):
+        q, p, t = self.particles()
+        o = np.argsort(t, kind="stable")
+        return q[o], p[o], t[o]
+
+    def set_precision(self, precision):
+        for u in self.units:
+            u.set_precision(precision)
+
+    def set_thermostat(self, kind, T=1.0, gamma=0.1, tau=0.1):
+        for u in self.units:
+            u.set_thermostat(kind, T, gamma, tau)
+
+    def kinetic_energy(self):
+        return float(sum(u.kinetic_energy() for u in self.units))
+
+    def observe_raw(self):
+        return tuple(float(v) for v in np.sum([u.observe_raw() for u in self.units], axis=0))
+
+    def observables(self):
+        """T, P, E of the whole box (MDManager::Temperature/Pressure/TotalEnergy summed over
+        its units, mdmanager.cc:379-429)."""
+        ke, pe, vir = self.observe_raw()
+        pn = np.float64(self.counts()[0])
+        T = (ke / pn) / 1.5
+        V = self.L[0] * self.L[1] * self.L[2]
+        return dict(ke=ke, pe=pe, vir=vir, T=float(T), P=float((T - vir / pn) * pn / V), E=float(ke / pn + pe / pn))
+
+    def enable_timing(self, on=True):
+        self.units[0].enable_timing(on)
+
+    def timing(self):
+        """Force launches and their summed time over all units, the steps' wall time."""
+        t = [u.timing() for u in self.units]
+        return dict(force_launches=sum(x["force_launches"] for x in t), force_ms=sum(x["force_ms"] for x in t),
+                    total_ms=t[0]["total_ms"], rebuilds=t[0]["rebuilds"], rebuild_ms=0.0)
+
+    def list_stats(self):
+        st = [u.list_stats() for u in self.units]
+        return {k: sum(x[k] for x in st) for k in st[0]}
+
+    def close(self):
+        for u in self.units:
+            u.close()

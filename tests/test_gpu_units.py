@@ -105,12 +105,11 @@ def lattice_state(L, rho, v0, seed, jitter=0.0):
 
 def test_one_unit_places_the_periodic_domains_ghosts():
     """A 1x1x1 grid: the unit is its own neighbour on every face, and its ghosts, pair
-    list, kicks and observables are those of the periodic domain (its rows in cell order,
-    as a unit keeps them), bit for bit."""
+    list, kicks and observables are those of the periodic domain, bit for bit, with rows
+    sorted by list length (the default) and in cell order."""
     L = 16.0
     q, p, ids = lattice_state(L, 0.8, 1.0, 3, jitter=0.2)
     d = make_domain(L, q, p, ids)
-    d.set_option(mdgpu.OPT_ROW_SORT, 0)
     (u,) = make_units(L, (1, 1, 1), q, p, ids)
     say("one unit vs periodic domain, n =", len(q))
     d.make_pair_list()
@@ -122,6 +121,13 @@ def test_one_unit_places_the_periodic_domains_ghosts():
     assert np.array_equal(d.particles()[0], u.particles()[0])
     assert np.array_equal(d.kick(), u.kick())
     assert d.observe_raw() == u.observe_raw()
+    for x in (d, u):
+        x.set_option(mdgpu.OPT_ROW_SORT, 0)
+    d.make_pair_list()
+    u.group_make_pair_list()
+    assert np.array_equal(d.particles()[0], u.particles()[0])
+    assert np.array_equal(d.ghosts()[2], u.ghosts()[2])
+    assert np.array_equal(d.kick(), u.kick())
 
 
 @pytest.mark.parametrize("L,grid", [
@@ -243,8 +249,8 @@ def test_units_follow_the_periodic_domain(grid):
 
 
 def test_unit_refusals():
-    """What a unit refuses: the fused step, rescaling, row sorting, a second unit at one grid
-    position, a unit of another grid; a pair list before its group's construction."""
+    """What a unit refuses: the fused step, rescaling, a second unit at one grid position, a
+    unit of another grid; a pair list before its group's construction."""
     L = 20.0
     units = mdgpu.unit_grid(L, (2, 1, 1), rc=RC, margin=MARGIN)
     q, p, ids = lattice_state(L, 0.5, 1.0, 2)
@@ -258,10 +264,65 @@ def test_unit_refusals():
     with pytest.raises(mdgpu.MDGPUError):
         units[0].change_scale(1.01)
     with pytest.raises(mdgpu.MDGPUError):
-        units[0].set_option(mdgpu.OPT_ROW_SORT, 8)
-    with pytest.raises(mdgpu.MDGPUError):
         mdgpu.Unit(L, (2, 1, 1), (0, 0, 0), (0, 0, 0), (10, 20, 20), join=units[0], rc=RC, margin=MARGIN)
     with pytest.raises(mdgpu.MDGPUError):
         mdgpu.Unit(L, (1, 2, 1), (0, 1, 0), (0, 10, 0), (20, 20, 20), join=units[0], rc=RC, margin=MARGIN)
     # the group still steps
     run_sequence(units, True, 5, 5)
+
+
+@pytest.mark.parametrize("thermostat", [mdgpu.NVE, mdgpu.LANGEVIN])
+def test_group_calculate_is_the_sequence(thermostat):
+    """mdgpu_group_calculate (the whole of MDManager::Calculate for the group in one call)
+    gives the trajectory of the same steps taken through the per-unit calls, bit for bit:
+    the same kernels in the same order, the expiry tests batched over the group."""
+    L = 20.0
+    q, p, ids = lattice_state(L, 0.8, 1.5, 13)
+    a = make_units(L, (2, 2, 2), q, p, ids)
+    b = make_units(L, (2, 2, 2), q, p, ids)
+    for u in a + b:
+        u.set_thermostat(thermostat, T=1.2, gamma=0.5)
+        u.set_seed(7)
+    say("group_calculate vs per-unit sequence, thermostat", thermostat)
+    if thermostat == mdgpu.LANGEVIN:
+        rows, rb = [], []
+        for s in range(200):
+            if any([u.is_pair_list_expired() for u in a]):
+                a[0].group_make_pair_list()
+                rb.append(s)
+            for u in a:
+                u.update_position_half()
+            for u in a:
+                u.send_border_particles()
+            for u in a:
+                u.calculate_force()
+                u.langevin()
+    else:
+        _, rb = run_sequence(a, True, 200, 50)
+    nb = b[0].group_calculate(200)
+    assert nb == len(rb), (nb, rb)
+    qa, pa, ta = state_by_id(a)
+    qb, pb, tb = state_by_id(b)
+    assert np.array_equal(ta, tb)
+    assert np.array_equal(qa, qb) and np.array_equal(pa, pb)
+
+
+def test_group_nose_hoover_follows_the_periodic_domain():
+    """The Nose-Hoover step of the group (MDManager::CalculateNoseHoover, the group's
+    temperature at both half steps) against the periodic domain's over 200 steps."""
+    L = 20.0
+    q, p, ids = lattice_state(L, 0.8, 1.5, 17)
+    d = make_domain(L, q, p, ids)
+    units = make_units(L, (2, 2, 2), q, p, ids)
+    for x in [d] + units:
+        x.set_thermostat(mdgpu.NOSEHOOVER, T=0.9, tau=0.1)
+    d.make_pair_list()
+    units[0].group_make_pair_list()
+    d.calculate(200)
+    units[0].group_calculate(200)
+    q1, p1, t1 = state_by_id([d])
+    q2, p2, t2 = state_by_id(units)
+    dq = np.remainder(q1 - q2 + L / 2, L) - L / 2
+    assert np.abs(dq).max() < 1e-9 and np.abs(p1 - p2).max() < 1e-9, (np.abs(dq).max(), np.abs(p1 - p2).max())
+    for u in units:
+        assert u.zeta() == pytest.approx(d.zeta(), rel=1e-9, abs=1e-12)
