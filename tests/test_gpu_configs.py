@@ -153,6 +153,7 @@ def test_config_C_mixed_force_error_against_fp64_oracle_4M():
         d32.add_particles(q, p, t)
         d32.make_pair_list()
         order = np.argsort(d32.particles()[2], kind="stable")
+        d32.set_option(mdgpu.OPT_MIXED_FRAME, 0)
         k32 = d32.kick()[order]
         d32.set_option(mdgpu.OPT_MIXED_FRAME, 1)
         k32b = d32.kick()[order]
