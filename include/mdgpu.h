@@ -44,6 +44,8 @@
  * above -- with one pair-force launch per step for all of them.  Each replica
  * keeps its own PairList budget and displacement check (pairlist.cc:26-121);
  * when the list of any replica expires, the lists of all replicas are rebuilt.
+ * NVE, Langevin or Nose-Hoover run in every replica with its own parameters;
+ * mdgpu_change_scale rescales every replica.
  *
  * Units of a process grid (MDManager with several MDUnits in one process,
  * mdmanager.cc:34-45, e.g. OMP_NUM_THREADS > 1): mdgpu_create_unit makes one
@@ -128,8 +130,9 @@ typedef struct {
 /* lifetime */
 int mdgpu_create(const mdgpu_params *prm, mdgpu_domain **out);
 /* An ensemble of nrep replicas sharing prm's dt, rc, margin, check list length,
-   precision and device (prm->L, prm->capacity unused); thermostat MDGPU_NVE or
-   MDGPU_LANGEVIN.  Replica r owns counts[r] particles, given replica after
+   precision and device (prm->L, prm->capacity unused); thermostat MDGPU_NVE,
+   MDGPU_LANGEVIN or MDGPU_NOSEHOOVER (each replica with its own temperature,
+   target T and zeta; tau from mdgpu_set_thermostat, 0.1 by default).  Replica r owns counts[r] particles, given replica after
    replica in q, p (n*3 doubles; p may be NULL) and type (NULL: 0..counts[r]-1),
    each inside [0, L) of its replica.  The pair lists are built before return. */
 int mdgpu_create_ensemble(const mdgpu_params *prm, int thermostat, int nrep, const mdgpu_replica *reps,
@@ -169,8 +172,10 @@ int mdgpu_group_calculate(mdgpu_domain *d, int nsteps, int *rebuilds);
    mdgpu_get_particles returns them replica after replica */
 int mdgpu_replica_number(const mdgpu_domain *d, int *nrep);
 int mdgpu_replica_counts(const mdgpu_domain *d, int *counts);
-/* Langevin temperature, friction and key of one replica */
+/* Langevin temperature, friction and key of one replica (its Nose-Hoover target too) */
 int mdgpu_set_replica_thermostat(mdgpu_domain *d, int replica, double T, double gamma, uint64_t seed);
+/* the Nose-Hoover zeta of every replica (Variables::Zeta of each system) */
+int mdgpu_replica_zetas(const mdgpu_domain *d, double *zeta);
 int mdgpu_destroy(mdgpu_domain *d);
 const char *mdgpu_last_error(const mdgpu_domain *d);
 int mdgpu_set_precision(mdgpu_domain *d, int precision);

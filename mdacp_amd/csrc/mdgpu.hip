@@ -43,6 +43,7 @@ struct ReplicaState {
   uint64_t seed = 0;            // Langevin key
   double buffer_length = 0.0;   // validity budget
   int lifetime = 0;
+  double zeta = 0.0;            // Nose-Hoover friction (Variables::Zeta)
 };
 
 // A captured fused step (ghost refresh, pair force, per-replica maxima and
@@ -125,6 +126,7 @@ struct mdgpu_domain {
   double *rep_out = nullptr;       // per replica observer sums (3 each)
   int *which = nullptr, *rep_exp = nullptr;  // replicas to check, their answers
   double *h_rep_v2 = nullptr, *h_rep_out = nullptr;
+  double *rep_factor = nullptr, *h_rep_factor = nullptr;  // per replica: Nose-Hoover momentum factor
   int *h_which = nullptr, *h_rep_exp = nullptr;
   unsigned long long *clock = nullptr;  // step counter read by replayed force launches
   unsigned long long *chk_dmax = nullptr;  // displacement check of listed replicas: max |dq|^2 bits,
@@ -1396,6 +1398,42 @@ int heatbath_momenta(mdgpu_domain *d) {
   return check_launch(d, "heatbath momenta");
 }
 
+// Nose-Hoover for every replica of an ensemble, each with its own temperature,
+// target and zeta (MDManager::CalculateNoseHoover, mdmanager.cc:160-175, per system).
+int replica_temperatures(mdgpu_domain *d, std::vector<double> &t) {
+  const int R = d->nrep;
+  hipLaunchKernelGGL(k_replica_kinetic, dim3(R), dim3(kBlock), 0, d->stream, d->mom, d->reps, d->rep_out);
+  CHK(check_launch(d, "replica kinetic energy"));
+  HIPCHK(d, hipMemcpyAsync(d->h_rep_out, d->rep_out, R * sizeof(double), hipMemcpyDeviceToHost, d->stream));
+  CHK(sync(d));
+  t.resize(R);
+  for (int r = 0; r < R; r++) t[r] = (d->h_rep_out[r] / (double)d->rh[r].pn) / 1.5;
+  return MDGPU_OK;
+}
+
+void replica_heatbath_zeta(mdgpu_domain *d, const std::vector<double> &t) {
+  const double dt2 = d->dt * 0.5;
+  for (int r = 0; r < d->nrep; r++) {
+    ReplicaState &S = d->rst[r];
+    const double t1 = (t[r] - S.T) / (d->tau * d->tau);
+    S.zeta += t1 * dt2;
+  }
+}
+
+int replica_heatbath_momenta(mdgpu_domain *d) {
+  for (int r = 0; r < d->nrep; r++) d->h_rep_factor[r] = std::exp(-(d->dt * 0.5) * d->rst[r].zeta);
+  HIPCHK(d, hipMemcpyAsync(d->rep_factor, d->h_rep_factor, d->nrep * sizeof(double), hipMemcpyHostToDevice,
+                           d->stream));
+  if (d->pn > 0)
+    hipLaunchKernelGGL(k_scale_momenta_replicas, dim3(nblk(d->pn)), dim3(kBlock), 0, d->stream, d->mom, d->rep,
+                       d->pn, d->rep_factor);
+  d->vmax_valid = false;
+  CHK(check_launch(d, "replica heat bath"));
+  // the pinned factors are read by the copy above before the next write: the
+  // next call follows a temperature read, which waits for the stream
+  return MDGPU_OK;
+}
+
 int drift(mdgpu_domain *d) {
   if (d->pn > 0)
     hipLaunchKernelGGL(k_drift, dim3(nblk(d->pn)), dim3(kBlock), 0, d->stream, d->pos, d->mom, d->pn,
@@ -1556,7 +1594,21 @@ int calculate_step(mdgpu_domain *d) {
   int expired = 0;
   CHK(check_expired(d, &expired));
   if (expired) CHK(rebuild_timed(d));
-  if (d->thermostat == MDGPU_NOSEHOOVER) {
+  if (d->thermostat == MDGPU_NOSEHOOVER && d->nrep > 1) {
+    // CalculateNoseHoover (mdmanager.cc:160-175) in every replica
+    invalidate(d);
+    CHK(drift(d));
+    CHK(refresh(d));
+    std::vector<double> t;
+    CHK(replica_temperatures(d, t));
+    replica_heatbath_zeta(d, t);
+    CHK(replica_heatbath_momenta(d));
+    CHK(force_in_place(d));
+    CHK(replica_heatbath_momenta(d));
+    CHK(replica_temperatures(d, t));
+    replica_heatbath_zeta(d, t);
+    CHK(drift(d));
+  } else if (d->thermostat == MDGPU_NOSEHOOVER) {
     // CalculateNoseHoover (mdmanager.cc:160-175), single domain
     invalidate(d);
     CHK(drift(d));
@@ -1853,7 +1905,8 @@ int create_domain(const mdgpu_params *prm, int nrep, const mdgpu_replica *rl, co
       (r = dalloc(d, &d->which, nrep)) || (r = dalloc(d, &d->rep_exp, nrep)) || (r = dalloc(d, &d->grng, nrep)) ||
       (r = dalloc(d, &d->clock, 1)) || (r = dalloc(d, &d->chk_dmax, nrep)) || (r = dalloc(d, &d->chk_cnt, nrep)) ||
       (r = dalloc(d, &d->gate, 1)) || (r = dalloc(d, &d->gbudget, nrep)) || (r = dalloc(d, &d->glife, nrep)) ||
-      (r = dalloc(d, &d->chk_cand, (size_t)nrep * d->check_list_length))) {
+      (r = dalloc(d, &d->chk_cand, (size_t)nrep * d->check_list_length)) ||
+      (r = dalloc(d, &d->rep_factor, nrep))) {
     mdgpu_destroy(d);
     return r;
   }
@@ -1863,7 +1916,8 @@ int create_domain(const mdgpu_params *prm, int nrep, const mdgpu_replica *rl, co
       hipHostMalloc((void **)&d->h_rep_exp, nrep * sizeof(int)) != hipSuccess ||
       hipHostMalloc((void **)&d->h_gate, sizeof(RunGate)) != hipSuccess ||
       hipHostMalloc((void **)&d->h_gbudget, nrep * sizeof(double)) != hipSuccess ||
-      hipHostMalloc((void **)&d->h_glife, nrep * sizeof(int)) != hipSuccess) {
+      hipHostMalloc((void **)&d->h_glife, nrep * sizeof(int)) != hipSuccess ||
+      hipHostMalloc((void **)&d->h_rep_factor, nrep * sizeof(double)) != hipSuccess) {
     mdgpu_destroy(d);
     return MDGPU_ERR_HIP;
   }
@@ -2610,7 +2664,8 @@ int mdgpu_create_ensemble(const mdgpu_params *prm, int thermostat, int nrep, con
                           const int *counts, const double *q, const double *p, const int *type, mdgpu_domain **out) {
   if (!prm || !out || nrep < 1 || !reps || !counts || !q) return MDGPU_ERR_ARGUMENT;
   *out = nullptr;
-  if (thermostat != MDGPU_NVE && thermostat != MDGPU_LANGEVIN) return MDGPU_ERR_ARGUMENT;
+  if (thermostat != MDGPU_NVE && thermostat != MDGPU_LANGEVIN && thermostat != MDGPU_NOSEHOOVER)
+    return MDGPU_ERR_ARGUMENT;
   if (prm->check_list_length > kCheckCap) return MDGPU_ERR_ARGUMENT;
   long long total = 0;
   for (int r = 0; r < nrep; r++) {
@@ -2679,13 +2734,14 @@ int mdgpu_destroy(mdgpu_domain *d) {
                   d->grng,    d->rep_v2,      d->rep_out,     d->which,     d->rep_exp,  d->clock,
                   d->perm_in, d->nbr_in,      d->ncode_in,    d->slen_in,   d->chk_dmax, d->chk_cnt,
                   d->chk_cand, d->gate,       d->gbudget,     d->glife,     d->xfer,     d->gunit,
-                  d->st_pos,  d->st_mom,      d->st_id,       d->src_tab,   d->ucnt,     d->uoff};
+                  d->st_pos,  d->st_mom,      d->st_id,       d->src_tab,   d->ucnt,     d->uoff,
+                  d->rep_factor};
   for (StepGraph &g : d->graphs) drop_graph(g);
   for (StepGraph &g : d->gated) drop_graph(g);
   for (void *b : bufs)
     if (b) (void)hipFree(b);
   for (void *h : {(void *)d->h_rep_v2, (void *)d->h_rep_out, (void *)d->h_which, (void *)d->h_rep_exp,
-                  (void *)d->h_gate, (void *)d->h_gbudget, (void *)d->h_glife})
+                  (void *)d->h_gate, (void *)d->h_gbudget, (void *)d->h_glife, (void *)d->h_rep_factor})
     if (h) (void)hipHostFree(h);
   if (d->h_u64) (void)hipHostFree(d->h_u64);
   if (d->h_i64) (void)hipHostFree(d->h_i64);
@@ -2871,10 +2927,6 @@ int mdgpu_get_ghosts(mdgpu_domain *d, int *src_type, int *shift_code, double *q)
 
 int mdgpu_set_thermostat(mdgpu_domain *d, int kind, double T, double gamma, double tau) {
   if (!d || kind < MDGPU_NVE || kind > MDGPU_LANGEVIN) return MDGPU_ERR_ARGUMENT;
-  if (d->nrep > 1 && kind == MDGPU_NOSEHOOVER) {
-    d->err = "Nose-Hoover needs the temperature of a single domain";
-    return MDGPU_ERR_ARGUMENT;
-  }
   invalidate(d);  // the next fused step recomputes its first half drift
   d->thermostat = kind;
   d->T_aim = T;
@@ -2901,6 +2953,16 @@ int mdgpu_set_seed(mdgpu_domain *d, uint64_t seed) {
 int mdgpu_get_zeta(const mdgpu_domain *d, double *zeta) {
   if (!d || !zeta) return MDGPU_ERR_ARGUMENT;
   *zeta = d->zeta;
+  return MDGPU_OK;
+}
+
+int mdgpu_replica_zetas(const mdgpu_domain *d, double *zeta) {
+  if (!d || !zeta) return MDGPU_ERR_ARGUMENT;
+  if (d->nrep == 1) {
+    zeta[0] = d->zeta;
+    return MDGPU_OK;
+  }
+  for (int r = 0; r < d->nrep; r++) zeta[r] = d->rst[r].zeta;
   return MDGPU_OK;
 }
 
@@ -3085,17 +3147,29 @@ int mdgpu_observe_kinetic(mdgpu_domain *d, double *ke) {
 }
 
 int mdgpu_change_scale(mdgpu_domain *d, double alpha) {
-  if (!d || !(alpha > 0.0) || d->nrep > 1) return MDGPU_ERR_ARGUMENT;
+  if (!d || !(alpha > 0.0)) return MDGPU_ERR_ARGUMENT;
   if (is_unit(d)) {
     d->err = "mdgpu_change_scale: a unit of a process grid does not rescale";
     return MDGPU_ERR_ARGUMENT;
   }
   invalidate(d);  // the next fused step recomputes its first half drift
   CHK(make_pair_list(d));
-  for (int k = 0; k < 3; k++) {
-    d->L[k] = d->L[k] * alpha;
-    d->rs[k] *= alpha;
-    d->re[k] *= alpha;
+  if (d->nrep == 1) {
+    for (int k = 0; k < 3; k++) {
+      d->L[k] = d->L[k] * alpha;
+      d->rs[k] *= alpha;
+      d->re[k] *= alpha;
+    }
+  } else {
+    // every replica, as MDManager::ChangeScale does to its one system; each
+    // replica's bricks are chosen again for its new density, as at creation
+    for (int r = 0; r < d->nrep; r++) {
+      Replica &R = d->rh[r];
+      for (int k = 0; k < 3; k++) R.L[k] = R.L[k] * alpha;
+      const double rho = R.pn / (R.L[0] * R.L[1] * R.L[2]);
+      auto_brick(d, rho, R.L, brick_budget(d), R.bd);
+    }
+    for (int k = 0; k < 3; k++) d->L[k] = d->rh[0].L[k];
   }
   if (d->pn > 0)
     hipLaunchKernelGGL(k_scale_pos, dim3(nblk(d->pn)), dim3(kBlock), 0, d->stream, d->pos, d->pn, alpha);

@@ -481,6 +481,18 @@ __global__ __launch_bounds__(kBlock) void k_replica_max_p2(const double4 *__rest
   if (threadIdx.x == 0) out[blockIdx.x] = m;
 }
 
+// ForceCalculator::HeatbathMomenta (fcalculator.cc:752-762) of every replica
+// with its own factor exp(-dt/2 zeta_r).
+__global__ void k_scale_momenta_replicas(double4 *__restrict__ mom, const int *__restrict__ rep, int n,
+                                         const double *__restrict__ f) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double s = f[rep[i]];
+  double4 p = mom[i];
+  p.x *= s; p.y *= s; p.z *= s; p.w *= s;
+  mom[i] = p;
+}
+
 // Per-replica sums of the observers' per-brick partials (3 per brick):
 // out[3r + c] = sum over the replica's bricks of part[3b + c].
 __global__ __launch_bounds__(kBlock) void k_replica_sum3(const double *__restrict__ part,

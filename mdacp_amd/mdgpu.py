@@ -30,7 +30,7 @@ EXPORTS = [
     "mdgpu_set_replica_thermostat", "mdgpu_observe_replicas", "mdgpu_get_replica_pairs",
     "mdgpu_observe_noalloc", "mdgpu_create_unit", "mdgpu_unit_send_particles", "mdgpu_unit_receive_particles",
     "mdgpu_unit_adjust_periodic_boundary", "mdgpu_unit_find_border_particles", "mdgpu_group_make_pair_list",
-    "mdgpu_group_calculate",
+    "mdgpu_group_calculate", "mdgpu_replica_zetas",
 ]
 # directions of a unit's faces (mdconfig.h:25)
 D_LEFT, D_RIGHT, D_BACK, D_FORWARD, D_DOWN, D_UP = range(6)
@@ -124,6 +124,7 @@ def load():
             "mdgpu_unit_find_border_particles": [P],
             "mdgpu_group_make_pair_list": [P],
             "mdgpu_group_calculate": [P, I, PI],
+            "mdgpu_replica_zetas": [P, P],
         }
         for name, args in sig.items():
             f = getattr(L, name)
@@ -398,6 +399,7 @@ class Ensemble(Domain):
                          device=device)
         self.ens = dict(T=np.array(T, dtype=np.float64), gamma=np.array(gamma, dtype=np.float64),
                         seed=np.array(seed), thermostat=thermostat)
+        self.thermo = (thermostat, 1.0, 0.1, 0.1, 0)
 
     def clone(self, **override):
         """A new ensemble with the same parameters, options and current particles (lists built)."""
@@ -426,6 +428,17 @@ class Ensemble(Domain):
 
     def set_replica_thermostat(self, r, T, gamma, seed):
         self._chk(load().mdgpu_set_replica_thermostat(self.h, int(r), T, gamma, int(seed)))
+
+    def replica_zetas(self):
+        z = np.zeros(self.R)
+        self._chk(load().mdgpu_replica_zetas(self.h, _ptr(z)))
+        return z
+
+    def change_scale(self, alpha):
+        """MDManager::ChangeScale(alpha) of every replica: boxes and positions scaled, lists rebuilt."""
+        self._chk(load().mdgpu_change_scale(self.h, alpha))
+        self.boxes = self.boxes * alpha
+        self.L = self.boxes[0]
 
     def observe_replicas_raw(self):
         ke, pe, vir = np.zeros(self.R), np.zeros(self.R), np.zeros(self.R)

@@ -326,3 +326,64 @@ def test_pruned_ensemble_is_bitwise_neutral():
     qa, pa, ta = a.particles()
     qb, pb, tb = b.particles()
     assert np.array_equal(ta, tb) and np.array_equal(qa, qb) and np.array_equal(pa, pb)
+
+
+def test_nose_hoover_replicas_follow_single_domains():
+    """Nose-Hoover in every replica (MDManager::CalculateNoseHoover per system): each replica
+    with its own temperature, target and zeta follows a single domain run with the same
+    thermostat for 300 steps (1e-9 in positions and momenta: the two rebuild at different
+    steps and sum the kinetic energy in another order), its zeta to 1e-9 relative."""
+    spec = SPEC[:4]
+    e, (boxes, counts, q, p, ids) = ensemble(spec, melt_seed=3, dt=0.002, thermostat=mdgpu.NOSEHOOVER)
+    e.set_thermostat(mdgpu.NOSEHOOVER, T=1.0, tau=0.2)
+    for r, (k, rho, T, seed) in enumerate(spec):
+        e.set_replica_thermostat(r, T, 1.0, seed)
+    e.make_pair_list()
+    e.calculate(300)
+    zs = e.replica_zetas()
+    qe, pe_, te = e.particles()
+    for r, (k, rho, T, seed) in enumerate(spec):
+        L = boxes[r][0]
+        d = mdgpu.Domain(L, dt=0.002, rc=RC, margin=MARGIN, capacity=int(counts[r]))
+        d.add_particles(split(q, counts)[r], split(p, counts)[r], split(ids, counts)[r])
+        d.set_thermostat(mdgpu.NOSEHOOVER, T=T, tau=0.2)
+        d.make_pair_list()
+        d.calculate(300)
+        q1, p1, t1 = d.particles_by_id()
+        sl = e.replica_slices()[r]
+        o = np.argsort(te[sl], kind="stable")
+        q2, p2, t2 = qe[sl][o], pe_[sl][o], te[sl][o]
+        assert np.array_equal(t1, t2)
+        dq = np.remainder(q1 - q2 + L / 2, L) - L / 2
+        say(f"replica {r}: T {T}, zeta {zs[r]:.6e} vs {d.zeta():.6e}, max |dq| {np.abs(dq).max():.2e}")
+        assert np.abs(dq).max() < 1e-9 and np.abs(p1 - p2).max() < 1e-9
+        assert zs[r] == pytest.approx(d.zeta(), rel=1e-9, abs=1e-12)
+        d.close()
+    e.close()
+
+
+def test_change_scale_of_every_replica():
+    """MDManager::ChangeScale(alpha) in every replica: positions become fl(q * alpha) of the
+    wrapped positions, boxes alpha L, and each replica's pair set equals a single domain's
+    after its own ChangeScale; the run goes on."""
+    alpha = 1.05
+    e, (boxes, counts, q, p, ids) = ensemble(SPEC, melt_seed=5)
+    e.calculate(20)
+    q0, _, t0 = e.particles()
+    e.change_scale(alpha)
+    q1, _, t1 = e.particles()
+    for r, sl in enumerate(e.replica_slices()):
+        L = boxes[r][0]
+        a = {int(t): x for t, x in zip(t0[sl], q0[sl])}
+        for t, x in zip(t1[sl], q1[sl]):
+            w = np.where(a[int(t)] < 0.0, a[int(t)] + L, np.where(a[int(t)] > L, a[int(t)] - L, a[int(t)]))
+            assert np.array_equal(x, w * alpha), (r, int(t))
+        d = mdgpu.Domain(L, dt=0.005, rc=RC, margin=MARGIN, capacity=int(counts[r]))
+        d.add_particles(np.mod(q0[sl], L), None, t0[sl])
+        d.make_pair_list()
+        d.change_scale(alpha)
+        assert np.array_equal(O.canonical_pairs(e.replica_pairs(r)), O.canonical_pairs(d.pairs())), r
+        d.close()
+    e.calculate(50)
+    assert np.all(np.isfinite(e.observables()["E"]))
+    e.close()
