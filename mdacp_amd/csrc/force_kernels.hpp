@@ -91,6 +91,16 @@ __device__ __forceinline__ void unpack_codes(const uint2 g, int c[kGroup]) {
   }
 }
 
+// The list groups of slice s, group g of lane l at [g * kWave + l]: the base
+// made wavefront-uniform, so the loads address it from scalar registers with
+// the lane's offset as the one vector operand (no 64-bit vector address to
+// compute per load, whose registers would collide with the buffers in flight).
+__device__ __forceinline__ const uint4 *slice_groups(const uint16_t *nbr, long long first) {
+  const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)first);
+  const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)((uint64_t)first >> 32));
+  return reinterpret_cast<const uint4 *>(nbr + (long long)(((uint64_t)hi << 32) | lo));
+}
+
 // Keeps a prefetched list group at its load: an empty asm that takes the
 // group as an operand together with values computed from the current group
 // (the row's running sums), so the compiler cannot move the load down to the
@@ -104,60 +114,103 @@ __device__ __forceinline__ void hold_prefetch(uint2 &g, const A &a, const A &b, 
   asm volatile("" : "+v"(g.x), "+v"(g.y) : "v"(a), "v"(b), "v"(c));
 }
 
+// A group of 8 list indices as one 128-bit register tuple: as an asm operand
+// it stays in four consecutive registers, where a load writes it.
+typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
+
+template <class A>
+__device__ __forceinline__ void hold_prefetch(u32x4 &g, const A &a, const A &b, const A &c) {
+  asm volatile("" : "+v"(g) : "v"(a), "v"(b), "v"(c));
+}
+
+__device__ __forceinline__ void unpack_group(const u32x4 g, int e[kGroup]) {
+  e[0] = g.x & 0xffff; e[1] = g.x >> 16; e[2] = g.y & 0xffff; e[3] = g.y >> 16;
+  e[4] = g.z & 0xffff; e[5] = g.z >> 16; e[6] = g.w & 0xffff; e[7] = g.w >> 16;
+}
+
+// One group of kGroup list entries of a row: partners from LDS (kStaged) or
+// through the gather list, the smoothed-LJ kick added to a.
+template <bool kStaged>
+__device__ __forceinline__ void group_kick_fp64(const int e[kGroup], const double4 qi,
+                                                const double *__restrict__ lds, const double4 *__restrict__ src,
+                                                const int *__restrict__ gl, int total, const ForceConsts &fc, int pn,
+                                                const double4 *__restrict__ ghost, double &ax, double &ay,
+                                                double &az) {
+  double qx[kGroup], qy[kGroup], qz[kGroup];
+#pragma unroll
+  for (int u = 0; u < kGroup; u++) {
+    if (kStaged) {
+      const double *q = lds + 3 * e[u];
+      qx[u] = q[0];
+      qy[u] = q[1];
+      qz[u] = q[2];
+    } else if (e[u] < total) {
+      const double4 q = gather_pos(src, ghost, pn, gl[e[u]]);
+      qx[u] = q.x;
+      qy[u] = q.y;
+      qz[u] = q.z;
+    } else {
+      qx[u] = qy[u] = qz[u] = 1.0e200;
+    }
+  }
+#pragma unroll
+  for (int u = 0; u < kGroup; u++) {
+    const double dx = qx[u] - qi.x, dy = qy[u] - qi.y, dz = qz[u] - qi.z;
+    const double r2 = fma(dx, dx, fma(dy, dy, dz * dz));
+    const double ir2 = rcp64(r2);
+    const double ir6 = ir2 * ir2 * ir2;
+    double df = fma(ir2 * ir6, fma(-fc.B, ir6, fc.A), fc.C);
+    df = (r2 <= fc.CL2) ? df : 0.0;
+    ax = fma(df, dx, ax);
+    ay = fma(df, dy, ay);
+    az = fma(df, dz, az);
+  }
+}
+
 // The FP64 kick of one row (one lane of a 64-row slice) over its list
-// entries, one group of kGroup per batch.  `rg` points at the lane's first
-// group (group g at rg[g * kWave]); the indices of the next P groups are in
-// flight while a group is computed (P <= kMaxAhead).  Reading up to P groups past the slice
-// stays inside the next slice or the padding the list arrays carry at their
-// end (kListTail), and those entries are never used.  Partners come from LDS
-// (kStaged) or through the gather list from global memory.  The cutoff is a
-// select, not a branch (padding entries point at a far sentinel).
+// entries, one group of kGroup per batch: group g at sg[g * kWave + lane]
+// (slice_groups).  P buffers hold the indices of the next P
+// groups: the loop is unrolled by P so that each buffer is refilled in place
+// right after it is unpacked, and waited for only after the P - 1 groups
+// computed in between (rotating the buffers instead would copy registers
+// whose loads are still in flight, and wait for them at once).  Reading up
+// to P groups past the slice stays inside the next slice or the padding the
+// list arrays carry at their end (kListTail, P <= kMaxAhead), and those
+// entries are never used.  The cutoff is a select, not a branch (padding
+// entries point at a far sentinel).
 template <bool kStaged, int P>
-__device__ __forceinline__ double3 row_kick_fp64(const uint4 *__restrict__ rg, int len, const double4 qi,
+__device__ __forceinline__ double3 row_kick_fp64(const uint4 *__restrict__ sg, int lane, int len, const double4 qi,
                                                  const double *__restrict__ lds, const double4 *__restrict__ src,
                                                  const int *__restrict__ gl, int total, const ForceConsts &fc,
                                                  int pn, const double4 *__restrict__ ghost) {
   double ax = 0.0, ay = 0.0, az = 0.0;
-  uint4 ahead[P];
+  const u32x4 *vg = reinterpret_cast<const u32x4 *>(sg);
+  u32x4 ahead[P];
+  const unsigned ul = (unsigned)lane;
 #pragma unroll
-  for (int j = 0; j < P; j++) ahead[j] = rg[j * kWave];
-  for (int k = 0; k < len; k += kGroup) {
-    int e[kGroup];
-    unpack_group(ahead[0], e);
+  for (int j = 0; j < P; j++) ahead[j] = (vg + j * kWave)[ul];
+  int k = 0;
+  for (; k + P * kGroup <= len; k += P * kGroup) {
 #pragma unroll
-    for (int j = 0; j + 1 < P; j++) ahead[j] = ahead[j + 1];
-    ahead[P - 1] = rg[(k / kGroup + P) * kWave];
-    __builtin_amdgcn_sched_barrier(0);  // the prefetch is issued before this group's work
-    double qx[kGroup], qy[kGroup], qz[kGroup];
-#pragma unroll
-    for (int u = 0; u < kGroup; u++) {
-      if (kStaged) {
-        const double *q = lds + 3 * e[u];
-        qx[u] = q[0];
-        qy[u] = q[1];
-        qz[u] = q[2];
-      } else if (e[u] < total) {
-        const double4 q = gather_pos(src, ghost, pn, gl[e[u]]);
-        qx[u] = q.x;
-        qy[u] = q.y;
-        qz[u] = q.z;
-      } else {
-        qx[u] = qy[u] = qz[u] = 1.0e200;
-      }
+    for (int j = 0; j < P; j++) {
+      int e[kGroup];
+      unpack_group(ahead[j], e);
+      ahead[j] = (vg + (k / kGroup + j + P) * kWave)[ul];
+      __builtin_amdgcn_sched_barrier(0);  // the prefetch is issued before this group's work
+      group_kick_fp64<kStaged>(e, qi, lds, src, gl, total, fc, pn, ghost, ax, ay, az);
+      // the buffer unpacked next was loaded P - 1 groups of work ago
+      hold_prefetch(ahead[(j + 1) % P], ax, ay, az);
     }
+  }
+  // the last groups of a slice whose length is not a multiple of P groups
+  // (len is the slice's: uniform across the wavefront)
 #pragma unroll
-    for (int u = 0; u < kGroup; u++) {
-      const double dx = qx[u] - qi.x, dy = qy[u] - qi.y, dz = qz[u] - qi.z;
-      const double r2 = fma(dx, dx, fma(dy, dy, dz * dz));
-      const double ir2 = rcp64(r2);
-      const double ir6 = ir2 * ir2 * ir2;
-      double df = fma(ir2 * ir6, fma(-fc.B, ir6, fc.A), fc.C);
-      df = (r2 <= fc.CL2) ? df : 0.0;
-      ax = fma(df, dx, ax);
-      ay = fma(df, dy, ay);
-      az = fma(df, dz, az);
+  for (int j = 0; j + 1 < P; j++) {
+    if (k + j * kGroup < len) {
+      int e[kGroup];
+      unpack_group(ahead[j], e);
+      group_kick_fp64<kStaged>(e, qi, lds, src, gl, total, fc, pn, ghost, ax, ay, az);
     }
-    hold_prefetch(ahead[P - 1], ax, ay, az);
   }
   return make_double3(ax, ay, az);
 }
@@ -237,66 +290,95 @@ __device__ __forceinline__ double3 row_kick_mixed(const uint4 *__restrict__ rg, 
   return make_double3((double)ax - (double)cx, (double)ay - (double)cy, (double)az - (double)cz);
 }
 
+// One group of a row's list entries in the brick-centred FP32 frame: FP32
+// pair kicks summed as kSum says into a (and its compensation cx, or the FP64
+// group sums s).
+template <bool kStaged, int kSum>
+__device__ __forceinline__ void group_kick_mixed_brick(const int e[kGroup], const float4 ri,
+                                                       const float4 *__restrict__ lds,
+                                                       const double4 *__restrict__ src,
+                                                       const double4 *__restrict__ ghost, int pn,
+                                                       const int *__restrict__ gl, int total,
+                                                       const ForceConstsF &fc, const double3 c, float &ax, float &ay,
+                                                       float &az, float &cx, float &cy, float &cz, double &sx,
+                                                       double &sy, double &sz) {
+  float4 rj[kGroup];
+#pragma unroll
+  for (int u = 0; u < kGroup; u++) {
+    if (kStaged) {
+      rj[u] = lds[e[u]];
+    } else if (e[u] < total) {
+      const double4 q = gather_pos(src, ghost, pn, gl[e[u]]);
+      rj[u] = make_float4((float)(q.x - c.x), (float)(q.y - c.y), (float)(q.z - c.z), 0.0f);
+    } else {
+      rj[u] = make_float4(1.0e30f, 1.0e30f, 1.0e30f, 0.0f);
+    }
+  }
+  if (kSum == kSumGroup64) ax = ay = az = 0.f;
+#pragma unroll
+  for (int u = 0; u < kGroup; u++) {
+    const float dx = rj[u].x - ri.x, dy = rj[u].y - ri.y, dz = rj[u].z - ri.z;
+    const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+    const float ir2 = __builtin_amdgcn_rcpf(r2);
+    const float ir6 = ir2 * ir2 * ir2;
+    float df = fmaf(ir2 * ir6, fmaf(-fc.B, ir6, fc.A), fc.C);
+    df = (r2 <= fc.CL2) ? df : 0.0f;
+    if (kSum == kSumKahan) {
+      float y, q;
+      y = fmaf(df, dx, -cx); q = ax + y; cx = (q - ax) - y; ax = q;
+      y = fmaf(df, dy, -cy); q = ay + y; cy = (q - ay) - y; ay = q;
+      y = fmaf(df, dz, -cz); q = az + y; cz = (q - az) - y; az = q;
+    } else {
+      ax = fmaf(df, dx, ax);
+      ay = fmaf(df, dy, ay);
+      az = fmaf(df, dz, az);
+    }
+  }
+  if (kSum == kSumGroup64) {
+    sx += (double)ax;
+    sy += (double)ay;
+    sz += (double)az;
+  }
+}
+
 // Mixed-precision kick of one row in a brick-centred FP32 frame: partners and
 // the row are (float)(q - c) with c the centre of the row's brick, so dx =
-// rj - ri needs no cell offset.  Same groups, loads and sums as row_kick_mixed.
+// rj - ri needs no cell offset.  Groups, loads and buffers as row_kick_fp64.
 template <bool kStaged, int kSum, int P>
-__device__ __forceinline__ double3 row_kick_mixed_brick(const uint4 *__restrict__ rg, int len, const float4 ri,
-                                                        const float4 *__restrict__ lds,
+__device__ __forceinline__ double3 row_kick_mixed_brick(const uint4 *__restrict__ sg, int lane, int len,
+                                                        const float4 ri, const float4 *__restrict__ lds,
                                                         const double4 *__restrict__ src,
                                                         const double4 *__restrict__ ghost, int pn,
                                                         const int *__restrict__ gl, int total,
                                                         const ForceConstsF &fc, const double3 c) {
   float ax = 0.f, ay = 0.f, az = 0.f, cx = 0.f, cy = 0.f, cz = 0.f;
   double sx = 0.0, sy = 0.0, sz = 0.0;
-  uint4 ahead[P];
+  const u32x4 *vg = reinterpret_cast<const u32x4 *>(sg);
+  u32x4 ahead[P];
+  const unsigned ul = (unsigned)lane;
 #pragma unroll
-  for (int j = 0; j < P; j++) ahead[j] = rg[j * kWave];
-  for (int k = 0; k < len; k += kGroup) {
-    int e[kGroup];
-    unpack_group(ahead[0], e);
+  for (int j = 0; j < P; j++) ahead[j] = (vg + j * kWave)[ul];
+  int k = 0;
+  for (; k + P * kGroup <= len; k += P * kGroup) {
 #pragma unroll
-    for (int j = 0; j + 1 < P; j++) ahead[j] = ahead[j + 1];
-    ahead[P - 1] = rg[(k / kGroup + P) * kWave];
-    __builtin_amdgcn_sched_barrier(0);  // the prefetch is issued before this group's work
-    float4 rj[kGroup];
-#pragma unroll
-    for (int u = 0; u < kGroup; u++) {
-      if (kStaged) {
-        rj[u] = lds[e[u]];
-      } else if (e[u] < total) {
-        const double4 q = gather_pos(src, ghost, pn, gl[e[u]]);
-        rj[u] = make_float4((float)(q.x - c.x), (float)(q.y - c.y), (float)(q.z - c.z), 0.0f);
-      } else {
-        rj[u] = make_float4(1.0e30f, 1.0e30f, 1.0e30f, 0.0f);
-      }
+    for (int j = 0; j < P; j++) {
+      int e[kGroup];
+      unpack_group(ahead[j], e);
+      ahead[j] = (vg + (k / kGroup + j + P) * kWave)[ul];
+      __builtin_amdgcn_sched_barrier(0);  // the prefetch is issued before this group's work
+      group_kick_mixed_brick<kStaged, kSum>(e, ri, lds, src, ghost, pn, gl, total, fc, c, ax, ay, az, cx, cy, cz,
+                                            sx, sy, sz);
+      hold_prefetch(ahead[(j + 1) % P], ax, ay, az);
     }
-    if (kSum == kSumGroup64) ax = ay = az = 0.f;
+  }
 #pragma unroll
-    for (int u = 0; u < kGroup; u++) {
-      const float dx = rj[u].x - ri.x, dy = rj[u].y - ri.y, dz = rj[u].z - ri.z;
-      const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
-      const float ir2 = __builtin_amdgcn_rcpf(r2);
-      const float ir6 = ir2 * ir2 * ir2;
-      float df = fmaf(ir2 * ir6, fmaf(-fc.B, ir6, fc.A), fc.C);
-      df = (r2 <= fc.CL2) ? df : 0.0f;
-      if (kSum == kSumKahan) {
-        float y, q;
-        y = fmaf(df, dx, -cx); q = ax + y; cx = (q - ax) - y; ax = q;
-        y = fmaf(df, dy, -cy); q = ay + y; cy = (q - ay) - y; ay = q;
-        y = fmaf(df, dz, -cz); q = az + y; cz = (q - az) - y; az = q;
-      } else {
-        ax = fmaf(df, dx, ax);
-        ay = fmaf(df, dy, ay);
-        az = fmaf(df, dz, az);
-      }
+  for (int j = 0; j + 1 < P; j++) {
+    if (k + j * kGroup < len) {
+      int e[kGroup];
+      unpack_group(ahead[j], e);
+      group_kick_mixed_brick<kStaged, kSum>(e, ri, lds, src, ghost, pn, gl, total, fc, c, ax, ay, az, cx, cy, cz,
+                                            sx, sy, sz);
     }
-    if (kSum == kSumGroup64) {
-      sx += (double)ax;
-      sy += (double)ay;
-      sz += (double)az;
-    }
-    hold_prefetch(ahead[P - 1], ax, ay, az);
   }
   if (kSum == kSumGroup64) return make_double3(sx, sy, sz);
   return make_double3((double)ax - (double)cx, (double)ay - (double)cy, (double)az - (double)cz);
@@ -392,9 +474,9 @@ __global__ __launch_bounds__(kWave *W) void k_force_brick_fp64(const double4 *__
     double4 p = mom[i];  // loaded before the pair loop, which hides its latency
     const RowExtra x = row_extra(i, epilogue, id, nd);
     const int len = bl.slen[s];
-    const uint4 *rg = reinterpret_cast<const uint4 *>(bl.nbr + bl.sbase[s] + lane * kGroup);
-    const double3 a = staged ? row_kick_fp64<true, U>(rg, len, qi, lds64, src, gl, total, fc, bl.pn, bl.ghost)
-                             : row_kick_fp64<false, U>(rg, len, qi, lds64, src, gl, total, fc, bl.pn, bl.ghost);
+    const uint4 *sg = slice_groups(bl.nbr, bl.sbase[s]);
+    const double3 a = staged ? row_kick_fp64<true, U>(sg, lane, len, qi, lds64, src, gl, total, fc, bl.pn, bl.ghost)
+                             : row_kick_fp64<false, U>(sg, lane, len, qi, lds64, src, gl, total, fc, bl.pn, bl.ghost);
     if (ok) {
       p.x += a.x;
       p.y += a.y;
@@ -575,7 +657,8 @@ __global__ __launch_bounds__(kWave *(W + P)) void k_force_pipe_fp64(const double
           const int len = bl.slen[s];
           const uint4 *rg = reinterpret_cast<const uint4 *>(bl.nbr + bl.sbase[s] + lane * kGroup);
           const double3 a = staged ? row_kick_fp64_buf<U>(rg, len, qi, lbuf, fc)
-                                   : row_kick_fp64<false, U>(rg, len, qi, nullptr, src, gl, total, fc, bl.pn, bl.ghost);
+                                   : row_kick_fp64<false, U>(slice_groups(bl.nbr, bl.sbase[s]), lane, len, qi,
+                                                             nullptr, src, gl, total, fc, bl.pn, bl.ghost);
           if (ok) {
             p.x += a.x;
             p.y += a.y;
@@ -749,10 +832,11 @@ __global__ __launch_bounds__(kWave *W) void k_force_brick_mixed_brick(
       ri = make_float4((float)(q.x - c.x), (float)(q.y - c.y), (float)(q.z - c.z), 0.0f);
     }
     const int len = bl.slen[s];
-    const uint4 *rg = reinterpret_cast<const uint4 *>(bl.nbr + bl.sbase[s] + lane * kGroup);
+    const uint4 *sg = slice_groups(bl.nbr, bl.sbase[s]);
     const double3 a =
-        staged ? row_kick_mixed_brick<true, kSum, U>(rg, len, ri, lds32, src, bl.ghost, bl.pn, gl, total, fc, c)
-               : row_kick_mixed_brick<false, kSum, U>(rg, len, ri, lds32, src, bl.ghost, bl.pn, gl, total, fc, c);
+        staged ? row_kick_mixed_brick<true, kSum, U>(sg, lane, len, ri, lds32, src, bl.ghost, bl.pn, gl, total, fc, c)
+               : row_kick_mixed_brick<false, kSum, U>(sg, lane, len, ri, lds32, src, bl.ghost, bl.pn, gl, total, fc,
+                                                      c);
     if (ok) {
       // loaded after the pair loop (registers)
       double4 p = mom[i];
