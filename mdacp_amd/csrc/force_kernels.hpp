@@ -173,10 +173,10 @@ __device__ __forceinline__ void group_kick_fp64(const int e[kGroup], const doubl
 // groups: the loop is unrolled by P so that each buffer is refilled in place
 // right after it is unpacked, and waited for only after the P - 1 groups
 // computed in between (rotating the buffers instead would copy registers
-// whose loads are still in flight, and wait for them at once).  Reading up
-// to P groups past the slice stays inside the next slice or the padding the
-// list arrays carry at their end (kListTail, P <= kMaxAhead), and those
-// entries are never used.  The cutoff is a select, not a branch (padding
+// whose loads are still in flight, and wait for them at once).  Prefetches
+// past the end of the slice re-read its last group, which is in the cache,
+// rather than the next slice's first groups, which would add list traffic
+// (P <= kMaxAhead).  The cutoff is a select, not a branch (padding
 // entries point at a far sentinel).
 template <bool kStaged, int P>
 __device__ __forceinline__ double3 row_kick_fp64(const uint4 *__restrict__ sg, int lane, int len, const double4 qi,
@@ -187,15 +187,16 @@ __device__ __forceinline__ double3 row_kick_fp64(const uint4 *__restrict__ sg, i
   const u32x4 *vg = reinterpret_cast<const u32x4 *>(sg);
   u32x4 ahead[P];
   const unsigned ul = (unsigned)lane;
+  const int last = max(len / kGroup - 1, 0);  // loads past the slice re-read its last group
 #pragma unroll
-  for (int j = 0; j < P; j++) ahead[j] = (vg + j * kWave)[ul];
+  for (int j = 0; j < P; j++) ahead[j] = (vg + min(j, last) * kWave)[ul];
   int k = 0;
   for (; k + P * kGroup <= len; k += P * kGroup) {
 #pragma unroll
     for (int j = 0; j < P; j++) {
       int e[kGroup];
       unpack_group(ahead[j], e);
-      ahead[j] = (vg + (k / kGroup + j + P) * kWave)[ul];
+      ahead[j] = (vg + min(k / kGroup + j + P, last) * kWave)[ul];
       __builtin_amdgcn_sched_barrier(0);  // the prefetch is issued before this group's work
       group_kick_fp64<kStaged>(e, qi, lds, src, gl, total, fc, pn, ghost, ax, ay, az);
       // the buffer unpacked next was loaded P - 1 groups of work ago
@@ -356,15 +357,16 @@ __device__ __forceinline__ double3 row_kick_mixed_brick(const uint4 *__restrict_
   const u32x4 *vg = reinterpret_cast<const u32x4 *>(sg);
   u32x4 ahead[P];
   const unsigned ul = (unsigned)lane;
+  const int last = max(len / kGroup - 1, 0);  // loads past the slice re-read its last group
 #pragma unroll
-  for (int j = 0; j < P; j++) ahead[j] = (vg + j * kWave)[ul];
+  for (int j = 0; j < P; j++) ahead[j] = (vg + min(j, last) * kWave)[ul];
   int k = 0;
   for (; k + P * kGroup <= len; k += P * kGroup) {
 #pragma unroll
     for (int j = 0; j < P; j++) {
       int e[kGroup];
       unpack_group(ahead[j], e);
-      ahead[j] = (vg + (k / kGroup + j + P) * kWave)[ul];
+      ahead[j] = (vg + min(k / kGroup + j + P, last) * kWave)[ul];
       __builtin_amdgcn_sched_barrier(0);  // the prefetch is issued before this group's work
       group_kick_mixed_brick<kStaged, kSum>(e, ri, lds, src, ghost, pn, gl, total, fc, c, ax, ay, az, cx, cy, cz,
                                             sx, sy, sz);
