@@ -156,7 +156,9 @@ def run_gpu(precision, x, p, L, args, observe_every=100):
                 pu_s=args.steps * n / (args.steps * ms_step * 1e-3 + t4 - t2))
     del qh, ph
     # the same number of steps again with HIP events around each pair-force launch and
-    # each construction (launched directly, so slower per step than the timed pass)
+    # each construction (launched directly, so slower per step than the timed pass; each
+    # kick leaves the final positions to a kernel of its own, as in the timed pass's runs,
+    # so the events time the kernel the timed steps launch)
     d.enable_timing(True)
     d.calculate(max(args.steps, 100))
     tm = d.timing()

@@ -1519,9 +1519,18 @@ int enqueue_fused(mdgpu_domain *d, unsigned long long *clock, bool prune = false
   if (prune) CHK(launch_prune(d));
   const int epi = d->thermostat == MDGPU_LANGEVIN ? kKickLangevin : kKickDrift;
   const NextDrift nd{d->pos_next, mixed ? d->rel_next : nullptr, d->cell, d->org};
-  // a run's steps leave the final owned positions out (run_gated writes them after the run)
-  return launch_force(d, d->pos_tmp, d->rel_tmp, gated ? nullptr : d->pos, epi, true, nd, clock,
-                      d->prune_delta > 0.0, gate);
+  // a run's steps leave the final owned positions out (run_gated writes them
+  // after the run); a timed step does too and writes them after its timed kick,
+  // so that the events time the kernel the runs launch
+  const bool timed = d->timing && clock == nullptr && !gated;
+  CHK(launch_force(d, d->pos_tmp, d->rel_tmp, gated || timed ? nullptr : d->pos, epi, true, nd, clock,
+                   d->prune_delta > 0.0, gate));
+  if (timed && d->pn > 0) {
+    hipLaunchKernelGGL(k_final_positions, dim3(nblk(d->pn)), dim3(kBlock), 0, st, d->pos, d->pos_tmp, d->mom, d->pn,
+                       d->dt * 0.5, d->thermostat == MDGPU_LANGEVIN ? 0 : 1);
+    CHK(check_launch(d, "final positions"));
+  }
+  return MDGPU_OK;
 }
 
 void drop_graph(StepGraph &g) {

@@ -330,6 +330,41 @@ def test_runs_ahead_match_steps_one_at_a_time(thermostat, precision, frame):
     b.close()
 
 
+@pytest.mark.parametrize("thermostat", ["nve", "langevin"])
+def test_timed_steps_match_runs(thermostat):
+    """Timed steps (HIP events around each kick, launched one at a time, final positions written
+    by a kernel of their own after the kick as a run does) give the same bits, constructions and
+    time as steps in runs, and the events time every kick of the call."""
+    say("case: timed steps", thermostat)
+    L = 16.0
+    x = fcc(L, 0.5)
+    p = initial_velocity(len(x), math.sqrt(3.0 * 1.5), 5)
+    ds = []
+    for timed in (True, False):
+        d = mdgpu.Domain(L, dt=0.005, capacity=len(x))
+        d.add_particles(x, p)
+        if thermostat == "langevin":
+            d.set_thermostat(mdgpu.LANGEVIN, 1.2, 1.0, 0.1)
+            d.set_seed(7)
+        d.enable_timing(timed)
+        ds.append(d)
+    a, b = ds
+    ra = rb = 0
+    for chunk in (1, 50, 99):
+        ra += a.calculate(chunk)
+        rb += b.calculate(chunk)
+        assert ra == rb, (chunk, ra, rb)
+        qa, pa, ta = a.particles()
+        qb, pb, tb = b.particles()
+        assert np.array_equal(ta, tb)
+        assert np.array_equal(qa, qb) and np.array_equal(pa, pb)
+    assert a.time() == b.time()
+    tm = a.timing()  # of the last call
+    assert tm["force_launches"] == 99 and tm["force_ms"] > 0.0, tm
+    a.close()
+    b.close()
+
+
 def test_runs_are_bitwise_deterministic():
     say("case: determinism")
     L = 20.0
