@@ -90,6 +90,7 @@ struct mdgpu_domain {
   int waves = 8;          // wavefronts per brick workgroup: 2, 4, 8 or 16
   int bd[3] = {4, 4, 4};  // cells per brick
   bool brick_auto = true; // a single domain picks bd from its density at the first construction
+  bool brick_user = false; // the caller set bd (MDGPU_OPT_BRICK): kept through scale changes
   int brick_kb = 0;         // LDS budget the automatic bricks are sized for, KiB (0: the default)
   int lds_kb = 0;         // LDS budget for one brick's staged partners, KiB (0: the bricks' budget)
   int lds_max_kb = 60;    // device limit per workgroup less the kernels' static LDS
@@ -3177,6 +3178,9 @@ int mdgpu_change_scale(mdgpu_domain *d, double alpha) {
       d->rs[k] *= alpha;
       d->re[k] *= alpha;
     }
+    // bricks picked from the density are picked again for the new one (at the
+    // construction below), as an ensemble's replicas' are
+    if (!d->brick_user) d->brick_auto = true;
   } else {
     // every replica, as MDManager::ChangeScale does to its one system; each
     // replica's bricks are chosen again for its new density, as at creation
@@ -3397,6 +3401,7 @@ int mdgpu_set_option(mdgpu_domain *d, int option, int value) {
           (bx + 2) * (by + 2) * (bz + 2) > kMaxBrickCells)
         return MDGPU_ERR_ARGUMENT;
       d->brick_auto = false;
+      d->brick_user = true;
       d->bd[0] = bx;
       d->bd[1] = by;
       d->bd[2] = bz;
@@ -3437,6 +3442,7 @@ int mdgpu_set_option(mdgpu_domain *d, int option, int value) {
       if (value < 8 || value > 160) return MDGPU_ERR_ARGUMENT;
       d->brick_kb = value;
       if (d->nrep == 1) {
+        d->brick_user = false;
         d->brick_auto = true;  // chosen at the next construction, which the next step runs
         d->initialized = false;
         return MDGPU_OK;

@@ -415,6 +415,33 @@ def test_change_scale_against_oracle():
     compare_state(d, o, L * alpha, 2.5, 0.001, "change_scale")
 
 
+def test_change_scale_picks_bricks_for_the_new_density():
+    """Bricks a domain picked from its density are picked again after ChangeScale, as a fresh
+    domain at the new density picks them; bricks the caller set are kept."""
+    say("case: ChangeScale bricks")
+    L = 40.0
+    x = fcc(L, 0.75)
+    alpha = (0.75 / 0.45) ** (1.0 / 3.0)
+    d = mdgpu.Domain(L, dt=0.001, capacity=len(x))
+    d.add_particles(x)
+    d.make_pair_list()
+    dense = d.get_option(mdgpu.OPT_BRICK)
+    d.change_scale(alpha)
+    f = mdgpu.Domain(L * alpha, dt=0.001, capacity=len(x))
+    f.add_particles(x * alpha)
+    f.make_pair_list()
+    say("bricks", hex(dense), hex(d.get_option(mdgpu.OPT_BRICK)), hex(f.get_option(mdgpu.OPT_BRICK)))
+    assert d.get_option(mdgpu.OPT_BRICK) == f.get_option(mdgpu.OPT_BRICK) != dense
+    u = mdgpu.Domain(L, dt=0.001, capacity=len(x))
+    u.add_particles(x)
+    u.set_option(mdgpu.OPT_BRICK, dense)
+    u.make_pair_list()
+    u.change_scale(alpha)
+    assert u.get_option(mdgpu.OPT_BRICK) == dense
+    for g in (d, f, u):
+        g.close()
+
+
 def test_error_paths_raise():
     say("case: error paths")
     L = 16.0
