@@ -291,7 +291,7 @@ int mdgpu_get_timing(const mdgpu_domain *d, int *force_launches, double *force_m
                                   together), or kept in cell order (0); the next step
                                   rebuilds the list */
 #define MDGPU_OPT_BRICK_KB 11  /* LDS budget, KiB (8..160), that the automatic bricks are sized
-                                  for (default 96 for a domain, 76 for an ensemble); setting it
+                                  for (default 78 for a domain, 76 for an ensemble); setting it
                                   chooses the bricks again and the next step rebuilds the list */
 #define MDGPU_OPT_SEARCH_CELLS 12  /* neighbour search with a wavefront per cell, lanes over 64
                                       candidates at a time (1, the default), or a thread per

@@ -758,11 +758,11 @@ long long axis_staged_cells(int ms, int bd) {
 
 // Brick dimensions for a box of edges L at density rho: among the bricks
 // whose neighbourhood fits kMaxBrickCells cells and, at the mean density,
-// 90% of kb KiB of LDS (positions, 24 B per staged entry), the ones that
-// stage the fewest cells in all (ties: fewer bricks).  Staging the
-// neighbourhood is the per-brick overhead of the search and force kernels.
-void auto_brick(const mdgpu_domain *d, double rho, const double L[3], int kb, int bd[3]) {
-  const double budget = 0.9 * std::min(kb, d->lds_max_kb) * 1024.0;
+// the fraction `fill` of kb KiB of LDS (positions, 24 B per staged entry),
+// the ones that stage the fewest cells in all (ties: fewer bricks).  Staging
+// the neighbourhood is the per-brick overhead of the search and force kernels.
+void auto_brick(const mdgpu_domain *d, double rho, const double L[3], int kb, int bd[3], double fill) {
+  const double budget = fill * std::min(kb, d->lds_max_kb) * 1024.0;
   int ms[3];
   double vol = 1.0;
   for (int k = 0; k < 3; k++) {
@@ -790,12 +790,21 @@ void auto_brick(const mdgpu_domain *d, double rho, const double L[3], int kb, in
       }
 }
 
-// Ensemble replicas: bricks for two workgroups per CU (160 KiB of LDS).
+// Ensemble replicas: bricks for two workgroups per CU (160 KiB of LDS),
+// their mean neighbourhood within 90% of the budget (small boxes: the
+// neighbourhoods' counts spread more).
 constexpr int kReplicaBrickKb = 76;
-// A single domain, unless the caller chose its bricks: the largest bricks the
-// force kernels stage within the default LDS budget, chosen at the first
-// construction from the density.
-constexpr int kDomainBrickKb = 96;
+constexpr double kReplicaFill = 0.9;
+// A single domain, unless the caller chose its bricks: bricks for two
+// workgroups per CU as well, chosen at the first construction (and after a
+// scale change) from the density.  The mean neighbourhood may take 96% of
+// the budget: in a uniform fluid of millions of particles the fullest
+// neighbourhood holds about 3% more than the mean (4M particles at rho =
+// 0.5: 3,099 entries against 3,003 for 4x4x4-cell bricks).  At rho = 0.6
+// this picks 4x4x3 cells, whose kick (16.4M particles) takes 1.45 ms against
+// 1.84 ms for 4x4x4, which holds one workgroup per CU.
+constexpr int kDomainBrickKb = 78;
+constexpr double kDomainFill = 0.96;
 
 int brick_budget(const mdgpu_domain *d) {
   return d->brick_kb ? d->brick_kb : (d->nrep == 1 ? kDomainBrickKb : kReplicaBrickKb);
@@ -803,7 +812,7 @@ int brick_budget(const mdgpu_domain *d) {
 
 // The force kernels' LDS budget: unless set, the one the bricks are sized
 // for, so that the largest bricks of a state do not push a workgroup past the
-// occupancy the bricks were sized for (two per CU at 76 KiB, one at 96).
+// occupancy the bricks were sized for (two per CU at 76 or 78 KiB).
 int force_lds_kb(const mdgpu_domain *d) { return d->lds_kb ? d->lds_kb : brick_budget(d); }
 
 bool is_unit(const mdgpu_domain *d) { return d->group != nullptr; }
@@ -815,7 +824,7 @@ int auto_bricks(mdgpu_domain *d) {
     d->brick_auto = false;
     const double rho = d->pn / (d->rh[0].L[0] * d->rh[0].L[1] * d->rh[0].L[2]);
     int b[3];
-    auto_brick(d, rho, d->rh[0].L, brick_budget(d), b);
+    auto_brick(d, rho, d->rh[0].L, brick_budget(d), b, kDomainFill);
     if (b[0] != d->bd[0] || b[1] != d->bd[1] || b[2] != d->bd[2]) {
       for (int k = 0; k < 3; k++) d->bd[k] = b[k];
       CHK(set_geometry(d));
@@ -1910,7 +1919,7 @@ int create_domain(const mdgpu_params *prm, int nrep, const mdgpu_replica *rl, co
       d->rst[r].gamma = rl[r].gamma;
       d->rst[r].seed = rl[r].seed;
       const double rho = counts[r] / (R.L[0] * R.L[1] * R.L[2]);
-      auto_brick(d, rho, R.L, brick_budget(d), R.bd);
+      auto_brick(d, rho, R.L, brick_budget(d), R.bd, kReplicaFill);
       R.p0 = (int)p0;
       R.pn = counts[r];
       p0 += counts[r];
@@ -3188,7 +3197,7 @@ int mdgpu_change_scale(mdgpu_domain *d, double alpha) {
       Replica &R = d->rh[r];
       for (int k = 0; k < 3; k++) R.L[k] = R.L[k] * alpha;
       const double rho = R.pn / (R.L[0] * R.L[1] * R.L[2]);
-      auto_brick(d, rho, R.L, brick_budget(d), R.bd);
+      auto_brick(d, rho, R.L, brick_budget(d), R.bd, kReplicaFill);
     }
     for (int k = 0; k < 3; k++) d->L[k] = d->rh[0].L[k];
   }
@@ -3447,7 +3456,7 @@ int mdgpu_set_option(mdgpu_domain *d, int option, int value) {
         d->initialized = false;
         return MDGPU_OK;
       }
-      for (Replica &R : d->rh) auto_brick(d, R.pn / (R.L[0] * R.L[1] * R.L[2]), R.L, value, R.bd);
+      for (Replica &R : d->rh) auto_brick(d, R.pn / (R.L[0] * R.L[1] * R.L[2]), R.L, value, R.bd, kReplicaFill);
       return set_geometry(d);  // the next step rebuilds the list
   }
   return MDGPU_ERR_ARGUMENT;
