@@ -89,3 +89,51 @@ def test_ensembles_keep_their_per_replica_bricks(monkeypatch):
     brick0 = d.opts[mdgpu.OPT_BRICK]
     res = A.autotune(d, steps=2)
     assert "brick" not in res["choice"] and d.opts[mdgpu.OPT_BRICK] == brick0
+
+
+class LuckyDomain(MockDomain):
+    """Force times: the current shape (8 wavefronts, 78 KiB, 16 ahead, not on the grid) takes
+    1.0; (4, 72, 32) takes 0.9 the first time it is timed over the grid's steps and 1.05 after
+    that (a lucky timing); every other shape takes 1.1."""
+
+    def __init__(self, ensemble=False):
+        super().__init__(ensemble)
+        self.opts[mdgpu.OPT_LDS_KB] = 78
+        self.opts[mdgpu.OPT_UNROLL] = 16
+        self.seen = {}
+        self.steps = 0
+
+    def clone(self, **kw):
+        c = LuckyDomain(hasattr(self, "R"))
+        c.opts = dict(self.opts)
+        self.clones.append(c)
+        return c
+
+    def calculate(self, n):
+        self.steps = n
+        return 0
+
+    def timing(self):
+        o = self.opts
+        f = (o[mdgpu.OPT_WAVES], o[mdgpu.OPT_LDS_KB], o[mdgpu.OPT_UNROLL])
+        if f == (8, 78, 16):
+            t = 1.0
+        elif f == (4, 72, 32):
+            k = self.seen.get(f, 0) + (self.steps > 3)
+            self.seen[f] = k
+            t = 0.9 if (self.steps > 3 and k == 1) else 1.05
+        else:
+            t = 1.1
+        return dict(force_ms=t * 10, force_launches=10, total_ms=0.0, rebuilds=0, rebuild_ms=0.0)
+
+
+def test_a_lucky_timing_does_not_displace_the_current_shape(monkeypatch):
+    monkeypatch.setattr(A, "_rebuild_ms", lambda d, reps=3: 5.0)
+    monkeypatch.setattr(A, "_step_ms", lambda d, steps: 1.0)
+    d = LuckyDomain()
+    res = A.autotune(d, steps=30)
+    ch = res["choice"]
+    assert "waves=8 lds_kb=78 unroll=16" in res["force"]  # the current shape is timed on the grid
+    assert res["force"]["waves=4 lds_kb=72 unroll=32"] == 0.9  # and loses to the lucky one there
+    assert (ch["waves"], ch["lds_kb"], ch["unroll"]) == (8, 78, 16)  # but not when timed again
+    assert d.opts[mdgpu.OPT_LDS_KB] == 78 and d.opts[mdgpu.OPT_UNROLL] == 16
