@@ -30,8 +30,10 @@ like the one to be run (after equilibration: a perfect lattice has uniform
 cells and times differently).
 
 - dynamic pruning: the inner-list radius rc + delta, delta in {off, 0.05,
-  0.1, 0.15, 0.2, 0.3}, by the time per step including the pruning passes
-  (the kick sums the same pairs in the same order whatever delta is);
+  0.1, 0.15, 0.2, 0.3}, by the time per step over a window of several
+  constructions, pruning passes included (the kick sums the same pairs in
+  the same order whatever delta is); the winner is timed again against the
+  current setting before it replaces it;
 - list layout (MDGPU_OPT_ROW_SORT): padded 64-row slices in cell order (0),
   slices of rows sorted by list length in steps of 8 (8), or by exact length,
   the CSR-like layout with the least padding (1), by the force kernel's time
@@ -94,19 +96,25 @@ def _pick(cands, times, current):
     return best
 
 
-def _confirm(c, apply, best, current, steps, reps=2):
-    """The grid's winner against the current setting, timed again in turn: the fastest of
-    many noisy timings tends to be a lucky one, so the winner replaces the current setting
-    only if it is also more than 2% faster over these runs."""
+def _confirm(c, apply, best, current, measure, reps=2):
+    """The grid's winner against the current setting, timed again in turn by measure(c): the
+    fastest of many noisy timings tends to be a lucky one, so the winner replaces the current
+    setting only if it is also more than 2% faster over these runs."""
     if best == current:
         return current
     t = {best: 0.0, current: 0.0}
     for _ in range(reps):
         for k in (current, best):
             apply(c, k)
-            _force_ms(c, 3)
-            t[k] += _force_ms(c, steps)
+            t[k] += measure(c)
     return best if t[best] < KEEP * t[current] else current
+
+
+def _force_measure(steps):
+    def measure(c):
+        _force_ms(c, 3)
+        return _force_ms(c, steps)
+    return measure
 
 
 def _step_ms(d, steps):
@@ -141,7 +149,7 @@ def autotune(d, steps=30, margins=None, log=None, cadence_steps=300):
             e.set_option(mdgpu.OPT_BRICK, _brick_code(b))
             e.make_pair_list()
 
-        brick = _confirm(c, set_brick, _pick(cands, bt, cur_b), cur_b, steps)
+        brick = _confirm(c, set_brick, _pick(cands, bt, cur_b), cur_b, _force_measure(steps))
         set_brick(c, brick)
     st = {}
     for w, s in SEARCH:
@@ -167,19 +175,26 @@ def autotune(d, steps=30, margins=None, log=None, cadence_steps=300):
         _force_ms(c, 3)
         res["force"][f"waves={w} lds_kb={kb} unroll={u}"] = ft[(w, kb, u)] = _force_ms(c, steps)
         say(f"force waves={w} lds_kb={kb} unroll={u}: {ft[(w, kb, u)]:.4f} ms")
-    best_f = _confirm(c, set_force, _pick(fcands, ft, cur_f), cur_f, steps)
+    best_f = _confirm(c, set_force, _pick(fcands, ft, cur_f), cur_f, _force_measure(steps))
     set_force(c, best_f)
     res["prune"] = {}
     pt = {}
+
+    def set_prune(e, m):
+        e.set_option(mdgpu.OPT_PRUNE, m)
+
+    def step_measure(e):
+        return _step_ms(e, cadence_steps)
+
+    # per step over a window holding several constructions, as the pruning passes and the
+    # steps that cannot run ahead of the host (with an inner list) cost at every step
     for m in PRUNE:
-        c.set_option(mdgpu.OPT_PRUNE, m)
-        c.calculate(3)
-        t0 = time.perf_counter()
-        c.calculate(2 * steps)
-        res["prune"][f"{m}"] = pt[m] = 1e3 * (time.perf_counter() - t0) / (2 * steps)
+        set_prune(c, m)
+        res["prune"][f"{m}"] = pt[m] = step_measure(c)
         say(f"prune delta={m / 1000}: {pt[m]:.4f} ms per step")
-    best_p = _pick(PRUNE, pt, d.get_option(mdgpu.OPT_PRUNE))
-    c.set_option(mdgpu.OPT_PRUNE, best_p)
+    cur_p = d.get_option(mdgpu.OPT_PRUNE)
+    best_p = _confirm(c, set_prune, _pick(PRUNE, pt, cur_p), cur_p, step_measure)
+    set_prune(c, best_p)
     res["layout"] = {}
     lt = {}
     for v in LAYOUT:
