@@ -810,10 +810,21 @@ int brick_budget(const mdgpu_domain *d) {
   return d->brick_kb ? d->brick_kb : (d->nrep == 1 ? kDomainBrickKb : kReplicaBrickKb);
 }
 
-// The force kernels' LDS budget: unless set, the one the bricks are sized
-// for, so that the largest bricks of a state do not push a workgroup past the
-// occupancy the bricks were sized for (two per CU at 76 or 78 KiB).
-int force_lds_kb(const mdgpu_domain *d) { return d->lds_kb ? d->lds_kb : brick_budget(d); }
+// The force kernels' LDS budget for a single domain whose bricks are sized
+// automatically: more than the bricks' 78 KiB, so that a state whose fullest
+// neighbourhoods outgrow that sizing (density fluctuations) runs one workgroup
+// per CU instead of reading those bricks' partners from memory, several
+// times slower.
+constexpr int kDomainForceKb = 96;
+
+// The force kernels' LDS budget: unless set, the one the bricks are sized for
+// (an ensemble's replicas: two workgroups per CU at 76 KiB), or
+// kDomainForceKb for a single domain with automatic bricks.
+int force_lds_kb(const mdgpu_domain *d) {
+  if (d->lds_kb) return d->lds_kb;
+  if (d->nrep == 1 && !d->brick_kb) return kDomainForceKb;
+  return brick_budget(d);
+}
 
 bool is_unit(const mdgpu_domain *d) { return d->group != nullptr; }
 
