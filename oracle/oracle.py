@@ -201,13 +201,16 @@ def ref_available():
 
 def run_reference(L, x, v, ids, steps, dump_every=1, dt=0.001, thermostat="nve", T=1.0,
                   gamma=0.1, tau=0.1, seed=1, init_v0=None, change_scale=None, timeout=600,
-                  binary=None, kick=True):
+                  binary=None, kick=True, dump_file_step=None):
     """Run the reference's MDManager (compiled unmodified, rc=3.0, margin=0.3).
 
     init_v0: MDManager::SetInitialVelocity(init_v0) after loading (the given
     momenta are replaced).  change_scale = (alpha, step): MDManager::ChangeScale(alpha)
     after that step, followed by a record with scaled=True.  binary: another build of
-    the driver (FACADE_DRIVER_BIN); kick=False leaves the kick section zero."""
+    the driver (FACADE_DRIVER_BIN); kick=False leaves the kick section zero.
+    dump_file_step: MDManager::SaveAsDumpFile after that step (0: before the first);
+    returns (records, the dump as an (n, 6) array of x, y, z, px, py, pz rows in the
+    program's own order)."""
     binary = REF_BIN if binary is None else binary
     if not os.path.exists(binary):
         raise RuntimeError(f"{binary} is not built (run `make -C oracle ref` / `facade_driver`)")
@@ -229,6 +232,8 @@ def run_reference(L, x, v, ids, steps, dump_every=1, dt=0.001, thermostat="nve",
         if change_scale is not None:
             lines.append(f"ChangeScaleAlpha={float(change_scale[0])!r}")
             lines.append(f"ChangeScaleStep={int(change_scale[1])}")
+        if dump_file_step is not None:
+            lines.append(f"DumpFileStep={int(dump_file_step)}")
         with open(spec, "w") as f:
             f.write("\n".join(lines) + "\n")
         pf = os.path.join(td, "particles.bin")
@@ -245,7 +250,10 @@ def run_reference(L, x, v, ids, steps, dump_every=1, dt=0.001, thermostat="nve",
         env = dict(os.environ, OMP_NUM_THREADS="1")
         subprocess.run([binary, spec, pf, out], check=True, env=env, cwd=td,
                        stdout=subprocess.DEVNULL, timeout=timeout)
-        return parse_records(open(out, "rb").read())
+        recs = parse_records(open(out, "rb").read())
+        if dump_file_step is None:
+            return recs
+        return recs, np.fromfile(os.path.join(td, "conf.dump"), dtype=np.float64).reshape(-1, 6)
 
 
 def parse_records(buf):

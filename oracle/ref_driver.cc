@@ -11,7 +11,9 @@
 //   HeatbathTau) plus driver keys read here: Steps, DumpEvery, InitV0, Seed,
 //   ChangeScaleAlpha and ChangeScaleStep (MDManager::ChangeScale(alpha) called
 //   after that step, mdmanager.cc:432-443, followed by a record flagged in
-//   header[6]), Kick (0: the kick section of each record is left zero).
+//   header[6]), Kick (0: the kick section of each record is left zero),
+//   DumpFileStep (MDManager::SaveAsDumpFile("conf.dump") after that step, 0:
+//   before the first; mdmanager.cc:474-504, the binary q, p dump).
 //
 // The same source also links against the MI355X facade (mdacp_amd/facade,
 // `make -C oracle facade_driver` -> oracle/_ref/mdacp_gpu_driver), where it
@@ -36,7 +38,7 @@
 namespace {
 
 struct Spec {
-  int steps = 0, dump_every = 1, seed = 1, scale_step = -1, kick = 1;
+  int steps = 0, dump_every = 1, seed = 1, scale_step = -1, kick = 1, dump_file_step = -1;
   double init_v0 = -1.0, scale_alpha = 1.0;
 };
 
@@ -55,6 +57,7 @@ Spec read_spec(const char *path) {
     if (key == "ChangeScaleAlpha") s.scale_alpha = std::stod(val);
     if (key == "ChangeScaleStep") s.scale_step = std::stoi(val);
     if (key == "Kick") s.kick = std::stoi(val);
+    if (key == "DumpFileStep") s.dump_file_step = std::stoi(val);
   }
   return s;
 }
@@ -257,6 +260,7 @@ int main(int argc, char **argv) {
   mdm->MakePairList();
   const bool kick_on = spec.kick != 0;
   write_record(out, mdm, 0, 1, SL, kick_on);
+  if (spec.dump_file_step == 0) mdm->SaveAsDumpFile("conf.dump");
   MeshGrab grab;
   for (int step = 1; step <= spec.steps; step++) {
     if (step - 1 == spec.scale_step) {
@@ -267,6 +271,7 @@ int main(int argc, char **argv) {
     const int before = grab.mesh->GetNumberOfConstructions();
     mdm->Calculate();
     const int rebuilt = grab.mesh->GetNumberOfConstructions() != before;
+    if (step == spec.dump_file_step) mdm->SaveAsDumpFile("conf.dump");
     if (step % spec.dump_every == 0) write_record(out, mdm, step, rebuilt, SL, kick_on);
     else {
       int64_t hdr[8] = {step, -1, 0, 0, rebuilt, 0, 0, 0};

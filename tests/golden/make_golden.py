@@ -3,7 +3,8 @@
 Runs oracle/_ref/mdacp_ref (kaityo256/mdacp compiled unmodified from its own
 sources, default flags, rc = 3.0 and margin 0.3 as compiled in) on seeded FCC
 inputs and stores small fixtures: hashes of canonical pair sets, per-particle
-kicks for a small system, observables and rebuild schedules.
+kicks for a small system, observables and rebuild schedules, and one binary
+dump (MDManager::SaveAsDumpFile).
 
     python tests/golden/make_golden.py [case ...]
 """
@@ -72,6 +73,18 @@ def run_case(name, L, rho, v0, steps, dump_every, dt=0.001, thermostat="nve", T=
     print(name, n, "rebuilds", out["rebuild_steps"][:12], flush=True)
 
 
+def dump_case(name, L, rho, v0, steps, dt=0.001):
+    """The reference's binary dump (MDManager::SaveAsDumpFile, mdmanager.cc:474-504: x, y, z,
+    px, py, pz per particle, unit by unit) after `steps` NVE steps, rows in the program's
+    own order."""
+    x = fcc(L, rho)
+    n = len(x)
+    _, dump = O.run_reference(L, x, initial_velocity(n, v0, 0), np.arange(n, dtype=np.int32), steps=steps,
+                              dump_every=steps, dt=dt, kick=False, dump_file_step=steps)
+    np.save(os.path.join(OUT, f"{name}.npy"), dump)
+    print(name, dump.shape, flush=True)
+
+
 CASES = {
     # small NVE system: full pair set and kicks at step 0, state hashes to 200 steps
     "fcc2048_nve": lambda: run_case("fcc2048_nve", 16.0, 0.5, np.sqrt(3.0), 200, 50, keep_kicks=True),
@@ -82,6 +95,8 @@ CASES = {
                                            thermostat="nosehoover", T=0.9, tau=0.2, reverse_ids=True),
     # 131072 particles, rho 0.5, L 64: 1000 NVE steps, observables every 100
     "fcc131072_nve": lambda: run_case("fcc131072_nve", 64.0, 0.5, np.sqrt(3.0), 1000, 100),
+    # the binary dump of the small NVE system after 100 steps
+    "fcc2048_nve_dump100": lambda: dump_case("fcc2048_nve_dump100", 16.0, 0.5, np.sqrt(3.0), 100),
     # the quench path in small: FCC at rho 0.75 with the reference's own initial
     # momenta (SetInitialVelocity), 50 NVE steps, ChangeScale to rho 0.6, 100 more
     "fcc2048_quench": lambda: run_case("fcc2048_quench", box_for(8, 0.75), 0.75, 1.5, 150, 25, dt=0.002,

@@ -185,6 +185,40 @@ def test_burst_with_eight_units(tmp_path):
         print(fn, "max |dq|", dq, "max |dp|", dp, flush=True)
 
 
+def test_facade_binary_dump_matches_reference_fixture():
+    """MDManager::SaveAsDumpFile (mdmanager.cc:474-504), compiled unmodified against the
+    facade, writes the device state through the facade's host mirrors: after 100 steps of the
+    fcc2048_nve case, the same particles as the reference's own dump
+    (tests/golden/fcc2048_nve_dump100.npy), within 1e-9 (the two programs sum their kicks in
+    different orders)."""
+    import sys
+
+    from scipy.spatial import cKDTree
+
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from helpers import load_golden
+    from mdacp_amd.lattice import fcc, initial_velocity
+    from oracle import oracle as O
+
+    if not os.path.exists(O.FACADE_DRIVER_BIN):
+        pytest.skip("oracle/_ref/mdacp_gpu_driver not built (make -C oracle facade_driver)")
+    g = load_golden("fcc2048_nve")
+    x = fcc(g["L"], g["density"])
+    n = len(x)
+    print("running mdacp_gpu_driver fcc2048_nve, binary dump after 100 steps", flush=True)
+    _, got = O.run_reference(g["L"], x, initial_velocity(n, g["v0"], 0), np.arange(n, dtype=np.int32),
+                             steps=100, dump_every=100, dt=g["dt"], binary=O.FACADE_DRIVER_BIN, kick=False,
+                             timeout=240, dump_file_step=100)
+    want = np.load(os.path.join(ROOT, "tests", "golden", "fcc2048_nve_dump100.npy"))
+    assert got.shape == want.shape == (n, 6)
+    dist, idx = cKDTree(want[:, :3]).query(got[:, :3])
+    assert len(np.unique(idx)) == n, "positions do not pair one to one"
+    dq = np.abs(got[:, :3] - want[idx, :3]).max()
+    dp = np.abs(got[:, 3:] - want[idx, 3:]).max()
+    print("max |dq|", dq, "max |dp|", dp, flush=True)
+    assert dq <= 1e-9 and dp <= 1e-9, (dq, dp)
+
+
 def test_facade_list_accessors_match_reference_fixture():
     """The reference's MDManager on the facade, driven by oracle/ref_driver.cc linked against
     it (oracle/_ref/mdacp_gpu_driver): the pair lists read through the facade's COO accessors

@@ -92,6 +92,28 @@ def test_oracle_kick_and_pairs_match_reference_fixture(golden_dir):
     assert np.array_equal(O.canonical_pairs(s.pairs()), np.load(f"{golden_dir}/fcc2048_nve_pairs0.npy"))
 
 
+def sorted_rows(a):
+    return a[np.lexsort(a.T[::-1])]
+
+
+def test_binary_dump_fixture_is_the_oracle_state(golden_dir):
+    """The reference's binary dump (MDManager::SaveAsDumpFile, mdmanager.cc:474-504) after
+    100 steps of the fcc2048_nve case holds, up to row order, the oracle's q and p after the
+    same steps, bit for bit: 6 doubles per particle (x, y, z, px, py, pz)."""
+    g = load_golden("fcc2048_nve")
+    x = fcc(g["L"], g["density"])
+    n = len(x)
+    s = O.OracleSystem(g["L"], g["dt"], g["rc"], g["margin"])
+    s.add(x, initial_velocity(n, g["v0"], 0))
+    s.make_pair_list()
+    for _ in range(100):
+        s.step()
+    q, p, _ = s.state()
+    want = np.load(f"{golden_dir}/fcc2048_nve_dump100.npy")
+    assert want.shape == (n, 6)
+    assert np.array_equal(sorted_rows(np.concatenate([q, p], axis=1)), sorted_rows(want))
+
+
 @pytest.mark.skipif(not O.ref_available(), reason="reference build (oracle/_ref) not present")
 def test_oracle_matches_reference_live_shuffled_ids():
     """Random ids, non-lattice start: owned order, migration and rebuilds all bitwise."""
