@@ -566,7 +566,7 @@ def main_units(args):
     bytes_pu, _, _, _ = algorithmic_per_update(args.density, args.rc, args.margin)
     achieved = bytes_pu * n / (r64["force_ms_per_step"] * 1e-3) / 1e9
     roofline = dict(bound="hbm", achieved=achieved, peak=HBM_PEAK_GBS, unit="GB/s", frac=achieved / HBM_PEAK_GBS,
-                    traffic=None, kernel="k_force_brick_fp64 (one launch per unit, summed over the units of a step)",
+                    traffic=None, kernel="k_force_group_fp64 (the bricks of all units in one launch per step; per-unit launches summed when a group cannot batch)",
                     kernel_avg_ms=r64["force_ms_per_step"], bytes_per_particle_update=bytes_pu)
     ref = reference_cpu()
     ref8 = [r for r in (ref or []) if r["cores"] == 8]

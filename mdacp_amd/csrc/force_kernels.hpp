@@ -437,18 +437,17 @@ __device__ __forceinline__ double finish_row(int i, double4 p, const double4 qi,
 
 // FP64 kick: one workgroup per brick, wavefronts over the brick's slices;
 // U groups of list indices in flight ahead of the group computed.
+// The FP64 kick of brick b by the calling workgroup (the body of
+// k_force_brick_fp64 and of the batched k_force_group_fp64).
 template <int W, int U>
-__global__ __launch_bounds__(kWave *W) void k_force_brick_fp64(const double4 *__restrict__ src,
-                                                              double4 *__restrict__ mom,
-                                                              double4 *__restrict__ pos_out, BrickList bl,
-                                                              ForceConsts fc, int epilogue, StepClock sc,
-                                                              const int *__restrict__ id,
-                                                              double *__restrict__ vpart, NextDrift nd) {
+__device__ __forceinline__ void brick_kick_fp64(const int b, const double4 *__restrict__ src,
+                                                double4 *__restrict__ mom, double4 *__restrict__ pos_out,
+                                                const BrickList &bl, const ForceConsts &fc, int epilogue,
+                                                const StepClock &sc, const int *__restrict__ id,
+                                                double *__restrict__ vpart, const NextDrift &nd) {
   extern __shared__ __align__(16) double lds64[];
   __shared__ int slice_next;  // the next slice a wavefront takes
   constexpr int T = kWave * W;
-  if (run_halted(sc.gate)) return;
-  const int b = blockIdx.x;
   const Replica &Rp = bl.reps[bl.brep[b]];
   const LangevinConsts lc = langevin_consts(Rp, sc.dt, step_of(sc));
   const int nrow = bl.nrow[b], row0 = bl.row0[b], vs0 = bl.vslice0[b];
@@ -487,6 +486,50 @@ __global__ __launch_bounds__(kWave *W) void k_force_brick_fp64(const double4 *__
     }
   }
   if (vpart) block_max_to_partial(vmax, vpart, b);
+}
+
+template <int W, int U>
+__global__ __launch_bounds__(kWave *W) void k_force_brick_fp64(const double4 *__restrict__ src,
+                                                              double4 *__restrict__ mom,
+                                                              double4 *__restrict__ pos_out, BrickList bl,
+                                                              ForceConsts fc, int epilogue, StepClock sc,
+                                                              const int *__restrict__ id,
+                                                              double *__restrict__ vpart, NextDrift nd) {
+  if (run_halted(sc.gate)) return;
+  brick_kick_fp64<W, U>(blockIdx.x, src, mom, pos_out, bl, fc, epilogue, sc, id, vpart, nd);
+}
+
+// The units of a process grid, kicked in one launch: unit u's bricks are the
+// launch's workgroups [brick0 of u, brick0 of u + 1), each doing what the
+// unit's own kick does (CalculateForce in place, no epilogue), on the unit's
+// own arrays.  One launch of all the units' bricks fills the CUs where one
+// launch per unit ends each unit with a partial round of workgroups.  The
+// table travels in the kernel arguments (1 KB for kMaxKickUnits units).
+struct UnitKick {
+  const double4 *src;  // positions, owned then ghosts (kicked in place: no position output)
+  double4 *mom;
+  BrickList bl;
+  int brick0;          // the unit's first workgroup in the launch
+};
+constexpr int kMaxKickUnits = 8;
+struct UnitKicks {
+  UnitKick u[kMaxKickUnits];
+  int n;
+};
+
+// The unit of workgroup b (units in brick0 order; uniform across the workgroup).
+__device__ __forceinline__ int group_kick_unit(const UnitKicks &uk, int b) {
+  int u = 0;
+  while (u + 1 < uk.n && b >= uk.u[u + 1].brick0) u++;
+  return u;
+}
+
+template <int W, int U>
+__global__ __launch_bounds__(kWave *W) void k_force_group_fp64(UnitKicks uk, ForceConsts fc, StepClock sc) {
+  const int b = blockIdx.x;
+  const UnitKick &k = uk.u[group_kick_unit(uk, b)];
+  brick_kick_fp64<W, U>(b - k.brick0, k.src, k.mom, nullptr, k.bl, fc, kKickOnly, sc, nullptr, nullptr,
+                        NextDrift{nullptr, nullptr, nullptr, nullptr});
 }
 
 // ---------------------------------------------------------------------------
@@ -774,15 +817,15 @@ __global__ __launch_bounds__(kWave *W) void k_force_brick_mixed(
 // (k_force_brick_mixed) rounds 4 times finer at the cost of one offset lookup
 // per pair.  Sums, epilogue and FP64 momenta as in k_force_brick_mixed.
 template <int W, int U, int kSum>
-__global__ __launch_bounds__(kWave *W) void k_force_brick_mixed_brick(
-    const double4 *__restrict__ src, double4 *__restrict__ mom, double4 *__restrict__ pos_out, BrickList bl,
-    ForceConstsF fc, double dt2, int epilogue, StepClock sc, const int *__restrict__ id, double *__restrict__ vpart,
-    NextDrift nd) {
+__device__ __forceinline__ void brick_kick_mixed_brick(const int b, const double4 *__restrict__ src,
+                                                       double4 *__restrict__ mom, double4 *__restrict__ pos_out,
+                                                       const BrickList &bl, const ForceConstsF &fc, double dt2,
+                                                       int epilogue, const StepClock &sc,
+                                                       const int *__restrict__ id, double *__restrict__ vpart,
+                                                       const NextDrift &nd) {
   extern __shared__ float4 lds32[];
   __shared__ int slice_next;  // the next slice a wavefront takes
   constexpr int T = kWave * W;
-  if (run_halted(sc.gate)) return;
-  const int b = blockIdx.x;
   const Replica &Rp = bl.reps[bl.brep[b]];
   const LangevinConsts lc = langevin_consts(Rp, sc.dt, step_of(sc));
   const int nrow = bl.nrow[b], row0 = bl.row0[b], vs0 = bl.vslice0[b];
@@ -851,6 +894,26 @@ __global__ __launch_bounds__(kWave *W) void k_force_brick_mixed_brick(
     }
   }
   if (vpart) block_max_to_partial(vmax, vpart, b);
+}
+
+template <int W, int U, int kSum>
+__global__ __launch_bounds__(kWave *W) void k_force_brick_mixed_brick(
+    const double4 *__restrict__ src, double4 *__restrict__ mom, double4 *__restrict__ pos_out, BrickList bl,
+    ForceConstsF fc, double dt2, int epilogue, StepClock sc, const int *__restrict__ id, double *__restrict__ vpart,
+    NextDrift nd) {
+  if (run_halted(sc.gate)) return;
+  brick_kick_mixed_brick<W, U, kSum>(blockIdx.x, src, mom, pos_out, bl, fc, dt2, epilogue, sc, id, vpart, nd);
+}
+
+// The units of a process grid kicked in one launch (UnitKick), mixed
+// precision in the brick-centred frame.
+template <int W, int U, int kSum>
+__global__ __launch_bounds__(kWave *W) void k_force_group_mixed_brick(UnitKicks uk, ForceConstsF fc, double dt2,
+                                                                     StepClock sc) {
+  const int b = blockIdx.x;
+  const UnitKick &k = uk.u[group_kick_unit(uk, b)];
+  brick_kick_mixed_brick<W, U, kSum>(b - k.brick0, k.src, k.mom, nullptr, k.bl, fc, dt2, kKickOnly, sc, nullptr,
+                                     nullptr, NextDrift{nullptr, nullptr, nullptr, nullptr});
 }
 
 // Dynamic pruning of the pair list (no counterpart in the reference: it

@@ -2490,6 +2490,94 @@ int group_temperature(mdgpu_domain *d, const std::vector<mdgpu_domain *> &us, do
 // (Langevin), or the Nose-Hoover sequence with the group's temperature.  The
 // units' thermostat is the one d has.  With timing on (mdgpu_enable_timing on
 // d), every unit records its force launches and d the wall time of the steps.
+// CalculateForce of every unit of the group (force_in_place on each), in one
+// launch of all their bricks when the units run the same kick (FP64, or mixed
+// precision in the brick frame) and there are at most kMaxKickUnits of them,
+// with the launch shape of the unit stepping the group; else one launch per
+// unit.  The units' results are those of their own launches bit for bit: each
+// workgroup does what the unit's own kick does to that brick.
+template <int W, int U>
+void launch_group_kick(mdgpu_domain *d, const UnitKicks &uk, int nb, size_t shm) {
+  const StepClock sc{d->dt, d->step, nullptr, nullptr};
+  if (d->precision == MDGPU_FP64) {
+    allow_lds(k_force_group_fp64<W, U>, shm);
+    hipLaunchKernelGGL((k_force_group_fp64<W, U>), dim3(nb), dim3(kWave * W), shm, d->stream, uk, fconsts(d), sc);
+    return;
+  }
+  const ForceConstsF f32 = fconsts32(d);
+  const double dt2 = d->dt * 0.5;
+  switch (d->kahan) {
+    case kSumPlain:
+      allow_lds(k_force_group_mixed_brick<W, U, kSumPlain>, shm);
+      hipLaunchKernelGGL((k_force_group_mixed_brick<W, U, kSumPlain>), dim3(nb), dim3(kWave * W), shm, d->stream, uk,
+                         f32, dt2, sc);
+      break;
+    case kSumKahan:
+      allow_lds(k_force_group_mixed_brick<W, U, kSumKahan>, shm);
+      hipLaunchKernelGGL((k_force_group_mixed_brick<W, U, kSumKahan>), dim3(nb), dim3(kWave * W), shm, d->stream, uk,
+                         f32, dt2, sc);
+      break;
+    default:
+      allow_lds(k_force_group_mixed_brick<W, U, kSumGroup64>, shm);
+      hipLaunchKernelGGL((k_force_group_mixed_brick<W, U, kSumGroup64>), dim3(nb), dim3(kWave * W), shm, d->stream,
+                         uk, f32, dt2, sc);
+      break;
+  }
+}
+
+template <int W>
+void launch_group_kick_w(mdgpu_domain *d, const UnitKicks &uk, int nb, size_t shm) {
+  if (unroll_of(d) == 32) launch_group_kick<W, 4>(d, uk, nb, shm);
+  else if (unroll_of(d) == 16) launch_group_kick<W, 2>(d, uk, nb, shm);
+  else launch_group_kick<W, 1>(d, uk, nb, shm);
+}
+
+int group_force(mdgpu_domain *d, const std::vector<mdgpu_domain *> &us) {
+  bool batch = (d->precision == MDGPU_FP64 || (d->precision == MDGPU_MIXED && d->mixed_frame == 1)) &&
+               !d->pipeline && us.size() <= (size_t)kMaxKickUnits;
+  for (mdgpu_domain *u : us)
+    batch = batch && u->precision == d->precision && u->mixed_frame == d->mixed_frame && u->kahan == d->kahan &&
+            u->dt == d->dt && u->CL2 == d->CL2 && u->C2 == d->C2 && u->step == d->step;
+  if (!batch) {
+    for (mdgpu_domain *u : us) CHK(force_in_place(u));
+    return MDGPU_OK;
+  }
+  const int bytes = d->precision == MDGPU_FP64 ? 3 * (int)sizeof(double) : (int)sizeof(float4);
+  UnitKicks uk{};
+  uk.n = (int)us.size();
+  int nb = 0;
+  size_t shm = 0;
+  for (size_t k = 0; k < us.size(); k++) {
+    mdgpu_domain *u = us[k];
+    invalidate(u);
+    u->vmax_valid = false;
+    UnitKick &e = uk.u[k];
+    e.src = u->pos;
+    e.mom = u->mom;
+    e.bl = brick_list(u, bytes);
+    e.brick0 = nb;
+    if (u->pn > 0) {
+      nb += u->nbrick;
+      shm = std::max(shm, (size_t)e.bl.lds_cap * bytes);
+    }
+  }
+  if (nb == 0) return MDGPU_OK;
+  const int ev0 = d->force_launches * 2;
+  const bool ev = d->timing && ev0 + 1 < (int)d->ev.size();
+  if (ev) HIPCHK(d, hipEventRecord(d->ev[ev0], d->stream));
+  switch (d->waves) {
+    case 2: launch_group_kick_w<2>(d, uk, nb, shm); break;
+    case 8: launch_group_kick_w<8>(d, uk, nb, shm); break;
+    case 16: launch_group_kick_w<16>(d, uk, nb, shm); break;
+    default: launch_group_kick_w<4>(d, uk, nb, shm); break;
+  }
+  if (ev) {
+    HIPCHK(d, hipEventRecord(d->ev[ev0 + 1], d->stream));
+    d->force_launches++;
+  }
+  return check_launch(d, "group force");
+}
+
 int group_calculate(mdgpu_domain *d, int nsteps, int *rebuilds) {
   const std::vector<mdgpu_domain *> us = group_units(d);
   for (mdgpu_domain *u : us)
@@ -2530,23 +2618,23 @@ int group_calculate(mdgpu_domain *d, int nsteps, int *rebuilds) {
     if (d->thermostat == MDGPU_NOSEHOOVER) {
       double t = 0.0;
       if ((r = group_temperature(d, us, &t)) != MDGPU_OK) break;
-      for (mdgpu_domain *u : us) {
-        if ((r = heatbath_zeta(u, t)) != MDGPU_OK || (r = heatbath_momenta(u)) != MDGPU_OK ||
-            (r = force_in_place(u)) != MDGPU_OK || (r = heatbath_momenta(u)) != MDGPU_OK)
-          break;
-      }
+      // each unit's sequence (scale, kick, scale) touches only its own particles
+      for (mdgpu_domain *u : us)
+        if ((r = heatbath_zeta(u, t)) != MDGPU_OK || (r = heatbath_momenta(u)) != MDGPU_OK) break;
+      if (r != MDGPU_OK || (r = group_force(d, us)) != MDGPU_OK) break;
+      for (mdgpu_domain *u : us)
+        if ((r = heatbath_momenta(u)) != MDGPU_OK) break;
       if (r != MDGPU_OK) break;
       if ((r = group_temperature(d, us, &t)) != MDGPU_OK) break;
       for (mdgpu_domain *u : us)
         if ((r = heatbath_zeta(u, t)) != MDGPU_OK) break;
       if (r != MDGPU_OK || (r = group_drift(d, us)) != MDGPU_OK) break;
     } else if (d->thermostat == MDGPU_LANGEVIN) {
+      if ((r = group_force(d, us)) != MDGPU_OK) break;
       for (mdgpu_domain *u : us)
-        if ((r = force_in_place(u)) != MDGPU_OK || (r = mdgpu_langevin(u)) != MDGPU_OK) break;
+        if ((r = mdgpu_langevin(u)) != MDGPU_OK) break;
     } else {
-      for (mdgpu_domain *u : us)
-        if ((r = force_in_place(u)) != MDGPU_OK) break;
-      if (r != MDGPU_OK || (r = group_drift(d, us)) != MDGPU_OK) break;
+      if ((r = group_force(d, us)) != MDGPU_OK || (r = group_drift(d, us)) != MDGPU_OK) break;
     }
     if (r != MDGPU_OK) break;
     for (mdgpu_domain *u : us) {

@@ -271,11 +271,13 @@ def test_unit_refusals():
     run_sequence(units, True, 5, 5)
 
 
-@pytest.mark.parametrize("thermostat", [mdgpu.NVE, mdgpu.LANGEVIN])
-def test_group_calculate_is_the_sequence(thermostat):
+@pytest.mark.parametrize("thermostat,precision", [(mdgpu.NVE, mdgpu.FP64), (mdgpu.LANGEVIN, mdgpu.FP64),
+                                                   (mdgpu.NVE, mdgpu.MIXED)])
+def test_group_calculate_is_the_sequence(thermostat, precision):
     """mdgpu_group_calculate (the whole of MDManager::Calculate for the group in one call)
     gives the trajectory of the same steps taken through the per-unit calls, bit for bit:
-    the same kernels in the same order, the expiry tests batched over the group."""
+    the same per-brick work in the same order (the units' kicks in one launch of all their
+    bricks instead of one launch each), the expiry tests batched over the group."""
     L = 20.0
     q, p, ids = lattice_state(L, 0.8, 1.5, 13)
     a = make_units(L, (2, 2, 2), q, p, ids)
@@ -283,7 +285,8 @@ def test_group_calculate_is_the_sequence(thermostat):
     for u in a + b:
         u.set_thermostat(thermostat, T=1.2, gamma=0.5)
         u.set_seed(7)
-    say("group_calculate vs per-unit sequence, thermostat", thermostat)
+        u.set_precision(precision)
+    say("group_calculate vs per-unit sequence, thermostat", thermostat, "precision", precision)
     if thermostat == mdgpu.LANGEVIN:
         rows, rb = [], []
         for s in range(200):
