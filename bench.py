@@ -216,8 +216,8 @@ def energy_drift(d, steps, every=10, blocks=20):
 
 
 def drift_comparison(d64, d32):
-    """The north star's drift criterion, |slope(mixed)| <= 2 |slope(FP64)|, judged with the
-    FP64 slope replaced by its detection floor when it is below that floor."""
+    """The mixed-precision path's drift target, |slope(mixed)| <= 2 |slope(FP64)|, judged with
+    the FP64 slope replaced by its detection floor when it is below that floor."""
     if not d64 or not d32:
         return None
     ref = max(abs(d64["slope_per_1e4_steps"]), d64["floor_per_1e4_steps"])
