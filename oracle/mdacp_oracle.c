@@ -1,7 +1,7 @@
 /*
  * mdacp_oracle.c -- CPU restatement of MDACP's single-domain LJ step.
  *
- * TEST INFRASTRUCTURE ONLY: the checker for the HIP path (see mdacp_oracle.h).
+ * TEST INFRASTRUCTURE ONLY: the HIP path's results are tested against it (see mdacp_oracle.h).
  * Build with -O2 -ffp-contract=off and no -march flag so that no fused
  * multiply-add is emitted: the reference's default build has none either.
  *

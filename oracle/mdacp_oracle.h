@@ -1,7 +1,7 @@
 /*
  * mdacp_oracle.h -- CPU restatement of MDACP's single-domain Lennard-Jones step.
  *
- * TEST INFRASTRUCTURE ONLY.  This library is the checker for the HIP path: only
+ * TEST INFRASTRUCTURE ONLY.  The HIP path's results are tested against this library: only
  * tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may load it.
  * The product (mdacp_amd/libmdgpu.so) never links, loads or calls it.
  *

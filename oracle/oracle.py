@@ -207,7 +207,7 @@ def run_reference(L, x, v, ids, steps, dump_every=1, dt=0.001, thermostat="nve",
     init_v0: MDManager::SetInitialVelocity(init_v0) after loading (the given
     momenta are replaced).  change_scale = (alpha, step): MDManager::ChangeScale(alpha)
     after that step, followed by a record with scaled=True.  binary: another build of
-    the driver (FACADE_DRIVER_BIN); kick=False leaves the kick section zero.
+    the driver program (FACADE_DRIVER_BIN); kick=False leaves the kick section zero.
     dump_file_step: MDManager::SaveAsDumpFile after that step (0: before the first);
     returns (records, the dump as an (n, 6) array of x, y, z, px, py, pz rows in the
     program's own order)."""

@@ -1,4 +1,4 @@
-"""Pins the CPU oracle (oracle/liboracle.so) before it is trusted as the checker.
+"""Pins the CPU oracle (oracle/liboracle.so) before the GPU tests trust it.
 
 * against golden fixtures written by the reference itself (tests/golden);
 * against the compiled reference directly, when oracle/_ref is built here;
