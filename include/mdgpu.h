@@ -69,7 +69,7 @@
  *   mdgpu_group_make_pair_list    MDManager::MakePairList (mdmanager.cc:262-280) for every
  *                                 unit of the group, in the reference's order
  *   mdgpu_group_calculate         MDManager::Calculate (mdmanager.cc:116-192) for every unit
- *                                 of the group, n steps
+ *                                 of the group, n steps (the units' kicks in one launch)
  * MDManager calls each unit's operations between its phase barriers
  * (omp parallel for): a unit's work is queued on the group's stream after the
  * work of every unit's previous phase.  The other step operations
