@@ -132,6 +132,7 @@ struct mdgpu_domain {
   int *h_which = nullptr, *h_rep_exp = nullptr;
   unsigned long long *clock = nullptr;  // step counter read by replayed force launches
   unsigned long long *chk_dmax = nullptr;  // displacement check of listed replicas: max |dq|^2 bits,
+  unsigned long long *h_chk_dmax = nullptr;  // and its host copy
   int *chk_cnt = nullptr, *chk_cand = nullptr;  // candidate counts, candidates (cap per replica)
   bool clock_dirty = true;              // clock differs from `step`
 
@@ -1202,9 +1203,26 @@ int is_expired_ensemble(mdgpu_domain *d, int *expired) {
                      d->grng, cap, d->SL * d->SL, d->CL2, d->chk_cnt, d->chk_cand, d->rep_exp);
   CHK(check_launch(d, "displacement check"));
   HIPCHK(d, hipMemcpyAsync(d->h_rep_exp, d->rep_exp, nflag * sizeof(int), hipMemcpyDeviceToHost, st));
+  HIPCHK(d, hipMemcpyAsync(d->h_chk_dmax, d->chk_dmax, nflag * sizeof(unsigned long long), hipMemcpyDeviceToHost,
+                           st));
   CHK(sync(d));
   for (int k = 0; k < nflag; k++)
     if (d->h_rep_exp[k]) *expired = 1;
+  if (*expired) return MDGPU_OK;
+  // A replica whose list passed the check takes a budget again from its
+  // measured displacement: no particle has moved more than dmax, so pairs
+  // outside the list stay beyond rc while 2 (dmax + the motion since) stays
+  // below the margin, which the budget counts down as before.  While that
+  // budget lasts, the check could not expire the list (its candidates move
+  // more than margin - dmax, and none can while 2 dmax < margin), so the
+  // expiry step is the one of checking every step, with fewer checks and the
+  // steps in between running ahead of the host.
+  for (int k = 0; k < nflag; k++) {
+    double d2;
+    std::memcpy(&d2, &d->h_chk_dmax[k], sizeof d2);
+    const double left = d->margin - 2.0 * std::sqrt(d2);
+    if (left > 0.0) d->rst[d->h_which[k]].buffer_length = left;
+  }
   return MDGPU_OK;
 }
 
@@ -1955,7 +1973,8 @@ int create_domain(const mdgpu_params *prm, int nrep, const mdgpu_replica *rl, co
       hipHostMalloc((void **)&d->h_gate, sizeof(RunGate)) != hipSuccess ||
       hipHostMalloc((void **)&d->h_gbudget, nrep * sizeof(double)) != hipSuccess ||
       hipHostMalloc((void **)&d->h_glife, nrep * sizeof(int)) != hipSuccess ||
-      hipHostMalloc((void **)&d->h_rep_factor, nrep * sizeof(double)) != hipSuccess) {
+      hipHostMalloc((void **)&d->h_rep_factor, nrep * sizeof(double)) != hipSuccess ||
+      hipHostMalloc((void **)&d->h_chk_dmax, nrep * sizeof(unsigned long long)) != hipSuccess) {
     mdgpu_destroy(d);
     return MDGPU_ERR_HIP;
   }
@@ -2867,7 +2886,8 @@ int mdgpu_destroy(mdgpu_domain *d) {
   for (void *b : bufs)
     if (b) (void)hipFree(b);
   for (void *h : {(void *)d->h_rep_v2, (void *)d->h_rep_out, (void *)d->h_which, (void *)d->h_rep_exp,
-                  (void *)d->h_gate, (void *)d->h_gbudget, (void *)d->h_glife, (void *)d->h_rep_factor})
+                  (void *)d->h_gate, (void *)d->h_gbudget, (void *)d->h_glife, (void *)d->h_rep_factor,
+                  (void *)d->h_chk_dmax})
     if (h) (void)hipHostFree(h);
   if (d->h_u64) (void)hipHostFree(d->h_u64);
   if (d->h_i64) (void)hipHostFree(d->h_i64);
