@@ -1092,7 +1092,27 @@ int enqueue_displacement_check(mdgpu_domain *d) {
   }
   CHK(check_launch(d, "displacement check"));
   HIPCHK(d, hipMemcpyAsync(d->h_i32, d->i32, sizeof(int) * 4, hipMemcpyDeviceToHost, st));
+  HIPCHK(d, hipMemcpyAsync(d->h_u64 + 1, d->u64 + 1, sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
   return MDGPU_OK;
+}
+
+// A list that passed the displacement check takes a budget again from the
+// largest displacement dmax the check measured.  No particle has moved more
+// than dmax since the construction, so a pair outside the list stays beyond
+// rc while 2 (dmax + the motion since) stays below the margin, which the
+// budget counts down as before.  While that budget lasts the check could not
+// expire the list (its candidates have moved more than margin - dmax, and
+// none has while 2 dmax < margin), so the list expires on the step it would
+// with a check on every step, after fewer checks and with the steps in
+// between queued ahead of the host.  `dmax2` is max |dq|^2.
+double budget_after_check(double margin, double dmax2) { return margin - 2.0 * std::sqrt(dmax2); }
+
+// the check's max |dq|^2, read with its outcome
+void rearm_after_check(mdgpu_domain *d) {
+  double d2;
+  std::memcpy(&d2, d->h_u64 + 1, sizeof d2);
+  const double left = budget_after_check(d->margin, d2);
+  if (left > 0.0) d->buffer_length = left;
 }
 
 int displacement_check_result(const mdgpu_domain *d, int *expired) {
@@ -1143,8 +1163,12 @@ int is_expired(mdgpu_domain *d, int *expired) {
   }
   *expired = 0;
   if (d->buffer_length < 0.0) {
-    if (d->disp_check) CHK(check_by_displacement(d, expired));
-    else *expired = 1;  // rebuild cadence without the exact check: the budget alone
+    if (d->disp_check) {
+      CHK(check_by_displacement(d, expired));
+      if (!*expired) rearm_after_check(d);
+    } else {
+      *expired = 1;  // rebuild cadence without the exact check: the budget alone
+    }
   }
   return MDGPU_OK;
 }
@@ -1209,18 +1233,11 @@ int is_expired_ensemble(mdgpu_domain *d, int *expired) {
   for (int k = 0; k < nflag; k++)
     if (d->h_rep_exp[k]) *expired = 1;
   if (*expired) return MDGPU_OK;
-  // A replica whose list passed the check takes a budget again from its
-  // measured displacement: no particle has moved more than dmax, so pairs
-  // outside the list stay beyond rc while 2 (dmax + the motion since) stays
-  // below the margin, which the budget counts down as before.  While that
-  // budget lasts, the check could not expire the list (its candidates move
-  // more than margin - dmax, and none can while 2 dmax < margin), so the
-  // expiry step is the one of checking every step, with fewer checks and the
-  // steps in between running ahead of the host.
+  // each replica whose list passed takes a budget again (budget_after_check)
   for (int k = 0; k < nflag; k++) {
     double d2;
     std::memcpy(&d2, &d->h_chk_dmax[k], sizeof d2);
-    const double left = d->margin - 2.0 * std::sqrt(d2);
+    const double left = budget_after_check(d->margin, d2);
     if (left > 0.0) d->rst[d->h_which[k]].buffer_length = left;
   }
   return MDGPU_OK;
@@ -2432,6 +2449,10 @@ int group_expired(mdgpu_domain *d, const std::vector<mdgpu_domain *> &us, bool *
     CHK(displacement_check_result(us[k], &e));
     if (e) *any = true;
   }
+  // without an expiry, the units whose list passed take a budget again
+  if (!*any)
+    for (size_t k = 0; k < us.size(); k++)
+      if (check_of[k]) rearm_after_check(us[k]);
   return MDGPU_OK;
 }
 
