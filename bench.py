@@ -16,8 +16,8 @@ the host as the CPU baseline.
 --workload ensemble: the batched-replica workload instead -- 256 independent
 Langevin replicas (N = 4 k^3 drawn log-uniform in [2048, 65536], density in
 [0.3, 0.9], T in [0.7, 1.5], gamma = 1, own key each), rc = 2.5, margin 0.5,
-dt = 0.005, all advanced by one pair-force launch per step replayed from a
-captured graph; particle-updates/s summed over the replicas, FP64 and mixed.
+dt = 0.005, all advanced by one pair-force launch per step; particle-updates/s
+summed over the replicas, FP64 and mixed.
 
 --workload units: the default workload on the units of a process grid (--grid, default
 2,2,2: MDManager's decomposition for OMP_NUM_THREADS=8), all on this GPU, with
@@ -124,8 +124,8 @@ def run_gpu(precision, x, p, L, args, observe_every=100):
     tuned = tune(d, args)
     d.calculate(20)
     d.observables()
-    # the timed steps: graph-replayed fused steps, queued ahead of the host while the
-    # validity budgets allow (the library's defaults), observed after the first step
+    # the timed steps: fused steps queued ahead of the host while the validity budgets
+    # allow (the library's defaults), observed after the first step
     # and every observe_every steps (benchmark.cc:32-40)
     rebuilds = 0
     n_obs = 0
@@ -393,7 +393,7 @@ def main_ensemble(args):
         "dtype": "f64",
         "data": "synthetic: per-replica FCC lattices (confmaker.cc:8-44), |v0|=sqrt(3T) (variables.cc:82-108)",
         "config": {"workload": (f"{len(table)} Langevin replicas, {n} particles, rc={args.rc} margin={args.margin} "
-                                f"dt={args.dt} gamma=1, one pair-force launch per step (graph replay)"),
+                                f"dt={args.dt} gamma=1, one pair-force launch per step"),
                    "particles": n, "replicas": len(table), "thermalize": args.thermalize,
                    "rebuilds_in_timed_steps": r64["rebuilds"]},
         "amortisation": {"rebuilds_per_step": r64["rebuilds_per_step"], "cadence_sample": r64["cadence_sample"],

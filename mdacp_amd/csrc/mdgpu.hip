@@ -153,7 +153,7 @@ struct mdgpu_domain {
   long long ucap = 0;
 
   // captured fused steps, one per parity of the position buffers
-  bool use_graph = true;
+  bool use_graph = false;   // fused steps replayed from a captured graph (launched directly when false)
   long epoch = 0;  // bumped whenever a captured step may no longer match
   StepGraph graphs[2];
   // runs of steps queued without waiting for the host (k_gate at the head of
@@ -1745,11 +1745,11 @@ int calculate_step(mdgpu_domain *d) {
 }
 
 // Whether the next steps may be queued without waiting for the host: fused
-// NVE or Langevin steps replayed from graphs, no inner list, and the last step
+// NVE or Langevin steps (replayed from graphs or launched), no inner list, and the last step
 // a fused one (its maxima of |p|^2 and the next first half drift are on the
 // device).
 bool can_run_ahead(const mdgpu_domain *d) {
-  return d->run_ahead > 0 && d->use_graph && !d->timing && d->thermostat != MDGPU_NOSEHOOVER &&
+  return d->run_ahead > 0 && !d->timing && d->thermostat != MDGPU_NOSEHOOVER &&
          !(d->prune_delta > 0.0) && d->initialized && d->mid_valid && d->vmax_valid && d->pn > 0;
 }
 
@@ -1796,7 +1796,8 @@ int run_gated(mdgpu_domain *d, int k, int *done) {
     d->clock_dirty = false;
   }
   for (int j = 0; j < k; j++) {
-    CHK(launch_fused_graph(d, true));
+    if (d->use_graph) CHK(launch_fused_graph(d, true));
+    else CHK(enqueue_fused(d, d->clock, false, true));  // the same launches, each queued directly
     std::swap(d->pos_tmp, d->pos_next);
     if (mixed) std::swap(d->rel_tmp, d->rel_next);
   }

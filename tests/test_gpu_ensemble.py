@@ -1,5 +1,5 @@
 """GPU parity of the ensemble path (mdgpu_create_ensemble): independent replicas
-advanced by one pair-force launch per step, replayed from a captured graph.
+advanced by one pair-force launch per step (launched, or replayed from a captured graph).
 
 Bars:
 - each replica's pair set bit-exact with the oracle's for that replica alone;
@@ -188,7 +188,7 @@ def test_graph_replay_matches_launches(precision):
     say(f"graph replay vs launches, precision {precision}")
     a, _ = ensemble(SPEC, melt_seed=4, precision=precision)
     b, _ = ensemble(SPEC, melt_seed=4, precision=precision)
-    b.set_option(mdgpu.OPT_GRAPH, 0)
+    a.set_option(mdgpu.OPT_GRAPH, 1)
     assert a.get_option(mdgpu.OPT_GRAPH) == 1 and b.get_option(mdgpu.OPT_GRAPH) == 0
     ra = sum(a.calculate(10) for _ in range(8))
     rb = sum(b.calculate(10) for _ in range(8))

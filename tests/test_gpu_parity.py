@@ -330,6 +330,42 @@ def test_runs_ahead_match_steps_one_at_a_time(thermostat, precision, frame):
     b.close()
 
 
+@pytest.mark.parametrize("thermostat,precision", [("nve", mdgpu.FP64), ("langevin", mdgpu.FP64),
+                                                  ("langevin", mdgpu.MIXED)])
+def test_runs_launched_directly_match_graph_runs(thermostat, precision):
+    """A run's steps launched one kernel at a time (MDGPU_OPT_GRAPH 0) give the same bits, time
+    and constructions as the same run replayed from captured graphs."""
+    say("case: direct runs", thermostat, precision)
+    L = 16.0
+    x = fcc(L, 0.5)
+    n = len(x)
+    p = initial_velocity(n, math.sqrt(3.0 * 1.5), 4)
+    ds = []
+    for graph in (1, 0):
+        d = mdgpu.Domain(L, dt=0.005, precision=precision, capacity=n)
+        d.add_particles(x, p)
+        if thermostat == "langevin":
+            d.set_thermostat(mdgpu.LANGEVIN, 1.2, 1.0, 0.1)
+            d.set_seed(5)
+        d.set_option(mdgpu.OPT_GRAPH, graph)
+        ds.append(d)
+    a, b = ds
+    assert a.get_option(mdgpu.OPT_GRAPH) == 1 and b.get_option(mdgpu.OPT_GRAPH) == 0
+    ra = rb = 0
+    for chunk in (1, 64, 3, 90):
+        ra += a.calculate(chunk)
+        rb += b.calculate(chunk)
+        assert ra == rb, (chunk, ra, rb)
+    assert ra >= 4, ra
+    qa, pa, ta = a.particles()
+    qb, pb, tb = b.particles()
+    assert np.array_equal(ta, tb)
+    assert np.array_equal(qa, qb) and np.array_equal(pa, pb)
+    assert a.time() == b.time()
+    a.close()
+    b.close()
+
+
 @pytest.mark.parametrize("thermostat", ["nve", "langevin"])
 def test_timed_steps_match_runs(thermostat):
     """Timed steps (HIP events around each kick, launched one at a time, final positions written
