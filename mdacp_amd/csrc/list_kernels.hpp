@@ -391,33 +391,40 @@ __device__ __forceinline__ void row_position(int r, int ur0, const double *__res
   }
 }
 
-__device__ __forceinline__ void test_row(int r, int lane, double qx, double qy, double qz, double cx, double cy,
+// v with lane r (uniform) replaced by the uniform word s: one v_writelane_b32
+// (a compare and a select per word otherwise, plus a copy of s to a VGPR).  The
+// lane select is read from M0, as the instruction reads one SGPR besides it;
+// the nop covers a write of s to VCC, or of M0, just before.
+__device__ __forceinline__ uint32_t put_lane(uint32_t v, uint32_t s, int r) {
+  asm("s_nop 1\n\tv_writelane_b32 %0, %1, m0" : "+v"(v) : "s"(s), "{m0}"(r));
+  return v;
+}
+
+__device__ __forceinline__ void test_row(int r, double qx, double qy, double qz, double cx, double cy,
                                          double cz, double S2, uint32_t &lo, uint32_t &hi) {
   const double ex = qx - cx, ey = qy - cy, ez = qz - cz;
   const double r2 = (ex * ex + ey * ey + ez * ez);
   const unsigned long long m = __ballot(!(r2 > S2));
-  if (lane == r) {
-    lo = (uint32_t)m;
-    hi = (uint32_t)(m >> 32);
-  }
+  lo = put_lane(lo, (uint32_t)m, r);  // lane r takes the ballot
+  hi = put_lane(hi, (uint32_t)(m >> 32), r);
 }
 
 template <bool kStaged>
 __device__ __forceinline__ void test_rows(int nr, int ur0, const double *__restrict__ lds, double ax, double ay,
-                                          double az, double cx, double cy, double cz, double S2, int lane,
-                                          uint32_t &lo, uint32_t &hi) {
+                                          double az, double cx, double cy, double cz, double S2, uint32_t &lo,
+                                          uint32_t &hi) {
   int r = 0;
   for (; r + 4 <= nr; r += 4) {
     double q[4][3];
 #pragma unroll
     for (int j = 0; j < 4; j++) row_position<kStaged>(r + j, ur0, lds, ax, ay, az, q[j][0], q[j][1], q[j][2]);
 #pragma unroll
-    for (int j = 0; j < 4; j++) test_row(r + j, lane, q[j][0], q[j][1], q[j][2], cx, cy, cz, S2, lo, hi);
+    for (int j = 0; j < 4; j++) test_row(r + j, q[j][0], q[j][1], q[j][2], cx, cy, cz, S2, lo, hi);
   }
   for (; r < nr; r++) {
     double qx, qy, qz;
     row_position<kStaged>(r, ur0, lds, ax, ay, az, qx, qy, qz);
-    test_row(r, lane, qx, qy, qz, cx, cy, cz, S2, lo, hi);
+    test_row(r, qx, qy, qz, cx, cy, cz, S2, lo, hi);
   }
 }
 
@@ -516,9 +523,9 @@ __global__ __launch_bounds__(kWave *W) void k_search_cells(const double4 *__rest
         }
         uint32_t lo = 0, hi = 0;
         if (staged)
-          test_rows<true>(nr, urow0 + rb, lds64, 0.0, 0.0, 0.0, cx, cy, cz, S2, lane, lo, hi);
+          test_rows<true>(nr, urow0 + rb, lds64, 0.0, 0.0, 0.0, cx, cy, cz, S2, lo, hi);
         else
-          test_rows<false>(nr, urow0 + rb, nullptr, ax, ay, az, cx, cy, cz, S2, lane, lo, hi);
+          test_rows<false>(nr, urow0 + rb, nullptr, ax, ay, az, cx, cy, cz, S2, lo, hi);
         if (lane < nr) {
           // the row itself is candidate kself (the middle run holds the row's cell)
           const int kself = rp[4] + (urow0 + rb + lane - ru[4]) - c0;
