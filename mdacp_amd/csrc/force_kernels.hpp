@@ -968,15 +968,22 @@ __global__ __launch_bounds__(kWave *W) void k_prune_brick(const double4 *__restr
     const int len = bl.slen[s];
     const long long base = bl.sbase[s] + lane * kGroup;
     const uint4 *rg = reinterpret_cast<const uint4 *>(bl.nbr + base);
-    const uint2 *cg = reinterpret_cast<const uint2 *>(bl.code + base);
+    // a list without cell offset codes (FP64) gets an inner list without them
+    const bool codes = bl.code != nullptr;
+    const uint2 *cg = codes ? reinterpret_cast<const uint2 *>(bl.code + base) : nullptr;
     uint4 *og = reinterpret_cast<uint4 *>(nbr_in + base);
-    uint2 *oc = reinterpret_cast<uint2 *>(code_in + base);
+    uint2 *oc = codes ? reinterpret_cast<uint2 *>(code_in + base) : nullptr;
     uint32_t w0 = pad16, w1 = pad16, w2 = pad16, w3 = pad16, c0 = pad8, c1 = pad8;
     int kept = 0;
     for (int k = 0; k < len; k += kGroup) {
       int e[kGroup], cc[kGroup];
       unpack_group(rg[(k / kGroup) * kWave], e);
-      unpack_codes(cg[(k / kGroup) * kWave], cc);
+      if (codes) {
+        unpack_codes(cg[(k / kGroup) * kWave], cc);
+      } else {
+#pragma unroll
+        for (int u = 0; u < kGroup; u++) cc[u] = kZeroOffset;
+      }
 #pragma unroll
       for (int u = 0; u < kGroup; u++) {
         double qx, qy, qz;
@@ -1006,7 +1013,7 @@ __global__ __launch_bounds__(kWave *W) void k_prune_brick(const double4 *__restr
           kept++;
           if (kept % kGroup == 0) {
             og[(kept / kGroup - 1) * kWave] = make_uint4(w0, w1, w2, w3);
-            oc[(kept / kGroup - 1) * kWave] = make_uint2(c0, c1);
+            if (codes) oc[(kept / kGroup - 1) * kWave] = make_uint2(c0, c1);
             w0 = w1 = w2 = w3 = pad16;
             c0 = c1 = pad8;
           }
@@ -1017,11 +1024,11 @@ __global__ __launch_bounds__(kWave *W) void k_prune_brick(const double4 *__restr
     const int groups = wave_max_int(mine);          // groups of the slice's inner list
     if (kept % kGroup) {
       og[(kept / kGroup) * kWave] = make_uint4(w0, w1, w2, w3);
-      oc[(kept / kGroup) * kWave] = make_uint2(c0, c1);
+      if (codes) oc[(kept / kGroup) * kWave] = make_uint2(c0, c1);
     }
     for (int g = mine; g < groups; g++) {
       og[g * kWave] = make_uint4(pad16, pad16, pad16, pad16);
-      oc[g * kWave] = make_uint2(pad8, pad8);
+      if (codes) oc[g * kWave] = make_uint2(pad8, pad8);
     }
     if (lane == 0) slen_in[s] = groups * kGroup;
   }

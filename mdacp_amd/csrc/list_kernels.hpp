@@ -678,12 +678,16 @@ __device__ __forceinline__ void cursor_run(MaskCursor &c, const RowCell &rc, con
 // MeshList::MakeList (meshlist.cc:58-87) in sliced-ELL form: one wavefront per
 // 64-row slice (rows in the order of k_brick_row_order, rowsrc[row] = the
 // row's index in the search) turns the rows' candidate masks into staged
-// indices and cell offset codes.  All lanes produce entry j of group g
-// together, each from its own row's mask (or the brick's pad entry once the
-// row is exhausted), and store group g as one 16-byte piece of indices and one
-// 8-byte piece of codes per lane: piece g * 64 + lane of the slice, the layout
-// of slot_of().
-template <int W>
+// indices and, with kCodes, cell offset codes.  All lanes produce entry j of
+// group g together, each from its own row's mask (or the brick's pad entry
+// once the row is exhausted), and store group g as one 16-byte piece of
+// indices (and one 8-byte piece of codes) per lane: piece g * 64 + lane of the
+// slice, the layout of slot_of().  Without codes, a lane holds its row's nine
+// candidate runs (first candidate, staged offset) in registers and maps a
+// candidate to its staged index with eight compare-selects; with codes, it
+// walks the runs through the brick's range table, which also gives the
+// partner's cell offset.
+template <int W, bool kCodes>
 __global__ __launch_bounds__(kWave *W) void k_emit_brick(const int *__restrict__ cell, Bricks B, SearchTables st,
                                                         const int *__restrict__ rowsrc,
                                                         const uint32_t *__restrict__ mask, int mcap,
@@ -712,15 +716,22 @@ __global__ __launch_bounds__(kWave *W) void k_emit_brick(const int *__restrict__
     c.ncand = 0;
     c.nw = 0;
     c.mk = nullptr;
+    int kb[9], du[9];  // without codes: run q's first candidate, and staged index - candidate in it
+#pragma unroll
+    for (int q = 0; q < 9; q++) kb[q] = du[q] = 0;
     if (v < nrow) {
       const int i = row0 + v;
       rc = row_cell(i, cell[i], X, Rp, tb, ts);
       // masks are indexed by the row order of the search; mcap is even
       c.mk = reinterpret_cast<const unsigned long long *>(mask + (long long)rowsrc[i] * mcap);
       int n = 0;
+#pragma unroll
       for (int q = 0; q < 9; q++) {
         const int r0 = ((rc.lz + q / 3 - 1) * X.n[1] + (rc.ly + q % 3 - 1)) * X.n[0] + rc.lx - 1;
-        n += tb[kRangesPerCell * (r0 + 3)] - tb[kRangesPerCell * r0];
+        const int u0 = tb[kRangesPerCell * r0];
+        kb[q] = n;
+        du[q] = u0 - n;
+        n += tb[kRangesPerCell * (r0 + 3)] - u0;
       }
       c.ncand = n;
       c.nw = (n + 63) / 64;
@@ -730,9 +741,9 @@ __global__ __launch_bounds__(kWave *W) void k_emit_brick(const int *__restrict__
     c.wn = mask_word(c, 1);
     c.q = 0;
     c.ke = 0;
-    if (c.nw > 0) cursor_run(c, rc, X, tb);
+    if (kCodes && c.nw > 0) cursor_run(c, rc, X, tb);
     uint4 *gn = reinterpret_cast<uint4 *>(nbr + sbase[s]);  // sbase: a multiple of 512
-    uint2 *gc = reinterpret_cast<uint2 *>(ncode + sbase[s]);
+    uint2 *gc = kCodes ? reinterpret_cast<uint2 *>(ncode + sbase[s]) : nullptr;
     for (int g = 0; g < len / kGroup; g++) {
       unsigned e[kGroup], cd[kGroup];
 #pragma unroll
@@ -748,19 +759,29 @@ __global__ __launch_bounds__(kWave *W) void k_emit_brick(const int *__restrict__
           const int p = __ffsll((long long)c.w) - 1;
           c.w &= c.w - 1ull;
           const int kc = 64 * c.wi + p;
-          while (kc >= c.ke) {
-            c.q++;
-            cursor_run(c, rc, X, tb);
+          if (kCodes) {
+            while (kc >= c.ke) {
+              c.q++;
+              cursor_run(c, rc, X, tb);
+            }
+            const int u = c.u0 + (kc - c.kb);
+            const int dx = u < c.t1 ? -1 : (u < c.t2 ? 0 : 1);
+            e[j] = (unsigned)u;
+            cd[j] = offset_code(dx, c.q % 3 - 1, c.q / 3 - 1);
+          } else {
+            // the last run that starts at or before kc holds it (an empty run
+            // starts where the next one does)
+            int d = du[0];
+#pragma unroll
+            for (int q = 1; q < 9; q++) d = kc >= kb[q] ? du[q] : d;
+            e[j] = (unsigned)(kc + d);
           }
-          const int u = c.u0 + (kc - c.kb);
-          const int dx = u < c.t1 ? -1 : (u < c.t2 ? 0 : 1);
-          e[j] = (unsigned)u;
-          cd[j] = offset_code(dx, c.q % 3 - 1, c.q / 3 - 1);
         }
       }
       gn[g * kWave + lane] = make_uint4(e[0] | e[1] << 16, e[2] | e[3] << 16, e[4] | e[5] << 16, e[6] | e[7] << 16);
-      gc[g * kWave + lane] = make_uint2(cd[0] | cd[1] << 8 | cd[2] << 16 | cd[3] << 24,
-                                        cd[4] | cd[5] << 8 | cd[6] << 16 | cd[7] << 24);
+      if (kCodes)
+        gc[g * kWave + lane] = make_uint2(cd[0] | cd[1] << 8 | cd[2] << 16 | cd[3] << 24,
+                                          cd[4] | cd[5] << 8 | cd[6] << 16 | cd[7] << 24);
     }
     // entries left over: the slice is shorter than the row
     if (c.w != 0ull || c.wn != 0ull || c.wi + 2 < c.nw) {

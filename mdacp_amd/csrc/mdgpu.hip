@@ -281,6 +281,10 @@ inline int unroll_of(const mdgpu_domain *d) {
 }
 
 inline bool rel_frame(const mdgpu_domain *d) { return d->precision == MDGPU_MIXED && d->mixed_frame == 0; }
+// The list carries each partner's cell offset code for the cell-frame mixed
+// kick (rel_frame).  A mixed domain writes them whatever its frame, which may
+// change between constructions; an FP64 domain never reads them.
+inline bool list_codes(const mdgpu_domain *d) { return d->precision == MDGPU_MIXED; }
 
 inline unsigned nblk(long long n, int b = kBlock) { return (unsigned)std::max<long long>(1, (n + b - 1) / b); }
 
@@ -702,8 +706,12 @@ int launch_search(mdgpu_domain *d, const Bricks &B, const SearchTables &t) {
 
 // one wavefront per 64-row slice, four per brick workgroup
 int launch_emit(mdgpu_domain *d, const Bricks &B, const SearchTables &t) {
-  hipLaunchKernelGGL(k_emit_brick<kEmitWaves>, dim3(d->nbrick), dim3(kWave * kEmitWaves), 0, d->stream, d->cell, B,
-                     t, d->rowsrc, d->mask, d->mcap, d->sbase, d->slen, d->nbr, d->ncode, d->i32);
+  if (list_codes(d))
+    hipLaunchKernelGGL((k_emit_brick<kEmitWaves, true>), dim3(d->nbrick), dim3(kWave * kEmitWaves), 0, d->stream,
+                       d->cell, B, t, d->rowsrc, d->mask, d->mcap, d->sbase, d->slen, d->nbr, d->ncode, d->i32);
+  else
+    hipLaunchKernelGGL((k_emit_brick<kEmitWaves, false>), dim3(d->nbrick), dim3(kWave * kEmitWaves), 0, d->stream,
+                       d->cell, B, t, d->rowsrc, d->mask, d->mcap, d->sbase, d->slen, d->nbr, nullptr, d->i32);
   return check_launch(d, "neighbour list emit");
 }
 
@@ -1011,9 +1019,11 @@ int build_lists(mdgpu_domain *d) {
     // kListTail readable entries past the last slice (the force kernels read ahead)
     const long long nc = slots + slots / 8 + 1024;
     CHK(dalloc(d, &d->nbr, (size_t)(nc + kListTail)));
-    CHK(dalloc(d, &d->ncode, (size_t)(nc + kListTail)));
     CHK(dalloc(d, &d->nbr_in, (size_t)(nc + kListTail)));
-    CHK(dalloc(d, &d->ncode_in, (size_t)(nc + kListTail)));
+    if (list_codes(d)) {  // the lists of an FP64 domain have no codes
+      CHK(dalloc(d, &d->ncode, (size_t)(nc + kListTail)));
+      CHK(dalloc(d, &d->ncode_in, (size_t)(nc + kListTail)));
+    }
     d->nbr_cap = nc;
   }
   d->nbr_slots = slots;
@@ -1278,7 +1288,7 @@ ForceConstsF fconsts32(const mdgpu_domain *d) {
 BrickList brick_list(const mdgpu_domain *d, int bytes_per_entry, bool inner = false) {
   BrickList bl;
   bl.nbr = inner ? d->nbr_in : d->nbr;
-  bl.code = inner ? d->ncode_in : d->ncode;
+  bl.code = !list_codes(d) ? nullptr : inner ? d->ncode_in : d->ncode;
   bl.sbase = d->sbase;
   bl.slen = inner ? d->slen_in : d->slen;
   bl.glist = d->glist;
