@@ -312,7 +312,9 @@ int mdgpu_get_timing(const mdgpu_domain *d, int *force_launches, double *force_m
                                      the row's cell brick, staged from the FP64 positions (default: no
                                      offsets to read; coordinates up to about half a brick
                                      neighbourhood, so rounding about 2x coarser); 0 = relative to each
-                                     particle's cell, with the partner's cell offset per list entry */
+                                     particle's cell, with the partner's cell offset per list entry
+                                     (written by constructions in this frame only; setting 0 writes
+                                     them for the current list) */
 #define MDGPU_OPT_DISPLACEMENT_CHECK 16  /* rebuild cadence: 1 = once the Verlet budget is spent, the
                                             exact displacement check each step (pairlist.cc:26-121;
                                             the default and the reference's schedule); 0 = the list

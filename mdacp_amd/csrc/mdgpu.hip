@@ -205,7 +205,9 @@ struct mdgpu_domain {
   long long glist_cap = 0;
   // list
   uint16_t *nbr = nullptr;   // staged partner index per list slot
-  uint8_t *ncode = nullptr;  // partner cell offset per list slot
+  uint8_t *ncode = nullptr;  // partner cell offset per list slot (cell-frame mixed kick only)
+  long long code_cap = 0;    // slots ncode and ncode_in hold
+  bool list_coded = false;   // the current list (and inner list) carry the codes
   // inner (pruned) list of the fused step: same slots, per-slice inner length
   uint16_t *nbr_in = nullptr;
   uint8_t *ncode_in = nullptr;
@@ -281,10 +283,10 @@ inline int unroll_of(const mdgpu_domain *d) {
 }
 
 inline bool rel_frame(const mdgpu_domain *d) { return d->precision == MDGPU_MIXED && d->mixed_frame == 0; }
-// The list carries each partner's cell offset code for the cell-frame mixed
-// kick (rel_frame).  A mixed domain writes them whatever its frame, which may
-// change between constructions; an FP64 domain never reads them.
-inline bool list_codes(const mdgpu_domain *d) { return d->precision == MDGPU_MIXED; }
+// Only the cell-frame mixed kick (rel_frame) reads each partner's cell offset
+// code: a construction writes them for that frame alone, and a switch to it
+// writes them for the current list (recode_list).
+inline bool list_codes(const mdgpu_domain *d) { return d->list_coded; }
 
 inline unsigned nblk(long long n, int b = kBlock) { return (unsigned)std::max<long long>(1, (n + b - 1) / b); }
 
@@ -704,15 +706,40 @@ int launch_search(mdgpu_domain *d, const Bricks &B, const SearchTables &t) {
   return check_launch(d, "neighbour search");
 }
 
-// one wavefront per 64-row slice, four per brick workgroup
+// Room for a code per slot of the list and of the inner list.
+int ensure_codes(mdgpu_domain *d) {
+  if (d->ncode && d->code_cap >= d->nbr_cap) return MDGPU_OK;
+  d->code_cap = 0;
+  CHK(dalloc(d, &d->ncode, (size_t)(d->nbr_cap + kListTail)));
+  CHK(dalloc(d, &d->ncode_in, (size_t)(d->nbr_cap + kListTail)));
+  d->code_cap = d->nbr_cap;
+  return MDGPU_OK;
+}
+
+// one wavefront per 64-row slice, four per brick workgroup; with codes for
+// the cell-frame mixed kick only (recode_list gives them to a built list)
 int launch_emit(mdgpu_domain *d, const Bricks &B, const SearchTables &t) {
-  if (list_codes(d))
+  const bool codes = rel_frame(d);
+  if (codes) {
+    CHK(ensure_codes(d));
     hipLaunchKernelGGL((k_emit_brick<kEmitWaves, true>), dim3(d->nbrick), dim3(kWave * kEmitWaves), 0, d->stream,
                        d->cell, B, t, d->rowsrc, d->mask, d->mcap, d->sbase, d->slen, d->nbr, d->ncode, d->i32);
-  else
+  } else {
     hipLaunchKernelGGL((k_emit_brick<kEmitWaves, false>), dim3(d->nbrick), dim3(kWave * kEmitWaves), 0, d->stream,
                        d->cell, B, t, d->rowsrc, d->mask, d->mcap, d->sbase, d->slen, d->nbr, nullptr, d->i32);
+  }
+  d->list_coded = codes;
   return check_launch(d, "neighbour list emit");
+}
+
+// A switch to the cell frame gives the current list its codes: the emit runs
+// again from the construction's candidate masks, which stay in place until
+// the next construction, and writes the same indices with the codes.  The
+// inner list is pruned again at the next step (the option change invalidates
+// it).
+int recode_list(mdgpu_domain *d) {
+  if (!rel_frame(d) || d->list_coded || !d->initialized || d->pn == 0 || d->nbr_slots == 0) return MDGPU_OK;
+  return launch_emit(d, bricks_of(d), search_tables(d));
 }
 
 // Rows of each brick by decreasing list length: permute the owned particles
@@ -1020,14 +1047,11 @@ int build_lists(mdgpu_domain *d) {
     const long long nc = slots + slots / 8 + 1024;
     CHK(dalloc(d, &d->nbr, (size_t)(nc + kListTail)));
     CHK(dalloc(d, &d->nbr_in, (size_t)(nc + kListTail)));
-    if (list_codes(d)) {  // the lists of an FP64 domain have no codes
-      CHK(dalloc(d, &d->ncode, (size_t)(nc + kListTail)));
-      CHK(dalloc(d, &d->ncode_in, (size_t)(nc + kListTail)));
-    }
     d->nbr_cap = nc;
   }
   d->nbr_slots = slots;
   if (pn > 0 && scap > 0) CHK(launch_emit(d, B, STb));
+  else d->list_coded = false;  // no slots to code
   CHK(check_launch(d, "neighbour list"));
   // PairList::Init: snapshot of owned + ghost positions (pairlist.cc:11-23)
   if (d->tn > 0)
@@ -2956,7 +2980,7 @@ int mdgpu_set_precision(mdgpu_domain *d, int precision) {
   if (!d || (precision != MDGPU_FP64 && precision != MDGPU_MIXED)) return MDGPU_ERR_ARGUMENT;
   invalidate(d);  // the next fused step recomputes its first half drift
   d->precision = precision;
-  return MDGPU_OK;
+  return recode_list(d);
 }
 
 int mdgpu_add_particles(mdgpu_domain *d, int n, const double *q, const double *p, const int *type) {
@@ -3546,7 +3570,7 @@ int mdgpu_set_option(mdgpu_domain *d, int option, int value) {
     case MDGPU_OPT_MIXED_FRAME:
       if (value != 0 && value != 1) return MDGPU_ERR_ARGUMENT;
       d->mixed_frame = value;
-      return MDGPU_OK;
+      return recode_list(d);
     case MDGPU_OPT_DISPLACEMENT_CHECK:
       if (value != 0 && value != 1) return MDGPU_ERR_ARGUMENT;
       d->disp_check = value;

@@ -330,6 +330,35 @@ def test_runs_ahead_match_steps_one_at_a_time(thermostat, precision, frame):
     b.close()
 
 
+def test_switch_to_cell_frame_codes_the_built_list():
+    """Only the cell-frame mixed kick reads the list's cell-offset codes, so a construction in
+    another frame (or in FP64) writes none.  Switching a built list to the cell frame writes
+    them: its kicks and steps are then bit-identical to those of a list built in that frame."""
+    say("case: switch to the cell frame after construction")
+    L = 16.0
+    xx, pp, ids = melted_state(L, 0.5, steps=100, seed=6)
+    a = mdgpu.Domain(L, dt=0.002, precision=mdgpu.MIXED)
+    a.set_option(mdgpu.OPT_MIXED_FRAME, 0)
+    b = mdgpu.Domain(L, dt=0.002, precision=mdgpu.MIXED)  # brick frame
+    c = mdgpu.Domain(L, dt=0.002)  # FP64
+    for d in (a, b, c):
+        d.add_particles(xx, pp, ids)
+        d.make_pair_list()
+    b.set_option(mdgpu.OPT_MIXED_FRAME, 0)
+    c.set_precision(mdgpu.MIXED)
+    c.set_option(mdgpu.OPT_MIXED_FRAME, 0)
+    ka = a.kick()
+    assert np.array_equal(ka, b.kick()) and np.array_equal(ka, c.kick())
+    for d in (a, b, c):
+        d.calculate(30)
+    qa, pa, _ = a.particles()
+    for d in (b, c):
+        q, p, _ = d.particles()
+        assert np.array_equal(qa, q) and np.array_equal(pa, p)
+    for d in (a, b, c):
+        d.close()
+
+
 @pytest.mark.parametrize("thermostat,precision", [("nve", mdgpu.FP64), ("langevin", mdgpu.FP64),
                                                   ("langevin", mdgpu.MIXED)])
 def test_runs_launched_directly_match_graph_runs(thermostat, precision):
