@@ -982,18 +982,20 @@ int build_lists(mdgpu_domain *d) {
   hipLaunchKernelGGL(k_brick_ranges, dim3(nblk((long long)d->nbrick * kWave)), dim3(kBlock), 0, st, B, T,
                      d->nbrick, pn, d->tstart, d->tbase, d->nrange, d->btotal, d->bpart);
   hipLaunchKernelGGL(k_reduce_max, dim3(1), dim3(kBlock), 0, st, d->bpart, d->nbrick, d->u64 + 2);
+  // gather lists: gbase = exclusive scan of the staged counts, gbase[nbrick] = total;
+  // the largest neighbourhood and the total are read after one wait
+  exclusive_scan<int, long long>(d->btotal, d->gbase, d->nbrick, d->scan_scratch, d->i64, st);
+  HIPCHK(d, hipMemcpyAsync(d->gbase + d->nbrick, d->i64, sizeof(long long), hipMemcpyDeviceToDevice, st));
+  HIPCHK(d, hipMemcpyAsync(d->h_u64 + 2, d->u64 + 2, sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
+  long long gtotal = 0;
+  CHK(read_total(d, &gtotal));
   double mstaged = 0.0;
-  CHK(read_max(d, 2, &mstaged));
+  std::memcpy(&mstaged, d->h_u64 + 2, sizeof(double));
   d->max_staged = (int)mstaged;
   if (d->max_staged > 65534) {
     d->err = "a brick's neighbourhood exceeds 16-bit indexing (extreme local density): use smaller bricks";
     return MDGPU_ERR_CAPACITY;
   }
-  // gather lists: gbase = exclusive scan of the staged counts, gbase[nbrick] = total
-  exclusive_scan<int, long long>(d->btotal, d->gbase, d->nbrick, d->scan_scratch, d->i64, st);
-  HIPCHK(d, hipMemcpyAsync(d->gbase + d->nbrick, d->i64, sizeof(long long), hipMemcpyDeviceToDevice, st));
-  long long gtotal = 0;
-  CHK(read_total(d, &gtotal));
   d->gtotal = gtotal;
   if (gtotal > d->glist_cap) {
     const long long gc = gtotal + gtotal / 8 + 1024;
@@ -1027,17 +1029,23 @@ int build_lists(mdgpu_domain *d) {
       d->mcap *= 2;
     }
     CHK(order_rows(d));
-    // number of listed (row, partner) entries, for accounting
+    // scratch for both scans before either is queued (a reallocation frees
+    // the buffer a queued scan would use)
+    CHK(ensure_scan(d, std::max<long long>(pn, d->nslice)));
+    // number of listed (row, partner) entries, for accounting: copied now,
+    // read after the next wait
     exclusive_scan<int, long long>(d->row_len, d->key, pn, d->scan_scratch, d->i64, st);
-    CHK(read_total(d, &d->nbr_entries));
+    HIPCHK(d, hipMemcpyAsync(d->h_i64 + 1, d->i64, sizeof(long long), hipMemcpyDeviceToHost, st));
     hipLaunchKernelGGL(k_slice_len, dim3(nblk(d->nslice)), dim3(kBlock), 0, st, d->row_len, d->slice_brick,
                        d->vslice0, d->row0, d->nrow, d->nslice, d->slen, d->slots64, d->spart);
     hipLaunchKernelGGL(k_reduce_max, dim3(1), dim3(kBlock), 0, st, d->spart, (int)nblk(d->nslice), d->u64 + 3);
-    CHK(ensure_scan(d, d->nslice));
     exclusive_scan<long long, long long>(d->slots64, d->sbase, d->nslice, d->scan_scratch, d->i64, st);
+    // the slots and the longest slice after one wait
+    HIPCHK(d, hipMemcpyAsync(d->h_u64 + 3, d->u64 + 3, sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
     CHK(read_total(d, &slots));
+    d->nbr_entries = d->h_i64[1];
     double smax = 0.0;
-    CHK(read_max(d, 3, &smax));
+    std::memcpy(&smax, d->h_u64 + 3, sizeof(double));
     scap = (int)smax;
   } else {
     d->nbr_entries = 0;
