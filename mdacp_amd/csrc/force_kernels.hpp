@@ -1090,34 +1090,42 @@ __global__ __launch_bounds__(kWave *W) void k_observe_brick(const double4 *__res
       unpack_group(ahead, e);
       ahead = rg[(k / kGroup + 1) * kWave];
       __builtin_amdgcn_sched_barrier(0);  // the prefetch is issued before this group's work
-      double qx[kGroup], qy[kGroup], qz[kGroup];
+      // two halves of four partners, each loaded then summed in list order:
+      // all eight loaded at once held the kernel above 128 registers, one
+      // workgroup per CU instead of two
 #pragma unroll
-      for (int u = 0; u < kGroup; u++) {
-        if (staged) {  // padding reads the staged far sentinel
-          qx[u] = lds64[3 * e[u]];
-          qy[u] = lds64[3 * e[u] + 1];
-          qz[u] = lds64[3 * e[u] + 2];
-        } else if (e[u] < total) {
-          const double4 q = pos[gl[e[u]]];
-          qx[u] = q.x;
-          qy[u] = q.y;
-          qz[u] = q.z;
-        } else {
-          qx[u] = qy[u] = qz[u] = 1.0e200;
+      for (int h = 0; h < kGroup; h += kGroup / 2) {
+        double qx[kGroup / 2], qy[kGroup / 2], qz[kGroup / 2];
+#pragma unroll
+        for (int u = 0; u < kGroup / 2; u++) {
+          const int eu = e[h + u];
+          if (staged) {  // padding reads the staged far sentinel
+            qx[u] = lds64[3 * eu];
+            qy[u] = lds64[3 * eu + 1];
+            qz[u] = lds64[3 * eu + 2];
+          } else if (eu < total) {
+            const double4 q = pos[gl[eu]];
+            qx[u] = q.x;
+            qy[u] = q.y;
+            qz[u] = q.z;
+          } else {
+            qx[u] = qy[u] = qz[u] = 1.0e200;
+          }
         }
-      }
 #pragma unroll
-      for (int u = 0; u < kGroup; u++) {
-        const double dx = qi.x - qx[u], dy = qi.y - qy[u], dz = qi.z - qz[u];
-        const double r2 = (dx * dx + dy * dy + dz * dz);
-        const double ir2 = rcp64_exact(r2);  // padding: r2 = inf, NaN terms, dropped by the select
-        const double ir6 = ir2 * ir2 * ir2;
-        const double c2r2 = C2 * r2;
-        const bool in = !(r2 > CL2);
-        const double tpe = fma(ir6, ir6, -ir6) + (c2r2 + C0);
-        const double tvir = fma(ir6, fma(-6.0, ir6, 3.0), c2r2);
-        spe += in ? tpe : 0.0;
-        svir += in ? tvir : 0.0;
+        for (int u = 0; u < kGroup / 2; u++) {
+          const double dx = qi.x - qx[u], dy = qi.y - qy[u], dz = qi.z - qz[u];
+          const double r2 = (dx * dx + dy * dy + dz * dz);
+          const double ir2 = rcp64_exact(r2);  // padding: r2 = inf, NaN terms, dropped by the select
+          const double ir6 = ir2 * ir2 * ir2;
+          const double c2r2 = C2 * r2;
+          const bool in = !(r2 > CL2);
+          const double tpe = fma(ir6, ir6, -ir6) + (c2r2 + C0);
+          const double tvir = fma(ir6, fma(-6.0, ir6, 3.0), c2r2);
+          spe += in ? tpe : 0.0;
+          svir += in ? tvir : 0.0;
+        }
+        __builtin_amdgcn_sched_barrier(0);  // keeps the halves apart
       }
       hold_prefetch(ahead, spe, svir, ke);
     }
