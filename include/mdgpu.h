@@ -261,7 +261,8 @@ int mdgpu_get_timing(const mdgpu_domain *d, int *force_launches, double *force_m
    them beyond floating-point summation order) */
 #define MDGPU_OPT_UNROLL 1   /* list entries whose indices the force kernels load ahead of
                                 the 8 being computed: 8, 16 or 32 (default 16 for FP64,
-                                8 for mixed precision) */
+                                8 for mixed precision and for a group's one kick of all
+                                its units) */
 #define MDGPU_OPT_KAHAN 2    /* mixed precision row sums: 2 = FP32 sums of each 8 list
                                 entries added in FP64 (default), 1 = compensated FP32
                                 (Kahan), 0 = plain FP32 */
