@@ -477,14 +477,16 @@ __global__ __launch_bounds__(kWave *W) void k_search_cells(const double4 *__rest
     const int nrows = __builtin_amdgcn_readfirstlane(tb[rs + 1]) - urow0;
     if (nrows == 0) continue;
     const int irow0 = __builtin_amdgcn_readfirstlane(ts[rs]);  // particle index of that row
-    // runs of the candidate sequence: staged start ru[q], first candidate rp[q]
-    int ru[9], rp[10];
+    // runs of the candidate sequence: staged start ru[q], first candidate
+    // rp[q], and du[q] = ru[q] - rp[q] (candidate k of run q is staged at k + du[q])
+    int ru[9], rp[10], du[9];
     rp[0] = 0;
 #pragma unroll
     for (int q = 0; q < 9; q++) {
       const int r0 = ((lz + q / 3 - 1) * X.n[1] + (ly + q % 3 - 1)) * X.n[0] + lx - 1;
       ru[q] = __builtin_amdgcn_readfirstlane(tb[kRangesPerCell * r0]);
       rp[q + 1] = rp[q] + __builtin_amdgcn_readfirstlane(tb[kRangesPerCell * (r0 + 3)]) - ru[q];
+      du[q] = ru[q] - rp[q];
     }
     const int ncand = rp[9];
     if (ncand > 32 * mcap) {  // the host retries with longer masks
@@ -504,10 +506,12 @@ __global__ __launch_bounds__(kWave *W) void k_search_cells(const double4 *__rest
       }
       for (int c0 = 0; c0 < ncand; c0 += kWave) {
         const int k = c0 + lane;
-        int u = ru[0] + k;
+        // the last run that starts at or before k holds it (an empty run starts
+        // where the next one does): one compare and one select per run
+        int dk = du[0];
 #pragma unroll
-        for (int q = 1; q < 9; q++)
-          if (k >= rp[q]) u = ru[q] + (k - rp[q]);
+        for (int q = 1; q < 9; q++) dk = k >= rp[q] ? du[q] : dk;
+        const int u = k + dk;
         double cx = 1.0e200, cy = 1.0e200, cz = 1.0e200;  // lanes past the last candidate
         if (k < ncand) {
           if (staged) {
