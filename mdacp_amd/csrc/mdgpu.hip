@@ -1260,13 +1260,14 @@ int is_expired_ensemble(mdgpu_domain *d, int *expired) {
   }
   HIPCHK(d, hipMemcpyAsync(d->which, d->h_which, nflag * sizeof(int), hipMemcpyHostToDevice, st));
   const int cap = d->check_list_length;
-  hipLaunchKernelGGL(k_zero_check, dim3(nblk(nflag)), dim3(kBlock), 0, st, d->chk_dmax, d->chk_cnt, nflag);
+  hipLaunchKernelGGL(k_zero_check, dim3(nblk(nflag)), dim3(kBlock), 0, st, d->chk_dmax, d->chk_cnt, d->rep_exp,
+                     nflag);
   hipLaunchKernelGGL(k_check_dmax, dim3(nflag * kCheckBlocks), dim3(kBlock), 0, st, d->pos, d->pos_old, d->which,
                      d->reps, d->grng, d->pn, d->disp, d->chk_dmax);
   hipLaunchKernelGGL(k_check_cand, dim3(nflag * kCheckBlocks), dim3(kBlock), 0, st, d->disp, d->which, d->reps,
                      d->grng, d->pn, d->margin, cap, d->chk_dmax, d->chk_cnt, d->chk_cand);
-  hipLaunchKernelGGL(k_check_pairs, dim3(nflag), dim3(kBlock), 0, st, d->pos, d->pos_old, d->which, d->reps,
-                     d->grng, cap, d->SL * d->SL, d->CL2, d->chk_cnt, d->chk_cand, d->rep_exp);
+  hipLaunchKernelGGL(k_check_pairs, dim3(nflag, kPairBlocks), dim3(kBlock), 0, st, d->pos, d->pos_old, d->which,
+                     d->reps, d->grng, cap, d->SL * d->SL, d->CL2, d->chk_cnt, d->chk_cand, d->rep_exp);
   CHK(check_launch(d, "displacement check"));
   HIPCHK(d, hipMemcpyAsync(d->h_rep_exp, d->rep_exp, nflag * sizeof(int), hipMemcpyDeviceToHost, st));
   HIPCHK(d, hipMemcpyAsync(d->h_chk_dmax, d->chk_dmax, nflag * sizeof(unsigned long long), hipMemcpyDeviceToHost,

@@ -531,14 +531,15 @@ __global__ __launch_bounds__(kBlock) void k_replica_kinetic(const double4 *__res
 //  2. k_check_cand, as many workgroups: the entries displaced by more than
 //     margin - max displacement (read back from disp), up to Nl = min(cap, n)
 //     of them stored, all of them counted in cnt[k];
-//  3. k_check_pairs, one workgroup per listed replica: expired once Nl
-//     candidates were found, else when a candidate pair that was beyond SL at
-//     the build is now inside rc.
-// dmax and cnt start at zero (k_zero_check); expired[k] receives the answer
-// for replica which[k].  The decision does not depend on the order in which
+//  3. k_check_pairs, kPairBlocks workgroups per listed replica sharing its
+//     candidate pairs: expired once Nl candidates were found, else when a
+//     candidate pair that was beyond SL at the build is now inside rc.
+// dmax, cnt and expired start at zero (k_zero_check); expired[k] receives the
+// answer for replica which[k].  The decision does not depend on the order in which
 // candidates are stored.
-constexpr int kCheckCap = 1024;   // largest CheckListLength an ensemble accepts (cap² pair tests per workgroup)
+constexpr int kCheckCap = 1024;   // largest CheckListLength an ensemble accepts (cap² pair tests per replica)
 constexpr int kCheckBlocks = 32;  // workgroups per listed replica in the first two launches
+constexpr int kPairBlocks = 8;    // workgroups per listed replica in the candidate pair test
 
 __device__ __forceinline__ int check_entry(int k, const Replica &R, int no, int g0, int pn_all) {
   return k < no ? R.p0 + k : pn_all + g0 + (k - no);
@@ -602,13 +603,15 @@ __global__ __launch_bounds__(kBlock) void k_check_pairs(const double4 *__restric
   const int Nl = n > cap ? cap : n;
   const int l = cnt[kr];
   if (l > 0 && l >= Nl) {
-    if (threadIdx.x == 0) expired[kr] = 1;
+    if (threadIdx.x == 0 && blockIdx.y == 0) expired[kr] = 1;
     return;
   }
   if (threadIdx.x == 0) flag = 0;
   __syncthreads();
   const int *c = cand + (long long)kr * cap;
-  for (int t = threadIdx.x; t < l * l; t += kBlock) {
+  // gridDim.y workgroups share the replica's pairs; any that finds one sets
+  // the flag (zeroed by k_zero_check), so the outcome is an OR over all pairs
+  for (int t = blockIdx.y * kBlock + threadIdx.x; t < l * l; t += kBlock * gridDim.y) {
     const int a = t / l, b = t % l;
     if (b <= a) continue;
     const double4 o1 = old[c[a]], o2 = old[c[b]];
@@ -621,14 +624,16 @@ __global__ __launch_bounds__(kBlock) void k_check_pairs(const double4 *__restric
     if (dx * dx + dy * dy + dz * dz < CL2) flag = 1;
   }
   __syncthreads();
-  if (threadIdx.x == 0) expired[kr] = flag;
+  if (threadIdx.x == 0 && flag) expired[kr] = 1;
 }
 
-__global__ void k_zero_check(unsigned long long *__restrict__ dmax, int *__restrict__ cnt, int n) {
+__global__ void k_zero_check(unsigned long long *__restrict__ dmax, int *__restrict__ cnt,
+                             int *__restrict__ expired, int n) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k < n) {
     dmax[k] = 0ull;
     cnt[k] = 0;
+    expired[k] = 0;
   }
 }
 
