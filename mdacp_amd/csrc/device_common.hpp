@@ -65,9 +65,18 @@ __device__ __forceinline__ void block_max_to_partial(double v, double *__restric
 // One workgroup: max of n partials, stored as the bits of a double.
 __global__ void k_reduce_max(const double *__restrict__ partial, int n, unsigned long long *__restrict__ out) {
   __shared__ double red[kBlock];
-  double m = 0.0;
-  for (int k = threadIdx.x; k < n; k += blockDim.x) m = fmax(m, partial[k]);
-  red[threadIdx.x] = m;
+  double m0 = 0.0, m1 = 0.0, m2 = 0.0, m3 = 0.0;  // four loads in flight per thread
+  int k = threadIdx.x;
+  for (; k + 3 * (int)blockDim.x < n; k += 4 * blockDim.x) {
+    const double a = partial[k], b = partial[k + blockDim.x];
+    const double c = partial[k + 2 * blockDim.x], e = partial[k + 3 * blockDim.x];
+    m0 = fmax(m0, a);
+    m1 = fmax(m1, b);
+    m2 = fmax(m2, c);
+    m3 = fmax(m3, e);
+  }
+  for (; k < n; k += blockDim.x) m0 = fmax(m0, partial[k]);
+  red[threadIdx.x] = fmax(fmax(m0, m1), fmax(m2, m3));
   __syncthreads();
   for (int o = blockDim.x / 2; o > 0; o >>= 1) {
     if ((int)threadIdx.x < o) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + o]);
